@@ -1,0 +1,198 @@
+/*
+ * octopic.h -- C ABI of the MI355X strong-strong PIC collision library.
+ *
+ * This library is a drop-in for one code path of Octopus.jl: the collision
+ *     collide!(solver::PICPoissonSolver, beam1::Beam, beam2::Beam, backend[, ctx]) -> Float64
+ * (src/tasks/strongstrong/pic_cpu.jl:1-32, pic_cuda.jl:94-143).  Julia host code binds it with
+ * `ccall` under a new backend tag; the Python package octopic (octopus.jl_amd/octopic) binds it with
+ * ctypes.  Every signature takes plain pointers, sizes and scalars.
+ *
+ * Conventions
+ *   - Beams are structure-of-arrays FP64 (x, px, y, py, z, pz), as Phase6DRep (src/beam/Beam.jl:45-56).
+ *     Coordinates are modified in place, as collide! does.
+ *   - Indices are 0-based in this ABI; the reference is 1-based.
+ *   - Every entry point returns an OCTOPIC_* status.  A nonzero status leaves a message that
+ *     octopic_last_error() returns.  The status codes mirror the reference's ArgumentError throws.
+ *   - A handle owns its device workspaces and the persistent slice-pair Green cache, i.e. what the
+ *     reference keeps in StrongStrongTask.runtime_cache (pic_cpu.jl:23-32, 488-515).  A handle is not
+ *     thread-safe: one caller per handle, as one task per collision label.
+ */
+#ifndef OCTOPIC_H
+#define OCTOPIC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OCTOPIC_ABI_VERSION 1
+
+/* Status codes (reference: the ArgumentError sites in pic_cpu.jl:517-576, 730-777, interface.jl:541-599). */
+enum {
+    OCTOPIC_OK = 0,
+    OCTOPIC_ERR_INVALID_OPTION = 1,       /* an option value outside its schema (interface.jl:1410-1476) */
+    OCTOPIC_ERR_UNSUPPORTED = 2,          /* a combination the reference rejects, e.g. :node + :quadratic (pic_cpu.jl:568-574) */
+    OCTOPIC_ERR_NONFINITE = 3,            /* non-finite coordinates (interface.jl:541-599) */
+    OCTOPIC_ERR_DEGENERATE_EXTENT = 4,    /* zero transverse extent with no floor (pic_cpu.jl:1646-1672) */
+    OCTOPIC_ERR_OOM = 5,                  /* device allocation failed */
+    OCTOPIC_ERR_DEVICE = 6,               /* a HIP runtime call failed */
+    OCTOPIC_ERR_INVALID_ARGUMENT = 7      /* null pointer, empty beam, size mismatch */
+};
+
+/* Enumerations of the PICPoissonSolver symbol options (interface.jl:1324-1356). */
+enum { OCTOPIC_CIC = 0, OCTOPIC_TSC = 1 };
+enum { OCTOPIC_GREEN_INTEGRATED = 0, OCTOPIC_GREEN_STANDARD = 1, OCTOPIC_GREEN_LATTICE = 2 };
+enum { OCTOPIC_CACHE_NONE = 0, OCTOPIC_CACHE_SLICE_PAIR = 1 };
+enum { OCTOPIC_INTERP_LINEAR = 0, OCTOPIC_INTERP_QUADRATIC = 1 };
+enum { OCTOPIC_GRID_SLICE_PAIR = 0, OCTOPIC_GRID_SOURCE_SLICE = 1, OCTOPIC_GRID_NODE = 2 };
+enum { OCTOPIC_EXTENT_EXTREMA = 0, OCTOPIC_EXTENT_SIGMA = 1 };
+enum { OCTOPIC_BATCH_SEQUENTIAL = 0, OCTOPIC_BATCH_WAVEFRONT = 1 };
+/* LongitudinalSlicing methods (interface.jl:800-835; slicing.jl:51-75). */
+enum {
+    OCTOPIC_SLICE_EQUAL_AREA = 0,
+    OCTOPIC_SLICE_EQUAL_COUNT = 1,
+    OCTOPIC_SLICE_EQUAL_WIDTH = 2,
+    OCTOPIC_SLICE_NORMAL_QUANTILE = 3,
+    OCTOPIC_SLICE_SPECIFIED = 4
+};
+enum { OCTOPIC_CENTER_CENTROID = 0, OCTOPIC_CENTER_MIDPOINT = 1 };
+
+/* LongitudinalSlicing (interface.jl:800-835).  `positions` holds nslices-1 internal boundaries in
+ * units of the bunch rms (method SPECIFIED only); it may be NULL otherwise. */
+typedef struct octopic_slicing {
+    int32_t nslices;
+    int32_t method;
+    int32_t resolution;
+    int32_t center_position;
+    int32_t npositions;
+    int32_t reserved;
+    const double *positions;
+} octopic_slicing;
+
+/* PICPoissonSolver{Float64} fields (interface.jl:1324-1356).  `kbb1`, `kbb2`, `luminosity_scale`
+ * are NaN for the reference's `nothing` (derive from the beam parameters).  `lum_nx`/`lum_ny` = 0
+ * and `lum_deposit` = -1 inherit `grid` / `deposit` (interface.jl:1513-1517).  The CUDA route flags
+ * (batch_mode, cuda_*) are accepted and recorded; they select nothing on this backend. */
+typedef struct octopic_options {
+    int32_t nx, ny;
+    int32_t deposit;               /* OCTOPIC_CIC | OCTOPIC_TSC */
+    int32_t green_type;            /* OCTOPIC_GREEN_* */
+    int32_t green_cache;           /* OCTOPIC_CACHE_* */
+    int32_t field_derivative;      /* 2 | 4 */
+    int32_t slice_interpolation;   /* OCTOPIC_INTERP_* */
+    int32_t interaction_grid;      /* OCTOPIC_GRID_* */
+    int32_t grid_extent;           /* OCTOPIC_EXTENT_* */
+    int32_t longitudinal_kick;     /* 0 | 1 */
+    int32_t lum_nx, lum_ny;        /* 0 = inherit grid */
+    int32_t lum_deposit;           /* -1 = inherit deposit */
+    int32_t batch_mode;            /* OCTOPIC_BATCH_*; recorded only */
+    int32_t cuda_flags;            /* bit0 cuda_async, bit1 cuda_batch_fft, bit2 cuda_wavefront_fft,
+                                      bit3 cuda_indexed_wavefront; recorded only */
+    int32_t reserved;
+    double grid_extent_sigma;
+    double min_extent_x, min_extent_y;
+    double grid_quantize;
+    double green_min_ratio;
+    double green_growth;
+    double kbb1, kbb2, luminosity_scale;
+    octopic_slicing slicing1, slicing2;
+} octopic_options;
+
+/* One beam: Phase6DRep + BeamParams (src/beam/Beam.jl:29-56).  With on_device = 1 the six pointers
+ * are device (HBM) pointers and the collision runs on them in place; with on_device = 0 they are
+ * host pointers that are copied in and out around the collision. */
+typedef struct octopic_beam {
+    double *x, *px, *y, *py, *z, *pz;
+    int64_t n;
+    double charge, mc2, E0, r0, npart;
+    int32_t on_device;
+    int32_t reserved;
+} octopic_beam;
+
+/* Counters of the last collide and of the handle's lifetime. */
+typedef struct octopic_stats {
+    int64_t green_hits, green_misses, green_rebuilds;   /* cumulative, _PICSlicePairGreenCache (interface.jl:716-742) */
+    int64_t dropped;                                    /* last collide (pic_cpu.jl:394-457) */
+    int64_t pairs, batches, widest_batch;               /* last collide's schedule */
+    int64_t green_builds;                               /* Green tables built in the last collide */
+    int64_t planes;                                     /* field planes solved in the last collide */
+    int64_t collides;
+} octopic_stats;
+
+/* One directed interaction of the last collide, in execution order (pair position p in collision
+ * order, then direction 1, 2).  cache_event: 0 = no cache, 1 = hit, 2 = miss, 3 = rebuild.
+ * Grids are (x0, y0, width, height) of the source and field meshes actually used. */
+typedef struct octopic_interaction_record {
+    int32_t p, i, j, dir;
+    int32_t cache_event;
+    int32_t reserved;
+    double source_grid[4];
+    double field_grid[4];
+} octopic_interaction_record;
+
+typedef struct octopic_handle octopic_handle;
+
+/* Fill `opt` with the PICPoissonSolver() defaults (interface.jl:1384-1407). */
+int octopic_options_default(octopic_options *opt);
+
+/* PICPoissonSolver{Float64}(...) validation + _validate_pic_solver (interface.jl:1384-1507,
+ * pic_cpu.jl:517-576) and the workspace/cache set-up of _pic_cpu_workspace_pool! / _pic_green_cache!
+ * (pic_cpu.jl:467-515).  `device` selects the GPU; the handle uses the null stream until
+ * octopic_set_stream is called. */
+int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **out);
+void octopic_destroy(octopic_handle *h);
+const char *octopic_last_error(const octopic_handle *h);
+int octopic_set_stream(octopic_handle *h, void *hip_stream);
+
+/* collide!(solver, beam1, beam2, backend, ctx) -> luminosity (pic_cpu.jl:82-275).
+ * `turn` is ctx.turn (-1 for ctx = nothing); `compute_luminosity` is the evaluated
+ * luminosity_schedule (interface.jl:2596-2620).  *luminosity is NaN when it is 0. */
+int octopic_collide(octopic_handle *h, octopic_beam *beam1, octopic_beam *beam2, int64_t turn,
+                    int32_t compute_luminosity, double *luminosity);
+
+int octopic_get_stats(const octopic_handle *h, octopic_stats *out);
+
+/* Per-pair luminosity of the last collide in collision order (the pair sink, interface.jl:389-410).
+ * Writes min(cap, npairs) entries; *n receives npairs.  ran[k] = 0 marks a skipped pair. */
+int octopic_pair_luminosity(const octopic_handle *h, int32_t *i, int32_t *j, int32_t *ran,
+                            double *lum, int64_t cap, int64_t *n);
+
+/* The last collide's longitudinal slices of beam `which` (1 or 2): LongitudinalSlices
+ * (slicing.jl:1-6).  boundaries has nslices+1 entries; indices receives the concatenated ascending
+ * member lists (n entries), counts the list lengths. */
+int octopic_last_slices(const octopic_handle *h, int32_t which, double *boundaries, double *centers,
+                        double *weights, int64_t *counts, int64_t *indices, int64_t n);
+
+/* The last collide's directed-interaction records (grid geometry + Green-cache decision). */
+int octopic_interaction_records(const octopic_handle *h, octopic_interaction_record *out, int64_t cap,
+                                int64_t *n);
+
+/* Linear 6D one-turn map (src/elements/linear6d.jl:110-118): coordinates <- M * coordinates, with M a
+ * row-major 6x6 matrix.  Runs on the handle's stream; beam pointers must be device pointers. */
+int octopic_track_linear6d(octopic_handle *h, const double *matrix36, octopic_beam *beam);
+
+/* Beam(N, ...) synthetic Gaussian construction on the device (Beam.jl:127-181, 545-553): Philox4x32-10
+ * counter normals, clamp to +-cutoff, standardise, scale by sigma/beta with alpha.  beta, alpha,
+ * sigma are (x, y, z) triples.  Writes the six device arrays of `beam` (n = beam->n). */
+int octopic_generate_gaussian_beam(octopic_handle *h, uint64_t seed, uint64_t rng_id,
+                                   const double *beta3, const double *alpha3, const double *sigma3,
+                                   double cutoff, octopic_beam *beam);
+
+/* Device-side CIC/TSC cell index and weights (pic_cpu.jl:2182-2252) for n normalised coordinates u
+ * on an axis of `npts` nodes: base (0-based first stencil node) and w (2 or 3 per point).
+ * u, base, w are host pointers.  Used by the parity tests of the cell indices. */
+int octopic_debug_weights(octopic_handle *h, int32_t method, const double *u, int64_t n, int32_t npts,
+                          int32_t *base, double *w);
+
+/* Mean duration in ms of every kernel family of the last collide (HIP events on the handle's stream),
+ * recorded only after octopic_set_timing(h, 1).  names: NUL-separated family names. */
+int octopic_set_timing(octopic_handle *h, int32_t enabled);
+int octopic_kernel_times(const octopic_handle *h, double *ms_total, int64_t *launches, int64_t cap,
+                         int64_t *n, char *names, int64_t names_cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCTOPIC_H */

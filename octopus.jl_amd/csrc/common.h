@@ -1,0 +1,141 @@
+// common.h -- device-side types and scalar helpers shared by the octopic HIP kernels.
+//
+// Scalar semantics follow the reference's Julia code exactly where a bit-exact result is owed
+// (bounds, grid geometry, cell indices): NaN-propagating min/max, Julia's clamp, and the
+// CIC/TSC weight closed forms (src/tasks/strongstrong/pic_cpu.jl:2182-2252).  Everything is
+// compiled with -ffp-contract=off so that no product is fused into an FMA.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "octopic.h"
+
+namespace octopic {
+
+constexpr int kFoldLanes = 4096;          // _SLICE_FOLD_LANES (slicing.jl:223)
+constexpr double kTemplateMargin = 1.5;   // _PIC_TEMPLATE_MARGIN_CELLS (interface.jl:674)
+
+// ------------------------------------------------------------------ Julia scalar semantics
+__host__ __device__ __forceinline__ double jmin(double x, double y) {
+    if (x != x) return x;
+    if (y != y) return y;
+    if (y < x) return y;
+    if (x < y) return x;
+    return __builtin_signbit(y) ? y : x;
+}
+__host__ __device__ __forceinline__ double jmax(double x, double y) {
+    if (x != x) return x;
+    if (y != y) return y;
+    if (y > x) return y;
+    if (x > y) return x;
+    return __builtin_signbit(y) ? x : y;
+}
+__host__ __device__ __forceinline__ double jclamp(double x, double lo, double hi) {
+    return x > hi ? hi : (x < lo ? lo : x);
+}
+__host__ __device__ __forceinline__ int iclamp(int x, int lo, int hi) { return x > hi ? hi : (x < lo ? lo : x); }
+__host__ __device__ __forceinline__ bool jisless(double a, double b) {
+    if (a != a) return false;
+    if (b != b) return true;
+    if (a < b) return true;
+    if (a > b) return false;
+    return __builtin_signbit(a) && !__builtin_signbit(b);
+}
+__host__ __device__ __forceinline__ bool finite4(double a, double b, double c, double d) {
+    return __builtin_isfinite(a) && __builtin_isfinite(b) && __builtin_isfinite(c) && __builtin_isfinite(d);
+}
+
+// CIC / TSC stencil on an axis of n nodes.  base0 is the 0-based first stencil node; out-of-range
+// or non-finite u gives zero weights at node 0, as the reference's (1, zeros) (pic_cpu.jl:2182-2252).
+__host__ __device__ __forceinline__ void cic_weights(double u, int n, int &base0, double w[3]) {
+    if (!(0.0 <= u && u <= (double)(n - 1))) { base0 = 0; w[0] = 0.0; w[1] = 0.0; w[2] = 0.0; return; }
+    int base = iclamp((int)floor(u) + 1, 1, n - 1);
+    double f = jclamp(u - (double)(base - 1), 0.0, 1.0);
+    w[0] = 1.0 - f; w[1] = f; w[2] = 0.0;
+    base0 = base - 1;
+}
+__host__ __device__ __forceinline__ void tsc_weights(double u, int n, int &base0, double w[3]) {
+    if (!(0.0 <= u && u <= (double)(n - 1))) { base0 = 0; w[0] = 0.0; w[1] = 0.0; w[2] = 0.0; return; }
+    double fl = floor(u);
+    int ix = (int)fl;
+    double f = u - fl;
+    int base;
+    if (f < 0.5) {
+        double t = f * f;
+        w[0] = 0.125 + 0.5 * (t - f); w[1] = 0.75 - t; w[2] = 0.125 + 0.5 * (t + f);
+        base = ix;
+    } else {
+        double fr = 1.0 - f;
+        double t = fr * fr;
+        w[0] = 0.125 + 0.5 * (t + fr); w[1] = 0.75 - t; w[2] = 0.125 + 0.5 * (t - fr);
+        base = ix + 1;
+    }
+    base0 = iclamp(base, 1, n - 2) - 1;
+}
+template <int METHOD>
+__host__ __device__ __forceinline__ void stencil(double u, int n, int &base0, double w[3]) {
+    if (METHOD == 0) cic_weights(u, n, base0, w); else tsc_weights(u, n, base0, w);
+}
+
+// ------------------------------------------------------------------ descriptors in HBM
+struct Grid { double x0, y0, width, height; };
+
+// Per-beam slice table, written by the slicing kernels.
+struct SliceTable {
+    double boundary[257];
+    double center[256];
+    double weight[256];
+    int64_t count[256];
+    int64_t offset[256];
+    double zmin, zmax, mean, sigma;
+    int32_t ns;
+    int32_t degenerate;
+};
+constexpr int kMaxSlices = 256;
+
+// One directed interaction of a wavefront batch (a source slice solved on a mesh, kicking a field slice).
+struct Inter {
+    int32_t p, i, j, dir;          // pair position in collision order, slices, direction (1, 2)
+    int32_t src_beam, fld_beam;    // 0 = beam 1, 1 = beam 2
+    int64_t src_off, src_n, fld_off, fld_n;
+    double c_src, c_fld, lb, rb;   // source centre; field slice centre and boundaries
+    double sL, sR, sM;             // source drifts to the field-slice boundaries / luminosity plane
+    double kick_scale;             // 2 * kbb
+    Grid sg, fg;                   // meshes (slice_pair mode)
+    int32_t plane0;                // first field plane (L, R[, M | Z]) of this interaction in the batch
+    int32_t nplanes;               // 2 or 3
+    int32_t event;                 // 0 none, 1 hit, 2 miss, 3 rebuild
+    int32_t valid;
+    Grid fgR;                      // node mode: the right node's field mesh
+};
+
+// One deposited plane: a source slice drifted by s onto a mesh.
+struct Plane {
+    int32_t inter;                 // owning interaction (or pair, for luminosity planes)
+    int32_t kind;                  // 0 field plane, 1 luminosity plane
+    int32_t beam;                  // source beam
+    int32_t n1, n2;                // nodes along x and y of the deposit mesh
+    int32_t slot;                  // Green spectrum slot (field planes)
+    int32_t build;                 // unused for planes
+    int32_t reserved;
+    int64_t off, n;                // particle range in the slice-sorted state
+    double s;                      // drift
+    double x0, y0, hxi, hyi;       // deposit mesh origin and inverse spacing
+    double hx, hy;                 // spacing (field derivative)
+};
+
+// A Green table to build into a cache slot.
+struct GreenJob {
+    int32_t slot;
+    int32_t valid;
+    double fx0, fy0, sx0, sy0, hx, hy;
+};
+
+struct ErrorInfo {
+    int32_t code;                  // OCTOPIC_ERR_* or 0
+    int32_t p, dir, stage;
+    double a, b;
+};
+
+}  // namespace octopic

@@ -1,0 +1,169 @@
+// k_misc.hip -- the turn around the collision: linear one-turn map, synthetic beams, stencil probe.
+//
+//   linear6d : track_particle(::Symplectic6DMap, ::Linear6D, ...) (src/elements/linear6d.jl:110-118).
+//   generate : Beam(N, ...) counter-RNG construction (src/beam/Beam.jl:127-181, 545-553) with the
+//              Philox4x32-10 stream of src/math/counter_rng.jl:184-277 and Box-Muller (153-173).
+#include "common.h"
+#include "launch.h"
+
+namespace octopic {
+
+struct Mat36 { double m[36]; };
+
+__global__ __launch_bounds__(256) void k_linear6d(Mat36 M, double *x, double *px, double *y, double *py, double *z,
+                                                  double *pz, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v[6] = {x[i], px[i], y[i], py[i], z[i], pz[i]};
+        double o[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            const double *q = M.m + 6 * r;
+            o[r] = q[0] * v[0] + q[1] * v[1] + q[2] * v[2] + q[3] * v[3] + q[4] * v[4] + q[5] * v[5];
+        }
+        x[i] = o[0]; px[i] = o[1]; y[i] = o[2]; py[i] = o[3]; z[i] = o[4]; pz[i] = o[5];
+    }
+}
+
+hipError_t launch_linear6d(const double *m36, double *const c[6], int64_t n, hipStream_t s) {
+    Mat36 M;
+    for (int k = 0; k < 36; ++k) M.m[k] = m36[k];
+    int g = (int)((n + 255) / 256);
+    if (g > 16384) g = 16384;
+    hipLaunchKernelGGL(k_linear6d, dim3(g), dim3(256), 0, s, M, c[0], c[1], c[2], c[3], c[4], c[5], n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ Philox4x32-10 counter RNG
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    uint64_t z = x + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t counter_uint64(uint64_t seed, uint64_t turn, uint64_t rng_id, uint64_t particle,
+                                                   uint64_t component) {
+    uint32_t c0 = (uint32_t)particle, c1 = (uint32_t)(particle >> 32), c2 = (uint32_t)turn, c3 = (uint32_t)(turn >> 32);
+    const uint64_t key = splitmix64(seed) ^ splitmix64(rng_id + 0x9e3779b97f4a7c15ull) ^
+                         splitmix64(component + 0xbf58476d1ce4e5b9ull);
+    uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1,
+                       n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return ((uint64_t)c0 << 32) | (uint64_t)c1;
+}
+
+__device__ __forceinline__ double uniform_open01(uint64_t v) { return ((double)(v >> 12) + 0.5) * 2.220446049250313e-16; }
+
+// One coordinate d (0..5) of every particle: octopus_normal(seed, philox, 0, rng_id, i, d + 1), clamped.
+__global__ __launch_bounds__(256) void k_normals(double *out, int64_t n, uint64_t seed, uint64_t rng_id, int d, double cutoff) {
+    const int pair_id = d / 2 + 1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t part = (uint64_t)(i + 1);
+        const double u1 = uniform_open01(counter_uint64(seed, 0, rng_id, part, 2 * pair_id - 1));
+        const double u2 = uniform_open01(counter_uint64(seed, 0, rng_id, part, 2 * pair_id));
+        const double r = sqrt(-2.0 * log(u1));
+        const double th = 6.283185307179586476925286766559 * u2;
+        double v = (d % 2 == 0) ? r * cos(th) : r * sin(th);
+        if (__builtin_isfinite(cutoff)) v = jclamp(v, -cutoff, cutoff);
+        out[i] = v;
+    }
+}
+
+// Deterministic two-level sum (fixed partition) of v or (v - mu)^2.
+__global__ __launch_bounds__(256) void k_partial_sum(const double *v, int64_t n, const double *mu, double *partial) {
+    __shared__ double red[256];
+    const double m = mu ? *mu : 0.0;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double a = mu ? (v[i] - m) * (v[i] - m) : v[i];
+        acc += a;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_finish_sum(const double *partial, int nb, int64_t n, double *out, int sqrt_it) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += partial[b];
+    s = s / (double)n;
+    *out = sqrt_it ? sqrt(s > 0.0 ? s : 0.0) : s;
+}
+
+__global__ __launch_bounds__(256) void k_standardize_scale(double *v, int64_t n, const double *mu, const double *sd, double mul) {
+    const double m = *mu, s = *sd;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = s == 0.0 ? 0.0 : ((v[i] - m) / s) * mul;
+}
+
+__global__ __launch_bounds__(256) void k_alpha(double *q, double *p, int64_t n, double alpha, double fac) {
+    // p <- (sigma/beta) * (p_std - alpha * q_std), applied before q is scaled; fac = sigma/beta
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = fac * (p[i] - alpha * q[i]);
+}
+
+hipError_t launch_generate(double *const c[6], int64_t n, uint64_t seed, uint64_t rng_id, double cutoff, hipStream_t s) {
+    // Standard-normal coordinates, standardised to mean 0 / rms 1; the Twiss scaling is applied by
+    // the host wrapper (octopic_generate_gaussian_beam) with launch_scale_beam.
+    int g = (int)((n + 255) / 256);
+    if (g > 2048) g = 2048;
+    double *scratch = nullptr;
+    hipError_t e = hipMallocAsync((void **)&scratch, sizeof(double) * (g + 2), s);
+    if (e != hipSuccess) return e;
+    for (int d = 0; d < 6; ++d) {
+        hipLaunchKernelGGL(k_normals, dim3(g), dim3(256), 0, s, c[d], n, seed, rng_id, d, cutoff);
+        hipLaunchKernelGGL(k_partial_sum, dim3(g), dim3(256), 0, s, c[d], n, (const double *)nullptr, scratch);
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(64), 0, s, scratch, g, n, scratch + g, 0);
+        hipLaunchKernelGGL(k_partial_sum, dim3(g), dim3(256), 0, s, c[d], n, (const double *)(scratch + g), scratch);
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(64), 0, s, scratch, g, n, scratch + g + 1, 1);
+        hipLaunchKernelGGL(k_standardize_scale, dim3(g), dim3(256), 0, s, c[d], n, (const double *)(scratch + g),
+                           (const double *)(scratch + g + 1), 1.0);
+    }
+    hipFreeAsync(scratch, s);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_scale(double *v, int64_t n, double mul) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) v[i] *= mul;
+}
+
+hipError_t launch_scale_beam(double *const c[6], int64_t n, const double *beta, const double *alpha, const double *sigma,
+                             hipStream_t s) {
+    int g = (int)((n + 255) / 256);
+    if (g > 4096) g = 4096;
+    for (int k = 0; k < 3; ++k) {   // _normalize_sigma! (Beam.jl:545-553)
+        hipLaunchKernelGGL(k_alpha, dim3(g), dim3(256), 0, s, c[2 * k], c[2 * k + 1], n, alpha[k], sigma[k] / beta[k]);
+        hipLaunchKernelGGL(k_scale, dim3(g), dim3(256), 0, s, c[2 * k], n, sigma[k]);
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ stencil probe
+__global__ void k_debug_weights(int method, const double *u, int64_t n, int npts, int *base, double *w) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int b;
+    double ww[3];
+    if (method == 0) stencil<0>(u[i], npts, b, ww); else stencil<1>(u[i], npts, b, ww);
+    base[i] = b;
+    w[3 * i] = ww[0]; w[3 * i + 1] = ww[1]; w[3 * i + 2] = ww[2];
+}
+
+hipError_t launch_debug_weights(int method, const double *u, int64_t n, int npts, int *base, double *w, hipStream_t s) {
+    hipLaunchKernelGGL(k_debug_weights, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, method, u, n, npts, base, w);
+    return hipGetLastError();
+}
+
+}  // namespace octopic

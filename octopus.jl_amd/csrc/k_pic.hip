@@ -1,0 +1,533 @@
+// k_pic.hip -- particle-streaming stages of one wavefront batch of slice pairs.
+//
+//   bounds    : one pass over each slice of a pair (pic_cpu.jl:961-1035, 2700-2706): the slice's
+//               source box at the field-slice boundary drifts sL/sR, its field box at the per-particle
+//               drift 1/2(z - c_other), and its luminosity box at sM.  Exact min/max, any order.
+//   geometry  : one lane per pair: the meshes (_pic_interaction_grids, pic_cpu.jl:1674-1718), the
+//               slice-pair Green-cache decision (_pic_slice_pair_green!, 1729-1808) against the
+//               cache geometry resident in HBM, the luminosity mesh (2700-2717), and the plane,
+//               interaction and Green-build descriptors for the later stages.  No host round trip.
+//   deposit   : CIC/TSC charge (pic_cpu.jl:2015-2163) into an LDS-privatised fixed-point (2^-k)
+//               uint64 copy of the mesh, flushed by 64-bit integer atomics: integer sums are
+//               associative, so the charge is bit-identical for every launch geometry.
+//   kick      : forward virtual drift, field gather (x outer, y inner), px/py/pz kick, back drift
+//               and the pz slingshot terms, fused (pic_cpu.jl:1218-1273, 2596-2680, 1448-1538).
+//   lum       : per-pair overlap sum q1.q2 over the full (lnx+1)(lny+1) node extent (2731-2735).
+#include "common.h"
+#include "launch.h"
+
+namespace octopic {
+
+__device__ __forceinline__ bool failed(const ErrorInfo *err) { return *((volatile const int32_t *)&err->code) != 0; }
+
+__device__ __forceinline__ void set_error(ErrorInfo *err, int code, int p, int dir, int stage, double a, double b) {
+    if (atomicCAS(&err->code, 0, code) == 0) {
+        err->p = p; err->dir = dir; err->stage = stage; err->a = a; err->b = b;
+    }
+}
+
+// ---------------------------------------------------------------------------------- bounds
+constexpr int kBoundsThreads = 256;
+
+__global__ __launch_bounds__(kBoundsThreads) void k_bounds(BoundsArgs a) {
+    if (failed(a.err)) return;
+    const int seg = blockIdx.y, k = seg >> 1, side = seg & 1;
+    const BatchPair pr = a.pairs[k];
+    const SliceTable *own = side == 0 ? a.tab1 : a.tab2;
+    const SliceTable *oth = side == 0 ? a.tab2 : a.tab1;
+    const int so = side == 0 ? pr.i : pr.j, sx = side == 0 ? pr.j : pr.i;
+    const double c_own = own->center[so], c_oth = oth->center[sx];
+    const double sL = 0.5 * (c_own - oth->boundary[sx]);
+    const double sR = 0.5 * (c_own - oth->boundary[sx + 1]);
+    const double sM = 0.5 * (c_own - c_oth);
+    const int64_t off = own->offset[so], n = own->count[so];
+    const int64_t lo = n * blockIdx.x / a.nchunks, hi = n * (blockIdx.x + 1) / a.nchunks;
+    const double *X = a.st.c[side][0] + off, *PX = a.st.c[side][1] + off, *Y = a.st.c[side][2] + off,
+                 *PY = a.st.c[side][3] + off, *Z = a.st.c[side][4] + off;
+    double v[12];
+    for (int q = 0; q < 12; q += 2) { v[q] = INFINITY; v[q + 1] = -INFINITY; }
+    for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+        const double x = X[t], px = PX[t], y = Y[t], py = PY[t], z = Z[t];
+        const double xl = x + px * sL, yl = y + py * sL, xr = x + px * sR, yr = y + py * sR;
+        v[0] = jmin(jmin(v[0], xl), xr); v[1] = jmax(jmax(v[1], xl), xr);
+        v[2] = jmin(jmin(v[2], yl), yr); v[3] = jmax(jmax(v[3], yl), yr);
+        const double s = 0.5 * (z - c_oth);
+        const double xf = x + s * px, yf = y + s * py;
+        v[4] = jmin(v[4], xf); v[5] = jmax(v[5], xf); v[6] = jmin(v[6], yf); v[7] = jmax(v[7], yf);
+        const double xv = x + px * sM, yv = y + py * sM;
+        v[8] = jmin(v[8], xv); v[9] = jmax(v[9], xv); v[10] = jmin(v[10], yv); v[11] = jmax(v[11], yv);
+    }
+    __shared__ double red[12][kBoundsThreads];
+    for (int q = 0; q < 12; ++q) red[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int w = kBoundsThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < 12; q += 2) {
+                red[q][threadIdx.x] = jmin(red[q][threadIdx.x], red[q][threadIdx.x + w]);
+                red[q + 1][threadIdx.x] = jmax(red[q + 1][threadIdx.x], red[q + 1][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 12) a.partial[((size_t)seg * a.nchunks + blockIdx.x) * 12 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+hipError_t launch_bounds(const BoundsArgs &a, hipStream_t s) {
+    if (a.npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bounds, dim3(a.nchunks, 2 * a.npairs), dim3(kBoundsThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------- geometry
+__device__ __forceinline__ void reduce_partials(const double *partial, int seg, int nchunks, double v[12]) {
+    for (int q = 0; q < 12; q += 2) { v[q] = INFINITY; v[q + 1] = -INFINITY; }
+    for (int c = 0; c < nchunks; ++c) {
+        const double *p = partial + ((size_t)seg * nchunks + c) * 12;
+        for (int q = 0; q < 12; q += 2) { v[q] = jmin(v[q], p[q]); v[q + 1] = jmax(v[q + 1], p[q + 1]); }
+    }
+}
+
+__device__ __forceinline__ bool resolve_extent(double xspan, double yspan, double fx, double fy, double &w, double &h) {
+    w = jmax(xspan, fx);
+    h = jmax(yspan, fy);
+    return __builtin_isfinite(w) && __builtin_isfinite(h) && w > 0.0 && h > 0.0;
+}
+
+__device__ __forceinline__ void align_origins(int gtype, double s0, double f0, double h, double &so, double &fo) {
+    const double f1 = s0 / h - floor(s0 / h);
+    const double f2 = f0 / h - floor(f0 / h);
+    double t = (f2 - f1) / 2.0;
+    if (gtype == OCTOPIC_GREEN_STANDARD) {
+        t += t > 0.0 ? -0.25 : 0.25;
+    } else {
+        if (t > 0.5) t -= 0.5;
+        else if (t < -0.5) t += 0.5;
+    }
+    const double shift = t * h;
+    so = s0 + shift;
+    fo = f0 - shift;
+}
+
+__device__ __forceinline__ double quantize_extent(double w, double q) {
+    if (!(q > 0.0 && __builtin_isfinite(w) && w > 0.0)) return w;
+    return exp2(ceil(log2(w) / q) * q);
+}
+
+// _pic_interaction_grids (pic_cpu.jl:1674-1718).  Returns false for a degenerate extent.
+__device__ bool interaction_grids(const GeomOpts &o, double sxmin, double sxmax, double symin, double symax, double fxmin,
+                                  double fxmax, double fymin, double fymax, Grid &sg, Grid &fg) {
+    double width, height;
+    if (!resolve_extent(jmax(sxmax - sxmin, fxmax - fxmin), jmax(symax - symin, fymax - fymin), o.min_x, o.min_y, width,
+                        height))
+        return false;
+    const double tx = width / (double)(o.nx - 4);
+    const double ty = height / (double)(o.ny - 4);
+    width += 3.0 * tx;
+    height += 3.0 * ty;
+    if (o.quantize > 0.0) {
+        width = quantize_extent(width, o.quantize);
+        height = quantize_extent(height, o.quantize);
+        const double hx = width / (double)(o.nx - 1);
+        const double hy = height / (double)(o.ny - 1);
+        double sx0 = rint(((sxmin + sxmax) / 2.0 - width / 2.0) / hx) * hx;
+        double sy0 = rint(((symin + symax) / 2.0 - height / 2.0) / hy) * hy;
+        double fx0 = rint(((fxmin + fxmax) / 2.0 - width / 2.0) / hx) * hx;
+        double fy0 = rint(((fymin + fymax) / 2.0 - height / 2.0) / hy) * hy;
+        align_origins(o.green_type, sx0, fx0, hx, sx0, fx0);
+        align_origins(o.green_type, sy0, fy0, hy, sy0, fy0);
+        sg = Grid{sx0, sy0, width, height};
+        fg = Grid{fx0, fy0, width, height};
+        return true;
+    }
+    double sx0 = sxmin - 1.5 * tx, sy0 = symin - 1.5 * ty;
+    double fx0 = fxmin - 1.5 * tx, fy0 = fymin - 1.5 * ty;
+    align_origins(o.green_type, sx0, fx0, tx, sx0, fx0);
+    align_origins(o.green_type, sy0, fy0, ty, sy0, fy0);
+    sg = Grid{sx0, sy0, width, height};
+    fg = Grid{fx0, fy0, width, height};
+    return true;
+}
+
+__device__ __forceinline__ bool size_usable(const Grid &c, const Grid &r, double min_ratio) {
+    if (!(c.width >= r.width)) return false;
+    if (!(c.height >= r.height)) return false;
+    if (!(r.width >= min_ratio * c.width)) return false;
+    if (!(r.height >= min_ratio * c.height)) return false;
+    return true;
+}
+__device__ __forceinline__ bool covers(const GeomOpts &o, const Grid &g, double xmin, double xmax, double ymin, double ymax) {
+    const double hx = g.width / (double)(o.nx - 1);
+    const double hy = g.height / (double)(o.ny - 1);
+    if (!(g.x0 + kTemplateMargin * hx <= xmin)) return false;
+    if (!(g.x0 + g.width - kTemplateMargin * hx >= xmax)) return false;
+    if (!(g.y0 + kTemplateMargin * hy <= ymin)) return false;
+    if (!(g.y0 + g.height - kTemplateMargin * hy >= ymax)) return false;
+    return true;
+}
+__device__ __forceinline__ Grid expand_grid(const Grid &g, double factor) {
+    if (factor <= 1.0) return g;
+    const double nw = g.width * factor, nh = g.height * factor;
+    return Grid{g.x0 - (nw - g.width) / 2.0, g.y0 - (nh - g.height) / 2.0, nw, nh};
+}
+
+__device__ __forceinline__ void fill_record(octopic_interaction_record &r, int p, int i, int j, int dir, int event,
+                                            const Grid &sg, const Grid &fg) {
+    r.p = p; r.i = i; r.j = j; r.dir = dir; r.cache_event = event; r.reserved = 0;
+    r.source_grid[0] = sg.x0; r.source_grid[1] = sg.y0; r.source_grid[2] = sg.width; r.source_grid[3] = sg.height;
+    r.field_grid[0] = fg.x0; r.field_grid[1] = fg.y0; r.field_grid[2] = fg.width; r.field_grid[3] = fg.height;
+}
+
+__device__ __forceinline__ void set_plane(Plane &pl, int inter, int kind, int beam, int n1, int n2, int slot, int64_t off,
+                                          int64_t n, double s, double x0, double y0, double hx, double hy) {
+    pl.inter = inter; pl.kind = kind; pl.beam = beam; pl.n1 = n1; pl.n2 = n2; pl.slot = slot; pl.build = 0;
+    pl.reserved = 0; pl.off = off; pl.n = n; pl.s = s; pl.x0 = x0; pl.y0 = y0;
+    pl.hx = hx; pl.hy = hy; pl.hxi = 1.0 / hx; pl.hyi = 1.0 / hy;
+}
+
+// Luminosity mesh of one pair (pic_cpu.jl:2700-2717) and its two deposit planes.
+__device__ bool lum_geometry(const GeomOpts &o, const double *v1, const double *v2, int k, int p, const SliceTable *t1,
+                             const SliceTable *t2, int i, int j, Plane *planes, int lum_plane0, double *lumgeo, ErrorInfo *err) {
+    double xmin = jmin(v1[8], v2[8]), xmax = jmax(v1[9], v2[9]);
+    double ymin = jmin(v1[10], v2[10]), ymax = jmax(v1[11], v2[11]);
+    double width, height;
+    if (!resolve_extent(xmax - xmin, ymax - ymin, o.min_x, o.min_y, width, height)) {
+        set_error(err, OCTOPIC_ERR_DEGENERATE_EXTENT, p, 0, 2, xmax - xmin, ymax - ymin);
+        return false;
+    }
+    const double tx = width / ((double)o.lum_nx - 1.1);
+    const double ty = height / ((double)o.lum_ny - 1.1);
+    width += 0.1 * tx;
+    height += 0.1 * ty;
+    xmin -= 0.05 * tx;
+    ymin -= 0.05 * ty;
+    const double hx = width / (double)(o.lum_nx - 1);
+    const double hy = height / (double)(o.lum_ny - 1);
+    const double c1 = t1->center[i], c2 = t2->center[j];
+    set_plane(planes[lum_plane0 + 2 * k], k, 1, 0, o.lum_nx + 1, o.lum_ny + 1, -1, t1->offset[i], t1->count[i],
+              0.5 * (c1 - c2), xmin, ymin, hx, hy);
+    set_plane(planes[lum_plane0 + 2 * k + 1], k, 1, 1, o.lum_nx + 1, o.lum_ny + 1, -1, t2->offset[j], t2->count[j],
+              0.5 * (c2 - c1), xmin, ymin, hx, hy);
+    lumgeo[3 * k + 0] = 1.0 / hx;
+    lumgeo[3 * k + 1] = 1.0 / hy;
+    lumgeo[3 * k + 2] = o.klum;
+    return true;
+}
+
+__global__ void k_geometry(GeometryArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.npairs) return;
+    if (failed(a.err)) return;
+    const GeomOpts &o = a.o;
+    const BatchPair pr = a.pairs[k];
+    const int i = pr.i, j = pr.j, p = pr.p;
+    const SliceTable *t1 = a.tab1, *t2 = a.tab2;
+    double v1[12], v2[12];   // side 0 = slice i of beam 1, side 1 = slice j of beam 2
+    reduce_partials(a.partial, 2 * k, a.nchunks, v1);
+    reduce_partials(a.partial, 2 * k + 1, a.nchunks, v2);
+    for (int dir = 1; dir <= 2; ++dir) {
+        const int d = 2 * k + dir - 1;
+        const double *vs = dir == 1 ? v1 : v2;   // source slice bounds
+        const double *vf = dir == 1 ? v2 : v1;   // field slice bounds
+        const SliceTable *ts = dir == 1 ? t1 : t2, *tf = dir == 1 ? t2 : t1;
+        const int si = dir == 1 ? i : j, fi = dir == 1 ? j : i;
+        Inter &it = a.inters[d];
+        it.valid = 0;
+        if (!finite4(vs[0], vs[1], vs[2], vs[3])) { set_error(a.err, OCTOPIC_ERR_NONFINITE, p, dir, 0, 0, 0); return; }
+        if (!finite4(vf[4], vf[5], vf[6], vf[7])) { set_error(a.err, OCTOPIC_ERR_NONFINITE, p, dir, 1, 0, 0); return; }
+        Grid sg, fg;
+        if (!interaction_grids(o, vs[0], vs[1], vs[2], vs[3], vf[4], vf[5], vf[6], vf[7], sg, fg)) {
+            set_error(a.err, OCTOPIC_ERR_DEGENERATE_EXTENT, p, dir, 3, jmax(vs[1] - vs[0], vf[5] - vf[4]),
+                      jmax(vs[3] - vs[2], vf[7] - vf[6]));
+            return;
+        }
+        int event = 0, slot, build;
+        if (o.green_cache == OCTOPIC_CACHE_SLICE_PAIR) {
+            const int key = (i * o.ns2 + j) * 2 + (dir - 1);
+            CacheGeo &e = a.cache[key];
+            bool usable = false;
+            if (e.valid)
+                usable = size_usable(e.sg, sg, o.min_ratio) && size_usable(e.fg, fg, o.min_ratio) &&
+                         covers(o, e.sg, vs[0], vs[1], vs[2], vs[3]) && covers(o, e.fg, vf[4], vf[5], vf[6], vf[7]);
+            slot = key;
+            if (usable) {
+                event = 1;
+                build = 0;
+                sg = e.sg;
+                fg = e.fg;
+                atomicAdd(&a.counters[0], 1ull);
+            } else {
+                Grid es = expand_grid(sg, 1.0 + o.growth);
+                Grid ef = expand_grid(fg, 1.0 + o.growth);
+                if (o.green_type != OCTOPIC_GREEN_INTEGRATED) {   // _pic_realign_expanded_grids (pic_cpu.jl:1849-1858)
+                    const double hx = es.width / (double)(o.nx - 1);
+                    const double hy = es.height / (double)(o.ny - 1);
+                    double sx0, fx0, sy0, fy0;
+                    align_origins(o.green_type, es.x0, ef.x0, hx, sx0, fx0);
+                    align_origins(o.green_type, es.y0, ef.y0, hy, sy0, fy0);
+                    es.x0 = sx0; es.y0 = sy0; ef.x0 = fx0; ef.y0 = fy0;
+                }
+                event = e.valid ? 3 : 2;
+                atomicAdd(&a.counters[e.valid ? 2 : 1], 1ull);
+                e.valid = 1;
+                e.sg = es;
+                e.fg = ef;
+                sg = es;
+                fg = ef;
+                build = 1;
+            }
+        } else {
+            slot = o.temp_slot0 + d;
+            build = 1;
+        }
+        const double hx = sg.width / (double)(o.nx - 1);
+        const double hy = sg.height / (double)(o.ny - 1);
+        GreenJob &job = a.jobs[d];
+        job.slot = slot;
+        job.valid = build;
+        job.fx0 = fg.x0; job.fy0 = fg.y0; job.sx0 = sg.x0; job.sy0 = sg.y0; job.hx = hx; job.hy = hy;
+        if (build) atomicAdd(&a.counters[3], 1ull);
+        const double c_src = ts->center[si];
+        it.p = p; it.i = i; it.j = j; it.dir = dir;
+        it.src_beam = dir == 1 ? 0 : 1;
+        it.fld_beam = dir == 1 ? 1 : 0;
+        it.src_off = ts->offset[si]; it.src_n = ts->count[si];
+        it.fld_off = tf->offset[fi]; it.fld_n = tf->count[fi];
+        it.c_src = c_src; it.c_fld = tf->center[fi];
+        it.lb = tf->boundary[fi]; it.rb = tf->boundary[fi + 1];
+        it.sL = 0.5 * (c_src - it.lb);
+        it.sR = 0.5 * (c_src - it.rb);
+        it.sM = 0.5 * (c_src - it.c_fld);
+        it.kick_scale = 2.0 * (dir == 1 ? o.kbb2 : o.kbb1);
+        it.sg = sg; it.fg = fg; it.fgR = fg;
+        it.plane0 = d * o.npl;
+        it.nplanes = o.npl;
+        it.event = event;
+        it.valid = 1;
+        const int src_beam = it.src_beam;
+        set_plane(a.planes[d * o.npl + 0], d, 0, src_beam, o.nx, o.ny, slot, it.src_off, it.src_n, it.sL, sg.x0, sg.y0, hx, hy);
+        set_plane(a.planes[d * o.npl + 1], d, 0, src_beam, o.nx, o.ny, slot, it.src_off, it.src_n, it.sR, sg.x0, sg.y0, hx, hy);
+        if (o.npl == 3)
+            set_plane(a.planes[d * o.npl + 2], d, 0, src_beam, o.nx, o.ny, slot, it.src_off, it.src_n, 0.5 * (it.sL + it.sR),
+                      sg.x0, sg.y0, hx, hy);
+        fill_record(a.records[2 * p + dir - 1], p, i, j, dir, event, sg, fg);
+    }
+    if (o.compute_lum) lum_geometry(o, v1, v2, k, p, t1, t2, i, j, a.planes, 2 * a.npairs * o.npl, a.lumgeo, a.err);
+}
+
+hipError_t launch_geometry(const GeometryArgs &a, hipStream_t s) {
+    if (a.npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_geometry, dim3((a.npairs + 63) / 64), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------- deposit
+constexpr int kDepositThreads = 1024;
+
+template <int METHOD>
+__global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
+    extern __shared__ unsigned long long tile[];
+    if (failed(a.err)) return;
+    const Plane pl = a.planes[blockIdx.y];
+    const int64_t lo = pl.n * blockIdx.x / a.chunks, hi = pl.n * (blockIdx.x + 1) / a.chunks;
+    if (lo >= hi) return;
+    const int n1 = pl.n1, n2 = pl.n2, ncell = n1 * n2;
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) tile[c] = 0ull;
+    __syncthreads();
+    const double *X = a.st.c[pl.beam][0] + pl.off, *PX = a.st.c[pl.beam][1] + pl.off, *Y = a.st.c[pl.beam][2] + pl.off,
+                 *PY = a.st.c[pl.beam][3] + pl.off;
+    const double s = pl.s, x0 = pl.x0, y0 = pl.y0, hxi = pl.hxi, hyi = pl.hyi, scale = a.scale;
+    constexpr int K = METHOD == 0 ? 2 : 3;
+    for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+        const double xd = X[t] + PX[t] * s;
+        const double yd = Y[t] + PY[t] * s;
+        int bx, by;
+        double wx[3], wy[3];
+        stencil<METHOD>((xd - x0) * hxi, n1, bx, wx);
+        stencil<METHOD>((yd - y0) * hyi, n2, by, wy);
+#pragma unroll
+        for (int m = 0; m < K; ++m)
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const unsigned long long v = __double2ull_rn(wx[m] * wy[q] * scale);
+                if (v) atomicAdd(&tile[(by + q) * n1 + bx + m], v);
+            }
+    }
+    __syncthreads();
+    unsigned long long *out = a.charge + (size_t)blockIdx.y * a.charge_stride;
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        const unsigned long long v = tile[c];
+        if (v) atomicAdd(&out[c], v);
+    }
+}
+
+hipError_t launch_deposit(const DepositArgs &a, hipStream_t s) {
+    if (a.nplanes <= 0) return hipSuccess;
+    const size_t lds = (size_t)a.ncells * sizeof(unsigned long long);
+    dim3 g(a.chunks, a.nplanes);
+    if (a.method == 0) {
+        hipFuncSetAttribute((const void *)k_deposit<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_deposit<0>, g, dim3(kDepositThreads), lds, s, a);
+    } else {
+        hipFuncSetAttribute((const void *)k_deposit<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_deposit<1>, g, dim3(kDepositThreads), lds, s, a);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------- kick
+constexpr int kKickThreads = 256;
+
+template <int METHOD, int MODE>
+__global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
+    if (failed(a.err)) return;
+    const Inter it = a.inters[blockIdx.y];
+    if (!it.valid) return;
+    const int64_t n = it.fld_n;
+    const int64_t lo = n * blockIdx.x / a.chunks, hi = n * (blockIdx.x + 1) / a.chunks;
+    const int nx = a.nx, ny = a.ny, C = 3 * a.npl;
+    const double *tab = a.table + (size_t)blockIdx.y * nx * ny * C;
+    double *X = a.st.c[it.fld_beam][0] + it.fld_off, *PX = a.st.c[it.fld_beam][1] + it.fld_off;
+    double *Y = a.st.c[it.fld_beam][2] + it.fld_off, *PY = a.st.c[it.fld_beam][3] + it.fld_off;
+    const double *Z = a.st.c[it.fld_beam][4] + it.fld_off;
+    double *PZ = a.st.c[it.fld_beam][5] + it.fld_off;
+    // _slice_interpolation_parameters (pic_cpu.jl:1540-1548)
+    double hzi, zbias;
+    {
+        const double denom = it.rb - it.lb;
+        if (!__builtin_isfinite(denom) || denom == 0.0) { hzi = 0.0; zbias = 0.5; }
+        else { hzi = 1.0 / denom; zbias = it.rb * hzi; }
+    }
+    const double hx = it.fg.width / (double)(nx - 1), hy = it.fg.height / (double)(ny - 1);
+    const double hxR = it.fgR.width / (double)(nx - 1), hyR = it.fgR.height / (double)(ny - 1);
+    const double c = it.c_src, ks = it.kick_scale;
+    const int lk = a.lkick;
+    constexpr int K = METHOD == 0 ? 2 : 3;
+    for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+        double x = X[t], px = PX[t], y = Y[t], py = PY[t];
+        const double z = Z[t];
+        double pz = PZ[t];
+        const double s = 0.5 * (z - c);
+        x += s * px;
+        y += s * py;
+        if (lk) pz -= 0.25 * (px * px + py * py);
+        const double zL = jclamp(-z * hzi + zbias, 0.0, 1.0);
+        double Kx = 0.0, Ky = 0.0, Kz = 0.0;
+        int bx, by;
+        double wx[3], wy[3];
+        stencil<METHOD>((x - it.fg.x0) / hx, nx, bx, wx);
+        stencil<METHOD>((y - it.fg.y0) / hy, ny, by, wy);
+        if (MODE == 0) {
+            const double zR = 1.0 - zL;
+#pragma unroll
+            for (int m = 0; m < K; ++m)
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const double *e = tab + ((size_t)(bx + m) * ny + (by + q)) * C;
+                    const double w = wx[m] * wy[q];
+                    Kx += w * (zL * e[1] + zR * e[4]);
+                    Ky += w * (zL * e[2] + zR * e[5]);
+                    Kz += w * (e[0] - e[3]);
+                }
+        } else if (MODE == 1) {
+            const double tt = 1.0 - zL;
+            const double t2 = tt * tt;
+            const double aL = 2.0 * t2 - 3.0 * tt + 1.0, aM = 4.0 * tt - 4.0 * t2, aR = 2.0 * t2 - tt;
+            const double bL = 3.0 - 4.0 * tt, bM = 8.0 * tt - 4.0, bR = 1.0 - 4.0 * tt;
+#pragma unroll
+            for (int m = 0; m < K; ++m)
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const double *e = tab + ((size_t)(bx + m) * ny + (by + q)) * C;   // L = 0..2, R = 3..5, M = 6..8
+                    const double w = wx[m] * wy[q];
+                    Kx += w * (aL * e[1] + aM * e[7] + aR * e[4]);
+                    Ky += w * (aL * e[2] + aM * e[8] + aR * e[5]);
+                    Kz += w * (bL * e[0] + bM * e[6] + bR * e[3]);
+                }
+        } else {
+            // :node (pic_cpu.jl:1507-1530): L on its own mesh, R on its own mesh, phi_L - phi_Z on L's mesh
+            double KxL = 0.0, KyL = 0.0, KxR = 0.0, KyR = 0.0;
+#pragma unroll
+            for (int m = 0; m < K; ++m)
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const double *e = tab + ((size_t)(bx + m) * ny + (by + q)) * C;
+                    const double w = wx[m] * wy[q];
+                    KxL += w * (1.0 * e[1] + 0.0 * e[1]);
+                    KyL += w * (1.0 * e[2] + 0.0 * e[2]);
+                    Kz += w * (e[0] - e[6]);
+                }
+            int bxr, byr;
+            double wxr[3], wyr[3];
+            stencil<METHOD>((x - it.fgR.x0) / hxR, nx, bxr, wxr);
+            stencil<METHOD>((y - it.fgR.y0) / hyR, ny, byr, wyr);
+#pragma unroll
+            for (int m = 0; m < K; ++m)
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const double *e = tab + ((size_t)(bxr + m) * ny + (byr + q)) * C;
+                    const double w = wxr[m] * wyr[q];
+                    KxR += w * (1.0 * e[4] + 0.0 * e[4]);
+                    KyR += w * (1.0 * e[5] + 0.0 * e[5]);
+                }
+            const double zR = 1.0 - zL;
+            Kx = zL * KxL + zR * KxR;
+            Ky = zL * KyL + zR * KyR;
+        }
+        px += ks * Kx;
+        py += ks * Ky;
+        if (lk) pz += ks * Kz * hzi;
+        const double s2 = 0.5 * (c - z);
+        x += s2 * px;
+        y += s2 * py;
+        if (lk) pz += 0.25 * (px * px + py * py);
+        X[t] = x; PX[t] = px; Y[t] = y; PY[t] = py; PZ[t] = pz;
+    }
+}
+
+hipError_t launch_kick(const KickArgs &a, hipStream_t s) {
+    if (a.ninters <= 0) return hipSuccess;
+    dim3 g(a.chunks, a.ninters), b(kKickThreads);
+#define OCTOPIC_KICK(M, MODE) hipLaunchKernelGGL((k_kick<M, MODE>), g, b, 0, s, a)
+    if (a.method == 0) {
+        if (a.mode == 0) OCTOPIC_KICK(0, 0); else if (a.mode == 1) OCTOPIC_KICK(0, 1); else OCTOPIC_KICK(0, 2);
+    } else {
+        if (a.mode == 0) OCTOPIC_KICK(1, 0); else if (a.mode == 1) OCTOPIC_KICK(1, 1); else OCTOPIC_KICK(1, 2);
+    }
+#undef OCTOPIC_KICK
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------- luminosity
+constexpr int kLumThreads = 256;
+
+__global__ __launch_bounds__(kLumThreads) void k_lum(LumArgs a) {
+    const int k = blockIdx.x;
+    unsigned long long *q1 = a.charge + (size_t)(2 * k) * a.charge_stride;
+    unsigned long long *q2 = a.charge + (size_t)(2 * k + 1) * a.charge_stride;
+    const double inv = a.inv_scale;
+    double acc = 0.0;
+    for (int c = threadIdx.x; c < a.ncells; c += blockDim.x) {
+        const unsigned long long u1 = q1[c], u2 = q2[c];
+        if (u1) q1[c] = 0ull;
+        if (u2) q2[c] = 0ull;
+        acc += ((double)u1 * inv) * ((double)u2 * inv);
+    }
+    __shared__ double red[kLumThreads];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kLumThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && !failed(a.err)) {
+        const double *g = a.lumgeo + 3 * k;
+        a.lum_parts[a.pairs[k].p] = red[0] * g[2] * g[0] * g[1];
+    }
+}
+
+hipError_t launch_lum(const LumArgs &a, hipStream_t s) {
+    if (a.npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lum, dim3(a.npairs), dim3(kLumThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace octopic

@@ -1,0 +1,379 @@
+// k_slicing.hip -- longitudinal slicing on the device (src/tasks/strongstrong/slicing.jl).
+//
+//   zlanes/zfold : _live_z_stats (slicing.jl:271-289): exact min/max, and the mean and sigma through
+//                  the canonical 4096-lane fold (lane t sums z[t], z[t+4096], ... in order; the lanes
+//                  are folded serially 0..4095), so the values are bit-identical to the CPU fold.
+//   boundaries   : :normal_quantile (slicing.jl:405-429), :equal_area (104-156, histogram of
+//                  ns*resolution bins), :equal_width (367-387), :specified (389-403).
+//   classify     : s = clamp(searchsortedlast(boundaries, z), 1, ns) per particle (slicing.jl:431-460)
+//                  plus a per-block slice histogram.
+//   scan/scatter : stable counting sort: slice-major exclusive scan of the block histograms, then
+//                  each particle's rank among same-slice particles of its block from wave ballots.
+//                  Member lists come out ascending, as the reference's index lists.  The scatter also
+//                  gathers the six coordinates into slice-contiguous state arrays.
+//   centroid     : the same 4096-lane fold over each member list (slicing.jl:245-257, 523-559).
+#include "common.h"
+#include "launch.h"
+
+namespace octopic {
+
+constexpr int kSliceBlock = 1024;     // particles per classify/scatter block
+constexpr int kSliceThreads = 256;
+
+__global__ __launch_bounds__(256) void k_zlanes(const double *z, int64_t n, const SliceTable *tab, int pass, double *lanes,
+                                                double *mm_partial) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;   // lane, 0..4095
+    const double mu = pass == 2 ? tab->mean : 0.0;
+    double acc = 0.0;
+    double lo = INFINITY, hi = -INFINITY;
+    int64_t i = t;
+    // Loads are issued ahead of the in-order accumulation; the summation order is the lane fold's.
+    for (; i + 3 * kFoldLanes < n; i += 4 * kFoldLanes) {
+        double v0 = z[i], v1 = z[i + kFoldLanes], v2 = z[i + 2 * kFoldLanes], v3 = z[i + 3 * kFoldLanes];
+        if (pass == 2) {
+            v0 = (v0 - mu) * (v0 - mu); v1 = (v1 - mu) * (v1 - mu);
+            v2 = (v2 - mu) * (v2 - mu); v3 = (v3 - mu) * (v3 - mu);
+        } else {
+            lo = jmin(jmin(jmin(jmin(lo, v0), v1), v2), v3);
+            hi = jmax(jmax(jmax(jmax(hi, v0), v1), v2), v3);
+        }
+        acc += v0; acc += v1; acc += v2; acc += v3;
+    }
+    for (; i < n; i += kFoldLanes) {
+        double v = z[i];
+        if (pass == 2) v = (v - mu) * (v - mu);
+        else { lo = jmin(lo, v); hi = jmax(hi, v); }
+        acc += v;
+    }
+    lanes[t] = acc;
+    if (pass == 1) {
+        __shared__ double slo[256], shi[256];
+        slo[threadIdx.x] = lo; shi[threadIdx.x] = hi;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) {
+                slo[threadIdx.x] = jmin(slo[threadIdx.x], slo[threadIdx.x + w]);
+                shi[threadIdx.x] = jmax(shi[threadIdx.x], shi[threadIdx.x + w]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) { mm_partial[2 * blockIdx.x] = slo[0]; mm_partial[2 * blockIdx.x + 1] = shi[0]; }
+    }
+}
+
+// Serial lane fold (one lane) + boundaries.  method: OCTOPIC slicing method; q: normal quantiles
+// sqrt(2)*erfinv(2s/ns - 1) or the sorted `specified` offsets, from the host.
+__global__ void k_zfold(const double *lanes, const double *mm_partial, int64_t n, SliceTable *tab, int pass, int method,
+                        int ns, const double *q, ErrorInfo *err) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int t = 0; t < kFoldLanes; t += 8) {
+        double l0 = lanes[t], l1 = lanes[t + 1], l2 = lanes[t + 2], l3 = lanes[t + 3];
+        double l4 = lanes[t + 4], l5 = lanes[t + 5], l6 = lanes[t + 6], l7 = lanes[t + 7];
+        s += l0; s += l1; s += l2; s += l3; s += l4; s += l5; s += l6; s += l7;
+    }
+    if (pass == 1) {
+        double lo = INFINITY, hi = -INFINITY;
+        for (int b = 0; b < kFoldLanes / 256; ++b) { lo = jmin(lo, mm_partial[2 * b]); hi = jmax(hi, mm_partial[2 * b + 1]); }
+        tab->zmin = lo;
+        tab->zmax = hi;
+        tab->mean = s / (double)n;
+        tab->ns = ns;
+        return;
+    }
+    double sigma = sqrt(jmax(s / (double)n, 0.0));
+    tab->sigma = sigma;
+    const double zmin = tab->zmin, zmax = tab->zmax, mean = tab->mean;
+    double *b = tab->boundary;
+    if (method == 3) {   // normal_quantile
+        if (sigma == 0.0) {
+            for (int k = 0; k <= ns; ++k) b[k] = mean;
+        } else {
+            b[0] = zmin; b[ns] = zmax;
+            for (int k = 1; k < ns; ++k) b[k] = jclamp(mean + sigma * q[k], b[0], b[ns]);
+        }
+    } else if (method == 4) {   // specified: q holds the positions; sort the internal boundaries
+        double tmp[kMaxSlices];
+        int m = ns - 1;
+        for (int k = 0; k < m; ++k) tmp[k] = mean + q[k] * sigma;
+        for (int a = 1; a < m; ++a) {   // stable insertion sort by isless
+            double v = tmp[a];
+            int c = a - 1;
+            while (c >= 0 && jisless(v, tmp[c])) { tmp[c + 1] = tmp[c]; --c; }
+            tmp[c + 1] = v;
+        }
+        b[0] = zmin; b[ns] = zmax;
+        for (int k = 0; k < m; ++k) b[k + 1] = jclamp(tmp[k], b[0], b[ns]);
+    } else if (method == 2) {   // equal_width
+        b[0] = zmin; b[ns] = zmax;
+        double step = (zmax - zmin) / (double)ns;
+        for (int k = 1; k < ns; ++k) b[k] = zmin + (double)k * step;
+    }
+    // equal_area boundaries are produced by k_equal_area after the histogram
+    bool finite = true;
+    if (method != 0)
+        for (int k = 0; k <= ns; ++k) finite = finite && __builtin_isfinite(b[k]);
+    finite = finite && __builtin_isfinite(mean) && __builtin_isfinite(zmin) && __builtin_isfinite(zmax);
+    if (!finite) atomicCAS(&err->code, 0, 3);
+    tab->degenerate = (method == 0 || method == 2) ? (zmin == zmax) : (b[0] == b[ns]);
+}
+
+// :equal_area histogram (_threaded_histogram / _slice_bin, slicing.jl:172-176, 291-328), LDS-privatised.
+__global__ __launch_bounds__(256) void k_equal_area_hist(const double *z, int64_t n, const SliceTable *tab, int bins,
+                                                         unsigned long long *counts) {
+    extern __shared__ unsigned int hcnt[];
+    const double zmin = tab->zmin, zmax = tab->zmax;
+    if (zmin == zmax) return;
+    const double width = (zmax - zmin) / (double)bins;
+    for (int k = threadIdx.x; k < bins; k += blockDim.x) hcnt[k] = 0u;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double d = (z[i] - zmin) / width;
+        if (!(d > -INFINITY && d < INFINITY)) continue;
+        int bin = iclamp((int)floor(d) + 1, 1, bins);
+        atomicAdd(&hcnt[bin - 1], 1u);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < bins; k += blockDim.x)
+        if (hcnt[k]) atomicAdd(&counts[k], (unsigned long long)hcnt[k]);
+}
+
+// :equal_area boundaries (slicing.jl:104-156); the count buffer is reused for the cumulative
+// fractions and reset to zero on exit.
+__global__ void k_equal_area(SliceTable *tab, int64_t n, int ns, int bins, unsigned long long *counts, ErrorInfo *err) {
+    if (threadIdx.x != 0) return;
+    const double zmin = tab->zmin, zmax = tab->zmax;
+    double *b = tab->boundary;
+    if (zmin == zmax) {
+        for (int k = 0; k <= ns; ++k) b[k] = zmin;
+        tab->degenerate = 1;
+    } else {
+        const double width = (zmax - zmin) / (double)bins;
+        b[0] = zmin; b[ns] = zmax;
+        double *cumd = reinterpret_cast<double *>(counts);
+        int64_t acc = 0;
+        for (int k = 0; k < bins; ++k) {
+            acc += (int64_t)counts[k];
+            cumd[k] = (double)acc / (double)(n > 1 ? n : 1);
+        }
+        cumd[bins - 1] = 1.0;
+        int current = 1;   // 1-based bin cursor
+        for (int s = 1; s < ns; ++s) {
+            double target = (double)s / (double)ns;
+            while (current <= bins && cumd[current - 1] <= target) current += 1;
+            double x1, x2, y1, y2;
+            if (current <= 1) {
+                x1 = b[0]; x2 = zmin + (1.0 - 0.5) * width; y1 = 0.0; y2 = cumd[0];
+            } else if (current > bins) {
+                x1 = zmin + ((double)bins - 0.5) * width; x2 = b[ns]; y1 = cumd[bins - 2]; y2 = 1.0;
+            } else {
+                x1 = zmin + ((double)(current - 1) - 0.5) * width;
+                x2 = zmin + ((double)current - 0.5) * width;
+                y1 = cumd[current - 2]; y2 = cumd[current - 1];
+            }
+            b[s] = (y2 == y1) ? (x1 + x2) / 2.0 : x2 * (target - y1) / (y2 - y1) + x1 * (target - y2) / (y1 - y2);
+        }
+        for (int k = 0; k < bins; ++k) counts[k] = 0ull;
+        tab->degenerate = (b[0] == b[ns]);
+    }
+    bool finite = true;
+    for (int k = 0; k <= ns; ++k) finite = finite && __builtin_isfinite(b[k]);
+    if (!finite) atomicCAS(&err->code, 0, 3);
+}
+
+__device__ __forceinline__ int slice_of(double zi, const double *b, int ns) {
+    // searchsortedlast(boundaries, z) over ns+1 entries, 1-based result; then clamp(., 1, ns)
+    int lo = 0, hi = ns + 2;
+    while (lo < hi - 1) {
+        int m = lo + ((hi - lo) >> 1);
+        if (jisless(zi, b[m - 1])) hi = m; else lo = m;
+    }
+    return iclamp(lo, 1, ns) - 1;
+}
+
+__global__ __launch_bounds__(kSliceThreads) void k_classify(const double *z, int64_t n, const SliceTable *tab, uint16_t *sid,
+                                                            int *hist, int nblocks, int by_bin, const ErrorInfo *err) {
+    __shared__ int cnt[kMaxSlices];
+    __shared__ double b[kMaxSlices + 1];
+    const int ns = tab->ns;
+    const bool degen = tab->degenerate != 0;
+    for (int k = threadIdx.x; k < ns; k += blockDim.x) cnt[k] = 0;
+    for (int k = threadIdx.x; k <= ns; k += blockDim.x) b[k] = tab->boundary[k];
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSliceBlock;
+    for (int r = 0; r < kSliceBlock / kSliceThreads; ++r) {
+        int64_t i = base + r * kSliceThreads + threadIdx.x;
+        if (i < n) {
+            int s;
+            if (degen) s = 0;
+            else if (by_bin) {   // :equal_width membership is _slice_bin (slicing.jl:382-385)
+                double d = (z[i] - b[0]) / ((b[ns] - b[0]) / (double)ns);
+                s = (d > -INFINITY && d < INFINITY) ? iclamp((int)floor(d) + 1, 1, ns) - 1 : 0;
+            } else s = slice_of(z[i], b, ns);
+            sid[i] = (uint16_t)s;
+            atomicAdd(&cnt[s], 1);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < ns; k += blockDim.x) hist[(size_t)k * nblocks + blockIdx.x] = cnt[k];
+}
+
+// Exclusive scan of hist (slice-major, ns * nblocks) in place; writes the per-slice count/offset.
+__global__ __launch_bounds__(1024) void k_scan(int *hist, int ns, int nblocks, SliceTable *tab) {
+    __shared__ int64_t part[1024];
+    const int64_t total = (int64_t)ns * nblocks;
+    const int64_t per = (total + blockDim.x - 1) / blockDim.x;
+    const int64_t lo = threadIdx.x * per, hi = lo + per < total ? lo + per : total;
+    int64_t sum = 0;
+    for (int64_t k = lo; k < hi; ++k) sum += hist[k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        for (int t = 0; t < (int)blockDim.x; ++t) { int64_t v = part[t]; part[t] = run; run += v; }
+    }
+    __syncthreads();
+    int64_t run = part[threadIdx.x];
+    for (int64_t k = lo; k < hi; ++k) {
+        int v = hist[k];
+        hist[k] = (int)run;
+        if (k % nblocks == 0) tab->offset[k / nblocks] = run;
+        run += v;
+    }
+}
+
+__global__ void k_counts(SliceTable *tab, int64_t n) {
+    const int ns = tab->ns;
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+        int64_t end = (s + 1 < ns) ? tab->offset[s + 1] : n;
+        tab->count[s] = end - tab->offset[s];
+    }
+}
+
+__global__ __launch_bounds__(kSliceThreads) void k_scatter(const double *const *beam, int64_t n, const uint16_t *sid,
+                                                           const int *hist, int nblocks, int ns, double *const *st,
+                                                           int64_t *perm) {
+    __shared__ int run[kMaxSlices];
+    __shared__ int wcnt[kSliceThreads / 64][kMaxSlices];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = threadIdx.x; k < ns; k += blockDim.x) run[k] = 0;
+    const int64_t base = (int64_t)blockIdx.x * kSliceBlock;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < kSliceBlock / kSliceThreads; ++r) {
+        for (int k = threadIdx.x; k < (kSliceThreads / 64) * kMaxSlices; k += blockDim.x) (&wcnt[0][0])[k] = 0;
+        __syncthreads();
+        const int64_t i = base + r * kSliceThreads + threadIdx.x;
+        const int s = i < n ? (int)sid[i] : -1;
+        int rank = 0;
+        unsigned long long remaining = __ballot(s >= 0);
+        while (remaining) {
+            int leader = __ffsll((long long)remaining) - 1;
+            int ls = __shfl(s, leader);
+            unsigned long long m = __ballot(s == ls);
+            if (s == ls) rank = __popcll(m & lt);
+            if (lane == leader) wcnt[wave][ls] = __popcll(m);
+            remaining &= ~m;
+        }
+        __syncthreads();
+        if (s >= 0) {
+            int pos = hist[(size_t)s * nblocks + blockIdx.x] + run[s] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][s];
+            for (int c = 0; c < 6; ++c) st[c][pos] = beam[c][i];
+            perm[pos] = i;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+            int add = 0;
+            for (int w = 0; w < kSliceThreads / 64; ++w) add += wcnt[w][k];
+            run[k] += add;
+        }
+        __syncthreads();
+    }
+}
+
+// Lane partials of every slice's member z (slice-contiguous state); grid (16, ns).
+__global__ __launch_bounds__(256) void k_centroid_lanes(const double *stz, const SliceTable *tab, double *lanes) {
+    const int s = blockIdx.y;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t off = tab->offset[s], cnt = tab->count[s];
+    double acc = 0.0;
+    int64_t k = t;
+    for (; k + 3 * kFoldLanes < cnt; k += 4 * kFoldLanes) {
+        double v0 = stz[off + k], v1 = stz[off + k + kFoldLanes], v2 = stz[off + k + 2 * kFoldLanes],
+               v3 = stz[off + k + 3 * kFoldLanes];
+        acc += v0; acc += v1; acc += v2; acc += v3;
+    }
+    for (; k < cnt; k += kFoldLanes) acc += stz[off + k];
+    lanes[(size_t)s * kFoldLanes + t] = acc;
+}
+
+__global__ void k_centroid_fold(const double *lanes, SliceTable *tab, int64_t n, int centroid) {
+    const int s = threadIdx.x;
+    const int ns = tab->ns;
+    if (s >= ns) return;
+    const int64_t cnt = tab->count[s];
+    tab->weight[s] = (double)cnt / (double)n;
+    const double *l = lanes + (size_t)s * kFoldLanes;
+    if (centroid && cnt > 0) {
+        double acc = 0.0;
+        for (int t = 0; t < kFoldLanes; t += 8) {
+            double l0 = l[t], l1 = l[t + 1], l2 = l[t + 2], l3 = l[t + 3];
+            double l4 = l[t + 4], l5 = l[t + 5], l6 = l[t + 6], l7 = l[t + 7];
+            acc += l0; acc += l1; acc += l2; acc += l3; acc += l4; acc += l5; acc += l6; acc += l7;
+        }
+        tab->center[s] = acc / (double)cnt;
+    } else {
+        tab->center[s] = (tab->boundary[s] + tab->boundary[s + 1]) / 2.0;
+    }
+}
+
+// Scatter the slice-contiguous states back into the beam (x, px, y, py, pz; z is never modified).
+__global__ __launch_bounds__(256) void k_store(double *const *beam, int64_t n, double *const *st, const int64_t *perm,
+                                               const ErrorInfo *err) {
+    if (err->code != 0) return;
+    for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < n; pos += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = perm[pos];
+        beam[0][i] = st[0][pos];
+        beam[1][i] = st[1][pos];
+        beam[2][i] = st[2][pos];
+        beam[3][i] = st[3][pos];
+        beam[5][i] = st[5][pos];
+    }
+}
+
+// ---------------------------------------------------------------------- launcher
+hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s) {
+    const int nblocks = (int)((a.n + kSliceBlock - 1) / kSliceBlock);
+    hipLaunchKernelGGL(k_zlanes, dim3(kFoldLanes / 256), dim3(256), 0, s, a.z, a.n, a.tab, 1, a.lanes, a.mm_partial);
+    hipLaunchKernelGGL(k_zfold, dim3(1), dim3(64), 0, s, a.lanes, a.mm_partial, a.n, a.tab, 1, a.method, a.ns, a.q, a.err);
+    hipLaunchKernelGGL(k_zlanes, dim3(kFoldLanes / 256), dim3(256), 0, s, a.z, a.n, a.tab, 2, a.lanes, a.mm_partial);
+    hipLaunchKernelGGL(k_zfold, dim3(1), dim3(64), 0, s, a.lanes, a.mm_partial, a.n, a.tab, 2, a.method, a.ns, a.q, a.err);
+    if (a.method == 0) {
+        const int bins = a.ns * a.resolution;
+        int g = (int)((a.n + 255) / 256);
+        if (g > 4096) g = 4096;
+        hipFuncSetAttribute((const void *)k_equal_area_hist, hipFuncAttributeMaxDynamicSharedMemorySize, bins * 4);
+        hipLaunchKernelGGL(k_equal_area_hist, dim3(g), dim3(256), bins * 4, s, a.z, a.n, a.tab, bins, a.bin_counts);
+        hipLaunchKernelGGL(k_equal_area, dim3(1), dim3(64), 0, s, a.tab, a.n, a.ns, bins, a.bin_counts, a.err);
+    }
+    hipLaunchKernelGGL(k_classify, dim3(nblocks), dim3(kSliceThreads), 0, s, a.z, a.n, a.tab, a.sid, a.hist, nblocks, a.method == 2 ? 1 : 0, a.err);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.hist, a.ns, nblocks, a.tab);
+    hipLaunchKernelGGL(k_counts, dim3(1), dim3(256), 0, s, a.tab, a.n);
+    hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(kSliceThreads), 0, s, a.beam, a.n, a.sid, a.hist, nblocks, a.ns, a.st,
+                       a.perm);
+    hipLaunchKernelGGL(k_centroid_lanes, dim3(kFoldLanes / 256, a.ns), dim3(256), 0, s, a.st_z, a.tab, a.centroid_lanes);
+    hipLaunchKernelGGL(k_centroid_fold, dim3(1), dim3(256), 0, s, a.centroid_lanes, a.tab, a.n, a.centroid);
+    return hipGetLastError();
+}
+
+int64_t slicing_hist_entries(int64_t n, int ns) { return ((n + kSliceBlock - 1) / kSliceBlock) * (int64_t)ns; }
+
+hipError_t launch_store(double *const *beam, int64_t n, double *const *st, const int64_t *perm, const ErrorInfo *err,
+                        hipStream_t s) {
+    int g = (int)((n + 255) / 256);
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_store, dim3(g), dim3(256), 0, s, beam, n, st, perm, err);
+    return hipGetLastError();
+}
+
+}  // namespace octopic

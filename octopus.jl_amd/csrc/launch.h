@@ -1,0 +1,175 @@
+// launch.h -- host-side launchers of the octopic kernels (one translation unit per stage family).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+#include "fft.h"
+
+namespace octopic {
+
+// Slice-contiguous particle states of both beams: st.c[beam][coord], coord = x, px, y, py, z, pz.
+struct States {
+    double *c[2][6];
+};
+
+struct SlicingArgs {
+    const double *z;
+    int64_t n;
+    int method, ns, resolution, centroid;
+    const double *q;                 // quantiles (normal_quantile) or positions (specified)
+    SliceTable *tab;
+    double *lanes, *mm_partial, *centroid_lanes;
+    unsigned long long *bin_counts;  // equal_area bins (zero on entry, reset on exit)
+    uint16_t *sid;
+    int *hist;
+    double *const *beam;             // device array of 6 device pointers (input beam)
+    double *const *st;               // device array of 6 device pointers (slice-contiguous state)
+    const double *st_z;
+    int64_t *perm;
+    ErrorInfo *err;
+};
+hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s);
+int64_t slicing_hist_entries(int64_t n, int ns);
+hipError_t launch_store(double *const *beam, int64_t n, double *const *st, const int64_t *perm, const ErrorInfo *err,
+                        hipStream_t s);
+
+struct BatchPair { int32_t p, i, j, reserved; };
+
+struct CacheGeo { int32_t valid, reserved; Grid sg, fg; };
+
+// Static options the geometry kernel needs.
+struct GeomOpts {
+    int nx, ny, lum_nx, lum_ny;
+    int green_type, green_cache, quadratic, interaction_grid, compute_lum;
+    int ns1, ns2;
+    int npl;                     // planes per interaction (2 or 3)
+    int temp_slot0;              // first per-batch temporary Green slot (green_cache = none)
+    double min_x, min_y, quantize, min_ratio, growth;
+    double kbb1, kbb2, klum;
+};
+
+struct BoundsArgs {
+    const BatchPair *pairs;
+    int npairs;
+    const SliceTable *tab1, *tab2;
+    States st;
+    int nchunks;
+    double *partial;             // [2*npairs][nchunks][12]
+    const ErrorInfo *err;
+};
+hipError_t launch_bounds(const BoundsArgs &a, hipStream_t s);
+
+struct GeometryArgs {
+    const BatchPair *pairs;
+    int npairs;
+    const SliceTable *tab1, *tab2;
+    const double *partial;
+    int nchunks;
+    GeomOpts o;
+    CacheGeo *cache;             // [2*ns1*ns2]
+    Inter *inters;               // [2*npairs]
+    Plane *planes;               // field planes [2*npairs*npl], then luminosity planes [2*npairs]
+    GreenJob *jobs;              // [2*npairs]
+    double *lumgeo;              // [npairs][3] = hxi, hyi, klum
+    octopic_interaction_record *records;
+    unsigned long long *counters;   // hits, misses, rebuilds, builds
+    ErrorInfo *err;
+};
+hipError_t launch_geometry(const GeometryArgs &a, hipStream_t s);
+
+struct DepositArgs {
+    const Plane *planes;
+    int nplanes;
+    States st;
+    int method;
+    int ncells;                  // n1*n2 of these planes
+    int chunks;
+    double scale;                // fixed-point scale 2^k
+    unsigned long long *charge;  // [plane][charge_stride]
+    int64_t charge_stride;
+    const ErrorInfo *err;
+};
+hipError_t launch_deposit(const DepositArgs &a, hipStream_t s);
+
+struct KickArgs {
+    const Inter *inters;
+    int ninters;
+    States st;
+    const double *table;
+    int nx, ny, npl, method, mode;   // mode: 0 linear, 1 quadratic, 2 node
+    int lkick;
+    int chunks;
+    const ErrorInfo *err;
+};
+hipError_t launch_kick(const KickArgs &a, hipStream_t s);
+
+struct LumArgs {
+    const BatchPair *pairs;
+    int npairs;
+    unsigned long long *charge;  // luminosity planes, pair k at planes 2k, 2k+1
+    int64_t charge_stride;
+    int ncells;
+    double inv_scale;
+    const double *lumgeo;
+    double *lum_parts;           // indexed by pair position p
+    const ErrorInfo *err;
+};
+hipError_t launch_lum(const LumArgs &a, hipStream_t s);
+
+// FFT stages (k_fft.hip)
+hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride, int nx, int ny,
+                              int integrated, const FFTPlanDev &px, const FFTPlanDev &py, hipStream_t s);
+hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long *charge, int64_t charge_stride, double2 *qh,
+                           int64_t qh_stride, int nx, int ny, double inv_scale, const FFTPlanDev &px, const ErrorInfo *err,
+                           hipStream_t s);
+hipError_t launch_cols_conv(const Plane *planes, int nplanes, double2 *qh, int64_t qh_stride, const double2 *gspec,
+                            int64_t slot_stride, int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, hipStream_t s);
+hipError_t launch_rows_inv(const Plane *planes, int nplanes, const double2 *qh, int64_t qh_stride, double *phi, int nx,
+                           int ny, const FFTPlanDev &px, const ErrorInfo *err, hipStream_t s);
+hipError_t launch_field(const Inter *inters, int ninters, const Plane *planes, const double *phi, double *table, int nx,
+                        int ny, int fourth, int npl, const ErrorInfo *err, hipStream_t s);
+
+// Node-mode mesh preparation (k_node.hip)
+struct NodeArgs {
+    const SliceTable *tab_src, *tab_fld;
+    States st;
+    int src_beam, fld_beam;
+    int dir;                     // 1 or 2
+    int nchunks;
+    double *partial;             // per source slice: [nb][nchunks][4] source; per field slice box: [ns_src][ns_fld][nchunks][4]
+    GeomOpts o;
+    int slot0;                   // first Green slot of this direction's node meshes
+    CacheGeo *nodes;             // [ns_src][nb] node meshes of this direction
+    GreenJob *jobs;              // [ns_src][nb]
+    unsigned long long *counters;
+    ErrorInfo *err;
+};
+hipError_t launch_node_prepare(const NodeArgs &a, hipStream_t s);
+
+struct NodeGeometryArgs {
+    const BatchPair *pairs;
+    int npairs;
+    const SliceTable *tab1, *tab2;
+    const double *partial;       // bounds partials of this batch (luminosity boxes)
+    int nchunks;
+    GeomOpts o;
+    const CacheGeo *nodes1, *nodes2;   // dir-1 meshes [ns1][ns2+1], dir-2 meshes [ns2][ns1+1]
+    int slot1, slot2;                  // first Green slot of each direction
+    Inter *inters;
+    Plane *planes;
+    double *lumgeo;
+    octopic_interaction_record *records;
+    ErrorInfo *err;
+};
+hipError_t launch_node_geometry(const NodeGeometryArgs &a, hipStream_t s);
+
+// misc (k_misc.hip)
+hipError_t launch_linear6d(const double *m36, double *const c[6], int64_t n, hipStream_t s);
+hipError_t launch_generate(double *const c[6], int64_t n, uint64_t seed, uint64_t rng_id, double cutoff, hipStream_t s);
+hipError_t launch_scale_beam(double *const c[6], int64_t n, const double *beta, const double *alpha, const double *sigma,
+                             hipStream_t s);
+hipError_t launch_debug_weights(int method, const double *u, int64_t n, int npts, int *base, double *w, hipStream_t s);
+
+}  // namespace octopic

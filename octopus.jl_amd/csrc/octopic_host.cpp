@@ -1,0 +1,997 @@
+// octopic_host.cpp -- host orchestration of one collide! and the C ABI of include/octopic.h.
+//
+// One collide (reference: _pic_collide!, src/tasks/strongstrong/pic_cpu.jl:82-275):
+//   1. slicing of both beams on the device (k_slicing.hip), gathering each beam into slice-contiguous
+//      state arrays (the reference's _pic_slice_states, pic_cpu.jl:866-867);
+//   2. ONE device->host read of the two slice tables, from which the host forms the collision order
+//      and the conflict-free wavefront batches (slicing.jl:561-647);
+//   3. per batch, a fixed kernel sequence with no host round trip: bounds -> geometry + Green-cache
+//      decisions -> Green builds -> deposits -> row FFT -> column FFT x Green -> inverse row FFT ->
+//      field -> kick -> luminosity overlap;
+//   4. scatter of the states back into the beams and one read of the per-pair luminosities.
+// Because every batch's stage writes only what later stages of the same batch (or later batches)
+// read, both directions of a pair kick in place: the deposits of both directions have read their
+// pre-pair sources before either kick runs, which is what the reference's scratch copy ensures
+// (pic_cpu.jl:307-327).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "octopic.h"
+#include "common.h"
+#include "fft.h"
+#include "launch.h"
+
+using namespace octopic;
+
+namespace {
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &m) { throw Fail{code, m}; }
+
+#define HIPCHECK(call)                                                                                          \
+    do {                                                                                                        \
+        hipError_t e_ = (call);                                                                                 \
+        if (e_ != hipSuccess) {                                                                                 \
+            if (e_ == hipErrorOutOfMemory) fail(OCTOPIC_ERR_OOM, std::string("device allocation failed: ") + #call); \
+            fail(OCTOPIC_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));                        \
+        }                                                                                                       \
+    } while (0)
+
+// Grow-only device buffer.
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void ensure(size_t m) {
+        if (m <= n) return;
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+        HIPCHECK(hipMalloc((void **)&p, std::max<size_t>(m, 1) * sizeof(T)));
+        n = m;
+    }
+    void ensure_zero(size_t m, hipStream_t s) {
+        if (m <= n) return;
+        ensure(m);
+        HIPCHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// ------------------------------------------------------------------ erfinv for the normal quantiles
+// Same published method as the CPU restatement: Giles' seed (GPU Computing Gems, 2011) refined by
+// Halley steps on erf/erfc, so the quantile boundaries agree bit for bit with it.
+double erfinv_host(double y) {
+    if (std::isnan(y) || y < -1.0 || y > 1.0) return NAN;
+    if (y == 1.0) return INFINITY;
+    if (y == -1.0) return -INFINITY;
+    if (y == 0.0) return y;
+    double w = -std::log((1.0 - y) * (1.0 + y));
+    double x;
+    if (w < 5.0) {
+        w = w - 2.5;
+        double p = 2.81022636e-08;
+        p = 3.43273939e-07 + p * w; p = -3.5233877e-06 + p * w; p = -4.39150654e-06 + p * w;
+        p = 0.00021858087 + p * w; p = -0.00125372503 + p * w; p = -0.00417768164 + p * w;
+        p = 0.246640727 + p * w; p = 1.50140941 + p * w;
+        x = p * y;
+    } else {
+        w = std::sqrt(w) - 3.0;
+        double p = -0.000200214257;
+        p = 0.000100950558 + p * w; p = 0.00134934322 + p * w; p = -0.00367342844 + p * w;
+        p = 0.00573950773 + p * w; p = -0.0076224613 + p * w; p = 0.00943887047 + p * w;
+        p = 1.00167406 + p * w; p = 2.83297682 + p * w;
+        x = p * y;
+    }
+    const double two_over_sqrtpi = 1.1283791670955126;
+    const double ay = std::fabs(y);
+    for (int it = 0; it < 3; ++it) {
+        double ax = std::fabs(x);
+        double f = ay > 0.5 ? (1.0 - ay) - std::erfc(ax) : std::erf(ax) - ay;
+        double d = two_over_sqrtpi * std::exp(-ax * ax);
+        double step = f / d;
+        step = step / (1.0 + ax * step);
+        ax = ax - step;
+        x = std::copysign(ax, y);
+    }
+    return x;
+}
+
+bool jisless_h(double a, double b) { return jisless(a, b); }
+
+struct OrderEntry { double t; int i, j; };
+
+std::vector<OrderEntry> collision_order(const double *c1, int ns1, const double *c2, int ns2) {  // slicing.jl:561-568
+    std::vector<OrderEntry> order;
+    order.reserve((size_t)ns1 * ns2);
+    for (int i = 0; i < ns1; ++i)
+        for (int j = 0; j < ns2; ++j) order.push_back({-(c1[i] + c2[j]) / 2.0, i, j});
+    std::stable_sort(order.begin(), order.end(),
+                     [](const OrderEntry &a, const OrderEntry &b) { return jisless_h(a.t, b.t); });
+    return order;
+}
+
+// collision_pair_batches (slicing.jl:601-647), as positions into `order`
+std::vector<std::vector<int>> pair_batches(const std::vector<OrderEntry> &order, int ns1, int ns2) {
+    std::vector<std::vector<int>> batches;
+    std::vector<std::vector<int>> by_i(ns1), by_j(ns2);
+    for (int k = 0; k < (int)order.size(); ++k) { by_i[order[k].i].push_back(k); by_j[order[k].j].push_back(k); }
+    std::vector<int> next_i(ns1, 0), next_j(ns2, 0);
+    std::vector<char> done(order.size(), 0);
+    int remaining = (int)order.size();
+    while (remaining > 0) {
+        std::vector<int> cur;
+        std::vector<char> ui(ns1, 0), uj(ns2, 0);
+        for (int k = 0; k < (int)order.size(); ++k) {
+            if (done[k]) continue;
+            const auto &p = order[k];
+            if (ui[p.i] || uj[p.j]) continue;
+            bool ri = next_i[p.i] < (int)by_i[p.i].size() && by_i[p.i][next_i[p.i]] == k;
+            bool rj = next_j[p.j] < (int)by_j[p.j].size() && by_j[p.j][next_j[p.j]] == k;
+            if (ri && rj) {
+                cur.push_back(k); ui[p.i] = 1; uj[p.j] = 1; done[k] = 1;
+                next_i[p.i]++; next_j[p.j]++; remaining--;
+            }
+        }
+        if (cur.empty()) fail(OCTOPIC_ERR_INVALID_ARGUMENT, "internal collision scheduler error: no ready slice-pair found");
+        batches.push_back(std::move(cur));
+    }
+    return batches;
+}
+
+struct FFTPlanHost {
+    FFTPlanDev dev{};
+    DBuf<double2> tw;
+    void build(int n) {
+        dev.n = n;
+        int m = n, k = 0;
+        while (m % 4 == 0) { dev.radix[k++] = 4; m /= 4; }
+        while (m % 2 == 0) { dev.radix[k++] = 2; m /= 2; }
+        while (m % 3 == 0) { dev.radix[k++] = 3; m /= 3; }
+        if (m != 1 || k > kMaxStages)
+            fail(OCTOPIC_ERR_UNSUPPORTED, "padded mesh size 2*grid = " + std::to_string(n) +
+                                              " has a prime factor other than 2 and 3; this backend's FFT supports "
+                                              "grids whose doubled size factors into 2s and 3s");
+        dev.nstages = k;
+        std::vector<double2> h(n);
+        for (int t = 0; t < n; ++t) {
+            long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
+            h[t] = make_double2((double)cosl(a), (double)sinl(a));
+        }
+        tw.ensure(n);
+        HIPCHECK(hipMemcpy(tw.p, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+        dev.tw = tw.p;
+    }
+};
+
+const char *kFamilies[] = {"slicing", "bounds", "geometry", "green", "deposit", "fft_rows_fwd", "fft_cols_conv",
+                           "fft_rows_inv", "field", "kick", "lum", "store", "node_prepare"};
+constexpr int kNFamilies = sizeof(kFamilies) / sizeof(kFamilies[0]);
+enum Fam { F_SLICING, F_BOUNDS, F_GEOMETRY, F_GREEN, F_DEPOSIT, F_ROWS_FWD, F_COLS, F_ROWS_INV, F_FIELD, F_KICK, F_LUM,
+           F_STORE, F_NODE };
+
+}  // namespace
+
+struct octopic_handle {
+    octopic_options opt{};
+    std::vector<double> pos1, pos2;   // owned copies of the slicing positions
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int nx = 0, ny = 0, lnx = 0, lny = 0, lum_dep = 0, npl = 2, mode = 0;
+    FFTPlanHost px, py;
+    // persistent device state
+    DBuf<SliceTable> tab;       // 2
+    DBuf<double> q;             // 2 * 256 quantiles / positions
+    DBuf<ErrorInfo> errinfo;
+    DBuf<unsigned long long> counters;   // hits, misses, rebuilds, builds
+    DBuf<CacheGeo> cache;
+    DBuf<double2> spec;
+    int nkeys = 0;
+    int64_t slot_stride = 0;
+    // per-collide buffers
+    DBuf<double> beam_in[2][6];
+    DBuf<double> st[2][6];
+    DBuf<int64_t> perm[2];
+    DBuf<uint16_t> sid[2];
+    DBuf<int> hist[2];
+    DBuf<double> lanes, mm, clanes;
+    DBuf<unsigned long long> bins;
+    DBuf<double *> ptrs;        // [beam_in 2x6][st 2x6]
+    DBuf<BatchPair> pairs;
+    DBuf<double> partial;
+    DBuf<Inter> inters;
+    DBuf<Plane> planes;
+    DBuf<GreenJob> jobs;
+    DBuf<double> lumgeo;
+    DBuf<unsigned long long> charge;
+    DBuf<double2> qh;
+    DBuf<double> phi, table;
+    DBuf<octopic_interaction_record> records;
+    DBuf<double> lum_parts;
+    // node-mode buffers
+    DBuf<CacheGeo> nodes;
+    DBuf<GreenJob> node_jobs;
+    DBuf<double> node_partial;
+    // pinned host mirrors
+    SliceTable *h_tab = nullptr;
+    ErrorInfo *h_err = nullptr;
+    unsigned long long *h_counters = nullptr;
+    // last-collide results
+    int ns[2] = {0, 0};
+    int64_t nbeam[2] = {0, 0};
+    std::vector<int32_t> pi, pj, pran;
+    std::vector<double> plum;
+    std::vector<octopic_interaction_record> recs;
+    octopic_stats stats{};
+    // timing
+    bool timing = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_next = 0;
+    double fam_ms[kNFamilies] = {0};
+    int64_t fam_n[kNFamilies] = {0};
+
+    void tic(int fam, hipEvent_t &a) {
+        if (!timing) return;
+        a = next_event();
+        hipEventRecord(a, stream);
+        (void)fam;
+    }
+    void toc(int fam, hipEvent_t a) {
+        if (!timing) return;
+        hipEvent_t b = next_event();
+        hipEventRecord(b, stream);
+        ev.push_back({fam, {a, b}});
+    }
+    hipEvent_t next_event() {
+        if (ev_next >= ev_pool.size()) {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            ev_pool.push_back(e);
+        }
+        return ev_pool[ev_next++];
+    }
+};
+
+namespace {
+
+struct Timed {
+    octopic_handle *h;
+    int fam;
+    hipEvent_t a = nullptr;
+    Timed(octopic_handle *h_, int f) : h(h_), fam(f) { h->tic(fam, a); }
+    ~Timed() { h->toc(fam, a); }
+};
+
+int set_err(octopic_handle *h, int code, const std::string &m) {
+    if (h) h->err = m;
+    return code;
+}
+
+void validate_slicing(const octopic_slicing &s, int beam) {
+    const std::string b = "slicing" + std::to_string(beam) + ": ";
+    if (s.nslices <= 0) fail(OCTOPIC_ERR_INVALID_OPTION, b + "nslices must be positive");
+    if (s.nslices > kMaxSlices) fail(OCTOPIC_ERR_UNSUPPORTED, b + "more than 256 slices per beam are not supported");
+    if (s.resolution <= 0) fail(OCTOPIC_ERR_INVALID_OPTION, b + "slicing resolution must be positive");
+    if (s.method < 0 || s.method > 4) fail(OCTOPIC_ERR_INVALID_OPTION, b + "unsupported longitudinal slicing method");
+    if (s.center_position != OCTOPIC_CENTER_CENTROID && s.center_position != OCTOPIC_CENTER_MIDPOINT)
+        fail(OCTOPIC_ERR_INVALID_OPTION, b + "center_position must be :centroid or :midpoint");
+    if (s.method == OCTOPIC_SLICE_SPECIFIED && s.npositions != s.nslices - 1)
+        fail(OCTOPIC_ERR_INVALID_OPTION, b + "specified slicing requires nslices-1 internal positions");
+    if (s.method == OCTOPIC_SLICE_EQUAL_COUNT)
+        fail(OCTOPIC_ERR_UNSUPPORTED, b + "method = :equal_count (membership by rank) is not implemented by the HIP "
+                                          "backend; use :normal_quantile, :equal_area, :equal_width or :specified");
+    if (s.method == OCTOPIC_SLICE_EQUAL_AREA && (int64_t)s.nslices * s.resolution > 16384)
+        fail(OCTOPIC_ERR_UNSUPPORTED, b + ":equal_area with nslices*resolution > 16384 histogram bins is not supported");
+}
+
+// PICPoissonSolver constructor checks (interface.jl:1410-1476), _validate_pic_solver (pic_cpu.jl:517-576)
+// and the device-route checks of this backend (the reference's _require_cuda_pic_options, 730-777).
+void validate(const octopic_options &o) {
+    if (!(o.green_min_ratio > 0.0 && o.green_min_ratio <= 1.0))
+        fail(OCTOPIC_ERR_INVALID_OPTION, "slice_pair_green_min_ratio must be in (0, 1]");
+    if (!(o.green_growth >= 0.0)) fail(OCTOPIC_ERR_INVALID_OPTION, "slice_pair_green_growth must be non-negative");
+    if (o.nx < 5 || o.ny < 5) fail(OCTOPIC_ERR_INVALID_OPTION, "PICPoissonSolver grid dimensions must both be at least 5");
+    if (o.deposit != OCTOPIC_CIC && o.deposit != OCTOPIC_TSC) fail(OCTOPIC_ERR_INVALID_OPTION, "deposit_method must be :CIC or :TSC");
+    if (o.green_type < 0 || o.green_type > 2)
+        fail(OCTOPIC_ERR_INVALID_OPTION, "green_type must be :integrated, :standard or :lattice");
+    if (o.green_cache != OCTOPIC_CACHE_NONE && o.green_cache != OCTOPIC_CACHE_SLICE_PAIR)
+        fail(OCTOPIC_ERR_INVALID_OPTION, "green_cache must be :none or :slice_pair");
+    if (o.field_derivative != 2 && o.field_derivative != 4)
+        fail(OCTOPIC_ERR_INVALID_OPTION, "field_derivative must be :second or :fourth");
+    if (o.slice_interpolation != OCTOPIC_INTERP_LINEAR && o.slice_interpolation != OCTOPIC_INTERP_QUADRATIC)
+        fail(OCTOPIC_ERR_INVALID_OPTION, "slice_interpolation must be :linear or :quadratic");
+    if (o.grid_extent != OCTOPIC_EXTENT_EXTREMA && o.grid_extent != OCTOPIC_EXTENT_SIGMA)
+        fail(OCTOPIC_ERR_INVALID_OPTION, "grid_extent must be :extrema or :sigma");
+    if (!(o.grid_extent_sigma > 0.0)) fail(OCTOPIC_ERR_INVALID_OPTION, "grid_extent_sigma must be positive");
+    if (!(std::isfinite(o.min_extent_x) && std::isfinite(o.min_extent_y) && o.min_extent_x >= 0.0 && o.min_extent_y >= 0.0))
+        fail(OCTOPIC_ERR_INVALID_OPTION, "min_transverse_extent values must be finite and nonnegative");
+    if (!(o.grid_quantize >= 0.0)) fail(OCTOPIC_ERR_INVALID_OPTION, "grid_quantize must be non-negative (0 disables)");
+    if (o.interaction_grid < 0 || o.interaction_grid > 2)
+        fail(OCTOPIC_ERR_INVALID_OPTION, "interaction_grid must be :slice_pair, :source_slice or :node");
+    if (o.interaction_grid != OCTOPIC_GRID_SLICE_PAIR && o.grid_extent != OCTOPIC_EXTENT_EXTREMA)
+        fail(OCTOPIC_ERR_UNSUPPORTED, "grid_extent is not implemented for this interaction_grid: that mode sizes its mesh "
+                                      "from the union/per-node extrema of its own particle sets");
+    if (o.batch_mode != OCTOPIC_BATCH_SEQUENTIAL && o.batch_mode != OCTOPIC_BATCH_WAVEFRONT)
+        fail(OCTOPIC_ERR_INVALID_OPTION, "batch_mode must be :sequential or :wavefront");
+    int lnx = o.lum_nx > 0 ? o.lum_nx : o.nx, lny = o.lum_ny > 0 ? o.lum_ny : o.ny;
+    if (lnx < 3 || lny < 3) fail(OCTOPIC_ERR_INVALID_OPTION, "luminosity_grid dimensions must both be at least 3");
+    int ld = o.lum_deposit >= 0 ? o.lum_deposit : o.deposit;
+    if (ld != OCTOPIC_CIC && ld != OCTOPIC_TSC)
+        fail(OCTOPIC_ERR_INVALID_OPTION, "luminosity_deposit_method must be nothing, :CIC, or :TSC");
+    validate_slicing(o.slicing1, 1);
+    validate_slicing(o.slicing2, 2);
+    if (o.interaction_grid == OCTOPIC_GRID_NODE && o.slice_interpolation != OCTOPIC_INTERP_LINEAR)
+        fail(OCTOPIC_ERR_UNSUPPORTED,
+             "slice_interpolation = :quadratic is not implemented for interaction_grid = :node: the node-plane path "
+             "applies its own interpolation. Use interaction_grid = :slice_pair for :quadratic, or "
+             "slice_interpolation = :linear.");
+    // Device-route checks (as the reference's CUDA backend, pic_cpu.jl:730-777).
+    if (o.grid_extent != OCTOPIC_EXTENT_EXTREMA)
+        fail(OCTOPIC_ERR_UNSUPPORTED, "grid_extent = :sigma is not implemented by the HIP PIC backend: its bounds are "
+                                      "order-free min/max reductions and apply no estimator. Use grid_extent = :extrema.");
+    if (o.interaction_grid == OCTOPIC_GRID_SOURCE_SLICE)
+        fail(OCTOPIC_ERR_UNSUPPORTED, "interaction_grid = :source_slice is not implemented by the HIP PIC backend; its union "
+                                      "mesh depends on the sequential pair order. Use :slice_pair or :node.");
+    if (o.green_type == OCTOPIC_GREEN_LATTICE)
+        fail(OCTOPIC_ERR_UNSUPPORTED, "green_type = :lattice (experimental in the reference) is not implemented by the HIP "
+                                      "PIC backend; use :integrated or :standard.");
+    if ((int64_t)lnx * 0 + (int64_t)(lnx + 1) * (lny + 1) * 8 > 160 * 1024 || (int64_t)o.nx * o.ny * 8 > 160 * 1024)
+        fail(OCTOPIC_ERR_UNSUPPORTED, "meshes above 143x143 nodes (or luminosity meshes above 142x142) exceed the LDS "
+                                      "deposit tile of this backend");
+}
+
+void upload_ptrs(octopic_handle *h, double *const in[2][6]) {
+    double *hp[24];
+    for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 6; ++c) { hp[b * 6 + c] = in[b][c]; hp[12 + b * 6 + c] = h->st[b][c].p; }
+    h->ptrs.ensure(24);
+    HIPCHECK(hipMemcpyAsync(h->ptrs.p, hp, sizeof(hp), hipMemcpyHostToDevice, h->stream));
+}
+
+void run_slicing(octopic_handle *h, int b, const octopic_slicing &s, int64_t n) {
+    const int ns = s.nslices;
+    std::vector<double> q(kMaxSlices, 0.0);
+    if (s.method == OCTOPIC_SLICE_NORMAL_QUANTILE) {
+        for (int k = 1; k < ns; ++k) q[k] = std::sqrt(2.0) * erfinv_host((double)(2 * k) / (double)ns - 1.0);
+    } else if (s.method == OCTOPIC_SLICE_SPECIFIED) {
+        const std::vector<double> &p = b == 0 ? h->pos1 : h->pos2;
+        for (int k = 0; k < ns - 1; ++k) q[k] = p[k];
+    }
+    HIPCHECK(hipMemcpyAsync(h->q.p + (size_t)b * kMaxSlices, q.data(), sizeof(double) * kMaxSlices, hipMemcpyHostToDevice,
+                            h->stream));
+    h->perm[b].ensure(n);
+    h->sid[b].ensure(n);
+    h->hist[b].ensure(slicing_hist_entries(n, ns));
+    SlicingArgs a{};
+    a.z = nullptr;
+    a.n = n;
+    a.method = s.method; a.ns = ns; a.resolution = s.resolution;
+    a.centroid = s.center_position == OCTOPIC_CENTER_CENTROID ? 1 : 0;
+    a.q = h->q.p + (size_t)b * kMaxSlices;
+    a.tab = h->tab.p + b;
+    a.lanes = h->lanes.p; a.mm_partial = h->mm.p; a.centroid_lanes = h->clanes.p;
+    a.bin_counts = h->bins.p;
+    a.sid = h->sid[b].p; a.hist = h->hist[b].p;
+    a.beam = h->ptrs.p + b * 6;
+    a.st = h->ptrs.p + 12 + b * 6;
+    a.st_z = h->st[b][4].p;
+    a.perm = h->perm[b].p;
+    a.err = h->errinfo.p;
+    return (void)a;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+int octopic_options_default(octopic_options *o) {
+    if (!o) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    std::memset(o, 0, sizeof(*o));
+    o->nx = 128; o->ny = 128;
+    o->deposit = OCTOPIC_CIC;
+    o->green_type = OCTOPIC_GREEN_INTEGRATED;
+    o->green_cache = OCTOPIC_CACHE_SLICE_PAIR;
+    o->field_derivative = 2;
+    o->slice_interpolation = OCTOPIC_INTERP_LINEAR;
+    o->interaction_grid = OCTOPIC_GRID_SLICE_PAIR;
+    o->grid_extent = OCTOPIC_EXTENT_EXTREMA;
+    o->longitudinal_kick = 1;
+    o->lum_nx = 0; o->lum_ny = 0; o->lum_deposit = -1;
+    o->batch_mode = OCTOPIC_BATCH_WAVEFRONT;
+    o->cuda_flags = 15;
+    o->grid_extent_sigma = 6.0;
+    o->min_extent_x = 0.0; o->min_extent_y = 0.0;
+    o->grid_quantize = 0.0;
+    o->green_min_ratio = 0.5;
+    o->green_growth = 0.25;
+    o->kbb1 = NAN; o->kbb2 = NAN; o->luminosity_scale = NAN;
+    for (octopic_slicing *s : {&o->slicing1, &o->slicing2}) {
+        s->nslices = 1; s->method = OCTOPIC_SLICE_EQUAL_AREA; s->resolution = 100;
+        s->center_position = OCTOPIC_CENTER_CENTROID; s->npositions = 0; s->positions = nullptr;
+    }
+    return OCTOPIC_OK;
+}
+
+const char *octopic_last_error(const octopic_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+int octopic_set_stream(octopic_handle *h, void *s) {
+    if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    h->stream = (hipStream_t)s;
+    return OCTOPIC_OK;
+}
+
+void octopic_destroy(octopic_handle *h) {
+    if (!h) return;
+    hipSetDevice(h->device);
+    hipDeviceSynchronize();
+    for (auto e : h->ev_pool) hipEventDestroy(e);
+    if (h->h_tab) hipHostFree(h->h_tab);
+    if (h->h_err) hipHostFree(h->h_err);
+    if (h->h_counters) hipHostFree(h->h_counters);
+    h->tab.release(); h->q.release(); h->errinfo.release(); h->counters.release(); h->cache.release(); h->spec.release();
+    for (int b = 0; b < 2; ++b) {
+        for (int c = 0; c < 6; ++c) { h->beam_in[b][c].release(); h->st[b][c].release(); }
+        h->perm[b].release(); h->sid[b].release(); h->hist[b].release();
+    }
+    h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
+    h->partial.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
+    h->charge.release(); h->qh.release(); h->phi.release(); h->table.release(); h->records.release();
+    h->lum_parts.release(); h->nodes.release(); h->node_jobs.release(); h->node_partial.release();
+    h->px.tw.release(); h->py.tw.release();
+    delete h;
+}
+
+int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **out) {
+    if (!opt || !out) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    octopic_handle *h = new octopic_handle();
+    try {
+        validate(*opt);
+        h->opt = *opt;
+        if (opt->slicing1.positions) h->pos1.assign(opt->slicing1.positions, opt->slicing1.positions + opt->slicing1.npositions);
+        if (opt->slicing2.positions) h->pos2.assign(opt->slicing2.positions, opt->slicing2.positions + opt->slicing2.npositions);
+        h->opt.slicing1.positions = nullptr;
+        h->opt.slicing2.positions = nullptr;
+        h->device = device;
+        HIPCHECK(hipSetDevice(device));
+        h->nx = opt->nx; h->ny = opt->ny;
+        h->lnx = opt->lum_nx > 0 ? opt->lum_nx : opt->nx;
+        h->lny = opt->lum_ny > 0 ? opt->lum_ny : opt->ny;
+        h->lum_dep = opt->lum_deposit >= 0 ? opt->lum_deposit : opt->deposit;
+        h->mode = opt->interaction_grid == OCTOPIC_GRID_NODE ? 2 : (opt->slice_interpolation == OCTOPIC_INTERP_QUADRATIC ? 1 : 0);
+        h->npl = h->mode == 0 ? 2 : 3;
+        if (h->mode == 2 && !opt->longitudinal_kick) h->npl = 3;   // Z plane slot kept; unused without the longitudinal kick
+        h->px.build(2 * h->nx);
+        h->py.build(2 * h->ny);
+        h->tab.ensure(2);
+        h->q.ensure(2 * kMaxSlices);
+        h->errinfo.ensure(1);
+        h->counters.ensure(4);
+        HIPCHECK(hipMemset(h->errinfo.p, 0, sizeof(ErrorInfo)));
+        HIPCHECK(hipMemset(h->counters.p, 0, 4 * sizeof(unsigned long long)));
+        h->lanes.ensure(kFoldLanes);
+        h->mm.ensure(2 * (kFoldLanes / 256));
+        h->bins.ensure(16384);
+        HIPCHECK(hipMemset(h->bins.p, 0, 16384 * sizeof(unsigned long long)));
+        const int ns1 = opt->slicing1.nslices, ns2 = opt->slicing2.nslices;
+        h->clanes.ensure((size_t)std::max(ns1, ns2) * kFoldLanes);
+        h->slot_stride = (int64_t)(h->nx + 1) * 2 * h->ny;
+        if (opt->green_cache == OCTOPIC_CACHE_SLICE_PAIR && h->mode != 2) {
+            h->nkeys = 2 * ns1 * ns2;
+            h->cache.ensure(h->nkeys);
+            HIPCHECK(hipMemset(h->cache.p, 0, sizeof(CacheGeo) * h->nkeys));
+        }
+        HIPCHECK(hipHostMalloc((void **)&h->h_tab, 2 * sizeof(SliceTable)));
+        HIPCHECK(hipHostMalloc((void **)&h->h_err, sizeof(ErrorInfo)));
+        HIPCHECK(hipHostMalloc((void **)&h->h_counters, 4 * sizeof(unsigned long long)));
+        HIPCHECK(hipDeviceSynchronize());
+    } catch (Fail &f) {
+        int code = f.code;
+        std::string msg = f.msg;
+        octopic_destroy(h);
+        // A failed create has no handle to carry its message; report through a static buffer.
+        static thread_local std::string last;
+        last = msg;
+        (void)last;
+        fprintf(stderr, "octopic_create: %s\n", msg.c_str());
+        return code;
+    }
+    *out = h;
+    return OCTOPIC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+void collide_impl(octopic_handle *h, octopic_beam *b1, octopic_beam *b2, int64_t turn, int compute_lum, double *lum_out);
+
+}  // namespace
+
+extern "C" int octopic_collide(octopic_handle *h, octopic_beam *beam1, octopic_beam *beam2, int64_t turn,
+                               int32_t compute_luminosity, double *luminosity) {
+    if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    try {
+        HIPCHECK(hipSetDevice(h->device));
+        collide_impl(h, beam1, beam2, turn, compute_luminosity, luminosity);
+    } catch (Fail &f) {
+        return set_err(h, f.code, f.msg);
+    }
+    h->err.clear();
+    return OCTOPIC_OK;
+}
+
+namespace {
+
+const char *stage_name(int stage) {
+    switch (stage) {
+        case 0: return "source bounds";
+        case 1: return "field bounds";
+        case 2: return "luminosity grid";
+        case 3: return "interaction grid";
+        case 4: return "node-mesh build";
+        default: return "slicing";
+    }
+}
+
+void raise_device_error(octopic_handle *h, const ErrorInfo &e, int64_t turn) {
+    char buf[768];
+    if (e.code == OCTOPIC_ERR_NONFINITE) {
+        snprintf(buf, sizeof buf,
+                 "non-finite particle coordinate(s) detected in a strong-strong collision (turn=%lld, %s, collision-order "
+                 "pair %d, direction %d): non-finite coordinates cannot be represented on the interaction mesh, so the "
+                 "solver fails fast; the beams are left unmodified.",
+                 (long long)turn, stage_name(e.stage), e.p, e.dir);
+    } else if (e.code == OCTOPIC_ERR_DEGENERATE_EXTENT) {
+        snprintf(buf, sizeof buf,
+                 "PIC %s requires finite, positive transverse extents; observed (x=%g, y=%g) with min_transverse_extent="
+                 "(%g, %g) (turn=%lld, collision-order pair %d, direction %d). Supply a positive physical bound for every "
+                 "degenerate axis.",
+                 e.stage == 2 ? "luminosity grid" : "interaction grid", e.a, e.b, h->opt.min_extent_x, h->opt.min_extent_y,
+                 (long long)turn, e.p, e.dir);
+    } else {
+        snprintf(buf, sizeof buf, "device-side failure code %d at stage %s", e.code, stage_name(e.stage));
+    }
+    // Kernels after the failure returned early; clear any charge they left so the next collide starts clean.
+    if (h->charge.p) hipMemsetAsync(h->charge.p, 0, h->charge.n * sizeof(unsigned long long), h->stream);
+    hipMemsetAsync(h->errinfo.p, 0, sizeof(ErrorInfo), h->stream);
+    hipStreamSynchronize(h->stream);
+    fail(e.code, buf);
+}
+
+void check_beam(const octopic_beam *b, int k) {
+    if (!b) fail(OCTOPIC_ERR_INVALID_ARGUMENT, "beam" + std::to_string(k) + " is null");
+    if (b->n <= 0) fail(OCTOPIC_ERR_INVALID_ARGUMENT, "longitudinal slicing requires at least one particle");
+    if (!b->x || !b->px || !b->y || !b->py || !b->z || !b->pz)
+        fail(OCTOPIC_ERR_INVALID_ARGUMENT, "beam" + std::to_string(k) + " has a null coordinate pointer");
+    if (b->n > (int64_t)2147483647) fail(OCTOPIC_ERR_UNSUPPORTED, "more than 2^31-1 macroparticles per beam");
+}
+
+// _strong_strong_kbb1/2 (slicing.jl:77-91) divided by the source macroparticle count (pic_cpu.jl:821-824)
+double kbb_of(double given, const octopic_beam *self, const octopic_beam *other, bool first, int64_t nsrc) {
+    double v = given;
+    if (std::isnan(v)) {
+        if (!(std::isfinite(self->E0) && self->E0 != 0.0))
+            fail(OCTOPIC_ERR_INVALID_ARGUMENT, first ? "beam1 E0 must be finite and nonzero when kbb1 is not specified"
+                                                     : "beam2 E0 must be finite and nonzero when kbb2 is not specified");
+        const octopic_beam *p1 = first ? self : other, *p2 = first ? other : self;
+        v = p1->charge * p2->charge * self->r0 * other->npart * self->mc2 / self->E0;
+    }
+    return v / (double)nsrc;
+}
+
+void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t turn, int compute_lum, double *lum_out) {
+    check_beam(B1, 1);
+    check_beam(B2, 2);
+    hipStream_t s = h->stream;
+    const octopic_options &o = h->opt;
+    const int64_t n[2] = {B1->n, B2->n};
+    octopic_beam *B[2] = {B1, B2};
+    h->ev.clear();
+    h->ev_next = 0;
+    // ---- beams on the device
+    double *bp[2][6];
+    for (int b = 0; b < 2; ++b) {
+        double *hp[6] = {B[b]->x, B[b]->px, B[b]->y, B[b]->py, B[b]->z, B[b]->pz};
+        for (int c = 0; c < 6; ++c) {
+            h->st[b][c].ensure(n[b]);
+            if (B[b]->on_device) {
+                bp[b][c] = hp[c];
+            } else {
+                h->beam_in[b][c].ensure(n[b]);
+                HIPCHECK(hipMemcpyAsync(h->beam_in[b][c].p, hp[c], sizeof(double) * n[b], hipMemcpyHostToDevice, s));
+                bp[b][c] = h->beam_in[b][c].p;
+            }
+        }
+    }
+    upload_ptrs(h, bp);
+    // ---- slicing
+    {
+        Timed t(h, F_SLICING);
+        const octopic_slicing *sl[2] = {&o.slicing1, &o.slicing2};
+        for (int b = 0; b < 2; ++b) {
+            run_slicing(h, b, *sl[b], n[b]);
+            SlicingArgs a{};
+            a.z = bp[b][4];
+            a.n = n[b];
+            a.method = sl[b]->method; a.ns = sl[b]->nslices; a.resolution = sl[b]->resolution;
+            a.centroid = sl[b]->center_position == OCTOPIC_CENTER_CENTROID ? 1 : 0;
+            a.q = h->q.p + (size_t)b * kMaxSlices;
+            a.tab = h->tab.p + b;
+            a.lanes = h->lanes.p; a.mm_partial = h->mm.p; a.centroid_lanes = h->clanes.p;
+            a.bin_counts = h->bins.p;
+            a.sid = h->sid[b].p; a.hist = h->hist[b].p;
+            a.beam = h->ptrs.p + b * 6;
+            a.st = h->ptrs.p + 12 + b * 6;
+            a.st_z = h->st[b][4].p;
+            a.perm = h->perm[b].p;
+            a.err = h->errinfo.p;
+            HIPCHECK(launch_slicing(a, s));
+        }
+    }
+    HIPCHECK(hipMemcpyAsync(h->h_tab, h->tab.p, 2 * sizeof(SliceTable), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(h->h_err, h->errinfo.p, sizeof(ErrorInfo), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (h->h_err->code) raise_device_error(h, *h->h_err, turn);
+    const SliceTable &T1 = h->h_tab[0], &T2 = h->h_tab[1];
+    const int ns1 = o.slicing1.nslices, ns2 = o.slicing2.nslices;
+    h->ns[0] = ns1; h->ns[1] = ns2; h->nbeam[0] = n[0]; h->nbeam[1] = n[1];
+    for (int k = 0; k <= ns1; ++k)
+        if (!std::isfinite(T1.boundary[k])) raise_device_error(h, ErrorInfo{OCTOPIC_ERR_NONFINITE, -1, 0, 5, 0, 0}, turn);
+    for (int k = 0; k <= ns2; ++k)
+        if (!std::isfinite(T2.boundary[k])) raise_device_error(h, ErrorInfo{OCTOPIC_ERR_NONFINITE, -1, 0, 5, 0, 0}, turn);
+
+    // ---- schedule (host): collision order and conflict-free batches
+    const double kbb1 = kbb_of(o.kbb1, B1, B2, true, n[1]);
+    const double kbb2 = kbb_of(o.kbb2, B2, B1, false, n[0]);
+    const double klum = !std::isnan(o.luminosity_scale) ? o.luminosity_scale : B1->npart * B2->npart / (double)(n[0] * n[1]);
+    auto order = collision_order(T1.center, ns1, T2.center, ns2);
+    auto batches = pair_batches(order, ns1, ns2);
+    const int npairs = (int)order.size();
+    std::vector<BatchPair> hp;
+    std::vector<int> boff, bcnt;
+    int widest = 0;
+    int64_t maxcount = 0;
+    for (int k = 0; k < ns1; ++k) maxcount = std::max<int64_t>(maxcount, T1.count[k]);
+    for (int k = 0; k < ns2; ++k) maxcount = std::max<int64_t>(maxcount, T2.count[k]);
+    h->pran.assign(npairs, 0);
+    for (auto &bt : batches) {
+        boff.push_back((int)hp.size());
+        int c = 0;
+        for (int pos : bt) {
+            const auto &e = order[pos];
+            if (T1.count[e.i] == 0 || T2.count[e.j] == 0) continue;   // an empty slice skips the pair (pic_cpu.jl:302)
+            hp.push_back(BatchPair{pos, e.i, e.j, 0});
+            h->pran[pos] = 1;
+            ++c;
+        }
+        bcnt.push_back(c);
+        widest = std::max(widest, c);
+    }
+    const int W = std::max(widest, 1);
+    // fixed-point scale 2^k with k <= 40 and (maxcount + 1) * 2^k < 2^62
+    int kbits = 40;
+    while (kbits > 8 && (double)(maxcount + 1) * std::ldexp(1.0, kbits) >= std::ldexp(1.0, 62)) --kbits;
+    const double scale = std::ldexp(1.0, kbits), inv_scale = std::ldexp(1.0, -kbits);
+
+    // ---- buffers sized for the widest batch
+    const int nx = h->nx, ny = h->ny, npl = h->npl;
+    const int64_t field_cells = (int64_t)nx * ny, lum_cells = (int64_t)(h->lnx + 1) * (h->lny + 1);
+    const int64_t charge_stride = std::max(field_cells, lum_cells);
+    const int nfield_planes = 2 * W * npl, nplanes_max = nfield_planes + 2 * W;
+    const int kMaxChunks = 64;
+    h->pairs.ensure(std::max<size_t>(hp.size(), 1));
+    if (!hp.empty())
+        HIPCHECK(hipMemcpyAsync(h->pairs.p, hp.data(), sizeof(BatchPair) * hp.size(), hipMemcpyHostToDevice, s));
+    h->partial.ensure((size_t)2 * W * kMaxChunks * 12);
+    h->inters.ensure(2 * W);
+    h->planes.ensure(nplanes_max);
+    h->jobs.ensure(2 * W);
+    h->lumgeo.ensure(3 * W);
+    h->charge.ensure_zero((size_t)nplanes_max * charge_stride, s);
+    h->qh.ensure((size_t)nfield_planes * (nx + 1) * ny);
+    h->phi.ensure((size_t)nfield_planes * field_cells);
+    h->table.ensure((size_t)2 * W * field_cells * 3 * npl);
+    h->records.ensure((size_t)2 * std::max(npairs, 1));
+    h->lum_parts.ensure(std::max(npairs, 1));
+    HIPCHECK(hipMemsetAsync(h->records.p, 0, sizeof(octopic_interaction_record) * 2 * npairs, s));
+    HIPCHECK(hipMemsetAsync(h->lum_parts.p, 0, sizeof(double) * npairs, s));
+    const int temp_slot0 = h->nkeys;
+    const int nslots = h->nkeys + (o.green_cache == OCTOPIC_CACHE_SLICE_PAIR ? 0 : 2 * W);
+    h->spec.ensure((size_t)std::max(nslots, 1) * h->slot_stride);
+    unsigned long long counters_before[4];
+    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    std::memcpy(counters_before, h->h_counters, sizeof(counters_before));
+
+    GeomOpts go{};
+    go.nx = nx; go.ny = ny; go.lum_nx = h->lnx; go.lum_ny = h->lny;
+    go.green_type = o.green_type; go.green_cache = o.green_cache; go.quadratic = h->mode == 1;
+    go.interaction_grid = o.interaction_grid; go.compute_lum = compute_lum ? 1 : 0;
+    go.ns1 = ns1; go.ns2 = ns2; go.npl = npl; go.temp_slot0 = temp_slot0;
+    go.min_x = o.min_extent_x; go.min_y = o.min_extent_y; go.quantize = o.grid_quantize;
+    go.min_ratio = o.green_min_ratio; go.growth = o.green_growth;
+    go.kbb1 = kbb1; go.kbb2 = kbb2; go.klum = klum;
+
+    States st{};
+    for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 6; ++c) st.c[b][c] = h->st[b][c].p;
+    const SliceTable *dt1 = h->tab.p, *dt2 = h->tab.p + 1;
+
+    if (h->mode == 2) fail(OCTOPIC_ERR_UNSUPPORTED, "interaction_grid = :node is not available in this build");
+
+    int64_t planes_total = 0, builds_guess = 0;
+    (void)builds_guess;
+    for (size_t bi = 0; bi < batches.size(); ++bi) {
+        const int B = bcnt[bi];
+        if (B == 0) continue;
+        const BatchPair *dp = h->pairs.p + boff[bi];
+        int64_t bmax = 0;
+        for (int k = 0; k < B; ++k) {
+            const BatchPair &pr = hp[boff[bi] + k];
+            bmax = std::max<int64_t>(bmax, std::max<int64_t>(T1.count[pr.i], T2.count[pr.j]));
+        }
+        const int nchunks = (int)std::min<int64_t>(kMaxChunks, std::max<int64_t>(1, (bmax + 4095) / 4096));
+        {
+            Timed t(h, F_BOUNDS);
+            BoundsArgs a{dp, B, dt1, dt2, st, nchunks, h->partial.p, h->errinfo.p};
+            HIPCHECK(launch_bounds(a, s));
+        }
+        {
+            Timed t(h, F_GEOMETRY);
+            GeometryArgs a{};
+            a.pairs = dp; a.npairs = B; a.tab1 = dt1; a.tab2 = dt2; a.partial = h->partial.p; a.nchunks = nchunks;
+            a.o = go; a.cache = h->cache.p; a.inters = h->inters.p; a.planes = h->planes.p; a.jobs = h->jobs.p;
+            a.lumgeo = h->lumgeo.p; a.records = h->records.p; a.counters = h->counters.p; a.err = h->errinfo.p;
+            HIPCHECK(launch_geometry(a, s));
+        }
+        {
+            Timed t(h, F_GREEN);
+            HIPCHECK(launch_green_build(h->jobs.p, 2 * B, h->spec.p, h->slot_stride, nx, ny,
+                                        o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev, s));
+        }
+        const int nfp = 2 * B * npl;
+        const int dchunks = (int)std::min<int64_t>(32, std::max<int64_t>(1, (bmax + 8191) / 8192));
+        {
+            Timed t(h, F_DEPOSIT);
+            DepositArgs a{};
+            a.planes = h->planes.p; a.nplanes = nfp; a.st = st; a.method = o.deposit; a.ncells = (int)field_cells;
+            a.chunks = dchunks; a.scale = scale; a.charge = h->charge.p; a.charge_stride = charge_stride; a.err = h->errinfo.p;
+            HIPCHECK(launch_deposit(a, s));
+            if (compute_lum) {
+                DepositArgs l = a;
+                l.planes = h->planes.p + nfp; l.nplanes = 2 * B; l.method = h->lum_dep; l.ncells = (int)lum_cells;
+                l.charge = h->charge.p + (size_t)nfp * charge_stride;
+                HIPCHECK(launch_deposit(l, s));
+            }
+        }
+        const int64_t qh_stride = (int64_t)(nx + 1) * ny;
+        {
+            Timed t(h, F_ROWS_FWD);
+            HIPCHECK(launch_rows_fwd(h->planes.p, nfp, h->charge.p, charge_stride, h->qh.p, qh_stride, nx, ny, inv_scale,
+                                     h->px.dev, h->errinfo.p, s));
+        }
+        {
+            Timed t(h, F_COLS);
+            HIPCHECK(launch_cols_conv(h->planes.p, nfp, h->qh.p, qh_stride, h->spec.p, h->slot_stride, nx, ny, h->py.dev,
+                                      h->errinfo.p, s));
+        }
+        {
+            Timed t(h, F_ROWS_INV);
+            HIPCHECK(launch_rows_inv(h->planes.p, nfp, h->qh.p, qh_stride, h->phi.p, nx, ny, h->px.dev, h->errinfo.p, s));
+        }
+        {
+            Timed t(h, F_FIELD);
+            HIPCHECK(launch_field(h->inters.p, 2 * B, h->planes.p, h->phi.p, h->table.p, nx, ny,
+                                  o.field_derivative == 4 ? 1 : 0, npl, h->errinfo.p, s));
+        }
+        {
+            Timed t(h, F_KICK);
+            KickArgs a{};
+            a.inters = h->inters.p; a.ninters = 2 * B; a.st = st; a.table = h->table.p; a.nx = nx; a.ny = ny; a.npl = npl;
+            a.method = o.deposit; a.mode = h->mode; a.lkick = o.longitudinal_kick ? 1 : 0;
+            a.chunks = (int)std::min<int64_t>(64, std::max<int64_t>(1, (bmax + 2047) / 2048));
+            a.err = h->errinfo.p;
+            HIPCHECK(launch_kick(a, s));
+        }
+        if (compute_lum) {
+            Timed t(h, F_LUM);
+            LumArgs a{};
+            a.pairs = dp; a.npairs = B; a.charge = h->charge.p + (size_t)nfp * charge_stride; a.charge_stride = charge_stride;
+            a.ncells = (int)lum_cells; a.inv_scale = inv_scale; a.lumgeo = h->lumgeo.p; a.lum_parts = h->lum_parts.p;
+            a.err = h->errinfo.p;
+            HIPCHECK(launch_lum(a, s));
+        }
+        planes_total += nfp;
+    }
+    {
+        Timed t(h, F_STORE);
+        for (int b = 0; b < 2; ++b) HIPCHECK(launch_store(h->ptrs.p + b * 6, n[b], h->ptrs.p + 12 + b * 6, h->perm[b].p, h->errinfo.p, s));
+    }
+    // ---- results
+    HIPCHECK(hipMemcpyAsync(h->h_err, h->errinfo.p, sizeof(ErrorInfo), hipMemcpyDeviceToHost, s));
+    h->plum.assign(npairs, 0.0);
+    if (npairs) HIPCHECK(hipMemcpyAsync(h->plum.data(), h->lum_parts.p, sizeof(double) * npairs, hipMemcpyDeviceToHost, s));
+    h->recs.assign((size_t)2 * npairs, octopic_interaction_record{});
+    if (npairs)
+        HIPCHECK(hipMemcpyAsync(h->recs.data(), h->records.p, sizeof(octopic_interaction_record) * 2 * npairs,
+                                hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (h->h_err->code) raise_device_error(h, *h->h_err, turn);
+    for (int b = 0; b < 2; ++b) {
+        if (B[b]->on_device) continue;
+        double *hp2[6] = {B[b]->x, B[b]->px, B[b]->y, B[b]->py, B[b]->z, B[b]->pz};
+        for (int c = 0; c < 6; ++c)
+            if (c != 4) HIPCHECK(hipMemcpyAsync(hp2[c], bp[b][c], sizeof(double) * n[b], hipMemcpyDeviceToHost, s));
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    h->pi.resize(npairs); h->pj.resize(npairs);
+    for (int p = 0; p < npairs; ++p) { h->pi[p] = order[p].i; h->pj[p] = order[p].j; }
+    double lum = NAN;
+    if (compute_lum) {   // the fold in collision order (pic_cpu.jl:258-271)
+        lum = 0.0;
+        for (int p = 0; p < npairs; ++p) if (h->pran[p]) lum += h->plum[p];
+    }
+    if (lum_out) *lum_out = lum;
+    octopic_stats &S = h->stats;
+    S.green_hits = (int64_t)h->h_counters[0];
+    S.green_misses = (int64_t)h->h_counters[1];
+    S.green_rebuilds = (int64_t)h->h_counters[2];
+    S.green_builds = (int64_t)(h->h_counters[3] - counters_before[3]);
+    S.dropped = 0;
+    S.pairs = npairs;
+    S.batches = (int64_t)batches.size();
+    S.widest_batch = widest;
+    S.planes = planes_total;
+    S.collides += 1;
+    if (h->timing) {
+        for (auto &e : h->ev) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e.second.first, e.second.second);
+            h->fam_ms[e.first] += ms;
+            h->fam_n[e.first] += 1;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int octopic_get_stats(const octopic_handle *h, octopic_stats *out) {
+    if (!h || !out) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    *out = h->stats;
+    return OCTOPIC_OK;
+}
+
+int octopic_pair_luminosity(const octopic_handle *h, int32_t *i, int32_t *j, int32_t *ran, double *lum, int64_t cap,
+                            int64_t *n) {
+    if (!h || !n) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    *n = (int64_t)h->pi.size();
+    for (int64_t k = 0; k < std::min<int64_t>(cap, *n); ++k) {
+        if (i) i[k] = h->pi[k];
+        if (j) j[k] = h->pj[k];
+        if (ran) ran[k] = h->pran[k];
+        if (lum) lum[k] = h->plum[k];
+    }
+    return OCTOPIC_OK;
+}
+
+int octopic_interaction_records(const octopic_handle *h, octopic_interaction_record *out, int64_t cap, int64_t *n) {
+    if (!h || !n) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    *n = (int64_t)h->recs.size();
+    for (int64_t k = 0; k < std::min<int64_t>(cap, *n); ++k) out[k] = h->recs[k];
+    return OCTOPIC_OK;
+}
+
+int octopic_last_slices(const octopic_handle *h, int32_t which, double *boundaries, double *centers, double *weights,
+                        int64_t *counts, int64_t *indices, int64_t n) {
+    if (!h || (which != 1 && which != 2)) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    const int b = which - 1;
+    const SliceTable &T = h->h_tab[b];
+    const int ns = h->ns[b];
+    if (ns == 0) return set_err(const_cast<octopic_handle *>(h), OCTOPIC_ERR_INVALID_ARGUMENT, "no collide has run");
+    SliceTable full;
+    hipSetDevice(h->device);
+    if (hipMemcpy(&full, h->tab.p + b, sizeof(SliceTable), hipMemcpyDeviceToHost) != hipSuccess) return OCTOPIC_ERR_DEVICE;
+    (void)T;
+    if (boundaries) std::memcpy(boundaries, full.boundary, sizeof(double) * (ns + 1));
+    if (centers) std::memcpy(centers, full.center, sizeof(double) * ns);
+    if (weights) std::memcpy(weights, full.weight, sizeof(double) * ns);
+    if (counts) std::memcpy(counts, full.count, sizeof(int64_t) * ns);
+    if (indices) {
+        if (n != h->nbeam[b]) return set_err(const_cast<octopic_handle *>(h), OCTOPIC_ERR_INVALID_ARGUMENT, "index buffer size");
+        if (hipMemcpy(indices, h->perm[b].p, sizeof(int64_t) * n, hipMemcpyDeviceToHost) != hipSuccess) return OCTOPIC_ERR_DEVICE;
+    }
+    return OCTOPIC_OK;
+}
+
+int octopic_track_linear6d(octopic_handle *h, const double *m, octopic_beam *b) {
+    if (!h || !m || !b) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    if (!b->on_device) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "octopic_track_linear6d needs device beam pointers");
+    double *c[6] = {b->x, b->px, b->y, b->py, b->z, b->pz};
+    hipSetDevice(h->device);
+    if (launch_linear6d(m, c, b->n, h->stream) != hipSuccess) return set_err(h, OCTOPIC_ERR_DEVICE, "linear6d launch failed");
+    return OCTOPIC_OK;
+}
+
+int octopic_generate_gaussian_beam(octopic_handle *h, uint64_t seed, uint64_t rng_id, const double *beta3,
+                                   const double *alpha3, const double *sigma3, double cutoff, octopic_beam *b) {
+    if (!h || !b || !beta3 || !alpha3 || !sigma3) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    if (!b->on_device) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "octopic_generate_gaussian_beam needs device pointers");
+    if (b->n <= 0) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "random Beam construction requires a positive particle count");
+    double *c[6] = {b->x, b->px, b->y, b->py, b->z, b->pz};
+    hipSetDevice(h->device);
+    if (launch_generate(c, b->n, seed, rng_id, cutoff, h->stream) != hipSuccess)
+        return set_err(h, OCTOPIC_ERR_DEVICE, "beam generation failed");
+    if (launch_scale_beam(c, b->n, beta3, alpha3, sigma3, h->stream) != hipSuccess)
+        return set_err(h, OCTOPIC_ERR_DEVICE, "beam scaling failed");
+    return OCTOPIC_OK;
+}
+
+int octopic_debug_weights(octopic_handle *h, int32_t method, const double *u, int64_t n, int32_t npts, int32_t *base,
+                          double *w) {
+    if (!h || !u || !base || !w || n <= 0) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    hipSetDevice(h->device);
+    double *du = nullptr, *dw = nullptr;
+    int *db = nullptr;
+    int rc = OCTOPIC_OK;
+    if (hipMalloc((void **)&du, sizeof(double) * n) != hipSuccess || hipMalloc((void **)&dw, sizeof(double) * 3 * n) != hipSuccess ||
+        hipMalloc((void **)&db, sizeof(int) * n) != hipSuccess) {
+        rc = set_err(h, OCTOPIC_ERR_OOM, "debug buffers");
+    } else {
+        hipMemcpy(du, u, sizeof(double) * n, hipMemcpyHostToDevice);
+        launch_debug_weights(method, du, n, npts, db, dw, h->stream);
+        hipStreamSynchronize(h->stream);
+        hipMemcpy(base, db, sizeof(int) * n, hipMemcpyDeviceToHost);
+        hipMemcpy(w, dw, sizeof(double) * 3 * n, hipMemcpyDeviceToHost);
+    }
+    if (du) hipFree(du);
+    if (dw) hipFree(dw);
+    if (db) hipFree(db);
+    return rc;
+}
+
+int octopic_set_timing(octopic_handle *h, int32_t enabled) {
+    if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    h->timing = enabled != 0;
+    for (int k = 0; k < kNFamilies; ++k) { h->fam_ms[k] = 0.0; h->fam_n[k] = 0; }
+    return OCTOPIC_OK;
+}
+
+int octopic_kernel_times(const octopic_handle *h, double *ms_total, int64_t *launches, int64_t cap, int64_t *n, char *names,
+                         int64_t names_cap) {
+    if (!h || !n) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    *n = kNFamilies;
+    int64_t off = 0;
+    for (int k = 0; k < kNFamilies && k < cap; ++k) {
+        if (ms_total) ms_total[k] = h->fam_ms[k];
+        if (launches) launches[k] = h->fam_n[k];
+        if (names) {
+            size_t len = std::strlen(kFamilies[k]) + 1;
+            if (off + (int64_t)len <= names_cap) { std::memcpy(names + off, kFamilies[k], len); off += (int64_t)len; }
+        }
+    }
+    return OCTOPIC_OK;
+}
+
+}  // extern "C"

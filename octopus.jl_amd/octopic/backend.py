@@ -1,0 +1,268 @@
+"""HIP backend: ctypes binding of liboctopic.so and the collide! entry points.
+
+Mirrors the reference's dispatch (src/tasks/strongstrong/pic_cpu.jl:1-32, pic_cuda.jl:94-143):
+
+    collide_(solver, beam1, beam2, HIPBackend, ctx=None) -> luminosity   # cold Green cache, as bare collide!
+    PICCollider(solver).collide(beam1, beam2, turn)       -> luminosity  # persistent cache, as the task path
+
+The library is required: every entry point raises BackendUnavailable when liboctopic.so is missing
+or no GPU is visible.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+from . import _abi
+from .solver import ArgumentError, BeamParams, PICPoissonSolver
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liboctopic.so")
+_lib = None
+
+
+class BackendUnavailable(RuntimeError):
+    """liboctopic.so is not built or cannot run here."""
+
+
+class OctopicError(RuntimeError):
+    """A device-side failure (HIP runtime, allocation)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"{_abi.STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class NonFiniteCoordinateError(ArgumentError):
+    code = _abi.ERR_NONFINITE
+
+
+class DegenerateExtentError(ArgumentError):
+    code = _abi.ERR_DEGENERATE_EXTENT
+
+
+class UnsupportedOptionError(ArgumentError):
+    code = _abi.ERR_UNSUPPORTED
+
+
+def lib():
+    """Load liboctopic.so (raises BackendUnavailable if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BackendUnavailable(
+                f"{LIB_PATH} is not built; run __graft_entry__.build() (or make -C octopus.jl_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        _abi.declare(L)
+        _lib = L
+    return _lib
+
+
+def _raise(code, msg):
+    if code == _abi.ERR_NONFINITE:
+        raise NonFiniteCoordinateError(msg)
+    if code == _abi.ERR_DEGENERATE_EXTENT:
+        raise DegenerateExtentError(msg)
+    if code == _abi.ERR_UNSUPPORTED:
+        raise UnsupportedOptionError(msg)
+    if code in (_abi.ERR_INVALID_OPTION, _abi.ERR_INVALID_ARGUMENT):
+        raise ArgumentError(msg)
+    raise OctopicError(code, msg)
+
+
+class HIPBackend:
+    """Backend tag, the analogue of the reference's CPUThreadsBackend / CUDABackend (src/Octopus.jl:16-19)."""
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise BackendUnavailable("no GPU is visible to torch; the HIP backend needs one")
+    return torch
+
+
+@dataclass
+class Beam:
+    """Beam{HIPBackend}: Phase6DRep coordinates as one [6, n] float64 device tensor + BeamParams.
+
+    Row order x, px, y, py, z, pz (src/beam/Beam.jl:45-56)."""
+
+    coords: object
+    params: BeamParams
+
+    @property
+    def n(self) -> int:
+        return int(self.coords.shape[1])
+
+    def __len__(self):
+        return self.n
+
+    def struct(self):
+        b = _abi.BeamStruct()
+        c = self.coords
+        assert c.dtype.is_floating_point and c.element_size() == 8 and c.is_contiguous()
+        base = c.data_ptr()
+        stride = c.stride(0) * 8
+        b.x, b.px, b.y, b.py, b.z, b.pz = (base + k * stride for k in range(6))
+        b.n = self.n
+        b.charge, b.mc2, b.E0, b.r0, b.npart = (float(self.params.charge), float(self.params.mc2),
+                                               float(self.params.E0), float(self.params.r0),
+                                               float(self.params.npart))
+        b.on_device = 1
+        return b
+
+    @staticmethod
+    def from_host(coords, params: BeamParams, device="cuda"):
+        torch = _torch()
+        t = torch.as_tensor(coords, dtype=torch.float64).contiguous().to(device)
+        assert t.ndim == 2 and t.shape[0] == 6
+        return Beam(t, params)
+
+    def to_host(self):
+        return self.coords.detach().cpu().numpy()
+
+
+class PICCollider:
+    """A collide! handle with persistent device state: workspaces and the slice-pair Green cache in HBM,
+    the reference's StrongStrongTask.runtime_cache (pic_cpu.jl:23-32, 467-515)."""
+
+    def __init__(self, solver: PICPoissonSolver, device: int = 0):
+        torch = _torch()
+        self.solver = solver
+        self.device = device
+        o, keep = solver.to_options()
+        self._keep = keep
+        h = C.c_void_p()
+        rc = lib().octopic_create(C.byref(o), int(device), C.byref(h))
+        if rc != 0:
+            _raise(rc, "octopic_create rejected the solver options (see stderr for the message)")
+        self.h = h
+        self.set_stream(torch.cuda.current_stream(device).cuda_stream)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().octopic_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            _raise(rc, lib().octopic_last_error(self.h).decode())
+
+    def set_stream(self, stream_ptr):
+        self._check(lib().octopic_set_stream(self.h, C.c_void_p(stream_ptr)))
+
+    def collide(self, beam1: Beam, beam2: Beam, turn: Optional[int] = None) -> float:
+        """collide!(solver, beam1, beam2, HIPBackend, ctx) with ctx.turn = turn (None for ctx = nothing)."""
+        if beam1.coords.device != beam2.coords.device:
+            raise ArgumentError("beam1 and beam2 must live on the same device")
+        s1, s2 = beam1.struct(), beam2.struct()
+        lum = C.c_double()
+        cl = self.solver.compute_luminosity(turn)
+        self._check(lib().octopic_collide(self.h, C.byref(s1), C.byref(s2), -1 if turn is None else int(turn),
+                                          1 if cl else 0, C.byref(lum)))
+        return lum.value
+
+    def stats(self):
+        s = _abi.Stats()
+        self._check(lib().octopic_get_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _abi.Stats._fields_}
+
+    def pair_luminosity(self):
+        n = C.c_int64()
+        self._check(lib().octopic_pair_luminosity(self.h, None, None, None, None, 0, C.byref(n)))
+        m = max(n.value, 1)
+        i = (C.c_int32 * m)()
+        j = (C.c_int32 * m)()
+        r = (C.c_int32 * m)()
+        v = (C.c_double * m)()
+        self._check(lib().octopic_pair_luminosity(self.h, i, j, r, v, m, C.byref(n)))
+        k = n.value
+        return list(i)[:k], list(j)[:k], list(r)[:k], list(v)[:k]
+
+    def interaction_records(self):
+        n = C.c_int64()
+        self._check(lib().octopic_interaction_records(self.h, None, 0, C.byref(n)))
+        arr = (_abi.InteractionRecord * max(n.value, 1))()
+        self._check(lib().octopic_interaction_records(self.h, arr, n.value, C.byref(n)))
+        return list(arr)[: n.value]
+
+    def last_slices(self, which: int, n: int):
+        """(boundaries, centers, weights, counts, indices) of the last collide's slicing of beam `which`."""
+        import numpy as np
+        sl = self.solver.resolved_slicing1 if which == 1 else self.solver.resolved_slicing2
+        ns = sl.nslices
+        b = np.zeros(ns + 1)
+        c = np.zeros(ns)
+        w = np.zeros(ns)
+        counts = np.zeros(ns, dtype=np.int64)
+        idx = np.zeros(n, dtype=np.int64)
+        D = C.POINTER(C.c_double)
+        I = C.POINTER(C.c_int64)
+        self._check(lib().octopic_last_slices(self.h, which, b.ctypes.data_as(D), c.ctypes.data_as(D),
+                                              w.ctypes.data_as(D), counts.ctypes.data_as(I),
+                                              idx.ctypes.data_as(I), n))
+        return b, c, w, counts, idx
+
+    def track_linear6d(self, beam: Beam, matrix36: Sequence[float]):
+        m = (C.c_double * 36)(*[float(v) for v in matrix36])
+        s = beam.struct()
+        self._check(lib().octopic_track_linear6d(self.h, m, C.byref(s)))
+
+    def set_timing(self, enabled: bool):
+        self._check(lib().octopic_set_timing(self.h, 1 if enabled else 0))
+
+    def kernel_times(self):
+        """{family: (total_ms, launches)} accumulated since set_timing(True)."""
+        n = C.c_int64()
+        ms = (C.c_double * 32)()
+        cnt = (C.c_int64 * 32)()
+        names = C.create_string_buffer(1024)
+        self._check(lib().octopic_kernel_times(self.h, ms, cnt, 32, C.byref(n), names, 1024))
+        keys = names.raw.split(b"\0")[: n.value]
+        return {k.decode(): (ms[i], cnt[i]) for i, k in enumerate(keys)}
+
+    def debug_weights(self, method: str, u, npts: int):
+        """Device CIC/TSC stencil (0-based base, weights) for normalised coordinates u."""
+        import numpy as np
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        base = np.zeros(len(u), dtype=np.int32)
+        w = np.zeros((len(u), 3))
+        self._check(lib().octopic_debug_weights(self.h, _abi.DEPOSIT[method], u.ctypes.data_as(C.POINTER(C.c_double)),
+                                                len(u), int(npts), base.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                w.ctypes.data_as(C.POINTER(C.c_double))))
+        return base, w
+
+
+def generate_beam(collider: PICCollider, n: int, params: BeamParams, beta, alpha, sigma, seed: int, rng_id: int,
+                  cutoff: float = 5.0, device="cuda") -> Beam:
+    """Beam(N, HIPBackend; beta, alpha, sigma, cutoff, rng_id) built on the device from the Philox counter RNG
+    (Beam.jl:127-181, 466-515, 545-553)."""
+    torch = _torch()
+    t = torch.empty((6, n), dtype=torch.float64, device=device)
+    beam = Beam(t, params)
+    s = beam.struct()
+    D3 = C.c_double * 3
+    rc = lib().octopic_generate_gaussian_beam(collider.h, int(seed), int(rng_id), D3(*beta), D3(*alpha), D3(*sigma),
+                                              float(cutoff), C.byref(s))
+    collider._check(rc)
+    return beam
+
+
+def collide_(solver: PICPoissonSolver, beam1: Beam, beam2: Beam, backend=HIPBackend, ctx=None) -> float:
+    """collide!(solver, beam1, beam2, HIPBackend[, ctx]): a bare call uses a cold Green cache, as the
+    reference's bare collide! does (pic_cpu.jl:15-21).  ctx may be None or an int turn."""
+    if backend is not HIPBackend:
+        raise ArgumentError("octopic.collide_ runs the HIP backend only")
+    col = PICCollider(solver, device=beam1.coords.device.index or 0)
+    try:
+        return col.collide(beam1, beam2, turn=ctx)
+    finally:
+        col.close()

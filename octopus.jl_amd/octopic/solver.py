@@ -1,0 +1,280 @@
+"""Host-side mirror of the reference's PIC solver configuration.
+
+`PICPoissonSolver`, `LongitudinalSlicing`, `EveryNSteps` and `AtTurns` keep the reference's keyword
+names, defaults, validation order and error type, so that a caller of
+    PICPoissonSolver(; grid=(128,128), deposit_method=:TSC, ...)
+(src/tasks/strongstrong/interface.jl:1103-1507) writes
+    PICPoissonSolver(grid=(128, 128), deposit_method="TSC", ...)
+here.  Symbols are strings.  `to_options()` lowers a solver to the C ABI struct of include/octopic.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Optional, Sequence, Tuple
+
+from . import _abi
+
+# Physical constants (src/constants/Constants.jl)
+RE = 2.8179403205e-15
+EMASS_EV = 0.51099895069e6
+ME0 = EMASS_EV
+PMASS_EV = 938.27208943e6
+TWOPI = 6.283185307179586476925286766559005768394338
+
+
+class ArgumentError(ValueError):
+    """The reference's ArgumentError: an invalid or unsupported option, or invalid input."""
+
+
+@dataclass(frozen=True)
+class LongitudinalSlicing:
+    """LongitudinalSlicing(; nslices=1, method=:equal_area, resolution=100, center_position=:centroid,
+    positions=Float64[]) (interface.jl:800-835)."""
+
+    nslices: int = 1
+    method: str = "equal_area"
+    resolution: int = 100
+    center_position: str = "centroid"
+    positions: Tuple[float, ...] = ()
+
+    def __post_init__(self):
+        if self.nslices <= 0:
+            raise ArgumentError("nslices must be positive")
+        if self.resolution <= 0:
+            raise ArgumentError("slicing resolution must be positive")
+        if self.method not in _abi.SLICE_METHOD:
+            raise ArgumentError(f"unsupported longitudinal slicing method :{self.method}")
+        if self.center_position not in _abi.CENTER_POSITION:
+            raise ArgumentError("center_position must be :centroid or :midpoint")
+        object.__setattr__(self, "positions", tuple(float(p) for p in self.positions))
+        if self.method == "specified" and len(self.positions) != self.nslices - 1:
+            raise ArgumentError("specified slicing requires nslices-1 internal positions")
+
+    def _lower(self, keep):
+        s = _abi.Slicing()
+        s.nslices = self.nslices
+        s.method = _abi.SLICE_METHOD[self.method]
+        s.resolution = self.resolution
+        s.center_position = _abi.CENTER_POSITION[self.center_position]
+        s.npositions = len(self.positions)
+        if self.positions:
+            arr = (C.c_double * len(self.positions))(*self.positions)
+            keep.append(arr)
+            s.positions = C.cast(arr, C.POINTER(C.c_double))
+        return s
+
+
+@dataclass(frozen=True)
+class EveryNSteps:
+    """EveryNSteps(; start=0, stop=typemax(Int), step=1) (src/tasks/BeamObservers.jl:40-55, 74-78)."""
+
+    start: int = 0
+    stop: int = 2**63 - 1
+    step: int = 1
+
+    def __post_init__(self):
+        if self.step <= 0:
+            raise ArgumentError("step must be positive")
+
+    def should_run(self, turn: int) -> bool:
+        return turn >= self.start and turn < self.stop and (turn - self.start) % self.step == 0
+
+
+@dataclass(frozen=True)
+class AtTurns:
+    """AtTurns(turns) (src/tasks/BeamObservers.jl:57-60)."""
+
+    turns: frozenset = field(default_factory=frozenset)
+
+    def __init__(self, turns):
+        object.__setattr__(self, "turns", frozenset(int(t) for t in turns))
+
+    def should_run(self, turn: int) -> bool:
+        return turn in self.turns
+
+
+def _extent_pair(value):
+    """_solver_transverse_extent (interface.jl:1087-1101)."""
+    if isinstance(value, (int, float)):
+        req = (float(value), float(value))
+    elif isinstance(value, (tuple, list)) and len(value) == 2:
+        req = (float(value[0]), float(value[1]))
+    else:
+        raise ArgumentError(f"min_transverse_extent must be a real scalar or a two-real tuple; got {value!r}.")
+    if not all(math.isfinite(v) and v >= 0.0 for v in req):
+        raise ArgumentError(f"min_transverse_extent values must be finite and nonnegative; got {value!r}.")
+    return req
+
+
+@dataclass(frozen=True)
+class PICPoissonSolver:
+    """PICPoissonSolver{Float64} (interface.jl:1324-1507): keyword constructor with eager validation."""
+
+    kbb1: Optional[float] = None
+    kbb2: Optional[float] = None
+    luminosity_scale: Optional[float] = None
+    grid: Tuple[int, int] = (128, 128)
+    deposit_method: str = "CIC"
+    green_type: str = "integrated"
+    green_cache: str = "slice_pair"
+    field_derivative: str = "second"
+    slice_interpolation: str = "linear"
+    interaction_grid: str = "slice_pair"
+    grid_extent: str = "extrema"
+    grid_extent_sigma: float = 6.0
+    min_transverse_extent: Tuple[float, float] = (0.0, 0.0)
+    grid_quantize: float = 0.0
+    slice_pair_green_min_ratio: float = 0.50
+    slice_pair_green_growth: float = 0.25
+    longitudinal_kick: bool = True
+    batch_mode: str = "wavefront"
+    cuda_async: bool = True
+    cuda_batch_fft: bool = True
+    cuda_wavefront_fft: bool = True
+    cuda_indexed_wavefront: bool = True
+    luminosity_grid: Optional[Tuple[int, int]] = None
+    luminosity_deposit_method: Optional[str] = None
+    luminosity_schedule: object = None
+    slicing: LongitudinalSlicing = LongitudinalSlicing()
+    slicing1: Optional[LongitudinalSlicing] = None
+    slicing2: Optional[LongitudinalSlicing] = None
+
+    def __post_init__(self):
+        # Same order of checks as the reference constructor (interface.jl:1410-1476).
+        if not (0.0 < float(self.slice_pair_green_min_ratio) <= 1.0):
+            raise ArgumentError(
+                f"slice_pair_green_min_ratio must be in (0, 1]; got {self.slice_pair_green_min_ratio}.")
+        if not (float(self.slice_pair_green_growth) >= 0.0):
+            raise ArgumentError(f"slice_pair_green_growth must be non-negative; got {self.slice_pair_green_growth}.")
+        g = (int(self.grid[0]), int(self.grid[1]))
+        if not all(v >= 5 for v in g):
+            raise ArgumentError(f"PICPoissonSolver grid dimensions must both be at least 5; got {self.grid}.")
+        object.__setattr__(self, "grid", g)
+        if self.deposit_method not in _abi.DEPOSIT:
+            raise ArgumentError(f"deposit_method must be :CIC or :TSC; got :{self.deposit_method}.")
+        if self.green_type not in _abi.GREEN_TYPE:
+            raise ArgumentError(f"green_type must be :integrated, :standard or :lattice; got :{self.green_type}.")
+        if self.green_cache not in _abi.GREEN_CACHE:
+            raise ArgumentError(f"green_cache must be :none or :slice_pair; got :{self.green_cache}.")
+        if self.field_derivative not in _abi.FIELD_DERIVATIVE:
+            raise ArgumentError(f"field_derivative must be :second or :fourth; got :{self.field_derivative}.")
+        if self.slice_interpolation not in _abi.SLICE_INTERPOLATION:
+            raise ArgumentError(f"slice_interpolation must be :linear or :quadratic; got :{self.slice_interpolation}.")
+        if self.grid_extent not in _abi.GRID_EXTENT:
+            raise ArgumentError(f"grid_extent must be :extrema or :sigma; got :{self.grid_extent}.")
+        if not (float(self.grid_extent_sigma) > 0.0):
+            raise ArgumentError(f"grid_extent_sigma must be positive; got {self.grid_extent_sigma}.")
+        object.__setattr__(self, "min_transverse_extent", _extent_pair(self.min_transverse_extent))
+        if not (float(self.grid_quantize) >= 0.0):
+            raise ArgumentError(f"grid_quantize must be non-negative (0 disables); got {self.grid_quantize}.")
+        if self.interaction_grid not in _abi.INTERACTION_GRID:
+            raise ArgumentError(
+                f"interaction_grid must be :slice_pair, :source_slice or :node; got :{self.interaction_grid}.")
+        if not (self.interaction_grid == "slice_pair" or self.grid_extent == "extrema"):
+            raise ArgumentError(
+                f"grid_extent = :{self.grid_extent} is not implemented for interaction_grid = "
+                f":{self.interaction_grid}: that mode sizes its mesh from the union/per-node extrema of its own "
+                "particle sets and applies no extent estimator, so a non-default value would be silently ignored.")
+        if self.batch_mode not in _abi.BATCH_MODE:
+            raise ArgumentError(f"batch_mode must be :sequential or :wavefront; got :{self.batch_mode}.")
+        if self.luminosity_grid is not None:
+            lg = (int(self.luminosity_grid[0]), int(self.luminosity_grid[1]))
+            if not all(v >= 3 for v in lg):
+                raise ArgumentError(f"luminosity_grid dimensions must both be at least 3; got {self.luminosity_grid}.")
+            object.__setattr__(self, "luminosity_grid", lg)
+        if self.luminosity_deposit_method not in (None, "CIC", "TSC"):
+            raise ArgumentError(
+                f"luminosity_deposit_method must be nothing, :CIC, or :TSC; got :{self.luminosity_deposit_method}.")
+
+    # --- resolved views (interface.jl:1513-1517) ---
+    @property
+    def resolved_slicing1(self) -> LongitudinalSlicing:
+        return self.slicing1 if self.slicing1 is not None else self.slicing
+
+    @property
+    def resolved_slicing2(self) -> LongitudinalSlicing:
+        return self.slicing2 if self.slicing2 is not None else self.slicing
+
+    @property
+    def resolved_luminosity_grid(self):
+        return self.grid if self.luminosity_grid is None else self.luminosity_grid
+
+    @property
+    def resolved_luminosity_deposit_method(self):
+        return self.deposit_method if self.luminosity_deposit_method is None else self.luminosity_deposit_method
+
+    def compute_luminosity(self, turn: Optional[int]) -> bool:
+        """_pic_compute_luminosity (interface.jl:2596-2620): every turn unless a schedule says otherwise."""
+        if turn is None or self.luminosity_schedule is None:
+            return True
+        return bool(self.luminosity_schedule.should_run(turn))
+
+    def to_options(self):
+        """Lower to the C ABI options struct.  Returns (struct, keepalive)."""
+        keep = []
+        o = _abi.Options()
+        o.nx, o.ny = self.grid
+        o.deposit = _abi.DEPOSIT[self.deposit_method]
+        o.green_type = _abi.GREEN_TYPE[self.green_type]
+        o.green_cache = _abi.GREEN_CACHE[self.green_cache]
+        o.field_derivative = _abi.FIELD_DERIVATIVE[self.field_derivative]
+        o.slice_interpolation = _abi.SLICE_INTERPOLATION[self.slice_interpolation]
+        o.interaction_grid = _abi.INTERACTION_GRID[self.interaction_grid]
+        o.grid_extent = _abi.GRID_EXTENT[self.grid_extent]
+        o.longitudinal_kick = 1 if self.longitudinal_kick else 0
+        lg = self.resolved_luminosity_grid
+        o.lum_nx, o.lum_ny = lg
+        o.lum_deposit = _abi.DEPOSIT[self.resolved_luminosity_deposit_method]
+        o.batch_mode = _abi.BATCH_MODE[self.batch_mode]
+        o.cuda_flags = (int(self.cuda_async) | (int(self.cuda_batch_fft) << 1) |
+                        (int(self.cuda_wavefront_fft) << 2) | (int(self.cuda_indexed_wavefront) << 3))
+        o.grid_extent_sigma = float(self.grid_extent_sigma)
+        o.min_extent_x, o.min_extent_y = self.min_transverse_extent
+        o.grid_quantize = float(self.grid_quantize)
+        o.green_min_ratio = float(self.slice_pair_green_min_ratio)
+        o.green_growth = float(self.slice_pair_green_growth)
+        nan = float("nan")
+        o.kbb1 = nan if self.kbb1 is None else float(self.kbb1)
+        o.kbb2 = nan if self.kbb2 is None else float(self.kbb2)
+        o.luminosity_scale = nan if self.luminosity_scale is None else float(self.luminosity_scale)
+        o.slicing1 = self.resolved_slicing1._lower(keep)
+        o.slicing2 = self.resolved_slicing2._lower(keep)
+        return o, keep
+
+
+@dataclass
+class BeamParams:
+    """BeamParams(; charge, mc2, E0, r0, npart) (src/beam/Beam.jl:29-35)."""
+
+    charge: float = 0.0
+    mc2: float = 0.0
+    E0: float = 0.0
+    r0: float = 0.0
+    npart: float = 0.0
+
+
+def beam_params(charge, mass_ev, energy_ev, npart):
+    """The production recipe: r0 = RE * ME0 / mass (examples/strong_strong_tracking.jl:158-166)."""
+    return BeamParams(charge=float(charge), mc2=float(mass_ev), E0=float(energy_ev),
+                      r0=RE * ME0 / float(mass_ev), npart=float(npart))
+
+
+def linear6d_matrix(beta1: Sequence[float], dmu: Sequence[float], beta2: Optional[Sequence[float]] = None,
+                    alpha1: Sequence[float] = (0.0, 0.0, 0.0), alpha2: Optional[Sequence[float]] = None):
+    """Linear6DSpec(; beta1, dmu, beta2=beta1, alpha1, alpha2=alpha1) with zero coupling, dispersion
+    and crab dispersion: the block-diagonal Twiss map (src/elements/linear6d.jl:233-276).
+    Returns a row-major list of 36 floats."""
+    beta2 = beta1 if beta2 is None else beta2
+    alpha2 = alpha1 if alpha2 is None else alpha2
+    m = [0.0] * 36
+    for i in range(3):
+        s, c = math.sin(dmu[i]), math.cos(dmu[i])
+        r = 2 * i
+        b1, b2, a1, a2 = float(beta1[i]), float(beta2[i]), float(alpha1[i]), float(alpha2[i])
+        m[r * 6 + r] = math.sqrt(b2 / b1) * (c + a1 * s)
+        m[r * 6 + r + 1] = math.sqrt(b2 * b1) * s
+        m[(r + 1) * 6 + r] = (-(1.0 + a1 * a2) * s + (a1 - a2) * c) / math.sqrt(b1 * b2)
+        m[(r + 1) * 6 + r + 1] = math.sqrt(b1 / b2) * (c - a2 * s)
+    return m

@@ -1,0 +1,1845 @@
+// pic_oracle.cpp -- CPU oracle for the strong-strong PIC collision.
+//
+// TEST INFRASTRUCTURE ONLY.  This is a single-threaded (optionally OpenMP pair-batched) FP64
+// restatement of the reference's CPU algorithm, used as the checker of the HIP library and as the
+// timed CPU baseline.  Nothing in the product (octopus.jl_amd/) links, loads or calls it; only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg do.
+//
+// It follows, function by function (file:line cite the reference tree):
+//   src/tasks/strongstrong/pic_cpu.jl    the collision (_pic_collide!, _pic_interaction!, ...)
+//   src/tasks/strongstrong/slicing.jl    longitudinal slicing and the collision schedule
+//   src/math/counter_rng.jl              Philox4x32-10 counter RNG + Box-Muller
+//   src/beam/Beam.jl                     synthetic Gaussian beams
+//   src/elements/linear6d.jl             the linear one-turn map
+// Julia evaluation order is kept expression by expression (a*b*c == (a*b)*c, a-b+c == (a-b)+c),
+// Julia's NaN-propagating min/max/clamp are restated, and the file is compiled with
+// -ffp-contract=off so that no product is fused into an FMA the reference does not perform.
+// FFTW (the reference's FFT, third-party FFTW_jll 3.3.12) is restated as a mixed-radix DFT; its
+// rounding is not reproduced bitwise (the oracle is pinned against numpy.fft and a direct DFT).
+// SpecialFunctions.jl 2.8.0 erfinv is restated as erf-based Newton refinement of a rational seed.
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/octopic.h"
+
+namespace {
+
+using cd = std::complex<double>;
+using Vec = std::vector<double>;
+
+struct OracleError : std::runtime_error {
+    int code;
+    OracleError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+thread_local std::string g_last_error;
+
+// ------------------------------------------------------------------------------------------
+// Julia scalar semantics
+// ------------------------------------------------------------------------------------------
+// Base.min / Base.max for floats: NaN-propagating, -0.0 < 0.0.
+inline double jmin(double x, double y) {
+    if (std::isnan(x)) return x;
+    if (std::isnan(y)) return y;
+    if (y < x) return y;
+    if (x < y) return x;
+    return std::signbit(y) ? y : x;
+}
+inline double jmax(double x, double y) {
+    if (std::isnan(x)) return x;
+    if (std::isnan(y)) return y;
+    if (y > x) return y;
+    if (x > y) return x;
+    return std::signbit(y) ? x : y;
+}
+// Base.clamp(x, lo, hi) = ifelse(x > hi, hi, ifelse(x < lo, lo, x))
+inline double jclamp(double x, double lo, double hi) { return x > hi ? hi : (x < lo ? lo : x); }
+inline int64_t iclamp(int64_t x, int64_t lo, int64_t hi) { return x > hi ? hi : (x < lo ? lo : x); }
+// Base.isless for Float64 (NaN largest, -0.0 < 0.0)
+inline bool jisless(double a, double b) {
+    if (std::isnan(a)) return false;
+    if (std::isnan(b)) return true;
+    if (a < b) return true;
+    if (a > b) return false;
+    return std::signbit(a) && !std::signbit(b);
+}
+// Base.searchsortedlast(v, x) on a sorted vector, 1-based result (0 = before the first).
+inline int64_t searchsortedlast(const Vec &v, double x) {
+    int64_t lo = 0, hi = (int64_t)v.size() + 1;
+    while (lo < hi - 1) {
+        int64_t m = lo + ((hi - lo) >> 1);
+        if (jisless(x, v[m - 1])) hi = m; else lo = m;
+    }
+    return lo;
+}
+inline int64_t fld(int64_t a, int64_t b) {  // floor division, b > 0
+    int64_t q = a / b;
+    if ((a % b != 0) && ((a < 0) != (b < 0))) --q;
+    return q;
+}
+
+// ------------------------------------------------------------------------------------------
+// Counter RNG (counter_rng.jl:105-431)
+// ------------------------------------------------------------------------------------------
+constexpr uint32_t PHILOX_M0 = 0xD2511F53u, PHILOX_M1 = 0xCD9E8D57u;
+constexpr uint32_t PHILOX_W0 = 0x9E3779B9u, PHILOX_W1 = 0xBB67AE85u;
+constexpr double TWO_NEG_52 = 2.220446049250313e-16;
+constexpr double COUNTER_RNG_TWO_PI = 6.283185307179586476925286766559;
+
+inline uint64_t splitmix64(uint64_t x) {  // counter_rng.jl:405-410
+    uint64_t z = x + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+inline void philox_block(uint32_t c[4], uint32_t k0, uint32_t k1) {  // counter_rng.jl:232-240, 420-431
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)PHILOX_M0 * c[0];
+        uint64_t p1 = (uint64_t)PHILOX_M1 * c[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n1 = lo1, n2 = hi0 ^ c[3] ^ k1, n3 = lo0;
+        c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+        k0 += PHILOX_W0;
+        k1 += PHILOX_W1;
+    }
+}
+
+inline uint64_t counter_uint64(uint64_t seed, uint64_t turn, uint64_t rng_id, uint64_t particle,
+                               uint64_t component) {  // counter_rng.jl:184-220, 273-277
+    uint32_t c[4] = {(uint32_t)particle, (uint32_t)(particle >> 32), (uint32_t)turn,
+                     (uint32_t)(turn >> 32)};
+    uint64_t key = splitmix64(seed) ^ splitmix64(rng_id + 0x9e3779b97f4a7c15ull) ^
+                   splitmix64(component + 0xbf58476d1ce4e5b9ull);
+    philox_block(c, (uint32_t)key, (uint32_t)(key >> 32));
+    return ((uint64_t)c[0] << 32) | (uint64_t)c[1];
+}
+
+inline double uniform_open01(uint64_t v) {  // counter_rng.jl:337-340
+    uint64_t bits = v >> 12;
+    return ((double)bits + 0.5) * TWO_NEG_52;
+}
+
+inline double octopus_normal(uint64_t seed, uint64_t turn, uint64_t rng_id, uint64_t particle,
+                             int64_t component) {  // counter_rng.jl:153-173
+    int64_t pair_id = (component + 1) / 2;
+    double u1 = uniform_open01(counter_uint64(seed, turn, rng_id, particle, 2 * pair_id - 1));
+    double u2 = uniform_open01(counter_uint64(seed, turn, rng_id, particle, 2 * pair_id));
+    double r = std::sqrt(-2.0 * std::log(u1));
+    double theta = COUNTER_RNG_TWO_PI * u2;
+    return (component % 2 == 1) ? r * std::cos(theta) : r * std::sin(theta);
+}
+
+// Pairwise sum (Julia's `sum` reduces pairwise; its SIMD blocking is not reproduced).
+double pairwise_sum(const double *v, int64_t n) {
+    if (n <= 1024) {
+        double s = 0.0;
+        for (int64_t i = 0; i < n; ++i) s += v[i];
+        return s;
+    }
+    int64_t h = n >> 1;
+    return pairwise_sum(v, h) + pairwise_sum(v + h, n - h);
+}
+
+void standardize(Vec &v) {  // Beam.jl:148-160
+    int64_t N = (int64_t)v.size();
+    double mu = pairwise_sum(v.data(), N) / (double)N;
+    for (auto &a : v) a -= mu;
+    Vec sq(N);
+    for (int64_t i = 0; i < N; ++i) sq[i] = v[i] * v[i];
+    double s2 = pairwise_sum(sq.data(), N) / (double)N;
+    double s = std::sqrt(jmax(s2, 0.0));
+    if (s == 0.0) {
+        for (auto &a : v) a = 0.0;
+    } else {
+        for (auto &a : v) a /= s;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// erfinv (SpecialFunctions.erfinv via SpecialMath.jl:4) -- rational seed + Newton on erf/erfc
+// ------------------------------------------------------------------------------------------
+double erfinv(double y) {
+    if (std::isnan(y) || y < -1.0 || y > 1.0) return NAN;
+    if (y == 1.0) return INFINITY;
+    if (y == -1.0) return -INFINITY;
+    if (y == 0.0) return y;
+    // Seed: Giles, "Approximating the erfinv function" (GPU Computing Gems, 2011), single-precision form.
+    double w = -std::log((1.0 - y) * (1.0 + y));
+    double x;
+    if (w < 5.0) {
+        w = w - 2.5;
+        double p = 2.81022636e-08;
+        p = 3.43273939e-07 + p * w; p = -3.5233877e-06 + p * w; p = -4.39150654e-06 + p * w;
+        p = 0.00021858087 + p * w; p = -0.00125372503 + p * w; p = -0.00417768164 + p * w;
+        p = 0.246640727 + p * w; p = 1.50140941 + p * w;
+        x = p * y;
+    } else {
+        w = std::sqrt(w) - 3.0;
+        double p = -0.000200214257;
+        p = 0.000100950558 + p * w; p = 0.00134934322 + p * w; p = -0.00367342844 + p * w;
+        p = 0.00573950773 + p * w; p = -0.0076224613 + p * w; p = 0.00943887047 + p * w;
+        p = 1.00167406 + p * w; p = 2.83297682 + p * w;
+        x = p * y;
+    }
+    const double two_over_sqrtpi = 1.1283791670955126;
+    double ay = std::fabs(y);
+    for (int it = 0; it < 3; ++it) {
+        double ax = std::fabs(x);
+        double f;
+        if (ay > 0.5) f = (1.0 - ay) - std::erfc(ax);   // erf(ax) - ay, without cancellation
+        else f = std::erf(ax) - ay;
+        double d = two_over_sqrtpi * std::exp(-ax * ax);
+        double step = f / d;
+        // Halley correction: f'' / f' = -2x
+        step = step / (1.0 + ax * step);
+        ax = ax - step;
+        x = std::copysign(ax, y);
+    }
+    return x;
+}
+
+// ------------------------------------------------------------------------------------------
+// FFT: mixed-radix decimation-in-time DFT (restates FFTW's unnormalised forward / normalised inverse)
+// ------------------------------------------------------------------------------------------
+struct FFTPlan {
+    int n = 0;
+    std::vector<int> factors;
+    std::vector<cd> tw_fwd;   // exp(-2 pi i k / n)
+    explicit FFTPlan(int n_) : n(n_) {
+        int m = n;
+        for (int r : {4, 2, 3, 5}) {
+            while (m % r == 0) { factors.push_back(r); m /= r; }
+        }
+        if (m > 1) factors.push_back(m);
+        tw_fwd.resize(n);
+        for (int k = 0; k < n; ++k) {
+            long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+            tw_fwd[k] = cd((double)cosl(a), (double)sinl(a));
+        }
+    }
+    cd tw(int64_t k, int sign) const {
+        cd t = tw_fwd[(size_t)(k % n)];
+        return sign < 0 ? t : std::conj(t);
+    }
+    void rec(const cd *in, cd *out, int len, int stride, size_t fi, int sign) const {
+        if (len == 1) { out[0] = in[0]; return; }
+        int r = factors[fi];
+        int m = len / r;
+        for (int q = 0; q < r; ++q) rec(in + (size_t)q * stride, out + (size_t)q * m, m, stride * r, fi + 1, sign);
+        int step = n / len;
+        cd t[64];
+        std::vector<cd> tbig;
+        cd *tt = t;
+        if (r > 64) { tbig.resize(r); tt = tbig.data(); }
+        for (int k = 0; k < m; ++k) {
+            for (int q = 0; q < r; ++q) tt[q] = out[(size_t)q * m + k] * tw((int64_t)q * k * step, sign);
+            for (int s = 0; s < r; ++s) {
+                cd acc = tt[0];
+                for (int q = 1; q < r; ++q) acc += tt[q] * tw((int64_t)((q * s) % r) * (n / r), sign);
+                out[(size_t)k + (size_t)m * s] = acc;
+            }
+        }
+    }
+    // In-place transform of `len` = n points with the given element stride.
+    void exec(cd *data, int64_t stride, int sign, std::vector<cd> &scratch_in, std::vector<cd> &scratch_out) const {
+        scratch_in.resize(n); scratch_out.resize(n);
+        for (int k = 0; k < n; ++k) scratch_in[k] = data[k * stride];
+        rec(scratch_in.data(), scratch_out.data(), n, 1, 0, sign);
+        for (int k = 0; k < n; ++k) data[k * stride] = scratch_out[k];
+    }
+};
+
+// Column-major P1 x P2 complex array; full 2D DFT.  sign -1 forward (unnormalised), +1 inverse (1/(P1 P2)).
+void fft2(cd *a, int P1, int P2, int sign, const FFTPlan &p1, const FFTPlan &p2) {
+    std::vector<cd> si, so;
+    for (int j = 0; j < P2; ++j) p1.exec(a + (size_t)j * P1, 1, sign, si, so);
+    for (int i = 0; i < P1; ++i) p2.exec(a + i, P1, sign, si, so);
+    if (sign > 0) {
+        double scale = 1.0 / (double)((int64_t)P1 * (int64_t)P2);
+        for (int64_t k = 0; k < (int64_t)P1 * P2; ++k) a[k] *= scale;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Options (PICPoissonSolver, interface.jl:1324-1507)
+// ------------------------------------------------------------------------------------------
+struct Slicing {
+    int nslices = 1, method = OCTOPIC_SLICE_EQUAL_AREA, resolution = 100, center = OCTOPIC_CENTER_CENTROID;
+    Vec positions;
+};
+
+struct Opt {
+    int nx = 128, ny = 128, deposit = OCTOPIC_CIC, green_type = OCTOPIC_GREEN_INTEGRATED;
+    int green_cache = OCTOPIC_CACHE_SLICE_PAIR, field_derivative = 2, interp = OCTOPIC_INTERP_LINEAR;
+    int igrid = OCTOPIC_GRID_SLICE_PAIR, extent = OCTOPIC_EXTENT_EXTREMA, lkick = 1;
+    int lum_nx = 128, lum_ny = 128, lum_deposit = OCTOPIC_CIC;
+    double extent_sigma = 6.0, min_x = 0.0, min_y = 0.0, quantize = 0.0, min_ratio = 0.5, growth = 0.25;
+    double kbb1 = NAN, kbb2 = NAN, lum_scale = NAN;
+    Slicing s1, s2;
+};
+
+Slicing to_slicing(const octopic_slicing &s) {
+    Slicing r;
+    r.nslices = s.nslices; r.method = s.method; r.resolution = s.resolution; r.center = s.center_position;
+    if (s.positions && s.npositions > 0) r.positions.assign(s.positions, s.positions + s.npositions);
+    return r;
+}
+
+void validate_slicing(const Slicing &s) {  // interface.jl:816-835
+    if (s.nslices <= 0) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "nslices must be positive");
+    if (s.resolution <= 0) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "slicing resolution must be positive");
+    if (s.method < 0 || s.method > 4) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "unsupported longitudinal slicing method");
+    if (s.center != OCTOPIC_CENTER_CENTROID && s.center != OCTOPIC_CENTER_MIDPOINT)
+        throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "center_position must be :centroid or :midpoint");
+    if (s.method == OCTOPIC_SLICE_SPECIFIED && (int)s.positions.size() != s.nslices - 1)
+        throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "specified slicing requires nslices-1 internal positions");
+}
+
+Opt to_opt(const octopic_options &o) {
+    Opt r;
+    r.nx = o.nx; r.ny = o.ny; r.deposit = o.deposit; r.green_type = o.green_type; r.green_cache = o.green_cache;
+    r.field_derivative = o.field_derivative; r.interp = o.slice_interpolation; r.igrid = o.interaction_grid;
+    r.extent = o.grid_extent; r.lkick = o.longitudinal_kick ? 1 : 0;
+    r.lum_nx = o.lum_nx > 0 ? o.lum_nx : o.nx;
+    r.lum_ny = o.lum_ny > 0 ? o.lum_ny : o.ny;
+    r.lum_deposit = o.lum_deposit >= 0 ? o.lum_deposit : o.deposit;
+    r.extent_sigma = o.grid_extent_sigma; r.min_x = o.min_extent_x; r.min_y = o.min_extent_y;
+    r.quantize = o.grid_quantize; r.min_ratio = o.green_min_ratio; r.growth = o.green_growth;
+    r.kbb1 = o.kbb1; r.kbb2 = o.kbb2; r.lum_scale = o.luminosity_scale;
+    r.s1 = to_slicing(o.slicing1); r.s2 = to_slicing(o.slicing2);
+    // Constructor checks (interface.jl:1410-1476) and _validate_pic_solver (pic_cpu.jl:517-576).
+    if (!(r.min_ratio > 0.0 && r.min_ratio <= 1.0))
+        throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "slice_pair_green_min_ratio must be in (0, 1]");
+    if (!(r.growth >= 0.0)) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "slice_pair_green_growth must be non-negative");
+    if (r.nx < 5 || r.ny < 5) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "PICPoissonSolver grid must be at least (5, 5)");
+    if (r.deposit != OCTOPIC_CIC && r.deposit != OCTOPIC_TSC) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "deposit_method must be :CIC or :TSC");
+    if (r.lum_deposit != OCTOPIC_CIC && r.lum_deposit != OCTOPIC_TSC)
+        throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "luminosity_deposit_method must resolve to :CIC or :TSC");
+    if (r.green_type < 0 || r.green_type > 2) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "green_type must be :integrated, :standard or :lattice");
+    if (r.green_cache != OCTOPIC_CACHE_NONE && r.green_cache != OCTOPIC_CACHE_SLICE_PAIR)
+        throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "green_cache must be :none or :slice_pair");
+    if (r.field_derivative != 2 && r.field_derivative != 4)
+        throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "field_derivative must be :second or :fourth");
+    if (r.interp != OCTOPIC_INTERP_LINEAR && r.interp != OCTOPIC_INTERP_QUADRATIC)
+        throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "slice_interpolation must be :linear or :quadratic");
+    if (r.igrid < 0 || r.igrid > 2) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "interaction_grid must be :slice_pair, :source_slice or :node");
+    if (r.extent != OCTOPIC_EXTENT_EXTREMA && r.extent != OCTOPIC_EXTENT_SIGMA)
+        throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "grid_extent must be :extrema or :sigma");
+    if (!(r.extent_sigma > 0.0)) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "grid_extent_sigma must be positive");
+    if (!(std::isfinite(r.min_x) && std::isfinite(r.min_y) && r.min_x >= 0.0 && r.min_y >= 0.0))
+        throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "min_transverse_extent values must be finite and nonnegative");
+    if (!(r.quantize >= 0.0)) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "grid_quantize must be non-negative");
+    if (r.lum_nx < 3 || r.lum_ny < 3) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "luminosity_grid dimensions must both be at least 3");
+    if (r.igrid != OCTOPIC_GRID_SLICE_PAIR && r.extent != OCTOPIC_EXTENT_EXTREMA)
+        throw OracleError(OCTOPIC_ERR_UNSUPPORTED, "grid_extent is not implemented for this interaction_grid");
+    if (r.igrid == OCTOPIC_GRID_NODE && r.interp != OCTOPIC_INTERP_LINEAR)
+        throw OracleError(OCTOPIC_ERR_UNSUPPORTED,
+                          "slice_interpolation = :quadratic is not implemented for interaction_grid = :node");
+    if (r.green_type == OCTOPIC_GREEN_LATTICE)
+        throw OracleError(OCTOPIC_ERR_UNSUPPORTED, "green_type = :lattice is not restated by the oracle");
+    validate_slicing(r.s1);
+    validate_slicing(r.s2);
+    return r;
+}
+
+// ------------------------------------------------------------------------------------------
+// Longitudinal slicing (slicing.jl)
+// ------------------------------------------------------------------------------------------
+constexpr int64_t FOLD_LANES = 4096;   // _SLICE_FOLD_LANES, slicing.jl:223
+
+struct Slices {
+    Vec center, weight, boundary;
+    std::vector<std::vector<int64_t>> indices;   // 0-based particle indices
+};
+
+struct ZStats { int64_t n; double zmin, zmax, mean, sigma; };
+
+double lane_z_moment(const double *z, int64_t n, double mu, int pw) {  // slicing.jl:227-241
+    std::vector<double> acc(FOLD_LANES, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+        double v = pw == 2 ? (z[i] - mu) * (z[i] - mu) : z[i];
+        acc[i % FOLD_LANES] += v;
+    }
+    double s = 0.0;
+    for (int64_t t = 0; t < FOLD_LANES; ++t) s += acc[t];
+    return s;
+}
+
+double lane_indexed_sum(const double *z, const std::vector<int64_t> &idx) {  // slicing.jl:245-257
+    std::vector<double> acc(FOLD_LANES, 0.0);
+    for (size_t k = 0; k < idx.size(); ++k) acc[k % FOLD_LANES] += z[idx[k]];
+    double s = 0.0;
+    for (int64_t t = 0; t < FOLD_LANES; ++t) s += acc[t];
+    return s;
+}
+
+ZStats live_z_stats(const double *z, int64_t n) {  // slicing.jl:271-289
+    double zmin = INFINITY, zmax = -INFINITY;
+    for (int64_t i = 0; i < n; ++i) { zmin = jmin(zmin, z[i]); zmax = jmax(zmax, z[i]); }
+    double mu = lane_z_moment(z, n, 0.0, 1) / (double)n;
+    double s2 = lane_z_moment(z, n, mu, 2);
+    return ZStats{n, zmin, zmax, mu, std::sqrt(jmax(s2 / (double)n, 0.0))};
+}
+
+inline int64_t slice_bin(double zi, double zmin, double width, int64_t bins) {  // slicing.jl:172-176
+    double d = (zi - zmin) / width;
+    if (!(d > -INFINITY && d < INFINITY)) return 0;
+    return iclamp((int64_t)std::floor(d) + 1, 1, bins);
+}
+
+Slices finish_slices(const double *z, int64_t n, const Slicing &sl, std::vector<std::vector<int64_t>> indices,
+                     const Vec &boundaries) {  // slicing.jl:523-559
+    for (double b : boundaries)
+        if (!std::isfinite(b))
+            throw OracleError(OCTOPIC_ERR_NONFINITE, "non-finite particle coordinate(s) detected (longitudinal slicing)");
+    int64_t total = n;
+    size_t ns = indices.size();
+    Slices s;
+    s.center.resize(ns); s.weight.resize(ns);
+    for (size_t k = 0; k < ns; ++k) {
+        const auto &idx = indices[k];
+        s.weight[k] = (double)idx.size() / (double)total;
+        if (sl.center == OCTOPIC_CENTER_CENTROID)
+            s.center[k] = idx.empty() ? (boundaries[k] + boundaries[k + 1]) / 2.0
+                                      : lane_indexed_sum(z, idx) / (double)idx.size();
+        else
+            s.center[k] = (boundaries[k] + boundaries[k + 1]) / 2.0;
+    }
+    s.boundary = boundaries;
+    s.indices = std::move(indices);
+    return s;
+}
+
+Slices slices_from_boundaries(const double *z, int64_t n, const Slicing &sl, const Vec &b) {  // slicing.jl:431-460
+    int64_t ns = (int64_t)b.size() - 1;
+    std::vector<std::vector<int64_t>> idx(ns);
+    if (ns > 0 && b[0] == b[ns]) {
+        for (int64_t i = 0; i < n; ++i) idx[0].push_back(i);
+        return finish_slices(z, n, sl, std::move(idx), b);
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t s = iclamp(searchsortedlast(b, z[i]), 1, ns);
+        idx[s - 1].push_back(i);
+    }
+    return finish_slices(z, n, sl, std::move(idx), b);
+}
+
+// range(zmin, zmax; length = ns+1).  Julia evaluates the range in TwicePrecision; this restatement
+// uses zmin + k*((zmax - zmin)/ns), which can differ from it in the last ulp (parity unpinned).
+Vec linspace(double a, double b, int64_t len) {
+    Vec r(len);
+    double step = (b - a) / (double)(len - 1);
+    for (int64_t i = 0; i < len; ++i) r[i] = a + (double)i * step;
+    r[0] = a; r[len - 1] = b;
+    return r;
+}
+
+Slices longitudinal_slices(const double *z, int64_t n, const Slicing &sl) {  // slicing.jl:51-75
+    if (sl.nslices <= 0) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "nslices must be positive");
+    if (n <= 0) throw OracleError(OCTOPIC_ERR_INVALID_ARGUMENT, "longitudinal slicing requires at least one particle");
+    int64_t ns = sl.nslices;
+    if (sl.method == OCTOPIC_SLICE_EQUAL_AREA) {  // slicing.jl:104-156
+        int64_t bins = ns * sl.resolution;
+        ZStats st = live_z_stats(z, n);
+        if (st.zmin == st.zmax) {
+            Vec b(ns + 1, st.zmin);
+            std::vector<std::vector<int64_t>> idx(ns);
+            for (int64_t i = 0; i < n; ++i) idx[0].push_back(i);
+            return finish_slices(z, n, sl, std::move(idx), b);
+        }
+        double width = (st.zmax - st.zmin) / (double)bins;
+        std::vector<int64_t> counts(bins, 0);
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t bin = slice_bin(z[i], st.zmin, width, bins);
+            if (bin != 0) counts[bin - 1] += 1;
+        }
+        Vec cum(bins);
+        int64_t run = 0;
+        for (int64_t k = 0; k < bins; ++k) { run += counts[k]; cum[k] = (double)run / (double)std::max<int64_t>(st.n, 1); }
+        cum[bins - 1] = 1.0;
+        Vec centers(bins);
+        for (int64_t k = 0; k < bins; ++k) centers[k] = st.zmin + ((double)(k + 1) - 0.5) * width;
+        Vec b(ns + 1);
+        b[0] = st.zmin; b[ns] = st.zmax;
+        int64_t current = 1;
+        for (int64_t s = 1; s <= ns - 1; ++s) {
+            double target = (double)s / (double)ns;
+            while (current <= bins && cum[current - 1] <= target) current += 1;
+            double x1, x2, y1, y2;
+            if (current <= 1) { x1 = b[0]; x2 = centers[0]; y1 = 0.0; y2 = cum[0]; }
+            else if (current > bins) { x1 = centers[bins - 1]; x2 = b[ns]; y1 = cum[bins - 2]; y2 = 1.0; }
+            else { x1 = centers[current - 2]; x2 = centers[current - 1]; y1 = cum[current - 2]; y2 = cum[current - 1]; }
+            b[s] = (y2 == y1) ? (x1 + x2) / 2.0 : x2 * (target - y1) / (y2 - y1) + x1 * (target - y2) / (y1 - y2);
+        }
+        return slices_from_boundaries(z, n, sl, b);
+    }
+    if (sl.method == OCTOPIC_SLICE_EQUAL_COUNT) {  // slicing.jl:330-365
+        std::vector<int64_t> order(n);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return jisless(z[a], z[b]); });
+        std::vector<std::vector<int64_t>> idx(ns);
+        for (int64_t s = 1; s <= ns; ++s) {
+            int64_t first = (int64_t)std::floor((double)((s - 1) * n) / (double)ns) + 1;
+            int64_t last = (int64_t)std::floor((double)(s * n) / (double)ns);
+            for (int64_t p = first; p <= last; ++p) idx[s - 1].push_back(order[p - 1]);
+        }
+        ZStats st = live_z_stats(z, n);
+        Vec b(ns + 1);
+        b[0] = st.zmin; b[ns] = st.zmax;
+        for (int64_t s = 1; s <= ns - 1; ++s) {
+            int64_t pos = (int64_t)std::floor((double)(s * n) / (double)ns);
+            if (pos == 0) b[s] = z[order[0]];
+            else if (pos == n) b[s] = z[order[n - 1]];
+            else b[s] = (z[order[pos - 1]] + z[order[pos]]) / 2.0;
+        }
+        return finish_slices(z, n, sl, std::move(idx), b);
+    }
+    if (sl.method == OCTOPIC_SLICE_EQUAL_WIDTH) {  // slicing.jl:367-387
+        ZStats st = live_z_stats(z, n);
+        Vec b = linspace(st.zmin, st.zmax, ns + 1);
+        std::vector<std::vector<int64_t>> idx(ns);
+        if (st.zmin == st.zmax) {
+            for (int64_t i = 0; i < n; ++i) idx[0].push_back(i);
+            return finish_slices(z, n, sl, std::move(idx), b);
+        }
+        double width = (st.zmax - st.zmin) / (double)ns;
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t s = slice_bin(z[i], st.zmin, width, ns);
+            if (s != 0) idx[s - 1].push_back(i);
+        }
+        return finish_slices(z, n, sl, std::move(idx), b);
+    }
+    if (sl.method == OCTOPIC_SLICE_NORMAL_QUANTILE) {  // slicing.jl:405-429
+        ZStats st = live_z_stats(z, n);
+        if (st.sigma == 0.0) return slices_from_boundaries(z, n, sl, Vec(ns + 1, st.mean));
+        Vec b(ns + 1);
+        b[0] = st.zmin; b[ns] = st.zmax;
+        for (int64_t s = 1; s <= ns - 1; ++s) {
+            double q = std::sqrt(2.0) * erfinv((double)(2 * s) / (double)ns - 1.0);
+            b[s] = jclamp(st.mean + st.sigma * q, b[0], b[ns]);
+        }
+        return slices_from_boundaries(z, n, sl, b);
+    }
+    if (sl.method == OCTOPIC_SLICE_SPECIFIED) {  // slicing.jl:389-403
+        ZStats st = live_z_stats(z, n);
+        Vec internal;
+        for (double p : sl.positions) internal.push_back(st.mean + p * st.sigma);
+        std::stable_sort(internal.begin(), internal.end(), jisless);
+        Vec b(internal.size() + 2);
+        b[0] = st.zmin; b[b.size() - 1] = st.zmax;
+        for (size_t k = 0; k < internal.size(); ++k) b[k + 1] = jclamp(internal[k], b[0], b[b.size() - 1]);
+        return slices_from_boundaries(z, n, sl, b);
+    }
+    throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "unknown longitudinal slicing method");
+}
+
+struct OrderEntry { double t; int i, j; };
+
+std::vector<OrderEntry> collision_order(const Vec &c1, const Vec &c2) {  // slicing.jl:561-578
+    std::vector<OrderEntry> order;
+    for (int i = 0; i < (int)c1.size(); ++i)
+        for (int j = 0; j < (int)c2.size(); ++j) order.push_back({-(c1[i] + c2[j]) / 2.0, i, j});
+    std::stable_sort(order.begin(), order.end(), [](const OrderEntry &a, const OrderEntry &b) { return jisless(a.t, b.t); });
+    return order;
+}
+
+std::vector<std::vector<OrderEntry>> collision_pair_batches(const Vec &c1, const Vec &c2) {  // slicing.jl:601-647
+    auto pairs = collision_order(c1, c2);
+    std::vector<std::vector<OrderEntry>> batches;
+    if (pairs.empty()) return batches;
+    int ns1 = (int)c1.size(), ns2 = (int)c2.size();
+    std::vector<std::vector<int>> by_i(ns1), by_j(ns2);
+    for (int k = 0; k < (int)pairs.size(); ++k) { by_i[pairs[k].i].push_back(k); by_j[pairs[k].j].push_back(k); }
+    std::vector<int> next_i(ns1, 0), next_j(ns2, 0);
+    std::vector<char> done(pairs.size(), 0);
+    int remaining = (int)pairs.size();
+    while (remaining > 0) {
+        std::vector<OrderEntry> cur;
+        std::vector<char> used_i(ns1, 0), used_j(ns2, 0);
+        for (int k = 0; k < (int)pairs.size(); ++k) {
+            if (done[k]) continue;
+            const auto &p = pairs[k];
+            if (used_i[p.i] || used_j[p.j]) continue;
+            bool ri = next_i[p.i] < (int)by_i[p.i].size() && by_i[p.i][next_i[p.i]] == k;
+            bool rj = next_j[p.j] < (int)by_j[p.j].size() && by_j[p.j][next_j[p.j]] == k;
+            if (ri && rj) {
+                cur.push_back(p); used_i[p.i] = 1; used_j[p.j] = 1; done[k] = 1;
+                next_i[p.i] += 1; next_j[p.j] += 1; remaining -= 1;
+            }
+        }
+        if (cur.empty()) throw OracleError(OCTOPIC_ERR_INVALID_ARGUMENT, "internal collision scheduler error");
+        batches.push_back(std::move(cur));
+    }
+    return batches;
+}
+
+// ------------------------------------------------------------------------------------------
+// Deposition weights (pic_cpu.jl:2182-2252)
+// ------------------------------------------------------------------------------------------
+// base is the 1-based first stencil node, as in the reference.
+inline void cic_weights(double u, int64_t n, int64_t &base, double w[3]) {
+    if (!(0.0 <= u && u <= (double)(n - 1))) { base = 1; w[0] = w[1] = 0.0; w[2] = 0.0; return; }
+    base = iclamp((int64_t)std::floor(u) + 1, 1, n - 1);
+    double f = jclamp(u - (double)(base - 1), 0.0, 1.0);
+    w[0] = 1.0 - f; w[1] = f; w[2] = 0.0;
+}
+inline void tsc_weights(double u, int64_t n, int64_t &base, double w[3]) {
+    if (!(0.0 <= u && u <= (double)(n - 1))) { base = 1; w[0] = w[1] = w[2] = 0.0; return; }
+    int64_t ix = (int64_t)std::floor(u);
+    double f = u - std::floor(u);
+    if (f < 0.5) {
+        double t = f * f;
+        w[0] = 0.125 + 0.5 * (t - f); w[1] = 0.75 - t; w[2] = 0.125 + 0.5 * (t + f);
+        base = ix;
+    } else {
+        double fr = 1.0 - f;
+        double t = fr * fr;
+        w[0] = 0.125 + 0.5 * (t + fr); w[1] = 0.75 - t; w[2] = 0.125 + 0.5 * (t - fr);
+        base = ix + 1;
+    }
+    base = iclamp(base, 1, n - 2);
+}
+inline int weights(int method, double u, int64_t n, int64_t &base, double w[3]) {
+    if (method == OCTOPIC_CIC) { cic_weights(u, n, base, w); return 2; }
+    tsc_weights(u, n, base, w);
+    return 3;
+}
+
+// ------------------------------------------------------------------------------------------
+// Green function (pic_cpu.jl:2254-2272, 2451-2509)
+// ------------------------------------------------------------------------------------------
+inline double atan_ratio(double num, double den) {
+    if (den == 0.0) {
+        if (num == 0.0) return 0.0;
+        return std::copysign(M_PI / 2.0, num);
+    }
+    return std::atan(num / den);
+}
+inline double kernel_integral(double x, double y) {
+    double r2 = x * x + y * y;
+    if (r2 == 0.0) return 0.0;
+    return (std::log(r2) - 3.0) * x * y + atan_ratio(y, x) * x * x + atan_ratio(x, y) * y * y;
+}
+
+// green: column-major 2nx x 2ny
+void green_table(double *green, int gtype, double fx0, double fy0, double sx0, double sy0, double hx, double hy,
+                 int nx, int ny) {
+    double half_hx = hx / 2.0, half_hy = hy / 2.0;
+    double hxihyi = -0.5 / (hx * hy);
+    bool integrated = gtype == OCTOPIC_GREEN_INTEGRATED;
+    int64_t P1 = 2 * (int64_t)nx;
+    for (int64_t j = 0; j < 2 * (int64_t)ny; ++j) {
+        for (int64_t i = 0; i < P1; ++i) {
+            int64_t ii = i < nx ? i : i - 2 * (int64_t)nx;
+            int64_t jj = j < ny ? j : j - 2 * (int64_t)ny;
+            double x = fx0 - sx0 + (double)ii * hx;
+            double y = fy0 - sy0 + (double)jj * hy;
+            double val;
+            if (integrated || (x * x + y * y) == 0.0) {
+                val = kernel_integral(x + half_hx, y + half_hy);
+                val += kernel_integral(x - half_hx, y - half_hy);
+                val -= kernel_integral(x + half_hx, y - half_hy);
+                val -= kernel_integral(x - half_hx, y + half_hy);
+                green[i + j * P1] = hxihyi * val;
+            } else {
+                green[i + j * P1] = -0.5 * std::log(x * x + y * y);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Field derivative (pic_cpu.jl:2531-2591), column-major nx x ny
+// ------------------------------------------------------------------------------------------
+void field_derivative(double *Ex, double *Ey, const double *phi, int nx, int ny, double hx, double hy, bool fourth) {
+    auto P = [&](int i, int j) { return phi[(i - 1) + (int64_t)(j - 1) * nx]; };
+    auto EX = [&](int i, int j) -> double & { return Ex[(i - 1) + (int64_t)(j - 1) * nx]; };
+    auto EY = [&](int i, int j) -> double & { return Ey[(i - 1) + (int64_t)(j - 1) * nx]; };
+    double hxi = 1.0 / hx, hyi = 1.0 / hy;
+    double c4 = 1.0 / 12.0;
+    for (int i = 1; i <= nx; ++i) {
+        EY(i, 1) = hyi * (1.5 * P(i, 1) - 2.0 * P(i, 2) + 0.5 * P(i, 3));
+        EY(i, ny) = hyi * (-1.5 * P(i, ny) + 2.0 * P(i, ny - 1) - 0.5 * P(i, ny - 2));
+    }
+    if (fourth && ny >= 5) {
+        for (int i = 1; i <= nx; ++i) {
+            EY(i, 2) = 0.5 * hyi * (P(i, 1) - P(i, 3));
+            EY(i, ny - 1) = 0.5 * hyi * (P(i, ny - 2) - P(i, ny));
+        }
+        for (int j = 3; j <= ny - 2; ++j)
+            for (int i = 1; i <= nx; ++i)
+                EY(i, j) = c4 * hyi * ((P(i, j + 2) - P(i, j - 2)) + 8.0 * (P(i, j - 1) - P(i, j + 1)));
+    } else {
+        for (int j = 2; j <= ny - 1; ++j)
+            for (int i = 1; i <= nx; ++i) EY(i, j) = 0.5 * hyi * (P(i, j - 1) - P(i, j + 1));
+    }
+    for (int j = 1; j <= ny; ++j) {
+        EX(1, j) = hxi * (1.5 * P(1, j) - 2.0 * P(2, j) + 0.5 * P(3, j));
+        EX(nx, j) = hxi * (-1.5 * P(nx, j) + 2.0 * P(nx - 1, j) - 0.5 * P(nx - 2, j));
+        if (fourth && nx >= 5) {
+            EX(2, j) = 0.5 * hxi * (P(1, j) - P(3, j));
+            EX(nx - 1, j) = 0.5 * hxi * (P(nx - 2, j) - P(nx, j));
+            for (int i = 3; i <= nx - 2; ++i)
+                EX(i, j) = c4 * hxi * ((P(i + 2, j) - P(i - 2, j)) + 8.0 * (P(i - 1, j) - P(i + 1, j)));
+        } else {
+            for (int i = 2; i <= nx - 1; ++i) EX(i, j) = 0.5 * hxi * (P(i - 1, j) - P(i + 1, j));
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Grids (pic_cpu.jl:1634-1718, 1780-1905)
+// ------------------------------------------------------------------------------------------
+struct Grid { double x0, y0, width, height; };
+struct Box { double xmin, xmax, ymin, ymax; };
+
+void resolve_extent(const Opt &o, double xspan, double yspan, double &w, double &h, const char *ctx) {
+    w = jmax(xspan, o.min_x);
+    h = jmax(yspan, o.min_y);
+    if (!(std::isfinite(w) && std::isfinite(h) && w > 0.0 && h > 0.0)) {
+        char buf[512];
+        snprintf(buf, sizeof buf,
+                 "%s requires finite, positive transverse extents; observed (x=%g, y=%g) with "
+                 "min_transverse_extent=(%g, %g). Supply a positive physical bound for every degenerate axis.",
+                 ctx, xspan, yspan, o.min_x, o.min_y);
+        throw OracleError(OCTOPIC_ERR_DEGENERATE_EXTENT, buf);
+    }
+}
+
+inline double quantize_extent(double w, double q) {
+    if (!(q > 0.0 && std::isfinite(w) && w > 0.0)) return w;
+    return std::exp2(std::ceil(std::log2(w) / q) * q);
+}
+
+inline void align_origins(int gtype, double s0, double f0, double h, double &so, double &fo) {
+    double f1 = s0 / h - std::floor(s0 / h);
+    double f2 = f0 / h - std::floor(f0 / h);
+    double t = (f2 - f1) / 2.0;
+    if (gtype == OCTOPIC_GREEN_STANDARD) {
+        t += t > 0.0 ? -0.25 : 0.25;
+    } else {
+        if (t > 0.5) t -= 0.5;
+        else if (t < -0.5) t += 0.5;
+    }
+    double shift = t * h;
+    so = s0 + shift;
+    fo = f0 - shift;
+}
+
+void interaction_grids(const Opt &o, double sxmin, double sxmax, double symin, double symax, double fxmin,
+                       double fxmax, double fymin, double fymax, Grid &sg, Grid &fg) {
+    double width, height;
+    resolve_extent(o, jmax(sxmax - sxmin, fxmax - fxmin), jmax(symax - symin, fymax - fymin), width, height,
+                   "PIC interaction grid");
+    double tx = width / (double)(o.nx - 4);
+    double ty = height / (double)(o.ny - 4);
+    width += 3.0 * tx;
+    height += 3.0 * ty;
+    double q = o.quantize;
+    if (q > 0.0) {
+        width = quantize_extent(width, q);
+        height = quantize_extent(height, q);
+        double hx = width / (double)(o.nx - 1);
+        double hy = height / (double)(o.ny - 1);
+        double sx0 = std::nearbyint(((sxmin + sxmax) / 2.0 - width / 2.0) / hx) * hx;
+        double sy0 = std::nearbyint(((symin + symax) / 2.0 - height / 2.0) / hy) * hy;
+        double fx0 = std::nearbyint(((fxmin + fxmax) / 2.0 - width / 2.0) / hx) * hx;
+        double fy0 = std::nearbyint(((fymin + fymax) / 2.0 - height / 2.0) / hy) * hy;
+        align_origins(o.green_type, sx0, fx0, hx, sx0, fx0);
+        align_origins(o.green_type, sy0, fy0, hy, sy0, fy0);
+        sg = {sx0, sy0, width, height};
+        fg = {fx0, fy0, width, height};
+        return;
+    }
+    double sx0 = sxmin - 1.5 * tx, sy0 = symin - 1.5 * ty;
+    double fx0 = fxmin - 1.5 * tx, fy0 = fymin - 1.5 * ty;
+    align_origins(o.green_type, sx0, fx0, tx, sx0, fx0);
+    align_origins(o.green_type, sy0, fy0, ty, sy0, fy0);
+    sg = {sx0, sy0, width, height};
+    fg = {fx0, fy0, width, height};
+}
+
+inline bool grid_size_usable(const Grid &c, const Grid &r, double min_ratio) {
+    if (!(c.width >= r.width)) return false;
+    if (!(c.height >= r.height)) return false;
+    if (!(r.width >= min_ratio * c.width)) return false;
+    if (!(r.height >= min_ratio * c.height)) return false;
+    return true;
+}
+inline bool grid_covers(const Opt &o, const Grid &g, const Box &b) {
+    double hx = g.width / (double)(o.nx - 1);
+    double hy = g.height / (double)(o.ny - 1);
+    double margin = 1.5;
+    if (!(g.x0 + margin * hx <= b.xmin)) return false;
+    if (!(g.x0 + g.width - margin * hx >= b.xmax)) return false;
+    if (!(g.y0 + margin * hy <= b.ymin)) return false;
+    if (!(g.y0 + g.height - margin * hy >= b.ymax)) return false;
+    return true;
+}
+inline Grid expand_grid(const Grid &g, double factor) {
+    if (factor <= 1.0) return g;
+    double nw = g.width * factor, nh = g.height * factor;
+    return Grid{g.x0 - (nw - g.width) / 2.0, g.y0 - (nh - g.height) / 2.0, nw, nh};
+}
+
+// ------------------------------------------------------------------------------------------
+// Workspace, caches
+// ------------------------------------------------------------------------------------------
+struct FieldWS { Vec phi, Ex, Ey; };
+struct Plans {
+    std::unique_ptr<FFTPlan> p1, p2;
+    Plans(int P1, int P2) : p1(new FFTPlan(P1)), p2(new FFTPlan(P2)) {}
+};
+
+struct Workspace {
+    int nx, ny;
+    Vec charge;
+    std::vector<cd> spectral;
+    Vec green;
+    std::vector<Vec> local_charge;
+    FieldWS left, right, mid;
+    Vec q1, q2;
+    int64_t dropped = 0;
+    Workspace(int nx_, int ny_, int lnx, int lny) : nx(nx_), ny(ny_) {
+        size_t P = (size_t)4 * nx * ny;
+        charge.assign(P, 0.0);
+        spectral.assign(P, cd(0, 0));
+        green.assign(P, 0.0);
+        for (FieldWS *f : {&left, &right, &mid}) {
+            f->phi.assign((size_t)nx * ny, 0.0); f->Ex.assign((size_t)nx * ny, 0.0); f->Ey.assign((size_t)nx * ny, 0.0);
+        }
+        q1.assign((size_t)(lnx + 1) * (lny + 1), 0.0);
+        q2.assign((size_t)(lnx + 1) * (lny + 1), 0.0);
+    }
+};
+
+struct GreenEntry {
+    Grid sg, fg;
+    std::vector<cd> gfft;
+    int64_t uses = 0, rebuilds = 0;
+};
+
+struct GreenCache {
+    std::map<std::tuple<int, int, int>, GreenEntry> entries;
+    int64_t hits = 0, misses = 0, rebuilds = 0;
+    std::mutex lock;
+};
+
+struct NodeEntry { Grid sg, fg; std::vector<cd> gfft; };
+
+struct Trace {
+    std::vector<octopic_interaction_record> recs;   // 2 per pair, indexed by 2*p + dir - 1
+};
+
+// ------------------------------------------------------------------------------------------
+// Deposit (pic_cpu.jl:1992-2163)
+// ------------------------------------------------------------------------------------------
+constexpr int64_t PIC_DEPOSIT_CHUNKS = 16;
+inline bool deposit_parallel(int64_t n, int64_t nx, int64_t ny) { return n >= 4096 && n >= 160 * nx * ny; }
+
+// charge: column-major with leading dimension ld
+void deposit_range(double *charge, int64_t ld, int method, const double *x, const double *px, const double *y,
+                   const double *py, double s, bool drifted, double x0, double y0, double hx, double hy, int64_t nx,
+                   int64_t ny, int64_t first, int64_t last) {
+    double hxi = 1.0 / hx, hyi = 1.0 / hy;
+    for (int64_t i = first; i < last; ++i) {
+        double xd = drifted ? x[i] + px[i] * s : x[i];
+        double yd = drifted ? y[i] + py[i] * s : y[i];
+        int64_t ix, iy;
+        double wx[3], wy[3];
+        int kx = weights(method, (xd - x0) * hxi, nx, ix, wx);
+        int ky = weights(method, (yd - y0) * hyi, ny, iy, wy);
+        for (int m = 0; m < kx; ++m)
+            for (int q = 0; q < ky; ++q) charge[(ix - 1 + m) + (iy - 1 + q) * ld] += wx[m] * wy[q];
+    }
+}
+
+void deposit(double *charge, int64_t ld, int64_t ncols, int method, const double *x, const double *px,
+             const double *y, const double *py, double s, bool drifted, double x0, double y0, double hx, double hy,
+             int64_t nx, int64_t ny, int64_t n, std::vector<Vec> *local) {
+    if (!deposit_parallel(n, nx, ny)) {
+        deposit_range(charge, ld, method, x, px, y, py, s, drifted, x0, y0, hx, hy, nx, ny, 0, n);
+        return;
+    }
+    size_t sz = (size_t)ld * ncols;
+    std::vector<Vec> own;
+    std::vector<Vec> &loc = local ? *local : own;
+    loc.resize(PIC_DEPOSIT_CHUNKS);
+    for (int64_t c = 1; c <= PIC_DEPOSIT_CHUNKS; ++c) {
+        Vec &g = loc[c - 1];
+        g.assign(sz, 0.0);
+        int64_t first = fld((c - 1) * n, PIC_DEPOSIT_CHUNKS);   // 0-based start = first_i - 1
+        int64_t last = fld(c * n, PIC_DEPOSIT_CHUNKS);
+        deposit_range(g.data(), ld, method, x, px, y, py, s, drifted, x0, y0, hx, hy, nx, ny, first, last);
+    }
+    for (int64_t c = 0; c < PIC_DEPOSIT_CHUNKS; ++c)
+        for (size_t k = 0; k < sz; ++k) charge[k] += loc[c][k];
+}
+
+// ------------------------------------------------------------------------------------------
+// The collision
+// ------------------------------------------------------------------------------------------
+struct Coords { Vec x, px, y, py, z, pz; size_t size() const { return x.size(); } };
+struct SliceParam { double weight, lb, center, rb; };
+
+struct Context {
+    Opt o;
+    Plans plans;
+    std::vector<std::unique_ptr<Workspace>> ws;
+    GreenCache cache;
+    Trace *trace = nullptr;
+    Context(const Opt &o_) : o(o_), plans(2 * o_.nx, 2 * o_.ny) {}
+};
+
+std::vector<cd> green_fft(Context &C, Workspace &w, const Grid &sg, const Grid &fg) {  // pic_cpu.jl:1917-1939
+    const Opt &o = C.o;
+    double hx = sg.width / (double)(o.nx - 1);
+    double hy = sg.height / (double)(o.ny - 1);
+    green_table(w.green.data(), o.green_type, fg.x0, fg.y0, sg.x0, sg.y0, hx, hy, o.nx, o.ny);
+    std::vector<cd> g(w.green.size());
+    for (size_t k = 0; k < g.size(); ++k) g[k] = cd(w.green[k], 0.0);
+    fft2(g.data(), 2 * o.nx, 2 * o.ny, -1, *C.plans.p1, *C.plans.p2);
+    return g;
+}
+
+const std::vector<cd> *slice_pair_green(Context &C, Workspace &w, const std::tuple<int, int, int> *key, Grid &sg, Grid &fg,
+                                        const Box &sb, const Box &fb, std::vector<cd> &tmp, int &event) {  // pic_cpu.jl:1729-1778
+    const Opt &o = C.o;
+    event = 0;
+    if (o.green_cache == OCTOPIC_CACHE_SLICE_PAIR && key) {
+        bool usable = false;
+        bool have = false;
+        {
+            std::lock_guard<std::mutex> g(C.cache.lock);
+            auto it = C.cache.entries.find(*key);
+            have = it != C.cache.entries.end();
+            if (have) {
+                const GreenEntry &e = it->second;
+                usable = grid_size_usable(e.sg, sg, o.min_ratio) && grid_size_usable(e.fg, fg, o.min_ratio) &&
+                         grid_covers(o, e.sg, sb) && grid_covers(o, e.fg, fb);
+                if (usable) {
+                    C.cache.hits += 1;
+                    it->second.uses += 1;
+                    sg = e.sg; fg = e.fg;
+                    event = 1;
+                    return &it->second.gfft;
+                }
+            }
+        }
+        Grid es = expand_grid(sg, 1.0 + o.growth);
+        Grid ef = expand_grid(fg, 1.0 + o.growth);
+        if (o.green_type != OCTOPIC_GREEN_INTEGRATED) {   // _pic_realign_expanded_grids, pic_cpu.jl:1849-1858
+            double hx = es.width / (double)(o.nx - 1);
+            double hy = es.height / (double)(o.ny - 1);
+            double sx0, fx0, sy0, fy0;
+            align_origins(o.green_type, es.x0, ef.x0, hx, sx0, fx0);
+            align_origins(o.green_type, es.y0, ef.y0, hy, sy0, fy0);
+            es.x0 = sx0; es.y0 = sy0; ef.x0 = fx0; ef.y0 = fy0;
+        }
+        std::vector<cd> g = green_fft(C, w, es, ef);
+        std::lock_guard<std::mutex> gl(C.cache.lock);
+        GreenEntry &e = C.cache.entries[*key];
+        if (!have) {
+            C.cache.misses += 1;
+            e.rebuilds = 0;
+            event = 2;
+        } else {
+            C.cache.rebuilds += 1;
+            e.rebuilds += 1;
+            event = 3;
+        }
+        e.sg = es; e.fg = ef; e.gfft = std::move(g); e.uses = 1;
+        sg = es; fg = ef;
+        return &e.gfft;
+    }
+    tmp = green_fft(C, w, sg, fg);
+    return &tmp;
+}
+
+void solve_drifted_field(Context &C, Workspace &w, FieldWS &f, const Coords &src, double s, const Grid &sg,
+                         const std::vector<cd> &gfft) {  // pic_cpu.jl:1965-1990
+    const Opt &o = C.o;
+    int nx = o.nx, ny = o.ny;
+    int64_t P1 = 2 * nx, P2 = 2 * ny;
+    double hx = sg.width / (double)(nx - 1);
+    double hy = sg.height / (double)(ny - 1);
+    std::fill(w.charge.begin(), w.charge.end(), 0.0);
+    deposit(w.charge.data(), P1, P2, o.deposit, src.x.data(), src.px.data(), src.y.data(), src.py.data(), s, true,
+            sg.x0, sg.y0, hx, hy, nx, ny, (int64_t)src.size(), &w.local_charge);
+    for (size_t k = 0; k < w.charge.size(); ++k) w.spectral[k] = cd(w.charge[k], 0.0);
+    fft2(w.spectral.data(), (int)P1, (int)P2, -1, *C.plans.p1, *C.plans.p2);
+    for (size_t k = 0; k < w.spectral.size(); ++k) w.spectral[k] *= gfft[k];
+    fft2(w.spectral.data(), (int)P1, (int)P2, +1, *C.plans.p1, *C.plans.p2);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) f.phi[i + (size_t)j * nx] = w.spectral[i + (size_t)j * P1].real();
+    field_derivative(f.Ex.data(), f.Ey.data(), f.phi.data(), nx, ny, hx, hy, o.field_derivative == 4);
+}
+
+// _pic_interpolate_kick (pic_cpu.jl:2596-2619); planes are column-major nx x ny
+inline void interpolate_kick(const Opt &o, const Grid &g, double x, double y, const double *phiL, const double *ExL,
+                             const double *EyL, const double *phiR, const double *ExR, const double *EyR, double zL,
+                             double zR, double &Kx, double &Ky, double &Kz) {
+    int nx = o.nx, ny = o.ny;
+    double hx = g.width / (double)(nx - 1);
+    double hy = g.height / (double)(ny - 1);
+    int64_t ix, iy;
+    double wx[3], wy[3];
+    int kx = weights(o.deposit, (x - g.x0) / hx, nx, ix, wx);
+    int ky = weights(o.deposit, (y - g.y0) / hy, ny, iy, wy);
+    Kx = 0.0; Ky = 0.0; Kz = 0.0;
+    for (int m = 0; m < kx; ++m)
+        for (int q = 0; q < ky; ++q) {
+            size_t c = (size_t)(ix - 1 + m) + (size_t)(iy - 1 + q) * nx;
+            double wgt = wx[m] * wy[q];
+            Kx += wgt * (zL * ExL[c] + zR * ExR[c]);
+            Ky += wgt * (zL * EyL[c] + zR * EyR[c]);
+            Kz += wgt * (phiL[c] - phiR[c]);
+        }
+}
+inline void interpolate_kick(const Opt &o, const Grid &g, double x, double y, const FieldWS &L, const FieldWS &R,
+                             double zL, double zR, double &Kx, double &Ky, double &Kz) {
+    interpolate_kick(o, g, x, y, L.phi.data(), L.Ex.data(), L.Ey.data(), R.phi.data(), R.Ex.data(), R.Ey.data(), zL,
+                     zR, Kx, Ky, Kz);
+}
+
+// _pic_interpolate_kick_quadratic (pic_cpu.jl:2648-2680)
+inline void interpolate_kick_quadratic(const Opt &o, const Grid &g, double x, double y, const FieldWS &L,
+                                       const FieldWS &M, const FieldWS &R, double t, double &Kx, double &Ky,
+                                       double &Kz) {
+    int nx = o.nx, ny = o.ny;
+    double hx = g.width / (double)(nx - 1);
+    double hy = g.height / (double)(ny - 1);
+    int64_t ix, iy;
+    double wx[3], wy[3];
+    int kx = weights(o.deposit, (x - g.x0) / hx, nx, ix, wx);
+    int ky = weights(o.deposit, (y - g.y0) / hy, ny, iy, wy);
+    double t2 = t * t;
+    double aL = 2.0 * t2 - 3.0 * t + 1.0;
+    double aM = 4.0 * t - 4.0 * t2;
+    double aR = 2.0 * t2 - t;
+    double bL = 3.0 - 4.0 * t;
+    double bM = 8.0 * t - 4.0;
+    double bR = 1.0 - 4.0 * t;
+    Kx = 0.0; Ky = 0.0; Kz = 0.0;
+    for (int m = 0; m < kx; ++m)
+        for (int q = 0; q < ky; ++q) {
+            size_t c = (size_t)(ix - 1 + m) + (size_t)(iy - 1 + q) * nx;
+            double wgt = wx[m] * wy[q];
+            Kx += wgt * (aL * L.Ex[c] + aM * M.Ex[c] + aR * R.Ex[c]);
+            Ky += wgt * (aL * L.Ey[c] + aM * M.Ey[c] + aR * R.Ey[c]);
+            Kz += wgt * (bL * L.phi[c] + bM * M.phi[c] + bR * R.phi[c]);
+        }
+}
+
+inline void slice_interp_params(double lb, double rb, double &hzi, double &zbias) {  // pic_cpu.jl:1540-1548
+    double denom = rb - lb;
+    if (!std::isfinite(denom) || denom == 0.0) { hzi = 0.0; zbias = 0.5; return; }
+    hzi = 1.0 / denom;
+    zbias = rb * hzi;
+}
+
+inline void axis_extent(int method, double &lo, double &hi, double origin, double ds, double ds2, int64_t n, double k) {
+    if (method == OCTOPIC_EXTENT_EXTREMA) return;     // pic_cpu.jl:1581-1593
+    if (n < 2) return;
+    double invn = 1.0 / (double)n;
+    double mean_offset = ds * invn;
+    double m = origin + mean_offset;
+    double var = jmax(std::fma(-mean_offset, mean_offset, ds2 * invn), 0.0);   // muladd
+    double sd = std::sqrt(var);
+    double half = k * sd;
+    if (!(half > 0.0)) return;
+    lo = m - half;
+    hi = m + half;
+}
+
+int64_t count_outside_box(const Vec &xs, const Vec &ys, double xlo, double xhi, double ylo, double yhi) {
+    int64_t c = 0;
+    for (size_t i = 0; i < xs.size(); ++i) {
+        double x = xs[i], y = ys[i];
+        if (!(std::isfinite(x) && xlo <= x && x <= xhi && std::isfinite(y) && ylo <= y && y <= yhi)) c += 1;
+    }
+    return c;
+}
+int64_t count_outside_box_drifted(const Coords &s, double sL, double sR, double xlo, double xhi, double ylo, double yhi) {
+    int64_t c = 0;
+    for (size_t i = 0; i < s.size(); ++i) {
+        double xl = s.x[i] + s.px[i] * sL, yl = s.y[i] + s.py[i] * sL;
+        double xr = s.x[i] + s.px[i] * sR, yr = s.y[i] + s.py[i] * sR;
+        if (!(std::isfinite(xl) && xlo <= xl && xl <= xhi && std::isfinite(yl) && ylo <= yl && yl <= yhi &&
+              std::isfinite(xr) && xlo <= xr && xr <= xhi && std::isfinite(yr) && ylo <= yr && yr <= yhi))
+            c += 1;
+    }
+    return c;
+}
+
+[[noreturn]] void nonfinite(const char *role, const char *ctx) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "non-finite particle coordinate(s) detected in the %s particles of a strong-strong collision (%s)",
+             role, ctx);
+    throw OracleError(OCTOPIC_ERR_NONFINITE, buf);
+}
+
+struct UnionBounds { Box s, f; };
+
+// _pic_interaction! (pic_cpu.jl:954-1144).  Returns the virtual source positions (vx, vy).
+void interaction(Context &C, Workspace &w, const Coords &src, const SliceParam &ps, Coords &fld, const SliceParam &pf,
+                 double kbb, const std::tuple<int, int, int> *key, const UnionBounds *ov, Vec &vx, Vec &vy,
+                 octopic_interaction_record *rec) {
+    const Opt &o = C.o;
+    int64_t nsource = (int64_t)src.size(), nfield = (int64_t)fld.size();
+    double sL = 0.5 * (ps.center - pf.lb);
+    double sR = 0.5 * (ps.center - pf.rb);
+    double sxl = src.x[0] + src.px[0] * sL, syl = src.y[0] + src.py[0] * sL;
+    double sxr = src.x[0] + src.px[0] * sR, syr = src.y[0] + src.py[0] * sR;
+    double sxmin = jmin(sxl, sxr), sxmax = jmax(sxl, sxr), symin = jmin(syl, syr), symax = jmax(syl, syr);
+    int ge = o.extent;
+    double sx0 = sxl, sy0 = syl;
+    double sxs = 0, sxs2 = 0, sys = 0, sys2 = 0;
+    for (int64_t i = 0; i < nsource; ++i) {
+        double xl = src.x[i] + src.px[i] * sL, yl = src.y[i] + src.py[i] * sL;
+        double xr = src.x[i] + src.px[i] * sR, yr = src.y[i] + src.py[i] * sR;
+        sxmin = jmin(jmin(sxmin, xl), xr); sxmax = jmax(jmax(sxmax, xl), xr);
+        symin = jmin(jmin(symin, yl), yr); symax = jmax(jmax(symax, yl), yr);
+        if (ge != OCTOPIC_EXTENT_EXTREMA) {
+            double dxl = xl - sx0, dxr = xr - sx0, dyl = yl - sy0, dyr = yr - sy0;
+            sxs += dxl + dxr; sxs2 += dxl * dxl + dxr * dxr;
+            sys += dyl + dyr; sys2 += dyl * dyl + dyr * dyr;
+        }
+    }
+    double kext = o.extent_sigma;
+    axis_extent(ge, sxmin, sxmax, sx0, sxs, sxs2, 2 * nsource, kext);
+    axis_extent(ge, symin, symax, sy0, sys, sys2, 2 * nsource, kext);
+    if (!(std::isfinite(sxmin) && std::isfinite(sxmax) && std::isfinite(symin) && std::isfinite(symax)))
+        nonfinite("source", "slice pair");
+
+    double c = ps.center;
+    double fxmin = fld.x[0] + 0.5 * (fld.z[0] - c) * fld.px[0];
+    double fxmax = fxmin;
+    double fymin = fld.y[0] + 0.5 * (fld.z[0] - c) * fld.py[0];
+    double fymax = fymin;
+    double fx0 = fxmin, fy0 = fymin;
+    double fxs = 0, fxs2 = 0, fys = 0, fys2 = 0;
+    for (int64_t i = 0; i < nfield; ++i) {
+        double s = 0.5 * (fld.z[i] - c);
+        fld.x[i] += s * fld.px[i];
+        fld.y[i] += s * fld.py[i];
+        if (o.lkick) fld.pz[i] -= 0.25 * (fld.px[i] * fld.px[i] + fld.py[i] * fld.py[i]);
+        fxmin = jmin(fxmin, fld.x[i]); fxmax = jmax(fxmax, fld.x[i]);
+        fymin = jmin(fymin, fld.y[i]); fymax = jmax(fymax, fld.y[i]);
+        if (ge != OCTOPIC_EXTENT_EXTREMA) {
+            double dx = fld.x[i] - fx0, dy = fld.y[i] - fy0;
+            fxs += dx; fxs2 += dx * dx; fys += dy; fys2 += dy * dy;
+        }
+    }
+    axis_extent(ge, fxmin, fxmax, fx0, fxs, fxs2, nfield, kext);
+    axis_extent(ge, fymin, fymax, fy0, fys, fys2, nfield, kext);
+    if (!(std::isfinite(fxmin) && std::isfinite(fxmax) && std::isfinite(fymin) && std::isfinite(fymax)))
+        nonfinite("field", "slice pair");
+
+    Grid sg, fg;
+    if (!ov) interaction_grids(o, sxmin, sxmax, symin, symax, fxmin, fxmax, fymin, fymax, sg, fg);
+    else interaction_grids(o, ov->s.xmin, ov->s.xmax, ov->s.ymin, ov->s.ymax, ov->f.xmin, ov->f.xmax, ov->f.ymin,
+                           ov->f.ymax, sg, fg);
+    Box sb{sxmin, sxmax, symin, symax}, fb{fxmin, fxmax, fymin, fymax};
+    std::vector<cd> tmp;
+    int event = 0;
+    const std::vector<cd> *gfft = slice_pair_green(C, w, key, sg, fg, sb, fb, tmp, event);
+    if (rec) {
+        rec->cache_event = event;
+        rec->source_grid[0] = sg.x0; rec->source_grid[1] = sg.y0; rec->source_grid[2] = sg.width; rec->source_grid[3] = sg.height;
+        rec->field_grid[0] = fg.x0; rec->field_grid[1] = fg.y0; rec->field_grid[2] = fg.width; rec->field_grid[3] = fg.height;
+    }
+    if (ge != OCTOPIC_EXTENT_EXTREMA || o.igrid != OCTOPIC_GRID_SLICE_PAIR) {
+        w.dropped += count_outside_box(fld.x, fld.y, fg.x0, fg.x0 + fg.width, fg.y0, fg.y0 + fg.height);
+        w.dropped += count_outside_box_drifted(src, sL, sR, sg.x0, sg.x0 + sg.width, sg.y0, sg.y0 + sg.height);
+    }
+    solve_drifted_field(C, w, w.left, src, sL, sg, *gfft);
+    solve_drifted_field(C, w, w.right, src, sR, sg, *gfft);
+    double kick_scale = 2.0 * kbb;
+    double hzi, zbias;
+    slice_interp_params(pf.lb, pf.rb, hzi, zbias);
+    bool quad = o.interp == OCTOPIC_INTERP_QUADRATIC;
+    if (quad) {
+        double s_mid = 0.5 * (sL + sR);
+        solve_drifted_field(C, w, w.mid, src, s_mid, sg, *gfft);
+    }
+    for (int64_t i = 0; i < nfield; ++i) {
+        double zL = jclamp(-fld.z[i] * hzi + zbias, 0.0, 1.0);
+        double Kx, Ky, Kz;
+        if (quad) {
+            interpolate_kick_quadratic(o, fg, fld.x[i], fld.y[i], w.left, w.mid, w.right, 1.0 - zL, Kx, Ky, Kz);
+        } else {
+            double zR = 1.0 - zL;
+            interpolate_kick(o, fg, fld.x[i], fld.y[i], w.left, w.right, zL, zR, Kx, Ky, Kz);
+        }
+        fld.px[i] += kick_scale * Kx;
+        fld.py[i] += kick_scale * Ky;
+        if (o.lkick) fld.pz[i] += kick_scale * Kz * hzi;
+        double s = 0.5 * (c - fld.z[i]);
+        fld.x[i] += s * fld.px[i];
+        fld.y[i] += s * fld.py[i];
+        if (o.lkick) fld.pz[i] += 0.25 * (fld.px[i] * fld.px[i] + fld.py[i] * fld.py[i]);
+    }
+    double sM = 0.5 * (ps.center - pf.center);
+    vx.resize(nsource); vy.resize(nsource);
+    for (int64_t i = 0; i < nsource; ++i) {
+        vx[i] = src.x[i] + src.px[i] * sM;
+        vy[i] = src.y[i] + src.py[i] * sM;
+    }
+}
+
+// _pic_build_node_grids! (pic_cpu.jl:1317-1382): meshes for every node of one (source slice, dir).
+void build_node_grids(Context &C, Workspace &w, std::map<int, NodeEntry> &cache, const Coords &src, double center,
+                      const Coords &fld_rep, const Slices &fsl) {
+    const Opt &o = C.o;
+    int nb = (int)fsl.boundary.size();
+    int ns = nb - 1;
+    double c = center;
+    Vec drifts(nb);
+    for (int b = 0; b < nb; ++b) drifts[b] = 0.5 * (c - fsl.boundary[b]);
+    Vec sxlo(nb, INFINITY), sxhi(nb, -INFINITY), sylo(nb, INFINITY), syhi(nb, -INFINITY);
+    for (size_t i = 0; i < src.size(); ++i) {
+        double x = src.x[i], px = src.px[i], y = src.y[i], py = src.py[i];
+        for (int b = 0; b < nb; ++b) {
+            double d = drifts[b];
+            double xv = x + px * d, yv = y + py * d;
+            sxlo[b] = jmin(sxlo[b], xv); sxhi[b] = jmax(sxhi[b], xv);
+            sylo[b] = jmin(sylo[b], yv); syhi[b] = jmax(syhi[b], yv);
+        }
+    }
+    Vec fxlo(ns, INFINITY), fxhi(ns, -INFINITY), fylo(ns, INFINITY), fyhi(ns, -INFINITY);
+    for (int sl = 0; sl < ns; ++sl)
+        for (int64_t i : fsl.indices[sl]) {
+            double sh = 0.5 * (fld_rep.z[i] - c);
+            double xv = fld_rep.x[i] + sh * fld_rep.px[i];
+            double yv = fld_rep.y[i] + sh * fld_rep.py[i];
+            fxlo[sl] = jmin(fxlo[sl], xv); fxhi[sl] = jmax(fxhi[sl], xv);
+            fylo[sl] = jmin(fylo[sl], yv); fyhi[sl] = jmax(fyhi[sl], yv);
+        }
+    for (int b = 0; b < nb; ++b)
+        if (std::isnan(sxlo[b]) || std::isnan(sxhi[b]) || std::isnan(sylo[b]) || std::isnan(syhi[b]))
+            nonfinite("source", "node-mesh build");
+    for (int sl = 0; sl < ns; ++sl)
+        if (std::isnan(fxlo[sl]) || std::isnan(fxhi[sl]) || std::isnan(fylo[sl]) || std::isnan(fyhi[sl]))
+            nonfinite("field", "node-mesh build");
+    for (int b = 0; b < nb; ++b) {
+        int bn = std::min(b + 1, nb - 1);
+        double sxmin = jmin(sxlo[b], sxlo[bn]), sxmax = jmax(sxhi[b], sxhi[bn]);
+        double symin = jmin(sylo[b], sylo[bn]), symax = jmax(syhi[b], syhi[bn]);
+        double fxmin = INFINITY, fxmax = -INFINITY, fymin = INFINITY, fymax = -INFINITY;
+        for (int adj : {b - 1, b}) {
+            if (!(0 <= adj && adj < ns)) continue;
+            if (!std::isfinite(fxlo[adj])) continue;
+            fxmin = jmin(fxmin, fxlo[adj]); fxmax = jmax(fxmax, fxhi[adj]);
+            fymin = jmin(fymin, fylo[adj]); fymax = jmax(fymax, fyhi[adj]);
+        }
+        if (!std::isfinite(fxmin)) continue;
+        NodeEntry e;
+        interaction_grids(o, sxmin, sxmax, symin, symax, fxmin, fxmax, fymin, fymax, e.sg, e.fg);
+        e.gfft = green_fft(C, w, e.sg, e.fg);
+        cache[b] = std::move(e);
+    }
+}
+
+// _pic_interaction_node! (pic_cpu.jl:1448-1538)
+void interaction_node(Context &C, Workspace &w, const Coords &src, const SliceParam &ps, Coords &fld,
+                      const SliceParam &pf, double kbb, const NodeEntry &gL, const NodeEntry &gR, Vec &vx, Vec &vy,
+                      octopic_interaction_record *rec) {
+    const Opt &o = C.o;
+    int64_t nsource = (int64_t)src.size(), nfield = (int64_t)fld.size();
+    double c = ps.center;
+    double sL = 0.5 * (c - pf.lb), sR = 0.5 * (c - pf.rb);
+    for (int64_t i = 0; i < nfield; ++i) {
+        double s = 0.5 * (fld.z[i] - c);
+        fld.x[i] += s * fld.px[i];
+        fld.y[i] += s * fld.py[i];
+        if (o.lkick) fld.pz[i] -= 0.25 * (fld.px[i] * fld.px[i] + fld.py[i] * fld.py[i]);
+    }
+    w.dropped += count_outside_box_drifted(src, sL, sL, gL.sg.x0, gL.sg.x0 + gL.sg.width, gL.sg.y0, gL.sg.y0 + gL.sg.height);
+    w.dropped += count_outside_box_drifted(src, sR, sR, gR.sg.x0, gR.sg.x0 + gR.sg.width, gR.sg.y0, gR.sg.y0 + gR.sg.height);
+    w.dropped += count_outside_box(fld.x, fld.y, gL.fg.x0, gL.fg.x0 + gL.fg.width, gL.fg.y0, gL.fg.y0 + gL.fg.height);
+    if (rec) {
+        rec->cache_event = 0;
+        rec->source_grid[0] = gL.sg.x0; rec->source_grid[1] = gL.sg.y0; rec->source_grid[2] = gL.sg.width; rec->source_grid[3] = gL.sg.height;
+        rec->field_grid[0] = gL.fg.x0; rec->field_grid[1] = gL.fg.y0; rec->field_grid[2] = gL.fg.width; rec->field_grid[3] = gL.fg.height;
+    }
+    solve_drifted_field(C, w, w.left, src, sL, gL.sg, gL.gfft);
+    solve_drifted_field(C, w, w.right, src, sR, gR.sg, gR.gfft);
+    if (o.lkick) solve_drifted_field(C, w, w.mid, src, sR, gL.sg, gL.gfft);
+    double kick_scale = 2.0 * kbb;
+    double hzi, zbias;
+    slice_interp_params(pf.lb, pf.rb, hzi, zbias);
+    for (int64_t i = 0; i < nfield; ++i) {
+        double zL = jclamp(-fld.z[i] * hzi + zbias, 0.0, 1.0);
+        double zR = 1.0 - zL;
+        double x = fld.x[i], y = fld.y[i];
+        double KxL, KyL, KzL, KxR, KyR, KzR;
+        interpolate_kick(o, gL.fg, x, y, w.left, w.left, 1.0, 0.0, KxL, KyL, KzL);
+        interpolate_kick(o, gR.fg, x, y, w.right, w.right, 1.0, 0.0, KxR, KyR, KzR);
+        fld.px[i] += kick_scale * (zL * KxL + zR * KxR);
+        fld.py[i] += kick_scale * (zL * KyL + zR * KyR);
+        if (o.lkick) {
+            double a, b, Kz;
+            // phiL - phiZ on gL; the Z plane's field parts are the L plane's, as in the reference
+            interpolate_kick(o, gL.fg, x, y, w.left.phi.data(), w.left.Ex.data(), w.left.Ey.data(), w.mid.phi.data(),
+                             w.left.Ex.data(), w.left.Ey.data(), 1.0, 0.0, a, b, Kz);
+            fld.pz[i] += kick_scale * Kz * hzi;
+        }
+        double s = 0.5 * (c - fld.z[i]);
+        fld.x[i] += s * fld.px[i];
+        fld.y[i] += s * fld.py[i];
+        if (o.lkick) fld.pz[i] += 0.25 * (fld.px[i] * fld.px[i] + fld.py[i] * fld.py[i]);
+    }
+    double sM = 0.5 * (ps.center - pf.center);
+    vx.resize(nsource); vy.resize(nsource);
+    for (int64_t i = 0; i < nsource; ++i) {
+        vx[i] = src.x[i] + src.px[i] * sM;
+        vy[i] = src.y[i] + src.py[i] * sM;
+    }
+}
+
+// _pic_luminosity! (pic_cpu.jl:2700-2736)
+double luminosity(const Opt &o, const Vec &x1, const Vec &y1, const Vec &x2, const Vec &y2, double klum, Vec &q1, Vec &q2) {
+    int64_t nx = o.lum_nx, ny = o.lum_ny;
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    double a = INFINITY, b = -INFINITY;
+    for (double v : x1) { a = jmin(a, v); b = jmax(b, v); }
+    double c = INFINITY, d = -INFINITY;
+    for (double v : x2) { c = jmin(c, v); d = jmax(d, v); }
+    xmin = jmin(a, c); xmax = jmax(b, d);
+    a = INFINITY; b = -INFINITY;
+    for (double v : y1) { a = jmin(a, v); b = jmax(b, v); }
+    c = INFINITY; d = -INFINITY;
+    for (double v : y2) { c = jmin(c, v); d = jmax(d, v); }
+    ymin = jmin(a, c); ymax = jmax(b, d);
+    double width, height;
+    resolve_extent(o, xmax - xmin, ymax - ymin, width, height, "PIC luminosity grid");
+    double tx = width / ((double)nx - 1.1);
+    double ty = height / ((double)ny - 1.1);
+    width += 0.1 * tx;
+    height += 0.1 * ty;
+    xmin -= 0.05 * tx;
+    ymin -= 0.05 * ty;
+    double hx = width / (double)(nx - 1);
+    double hy = height / (double)(ny - 1);
+    double hxi = 1.0 / hx, hyi = 1.0 / hy;
+    std::fill(q1.begin(), q1.end(), 0.0);
+    std::fill(q2.begin(), q2.end(), 0.0);
+    deposit(q1.data(), nx + 1, ny + 1, o.lum_deposit, x1.data(), nullptr, y1.data(), nullptr, 0.0, false, xmin, ymin, hx,
+            hy, nx + 1, ny + 1, (int64_t)x1.size(), nullptr);
+    deposit(q2.data(), nx + 1, ny + 1, o.lum_deposit, x2.data(), nullptr, y2.data(), nullptr, 0.0, false, xmin, ymin, hx,
+            hy, nx + 1, ny + 1, (int64_t)x2.size(), nullptr);
+    double lum = 0.0;
+    for (int64_t j = 0; j < ny + 1; ++j)
+        for (int64_t i = 0; i < nx + 1; ++i) lum += q1[i + j * (nx + 1)] * q2[i + j * (nx + 1)];
+    return lum * klum * hxi * hyi;
+}
+
+Coords extract_slice(double *const b[6], const std::vector<int64_t> &idx) {
+    Coords c;
+    Vec *dst[6] = {&c.x, &c.px, &c.y, &c.py, &c.z, &c.pz};
+    for (int k = 0; k < 6; ++k) {
+        dst[k]->resize(idx.size());
+        for (size_t j = 0; j < idx.size(); ++j) (*dst[k])[j] = b[k][idx[j]];
+    }
+    return c;
+}
+void store_slice(double *const b[6], const std::vector<int64_t> &idx, const Coords &c) {
+    const Vec *src[6] = {&c.x, &c.px, &c.y, &c.py, &c.z, &c.pz};
+    for (int k = 0; k < 6; ++k)
+        for (size_t j = 0; j < idx.size(); ++j) b[k][idx[j]] = (*src[k])[j];
+}
+
+// _pic_union_bounds (pic_cpu.jl:665-716), the resident-state form
+bool union_bounds(const Coords &src, double center, const std::vector<Coords> &states, UnionBounds &out) {
+    double c = center;
+    double zmin = INFINITY, zmax = -INFINITY;
+    double fxmin = INFINITY, fxmax = -INFINITY, fymin = INFINITY, fymax = -INFINITY;
+    for (const Coords &fc : states)
+        for (size_t p = 0; p < fc.size(); ++p) {
+            double zi = fc.z[p];
+            zmin = jmin(zmin, zi); zmax = jmax(zmax, zi);
+            double s = 0.5 * (zi - c);
+            double xv = fc.x[p] + s * fc.px[p], yv = fc.y[p] + s * fc.py[p];
+            fxmin = jmin(fxmin, xv); fxmax = jmax(fxmax, xv); fymin = jmin(fymin, yv); fymax = jmax(fymax, yv);
+        }
+    if (std::isnan(zmin) || !(std::isfinite(fxmin) && std::isfinite(fxmax) && std::isfinite(fymin) && std::isfinite(fymax))) {
+        if (std::isnan(zmin) || std::isfinite(zmin)) nonfinite("field", "source-slice union bounds");
+    }
+    if (!std::isfinite(zmin)) return false;
+    double sA = 0.5 * (c - zmax), sB = 0.5 * (c - zmin);
+    double sxmin = INFINITY, sxmax = -INFINITY, symin = INFINITY, symax = -INFINITY;
+    for (size_t i = 0; i < src.size(); ++i) {
+        double xa = src.x[i] + src.px[i] * sA, xb = src.x[i] + src.px[i] * sB;
+        double ya = src.y[i] + src.py[i] * sA, yb = src.y[i] + src.py[i] * sB;
+        sxmin = jmin(jmin(sxmin, xa), xb); sxmax = jmax(jmax(sxmax, xa), xb);
+        symin = jmin(jmin(symin, ya), yb); symax = jmax(jmax(symax, ya), yb);
+    }
+    if (!(std::isfinite(sxmin) && std::isfinite(sxmax) && std::isfinite(symin) && std::isfinite(symax)))
+        nonfinite("source", "source-slice union bounds");
+    out.s = {sxmin, sxmax, symin, symax};
+    out.f = {fxmin, fxmax, fymin, fymax};
+    return true;
+}
+
+struct CollideOut {
+    double lum = NAN;
+    std::vector<int32_t> pi, pj, ran;
+    Vec plum;
+    int64_t dropped = 0, batches = 0, widest = 0;
+};
+
+struct State {
+    std::unique_ptr<Context> C;
+    std::vector<octopic_interaction_record> records;
+    Slices last1, last2;
+    CollideOut last;
+};
+
+double pic_kbb_phys(const Opt &, double given, const double p_self[5], const double p_other[5], bool first) {
+    // _strong_strong_kbb1/2 (slicing.jl:77-91); params = charge, mc2, E0, r0, npart
+    if (!std::isnan(given)) return given;
+    double E0 = p_self[2];
+    if (!(std::isfinite(E0) && E0 != 0.0))
+        throw OracleError(OCTOPIC_ERR_INVALID_ARGUMENT, first ? "beam1 E0 must be finite and nonzero when kbb1 is not specified"
+                                                              : "beam2 E0 must be finite and nonzero when kbb2 is not specified");
+    const double *p1 = first ? p_self : p_other;
+    const double *p2 = first ? p_other : p_self;
+    return p1[0] * p2[0] * p_self[3] * p_other[4] * p_self[1] / p_self[2];
+}
+
+void collide(State &S, double *b1[6], int64_t n1, const double p1[5], double *b2[6], int64_t n2, const double p2[5],
+             int compute_lum, int nthreads) {
+    Context &C = *S.C;
+    const Opt &o = C.o;
+    if (n1 <= 0 || n2 <= 0) throw OracleError(OCTOPIC_ERR_INVALID_ARGUMENT, "longitudinal slicing requires at least one particle");
+    Slices s1 = longitudinal_slices(b1[4], n1, o.s1);
+    Slices s2 = longitudinal_slices(b2[4], n2, o.s2);
+    double kbb1 = pic_kbb_phys(o, o.kbb1, p1, p2, true) / (double)n2;   // pic_cpu.jl:821-824
+    double kbb2 = pic_kbb_phys(o, o.kbb2, p2, p1, false) / (double)n1;
+    double klum = !std::isnan(o.lum_scale) ? o.lum_scale : p1[4] * p2[4] / (double)(n1 * n2);
+    int ns1 = (int)s1.center.size(), ns2 = (int)s2.center.size();
+    int pool = std::max(1, std::min(nthreads, std::min(ns1, ns2)));
+    bool node = o.igrid == OCTOPIC_GRID_NODE, share = o.igrid == OCTOPIC_GRID_SOURCE_SLICE;
+    while ((int)C.ws.size() < pool) C.ws.emplace_back(new Workspace(o.nx, o.ny, o.lum_nx, o.lum_ny));
+    for (auto &w : C.ws) w->dropped = 0;
+
+    std::map<std::pair<int, int>, std::map<int, NodeEntry>> node_cache;
+    if (node) {   // _pic_prebuild_node_caches! (pic_cpu.jl:1404-1419)
+        Coords rep1{Vec(b1[0], b1[0] + n1), Vec(b1[1], b1[1] + n1), Vec(b1[2], b1[2] + n1), Vec(b1[3], b1[3] + n1),
+                    Vec(b1[4], b1[4] + n1), Vec(b1[5], b1[5] + n1)};
+        Coords rep2{Vec(b2[0], b2[0] + n2), Vec(b2[1], b2[1] + n2), Vec(b2[2], b2[2] + n2), Vec(b2[3], b2[3] + n2),
+                    Vec(b2[4], b2[4] + n2), Vec(b2[5], b2[5] + n2)};
+        for (int i = 0; i < ns1; ++i) {
+            if (s1.indices[i].empty()) continue;
+            Coords src = extract_slice(b1, s1.indices[i]);
+            build_node_grids(C, *C.ws[0], node_cache[{i, 1}], src, s1.center[i], rep2, s2);
+        }
+        for (int j = 0; j < ns2; ++j) {
+            if (s2.indices[j].empty()) continue;
+            Coords src = extract_slice(b2, s2.indices[j]);
+            build_node_grids(C, *C.ws[0], node_cache[{j, 2}], src, s2.center[j], rep1, s1);
+        }
+    }
+    auto order = collision_order(s1.center, s2.center);
+    int npairs = (int)order.size();
+    std::map<std::pair<int, int>, int> pair_pos;
+    for (int p = 0; p < npairs; ++p) pair_pos[{order[p].i, order[p].j}] = p;
+    Vec lum_parts(npairs, 0.0);
+    std::vector<int32_t> ran(npairs, 0);
+    std::vector<Coords> st1(ns1), st2(ns2);
+    for (int i = 0; i < ns1; ++i) st1[i] = extract_slice(b1, s1.indices[i]);
+    for (int j = 0; j < ns2; ++j) st2[j] = extract_slice(b2, s2.indices[j]);
+    S.records.assign((size_t)2 * npairs, octopic_interaction_record{});
+    std::map<std::pair<int, int>, UnionBounds> ub;
+    std::map<std::pair<int, int>, bool> ub_have;
+
+    auto pair_fn = [&](int p, int i, int j, Workspace &w) -> bool {   // _pic_collide_pair! (pic_cpu.jl:292-392)
+        if (s1.indices[i].empty() || s2.indices[j].empty()) return false;
+        SliceParam q1{s1.weight[i], s1.boundary[i], s1.center[i], s1.boundary[i + 1]};
+        SliceParam q2{s2.weight[j], s2.boundary[j], s2.center[j], s2.boundary[j + 1]};
+        Coords &coord1 = st1[i];
+        Coords field2 = st2[j];
+        Vec vx1, vy1, vx2, vy2;
+        octopic_interaction_record *r1 = &S.records[(size_t)2 * p], *r2 = &S.records[(size_t)2 * p + 1];
+        r1->p = p; r1->i = i; r1->j = j; r1->dir = 1;
+        r2->p = p; r2->i = i; r2->j = j; r2->dir = 2;
+        if (node) {
+            auto it1 = node_cache.find({i, 1}), it2 = node_cache.find({j, 2});
+            if (it1 == node_cache.end() || it2 == node_cache.end() || it1->second.empty() || it2->second.empty())
+                throw OracleError(OCTOPIC_ERR_INVALID_ARGUMENT, "PIC node mesh cache is missing or empty at collision time");
+            auto &nc1 = it1->second;
+            auto &nc2 = it2->second;
+            auto gL1 = nc1.find(j), gR1 = nc1.find(j + 1), gL2 = nc2.find(i), gR2 = nc2.find(i + 1);
+            if (gL1 == nc1.end() || gR1 == nc1.end() || gL2 == nc2.end() || gR2 == nc2.end()) return false;
+            interaction_node(C, w, coord1, q1, field2, q2, kbb2, gL1->second, gR1->second, vx1, vy1, r1);
+            interaction_node(C, w, st2[j], q2, coord1, q1, kbb1, gL2->second, gR2->second, vx2, vy2, r2);
+        } else {
+            const UnionBounds *ov1 = nullptr, *ov2 = nullptr;
+            UnionBounds u1, u2;
+            if (share) {
+                auto k1 = std::make_pair(i, 1), k2 = std::make_pair(j, 2);
+                if (!ub_have.count(k1)) { ub_have[k1] = union_bounds(coord1, q1.center, st2, ub[k1]); }
+                if (!ub_have.count(k2)) { ub_have[k2] = union_bounds(st2[j], q2.center, st1, ub[k2]); }
+                if (ub_have[k1]) { u1 = ub[k1]; ov1 = &u1; }
+                if (ub_have[k2]) { u2 = ub[k2]; ov2 = &u2; }
+            }
+            auto key1 = share ? std::make_tuple(i, -1, 1) : std::make_tuple(i, j, 1);
+            auto key2 = share ? std::make_tuple(j, -1, 2) : std::make_tuple(i, j, 2);
+            interaction(C, w, coord1, q1, field2, q2, kbb2, &key1, ov1, vx1, vy1, r1);
+            interaction(C, w, st2[j], q2, coord1, q1, kbb1, &key2, ov2, vx2, vy2, r2);
+        }
+        st2[j] = std::move(field2);
+        if (compute_lum) lum_parts[p] = luminosity(o, vx1, vy1, vx2, vy2, klum, w.q1, w.q2);
+        return true;
+    };
+
+    S.last.batches = 0; S.last.widest = 0;
+    bool batchable = !share;
+    if (batchable && pool > 1 && npairs > 1) {
+        auto batches = collision_pair_batches(s1.center, s2.center);
+        S.last.batches = (int64_t)batches.size();
+        for (auto &b : batches) S.last.widest = std::max<int64_t>(S.last.widest, (int64_t)b.size());
+        for (auto &batch : batches) {
+            int nb = (int)batch.size();
+            std::vector<std::string> errs(nb);
+            std::vector<int> codes(nb, 0);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(std::min(pool, nb)) schedule(static, 1)
+#endif
+            for (int k = 0; k < nb; ++k) {
+                int tid = 0;
+#ifdef _OPENMP
+                tid = omp_get_thread_num();
+#endif
+                try {
+                    int p = pair_pos.at({batch[k].i, batch[k].j});
+                    ran[p] = pair_fn(p, batch[k].i, batch[k].j, *C.ws[tid]) ? 1 : 0;
+                } catch (OracleError &e) {
+                    codes[k] = e.code; errs[k] = e.what();
+                }
+            }
+            for (int k = 0; k < nb; ++k)
+                if (codes[k]) throw OracleError(codes[k], errs[k]);
+        }
+    } else {
+        for (int p = 0; p < npairs; ++p) ran[p] = pair_fn(p, order[p].i, order[p].j, *C.ws[0]) ? 1 : 0;
+    }
+    for (int i = 0; i < ns1; ++i) if (!s1.indices[i].empty()) store_slice(b1, s1.indices[i], st1[i]);
+    for (int j = 0; j < ns2; ++j) if (!s2.indices[j].empty()) store_slice(b2, s2.indices[j], st2[j]);
+    double lum = NAN;
+    if (compute_lum) {
+        lum = 0.0;
+        for (int p = 0; p < npairs; ++p) if (ran[p]) lum += lum_parts[p];
+    }
+    S.last.lum = lum;
+    S.last.pi.resize(npairs); S.last.pj.resize(npairs); S.last.ran = ran; S.last.plum = lum_parts;
+    for (int p = 0; p < npairs; ++p) { S.last.pi[p] = order[p].i; S.last.pj[p] = order[p].j; }
+    S.last.dropped = 0;
+    for (auto &w : C.ws) S.last.dropped += w->dropped;
+    S.last1 = std::move(s1);
+    S.last2 = std::move(s2);
+}
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        return OCTOPIC_OK;
+    } catch (OracleError &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (std::exception &e) {
+        g_last_error = e.what();
+        return OCTOPIC_ERR_INVALID_ARGUMENT;
+    }
+}
+
+}  // namespace
+
+// ==========================================================================================
+// C API of the oracle (ctypes; test infrastructure)
+// ==========================================================================================
+extern "C" {
+
+const char *oracle_last_error(void) { return g_last_error.c_str(); }
+
+int oracle_philox_block(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    philox_block(c, key[0], key[1]);
+    for (int k = 0; k < 4; ++k) out[k] = c[k];
+    return 0;
+}
+uint64_t oracle_counter_uint64(uint64_t seed, uint64_t turn, uint64_t rng_id, uint64_t particle, uint64_t component) {
+    return counter_uint64(seed, turn, rng_id, particle, component);
+}
+double oracle_octopus_normal(uint64_t seed, uint64_t turn, uint64_t rng_id, uint64_t particle, int64_t component) {
+    return octopus_normal(seed, turn, rng_id, particle, component);
+}
+
+// Beam(N, ...; beta, alpha, sigma, cutoff, rng_id) with the counter RNG (Beam.jl:127-181, 466-515, 545-553).
+// Particle indices are 1-based in the counter, as in the reference.
+int oracle_generate_beam(int64_t N, uint64_t seed, uint64_t rng_id, const double beta[3], const double alpha[3],
+                         const double sigma[3], double cutoff, double *out[6]) {
+    return guarded([&] {
+        std::vector<Vec> c(6, Vec(N));
+        for (int d = 0; d < 6; ++d) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+            for (int64_t i = 0; i < N; ++i) {
+                double x = octopus_normal(seed, 0, rng_id, (uint64_t)(i + 1), d + 1);
+                c[d][i] = std::isfinite(cutoff) ? jclamp(x, -cutoff, cutoff) : x;
+            }
+            standardize(c[d]);
+        }
+        for (int k = 0; k < 3; ++k) {
+            Vec &q = c[2 * k], &p = c[2 * k + 1];
+            double f = sigma[k] / beta[k];
+            for (int64_t i = 0; i < N; ++i) p[i] = f * (p[i] - alpha[k] * q[i]);
+            for (int64_t i = 0; i < N; ++i) q[i] *= sigma[k];
+        }
+        for (int d = 0; d < 6; ++d) std::memcpy(out[d], c[d].data(), sizeof(double) * N);
+    });
+}
+
+double oracle_erfinv(double y) { return erfinv(y); }
+
+int oracle_weights(int32_t method, double u, int64_t n, int64_t *base, double w[3]) {
+    int64_t b;
+    weights(method, u, n, b, w);
+    *base = b;
+    return 0;
+}
+double oracle_kernel_integral(double x, double y) { return kernel_integral(x, y); }
+double oracle_atan_ratio(double n, double d) { return atan_ratio(n, d); }
+
+int oracle_green_table(int32_t gtype, double fx0, double fy0, double sx0, double sy0, double hx, double hy, int32_t nx,
+                       int32_t ny, double *out) {
+    green_table(out, gtype, fx0, fy0, sx0, sy0, hx, hy, nx, ny);
+    return 0;
+}
+
+// Column-major n1 x n2 complex (interleaved re,im) in-place 2D DFT.
+int oracle_fft2(double *data, int32_t n1, int32_t n2, int32_t sign) {
+    FFTPlan p1(n1), p2(n2);
+    fft2(reinterpret_cast<cd *>(data), n1, n2, sign < 0 ? -1 : +1, p1, p2);
+    return 0;
+}
+
+int oracle_field(const double *phi, int32_t nx, int32_t ny, double hx, double hy, int32_t fourth, double *Ex, double *Ey) {
+    field_derivative(Ex, Ey, phi, nx, ny, hx, hy, fourth != 0);
+    return 0;
+}
+
+int oracle_interaction_grids(const octopic_options *opt, const double b[8], double sg[4], double fg[4]) {
+    return guarded([&] {
+        Opt o = to_opt(*opt);
+        Grid s, f;
+        interaction_grids(o, b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7], s, f);
+        sg[0] = s.x0; sg[1] = s.y0; sg[2] = s.width; sg[3] = s.height;
+        fg[0] = f.x0; fg[1] = f.y0; fg[2] = f.width; fg[3] = f.height;
+    });
+}
+
+// Undrifted deposit onto an ld x ncols column-major grid (_pic_deposit!, pic_cpu.jl:1992-2013).
+int oracle_deposit(double *charge, int64_t ld, int64_t ncols, int32_t method, const double *x, const double *y, int64_t n,
+                   double x0, double y0, double hx, double hy, int64_t nx, int64_t ny) {
+    deposit(charge, ld, ncols, method, x, nullptr, y, nullptr, 0.0, false, x0, y0, hx, hy, nx, ny, n, nullptr);
+    return 0;
+}
+
+// Field solve of one plane on a given source/field grid pair (_pic_solve_field, pic_cpu.jl:1907-1915):
+// deposit (x, y) + s*(px, py) on sg, convolve with the Green function of (sg, fg); phi/Ex/Ey nx x ny column-major.
+int oracle_solve_field(const octopic_options *opt, const double *x, const double *px, const double *y, const double *py,
+                       int64_t n, double s, const double sg_[4], const double fg_[4], double *phi, double *Ex, double *Ey,
+                       double *charge_out) {
+    return guarded([&] {
+        Opt o = to_opt(*opt);
+        Context C(o);
+        Workspace w(o.nx, o.ny, o.lum_nx, o.lum_ny);
+        Grid sg{sg_[0], sg_[1], sg_[2], sg_[3]}, fg{fg_[0], fg_[1], fg_[2], fg_[3]};
+        std::vector<cd> g = green_fft(C, w, sg, fg);
+        Coords src{Vec(x, x + n), Vec(px, px + n), Vec(y, y + n), Vec(py, py + n), Vec(n, 0.0), Vec(n, 0.0)};
+        solve_drifted_field(C, w, w.left, src, s, sg, g);
+        std::memcpy(phi, w.left.phi.data(), sizeof(double) * o.nx * o.ny);
+        std::memcpy(Ex, w.left.Ex.data(), sizeof(double) * o.nx * o.ny);
+        std::memcpy(Ey, w.left.Ey.data(), sizeof(double) * o.nx * o.ny);
+        if (charge_out) std::memcpy(charge_out, w.charge.data(), sizeof(double) * 4 * o.nx * o.ny);
+    });
+}
+
+int oracle_interpolate_kick(const octopic_options *opt, const double grid[4], double x, double y, const double *phiL,
+                            const double *ExL, const double *EyL, const double *phiR, const double *ExR,
+                            const double *EyR, double zL, double zR, double K[3]) {
+    return guarded([&] {
+        Opt o = to_opt(*opt);
+        size_t m = (size_t)o.nx * o.ny;
+        FieldWS L{Vec(phiL, phiL + m), Vec(ExL, ExL + m), Vec(EyL, EyL + m)};
+        FieldWS R{Vec(phiR, phiR + m), Vec(ExR, ExR + m), Vec(EyR, EyR + m)};
+        interpolate_kick(o, Grid{grid[0], grid[1], grid[2], grid[3]}, x, y, L, R, zL, zR, K[0], K[1], K[2]);
+    });
+}
+
+int oracle_interpolate_kick_quadratic(const octopic_options *opt, const double grid[4], double x, double y,
+                                      const double *const planes[9], double t, double K[3]) {
+    return guarded([&] {
+        Opt o = to_opt(*opt);
+        size_t m = (size_t)o.nx * o.ny;
+        FieldWS L{Vec(planes[0], planes[0] + m), Vec(planes[1], planes[1] + m), Vec(planes[2], planes[2] + m)};
+        FieldWS M{Vec(planes[3], planes[3] + m), Vec(planes[4], planes[4] + m), Vec(planes[5], planes[5] + m)};
+        FieldWS R{Vec(planes[6], planes[6] + m), Vec(planes[7], planes[7] + m), Vec(planes[8], planes[8] + m)};
+        interpolate_kick_quadratic(o, Grid{grid[0], grid[1], grid[2], grid[3]}, x, y, L, M, R, t, K[0], K[1], K[2]);
+    });
+}
+
+int oracle_luminosity(const octopic_options *opt, const double *x1, const double *y1, int64_t n1, const double *x2,
+                      const double *y2, int64_t n2, double klum, double *lum, double *q1_out, double *q2_out) {
+    return guarded([&] {
+        Opt o = to_opt(*opt);
+        Vec q1((size_t)(o.lum_nx + 1) * (o.lum_ny + 1)), q2(q1.size());
+        *lum = luminosity(o, Vec(x1, x1 + n1), Vec(y1, y1 + n1), Vec(x2, x2 + n2), Vec(y2, y2 + n2), klum, q1, q2);
+        if (q1_out) std::memcpy(q1_out, q1.data(), sizeof(double) * q1.size());
+        if (q2_out) std::memcpy(q2_out, q2.data(), sizeof(double) * q2.size());
+    });
+}
+
+// longitudinal_slices: boundaries[ns+1], centers[ns], weights[ns], counts[ns], indices[n] (concatenated, 0-based)
+int oracle_slices(const double *z, int64_t n, const octopic_slicing *s, double *boundaries, double *centers,
+                  double *weights_out, int64_t *counts, int64_t *indices) {
+    return guarded([&] {
+        Slicing sl = to_slicing(*s);
+        validate_slicing(sl);
+        Slices r = longitudinal_slices(z, n, sl);
+        std::memcpy(boundaries, r.boundary.data(), sizeof(double) * r.boundary.size());
+        std::memcpy(centers, r.center.data(), sizeof(double) * r.center.size());
+        std::memcpy(weights_out, r.weight.data(), sizeof(double) * r.weight.size());
+        int64_t off = 0;
+        for (size_t k = 0; k < r.indices.size(); ++k) {
+            counts[k] = (int64_t)r.indices[k].size();
+            for (int64_t v : r.indices[k]) indices[off++] = v;
+        }
+    });
+}
+
+// collision_pair_batches: out_i/out_j/out_batch have ns1*ns2 entries (0-based); returns number of batches
+int oracle_collision_batches(const double *c1, int32_t ns1, const double *c2, int32_t ns2, int32_t *out_i, int32_t *out_j,
+                             int32_t *out_batch, int32_t *nbatches) {
+    return guarded([&] {
+        auto b = collision_pair_batches(Vec(c1, c1 + ns1), Vec(c2, c2 + ns2));
+        int k = 0;
+        for (int q = 0; q < (int)b.size(); ++q)
+            for (auto &e : b[q]) { out_i[k] = e.i; out_j[k] = e.j; out_batch[k] = q; ++k; }
+        *nbatches = (int32_t)b.size();
+    });
+}
+int oracle_collision_order(const double *c1, int32_t ns1, const double *c2, int32_t ns2, int32_t *out_i, int32_t *out_j) {
+    auto o = collision_order(Vec(c1, c1 + ns1), Vec(c2, c2 + ns2));
+    for (size_t k = 0; k < o.size(); ++k) { out_i[k] = o[k].i; out_j[k] = o[k].j; }
+    return 0;
+}
+
+// Persistent collider state = the task's runtime cache (Green cache + workspaces).
+int oracle_create(const octopic_options *opt, void **out) {
+    *out = nullptr;
+    return guarded([&] {
+        Opt o = to_opt(*opt);
+        State *S = new State();
+        S->C.reset(new Context(o));
+        *out = S;
+    });
+}
+void oracle_destroy(void *s) { delete static_cast<State *>(s); }
+
+int oracle_collide(void *s, double *b1[6], int64_t n1, const double p1[5], double *b2[6], int64_t n2, const double p2[5],
+                   int32_t compute_lum, int32_t nthreads, double *lum) {
+    return guarded([&] {
+        State &S = *static_cast<State *>(s);
+        collide(S, b1, n1, p1, b2, n2, p2, compute_lum, nthreads);
+        *lum = S.last.lum;
+    });
+}
+
+int oracle_stats(void *s, int64_t out[6]) {
+    State &S = *static_cast<State *>(s);
+    out[0] = S.C->cache.hits; out[1] = S.C->cache.misses; out[2] = S.C->cache.rebuilds;
+    out[3] = S.last.dropped; out[4] = S.last.batches; out[5] = S.last.widest;
+    return 0;
+}
+
+int64_t oracle_npairs(void *s) { return (int64_t)static_cast<State *>(s)->last.pi.size(); }
+
+int oracle_pair_luminosity(void *s, int32_t *i, int32_t *j, int32_t *ran, double *lum) {
+    State &S = *static_cast<State *>(s);
+    for (size_t k = 0; k < S.last.pi.size(); ++k) { i[k] = S.last.pi[k]; j[k] = S.last.pj[k]; ran[k] = S.last.ran[k]; lum[k] = S.last.plum[k]; }
+    return 0;
+}
+
+int oracle_records(void *s, octopic_interaction_record *out, int64_t cap, int64_t *n) {
+    State &S = *static_cast<State *>(s);
+    *n = (int64_t)S.records.size();
+    for (int64_t k = 0; k < std::min<int64_t>(cap, *n); ++k) out[k] = S.records[k];
+    return 0;
+}
+
+int oracle_last_slices(void *s, int32_t which, double *boundaries, double *centers, double *weights_out, int64_t *counts,
+                       int64_t *indices) {
+    State &S = *static_cast<State *>(s);
+    const Slices &r = which == 1 ? S.last1 : S.last2;
+    std::memcpy(boundaries, r.boundary.data(), sizeof(double) * r.boundary.size());
+    std::memcpy(centers, r.center.data(), sizeof(double) * r.center.size());
+    std::memcpy(weights_out, r.weight.data(), sizeof(double) * r.weight.size());
+    int64_t off = 0;
+    for (size_t k = 0; k < r.indices.size(); ++k) {
+        counts[k] = (int64_t)r.indices[k].size();
+        for (int64_t v : r.indices[k]) indices[off++] = v;
+    }
+    return 0;
+}
+
+// Linear6D from optics with zero coupling/dispersion (linear6d.jl:233-276); out is row-major 6x6.
+int oracle_linear6d_matrix(const double beta1[3], const double beta2[3], const double alpha1[3], const double alpha2[3],
+                           const double dmu[3], double out[36]) {
+    for (int k = 0; k < 36; ++k) out[k] = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        double s = std::sin(dmu[i]), c = std::cos(dmu[i]);
+        int r = 2 * i;
+        out[r * 6 + r] = std::sqrt(beta2[i] / beta1[i]) * (c + alpha1[i] * s);
+        out[r * 6 + r + 1] = std::sqrt(beta2[i] * beta1[i]) * s;
+        out[(r + 1) * 6 + r] = (-(1.0 + alpha1[i] * alpha2[i]) * s + (alpha1[i] - alpha2[i]) * c) / std::sqrt(beta1[i] * beta2[i]);
+        out[(r + 1) * 6 + r + 1] = std::sqrt(beta1[i] / beta2[i]) * (c - alpha2[i] * s);
+    }
+    return 0;
+}
+
+// track_particle(::Symplectic6DMap, ::Linear6D, ...) (linear6d.jl:110-118)
+int oracle_track_linear6d(const double m[36], double *b[6], int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        double v[6] = {b[0][i], b[1][i], b[2][i], b[3][i], b[4][i], b[5][i]};
+        for (int r = 0; r < 6; ++r) {
+            const double *q = m + 6 * r;
+            b[r][i] = q[0] * v[0] + q[1] * v[1] + q[2] * v[2] + q[3] * v[3] + q[4] * v[4] + q[5] * v[5];
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"
