@@ -108,6 +108,8 @@ struct Inter {
     int32_t event;                 // 0 none, 1 hit, 2 miss, 3 rebuild
     int32_t valid;
     Grid fgR;                      // node mode: the right node's field mesh
+    int32_t count_field;           // 1: count field particles outside fg after the forward drift
+    int32_t reserved2;
 };
 
 // One deposited plane: a source slice drifted by s onto a mesh.
@@ -117,12 +119,13 @@ struct Plane {
     int32_t beam;                  // source beam
     int32_t n1, n2;                // nodes along x and y of the deposit mesh
     int32_t slot;                  // Green spectrum slot (field planes)
-    int32_t build;                 // unused for planes
+    int32_t count_box;             // 1: count particles whose drifted position leaves [x0, bx1] x [y0, by1]
     int32_t reserved;
     int64_t off, n;                // particle range in the slice-sorted state
     double s;                      // drift
     double x0, y0, hxi, hyi;       // deposit mesh origin and inverse spacing
     double hx, hy;                 // spacing (field derivative)
+    double bx1, by1;               // x0 + width, y0 + height (dropped-particle box)
 };
 
 // A Green table to build into a cache slot.

@@ -176,15 +176,16 @@ __device__ __forceinline__ void fill_record(octopic_interaction_record &r, int p
     r.field_grid[0] = fg.x0; r.field_grid[1] = fg.y0; r.field_grid[2] = fg.width; r.field_grid[3] = fg.height;
 }
 
-__device__ __forceinline__ void set_plane(Plane &pl, int inter, int kind, int beam, int n1, int n2, int slot, int64_t off,
-                                          int64_t n, double s, double x0, double y0, double hx, double hy) {
-    pl.inter = inter; pl.kind = kind; pl.beam = beam; pl.n1 = n1; pl.n2 = n2; pl.slot = slot; pl.build = 0;
+__device__ void set_plane(Plane &pl, int inter, int kind, int beam, int n1, int n2, int slot, int64_t off, int64_t n,
+                          double s, double x0, double y0, double hx, double hy) {
+    pl.inter = inter; pl.kind = kind; pl.beam = beam; pl.n1 = n1; pl.n2 = n2; pl.slot = slot; pl.count_box = 0;
     pl.reserved = 0; pl.off = off; pl.n = n; pl.s = s; pl.x0 = x0; pl.y0 = y0;
     pl.hx = hx; pl.hy = hy; pl.hxi = 1.0 / hx; pl.hyi = 1.0 / hy;
+    pl.bx1 = 0.0; pl.by1 = 0.0;
 }
 
 // Luminosity mesh of one pair (pic_cpu.jl:2700-2717) and its two deposit planes.
-__device__ bool lum_geometry(const GeomOpts &o, const double *v1, const double *v2, int k, int p, const SliceTable *t1,
+__device__ __noinline__ bool lum_geometry(const GeomOpts &o, const double *v1, const double *v2, int k, int p, const SliceTable *t1,
                              const SliceTable *t2, int i, int j, Plane *planes, int lum_plane0, double *lumgeo, ErrorInfo *err) {
     double xmin = jmin(v1[8], v2[8]), xmax = jmax(v1[9], v2[9]);
     double ymin = jmin(v1[10], v2[10]), ymax = jmax(v1[11], v2[11]);
@@ -302,6 +303,7 @@ __global__ void k_geometry(GeometryArgs a) {
         it.nplanes = o.npl;
         it.event = event;
         it.valid = 1;
+        it.count_field = 0;
         const int src_beam = it.src_beam;
         set_plane(a.planes[d * o.npl + 0], d, 0, src_beam, o.nx, o.ny, slot, it.src_off, it.src_n, it.sL, sg.x0, sg.y0, hx, hy);
         set_plane(a.planes[d * o.npl + 1], d, 0, src_beam, o.nx, o.ny, slot, it.src_off, it.src_n, it.sR, sg.x0, sg.y0, hx, hy);
@@ -336,9 +338,13 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
                  *PY = a.st.c[pl.beam][3] + pl.off;
     const double s = pl.s, x0 = pl.x0, y0 = pl.y0, hxi = pl.hxi, hyi = pl.hyi, scale = a.scale;
     constexpr int K = METHOD == 0 ? 2 : 3;
+    int dropped = 0;
     for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
         const double xd = X[t] + PX[t] * s;
         const double yd = Y[t] + PY[t] * s;
+        if (pl.count_box && !(__builtin_isfinite(xd) && x0 <= xd && xd <= pl.bx1 && __builtin_isfinite(yd) && y0 <= yd &&
+                              yd <= pl.by1))
+            ++dropped;
         int bx, by;
         double wx[3], wy[3];
         stencil<METHOD>((xd - x0) * hxi, n1, bx, wx);
@@ -350,6 +356,10 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
                 const unsigned long long v = __double2ull_rn(wx[m] * wy[q] * scale);
                 if (v) atomicAdd(&tile[(by + q) * n1 + bx + m], v);
             }
+    }
+    if (pl.count_box) {
+        for (int o = 32; o > 0; o >>= 1) dropped += __shfl_down(dropped, o);
+        if ((threadIdx.x & 63) == 0 && dropped) atomicAdd(&a.dropped[0], (unsigned long long)dropped);
     }
     __syncthreads();
     unsigned long long *out = a.charge + (size_t)blockIdx.y * a.charge_stride;
@@ -401,6 +411,8 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     const double c = it.c_src, ks = it.kick_scale;
     const int lk = a.lkick;
     constexpr int K = METHOD == 0 ? 2 : 3;
+    int dropped = 0;
+    const double fx1 = it.fg.x0 + it.fg.width, fy1 = it.fg.y0 + it.fg.height;
     for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
         double x = X[t], px = PX[t], y = Y[t], py = PY[t];
         const double z = Z[t];
@@ -409,6 +421,9 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
         x += s * px;
         y += s * py;
         if (lk) pz -= 0.25 * (px * px + py * py);
+        if (it.count_field && !(__builtin_isfinite(x) && it.fg.x0 <= x && x <= fx1 && __builtin_isfinite(y) &&
+                                it.fg.y0 <= y && y <= fy1))
+            ++dropped;
         const double zL = jclamp(-z * hzi + zbias, 0.0, 1.0);
         double Kx = 0.0, Ky = 0.0, Kz = 0.0;
         int bx, by;
@@ -481,6 +496,10 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
         if (lk) pz += 0.25 * (px * px + py * py);
         X[t] = x; PX[t] = px; Y[t] = y; PY[t] = py; PZ[t] = pz;
     }
+    if (it.count_field) {
+        for (int o = 32; o > 0; o >>= 1) dropped += __shfl_down(dropped, o);
+        if ((threadIdx.x & 63) == 0 && dropped) atomicAdd(&a.dropped[0], (unsigned long long)dropped);
+    }
 }
 
 hipError_t launch_kick(const KickArgs &a, hipStream_t s) {
@@ -527,6 +546,210 @@ __global__ __launch_bounds__(kLumThreads) void k_lum(LumArgs a) {
 hipError_t launch_lum(const LumArgs &a, hipStream_t s) {
     if (a.npairs <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_lum, dim3(a.npairs), dim3(kLumThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace octopic
+
+// ==================================================================================== :node meshes
+// _pic_build_node_grids! / _pic_prebuild_node_caches! (pic_cpu.jl:1317-1419): at turn start, for every
+// (source slice, direction), one mesh per node b of the field beam's slicing.  Node b's source box
+// spans the source drifted to nodes b and b+1; its field box is the union of the two field slices
+// adjacent to the node, each drifted to the source centre.  Built from the turn-start states, before
+// any pair, as the reference requires for schedule independence.
+namespace octopic {
+
+constexpr int kNodeThreads = 256;
+
+__device__ __forceinline__ void block_minmax4(double v[4], double *out) {
+    __shared__ double red[4][kNodeThreads];
+    for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int w = kNodeThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] = jmin(red[0][threadIdx.x], red[0][threadIdx.x + w]);
+            red[1][threadIdx.x] = jmax(red[1][threadIdx.x], red[1][threadIdx.x + w]);
+            red[2][threadIdx.x] = jmin(red[2][threadIdx.x], red[2][threadIdx.x + w]);
+            red[3][threadIdx.x] = jmax(red[3][threadIdx.x], red[3][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
+    __syncthreads();
+}
+
+// grid (nchunks, ns_src): source boxes at every node drift; partial[((i*nb + b)*nchunks + c)*4]
+__global__ __launch_bounds__(kNodeThreads) void k_node_src_bounds(NodeArgs a) {
+    if (failed(a.err)) return;
+    const int i = blockIdx.y;
+    const int nbF = a.tab_fld->ns + 1;
+    const double c = a.tab_src->center[i];
+    const int64_t off = a.tab_src->offset[i], n = a.tab_src->count[i];
+    const int64_t lo = n * blockIdx.x / a.nchunks, hi = n * (blockIdx.x + 1) / a.nchunks;
+    const double *X = a.st.c[a.src_beam][0] + off, *PX = a.st.c[a.src_beam][1] + off;
+    const double *Y = a.st.c[a.src_beam][2] + off, *PY = a.st.c[a.src_beam][3] + off;
+    for (int b = 0; b < nbF; ++b) {
+        const double d = 0.5 * (c - a.tab_fld->boundary[b]);
+        double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+        for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+            const double xv = X[t] + PX[t] * d, yv = Y[t] + PY[t] * d;
+            v[0] = jmin(v[0], xv); v[1] = jmax(v[1], xv); v[2] = jmin(v[2], yv); v[3] = jmax(v[3], yv);
+        }
+        block_minmax4(v, a.partial + (((size_t)i * nbF + b) * a.nchunks + blockIdx.x) * 4);
+    }
+}
+
+// grid (nchunks, ns_fld, ns_src): field slice sl drifted to the centre of source slice i
+__global__ __launch_bounds__(kNodeThreads) void k_node_fld_bounds(NodeArgs a, double *fpartial) {
+    if (failed(a.err)) return;
+    const int sl = blockIdx.y, i = blockIdx.z;
+    const int nsF = a.tab_fld->ns;
+    const double c = a.tab_src->center[i];
+    const int64_t off = a.tab_fld->offset[sl], n = a.tab_fld->count[sl];
+    const int64_t lo = n * blockIdx.x / a.nchunks, hi = n * (blockIdx.x + 1) / a.nchunks;
+    const double *X = a.st.c[a.fld_beam][0] + off, *PX = a.st.c[a.fld_beam][1] + off;
+    const double *Y = a.st.c[a.fld_beam][2] + off, *PY = a.st.c[a.fld_beam][3] + off;
+    const double *Z = a.st.c[a.fld_beam][4] + off;
+    double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+        const double sh = 0.5 * (Z[t] - c);
+        const double xv = X[t] + sh * PX[t], yv = Y[t] + sh * PY[t];
+        v[0] = jmin(v[0], xv); v[1] = jmax(v[1], xv); v[2] = jmin(v[2], yv); v[3] = jmax(v[3], yv);
+    }
+    block_minmax4(v, fpartial + (((size_t)i * nsF + sl) * a.nchunks + blockIdx.x) * 4);
+}
+
+__device__ __forceinline__ void reduce4(const double *p, int nchunks, double v[4]) {
+    v[0] = INFINITY; v[1] = -INFINITY; v[2] = INFINITY; v[3] = -INFINITY;
+    for (int c = 0; c < nchunks; ++c) {
+        v[0] = jmin(v[0], p[4 * c]); v[1] = jmax(v[1], p[4 * c + 1]);
+        v[2] = jmin(v[2], p[4 * c + 2]); v[3] = jmax(v[3], p[4 * c + 3]);
+    }
+}
+
+// one lane per (source slice i, node b)
+__global__ void k_node_meshes(NodeArgs a, const double *fpartial) {
+    const int nsS = a.tab_src->ns, nsF = a.tab_fld->ns, nbF = nsF + 1;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsS * nbF) return;
+    if (failed(a.err)) return;
+    const int i = k / nbF, b = k % nbF;
+    CacheGeo &node = a.nodes[k];
+    GreenJob &job = a.jobs[k];
+    node.valid = 0;
+    job.valid = 0;
+    job.slot = a.slot0 + k;
+    if (a.tab_src->count[i] == 0) return;
+    const int bn = b + 1 < nbF ? b + 1 : nbF - 1;
+    double s0[4], s1[4];
+    reduce4(a.partial + ((size_t)i * nbF + b) * a.nchunks * 4, a.nchunks, s0);
+    reduce4(a.partial + ((size_t)i * nbF + bn) * a.nchunks * 4, a.nchunks, s1);
+    for (int q = 0; q < 4; ++q)
+        if (s0[q] != s0[q] || s1[q] != s1[q]) { set_error(a.err, OCTOPIC_ERR_NONFINITE, -1, a.dir, 4, 0, 0); return; }
+    const double sxmin = jmin(s0[0], s1[0]), sxmax = jmax(s0[1], s1[1]);
+    const double symin = jmin(s0[2], s1[2]), symax = jmax(s0[3], s1[3]);
+    double fxmin = INFINITY, fxmax = -INFINITY, fymin = INFINITY, fymax = -INFINITY;
+    for (int adj = b - 1; adj <= b; ++adj) {
+        if (adj < 0 || adj >= nsF) continue;
+        double f[4];
+        reduce4(fpartial + ((size_t)i * nsF + adj) * a.nchunks * 4, a.nchunks, f);
+        for (int q = 0; q < 4; ++q)
+            if (f[q] != f[q]) { set_error(a.err, OCTOPIC_ERR_NONFINITE, -1, a.dir, 4, 0, 0); return; }
+        if (!__builtin_isfinite(f[0])) continue;
+        fxmin = jmin(fxmin, f[0]); fxmax = jmax(fxmax, f[1]);
+        fymin = jmin(fymin, f[2]); fymax = jmax(fymax, f[3]);
+    }
+    if (!__builtin_isfinite(fxmin)) return;
+    Grid sg, fg;
+    if (!interaction_grids(a.o, sxmin, sxmax, symin, symax, fxmin, fxmax, fymin, fymax, sg, fg)) {
+        set_error(a.err, OCTOPIC_ERR_DEGENERATE_EXTENT, -1, a.dir, 3, jmax(sxmax - sxmin, fxmax - fxmin),
+                  jmax(symax - symin, fymax - fymin));
+        return;
+    }
+    node.valid = 1;
+    node.sg = sg;
+    node.fg = fg;
+    job.valid = 1;
+    job.fx0 = fg.x0; job.fy0 = fg.y0; job.sx0 = sg.x0; job.sy0 = sg.y0;
+    job.hx = sg.width / (double)(a.o.nx - 1);
+    job.hy = sg.height / (double)(a.o.ny - 1);
+    atomicAdd(&a.counters[3], 1ull);
+}
+
+hipError_t launch_node_prepare(const NodeArgs &a, double *fpartial, int ns_src, int ns_fld, hipStream_t s) {
+    hipLaunchKernelGGL(k_node_src_bounds, dim3(a.nchunks, ns_src), dim3(kNodeThreads), 0, s, a);
+    hipLaunchKernelGGL(k_node_fld_bounds, dim3(a.nchunks, ns_fld, ns_src), dim3(kNodeThreads), 0, s, a, fpartial);
+    const int nn = ns_src * (ns_fld + 1);
+    hipLaunchKernelGGL(k_node_meshes, dim3((nn + 63) / 64), dim3(64), 0, s, a, (const double *)fpartial);
+    return hipGetLastError();
+}
+
+__device__ void set_node_plane(Plane &pl, int d, int beam, const GeomOpts &o, int slot, int64_t off, int64_t n, double s,
+                               const Grid &g, int count) {
+    const double hx = g.width / (double)(o.nx - 1), hy = g.height / (double)(o.ny - 1);
+    set_plane(pl, d, 0, beam, o.nx, o.ny, slot, off, n, s, g.x0, g.y0, hx, hy);
+    pl.count_box = count;
+    pl.bx1 = g.x0 + g.width;
+    pl.by1 = g.y0 + g.height;
+}
+
+// _pic_interaction_node! descriptors (pic_cpu.jl:1448-1538) for every pair of a batch
+__global__ void k_node_geometry(NodeGeometryArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.npairs) return;
+    if (failed(a.err)) return;
+    const GeomOpts &o = a.o;
+    const BatchPair pr = a.pairs[k];
+    const int i = pr.i, j = pr.j, p = pr.p;
+    const SliceTable *t1 = a.tab1, *t2 = a.tab2;
+    for (int dir = 1; dir <= 2; ++dir) {
+        const int d = 2 * k + dir - 1;
+        Inter &it = a.inters[d];
+        it.valid = 0;
+        const SliceTable *ts = dir == 1 ? t1 : t2, *tf = dir == 1 ? t2 : t1;
+        const int si = dir == 1 ? i : j, fi = dir == 1 ? j : i;
+        const int nbF = tf->ns + 1;
+        const CacheGeo *nodes = dir == 1 ? a.nodes1 : a.nodes2;
+        const int slot0 = dir == 1 ? a.slot1 : a.slot2;
+        const int kl = si * nbF + fi, kr = si * nbF + fi + 1;
+        const CacheGeo gL = nodes[kl], gR = nodes[kr];
+        if (!gL.valid || !gR.valid) { a.ran[p] = 0; return; }
+        const double c_src = ts->center[si];
+        it.p = p; it.i = i; it.j = j; it.dir = dir;
+        it.src_beam = dir == 1 ? 0 : 1;
+        it.fld_beam = dir == 1 ? 1 : 0;
+        it.src_off = ts->offset[si]; it.src_n = ts->count[si];
+        it.fld_off = tf->offset[fi]; it.fld_n = tf->count[fi];
+        it.c_src = c_src; it.c_fld = tf->center[fi];
+        it.lb = tf->boundary[fi]; it.rb = tf->boundary[fi + 1];
+        it.sL = 0.5 * (c_src - it.lb);
+        it.sR = 0.5 * (c_src - it.rb);
+        it.sM = 0.5 * (c_src - it.c_fld);
+        it.kick_scale = 2.0 * (dir == 1 ? o.kbb2 : o.kbb1);
+        it.sg = gL.sg; it.fg = gL.fg; it.fgR = gR.fg;
+        it.plane0 = d * o.npl;
+        it.nplanes = o.lkick ? 3 : 2;       // F_Z only with the longitudinal kick (pic_cpu.jl:1497-1503)
+        it.event = 0;
+        it.count_field = 1;
+        it.valid = 1;
+        const int sb = it.src_beam;
+        set_node_plane(a.planes[d * o.npl + 0], d, sb, o, slot0 + kl, it.src_off, it.src_n, it.sL, gL.sg, 1);
+        set_node_plane(a.planes[d * o.npl + 1], d, sb, o, slot0 + kr, it.src_off, it.src_n, it.sR, gR.sg, 1);
+        set_node_plane(a.planes[d * o.npl + 2], d, sb, o, slot0 + kl, it.src_off, it.src_n, it.sR, gL.sg, 0);
+        fill_record(a.records[2 * p + dir - 1], p, i, j, dir, 0, gL.sg, gL.fg);
+    }
+    a.ran[p] = 1;
+    if (o.compute_lum) {
+        double v1[12], v2[12];
+        reduce_partials(a.partial, 2 * k, a.nchunks, v1);
+        reduce_partials(a.partial, 2 * k + 1, a.nchunks, v2);
+        lum_geometry(o, v1, v2, k, p, t1, t2, i, j, a.planes, 2 * a.npairs * o.npl, a.lumgeo, a.err);
+    }
+}
+
+hipError_t launch_node_geometry(const NodeGeometryArgs &a, hipStream_t s) {
+    if (a.npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_node_geometry, dim3((a.npairs + 63) / 64), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

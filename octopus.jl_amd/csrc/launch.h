@@ -42,7 +42,7 @@ struct CacheGeo { int32_t valid, reserved; Grid sg, fg; };
 // Static options the geometry kernel needs.
 struct GeomOpts {
     int nx, ny, lum_nx, lum_ny;
-    int green_type, green_cache, quadratic, interaction_grid, compute_lum;
+    int green_type, green_cache, quadratic, interaction_grid, compute_lum, lkick;
     int ns1, ns2;
     int npl;                     // planes per interaction (2 or 3)
     int temp_slot0;              // first per-batch temporary Green slot (green_cache = none)
@@ -89,6 +89,7 @@ struct DepositArgs {
     double scale;                // fixed-point scale 2^k
     unsigned long long *charge;  // [plane][charge_stride]
     int64_t charge_stride;
+    unsigned long long *dropped; // dropped-particle counter
     const ErrorInfo *err;
 };
 hipError_t launch_deposit(const DepositArgs &a, hipStream_t s);
@@ -101,6 +102,7 @@ struct KickArgs {
     int nx, ny, npl, method, mode;   // mode: 0 linear, 1 quadratic, 2 node
     int lkick;
     int chunks;
+    unsigned long long *dropped;
     const ErrorInfo *err;
 };
 hipError_t launch_kick(const KickArgs &a, hipStream_t s);
@@ -138,7 +140,7 @@ struct NodeArgs {
     int src_beam, fld_beam;
     int dir;                     // 1 or 2
     int nchunks;
-    double *partial;             // per source slice: [nb][nchunks][4] source; per field slice box: [ns_src][ns_fld][nchunks][4]
+    double *partial;             // source boxes [ns_src][nb][nchunks][4]
     GeomOpts o;
     int slot0;                   // first Green slot of this direction's node meshes
     CacheGeo *nodes;             // [ns_src][nb] node meshes of this direction
@@ -146,7 +148,7 @@ struct NodeArgs {
     unsigned long long *counters;
     ErrorInfo *err;
 };
-hipError_t launch_node_prepare(const NodeArgs &a, hipStream_t s);
+hipError_t launch_node_prepare(const NodeArgs &a, double *fpartial, int ns_src, int ns_fld, hipStream_t s);
 
 struct NodeGeometryArgs {
     const BatchPair *pairs;
@@ -161,6 +163,7 @@ struct NodeGeometryArgs {
     Plane *planes;
     double *lumgeo;
     octopic_interaction_record *records;
+    int32_t *ran;                      // per pair position: 0 when a node mesh is missing
     ErrorInfo *err;
 };
 hipError_t launch_node_geometry(const NodeGeometryArgs &a, hipStream_t s);

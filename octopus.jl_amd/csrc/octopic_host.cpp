@@ -228,6 +228,7 @@ struct octopic_handle {
     DBuf<CacheGeo> nodes;
     DBuf<GreenJob> node_jobs;
     DBuf<double> node_partial;
+    DBuf<int32_t> ran_dev;
     // pinned host mirrors
     SliceTable *h_tab = nullptr;
     ErrorInfo *h_err = nullptr;
@@ -454,7 +455,7 @@ void octopic_destroy(octopic_handle *h) {
     h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
     h->partial.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
     h->charge.release(); h->qh.release(); h->phi.release(); h->table.release(); h->records.release();
-    h->lum_parts.release(); h->nodes.release(); h->node_jobs.release(); h->node_partial.release();
+    h->lum_parts.release(); h->nodes.release(); h->node_jobs.release(); h->node_partial.release(); h->ran_dev.release();
     h->px.tw.release(); h->py.tw.release();
     delete h;
 }
@@ -484,9 +485,9 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         h->tab.ensure(2);
         h->q.ensure(2 * kMaxSlices);
         h->errinfo.ensure(1);
-        h->counters.ensure(4);
+        h->counters.ensure(6);
         HIPCHECK(hipMemset(h->errinfo.p, 0, sizeof(ErrorInfo)));
-        HIPCHECK(hipMemset(h->counters.p, 0, 4 * sizeof(unsigned long long)));
+        HIPCHECK(hipMemset(h->counters.p, 0, 6 * sizeof(unsigned long long)));
         h->lanes.ensure(kFoldLanes);
         h->mm.ensure(2 * (kFoldLanes / 256));
         h->bins.ensure(16384);
@@ -501,7 +502,7 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         }
         HIPCHECK(hipHostMalloc((void **)&h->h_tab, 2 * sizeof(SliceTable)));
         HIPCHECK(hipHostMalloc((void **)&h->h_err, sizeof(ErrorInfo)));
-        HIPCHECK(hipHostMalloc((void **)&h->h_counters, 4 * sizeof(unsigned long long)));
+        HIPCHECK(hipHostMalloc((void **)&h->h_counters, 6 * sizeof(unsigned long long)));
         HIPCHECK(hipDeviceSynchronize());
     } catch (Fail &f) {
         int code = f.code;
@@ -717,8 +718,8 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     const int temp_slot0 = h->nkeys;
     const int nslots = h->nkeys + (o.green_cache == OCTOPIC_CACHE_SLICE_PAIR ? 0 : 2 * W);
     h->spec.ensure((size_t)std::max(nslots, 1) * h->slot_stride);
-    unsigned long long counters_before[4];
-    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    unsigned long long counters_before[6];
+    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     std::memcpy(counters_before, h->h_counters, sizeof(counters_before));
 
@@ -726,6 +727,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     go.nx = nx; go.ny = ny; go.lum_nx = h->lnx; go.lum_ny = h->lny;
     go.green_type = o.green_type; go.green_cache = o.green_cache; go.quadratic = h->mode == 1;
     go.interaction_grid = o.interaction_grid; go.compute_lum = compute_lum ? 1 : 0;
+    go.lkick = o.longitudinal_kick ? 1 : 0;
     go.ns1 = ns1; go.ns2 = ns2; go.npl = npl; go.temp_slot0 = temp_slot0;
     go.min_x = o.min_extent_x; go.min_y = o.min_extent_y; go.quantize = o.grid_quantize;
     go.min_ratio = o.green_min_ratio; go.growth = o.green_growth;
@@ -736,7 +738,43 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         for (int c = 0; c < 6; ++c) st.c[b][c] = h->st[b][c].p;
     const SliceTable *dt1 = h->tab.p, *dt2 = h->tab.p + 1;
 
-    if (h->mode == 2) fail(OCTOPIC_ERR_UNSUPPORTED, "interaction_grid = :node is not available in this build");
+    int slot_node1 = 0, slot_node2 = 0;
+    if (h->mode == 2) {
+        // node meshes of both directions at turn start (pic_cpu.jl:113-119, 1404-1419)
+        Timed t(h, F_NODE);
+        const int nn1 = ns1 * (ns2 + 1), nn2 = ns2 * (ns1 + 1);
+        slot_node1 = 0;
+        slot_node2 = nn1;
+        h->spec.ensure((size_t)(nn1 + nn2) * h->slot_stride);
+        h->nodes.ensure(nn1 + nn2);
+        h->node_jobs.ensure(nn1 + nn2);
+        const int nch = (int)std::min<int64_t>(32, std::max<int64_t>(1, (maxcount + 8191) / 8192));
+        const int nsmax = std::max(ns1, ns2);
+        h->node_partial.ensure((size_t)nsmax * (nsmax + 1) * nch * 4 * 2);
+        for (int dir = 1; dir <= 2; ++dir) {
+            NodeArgs a{};
+            a.tab_src = dir == 1 ? dt1 : dt2;
+            a.tab_fld = dir == 1 ? dt2 : dt1;
+            a.st = st;
+            a.src_beam = dir == 1 ? 0 : 1;
+            a.fld_beam = dir == 1 ? 1 : 0;
+            a.dir = dir;
+            a.nchunks = nch;
+            a.partial = h->node_partial.p;
+            a.o = go;
+            a.slot0 = dir == 1 ? slot_node1 : slot_node2;
+            a.nodes = h->nodes.p + (dir == 1 ? 0 : nn1);
+            a.jobs = h->node_jobs.p + (dir == 1 ? 0 : nn1);
+            a.counters = h->counters.p;
+            a.err = h->errinfo.p;
+            double *fpartial = h->node_partial.p + (size_t)nsmax * (nsmax + 1) * nch * 4;
+            HIPCHECK(launch_node_prepare(a, fpartial, dir == 1 ? ns1 : ns2, dir == 1 ? ns2 : ns1, s));
+        }
+        HIPCHECK(launch_green_build(h->node_jobs.p, nn1 + nn2, h->spec.p, h->slot_stride, nx, ny,
+                                    o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev, s));
+        h->ran_dev.ensure(std::max(npairs, 1));
+        HIPCHECK(hipMemsetAsync(h->ran_dev.p, 0, sizeof(int32_t) * npairs, s));
+    }
 
     int64_t planes_total = 0, builds_guess = 0;
     (void)builds_guess;
@@ -755,7 +793,16 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             BoundsArgs a{dp, B, dt1, dt2, st, nchunks, h->partial.p, h->errinfo.p};
             HIPCHECK(launch_bounds(a, s));
         }
-        {
+        if (h->mode == 2) {
+            Timed t(h, F_GEOMETRY);
+            NodeGeometryArgs a{};
+            a.pairs = dp; a.npairs = B; a.tab1 = dt1; a.tab2 = dt2; a.partial = h->partial.p; a.nchunks = nchunks;
+            a.o = go; a.nodes1 = h->nodes.p; a.nodes2 = h->nodes.p + ns1 * (ns2 + 1);
+            a.slot1 = slot_node1; a.slot2 = slot_node2;
+            a.inters = h->inters.p; a.planes = h->planes.p; a.lumgeo = h->lumgeo.p; a.records = h->records.p;
+            a.ran = h->ran_dev.p; a.err = h->errinfo.p;
+            HIPCHECK(launch_node_geometry(a, s));
+        } else {
             Timed t(h, F_GEOMETRY);
             GeometryArgs a{};
             a.pairs = dp; a.npairs = B; a.tab1 = dt1; a.tab2 = dt2; a.partial = h->partial.p; a.nchunks = nchunks;
@@ -763,7 +810,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.lumgeo = h->lumgeo.p; a.records = h->records.p; a.counters = h->counters.p; a.err = h->errinfo.p;
             HIPCHECK(launch_geometry(a, s));
         }
-        {
+        if (h->mode != 2) {
             Timed t(h, F_GREEN);
             HIPCHECK(launch_green_build(h->jobs.p, 2 * B, h->spec.p, h->slot_stride, nx, ny,
                                         o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev, s));
@@ -774,7 +821,8 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             Timed t(h, F_DEPOSIT);
             DepositArgs a{};
             a.planes = h->planes.p; a.nplanes = nfp; a.st = st; a.method = o.deposit; a.ncells = (int)field_cells;
-            a.chunks = dchunks; a.scale = scale; a.charge = h->charge.p; a.charge_stride = charge_stride; a.err = h->errinfo.p;
+            a.chunks = dchunks; a.scale = scale; a.charge = h->charge.p; a.charge_stride = charge_stride;
+            a.dropped = h->counters.p + 4; a.err = h->errinfo.p;
             HIPCHECK(launch_deposit(a, s));
             if (compute_lum) {
                 DepositArgs l = a;
@@ -809,6 +857,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.inters = h->inters.p; a.ninters = 2 * B; a.st = st; a.table = h->table.p; a.nx = nx; a.ny = ny; a.npl = npl;
             a.method = o.deposit; a.mode = h->mode; a.lkick = o.longitudinal_kick ? 1 : 0;
             a.chunks = (int)std::min<int64_t>(64, std::max<int64_t>(1, (bmax + 2047) / 2048));
+            a.dropped = h->counters.p + 4;
             a.err = h->errinfo.p;
             HIPCHECK(launch_kick(a, s));
         }
@@ -834,9 +883,16 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     if (npairs)
         HIPCHECK(hipMemcpyAsync(h->recs.data(), h->records.p, sizeof(octopic_interaction_record) * 2 * npairs,
                                 hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    std::vector<int32_t> ran_node;
+    if (h->mode == 2 && npairs) {
+        ran_node.resize(npairs);
+        HIPCHECK(hipMemcpyAsync(ran_node.data(), h->ran_dev.p, sizeof(int32_t) * npairs, hipMemcpyDeviceToHost, s));
+    }
     HIPCHECK(hipStreamSynchronize(s));
     if (h->h_err->code) raise_device_error(h, *h->h_err, turn);
+    if (h->mode == 2)
+        for (int p = 0; p < npairs; ++p) h->pran[p] = h->pran[p] && ran_node[p];
     for (int b = 0; b < 2; ++b) {
         if (B[b]->on_device) continue;
         double *hp2[6] = {B[b]->x, B[b]->px, B[b]->y, B[b]->py, B[b]->z, B[b]->pz};
@@ -857,7 +913,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     S.green_misses = (int64_t)h->h_counters[1];
     S.green_rebuilds = (int64_t)h->h_counters[2];
     S.green_builds = (int64_t)(h->h_counters[3] - counters_before[3]);
-    S.dropped = 0;
+    S.dropped = (int64_t)(h->h_counters[4] - counters_before[4]);
     S.pairs = npairs;
     S.batches = (int64_t)batches.size();
     S.widest_batch = widest;
