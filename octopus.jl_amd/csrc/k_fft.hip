@@ -21,9 +21,7 @@ constexpr int kFFTLanes = 64;        // lanes per transform
 constexpr int kFFTPerBlock = 4;      // transforms per workgroup
 constexpr int kFFTThreads = kFFTLanes * kFFTPerBlock;
 
-__device__ __forceinline__ bool failed(const ErrorInfo *err) {
-    return __builtin_nontemporal_load(&err->code) != 0;
-}
+__device__ __forceinline__ bool failed(const ErrorInfo *err) { return *((volatile const int32_t *)&err->code) != 0; }
 
 // ---------------------------------------------------------------------- Green kernel (pic_cpu.jl:2254-2272)
 __device__ __forceinline__ double atan_ratio(double num, double den) {
@@ -67,10 +65,12 @@ __device__ __forceinline__ void unpack_pair(const double2 *Z, int P, int k, doub
 
 // Green rows: rows (ya, ya+1) of the real table -> x-half-spectrum into slot[kx][y], y in [0, 2ny).
 __global__ __launch_bounds__(kFFTThreads) void k_green_rows(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride,
-                                                            int nx, int ny, int integrated, FFTPlanDev px) {
+                                                            int nslots, int nx, int ny, int integrated, FFTPlanDev px,
+                                                            const ErrorInfo *err) {
     extern __shared__ double2 lds[];
+    if (failed(err)) return;
     const GreenJob job = jobs[blockIdx.y];
-    if (!job.valid) return;
+    if (!job.valid || job.slot < 0 || job.slot >= nslots) return;
     const int P1 = 2 * nx, P2 = 2 * ny;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
     double2 *a = lds + (size_t)grp * 2 * P1;
@@ -99,10 +99,11 @@ __global__ __launch_bounds__(kFFTThreads) void k_green_rows(const GreenJob *jobs
 
 // Green columns: in-place FFT along y of slot[kx][0..2ny).
 __global__ __launch_bounds__(kFFTThreads) void k_green_cols(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride,
-                                                            int nx, int ny, FFTPlanDev py) {
+                                                            int nslots, int nx, int ny, FFTPlanDev py, const ErrorInfo *err) {
     extern __shared__ double2 lds[];
+    if (failed(err)) return;
     const GreenJob job = jobs[blockIdx.y];
-    if (!job.valid) return;
+    if (!job.valid || job.slot < 0 || job.slot >= nslots) return;
     const int P2 = 2 * ny;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
     double2 *a = lds + (size_t)grp * 2 * P2;
@@ -177,7 +178,8 @@ __global__ __launch_bounds__(kFFTThreads) void k_cols_conv(const Plane *planes, 
     __syncthreads();
     double2 *Z = fft_lds<-1>(a, b, py, lane, kFFTLanes);
     double2 *W = (Z == a) ? b : a;
-    const double2 *g = gspec + (size_t)planes[plane].slot * slot_stride + (size_t)(active ? kx : 0) * P2;
+    const int slot = planes[plane].slot;
+    const double2 *g = gspec + (size_t)(slot < 0 ? 0 : slot) * slot_stride + (size_t)(active ? kx : 0) * P2;
     for (int y = lane; y < P2; y += kFFTLanes) Z[y] = cmul(Z[y], g[y]);
     __syncthreads();
     double2 *R = fft_lds<+1>(Z, W, py, lane, kFFTLanes);
@@ -263,17 +265,19 @@ static hipError_t ensure_lds(const void *fn, size_t bytes) {
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride, int nx, int ny,
-                              int integrated, const FFTPlanDev &px, const FFTPlanDev &py, hipStream_t s) {
+hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride, int nslots, int nx, int ny,
+                              int integrated, const FFTPlanDev &px, const FFTPlanDev &py, const ErrorInfo *err,
+                              hipStream_t s) {
     if (njobs <= 0) return hipSuccess;
     const int P1 = 2 * nx, P2 = 2 * ny;
     ensure_lds((const void *)k_green_rows, fft_lds_bytes(P1));
     ensure_lds((const void *)k_green_cols, fft_lds_bytes(P2));
     dim3 g1((P2 / 2 + kFFTPerBlock - 1) / kFFTPerBlock, njobs);
-    hipLaunchKernelGGL(k_green_rows, g1, dim3(kFFTThreads), fft_lds_bytes(P1), s, jobs, njobs, spec, slot_stride, nx, ny,
-                       integrated, px);
+    hipLaunchKernelGGL(k_green_rows, g1, dim3(kFFTThreads), fft_lds_bytes(P1), s, jobs, njobs, spec, slot_stride, nslots, nx,
+                       ny, integrated, px, err);
     dim3 g2((nx + 1 + kFFTPerBlock - 1) / kFFTPerBlock, njobs);
-    hipLaunchKernelGGL(k_green_cols, g2, dim3(kFFTThreads), fft_lds_bytes(P2), s, jobs, njobs, spec, slot_stride, nx, ny, py);
+    hipLaunchKernelGGL(k_green_cols, g2, dim3(kFFTThreads), fft_lds_bytes(P2), s, jobs, njobs, spec, slot_stride, nslots, nx,
+                       ny, py, err);
     return hipGetLastError();
 }
 

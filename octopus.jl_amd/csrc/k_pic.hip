@@ -213,9 +213,31 @@ __device__ __noinline__ bool lum_geometry(const GeomOpts &o, const double *v1, c
     return true;
 }
 
+// Every descriptor a pair owns is made inert before anything can return early, so that no later
+// stage of the batch can read a stale or uninitialised descriptor.
+__device__ void neutralise_pair(int k, int npairs, const GeomOpts &o, Inter *inters, Plane *planes, GreenJob *jobs) {
+    for (int dir = 0; dir < 2; ++dir) {
+        const int d = 2 * k + dir;
+        inters[d].valid = 0;
+        inters[d].fld_n = 0;
+        inters[d].nplanes = 0;
+        if (jobs) { jobs[d].valid = 0; jobs[d].slot = -1; }
+        for (int q = 0; q < o.npl; ++q) {
+            Plane &pl = planes[d * o.npl + q];
+            pl.n = 0; pl.off = 0; pl.beam = 0; pl.slot = -1; pl.count_box = 0;
+        }
+    }
+    const int lum0 = 2 * npairs * o.npl;
+    for (int side = 0; side < 2; ++side) {
+        Plane &pl = planes[lum0 + 2 * k + side];
+        pl.n = 0; pl.off = 0; pl.beam = 0; pl.slot = -1; pl.count_box = 0;
+    }
+}
+
 __global__ void k_geometry(GeometryArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.npairs) return;
+    neutralise_pair(k, a.npairs, a.o, a.inters, a.planes, a.jobs);
     if (failed(a.err)) return;
     const GeomOpts &o = a.o;
     const BatchPair pr = a.pairs[k];
@@ -632,6 +654,8 @@ __global__ void k_node_meshes(NodeArgs a, const double *fpartial) {
     const int nsS = a.tab_src->ns, nsF = a.tab_fld->ns, nbF = nsF + 1;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsS * nbF) return;
+    a.jobs[k].valid = 0;
+    a.nodes[k].valid = 0;
     if (failed(a.err)) return;
     const int i = k / nbF, b = k % nbF;
     CacheGeo &node = a.nodes[k];
@@ -697,6 +721,7 @@ __device__ void set_node_plane(Plane &pl, int d, int beam, const GeomOpts &o, in
 __global__ void k_node_geometry(NodeGeometryArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.npairs) return;
+    neutralise_pair(k, a.npairs, a.o, a.inters, a.planes, nullptr);
     if (failed(a.err)) return;
     const GeomOpts &o = a.o;
     const BatchPair pr = a.pairs[k];
@@ -713,7 +738,11 @@ __global__ void k_node_geometry(NodeGeometryArgs a) {
         const int slot0 = dir == 1 ? a.slot1 : a.slot2;
         const int kl = si * nbF + fi, kr = si * nbF + fi + 1;
         const CacheGeo gL = nodes[kl], gR = nodes[kr];
-        if (!gL.valid || !gR.valid) { a.ran[p] = 0; return; }
+        if (!gL.valid || !gR.valid) {
+            neutralise_pair(k, a.npairs, o, a.inters, a.planes, nullptr);
+            a.ran[p] = 0;
+            return;
+        }
         const double c_src = ts->center[si];
         it.p = p; it.i = i; it.j = j; it.dir = dir;
         it.src_beam = dir == 1 ? 0 : 1;

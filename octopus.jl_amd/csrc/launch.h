@@ -121,8 +121,9 @@ struct LumArgs {
 hipError_t launch_lum(const LumArgs &a, hipStream_t s);
 
 // FFT stages (k_fft.hip)
-hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride, int nx, int ny,
-                              int integrated, const FFTPlanDev &px, const FFTPlanDev &py, hipStream_t s);
+hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride, int nslots, int nx, int ny,
+                              int integrated, const FFTPlanDev &px, const FFTPlanDev &py, const ErrorInfo *err,
+                              hipStream_t s);
 hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long *charge, int64_t charge_stride, double2 *qh,
                            int64_t qh_stride, int nx, int ny, double inv_scale, const FFTPlanDev &px, const ErrorInfo *err,
                            hipStream_t s);

@@ -703,9 +703,9 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     if (!hp.empty())
         HIPCHECK(hipMemcpyAsync(h->pairs.p, hp.data(), sizeof(BatchPair) * hp.size(), hipMemcpyHostToDevice, s));
     h->partial.ensure((size_t)2 * W * kMaxChunks * 12);
-    h->inters.ensure(2 * W);
-    h->planes.ensure(nplanes_max);
-    h->jobs.ensure(2 * W);
+    h->inters.ensure_zero(2 * W, s);
+    h->planes.ensure_zero(nplanes_max, s);
+    h->jobs.ensure_zero(2 * W, s);
     h->lumgeo.ensure(3 * W);
     h->charge.ensure_zero((size_t)nplanes_max * charge_stride, s);
     h->qh.ensure((size_t)nfield_planes * (nx + 1) * ny);
@@ -746,8 +746,8 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         slot_node1 = 0;
         slot_node2 = nn1;
         h->spec.ensure((size_t)(nn1 + nn2) * h->slot_stride);
-        h->nodes.ensure(nn1 + nn2);
-        h->node_jobs.ensure(nn1 + nn2);
+        h->nodes.ensure_zero(nn1 + nn2, s);
+        h->node_jobs.ensure_zero(nn1 + nn2, s);
         const int nch = (int)std::min<int64_t>(32, std::max<int64_t>(1, (maxcount + 8191) / 8192));
         const int nsmax = std::max(ns1, ns2);
         h->node_partial.ensure((size_t)nsmax * (nsmax + 1) * nch * 4 * 2);
@@ -770,8 +770,9 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             double *fpartial = h->node_partial.p + (size_t)nsmax * (nsmax + 1) * nch * 4;
             HIPCHECK(launch_node_prepare(a, fpartial, dir == 1 ? ns1 : ns2, dir == 1 ? ns2 : ns1, s));
         }
-        HIPCHECK(launch_green_build(h->node_jobs.p, nn1 + nn2, h->spec.p, h->slot_stride, nx, ny,
-                                    o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev, s));
+        HIPCHECK(launch_green_build(h->node_jobs.p, nn1 + nn2, h->spec.p, h->slot_stride, nn1 + nn2, nx, ny,
+                                    o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev, h->errinfo.p,
+                                    s));
         h->ran_dev.ensure(std::max(npairs, 1));
         HIPCHECK(hipMemsetAsync(h->ran_dev.p, 0, sizeof(int32_t) * npairs, s));
     }
@@ -812,8 +813,9 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         }
         if (h->mode != 2) {
             Timed t(h, F_GREEN);
-            HIPCHECK(launch_green_build(h->jobs.p, 2 * B, h->spec.p, h->slot_stride, nx, ny,
-                                        o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev, s));
+            HIPCHECK(launch_green_build(h->jobs.p, 2 * B, h->spec.p, h->slot_stride, nslots, nx, ny,
+                                        o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev,
+                                        h->errinfo.p, s));
         }
         const int nfp = 2 * B * npl;
         const int dchunks = (int)std::min<int64_t>(32, std::max<int64_t>(1, (bmax + 8191) / 8192));

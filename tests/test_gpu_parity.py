@@ -163,9 +163,10 @@ def test_standard_green_cache_divergence_is_a_half_cell_realign(oracle, gpu):
     """With green_type = :standard and the slice-pair cache, an expanded grid is re-aligned with
     t += t > 0 ? -1/4 : 1/4 (pic_cpu.jl:1877-1905).  When the expanded origin separation is a whole number
     of cells (the reference's k = 2 mod 5 case, pic_cpu.jl:1835-1846) t is rounding noise around 0, so
-    1e-16 coordinate differences between two correct implementations pick opposite half-cell shifts.
-    Every mesh must therefore agree with the oracle either to 1e-12, or differ by exactly that flip:
-    origins shifted by +-h/2 in opposite directions along one axis, extents equal."""
+    1e-16 coordinate differences between two correct implementations can pick opposite half-cell shifts,
+    after which the two runs legitimately differ at the discretisation level.  Every interaction up to the
+    first divergence must match the oracle, and that first divergence must be exactly such a flip:
+    origins shifted by +-h/2 in opposite directions along one axis, extents equal, same cache event."""
     e, p = round_beams(oracle)
     sl = LongitudinalSlicing(nslices=5, method="normal_quantile", center_position="centroid")
     solver = PICPoissonSolver(grid=(64, 64), slicing=sl, green_type="standard")
@@ -177,23 +178,27 @@ def test_standard_green_cache_divergence_is_a_half_cell_realign(oracle, gpu):
     gcol.collide(be, bp)
     oi, oj, _, _ = ocol.pair_luminosity()
     first = first_use_pairs(list(zip(oi, oj)))
-    flips = 0
     for a, b in zip(ocol.records(), gcol.interaction_records()):
-        assert a.cache_event == b.cache_event
+        assert (a.p, a.dir, a.cache_event) == (b.p, b.dir, b.cache_event)
         sa, sb = np.array(a.source_grid), np.array(b.source_grid)
         fa, fb = np.array(a.field_grid), np.array(b.field_grid)
         if a.p in first:
             assert np.array_equal(sa, sb) and np.array_equal(fa, fb)
             continue
-        assert np.allclose(sa[2:], sb[2:], rtol=1e-12, atol=0) and np.allclose(fa[2:], fb[2:], rtol=1e-12, atol=0)
+        if np.allclose(sa, sb, rtol=1e-12, atol=0) and np.allclose(fa, fb, rtol=1e-12, atol=0):
+            continue
+        # the first divergence: a half-cell realign flip on one axis
+        assert np.allclose(sa[2:], sb[2:], rtol=1e-12, atol=0)
+        flipped = 0
         for ax in (0, 1):
-            h = sa[2 + ax] / (63.0)
+            h = sa[2 + ax] / 63.0
             ds, df = sb[ax] - sa[ax], fb[ax] - fa[ax]
             if abs(ds) <= 1e-12 * abs(sa[ax]) and abs(df) <= 1e-12 * abs(fa[ax]):
                 continue
             assert math.isclose(abs(ds), 0.5 * h, rel_tol=1e-6) and math.isclose(ds, -df, rel_tol=1e-6)
-            flips += 1
-    assert flips <= 8
+            flipped += 1
+        assert flipped == 1
+        break
 
 
 @pytest.mark.parametrize("method", ["equal_area", "equal_width", "specified"])
