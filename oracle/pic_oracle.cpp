@@ -217,7 +217,9 @@ double erfinv(double y) {
 }
 
 // ------------------------------------------------------------------------------------------
-// FFT: mixed-radix decimation-in-time DFT (restates FFTW's unnormalised forward / normalised inverse)
+// FFT: mixed-radix decimation-in-time DFT (restates FFTW's unnormalised forward / normalised inverse).
+// Complex products are the algebraic formula (compiled with -fcx-limited-range), as Julia's
+// *(::Complex{Float64}, ::Complex{Float64}).
 // ------------------------------------------------------------------------------------------
 struct FFTPlan {
     int n = 0;
@@ -235,8 +237,8 @@ struct FFTPlan {
             tw_fwd[k] = cd((double)cosl(a), (double)sinl(a));
         }
     }
-    cd tw(int64_t k, int sign) const {
-        cd t = tw_fwd[(size_t)(k % n)];
+    cd tw(int64_t k, int sign) const {   // k < n for every call below
+        cd t = tw_fwd[(size_t)k];
         return sign < 0 ? t : std::conj(t);
     }
     void rec(const cd *in, cd *out, int len, int stride, size_t fi, int sign) const {
@@ -249,12 +251,25 @@ struct FFTPlan {
         std::vector<cd> tbig;
         cd *tt = t;
         if (r > 64) { tbig.resize(r); tt = tbig.data(); }
+        const cd imag_unit = sign < 0 ? cd(0.0, -1.0) : cd(0.0, 1.0);   // W_4 for this direction
         for (int k = 0; k < m; ++k) {
             for (int q = 0; q < r; ++q) tt[q] = out[(size_t)q * m + k] * tw((int64_t)q * k * step, sign);
-            for (int s = 0; s < r; ++s) {
-                cd acc = tt[0];
-                for (int q = 1; q < r; ++q) acc += tt[q] * tw((int64_t)((q * s) % r) * (n / r), sign);
-                out[(size_t)k + (size_t)m * s] = acc;
+            if (r == 2) {
+                out[(size_t)k] = tt[0] + tt[1];
+                out[(size_t)k + m] = tt[0] - tt[1];
+            } else if (r == 4) {
+                cd s02 = tt[0] + tt[2], d02 = tt[0] - tt[2];
+                cd s13 = tt[1] + tt[3], d13 = (tt[1] - tt[3]) * imag_unit;
+                out[(size_t)k] = s02 + s13;
+                out[(size_t)k + m] = d02 + d13;
+                out[(size_t)k + 2 * m] = s02 - s13;
+                out[(size_t)k + 3 * m] = d02 - d13;
+            } else {
+                for (int s = 0; s < r; ++s) {
+                    cd acc = tt[0];
+                    for (int q = 1; q < r; ++q) acc += tt[q] * tw((int64_t)((q * s) % r) * (n / r), sign);
+                    out[(size_t)k + (size_t)m * s] = acc;
+                }
             }
         }
     }

@@ -47,9 +47,11 @@ def flat_beams(oracle, n1=30000, n2=12000, seed=3):
     return e, p
 
 
-def coords_close(a, b, tol=1e-10):
+def coords_close(a, b, tol=1e-10, floor=None):
     for k in range(6):
         scale = np.max(np.abs(a[k]))
+        if floor is not None:
+            scale = max(scale, floor[k])
         err = np.max(np.abs(a[k] - b[k]))
         assert err <= tol * scale, (k, err, scale)
         # the reference contract's per-component form, atol = rtol = 1e-10
@@ -66,7 +68,7 @@ def first_use_pairs(order):
     return first
 
 
-def compare_collide(oracle, gpu, solver, e, p, collides=1, turn_map=None):
+def compare_collide(oracle, gpu, solver, e, p, collides=1, turn_map=None, floor=None):
     ce, cp = e.copy(), p.copy()
     ocol = oracle.Collider(solver)
     gcol = PICCollider(solver)
@@ -106,8 +108,8 @@ def compare_collide(oracle, gpu, solver, e, p, collides=1, turn_map=None):
         assert (so["hits"], so["misses"], so["rebuilds"]) == (sg["green_hits"], sg["green_misses"], sg["green_rebuilds"])
         assert so["dropped"] == sg["dropped"]
         assert math.isclose(lo, lg, rel_tol=1e-10)
-        coords_close(ce, be.to_host())
-        coords_close(cp, bp.to_host())
+        coords_close(ce, be.to_host(), floor=floor)
+        coords_close(cp, bp.to_host(), floor=floor)
         if turn_map is not None:
             me, mp = turn_map
             gcol.track_linear6d(be, me)
@@ -271,7 +273,10 @@ def test_degenerate_extent(oracle, gpu):
     with pytest.raises(DegenerateExtentError):
         PICCollider(PICPoissonSolver(grid=(16, 16), slicing=sl)).collide(Beam.from_host(e, PE, device=gpu),
                                                                          Beam.from_host(p, PP, device=gpu))
-    compare_collide(oracle, gpu, PICPoissonSolver(grid=(16, 16), slicing=sl, min_transverse_extent=(1e-4, 0.0)), e, p)
+    # x and px start at exactly 0: what the kick leaves there (~1e-20) is round-off of a field that vanishes by
+    # symmetry, so that axis is compared against the transverse scale of the other axis
+    compare_collide(oracle, gpu, PICPoissonSolver(grid=(16, 16), slicing=sl, min_transverse_extent=(1e-4, 0.0)), e, p,
+                    floor=(1e-4, 1e-4, 0, 0, 0, 0))
 
 
 def test_unsupported_combinations_are_rejected(gpu):
