@@ -185,9 +185,14 @@ int octopic_generate_gaussian_beam(octopic_handle *h, uint64_t seed, uint64_t rn
 int octopic_debug_weights(octopic_handle *h, int32_t method, const double *u, int64_t n, int32_t npts,
                           int32_t *base, double *w);
 
-/* Mean duration in ms of every kernel family of the last collide (HIP events on the handle's stream),
- * recorded only after octopic_set_timing(h, 1).  names: NUL-separated family names. */
+/* Per-kernel-family device time, from HIP events recorded on the handle's stream around each launch
+ * group.  octopic_set_timing(h, 1) times every family, (h, 0) none; octopic_set_timing_families
+ * times only the comma-separated families named (e.g. "kick"), which keeps the event overhead out of
+ * the other stages.  Either call resets the accumulators.  octopic_kernel_times returns, per family,
+ * the total ms and the number of timed launch groups since then; names: NUL-separated family names.
+ * Diagnostics of this library (no counterpart in the reference). */
 int octopic_set_timing(octopic_handle *h, int32_t enabled);
+int octopic_set_timing_families(octopic_handle *h, const char *families);
 int octopic_kernel_times(const octopic_handle *h, double *ms_total, int64_t *launches, int64_t cap,
                          int64_t *n, char *names, int64_t names_cap);
 

@@ -78,6 +78,40 @@ __host__ __device__ __forceinline__ void stencil(double u, int n, int &base0, do
     if (METHOD == 0) cic_weights(u, n, base0, w); else tsc_weights(u, n, base0, w);
 }
 
+// Order-preserving uint64 key of a non-NaN double (-0.0 sorts below +0.0, as Julia's min/max), so that
+// exact min/max reductions can use 64-bit integer atomics in any order.
+__host__ __device__ __forceinline__ unsigned long long order_key(double x) {
+    unsigned long long u = (unsigned long long)__builtin_bit_cast(long long, x);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__host__ __device__ __forceinline__ double key_value(unsigned long long k) {
+    unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __builtin_bit_cast(double, (long long)u);
+}
+
+// Bound slots for exact min/max folds with 64-bit atomicMax: a max slot holds order_key(v), a min slot
+// ~order_key(v); NaN is all-ones in both, so it wins every max as NaN wins Julia's min/max; zero is
+// the empty slot.
+__device__ __forceinline__ unsigned long long bound_slot(double v, bool is_min) {
+    if (v != v) return ~0ull;
+    const unsigned long long k = order_key(v);
+    return is_min ? ~k : k;
+}
+__device__ __forceinline__ double bound_value(unsigned long long k, bool is_min) {
+    if (k == ~0ull) return __builtin_nan("");
+    if (k == 0ull) return is_min ? INFINITY : -INFINITY;
+    return key_value(is_min ? ~k : k);
+}
+
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs (each with its own L2).  A 1-D
+// grid padded to a multiple of 8 and read through this map gives every XCD one contiguous range of
+// logical block ids, so the blocks that share a table (one interaction) share an L2.
+constexpr int kXCDs = 8;
+__host__ __device__ __forceinline__ int xcd_padded(int nblocks) { return (nblocks + kXCDs - 1) / kXCDs * kXCDs; }
+__device__ __forceinline__ int xcd_logical_block(int dispatched, int padded) {
+    return (dispatched % kXCDs) * (padded / kXCDs) + dispatched / kXCDs;
+}
+
 // ------------------------------------------------------------------ descriptors in HBM
 struct Grid { double x0, y0, width, height; };
 

@@ -1,8 +1,7 @@
 // fft.h -- in-LDS mixed-radix Stockham FFT for the padded PIC meshes (P = 2g points per axis).
 //
-// A group of `nthr` lanes transforms one P-point complex FP64 sequence held in LDS, ping-ponging
-// between two LDS buffers; every group of a workgroup runs the same stage sequence, so the
-// workgroup barrier between stages is uniform.  Stage s of radix r reads r inputs P/r apart,
+// One wave (64 lanes) transforms one P-point complex FP64 sequence held in LDS, ping-ponging between
+// two LDS buffers, with the twiddle table staged in LDS once per workgroup.  Stage s of radix r reads r inputs P/r apart,
 // twiddles them by W_P^(k*q*P/(Ns*r)) (k = j mod Ns, Ns = product of the earlier radices), applies
 // the r-point DFT and writes them Ns apart at (j/Ns)*Ns*r + k: the autosort (Stockham) form, so
 // the result is in natural order without a bit-reversal pass.
@@ -60,39 +59,78 @@ __device__ __forceinline__ void dft3(double2 *v) {
     v[2] = csub(t2, t3);
 }
 
+// Orders this wave's earlier LDS writes before its later LDS reads (a transform group is one wave, so
+// no workgroup barrier is needed between its stages).
+__device__ __forceinline__ void wave_lds_sync() {
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <int SIGN>
 __device__ __forceinline__ double2 twiddle(const double2 *tw, int t) {
     double2 w = tw[t];
     return SIGN < 0 ? w : cconj(w);
 }
 
-// Transforms the sequence in `a` (LDS) using `b` as scratch; returns the buffer holding the result.
-// Must be called by every lane of the workgroup (it contains workgroup barriers).
+// Copies the plan's twiddles into LDS (`tw_lds`, P entries); the caller barriers afterwards.
+__device__ __forceinline__ void load_twiddles(const FFTPlanDev &P, double2 *tw_lds) {
+    for (int t = threadIdx.x; t < P.n; t += blockDim.x) tw_lds[t] = P.tw[t];
+}
+
+// Transforms the sequence in `a` (LDS) using `b` as scratch; the result is in `a` when the plan has an
+// even number of stages, else in `b` (see fft_result).  Called by the 64 lanes of one wave, which own
+// `a` and `b`; `tw` is the LDS twiddle table.  The caller orders its writes of `a` before the call
+// (wave_lds_sync or a workgroup barrier).
 template <int SIGN>
-__device__ double2 *fft_lds(double2 *a, double2 *b, const FFTPlanDev &P, int lane, int nthr) {
+__device__ void fft_lds(double2 *a, double2 *b, const FFTPlanDev &P, const double2 *tw, int lane) {
     int Ns = 1;
     const int n = P.n;
     for (int s = 0; s < P.nstages; ++s) {
         const int r = P.radix[s];
         const int m = n / r;
         const int tstep = n / (Ns * r);
-        for (int j = lane; j < m; j += nthr) {
-            const int q0 = j / Ns;
-            const int k = j - q0 * Ns;
-            double2 v[4];
-            v[0] = a[j];
-            for (int q = 1; q < r; ++q) v[q] = cmul(a[j + q * m], twiddle<SIGN>(P.tw, k * q * tstep));
-            if (r == 4) dft4<SIGN>(v);
-            else if (r == 2) dft2<SIGN>(v);
-            else dft3<SIGN>(v);
-            const int base = q0 * Ns * r + k;
-            for (int q = 0; q < r; ++q) b[base + q * Ns] = v[q];
+        if (r == 4) {
+            for (int j = lane; j < m; j += 64) {
+                const int q0 = j / Ns, k = j - q0 * Ns;
+                double2 v[4];
+                v[0] = a[j];
+                v[1] = cmul(a[j + m], twiddle<SIGN>(tw, k * tstep));
+                v[2] = cmul(a[j + 2 * m], twiddle<SIGN>(tw, 2 * k * tstep));
+                v[3] = cmul(a[j + 3 * m], twiddle<SIGN>(tw, 3 * k * tstep));
+                dft4<SIGN>(v);
+                const int base = q0 * Ns * 4 + k;
+                b[base] = v[0]; b[base + Ns] = v[1]; b[base + 2 * Ns] = v[2]; b[base + 3 * Ns] = v[3];
+            }
+        } else if (r == 2) {
+            for (int j = lane; j < m; j += 64) {
+                const int q0 = j / Ns, k = j - q0 * Ns;
+                double2 v[2];
+                v[0] = a[j];
+                v[1] = cmul(a[j + m], twiddle<SIGN>(tw, k * tstep));
+                dft2<SIGN>(v);
+                const int base = q0 * Ns * 2 + k;
+                b[base] = v[0]; b[base + Ns] = v[1];
+            }
+        } else {
+            for (int j = lane; j < m; j += 64) {
+                const int q0 = j / Ns, k = j - q0 * Ns;
+                double2 v[3];
+                v[0] = a[j];
+                v[1] = cmul(a[j + m], twiddle<SIGN>(tw, k * tstep));
+                v[2] = cmul(a[j + 2 * m], twiddle<SIGN>(tw, 2 * k * tstep));
+                dft3<SIGN>(v);
+                const int base = q0 * Ns * 3 + k;
+                b[base] = v[0]; b[base + Ns] = v[1]; b[base + 2 * Ns] = v[2];
+            }
         }
-        __syncthreads();
+        wave_lds_sync();
         double2 *t = a; a = b; b = t;
         Ns *= r;
     }
-    return a;
+}
+
+__device__ __forceinline__ double2 *fft_result(double2 *a, double2 *b, const FFTPlanDev &P) {
+    return (P.nstages & 1) ? b : a;
 }
 
 }  // namespace octopic

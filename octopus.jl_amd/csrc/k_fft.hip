@@ -54,13 +54,33 @@ __device__ __forceinline__ double green_cell(int i, int j, int nx, int ny, int i
     return -0.5 * log(x * x + y * y);
 }
 
-// Unpack the FFT Z of (a + i b), a and b real, into A[k] and B[k] for k in [0, P/2].
-__device__ __forceinline__ void unpack_pair(const double2 *Z, int P, int k, double2 &A, double2 &B) {
-    double2 z = Z[k];
-    double2 zc = cconj(Z[k == 0 ? 0 : P - k]);
-    A = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y));
-    double2 d = csub(z, zc);
-    B = make_double2(0.5 * d.y, -0.5 * d.x);   // d / (2i)
+// Unpack the FFT Z of (a + i b), a and b real, into A[k] (odd = 0) or B[k] (odd = 1), k in [0, P/2].
+__device__ __forceinline__ double2 unpack_one(const double2 *Z, int P, int k, int odd) {
+    const double2 z = Z[k];
+    const double2 zc = cconj(Z[k == 0 ? 0 : P - k]);
+    if (!odd) return make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y));
+    const double2 d = csub(z, zc);
+    return make_double2(0.5 * d.y, -0.5 * d.x);   // d / (2i)
+}
+
+// Dynamic LDS of every transform kernel: the twiddle table (P entries), then per wave group two
+// P-entry buffers a, b.
+__device__ __forceinline__ double2 *group_buffer(double2 *lds, int P, int grp) { return lds + P + (size_t)grp * 2 * P; }
+
+// Row-pass epilogue shared by the Green and charge row transforms: the block's 2*kFFTPerBlock rows
+// (ya0 + r, r = 2*grp + {0, 1}) of the x-half-spectrum, stored as out[k * kstride + ya0 + r] so that
+// the 2*kFFTPerBlock consecutive rows of one k form one contiguous run.
+__device__ __forceinline__ void store_row_spectra(double2 *lds, const FFTPlanDev &plan, int nrows, int ya0, double2 *out,
+                                                  int64_t kstride) {
+    constexpr int R = 2 * kFFTPerBlock;
+    const int P = plan.n, nk = P / 2 + 1;
+    for (int t = threadIdx.x; t < nk * R; t += blockDim.x) {
+        const int r = t % R, k = t / R;
+        if (ya0 + r >= nrows) continue;
+        double2 *ga = group_buffer(lds, P, r >> 1);
+        const double2 *Z = fft_result(ga, ga + P, plan);
+        out[(size_t)k * kstride + ya0 + r] = unpack_one(Z, P, k, r & 1);
+    }
 }
 
 // Green rows: rows (ya, ya+1) of the real table -> x-half-spectrum into slot[kx][y], y in [0, 2ny).
@@ -73,28 +93,21 @@ __global__ __launch_bounds__(kFFTThreads) void k_green_rows(const GreenJob *jobs
     if (!job.valid || job.slot < 0 || job.slot >= nslots) return;
     const int P1 = 2 * nx, P2 = 2 * ny;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
-    double2 *a = lds + (size_t)grp * 2 * P1;
-    double2 *b = a + P1;
-    const int ya = 2 * (blockIdx.x * kFFTPerBlock + grp);
-    const bool active = ya < P2;
+    double2 *a = group_buffer(lds, P1, grp);
+    const int ya0 = 2 * kFFTPerBlock * blockIdx.x, ya = ya0 + 2 * grp;
+    load_twiddles(px, lds);
     for (int x = lane; x < P1; x += kFFTLanes) {
         double va = 0.0, vb = 0.0;
-        if (active) {
+        if (ya < P2) {
             va = green_cell(x, ya, nx, ny, integrated, job);
             vb = green_cell(x, ya + 1, nx, ny, integrated, job);
         }
         a[x] = make_double2(va, vb);
     }
     __syncthreads();
-    double2 *Z = fft_lds<-1>(a, b, px, lane, kFFTLanes);
-    if (!active) return;
-    double2 *out = spec + (size_t)job.slot * slot_stride;
-    for (int k = lane; k <= nx; k += kFFTLanes) {
-        double2 A, B;
-        unpack_pair(Z, P1, k, A, B);
-        out[(size_t)k * P2 + ya] = A;
-        out[(size_t)k * P2 + ya + 1] = B;
-    }
+    fft_lds<-1>(a, a + P1, px, lds, lane);
+    __syncthreads();
+    store_row_spectra(lds, px, P2, ya0, spec + (size_t)job.slot * slot_stride, P2);
 }
 
 // Green columns: in-place FFT along y of slot[kx][0..2ny).
@@ -106,15 +119,16 @@ __global__ __launch_bounds__(kFFTThreads) void k_green_cols(const GreenJob *jobs
     if (!job.valid || job.slot < 0 || job.slot >= nslots) return;
     const int P2 = 2 * ny;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
-    double2 *a = lds + (size_t)grp * 2 * P2;
-    double2 *b = a + P2;
+    double2 *a = group_buffer(lds, P2, grp);
     const int kx = blockIdx.x * kFFTPerBlock + grp;
     const bool active = kx <= nx;
     double2 *col = spec + (size_t)job.slot * slot_stride + (size_t)(active ? kx : 0) * P2;
+    load_twiddles(py, lds);
     for (int y = lane; y < P2; y += kFFTLanes) a[y] = active ? col[y] : make_double2(0.0, 0.0);
     __syncthreads();
-    double2 *Z = fft_lds<-1>(a, b, py, lane, kFFTLanes);
+    fft_lds<-1>(a, a + P2, py, lds, lane);
     if (!active) return;
+    const double2 *Z = fft_result(a, a + P2, py);
     for (int y = lane; y < P2; y += kFFTLanes) col[y] = Z[y];
 }
 
@@ -128,20 +142,20 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_fwd(const Plane *planes, u
     const int plane = blockIdx.y;
     const int P1 = 2 * nx;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
-    double2 *a = lds + (size_t)grp * 2 * P1;
-    double2 *b = a + P1;
-    const int ya = 2 * (blockIdx.x * kFFTPerBlock + grp);
+    double2 *a = group_buffer(lds, P1, grp);
+    const int ya0 = 2 * kFFTPerBlock * blockIdx.x, ya = ya0 + 2 * grp;
     const bool active = ya < ny;
     const bool hasb = ya + 1 < ny;
     unsigned long long *q = charge + (size_t)plane * charge_stride;
+    load_twiddles(px, lds);
     for (int x = lane; x < P1; x += kFFTLanes) {
         double va = 0.0, vb = 0.0;
         if (active && x < nx) {
-            unsigned long long ua = q[(size_t)ya * nx + x];
+            const unsigned long long ua = q[(size_t)ya * nx + x];
             va = (double)ua * inv_scale;
             if (ua) q[(size_t)ya * nx + x] = 0ull;
             if (hasb) {
-                unsigned long long ub = q[(size_t)(ya + 1) * nx + x];
+                const unsigned long long ub = q[(size_t)(ya + 1) * nx + x];
                 vb = (double)ub * inv_scale;
                 if (ub) q[(size_t)(ya + 1) * nx + x] = 0ull;
             }
@@ -149,15 +163,9 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_fwd(const Plane *planes, u
         a[x] = make_double2(va, vb);
     }
     __syncthreads();
-    double2 *Z = fft_lds<-1>(a, b, px, lane, kFFTLanes);
-    if (!active) return;
-    double2 *out = qh + (size_t)plane * qh_stride;
-    for (int k = lane; k <= nx; k += kFFTLanes) {
-        double2 A, B;
-        unpack_pair(Z, P1, k, A, B);
-        out[(size_t)k * ny + ya] = A;
-        if (hasb) out[(size_t)k * ny + ya + 1] = B;
-    }
+    fft_lds<-1>(a, a + P1, px, lds, lane);
+    __syncthreads();
+    store_row_spectra(lds, px, ny, ya0, qh + (size_t)plane * qh_stride, ny);
 }
 
 // Column pass: FFT_y of the zero-padded column, times the Green half spectrum, IFFT_y, keep y < ny.
@@ -169,25 +177,28 @@ __global__ __launch_bounds__(kFFTThreads) void k_cols_conv(const Plane *planes, 
     const int plane = blockIdx.y;
     const int P2 = 2 * ny;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
-    double2 *a = lds + (size_t)grp * 2 * P2;
-    double2 *b = a + P2;
+    double2 *a = group_buffer(lds, P2, grp), *b = a + P2;
     const int kx = blockIdx.x * kFFTPerBlock + grp;
     const bool active = kx <= nx;
     double2 *col = qh + (size_t)plane * qh_stride + (size_t)(active ? kx : 0) * ny;
+    load_twiddles(py, lds);
     for (int y = lane; y < P2; y += kFFTLanes) a[y] = (active && y < ny) ? col[y] : make_double2(0.0, 0.0);
     __syncthreads();
-    double2 *Z = fft_lds<-1>(a, b, py, lane, kFFTLanes);
-    double2 *W = (Z == a) ? b : a;
+    fft_lds<-1>(a, b, py, lds, lane);
+    double2 *Z = fft_result(a, b, py), *W = Z == a ? b : a;
     const int slot = planes[plane].slot;
     const double2 *g = gspec + (size_t)(slot < 0 ? 0 : slot) * slot_stride + (size_t)(active ? kx : 0) * P2;
     for (int y = lane; y < P2; y += kFFTLanes) Z[y] = cmul(Z[y], g[y]);
-    __syncthreads();
-    double2 *R = fft_lds<+1>(Z, W, py, lane, kFFTLanes);
+    wave_lds_sync();
+    fft_lds<+1>(Z, W, py, lds, lane);
     if (!active) return;
+    const double2 *R = fft_result(Z, W, py);
     for (int y = lane; y < ny; y += kFFTLanes) col[y] = R[y];
 }
 
-// Row inverse: rows (ya, ya+1) from the half spectrum, IFFT_x, keep x < nx, scale 1/(P1 P2).
+// Row inverse: rows (ya, ya+1) from the half spectrum, IFFT_x, keep x < nx, scale 1/(P1 P2).  The
+// block gathers its 2*kFFTPerBlock rows together: thread (kk, grp) reads rows ya, ya+1 of column kk
+// (adjacent threads adjacent rows) and fills both kk and its Hermitian mirror P1 - kk.
 __global__ __launch_bounds__(kFFTThreads) void k_rows_inv(const Plane *planes, const double2 *qh, int64_t qh_stride, double *phi,
                                                           int nx, int ny, FFTPlanDev px, const ErrorInfo *err) {
     extern __shared__ double2 lds[];
@@ -195,26 +206,30 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_inv(const Plane *planes, c
     const int plane = blockIdx.y;
     const int P1 = 2 * nx, P2 = 2 * ny;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
-    double2 *a = lds + (size_t)grp * 2 * P1;
-    double2 *b = a + P1;
-    const int ya = 2 * (blockIdx.x * kFFTPerBlock + grp);
-    const bool active = ya < ny;
-    const bool hasb = ya + 1 < ny;
+    const int ya0 = 2 * kFFTPerBlock * blockIdx.x;
     const double2 *in = qh + (size_t)plane * qh_stride;
-    for (int k = lane; k < P1; k += kFFTLanes) {
-        double2 z = make_double2(0.0, 0.0);
-        if (active) {
-            int kk = k <= nx ? k : P1 - k;
-            double2 A = in[(size_t)kk * ny + ya];
-            double2 B = hasb ? in[(size_t)kk * ny + ya + 1] : make_double2(0.0, 0.0);
-            if (k > nx) { A = cconj(A); B = cconj(B); }
-            z = make_double2(A.x - B.y, A.y + B.x);   // A + i B
+    load_twiddles(px, lds);
+    for (int t = threadIdx.x; t < (nx + 1) * kFFTPerBlock; t += blockDim.x) {
+        const int g = t % kFFTPerBlock, kk = t / kFFTPerBlock;
+        const int y = ya0 + 2 * g;
+        double2 A = make_double2(0.0, 0.0), B = A;
+        if (y < ny) A = in[(size_t)kk * ny + y];
+        if (y + 1 < ny) B = in[(size_t)kk * ny + y + 1];
+        double2 *ag = group_buffer(lds, P1, g);
+        ag[kk] = make_double2(A.x - B.y, A.y + B.x);   // A + i B
+        if (kk > 0 && kk < nx) {
+            A = cconj(A);
+            B = cconj(B);
+            ag[P1 - kk] = make_double2(A.x - B.y, A.y + B.x);
         }
-        a[k] = z;
     }
     __syncthreads();
-    double2 *Z = fft_lds<+1>(a, b, px, lane, kFFTLanes);
-    if (!active) return;
+    double2 *a = group_buffer(lds, P1, grp);
+    fft_lds<+1>(a, a + P1, px, lds, lane);
+    const int ya = ya0 + 2 * grp;
+    if (ya >= ny) return;
+    const bool hasb = ya + 1 < ny;
+    const double2 *Z = fft_result(a, a + P1, px);
     const double norm = 1.0 / ((double)P1 * (double)P2);
     double *out = phi + (size_t)plane * nx * ny;
     for (int x = lane; x < nx; x += kFFTLanes) {
@@ -225,41 +240,76 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_inv(const Plane *planes, c
 
 // E = -grad(phi) of every plane of an interaction into the interleaved kick table
 // table[inter][x][y][3*q + {phi, Ex, Ey}] (pic_cpu.jl:2531-2591; h of the plane's deposit mesh).
+// A block takes a kFieldTile x kFieldTile cell tile of one interaction: phi with a 2-cell halo is
+// staged in LDS from contiguous rows of phi[y][x], and the tile's table entries are written back as
+// one contiguous run per x.
+constexpr int kFieldTile = 16;
+constexpr int kFieldHalo = kFieldTile + 4;
+
 __global__ __launch_bounds__(256) void k_field(const Inter *inters, const Plane *planes, const double *phi, double *table,
                                                int nx, int ny, int fourth, int npl, const ErrorInfo *err) {
     if (failed(err)) return;
     const int d = blockIdx.y;
-    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= nx * ny) return;
-    const int x = cell / ny, y = cell % ny;     // table order: x outer, y inner
     const Inter it = inters[d];
     if (!it.valid) return;
-    double *out = table + ((size_t)d * nx * ny + cell) * (3 * npl);
+    __shared__ double ph[kFieldHalo][kFieldHalo + 1];                 // [y][x], origin (tx0 - 2, ty0 - 2)
+    __shared__ double stage[kFieldTile * kFieldTile * 9];             // [lx][ly][3q + c]
+    const int ntx = (nx + kFieldTile - 1) / kFieldTile;
+    const int tx0 = (blockIdx.x % ntx) * kFieldTile, ty0 = (blockIdx.x / ntx) * kFieldTile;
+    const int C = 3 * npl;
+    const int lx = threadIdx.x % kFieldTile, ly = threadIdx.x / kFieldTile;
+    const int x = tx0 + lx, y = ty0 + ly;
+    const bool inside = x < nx && y < ny;
     const double c4 = 1.0 / 12.0;
-    for (int q = 0; q < it.nplanes; ++q) {
-        const Plane &pl = planes[it.plane0 + q];
-        const double *P = phi + (size_t)(it.plane0 + q) * nx * ny;   // [y][x]
-        auto F = [&](int i, int j) { return P[(size_t)j * nx + i]; };
-        const double hxi = 1.0 / pl.hx, hyi = 1.0 / pl.hy;
-        double ex, ey;
-        if (y == 0) ey = hyi * (1.5 * F(x, 0) - 2.0 * F(x, 1) + 0.5 * F(x, 2));
-        else if (y == ny - 1) ey = hyi * (-1.5 * F(x, ny - 1) + 2.0 * F(x, ny - 2) - 0.5 * F(x, ny - 3));
-        else if (fourth && (y == 1 || y == ny - 2)) ey = 0.5 * hyi * (F(x, y - 1) - F(x, y + 1));
-        else if (fourth) ey = c4 * hyi * ((F(x, y + 2) - F(x, y - 2)) + 8.0 * (F(x, y - 1) - F(x, y + 1)));
-        else ey = 0.5 * hyi * (F(x, y - 1) - F(x, y + 1));
-        if (x == 0) ex = hxi * (1.5 * F(0, y) - 2.0 * F(1, y) + 0.5 * F(2, y));
-        else if (x == nx - 1) ex = hxi * (-1.5 * F(nx - 1, y) + 2.0 * F(nx - 2, y) - 0.5 * F(nx - 3, y));
-        else if (fourth && (x == 1 || x == nx - 2)) ex = 0.5 * hxi * (F(x - 1, y) - F(x + 1, y));
-        else if (fourth) ex = c4 * hxi * ((F(x + 2, y) - F(x - 2, y)) + 8.0 * (F(x - 1, y) - F(x + 1, y)));
-        else ex = 0.5 * hxi * (F(x - 1, y) - F(x + 1, y));
-        out[3 * q + 0] = F(x, y);
-        out[3 * q + 1] = ex;
-        out[3 * q + 2] = ey;
+    for (int q = 0; q < npl; ++q) {
+        const bool live = q < it.nplanes;
+        if (live) {
+            const double *P = phi + (size_t)(it.plane0 + q) * nx * ny;   // [y][x]
+            for (int k = threadIdx.x; k < kFieldHalo * kFieldHalo; k += blockDim.x) {
+                const int hy = k / kFieldHalo, hx = k % kFieldHalo;
+                const int gx = tx0 - 2 + hx, gy = ty0 - 2 + hy;
+                ph[hy][hx] = (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? P[(size_t)gy * nx + gx] : 0.0;
+            }
+        }
+        __syncthreads();
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        if (live && inside) {
+            const Plane &pl = planes[it.plane0 + q];
+            auto F = [&](int i, int j) { return ph[j - ty0 + 2][i - tx0 + 2]; };
+            const double hxi = 1.0 / pl.hx, hyi = 1.0 / pl.hy;
+            double ex, ey;
+            if (y == 0) ey = hyi * (1.5 * F(x, 0) - 2.0 * F(x, 1) + 0.5 * F(x, 2));
+            else if (y == ny - 1) ey = hyi * (-1.5 * F(x, ny - 1) + 2.0 * F(x, ny - 2) - 0.5 * F(x, ny - 3));
+            else if (fourth && (y == 1 || y == ny - 2)) ey = 0.5 * hyi * (F(x, y - 1) - F(x, y + 1));
+            else if (fourth) ey = c4 * hyi * ((F(x, y + 2) - F(x, y - 2)) + 8.0 * (F(x, y - 1) - F(x, y + 1)));
+            else ey = 0.5 * hyi * (F(x, y - 1) - F(x, y + 1));
+            if (x == 0) ex = hxi * (1.5 * F(0, y) - 2.0 * F(1, y) + 0.5 * F(2, y));
+            else if (x == nx - 1) ex = hxi * (-1.5 * F(nx - 1, y) + 2.0 * F(nx - 2, y) - 0.5 * F(nx - 3, y));
+            else if (fourth && (x == 1 || x == nx - 2)) ex = 0.5 * hxi * (F(x - 1, y) - F(x + 1, y));
+            else if (fourth) ex = c4 * hxi * ((F(x + 2, y) - F(x - 2, y)) + 8.0 * (F(x - 1, y) - F(x + 1, y)));
+            else ex = 0.5 * hxi * (F(x - 1, y) - F(x + 1, y));
+            v0 = F(x, y);
+            v1 = ex;
+            v2 = ey;
+        }
+        double *sv = stage + (lx * kFieldTile + ly) * 9 + 3 * q;
+        sv[0] = v0; sv[1] = v1; sv[2] = v2;
+        __syncthreads();
+    }
+    // one contiguous run of kFieldTile * C doubles per x of the tile
+    const int ylen = (ny - ty0) < kFieldTile ? (ny - ty0) : kFieldTile;
+    const int run = ylen * C;
+    for (int k = threadIdx.x; k < kFieldTile * run; k += blockDim.x) {
+        const int tlx = k / run, rem = k - tlx * run;
+        const int tly = rem / C, comp = rem - tly * C;
+        const int gx = tx0 + tlx;
+        if (gx >= nx) continue;
+        table[(((size_t)d * nx + gx) * ny + ty0 + tly) * C + comp] = stage[(tlx * kFieldTile + tly) * 9 + comp];
     }
 }
 
 // ---------------------------------------------------------------------- launchers
-static size_t fft_lds_bytes(int P) { return (size_t)kFFTPerBlock * 2 * P * sizeof(double2); }
+static size_t fft_lds_bytes(int P) { return (size_t)(1 + kFFTPerBlock * 2) * P * sizeof(double2); }
 
 static hipError_t ensure_lds(const void *fn, size_t bytes) {
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -272,7 +322,7 @@ hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, in
     const int P1 = 2 * nx, P2 = 2 * ny;
     ensure_lds((const void *)k_green_rows, fft_lds_bytes(P1));
     ensure_lds((const void *)k_green_cols, fft_lds_bytes(P2));
-    dim3 g1((P2 / 2 + kFFTPerBlock - 1) / kFFTPerBlock, njobs);
+    dim3 g1((P2 + 2 * kFFTPerBlock - 1) / (2 * kFFTPerBlock), njobs);
     hipLaunchKernelGGL(k_green_rows, g1, dim3(kFFTThreads), fft_lds_bytes(P1), s, jobs, njobs, spec, slot_stride, nslots, nx,
                        ny, integrated, px, err);
     dim3 g2((nx + 1 + kFFTPerBlock - 1) / kFFTPerBlock, njobs);
@@ -286,7 +336,7 @@ hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long 
                            hipStream_t s) {
     if (nplanes <= 0) return hipSuccess;
     ensure_lds((const void *)k_rows_fwd, fft_lds_bytes(2 * nx));
-    dim3 g(((ny + 1) / 2 + kFFTPerBlock - 1) / kFFTPerBlock, nplanes);
+    dim3 g((ny + 2 * kFFTPerBlock - 1) / (2 * kFFTPerBlock), nplanes);
     hipLaunchKernelGGL(k_rows_fwd, g, dim3(kFFTThreads), fft_lds_bytes(2 * nx), s, planes, charge, charge_stride, qh,
                        qh_stride, nx, ny, inv_scale, px, err);
     return hipGetLastError();
@@ -306,7 +356,7 @@ hipError_t launch_rows_inv(const Plane *planes, int nplanes, const double2 *qh, 
                            int ny, const FFTPlanDev &px, const ErrorInfo *err, hipStream_t s) {
     if (nplanes <= 0) return hipSuccess;
     ensure_lds((const void *)k_rows_inv, fft_lds_bytes(2 * nx));
-    dim3 g(((ny + 1) / 2 + kFFTPerBlock - 1) / kFFTPerBlock, nplanes);
+    dim3 g((ny + 2 * kFFTPerBlock - 1) / (2 * kFFTPerBlock), nplanes);
     hipLaunchKernelGGL(k_rows_inv, g, dim3(kFFTThreads), fft_lds_bytes(2 * nx), s, planes, qh, qh_stride, phi, nx, ny, px, err);
     return hipGetLastError();
 }
@@ -314,8 +364,9 @@ hipError_t launch_rows_inv(const Plane *planes, int nplanes, const double2 *qh, 
 hipError_t launch_field(const Inter *inters, int ninters, const Plane *planes, const double *phi, double *table, int nx,
                         int ny, int fourth, int npl, const ErrorInfo *err, hipStream_t s) {
     if (ninters <= 0) return hipSuccess;
-    dim3 g((nx * ny + 255) / 256, ninters);
-    hipLaunchKernelGGL(k_field, g, dim3(256), 0, s, inters, planes, phi, table, nx, ny, fourth, npl, err);
+    const int ntiles = ((nx + kFieldTile - 1) / kFieldTile) * ((ny + kFieldTile - 1) / kFieldTile);
+    hipLaunchKernelGGL(k_field, dim3(ntiles, ninters), dim3(kFieldTile * kFieldTile), 0, s, inters, planes, phi, table, nx,
+                       ny, fourth, npl, err);
     return hipGetLastError();
 }
 

@@ -27,8 +27,22 @@ __device__ __forceinline__ void set_error(ErrorInfo *err, int code, int p, int d
 }
 
 // ---------------------------------------------------------------------------------- bounds
+// Each segment (pair, side) owns 12 uint64 slots: q even = min, q odd = max of
+//   0-3  source box at the field-slice boundary drifts sL, sR      (x min/max, y min/max)
+//   4-7  field box at the per-particle drift 1/2(z - c_other)
+//   8-11 luminosity box at sM
+// A max slot holds order_key(v); a min slot holds ~order_key(v); NaN is all-ones in both and so wins
+// every max, as NaN wins Julia's min/max.  Zero is the empty value.  Blocks fold their values in with
+// 64-bit atomicMax (exact and order-free), and the geometry stage reads each slot and zeroes it.
 constexpr int kBoundsThreads = 256;
 
+__device__ __forceinline__ double take_bound(unsigned long long *slot, bool is_min) {
+    const double v = bound_value(*slot, is_min);
+    *slot = 0ull;
+    return v;
+}
+
+template <bool SRCFLD, bool LUM>
 __global__ __launch_bounds__(kBoundsThreads) void k_bounds(BoundsArgs a) {
     if (failed(a.err)) return;
     const int seg = blockIdx.y, k = seg >> 1, side = seg & 1;
@@ -47,43 +61,70 @@ __global__ __launch_bounds__(kBoundsThreads) void k_bounds(BoundsArgs a) {
     double v[12];
     for (int q = 0; q < 12; q += 2) { v[q] = INFINITY; v[q + 1] = -INFINITY; }
     for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
-        const double x = X[t], px = PX[t], y = Y[t], py = PY[t], z = Z[t];
-        const double xl = x + px * sL, yl = y + py * sL, xr = x + px * sR, yr = y + py * sR;
-        v[0] = jmin(jmin(v[0], xl), xr); v[1] = jmax(jmax(v[1], xl), xr);
-        v[2] = jmin(jmin(v[2], yl), yr); v[3] = jmax(jmax(v[3], yl), yr);
-        const double s = 0.5 * (z - c_oth);
-        const double xf = x + s * px, yf = y + s * py;
-        v[4] = jmin(v[4], xf); v[5] = jmax(v[5], xf); v[6] = jmin(v[6], yf); v[7] = jmax(v[7], yf);
-        const double xv = x + px * sM, yv = y + py * sM;
-        v[8] = jmin(v[8], xv); v[9] = jmax(v[9], xv); v[10] = jmin(v[10], yv); v[11] = jmax(v[11], yv);
+        const double x = X[t], px = PX[t], y = Y[t], py = PY[t];
+        if (SRCFLD) {
+            const double z = Z[t];
+            const double xl = x + px * sL, yl = y + py * sL, xr = x + px * sR, yr = y + py * sR;
+            v[0] = jmin(jmin(v[0], xl), xr); v[1] = jmax(jmax(v[1], xl), xr);
+            v[2] = jmin(jmin(v[2], yl), yr); v[3] = jmax(jmax(v[3], yl), yr);
+            const double s = 0.5 * (z - c_oth);
+            const double xf = x + s * px, yf = y + s * py;
+            v[4] = jmin(v[4], xf); v[5] = jmax(v[5], xf); v[6] = jmin(v[6], yf); v[7] = jmax(v[7], yf);
+        }
+        if (LUM) {
+            const double xv = x + px * sM, yv = y + py * sM;
+            v[8] = jmin(v[8], xv); v[9] = jmax(v[9], xv); v[10] = jmin(v[10], yv); v[11] = jmax(v[11], yv);
+        }
     }
-    __shared__ double red[12][kBoundsThreads];
-    for (int q = 0; q < 12; ++q) red[q][threadIdx.x] = v[q];
+    constexpr int q0 = SRCFLD ? 0 : 8, q1 = LUM ? 12 : 8;
+    constexpr int kWaves = kBoundsThreads / 64;
+    __shared__ unsigned long long red[kWaves][12];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = q0; q < q1; ++q) {
+        unsigned long long u = bound_slot(v[q], (q & 1) == 0);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long w = __shfl_xor(u, o);
+            u = w > u ? w : u;
+        }
+        if (lane == 0) red[wave][q] = u;
+    }
     __syncthreads();
-    for (int w = kBoundsThreads / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int q = 0; q < 12; q += 2) {
-                red[q][threadIdx.x] = jmin(red[q][threadIdx.x], red[q][threadIdx.x + w]);
-                red[q + 1][threadIdx.x] = jmax(red[q + 1][threadIdx.x], red[q + 1][threadIdx.x + w]);
-            }
-        __syncthreads();
+    const int q = q0 + (int)threadIdx.x;
+    if (q < q1) {
+        unsigned long long u = red[0][q];
+        for (int w = 1; w < kWaves; ++w) u = red[w][q] > u ? red[w][q] : u;
+        if (u) atomicMax(&a.keys[(size_t)seg * kBoundSlots + q], u);
     }
-    if (threadIdx.x < 12) a.partial[((size_t)seg * a.nchunks + blockIdx.x) * 12 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 hipError_t launch_bounds(const BoundsArgs &a, hipStream_t s) {
-    if (a.npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bounds, dim3(a.nchunks, 2 * a.npairs), dim3(kBoundsThreads), 0, s, a);
+    if (a.npairs <= 0 || !(a.srcfld || a.lum)) return hipSuccess;
+    const dim3 g(a.nchunks, 2 * a.npairs), b(kBoundsThreads);
+    if (a.srcfld && a.lum) hipLaunchKernelGGL((k_bounds<true, true>), g, b, 0, s, a);
+    else if (a.srcfld) hipLaunchKernelGGL((k_bounds<true, false>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_bounds<false, true>), g, b, 0, s, a);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------------- geometry
-__device__ __forceinline__ void reduce_partials(const double *partial, int seg, int nchunks, double v[12]) {
-    for (int q = 0; q < 12; q += 2) { v[q] = INFINITY; v[q + 1] = -INFINITY; }
-    for (int c = 0; c < nchunks; ++c) {
-        const double *p = partial + ((size_t)seg * nchunks + c) * 12;
-        for (int q = 0; q < 12; q += 2) { v[q] = jmin(v[q], p[q]); v[q + 1] = jmax(v[q + 1], p[q + 1]); }
+// Lane (k, dir) takes the slots it consumes: the source box (0-3) of the source side and the field box
+// (4-7) of the field side; lane dir 1 also takes both luminosity boxes (8-11).  Every slot of the batch
+// is thereby zeroed exactly once, before any early return.
+__device__ __forceinline__ void take_boxes(unsigned long long *keys, int k, int dir, double vs[4], double vf[4],
+                                           double l1[4], double l2[4]) {
+    unsigned long long *s0 = keys + (size_t)(2 * k) * kBoundSlots, *s1 = s0 + kBoundSlots;
+    unsigned long long *src = dir == 1 ? s0 : s1, *fld = dir == 1 ? s1 : s0;
+    for (int q = 0; q < 4; ++q) {
+        vs[q] = take_bound(src + q, (q & 1) == 0);
+        vf[q] = take_bound(fld + 4 + q, (q & 1) == 0);
     }
+    if (dir == 1)
+        for (int q = 0; q < 4; ++q) {
+            l1[q] = take_bound(s0 + 8 + q, (q & 1) == 0);
+            l2[q] = take_bound(s1 + 8 + q, (q & 1) == 0);
+        }
 }
 
 __device__ __forceinline__ bool resolve_extent(double xspan, double yspan, double fx, double fy, double &w, double &h) {
@@ -184,11 +225,13 @@ __device__ void set_plane(Plane &pl, int inter, int kind, int beam, int n1, int 
     pl.bx1 = 0.0; pl.by1 = 0.0;
 }
 
-// Luminosity mesh of one pair (pic_cpu.jl:2700-2717) and its two deposit planes.
-__device__ __noinline__ bool lum_geometry(const GeomOpts &o, const double *v1, const double *v2, int k, int p, const SliceTable *t1,
-                             const SliceTable *t2, int i, int j, Plane *planes, int lum_plane0, double *lumgeo, ErrorInfo *err) {
-    double xmin = jmin(v1[8], v2[8]), xmax = jmax(v1[9], v2[9]);
-    double ymin = jmin(v1[10], v2[10]), ymax = jmax(v1[11], v2[11]);
+// Luminosity mesh of one pair (pic_cpu.jl:2700-2717) and its two deposit planes.  l1, l2 are the
+// luminosity boxes (x min, x max, y min, y max) of slice i of beam 1 and slice j of beam 2 at sM.
+__device__ __noinline__ bool lum_geometry(const GeomOpts &o, const double *l1, const double *l2, int k, int p,
+                                          const SliceTable *t1, const SliceTable *t2, int i, int j, Plane *planes,
+                                          int lum_plane0, double *lumgeo, ErrorInfo *err) {
+    double xmin = jmin(l1[0], l2[0]), xmax = jmax(l1[1], l2[1]);
+    double ymin = jmin(l1[2], l2[2]), ymax = jmax(l1[3], l2[3]);
     double width, height;
     if (!resolve_extent(xmax - xmin, ymax - ymin, o.min_x, o.min_y, width, height)) {
         set_error(err, OCTOPIC_ERR_DEGENERATE_EXTENT, p, 0, 2, xmax - xmin, ymax - ymin);
@@ -213,133 +256,136 @@ __device__ __noinline__ bool lum_geometry(const GeomOpts &o, const double *v1, c
     return true;
 }
 
-// Every descriptor a pair owns is made inert before anything can return early, so that no later
-// stage of the batch can read a stale or uninitialised descriptor.
-__device__ void neutralise_pair(int k, int npairs, const GeomOpts &o, Inter *inters, Plane *planes, GreenJob *jobs) {
-    for (int dir = 0; dir < 2; ++dir) {
-        const int d = 2 * k + dir;
-        inters[d].valid = 0;
-        inters[d].fld_n = 0;
-        inters[d].nplanes = 0;
-        if (jobs) { jobs[d].valid = 0; jobs[d].slot = -1; }
-        for (int q = 0; q < o.npl; ++q) {
-            Plane &pl = planes[d * o.npl + q];
+// Every descriptor an interaction owns is made inert before anything can return early, so that no
+// later stage of the batch can read a stale or uninitialised descriptor.  Lane dir 1 of a pair also
+// owns the pair's two luminosity planes.
+__device__ void neutralise_inter(int k, int dir, int npairs, const GeomOpts &o, Inter *inters, Plane *planes,
+                                 GreenJob *jobs) {
+    const int d = 2 * k + dir - 1;
+    inters[d].valid = 0;
+    inters[d].fld_n = 0;
+    inters[d].nplanes = 0;
+    if (jobs) { jobs[d].valid = 0; jobs[d].slot = -1; }
+    for (int q = 0; q < o.npl; ++q) {
+        Plane &pl = planes[d * o.npl + q];
+        pl.n = 0; pl.off = 0; pl.beam = 0; pl.slot = -1; pl.count_box = 0;
+    }
+    if (dir == 1) {
+        const int lum0 = 2 * npairs * o.npl;
+        for (int side = 0; side < 2; ++side) {
+            Plane &pl = planes[lum0 + 2 * k + side];
             pl.n = 0; pl.off = 0; pl.beam = 0; pl.slot = -1; pl.count_box = 0;
         }
     }
-    const int lum0 = 2 * npairs * o.npl;
-    for (int side = 0; side < 2; ++side) {
-        Plane &pl = planes[lum0 + 2 * k + side];
-        pl.n = 0; pl.off = 0; pl.beam = 0; pl.slot = -1; pl.count_box = 0;
-    }
 }
 
+__device__ void fill_inter(Inter &it, const GeomOpts &o, int p, int i, int j, int dir, const SliceTable *ts,
+                           const SliceTable *tf, int si, int fi) {
+    const double c_src = ts->center[si];
+    it.p = p; it.i = i; it.j = j; it.dir = dir;
+    it.src_beam = dir == 1 ? 0 : 1;
+    it.fld_beam = dir == 1 ? 1 : 0;
+    it.src_off = ts->offset[si]; it.src_n = ts->count[si];
+    it.fld_off = tf->offset[fi]; it.fld_n = tf->count[fi];
+    it.c_src = c_src; it.c_fld = tf->center[fi];
+    it.lb = tf->boundary[fi]; it.rb = tf->boundary[fi + 1];
+    it.sL = 0.5 * (c_src - it.lb);
+    it.sR = 0.5 * (c_src - it.rb);
+    it.sM = 0.5 * (c_src - it.c_fld);
+    it.kick_scale = 2.0 * (dir == 1 ? o.kbb2 : o.kbb1);
+}
+
+// one lane per (pair k, direction): 2*npairs lanes
 __global__ void k_geometry(GeometryArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.npairs) return;
-    neutralise_pair(k, a.npairs, a.o, a.inters, a.planes, a.jobs);
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= 2 * a.npairs) return;
+    const int k = lane >> 1, dir = (lane & 1) + 1, d = lane;
+    neutralise_inter(k, dir, a.npairs, a.o, a.inters, a.planes, a.jobs);
+    double vs[4], vf[4], l1[4], l2[4];   // source box, field box, luminosity boxes
+    take_boxes(a.keys, k, dir, vs, vf, l1, l2);
     if (failed(a.err)) return;
     const GeomOpts &o = a.o;
     const BatchPair pr = a.pairs[k];
     const int i = pr.i, j = pr.j, p = pr.p;
-    const SliceTable *t1 = a.tab1, *t2 = a.tab2;
-    double v1[12], v2[12];   // side 0 = slice i of beam 1, side 1 = slice j of beam 2
-    reduce_partials(a.partial, 2 * k, a.nchunks, v1);
-    reduce_partials(a.partial, 2 * k + 1, a.nchunks, v2);
-    for (int dir = 1; dir <= 2; ++dir) {
-        const int d = 2 * k + dir - 1;
-        const double *vs = dir == 1 ? v1 : v2;   // source slice bounds
-        const double *vf = dir == 1 ? v2 : v1;   // field slice bounds
-        const SliceTable *ts = dir == 1 ? t1 : t2, *tf = dir == 1 ? t2 : t1;
-        const int si = dir == 1 ? i : j, fi = dir == 1 ? j : i;
-        Inter &it = a.inters[d];
-        it.valid = 0;
-        if (!finite4(vs[0], vs[1], vs[2], vs[3])) { set_error(a.err, OCTOPIC_ERR_NONFINITE, p, dir, 0, 0, 0); return; }
-        if (!finite4(vf[4], vf[5], vf[6], vf[7])) { set_error(a.err, OCTOPIC_ERR_NONFINITE, p, dir, 1, 0, 0); return; }
-        Grid sg, fg;
-        if (!interaction_grids(o, vs[0], vs[1], vs[2], vs[3], vf[4], vf[5], vf[6], vf[7], sg, fg)) {
-            set_error(a.err, OCTOPIC_ERR_DEGENERATE_EXTENT, p, dir, 3, jmax(vs[1] - vs[0], vf[5] - vf[4]),
-                      jmax(vs[3] - vs[2], vf[7] - vf[6]));
-            return;
-        }
-        int event = 0, slot, build;
-        if (o.green_cache == OCTOPIC_CACHE_SLICE_PAIR) {
-            const int key = (i * o.ns2 + j) * 2 + (dir - 1);
-            CacheGeo &e = a.cache[key];
-            bool usable = false;
-            if (e.valid)
-                usable = size_usable(e.sg, sg, o.min_ratio) && size_usable(e.fg, fg, o.min_ratio) &&
-                         covers(o, e.sg, vs[0], vs[1], vs[2], vs[3]) && covers(o, e.fg, vf[4], vf[5], vf[6], vf[7]);
-            slot = key;
-            if (usable) {
-                event = 1;
-                build = 0;
-                sg = e.sg;
-                fg = e.fg;
-                atomicAdd(&a.counters[0], 1ull);
-            } else {
-                Grid es = expand_grid(sg, 1.0 + o.growth);
-                Grid ef = expand_grid(fg, 1.0 + o.growth);
-                if (o.green_type != OCTOPIC_GREEN_INTEGRATED) {   // _pic_realign_expanded_grids (pic_cpu.jl:1849-1858)
-                    const double hx = es.width / (double)(o.nx - 1);
-                    const double hy = es.height / (double)(o.ny - 1);
-                    double sx0, fx0, sy0, fy0;
-                    align_origins(o.green_type, es.x0, ef.x0, hx, sx0, fx0);
-                    align_origins(o.green_type, es.y0, ef.y0, hy, sy0, fy0);
-                    es.x0 = sx0; es.y0 = sy0; ef.x0 = fx0; ef.y0 = fy0;
-                }
-                event = e.valid ? 3 : 2;
-                atomicAdd(&a.counters[e.valid ? 2 : 1], 1ull);
-                e.valid = 1;
-                e.sg = es;
-                e.fg = ef;
-                sg = es;
-                fg = ef;
-                build = 1;
-            }
+    const SliceTable *ts = dir == 1 ? a.tab1 : a.tab2, *tf = dir == 1 ? a.tab2 : a.tab1;
+    const int si = dir == 1 ? i : j, fi = dir == 1 ? j : i;
+    Inter &it = a.inters[d];
+    if (!finite4(vs[0], vs[1], vs[2], vs[3])) { set_error(a.err, OCTOPIC_ERR_NONFINITE, p, dir, 0, 0, 0); return; }
+    if (!finite4(vf[0], vf[1], vf[2], vf[3])) { set_error(a.err, OCTOPIC_ERR_NONFINITE, p, dir, 1, 0, 0); return; }
+    Grid sg, fg;
+    if (!interaction_grids(o, vs[0], vs[1], vs[2], vs[3], vf[0], vf[1], vf[2], vf[3], sg, fg)) {
+        set_error(a.err, OCTOPIC_ERR_DEGENERATE_EXTENT, p, dir, 3, jmax(vs[1] - vs[0], vf[1] - vf[0]),
+                  jmax(vs[3] - vs[2], vf[3] - vf[2]));
+        return;
+    }
+    int event = 0, slot, build;
+    if (o.green_cache == OCTOPIC_CACHE_SLICE_PAIR) {
+        const int key = (i * o.ns2 + j) * 2 + (dir - 1);
+        CacheGeo &e = a.cache[key];
+        bool usable = false;
+        if (e.valid)
+            usable = size_usable(e.sg, sg, o.min_ratio) && size_usable(e.fg, fg, o.min_ratio) &&
+                     covers(o, e.sg, vs[0], vs[1], vs[2], vs[3]) && covers(o, e.fg, vf[0], vf[1], vf[2], vf[3]);
+        slot = key;
+        if (usable) {
+            event = 1;
+            build = 0;
+            sg = e.sg;
+            fg = e.fg;
+            atomicAdd(&a.counters[0], 1ull);
         } else {
-            slot = o.temp_slot0 + d;
+            Grid es = expand_grid(sg, 1.0 + o.growth);
+            Grid ef = expand_grid(fg, 1.0 + o.growth);
+            if (o.green_type != OCTOPIC_GREEN_INTEGRATED) {   // _pic_realign_expanded_grids (pic_cpu.jl:1849-1858)
+                const double hx = es.width / (double)(o.nx - 1);
+                const double hy = es.height / (double)(o.ny - 1);
+                double sx0, fx0, sy0, fy0;
+                align_origins(o.green_type, es.x0, ef.x0, hx, sx0, fx0);
+                align_origins(o.green_type, es.y0, ef.y0, hy, sy0, fy0);
+                es.x0 = sx0; es.y0 = sy0; ef.x0 = fx0; ef.y0 = fy0;
+            }
+            event = e.valid ? 3 : 2;
+            atomicAdd(&a.counters[e.valid ? 2 : 1], 1ull);
+            e.valid = 1;
+            e.sg = es;
+            e.fg = ef;
+            sg = es;
+            fg = ef;
             build = 1;
         }
-        const double hx = sg.width / (double)(o.nx - 1);
-        const double hy = sg.height / (double)(o.ny - 1);
-        GreenJob &job = a.jobs[d];
-        job.slot = slot;
-        job.valid = build;
-        job.fx0 = fg.x0; job.fy0 = fg.y0; job.sx0 = sg.x0; job.sy0 = sg.y0; job.hx = hx; job.hy = hy;
-        if (build) atomicAdd(&a.counters[3], 1ull);
-        const double c_src = ts->center[si];
-        it.p = p; it.i = i; it.j = j; it.dir = dir;
-        it.src_beam = dir == 1 ? 0 : 1;
-        it.fld_beam = dir == 1 ? 1 : 0;
-        it.src_off = ts->offset[si]; it.src_n = ts->count[si];
-        it.fld_off = tf->offset[fi]; it.fld_n = tf->count[fi];
-        it.c_src = c_src; it.c_fld = tf->center[fi];
-        it.lb = tf->boundary[fi]; it.rb = tf->boundary[fi + 1];
-        it.sL = 0.5 * (c_src - it.lb);
-        it.sR = 0.5 * (c_src - it.rb);
-        it.sM = 0.5 * (c_src - it.c_fld);
-        it.kick_scale = 2.0 * (dir == 1 ? o.kbb2 : o.kbb1);
-        it.sg = sg; it.fg = fg; it.fgR = fg;
-        it.plane0 = d * o.npl;
-        it.nplanes = o.npl;
-        it.event = event;
-        it.valid = 1;
-        it.count_field = 0;
-        const int src_beam = it.src_beam;
-        set_plane(a.planes[d * o.npl + 0], d, 0, src_beam, o.nx, o.ny, slot, it.src_off, it.src_n, it.sL, sg.x0, sg.y0, hx, hy);
-        set_plane(a.planes[d * o.npl + 1], d, 0, src_beam, o.nx, o.ny, slot, it.src_off, it.src_n, it.sR, sg.x0, sg.y0, hx, hy);
-        if (o.npl == 3)
-            set_plane(a.planes[d * o.npl + 2], d, 0, src_beam, o.nx, o.ny, slot, it.src_off, it.src_n, 0.5 * (it.sL + it.sR),
-                      sg.x0, sg.y0, hx, hy);
-        fill_record(a.records[2 * p + dir - 1], p, i, j, dir, event, sg, fg);
+    } else {
+        slot = o.temp_slot0 + d;
+        build = 1;
     }
-    if (o.compute_lum) lum_geometry(o, v1, v2, k, p, t1, t2, i, j, a.planes, 2 * a.npairs * o.npl, a.lumgeo, a.err);
+    const double hx = sg.width / (double)(o.nx - 1);
+    const double hy = sg.height / (double)(o.ny - 1);
+    GreenJob &job = a.jobs[d];
+    job.slot = slot;
+    job.valid = build;
+    job.fx0 = fg.x0; job.fy0 = fg.y0; job.sx0 = sg.x0; job.sy0 = sg.y0; job.hx = hx; job.hy = hy;
+    if (build) atomicAdd(&a.counters[3], 1ull);
+    fill_inter(it, o, p, i, j, dir, ts, tf, si, fi);
+    it.sg = sg; it.fg = fg; it.fgR = fg;
+    it.plane0 = d * o.npl;
+    it.nplanes = o.npl;
+    it.event = event;
+    it.count_field = 0;
+    const int src_beam = it.src_beam;
+    set_plane(a.planes[d * o.npl + 0], d, 0, src_beam, o.nx, o.ny, slot, it.src_off, it.src_n, it.sL, sg.x0, sg.y0, hx, hy);
+    set_plane(a.planes[d * o.npl + 1], d, 0, src_beam, o.nx, o.ny, slot, it.src_off, it.src_n, it.sR, sg.x0, sg.y0, hx, hy);
+    if (o.npl == 3)
+        set_plane(a.planes[d * o.npl + 2], d, 0, src_beam, o.nx, o.ny, slot, it.src_off, it.src_n, 0.5 * (it.sL + it.sR),
+                  sg.x0, sg.y0, hx, hy);
+    fill_record(a.records[2 * p + dir - 1], p, i, j, dir, event, sg, fg);
+    it.valid = 1;
+    // after this lane's own checks, which see every x, px, y, py of both slices
+    if (dir == 1 && o.compute_lum)
+        lum_geometry(o, l1, l2, k, p, a.tab1, a.tab2, i, j, a.planes, 2 * a.npairs * o.npl, a.lumgeo, a.err);
 }
 
 hipError_t launch_geometry(const GeometryArgs &a, hipStream_t s) {
     if (a.npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_geometry, dim3((a.npairs + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_geometry, dim3((2 * a.npairs + 63) / 64), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
@@ -411,12 +457,15 @@ constexpr int kKickThreads = 256;
 template <int METHOD, int MODE>
 __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     if (failed(a.err)) return;
-    const Inter it = a.inters[blockIdx.y];
+    const int lblk = xcd_logical_block(blockIdx.x, gridDim.x);
+    if (lblk >= a.chunks * a.ninters) return;
+    const int d = lblk / a.chunks, chunk = lblk - d * a.chunks;
+    const Inter it = a.inters[d];
     if (!it.valid) return;
     const int64_t n = it.fld_n;
-    const int64_t lo = n * blockIdx.x / a.chunks, hi = n * (blockIdx.x + 1) / a.chunks;
+    const int64_t lo = n * chunk / a.chunks, hi = n * (chunk + 1) / a.chunks;
     const int nx = a.nx, ny = a.ny, C = 3 * a.npl;
-    const double *tab = a.table + (size_t)blockIdx.y * nx * ny * C;
+    const double *tab = a.table + (size_t)d * nx * ny * C;
     double *X = a.st.c[it.fld_beam][0] + it.fld_off, *PX = a.st.c[it.fld_beam][1] + it.fld_off;
     double *Y = a.st.c[it.fld_beam][2] + it.fld_off, *PY = a.st.c[it.fld_beam][3] + it.fld_off;
     const double *Z = a.st.c[it.fld_beam][4] + it.fld_off;
@@ -526,7 +575,7 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
 
 hipError_t launch_kick(const KickArgs &a, hipStream_t s) {
     if (a.ninters <= 0) return hipSuccess;
-    dim3 g(a.chunks, a.ninters), b(kKickThreads);
+    dim3 g(xcd_padded(a.chunks * a.ninters)), b(kKickThreads);
 #define OCTOPIC_KICK(M, MODE) hipLaunchKernelGGL((k_kick<M, MODE>), g, b, 0, s, a)
     if (a.method == 0) {
         if (a.mode == 0) OCTOPIC_KICK(0, 0); else if (a.mode == 1) OCTOPIC_KICK(0, 1); else OCTOPIC_KICK(0, 2);
@@ -538,36 +587,53 @@ hipError_t launch_kick(const KickArgs &a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------------- luminosity
+// Grid (kLumBlocks, npairs): block b of pair k sums a fixed cell range; the block that finishes last
+// (per-pair arrival counter, reset by that block) adds the kLumBlocks partials in block order, so the
+// result does not depend on scheduling.
 constexpr int kLumThreads = 256;
 
 __global__ __launch_bounds__(kLumThreads) void k_lum(LumArgs a) {
-    const int k = blockIdx.x;
+    const int k = blockIdx.y, b = blockIdx.x;
     unsigned long long *q1 = a.charge + (size_t)(2 * k) * a.charge_stride;
     unsigned long long *q2 = a.charge + (size_t)(2 * k + 1) * a.charge_stride;
     const double inv = a.inv_scale;
+    const int c0 = (int)((int64_t)a.ncells * b / kLumBlocks), c1 = (int)((int64_t)a.ncells * (b + 1) / kLumBlocks);
     double acc = 0.0;
-    for (int c = threadIdx.x; c < a.ncells; c += blockDim.x) {
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
         const unsigned long long u1 = q1[c], u2 = q2[c];
         if (u1) q1[c] = 0ull;
         if (u2) q2[c] = 0ull;
         acc += ((double)u1 * inv) * ((double)u2 * inv);
     }
     __shared__ double red[kLumThreads];
+    __shared__ int last;
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int w = kLumThreads / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0 && !failed(a.err)) {
+    if (threadIdx.x == 0) {
+        a.lum_partial[k * kLumBlocks + b] = red[0];
+        __threadfence();
+        last = atomicAdd(&a.lum_arrivals[k], 1u) == kLumBlocks - 1;
+    }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    const volatile double *part = a.lum_partial + k * kLumBlocks;
+    double sum = 0.0;
+    for (int q = 0; q < kLumBlocks; ++q) sum += part[q];
+    a.lum_arrivals[k] = 0u;
+    if (!failed(a.err)) {
         const double *g = a.lumgeo + 3 * k;
-        a.lum_parts[a.pairs[k].p] = red[0] * g[2] * g[0] * g[1];
+        a.lum_parts[a.pairs[k].p] = sum * g[2] * g[0] * g[1];
     }
 }
 
 hipError_t launch_lum(const LumArgs &a, hipStream_t s) {
     if (a.npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_lum, dim3(a.npairs), dim3(kLumThreads), 0, s, a);
+    hipLaunchKernelGGL(k_lum, dim3(kLumBlocks, a.npairs), dim3(kLumThreads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -717,68 +783,55 @@ __device__ void set_node_plane(Plane &pl, int d, int beam, const GeomOpts &o, in
     pl.by1 = g.y0 + g.height;
 }
 
-// _pic_interaction_node! descriptors (pic_cpu.jl:1448-1538) for every pair of a batch
+// _pic_interaction_node! descriptors (pic_cpu.jl:1448-1538): one lane per (pair, direction).  A pair
+// runs only when the node meshes of both its directions exist (pic_cpu.jl:1465-1470).
 __global__ void k_node_geometry(NodeGeometryArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.npairs) return;
-    neutralise_pair(k, a.npairs, a.o, a.inters, a.planes, nullptr);
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= 2 * a.npairs) return;
+    const int k = lane >> 1, dir = (lane & 1) + 1, d = lane;
+    neutralise_inter(k, dir, a.npairs, a.o, a.inters, a.planes, nullptr);
+    double vs[4], vf[4], l1[4], l2[4];
+    take_boxes(a.keys, k, dir, vs, vf, l1, l2);
     if (failed(a.err)) return;
     const GeomOpts &o = a.o;
     const BatchPair pr = a.pairs[k];
     const int i = pr.i, j = pr.j, p = pr.p;
-    const SliceTable *t1 = a.tab1, *t2 = a.tab2;
-    for (int dir = 1; dir <= 2; ++dir) {
-        const int d = 2 * k + dir - 1;
-        Inter &it = a.inters[d];
-        it.valid = 0;
-        const SliceTable *ts = dir == 1 ? t1 : t2, *tf = dir == 1 ? t2 : t1;
-        const int si = dir == 1 ? i : j, fi = dir == 1 ? j : i;
-        const int nbF = tf->ns + 1;
-        const CacheGeo *nodes = dir == 1 ? a.nodes1 : a.nodes2;
-        const int slot0 = dir == 1 ? a.slot1 : a.slot2;
-        const int kl = si * nbF + fi, kr = si * nbF + fi + 1;
-        const CacheGeo gL = nodes[kl], gR = nodes[kr];
-        if (!gL.valid || !gR.valid) {
-            neutralise_pair(k, a.npairs, o, a.inters, a.planes, nullptr);
-            a.ran[p] = 0;
-            return;
-        }
-        const double c_src = ts->center[si];
-        it.p = p; it.i = i; it.j = j; it.dir = dir;
-        it.src_beam = dir == 1 ? 0 : 1;
-        it.fld_beam = dir == 1 ? 1 : 0;
-        it.src_off = ts->offset[si]; it.src_n = ts->count[si];
-        it.fld_off = tf->offset[fi]; it.fld_n = tf->count[fi];
-        it.c_src = c_src; it.c_fld = tf->center[fi];
-        it.lb = tf->boundary[fi]; it.rb = tf->boundary[fi + 1];
-        it.sL = 0.5 * (c_src - it.lb);
-        it.sR = 0.5 * (c_src - it.rb);
-        it.sM = 0.5 * (c_src - it.c_fld);
-        it.kick_scale = 2.0 * (dir == 1 ? o.kbb2 : o.kbb1);
-        it.sg = gL.sg; it.fg = gL.fg; it.fgR = gR.fg;
-        it.plane0 = d * o.npl;
-        it.nplanes = o.lkick ? 3 : 2;       // F_Z only with the longitudinal kick (pic_cpu.jl:1497-1503)
-        it.event = 0;
-        it.count_field = 1;
-        it.valid = 1;
-        const int sb = it.src_beam;
-        set_node_plane(a.planes[d * o.npl + 0], d, sb, o, slot0 + kl, it.src_off, it.src_n, it.sL, gL.sg, 1);
-        set_node_plane(a.planes[d * o.npl + 1], d, sb, o, slot0 + kr, it.src_off, it.src_n, it.sR, gR.sg, 1);
-        set_node_plane(a.planes[d * o.npl + 2], d, sb, o, slot0 + kl, it.src_off, it.src_n, it.sR, gL.sg, 0);
-        fill_record(a.records[2 * p + dir - 1], p, i, j, dir, 0, gL.sg, gL.fg);
+    bool ready = true;
+    for (int dd = 1; dd <= 2; ++dd) {
+        const int si = dd == 1 ? i : j, fi = dd == 1 ? j : i;
+        const int nbF = (dd == 1 ? a.tab2 : a.tab1)->ns + 1;
+        const CacheGeo *nodes = dd == 1 ? a.nodes1 : a.nodes2;
+        ready = ready && nodes[si * nbF + fi].valid && nodes[si * nbF + fi + 1].valid;
     }
-    a.ran[p] = 1;
-    if (o.compute_lum) {
-        double v1[12], v2[12];
-        reduce_partials(a.partial, 2 * k, a.nchunks, v1);
-        reduce_partials(a.partial, 2 * k + 1, a.nchunks, v2);
-        lum_geometry(o, v1, v2, k, p, t1, t2, i, j, a.planes, 2 * a.npairs * o.npl, a.lumgeo, a.err);
-    }
+    if (dir == 1) a.ran[p] = ready ? 1 : 0;
+    if (!ready) return;
+    const SliceTable *ts = dir == 1 ? a.tab1 : a.tab2, *tf = dir == 1 ? a.tab2 : a.tab1;
+    const int si = dir == 1 ? i : j, fi = dir == 1 ? j : i;
+    const int nbF = tf->ns + 1;
+    const CacheGeo *nodes = dir == 1 ? a.nodes1 : a.nodes2;
+    const int slot0 = dir == 1 ? a.slot1 : a.slot2;
+    const int kl = si * nbF + fi, kr = si * nbF + fi + 1;
+    const CacheGeo gL = nodes[kl], gR = nodes[kr];
+    Inter &it = a.inters[d];
+    fill_inter(it, o, p, i, j, dir, ts, tf, si, fi);
+    it.sg = gL.sg; it.fg = gL.fg; it.fgR = gR.fg;
+    it.plane0 = d * o.npl;
+    it.nplanes = o.lkick ? 3 : 2;       // F_Z only with the longitudinal kick (pic_cpu.jl:1497-1503)
+    it.event = 0;
+    it.count_field = 1;
+    const int sb = it.src_beam;
+    set_node_plane(a.planes[d * o.npl + 0], d, sb, o, slot0 + kl, it.src_off, it.src_n, it.sL, gL.sg, 1);
+    set_node_plane(a.planes[d * o.npl + 1], d, sb, o, slot0 + kr, it.src_off, it.src_n, it.sR, gR.sg, 1);
+    set_node_plane(a.planes[d * o.npl + 2], d, sb, o, slot0 + kl, it.src_off, it.src_n, it.sR, gL.sg, 0);
+    fill_record(a.records[2 * p + dir - 1], p, i, j, dir, 0, gL.sg, gL.fg);
+    it.valid = 1;
+    if (dir == 1 && o.compute_lum)
+        lum_geometry(o, l1, l2, k, p, a.tab1, a.tab2, i, j, a.planes, 2 * a.npairs * o.npl, a.lumgeo, a.err);
 }
 
 hipError_t launch_node_geometry(const NodeGeometryArgs &a, hipStream_t s) {
     if (a.npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_node_geometry, dim3((a.npairs + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_node_geometry, dim3((2 * a.npairs + 63) / 64), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

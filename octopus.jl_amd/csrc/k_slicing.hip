@@ -12,6 +12,8 @@
 //                  Member lists come out ascending, as the reference's index lists.  The scatter also
 //                  gathers the six coordinates into slice-contiguous state arrays.
 //   centroid     : the same 4096-lane fold over each member list (slicing.jl:245-257, 523-559).
+//   tile order   : members regrouped by transverse tile within each slice (no counterpart in the
+//                  reference: a memory-layout choice for the field gathers; results do not depend on it).
 #include "common.h"
 #include "launch.h"
 
@@ -28,16 +30,17 @@ __global__ __launch_bounds__(256) void k_zlanes(const double *z, int64_t n, cons
     double lo = INFINITY, hi = -INFINITY;
     int64_t i = t;
     // Loads are issued ahead of the in-order accumulation; the summation order is the lane fold's.
-    for (; i + 3 * kFoldLanes < n; i += 4 * kFoldLanes) {
-        double v0 = z[i], v1 = z[i + kFoldLanes], v2 = z[i + 2 * kFoldLanes], v3 = z[i + 3 * kFoldLanes];
-        if (pass == 2) {
-            v0 = (v0 - mu) * (v0 - mu); v1 = (v1 - mu) * (v1 - mu);
-            v2 = (v2 - mu) * (v2 - mu); v3 = (v3 - mu) * (v3 - mu);
-        } else {
-            lo = jmin(jmin(jmin(jmin(lo, v0), v1), v2), v3);
-            hi = jmax(jmax(jmax(jmax(hi, v0), v1), v2), v3);
+    for (; i + 7 * kFoldLanes < n; i += 8 * kFoldLanes) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = z[i + u * kFoldLanes];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (pass == 2) v[u] = (v[u] - mu) * (v[u] - mu);
+            else { lo = jmin(lo, v[u]); hi = jmax(hi, v[u]); }
         }
-        acc += v0; acc += v1; acc += v2; acc += v3;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u];
     }
     for (; i < n; i += kFoldLanes) {
         double v = z[i];
@@ -63,15 +66,24 @@ __global__ __launch_bounds__(256) void k_zlanes(const double *z, int64_t n, cons
 
 // Serial lane fold (one lane) + boundaries.  method: OCTOPIC slicing method; q: normal quantiles
 // sqrt(2)*erfinv(2s/ns - 1) or the sorted `specified` offsets, from the host.
-__global__ void k_zfold(const double *lanes, const double *mm_partial, int64_t n, SliceTable *tab, int pass, int method,
-                        int ns, const double *q, ErrorInfo *err) {
-    if (threadIdx.x != 0) return;
+// The 4096 lane partials are staged in LDS by the whole block, then folded serially in lane order.
+__device__ __forceinline__ double fold_lanes_serial(const double *l) {
     double s = 0.0;
     for (int t = 0; t < kFoldLanes; t += 8) {
-        double l0 = lanes[t], l1 = lanes[t + 1], l2 = lanes[t + 2], l3 = lanes[t + 3];
-        double l4 = lanes[t + 4], l5 = lanes[t + 5], l6 = lanes[t + 6], l7 = lanes[t + 7];
+        const double l0 = l[t], l1 = l[t + 1], l2 = l[t + 2], l3 = l[t + 3];
+        const double l4 = l[t + 4], l5 = l[t + 5], l6 = l[t + 6], l7 = l[t + 7];
         s += l0; s += l1; s += l2; s += l3; s += l4; s += l5; s += l6; s += l7;
     }
+    return s;
+}
+
+__global__ __launch_bounds__(256) void k_zfold(const double *lanes, const double *mm_partial, int64_t n, SliceTable *tab,
+                                               int pass, int method, int ns, const double *q, ErrorInfo *err) {
+    __shared__ double sl[kFoldLanes];
+    for (int t = threadIdx.x; t < kFoldLanes; t += blockDim.x) sl[t] = lanes[t];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double s = fold_lanes_serial(sl);
     if (pass == 1) {
         double lo = INFINITY, hi = -INFINITY;
         for (int b = 0; b < kFoldLanes / 256; ++b) { lo = jmin(lo, mm_partial[2 * b]); hi = jmax(hi, mm_partial[2 * b + 1]); }
@@ -191,8 +203,9 @@ __device__ __forceinline__ int slice_of(double zi, const double *b, int ns) {
     return iclamp(lo, 1, ns) - 1;
 }
 
-__global__ __launch_bounds__(kSliceThreads) void k_classify(const double *z, int64_t n, const SliceTable *tab, uint16_t *sid,
-                                                            int *hist, int nblocks, int by_bin, const ErrorInfo *err) {
+__global__ __launch_bounds__(kSliceThreads) void k_classify(const double *z, const double *x, const double *y, int64_t n,
+                                                            const SliceTable *tab, uint16_t *sid, int *hist, int nblocks,
+                                                            int by_bin, unsigned long long *box, const ErrorInfo *err) {
     __shared__ int cnt[kMaxSlices];
     __shared__ double b[kMaxSlices + 1];
     const int ns = tab->ns;
@@ -201,6 +214,7 @@ __global__ __launch_bounds__(kSliceThreads) void k_classify(const double *z, int
     for (int k = threadIdx.x; k <= ns; k += blockDim.x) b[k] = tab->boundary[k];
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kSliceBlock;
+    double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
     for (int r = 0; r < kSliceBlock / kSliceThreads; ++r) {
         int64_t i = base + r * kSliceThreads + threadIdx.x;
         if (i < n) {
@@ -212,10 +226,32 @@ __global__ __launch_bounds__(kSliceThreads) void k_classify(const double *z, int
             } else s = slice_of(z[i], b, ns);
             sid[i] = (uint16_t)s;
             atomicAdd(&cnt[s], 1);
+            if (box) {
+                const double xi = x[i], yi = y[i];
+                v[0] = jmin(v[0], xi); v[1] = jmax(v[1], xi); v[2] = jmin(v[2], yi); v[3] = jmax(v[3], yi);
+            }
         }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < ns; k += blockDim.x) hist[(size_t)k * nblocks + blockIdx.x] = cnt[k];
+    if (box) {   // the beam's transverse box, for the tile order below
+        __shared__ unsigned long long red[kSliceThreads / 64][4];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int q = 0; q < 4; ++q) {
+            unsigned long long u = bound_slot(v[q], (q & 1) == 0);
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned long long w = __shfl_xor(u, o);
+                u = w > u ? w : u;
+            }
+            if (lane == 0) red[wave][q] = u;
+        }
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            unsigned long long u = 0ull;
+            for (int w = 0; w < kSliceThreads / 64; ++w) u = red[w][threadIdx.x] > u ? red[w][threadIdx.x] : u;
+            if (u) atomicMax(&box[threadIdx.x], u);
+        }
+    }
 }
 
 // Exclusive scan of hist (slice-major, ns * nblocks) in place; writes the per-slice count/offset.
@@ -307,21 +343,20 @@ __global__ __launch_bounds__(256) void k_centroid_lanes(const double *stz, const
     lanes[(size_t)s * kFoldLanes + t] = acc;
 }
 
-__global__ void k_centroid_fold(const double *lanes, SliceTable *tab, int64_t n, int centroid) {
-    const int s = threadIdx.x;
-    const int ns = tab->ns;
-    if (s >= ns) return;
+// one block per slice
+__global__ __launch_bounds__(256) void k_centroid_fold(const double *lanes, SliceTable *tab, int64_t n, int centroid) {
+    __shared__ double sl[kFoldLanes];
+    const int s = blockIdx.x;
     const int64_t cnt = tab->count[s];
+    const bool fold = centroid && cnt > 0;
+    if (fold) {
+        for (int t = threadIdx.x; t < kFoldLanes; t += blockDim.x) sl[t] = lanes[(size_t)s * kFoldLanes + t];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     tab->weight[s] = (double)cnt / (double)n;
-    const double *l = lanes + (size_t)s * kFoldLanes;
-    if (centroid && cnt > 0) {
-        double acc = 0.0;
-        for (int t = 0; t < kFoldLanes; t += 8) {
-            double l0 = l[t], l1 = l[t + 1], l2 = l[t + 2], l3 = l[t + 3];
-            double l4 = l[t + 4], l5 = l[t + 5], l6 = l[t + 6], l7 = l[t + 7];
-            acc += l0; acc += l1; acc += l2; acc += l3; acc += l4; acc += l5; acc += l6; acc += l7;
-        }
-        tab->center[s] = acc / (double)cnt;
+    if (fold) {
+        tab->center[s] = fold_lanes_serial(sl) / (double)cnt;
     } else {
         tab->center[s] = (tab->boundary[s] + tab->boundary[s + 1]) / 2.0;
     }
@@ -341,13 +376,118 @@ __global__ __launch_bounds__(256) void k_store(double *const *beam, int64_t n, d
     }
 }
 
+// ---------------------------------------------------------------------- transverse tile order
+// After the slice scatter and the centroid folds (which need each member list in ascending index
+// order), the members of every slice are regrouped by transverse tile: T x T tiles over the beam's
+// x-y box at slicing time, tile-major within the slice.  The field gathers of the kick then share
+// cache lines across a wave.  Only the order inside a slice changes, and no later result depends on
+// it: deposits are exact integer sums, bounds are exact min/max, kicks are per particle, and the
+// final store goes through the permutation.
+struct TileBox { double x0, y0, sx, sy; };
+
+__device__ __forceinline__ TileBox tile_box(const unsigned long long *box, int T) {
+    const double xmin = bound_value(box[0], true), xmax = bound_value(box[1], false);
+    const double ymin = bound_value(box[2], true), ymax = bound_value(box[3], false);
+    const double wx = xmax - xmin, wy = ymax - ymin;
+    TileBox b;
+    b.x0 = __builtin_isfinite(xmin) ? xmin : 0.0;
+    b.y0 = __builtin_isfinite(ymin) ? ymin : 0.0;
+    b.sx = (__builtin_isfinite(wx) && wx > 0.0) ? (double)T / wx : 0.0;
+    b.sy = (__builtin_isfinite(wy) && wy > 0.0) ? (double)T / wy : 0.0;
+    return b;
+}
+__device__ __forceinline__ int tile_axis(double u, int T) {
+    return u >= 0.0 ? (u < (double)T ? (int)u : T - 1) : 0;   // NaN -> 0
+}
+__device__ __forceinline__ int tile_of(double x, double y, const TileBox &b, int T) {
+    return tile_axis((x - b.x0) * b.sx, T) * T + tile_axis((y - b.y0) * b.sy, T);
+}
+
+constexpr int kTileThreads = 1024;
+
+// grid (G, ns): per-block tile counts, thist[(s * TT + tile) * G + c]
+__global__ __launch_bounds__(kTileThreads) void k_tile_hist(TileSortArgs a) {
+    extern __shared__ int cnt[];
+    const int s = blockIdx.y, c = blockIdx.x, T = a.T, TT = T * T, G = gridDim.x;
+    const TileBox tb = tile_box(a.box, T);
+    for (int t = threadIdx.x; t < TT; t += blockDim.x) cnt[t] = 0;
+    __syncthreads();
+    const int64_t off = a.tab->offset[s], n = a.tab->count[s];
+    const double *X = a.src[0] + off, *Y = a.src[2] + off;
+    for (int64_t base = (int64_t)c * kTileChunk; base < n; base += (int64_t)G * kTileChunk)
+        for (int64_t k = base + threadIdx.x; k < n && k < base + kTileChunk; k += blockDim.x)
+            atomicAdd(&cnt[tile_of(X[k], Y[k], tb, T)], 1);
+    __syncthreads();
+    for (int t = threadIdx.x; t < TT; t += blockDim.x) a.thist[((size_t)s * TT + t) * G + c] = cnt[t];
+}
+
+// one block per slice: exclusive scan of the slice's TT * G counts, plus the slice offset
+__global__ __launch_bounds__(1024) void k_tile_scan(TileSortArgs a, int G) {
+    __shared__ int64_t part[1024];
+    const int s = blockIdx.x;
+    const int64_t total = (int64_t)a.T * a.T * G;
+    int *h = a.thist + (size_t)s * total;
+    const int64_t per = (total + blockDim.x - 1) / blockDim.x;
+    const int64_t lo = threadIdx.x * per < total ? threadIdx.x * per : total;
+    const int64_t hi = lo + per < total ? lo + per : total;
+    int64_t sum = 0;
+    for (int64_t k = lo; k < hi; ++k) sum += h[k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = a.tab->offset[s];
+        for (int t = 0; t < (int)blockDim.x; ++t) { const int64_t v = part[t]; part[t] = run; run += v; }
+    }
+    __syncthreads();
+    int64_t run = part[threadIdx.x];
+    for (int64_t k = lo; k < hi; ++k) {
+        const int v = h[k];
+        h[k] = (int)run;
+        run += v;
+    }
+}
+
+// grid (G, ns): the same visits as k_tile_hist; a member's slot is its block's next free slot in its tile
+__global__ __launch_bounds__(kTileThreads) void k_tile_scatter(TileSortArgs a) {
+    extern __shared__ int cur[];
+    const int s = blockIdx.y, c = blockIdx.x, T = a.T, TT = T * T, G = gridDim.x;
+    const TileBox tb = tile_box(a.box, T);
+    for (int t = threadIdx.x; t < TT; t += blockDim.x) cur[t] = a.thist[((size_t)s * TT + t) * G + c];
+    __syncthreads();
+    const int64_t off = a.tab->offset[s], n = a.tab->count[s];
+    for (int64_t base = (int64_t)c * kTileChunk; base < n; base += (int64_t)G * kTileChunk)
+        for (int64_t k = base + threadIdx.x; k < n && k < base + kTileChunk; k += blockDim.x) {
+            const int64_t i = off + k;
+            const double x = a.src[0][i], y = a.src[2][i];
+            const int64_t pos = atomicAdd(&cur[tile_of(x, y, tb, T)], 1);
+            a.dst[0][pos] = x;
+            a.dst[1][pos] = a.src[1][i];
+            a.dst[2][pos] = y;
+            a.dst[3][pos] = a.src[3][i];
+            a.dst[4][pos] = a.src[4][i];
+            a.dst[5][pos] = a.src[5][i];
+            a.perm_dst[pos] = a.perm_src[i];
+        }
+}
+
+hipError_t launch_tile_sort(const TileSortArgs &a, int G, hipStream_t s) {
+    if (a.ns <= 0 || a.T <= 0 || G <= 0) return hipSuccess;
+    const size_t lds = (size_t)a.T * a.T * sizeof(int);
+    hipFuncSetAttribute((const void *)k_tile_hist, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipFuncSetAttribute((const void *)k_tile_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_tile_hist, dim3(G, a.ns), dim3(kTileThreads), lds, s, a);
+    hipLaunchKernelGGL(k_tile_scan, dim3(a.ns), dim3(1024), 0, s, a, G);
+    hipLaunchKernelGGL(k_tile_scatter, dim3(G, a.ns), dim3(kTileThreads), lds, s, a);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------- launcher
 hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s) {
     const int nblocks = (int)((a.n + kSliceBlock - 1) / kSliceBlock);
     hipLaunchKernelGGL(k_zlanes, dim3(kFoldLanes / 256), dim3(256), 0, s, a.z, a.n, a.tab, 1, a.lanes, a.mm_partial);
-    hipLaunchKernelGGL(k_zfold, dim3(1), dim3(64), 0, s, a.lanes, a.mm_partial, a.n, a.tab, 1, a.method, a.ns, a.q, a.err);
+    hipLaunchKernelGGL(k_zfold, dim3(1), dim3(256), 0, s, a.lanes, a.mm_partial, a.n, a.tab, 1, a.method, a.ns, a.q, a.err);
     hipLaunchKernelGGL(k_zlanes, dim3(kFoldLanes / 256), dim3(256), 0, s, a.z, a.n, a.tab, 2, a.lanes, a.mm_partial);
-    hipLaunchKernelGGL(k_zfold, dim3(1), dim3(64), 0, s, a.lanes, a.mm_partial, a.n, a.tab, 2, a.method, a.ns, a.q, a.err);
+    hipLaunchKernelGGL(k_zfold, dim3(1), dim3(256), 0, s, a.lanes, a.mm_partial, a.n, a.tab, 2, a.method, a.ns, a.q, a.err);
     if (a.method == 0) {
         const int bins = a.ns * a.resolution;
         int g = (int)((a.n + 255) / 256);
@@ -356,13 +496,14 @@ hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s) {
         hipLaunchKernelGGL(k_equal_area_hist, dim3(g), dim3(256), bins * 4, s, a.z, a.n, a.tab, bins, a.bin_counts);
         hipLaunchKernelGGL(k_equal_area, dim3(1), dim3(64), 0, s, a.tab, a.n, a.ns, bins, a.bin_counts, a.err);
     }
-    hipLaunchKernelGGL(k_classify, dim3(nblocks), dim3(kSliceThreads), 0, s, a.z, a.n, a.tab, a.sid, a.hist, nblocks, a.method == 2 ? 1 : 0, a.err);
+    hipLaunchKernelGGL(k_classify, dim3(nblocks), dim3(kSliceThreads), 0, s, a.z, a.x, a.y, a.n, a.tab, a.sid, a.hist, nblocks,
+                       a.method == 2 ? 1 : 0, a.box, a.err);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.hist, a.ns, nblocks, a.tab);
     hipLaunchKernelGGL(k_counts, dim3(1), dim3(256), 0, s, a.tab, a.n);
     hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(kSliceThreads), 0, s, a.beam, a.n, a.sid, a.hist, nblocks, a.ns, a.st,
                        a.perm);
     hipLaunchKernelGGL(k_centroid_lanes, dim3(kFoldLanes / 256, a.ns), dim3(256), 0, s, a.st_z, a.tab, a.centroid_lanes);
-    hipLaunchKernelGGL(k_centroid_fold, dim3(1), dim3(256), 0, s, a.centroid_lanes, a.tab, a.n, a.centroid);
+    hipLaunchKernelGGL(k_centroid_fold, dim3(a.ns), dim3(256), 0, s, a.centroid_lanes, a.tab, a.n, a.centroid);
     return hipGetLastError();
 }
 

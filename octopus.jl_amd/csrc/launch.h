@@ -15,7 +15,7 @@ struct States {
 };
 
 struct SlicingArgs {
-    const double *z;
+    const double *z, *x, *y;         // input beam coordinates (x, y only for the transverse box)
     int64_t n;
     int method, ns, resolution, centroid;
     const double *q;                 // quantiles (normal_quantile) or positions (specified)
@@ -28,9 +28,26 @@ struct SlicingArgs {
     double *const *st;               // device array of 6 device pointers (slice-contiguous state)
     const double *st_z;
     int64_t *perm;
+    unsigned long long *box;         // 4 bound slots (x min/max, y min/max), zero on entry; null: not folded
     ErrorInfo *err;
 };
 hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s);
+
+constexpr int kTileChunk = 4096;     // particles per block visit; block c of a slice takes visits c, c+G, ...
+constexpr int kTileBlockSpan = 16384;   // particles per block and slice the tile order aims for (long tile runs)
+
+// Transverse tile order inside each slice (after slicing, with the slice table on the device).
+struct TileSortArgs {
+    const SliceTable *tab;
+    int ns, T;                       // slices, tiles per axis
+    const unsigned long long *box;   // the beam's transverse bound slots
+    const double *src[6];
+    const int64_t *perm_src;
+    double *dst[6];
+    int64_t *perm_dst;
+    int *thist;                      // [ns][T*T][G]
+};
+hipError_t launch_tile_sort(const TileSortArgs &a, int G, hipStream_t s);
 int64_t slicing_hist_entries(int64_t n, int ns);
 hipError_t launch_store(double *const *beam, int64_t n, double *const *st, const int64_t *perm, const ErrorInfo *err,
                         hipStream_t s);
@@ -50,13 +67,16 @@ struct GeomOpts {
     double kbb1, kbb2, klum;
 };
 
+constexpr int kBoundSlots = 12;   // order-key slots per (pair, side) segment
+
 struct BoundsArgs {
     const BatchPair *pairs;
     int npairs;
     const SliceTable *tab1, *tab2;
     States st;
     int nchunks;
-    double *partial;             // [2*npairs][nchunks][12]
+    int srcfld, lum;             // which boxes to fold: source/field (slots 0-7), luminosity (8-11)
+    unsigned long long *keys;    // [2*npairs][kBoundSlots], zero on entry, zeroed again by the geometry stage
     const ErrorInfo *err;
 };
 hipError_t launch_bounds(const BoundsArgs &a, hipStream_t s);
@@ -65,8 +85,7 @@ struct GeometryArgs {
     const BatchPair *pairs;
     int npairs;
     const SliceTable *tab1, *tab2;
-    const double *partial;
-    int nchunks;
+    unsigned long long *keys;    // bounds slots of this batch, consumed (zeroed)
     GeomOpts o;
     CacheGeo *cache;             // [2*ns1*ns2]
     Inter *inters;               // [2*npairs]
@@ -116,8 +135,11 @@ struct LumArgs {
     double inv_scale;
     const double *lumgeo;
     double *lum_parts;           // indexed by pair position p
+    double *lum_partial;         // [npairs][kLumBlocks]
+    unsigned int *lum_arrivals;  // [npairs], zero between launches
     const ErrorInfo *err;
 };
+constexpr int kLumBlocks = 16;
 hipError_t launch_lum(const LumArgs &a, hipStream_t s);
 
 // FFT stages (k_fft.hip)
@@ -155,8 +177,7 @@ struct NodeGeometryArgs {
     const BatchPair *pairs;
     int npairs;
     const SliceTable *tab1, *tab2;
-    const double *partial;       // bounds partials of this batch (luminosity boxes)
-    int nchunks;
+    unsigned long long *keys;    // bounds slots of this batch (luminosity boxes), consumed (zeroed)
     GeomOpts o;
     const CacheGeo *nodes1, *nodes2;   // dir-1 meshes [ns1][ns2+1], dir-2 meshes [ns2][ns1+1]
     int slot1, slot2;                  // first Green slot of each direction

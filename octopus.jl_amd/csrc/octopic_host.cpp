@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -180,10 +181,10 @@ struct FFTPlanHost {
 };
 
 const char *kFamilies[] = {"slicing", "bounds", "geometry", "green", "deposit", "fft_rows_fwd", "fft_cols_conv",
-                           "fft_rows_inv", "field", "kick", "lum", "store", "node_prepare"};
+                           "fft_rows_inv", "field", "kick", "lum", "store", "node_prepare", "tile_order"};
 constexpr int kNFamilies = sizeof(kFamilies) / sizeof(kFamilies[0]);
 enum Fam { F_SLICING, F_BOUNDS, F_GEOMETRY, F_GREEN, F_DEPOSIT, F_ROWS_FWD, F_COLS, F_ROWS_INV, F_FIELD, F_KICK, F_LUM,
-           F_STORE, F_NODE };
+           F_STORE, F_NODE, F_TILE };
 
 }  // namespace
 
@@ -208,13 +209,19 @@ struct octopic_handle {
     DBuf<double> beam_in[2][6];
     DBuf<double> st[2][6];
     DBuf<int64_t> perm[2];
+    int tiles = 16;             // transverse tiles per axis of the in-slice order (0: slice order only)
+    DBuf<double> srt[2][6];     // tile-ordered states (tiles > 0)
+    DBuf<int64_t> perm_srt[2];
+    DBuf<unsigned long long> box;   // transverse bound slots of both beams
+    std::vector<double> hq[2];      // host quantiles / positions of the slicing
+    DBuf<int> thist;
     DBuf<uint16_t> sid[2];
     DBuf<int> hist[2];
     DBuf<double> lanes, mm, clanes;
     DBuf<unsigned long long> bins;
     DBuf<double *> ptrs;        // [beam_in 2x6][st 2x6]
     DBuf<BatchPair> pairs;
-    DBuf<double> partial;
+    DBuf<unsigned long long> keys;   // bounds slots [2 * widest batch][kBoundSlots], zero between batches
     DBuf<Inter> inters;
     DBuf<Plane> planes;
     DBuf<GreenJob> jobs;
@@ -224,6 +231,8 @@ struct octopic_handle {
     DBuf<double> phi, table;
     DBuf<octopic_interaction_record> records;
     DBuf<double> lum_parts;
+    DBuf<double> lum_partial;
+    DBuf<unsigned int> lum_arrivals;
     // node-mode buffers
     DBuf<CacheGeo> nodes;
     DBuf<GreenJob> node_jobs;
@@ -238,24 +247,26 @@ struct octopic_handle {
     int64_t nbeam[2] = {0, 0};
     std::vector<int32_t> pi, pj, pran;
     std::vector<double> plum;
+    bool unfinished = false;    // a collide left early (error): consume-and-zero buffers must be cleared
     std::vector<octopic_interaction_record> recs;
     octopic_stats stats{};
     // timing
     bool timing = false;
+    unsigned fam_mask = ~0u;     // families timed when timing is on
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_next = 0;
     double fam_ms[kNFamilies] = {0};
     int64_t fam_n[kNFamilies] = {0};
 
+    bool timed(int fam) const { return timing && ((fam_mask >> fam) & 1u); }
     void tic(int fam, hipEvent_t &a) {
-        if (!timing) return;
+        if (!timed(fam)) return;
         a = next_event();
         hipEventRecord(a, stream);
-        (void)fam;
     }
     void toc(int fam, hipEvent_t a) {
-        if (!timing) return;
+        if (!timed(fam)) return;
         hipEvent_t b = next_event();
         hipEventRecord(b, stream);
         ev.push_back({fam, {a, b}});
@@ -359,16 +370,21 @@ void validate(const octopic_options &o) {
 }
 
 void upload_ptrs(octopic_handle *h, double *const in[2][6]) {
-    double *hp[24];
+    double *hp[36];
     for (int b = 0; b < 2; ++b)
-        for (int c = 0; c < 6; ++c) { hp[b * 6 + c] = in[b][c]; hp[12 + b * 6 + c] = h->st[b][c].p; }
-    h->ptrs.ensure(24);
+        for (int c = 0; c < 6; ++c) {
+            hp[b * 6 + c] = in[b][c];
+            hp[12 + b * 6 + c] = h->st[b][c].p;
+            hp[24 + b * 6 + c] = h->srt[b][c].p;
+        }
+    h->ptrs.ensure(36);
     HIPCHECK(hipMemcpyAsync(h->ptrs.p, hp, sizeof(hp), hipMemcpyHostToDevice, h->stream));
 }
 
 void run_slicing(octopic_handle *h, int b, const octopic_slicing &s, int64_t n) {
     const int ns = s.nslices;
-    std::vector<double> q(kMaxSlices, 0.0);
+    std::vector<double> &q = h->hq[b];   // kept in the handle: the copy below is asynchronous
+    q.assign(kMaxSlices, 0.0);
     if (s.method == OCTOPIC_SLICE_NORMAL_QUANTILE) {
         for (int k = 1; k < ns; ++k) q[k] = std::sqrt(2.0) * erfinv_host((double)(2 * k) / (double)ns - 1.0);
     } else if (s.method == OCTOPIC_SLICE_SPECIFIED) {
@@ -380,22 +396,6 @@ void run_slicing(octopic_handle *h, int b, const octopic_slicing &s, int64_t n) 
     h->perm[b].ensure(n);
     h->sid[b].ensure(n);
     h->hist[b].ensure(slicing_hist_entries(n, ns));
-    SlicingArgs a{};
-    a.z = nullptr;
-    a.n = n;
-    a.method = s.method; a.ns = ns; a.resolution = s.resolution;
-    a.centroid = s.center_position == OCTOPIC_CENTER_CENTROID ? 1 : 0;
-    a.q = h->q.p + (size_t)b * kMaxSlices;
-    a.tab = h->tab.p + b;
-    a.lanes = h->lanes.p; a.mm_partial = h->mm.p; a.centroid_lanes = h->clanes.p;
-    a.bin_counts = h->bins.p;
-    a.sid = h->sid[b].p; a.hist = h->hist[b].p;
-    a.beam = h->ptrs.p + b * 6;
-    a.st = h->ptrs.p + 12 + b * 6;
-    a.st_z = h->st[b][4].p;
-    a.perm = h->perm[b].p;
-    a.err = h->errinfo.p;
-    return (void)a;
 }
 
 }  // namespace
@@ -449,13 +449,13 @@ void octopic_destroy(octopic_handle *h) {
     if (h->h_counters) hipHostFree(h->h_counters);
     h->tab.release(); h->q.release(); h->errinfo.release(); h->counters.release(); h->cache.release(); h->spec.release();
     for (int b = 0; b < 2; ++b) {
-        for (int c = 0; c < 6; ++c) { h->beam_in[b][c].release(); h->st[b][c].release(); }
-        h->perm[b].release(); h->sid[b].release(); h->hist[b].release();
+        for (int c = 0; c < 6; ++c) { h->beam_in[b][c].release(); h->st[b][c].release(); h->srt[b][c].release(); }
+        h->perm[b].release(); h->perm_srt[b].release(); h->sid[b].release(); h->hist[b].release();
     }
     h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
-    h->partial.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
+    h->keys.release(); h->box.release(); h->thist.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
     h->charge.release(); h->qh.release(); h->phi.release(); h->table.release(); h->records.release();
-    h->lum_parts.release(); h->nodes.release(); h->node_jobs.release(); h->node_partial.release(); h->ran_dev.release();
+    h->lum_parts.release(); h->lum_partial.release(); h->lum_arrivals.release(); h->nodes.release(); h->node_jobs.release(); h->node_partial.release(); h->ran_dev.release();
     h->px.tw.release(); h->py.tw.release();
     delete h;
 }
@@ -484,6 +484,12 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         h->py.build(2 * h->ny);
         h->tab.ensure(2);
         h->q.ensure(2 * kMaxSlices);
+        if (const char *e = std::getenv("OCTOPIC_TRANSVERSE_TILES")) {
+            const int t = std::atoi(e);
+            if (t < 0 || t > 128) fail(OCTOPIC_ERR_INVALID_ARGUMENT, "OCTOPIC_TRANSVERSE_TILES must be in [0, 128]");
+            h->tiles = t;
+        }
+        h->box.ensure(8);
         h->errinfo.ensure(1);
         h->counters.ensure(6);
         HIPCHECK(hipMemset(h->errinfo.p, 0, sizeof(ErrorInfo)));
@@ -532,7 +538,17 @@ extern "C" int octopic_collide(octopic_handle *h, octopic_beam *beam1, octopic_b
     if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
     try {
         HIPCHECK(hipSetDevice(h->device));
+        if (h->unfinished) {
+            // kernels of the abandoned collide may have left charge or bounds slots filled
+            if (h->charge.p) HIPCHECK(hipMemsetAsync(h->charge.p, 0, h->charge.n * sizeof(unsigned long long), h->stream));
+            if (h->keys.p) HIPCHECK(hipMemsetAsync(h->keys.p, 0, h->keys.n * sizeof(unsigned long long), h->stream));
+            HIPCHECK(hipMemsetAsync(h->errinfo.p, 0, sizeof(ErrorInfo), h->stream));
+            HIPCHECK(hipStreamSynchronize(h->stream));
+            h->unfinished = false;
+        }
+        h->unfinished = true;
         collide_impl(h, beam1, beam2, turn, compute_luminosity, luminosity);
+        h->unfinished = false;
     } catch (Fail &f) {
         return set_err(h, f.code, f.msg);
     }
@@ -612,6 +628,10 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     double *bp[2][6];
     for (int b = 0; b < 2; ++b) {
         double *hp[6] = {B[b]->x, B[b]->px, B[b]->y, B[b]->py, B[b]->z, B[b]->pz};
+        if (h->tiles > 0) {
+            for (int c = 0; c < 6; ++c) h->srt[b][c].ensure(n[b]);
+            h->perm_srt[b].ensure(n[b]);
+        }
         for (int c = 0; c < 6; ++c) {
             h->st[b][c].ensure(n[b]);
             if (B[b]->on_device) {
@@ -624,6 +644,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         }
     }
     upload_ptrs(h, bp);
+    if (h->tiles > 0) HIPCHECK(hipMemsetAsync(h->box.p, 0, 8 * sizeof(unsigned long long), s));
     // ---- slicing
     {
         Timed t(h, F_SLICING);
@@ -631,7 +652,8 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         for (int b = 0; b < 2; ++b) {
             run_slicing(h, b, *sl[b], n[b]);
             SlicingArgs a{};
-            a.z = bp[b][4];
+            a.z = bp[b][4]; a.x = bp[b][0]; a.y = bp[b][2];
+            a.box = h->tiles > 0 ? h->box.p + 4 * b : nullptr;
             a.n = n[b];
             a.method = sl[b]->method; a.ns = sl[b]->nslices; a.resolution = sl[b]->resolution;
             a.centroid = sl[b]->center_position == OCTOPIC_CENTER_CENTROID ? 1 : 0;
@@ -659,6 +681,27 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         if (!std::isfinite(T1.boundary[k])) raise_device_error(h, ErrorInfo{OCTOPIC_ERR_NONFINITE, -1, 0, 5, 0, 0}, turn);
     for (int k = 0; k <= ns2; ++k)
         if (!std::isfinite(T2.boundary[k])) raise_device_error(h, ErrorInfo{OCTOPIC_ERR_NONFINITE, -1, 0, 5, 0, 0}, turn);
+
+    // ---- transverse tile order inside the slices (queued before the host builds the schedule)
+    if (h->tiles > 0) {
+        Timed t(h, F_TILE);
+        const int TT = h->tiles * h->tiles;
+        for (int b = 0; b < 2; ++b) {
+            const SliceTable &T = h->h_tab[b];
+            int64_t mc = 0;
+            for (int k = 0; k < T.ns; ++k) mc = std::max<int64_t>(mc, T.count[k]);
+            // few blocks per slice: each block's members of one tile land in one contiguous run
+            int G = (int)std::min<int64_t>(64, std::max<int64_t>(1, (mc + kTileBlockSpan - 1) / kTileBlockSpan));
+            G = std::max(1, std::min<int>(G, (int)((int64_t)(1 << 22) / ((int64_t)T.ns * TT))));
+            h->thist.ensure((size_t)T.ns * TT * G);
+            TileSortArgs a{};
+            a.tab = h->tab.p + b; a.ns = T.ns; a.T = h->tiles; a.box = h->box.p + 4 * b;
+            for (int c = 0; c < 6; ++c) { a.src[c] = h->st[b][c].p; a.dst[c] = h->srt[b][c].p; }
+            a.perm_src = h->perm[b].p; a.perm_dst = h->perm_srt[b].p;
+            a.thist = h->thist.p;
+            HIPCHECK(launch_tile_sort(a, G, s));
+        }
+    }
 
     // ---- schedule (host): collision order and conflict-free batches
     const double kbb1 = kbb_of(o.kbb1, B1, B2, true, n[1]);
@@ -702,7 +745,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     h->pairs.ensure(std::max<size_t>(hp.size(), 1));
     if (!hp.empty())
         HIPCHECK(hipMemcpyAsync(h->pairs.p, hp.data(), sizeof(BatchPair) * hp.size(), hipMemcpyHostToDevice, s));
-    h->partial.ensure((size_t)2 * W * kMaxChunks * 12);
+    h->keys.ensure_zero((size_t)2 * W * kBoundSlots, s);
     h->inters.ensure_zero(2 * W, s);
     h->planes.ensure_zero(nplanes_max, s);
     h->jobs.ensure_zero(2 * W, s);
@@ -713,6 +756,8 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     h->table.ensure((size_t)2 * W * field_cells * 3 * npl);
     h->records.ensure((size_t)2 * std::max(npairs, 1));
     h->lum_parts.ensure(std::max(npairs, 1));
+    h->lum_partial.ensure((size_t)W * kLumBlocks);
+    h->lum_arrivals.ensure_zero(W, s);
     HIPCHECK(hipMemsetAsync(h->records.p, 0, sizeof(octopic_interaction_record) * 2 * npairs, s));
     HIPCHECK(hipMemsetAsync(h->lum_parts.p, 0, sizeof(double) * npairs, s));
     const int temp_slot0 = h->nkeys;
@@ -735,7 +780,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
 
     States st{};
     for (int b = 0; b < 2; ++b)
-        for (int c = 0; c < 6; ++c) st.c[b][c] = h->st[b][c].p;
+        for (int c = 0; c < 6; ++c) st.c[b][c] = h->tiles > 0 ? h->srt[b][c].p : h->st[b][c].p;
     const SliceTable *dt1 = h->tab.p, *dt2 = h->tab.p + 1;
 
     int slot_node1 = 0, slot_node2 = 0;
@@ -791,13 +836,13 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         const int nchunks = (int)std::min<int64_t>(kMaxChunks, std::max<int64_t>(1, (bmax + 4095) / 4096));
         {
             Timed t(h, F_BOUNDS);
-            BoundsArgs a{dp, B, dt1, dt2, st, nchunks, h->partial.p, h->errinfo.p};
+            BoundsArgs a{dp, B, dt1, dt2, st, nchunks, h->mode == 2 ? 0 : 1, compute_lum ? 1 : 0, h->keys.p, h->errinfo.p};
             HIPCHECK(launch_bounds(a, s));
         }
         if (h->mode == 2) {
             Timed t(h, F_GEOMETRY);
             NodeGeometryArgs a{};
-            a.pairs = dp; a.npairs = B; a.tab1 = dt1; a.tab2 = dt2; a.partial = h->partial.p; a.nchunks = nchunks;
+            a.pairs = dp; a.npairs = B; a.tab1 = dt1; a.tab2 = dt2; a.keys = h->keys.p;
             a.o = go; a.nodes1 = h->nodes.p; a.nodes2 = h->nodes.p + ns1 * (ns2 + 1);
             a.slot1 = slot_node1; a.slot2 = slot_node2;
             a.inters = h->inters.p; a.planes = h->planes.p; a.lumgeo = h->lumgeo.p; a.records = h->records.p;
@@ -806,7 +851,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         } else {
             Timed t(h, F_GEOMETRY);
             GeometryArgs a{};
-            a.pairs = dp; a.npairs = B; a.tab1 = dt1; a.tab2 = dt2; a.partial = h->partial.p; a.nchunks = nchunks;
+            a.pairs = dp; a.npairs = B; a.tab1 = dt1; a.tab2 = dt2; a.keys = h->keys.p;
             a.o = go; a.cache = h->cache.p; a.inters = h->inters.p; a.planes = h->planes.p; a.jobs = h->jobs.p;
             a.lumgeo = h->lumgeo.p; a.records = h->records.p; a.counters = h->counters.p; a.err = h->errinfo.p;
             HIPCHECK(launch_geometry(a, s));
@@ -868,6 +913,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             LumArgs a{};
             a.pairs = dp; a.npairs = B; a.charge = h->charge.p + (size_t)nfp * charge_stride; a.charge_stride = charge_stride;
             a.ncells = (int)lum_cells; a.inv_scale = inv_scale; a.lumgeo = h->lumgeo.p; a.lum_parts = h->lum_parts.p;
+            a.lum_partial = h->lum_partial.p; a.lum_arrivals = h->lum_arrivals.p;
             a.err = h->errinfo.p;
             HIPCHECK(launch_lum(a, s));
         }
@@ -875,7 +921,12 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     }
     {
         Timed t(h, F_STORE);
-        for (int b = 0; b < 2; ++b) HIPCHECK(launch_store(h->ptrs.p + b * 6, n[b], h->ptrs.p + 12 + b * 6, h->perm[b].p, h->errinfo.p, s));
+        for (int b = 0; b < 2; ++b) {
+            if (h->tiles > 0)
+                HIPCHECK(launch_store(h->ptrs.p + b * 6, n[b], h->ptrs.p + 24 + b * 6, h->perm_srt[b].p, h->errinfo.p, s));
+            else
+                HIPCHECK(launch_store(h->ptrs.p + b * 6, n[b], h->ptrs.p + 12 + b * 6, h->perm[b].p, h->errinfo.p, s));
+        }
     }
     // ---- results
     HIPCHECK(hipMemcpyAsync(h->h_err, h->errinfo.p, sizeof(ErrorInfo), hipMemcpyDeviceToHost, s));
@@ -1032,6 +1083,30 @@ int octopic_debug_weights(octopic_handle *h, int32_t method, const double *u, in
 int octopic_set_timing(octopic_handle *h, int32_t enabled) {
     if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
     h->timing = enabled != 0;
+    h->fam_mask = ~0u;
+    for (int k = 0; k < kNFamilies; ++k) { h->fam_ms[k] = 0.0; h->fam_n[k] = 0; }
+    return OCTOPIC_OK;
+}
+
+int octopic_set_timing_families(octopic_handle *h, const char *families) {
+    if (!h || !families) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    unsigned mask = 0u;
+    std::string list(families);
+    size_t pos = 0;
+    while (pos <= list.size()) {
+        size_t end = list.find(',', pos);
+        if (end == std::string::npos) end = list.size();
+        const std::string name = list.substr(pos, end - pos);
+        if (!name.empty()) {
+            int k = 0;
+            while (k < kNFamilies && name != kFamilies[k]) ++k;
+            if (k == kNFamilies) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "unknown kernel family '" + name + "'");
+            mask |= 1u << k;
+        }
+        pos = end + 1;
+    }
+    h->timing = mask != 0u;
+    h->fam_mask = mask;
     for (int k = 0; k < kNFamilies; ++k) { h->fam_ms[k] = 0.0; h->fam_n[k] = 0; }
     return OCTOPIC_OK;
 }

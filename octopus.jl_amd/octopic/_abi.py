@@ -166,6 +166,7 @@ def declare(lib):
         ),
         "octopic_debug_weights": (i32, [vp, i32, DPTR, i64, i32, C.POINTER(i32), DPTR]),
         "octopic_set_timing": (i32, [vp, i32]),
+        "octopic_set_timing_families": (i32, [vp, C.c_char_p]),
         "octopic_kernel_times": (
             i32,
             [vp, DPTR, C.POINTER(i64), i64, C.POINTER(i64), C.c_char_p, i64],
@@ -193,5 +194,6 @@ EXPORTED = [
     "octopic_generate_gaussian_beam",
     "octopic_debug_weights",
     "octopic_set_timing",
+    "octopic_set_timing_families",
     "octopic_kernel_times",
 ]

@@ -216,8 +216,12 @@ class PICCollider:
         s = beam.struct()
         self._check(lib().octopic_track_linear6d(self.h, m, C.byref(s)))
 
-    def set_timing(self, enabled: bool):
-        self._check(lib().octopic_set_timing(self.h, 1 if enabled else 0))
+    def set_timing(self, enabled: bool, families: Optional[str] = None):
+        """Per-family HIP-event timing: every family, or only the comma-separated `families`."""
+        if enabled and families:
+            self._check(lib().octopic_set_timing_families(self.h, families.encode()))
+        else:
+            self._check(lib().octopic_set_timing(self.h, 1 if enabled else 0))
 
     def kernel_times(self):
         """{family: (total_ms, launches)} accumulated since set_timing(True)."""

@@ -247,6 +247,34 @@ def test_bitwise_reproducible(oracle, gpu):
     assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][2], outs[1][2])
 
 
+@pytest.mark.parametrize("tiles", ["0", "7", "64"])
+def test_in_slice_order_does_not_change_results(oracle, gpu, monkeypatch, tiles):
+    """The transverse tile order inside slices is a layout choice only: any tiling (0 = slice order)
+    gives bit-identical coordinates, luminosities, records and cache history to the default one."""
+    e, p = round_beams(oracle, 30000, 12000)
+    sl = LongitudinalSlicing(nslices=7, method="normal_quantile")
+    solver = PICPoissonSolver(grid=(64, 64), slicing=sl, deposit_method="TSC", field_derivative="fourth",
+                              slice_interpolation="quadratic")
+    outs = []
+    for env in (None, tiles):
+        if env is None:
+            monkeypatch.delenv("OCTOPIC_TRANSVERSE_TILES", raising=False)
+        else:
+            monkeypatch.setenv("OCTOPIC_TRANSVERSE_TILES", env)
+        col = PICCollider(solver)
+        be, bp = Beam.from_host(e, PE, device=gpu), Beam.from_host(p, PP, device=gpu)
+        lums = [col.collide(be, bp) for _ in range(2)]
+        outs.append((lums, be.to_host(), bp.to_host(), col.pair_luminosity(), col.stats(),
+                     [(r.p, r.dir, r.cache_event, tuple(r.source_grid), tuple(r.field_grid))
+                      for r in col.interaction_records()]))
+    monkeypatch.delenv("OCTOPIC_TRANSVERSE_TILES", raising=False)
+    a, b = outs
+    assert a[0] == b[0]
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert np.array_equal(np.asarray(a[3]), np.asarray(b[3]))
+    assert a[4] == b[4] and a[5] == b[5]
+
+
 def test_nonfinite_fails_fast_and_leaves_beams(oracle, gpu):
     e, p = round_beams(oracle, 5000, 3000)
     e[2, 123] = np.nan
