@@ -37,6 +37,8 @@ P_CFG = dict(n=1_024_000, sigma=(95e-6, 8.5e-6, 6.0e-2), beta=(0.8, 0.072, 6.0e-
              charge=1.0, energy=275.0e9, npart=0.6881e11, rng_id=2, tunes=(0.228, 0.21, 0.01))
 NSLICES, GRID = 15, 128
 BREAKDOWN_TURNS = 10
+TRAFFIC_PROFILE = {"quadratic": "hbm_traffic_2p56M_1p024M_15sl_128_quadratic.md"}
+COPY_PROBE_BYTES = 1 << 30
 
 
 def make_solver(arm):
@@ -64,6 +66,19 @@ def bytes_model(n1, n2, ns1, ns2, npl):
         "slicing": 8.0 * 3 * (n1 + n2) + 2 * 48.0 * (n1 + n2),   # z passes + gather of the states
         "store": 2 * 40.0 * (n1 + n2),       # states -> beams (5 coordinates)
     }
+
+
+def measured_traffic(arm, family):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (copy-kernel calibrated), when one exists for this workload and kernel."""
+    kernel = {("quadratic", "kick"): "k_kick<1, 1>", ("node", "kick"): "k_kick<1, 2>"}.get((arm, family))
+    path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILE[arm]) if arm in TRAFFIC_PROFILE else None
+    if kernel is None or path is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def cpu_baseline(arm):
@@ -103,6 +118,8 @@ def main():
     ap.add_argument("--arm", choices=["quadratic", "node"], default="quadratic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel HIP event timing")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run plain device copies of 1 GiB (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the collision path has no multi-GPU "
@@ -110,6 +127,14 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        src = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "copies": 3}))
+        return
     from octopic import EMASS_EV, PMASS_EV, PICCollider, generate_beam, linear6d_matrix
 
     dev = torch.device("cuda:0")
@@ -168,8 +193,10 @@ def main():
             per_launch_bytes = model[best] * args.steps / launches
             avg_s = ms / 1e3 / launches
             achieved = per_launch_bytes / avg_s / 1e9
+            traffic = measured_traffic(args.arm, best)
             roof = {"bound": "hbm", "kernel": best, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": round(traffic) if traffic else None,
                     "avg_launch_ms": round(avg_s * 1e3, 4), "launches": int(launches),
                     "bytes_per_launch": round(per_launch_bytes)}
     out = {

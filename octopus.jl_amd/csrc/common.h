@@ -78,6 +78,20 @@ __host__ __device__ __forceinline__ void stencil(double u, int n, int &base0, do
     if (METHOD == 0) cic_weights(u, n, base0, w); else tsc_weights(u, n, base0, w);
 }
 
+// Stencil nodes [lo, hi] on an axis of n nodes used by every position whose normalised coordinate lies
+// in [umin, umax] (the first stencil node is monotonic in u over the in-range interval).  A bound that is
+// out of range or non-finite opens the window to that end of the axis; positions outside the axis
+// (first node 0, zero weights) may still fall outside the window, so users check each node.
+template <int METHOD>
+__device__ __forceinline__ void stencil_window(double umin, double umax, int n, int &lo, int &hi) {
+    constexpr int K = METHOD == 0 ? 2 : 3;
+    double w[3];
+    int b;
+    if (0.0 <= umin && umin <= (double)(n - 1)) { stencil<METHOD>(umin, n, b, w); lo = b; } else lo = 0;
+    if (0.0 <= umax && umax <= (double)(n - 1)) { stencil<METHOD>(umax, n, b, w); hi = b + K - 1; } else hi = n - 1;
+    if (hi < lo) { lo = 0; hi = n - 1; }
+}
+
 // Order-preserving uint64 key of a non-NaN double (-0.0 sorts below +0.0, as Julia's min/max), so that
 // exact min/max reductions can use 64-bit integer atomics in any order.
 __host__ __device__ __forceinline__ unsigned long long order_key(double x) {

@@ -96,6 +96,9 @@ __global__ __launch_bounds__(kBoundsThreads) void k_bounds(BoundsArgs a) {
         unsigned long long u = red[0][q];
         for (int w = 1; w < kWaves; ++w) u = red[w][q] > u ? red[w][q] : u;
         if (u) atomicMax(&a.keys[(size_t)seg * kBoundSlots + q], u);
+        // this chunk's own source box (sL and sR drifts) and field box, for the LDS windows of the
+        // deposit and kick blocks that take the same particle range
+        if (SRCFLD && q < 8) a.chunkbox[((size_t)seg * a.nchunks + blockIdx.x) * 8 + q] = bound_value(u, (q & 1) == 0);
     }
 }
 
@@ -392,16 +395,38 @@ hipError_t launch_geometry(const GeometryArgs &a, hipStream_t s) {
 // ---------------------------------------------------------------------------------- deposit
 constexpr int kDepositThreads = 1024;
 
+// The block's LDS copy of the whole mesh is zeroed and flushed only over the stencil window of its
+// chunks' source box: every position it deposits lies in that box (the sM and luminosity drifts lie
+// between sL and sR, and positions are x + px*s, monotonic in s), the first stencil node is monotonic
+// in the position, and positions off the mesh have zero weights.
 template <int METHOD>
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
     extern __shared__ unsigned long long tile[];
     if (failed(a.err)) return;
     const Plane pl = a.planes[blockIdx.y];
-    const int64_t lo = pl.n * blockIdx.x / a.chunks, hi = pl.n * (blockIdx.x + 1) / a.chunks;
+    // block b takes the chunks [b*group, (b+1)*group) of the batch's chunking
+    const int c0 = blockIdx.x * a.group, c1 = min(c0 + a.group, a.chunks);
+    const int64_t lo = pl.n * c0 / a.chunks, hi = pl.n * c1 / a.chunks;
     if (lo >= hi) return;
-    const int n1 = pl.n1, n2 = pl.n2, ncell = n1 * n2;
-    for (int c = threadIdx.x; c < ncell; c += blockDim.x) tile[c] = 0ull;
+    const int n1 = pl.n1, n2 = pl.n2;
+    int wx0 = 0, wx1 = n1 - 1, wy0 = 0, wy1 = n2 - 1;
+    if (a.chunkbox) {
+        const int seg = 2 * (pl.kind == 0 ? pl.inter / 2 : pl.inter) + pl.beam;
+        double b[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+        for (int c = c0; c < c1; ++c) {
+            const double *bc = a.chunkbox + ((size_t)seg * a.chunks + c) * 8;
+            b[0] = jmin(b[0], bc[0]); b[1] = jmax(b[1], bc[1]); b[2] = jmin(b[2], bc[2]); b[3] = jmax(b[3], bc[3]);
+        }
+        stencil_window<METHOD>((b[0] - pl.x0) * pl.hxi, (b[1] - pl.x0) * pl.hxi, n1, wx0, wx1);
+        stencil_window<METHOD>((b[2] - pl.y0) * pl.hyi, (b[3] - pl.y0) * pl.hyi, n2, wy0, wy1);
+    }
+    const int wnx = wx1 - wx0 + 1, wcells = wnx * (wy1 - wy0 + 1);
+    for (int c = threadIdx.x; c < wcells; c += blockDim.x) {
+        const int r = c / wnx;
+        tile[(wy0 + r) * n1 + wx0 + (c - r * wnx)] = 0ull;
+    }
     __syncthreads();
+    unsigned long long *out = a.charge + (size_t)blockIdx.y * a.charge_stride;
     const double *X = a.st.c[pl.beam][0] + pl.off, *PX = a.st.c[pl.beam][1] + pl.off, *Y = a.st.c[pl.beam][2] + pl.off,
                  *PY = a.st.c[pl.beam][3] + pl.off;
     const double s = pl.s, x0 = pl.x0, y0 = pl.y0, hxi = pl.hxi, hyi = pl.hyi, scale = a.scale;
@@ -417,6 +442,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
         double wx[3], wy[3];
         stencil<METHOD>((xd - x0) * hxi, n1, bx, wx);
         stencil<METHOD>((yd - y0) * hyi, n2, by, wy);
+        // nonzero weights only for in-range positions, whose stencils lie inside the window
 #pragma unroll
         for (int m = 0; m < K; ++m)
 #pragma unroll
@@ -430,17 +456,18 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
         if ((threadIdx.x & 63) == 0 && dropped) atomicAdd(&a.dropped[0], (unsigned long long)dropped);
     }
     __syncthreads();
-    unsigned long long *out = a.charge + (size_t)blockIdx.y * a.charge_stride;
-    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const unsigned long long v = tile[c];
-        if (v) atomicAdd(&out[c], v);
+    for (int c = threadIdx.x; c < wcells; c += blockDim.x) {
+        const int r = c / wnx;
+        const int cell = (wy0 + r) * n1 + wx0 + (c - r * wnx);
+        const unsigned long long v = tile[cell];
+        if (v) atomicAdd(&out[cell], v);
     }
 }
 
 hipError_t launch_deposit(const DepositArgs &a, hipStream_t s) {
     if (a.nplanes <= 0) return hipSuccess;
     const size_t lds = (size_t)a.ncells * sizeof(unsigned long long);
-    dim3 g(a.chunks, a.nplanes);
+    dim3 g((a.chunks + a.group - 1) / a.group, a.nplanes);
     if (a.method == 0) {
         hipFuncSetAttribute((const void *)k_deposit<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k_deposit<0>, g, dim3(kDepositThreads), lds, s, a);
@@ -453,9 +480,51 @@ hipError_t launch_deposit(const DepositArgs &a, hipStream_t s) {
 
 // ---------------------------------------------------------------------------------- kick
 constexpr int kKickThreads = 256;
+constexpr int kKickLdsBytes = 36 * 1024;   // LDS window of the field table per block (4 blocks per CU)
 
+// Gather of one particle's field from the interleaved table rows starting at e0 (row = one x, rows
+// `rs` doubles apart, C doubles per cell), for the linear (MODE 0) or quadratic (MODE 1) slice
+// interpolation (pic_cpu.jl:1218-1273, 2596-2680).
+template <int METHOD, int MODE>
+__device__ __forceinline__ void gather_field(const double *e0, int rs, int C, const double wx[3], const double wy[3],
+                                             double zL, double &Kx, double &Ky, double &Kz) {
+    constexpr int K = METHOD == 0 ? 2 : 3;
+    if (MODE == 0) {
+        const double zR = 1.0 - zL;
+#pragma unroll
+        for (int m = 0; m < K; ++m)
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const double *e = e0 + m * rs + q * C;
+                const double w = wx[m] * wy[q];
+                Kx += w * (zL * e[1] + zR * e[4]);
+                Ky += w * (zL * e[2] + zR * e[5]);
+                Kz += w * (e[0] - e[3]);
+            }
+    } else {
+        const double tt = 1.0 - zL;
+        const double t2 = tt * tt;
+        const double aL = 2.0 * t2 - 3.0 * tt + 1.0, aM = 4.0 * tt - 4.0 * t2, aR = 2.0 * t2 - tt;
+        const double bL = 3.0 - 4.0 * tt, bM = 8.0 * tt - 4.0, bR = 1.0 - 4.0 * tt;
+#pragma unroll
+        for (int m = 0; m < K; ++m)
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const double *e = e0 + m * rs + q * C;   // L = 0..2, R = 3..5, M = 6..8
+                const double w = wx[m] * wy[q];
+                Kx += w * (aL * e[1] + aM * e[7] + aR * e[4]);
+                Ky += w * (aL * e[2] + aM * e[8] + aR * e[5]);
+                Kz += w * (bL * e[0] + bM * e[6] + bR * e[3]);
+            }
+    }
+}
+
+// One block takes one chunk of an interaction's field slice.  In the slice-pair modes the table cells
+// of the chunk's stencil window (from the chunk's field box) are staged in LDS when they fit; a
+// particle whose stencil leaves the window reads the table in HBM.
 template <int METHOD, int MODE>
 __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
+    extern __shared__ double win[];
     if (failed(a.err)) return;
     const int lblk = xcd_logical_block(blockIdx.x, gridDim.x);
     if (lblk >= a.chunks * a.ninters) return;
@@ -482,6 +551,23 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     const double c = it.c_src, ks = it.kick_scale;
     const int lk = a.lkick;
     constexpr int K = METHOD == 0 ? 2 : 3;
+    // LDS window (uniform per block)
+    int wx0 = 0, wy0 = 0, wnx = 0, wny = 0;
+    if (MODE != 2 && a.chunkbox && lo < hi) {
+        const double *b = a.chunkbox + ((size_t)(2 * (d >> 1) + it.fld_beam) * a.chunks + chunk) * 8 + 4;
+        int x0, x1, y0, y1;
+        stencil_window<METHOD>((b[0] - it.fg.x0) / hx, (b[1] - it.fg.x0) / hx, nx, x0, x1);
+        stencil_window<METHOD>((b[2] - it.fg.y0) / hy, (b[3] - it.fg.y0) / hy, ny, y0, y1);
+        if ((x1 - x0 + 1) * (y1 - y0 + 1) * C * (int)sizeof(double) <= kKickLdsBytes) {
+            wx0 = x0; wy0 = y0; wnx = x1 - x0 + 1; wny = y1 - y0 + 1;
+            const int run = wny * C;
+            for (int k = threadIdx.x; k < wnx * run; k += blockDim.x) {
+                const int r = k / run;
+                win[k] = tab[((size_t)(wx0 + r) * ny + wy0) * C + (k - r * run)];
+            }
+            __syncthreads();
+        }
+    }
     int dropped = 0;
     const double fx1 = it.fg.x0 + it.fg.width, fy1 = it.fg.y0 + it.fg.height;
     for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
@@ -501,33 +587,11 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
         double wx[3], wy[3];
         stencil<METHOD>((x - it.fg.x0) / hx, nx, bx, wx);
         stencil<METHOD>((y - it.fg.y0) / hy, ny, by, wy);
-        if (MODE == 0) {
-            const double zR = 1.0 - zL;
-#pragma unroll
-            for (int m = 0; m < K; ++m)
-#pragma unroll
-                for (int q = 0; q < K; ++q) {
-                    const double *e = tab + ((size_t)(bx + m) * ny + (by + q)) * C;
-                    const double w = wx[m] * wy[q];
-                    Kx += w * (zL * e[1] + zR * e[4]);
-                    Ky += w * (zL * e[2] + zR * e[5]);
-                    Kz += w * (e[0] - e[3]);
-                }
-        } else if (MODE == 1) {
-            const double tt = 1.0 - zL;
-            const double t2 = tt * tt;
-            const double aL = 2.0 * t2 - 3.0 * tt + 1.0, aM = 4.0 * tt - 4.0 * t2, aR = 2.0 * t2 - tt;
-            const double bL = 3.0 - 4.0 * tt, bM = 8.0 * tt - 4.0, bR = 1.0 - 4.0 * tt;
-#pragma unroll
-            for (int m = 0; m < K; ++m)
-#pragma unroll
-                for (int q = 0; q < K; ++q) {
-                    const double *e = tab + ((size_t)(bx + m) * ny + (by + q)) * C;   // L = 0..2, R = 3..5, M = 6..8
-                    const double w = wx[m] * wy[q];
-                    Kx += w * (aL * e[1] + aM * e[7] + aR * e[4]);
-                    Ky += w * (aL * e[2] + aM * e[8] + aR * e[5]);
-                    Kz += w * (bL * e[0] + bM * e[6] + bR * e[3]);
-                }
+        if (MODE != 2) {
+            if (bx >= wx0 && bx + K <= wx0 + wnx && by >= wy0 && by + K <= wy0 + wny)
+                gather_field<METHOD, MODE>(win + ((bx - wx0) * wny + (by - wy0)) * C, wny * C, C, wx, wy, zL, Kx, Ky, Kz);
+            else
+                gather_field<METHOD, MODE>(tab + ((size_t)bx * ny + by) * C, ny * C, C, wx, wy, zL, Kx, Ky, Kz);
         } else {
             // :node (pic_cpu.jl:1507-1530): L on its own mesh, R on its own mesh, phi_L - phi_Z on L's mesh
             double KxL = 0.0, KyL = 0.0, KxR = 0.0, KyR = 0.0;
@@ -576,7 +640,12 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
 hipError_t launch_kick(const KickArgs &a, hipStream_t s) {
     if (a.ninters <= 0) return hipSuccess;
     dim3 g(xcd_padded(a.chunks * a.ninters)), b(kKickThreads);
-#define OCTOPIC_KICK(M, MODE) hipLaunchKernelGGL((k_kick<M, MODE>), g, b, 0, s, a)
+    const size_t lds = (a.mode != 2 && a.chunkbox) ? kKickLdsBytes : 0;
+#define OCTOPIC_KICK(M, MODE)                                                                             \
+    do {                                                                                                  \
+        hipFuncSetAttribute((const void *)k_kick<M, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        hipLaunchKernelGGL((k_kick<M, MODE>), g, b, lds, s, a);                                           \
+    } while (0)
     if (a.method == 0) {
         if (a.mode == 0) OCTOPIC_KICK(0, 0); else if (a.mode == 1) OCTOPIC_KICK(0, 1); else OCTOPIC_KICK(0, 2);
     } else {

@@ -379,7 +379,7 @@ __global__ __launch_bounds__(256) void k_store(double *const *beam, int64_t n, d
 // ---------------------------------------------------------------------- transverse tile order
 // After the slice scatter and the centroid folds (which need each member list in ascending index
 // order), the members of every slice are regrouped by transverse tile: T x T tiles over the beam's
-// x-y box at slicing time, tile-major within the slice.  The field gathers of the kick then share
+// x-y box at slicing time, in Morton order of the tiles within the slice.  The field gathers of the kick then share
 // cache lines across a wave.  Only the order inside a slice changes, and no later result depends on
 // it: deposits are exact integer sums, bounds are exact min/max, kicks are per particle, and the
 // final store goes through the permutation.
@@ -399,8 +399,17 @@ __device__ __forceinline__ TileBox tile_box(const unsigned long long *box, int T
 __device__ __forceinline__ int tile_axis(double u, int T) {
     return u >= 0.0 ? (u < (double)T ? (int)u : T - 1) : 0;   // NaN -> 0
 }
+// Tiles in Morton (Z) order, T a power of two: consecutive tiles form compact 2-D patches, so a run of
+// consecutive members (one block's chunk) covers a small square of the mesh.
+__device__ __forceinline__ unsigned spread_bits(unsigned v) {   // v < 2^8 -> bits at even positions
+    v = (v | (v << 4)) & 0x0F0Fu;
+    v = (v | (v << 2)) & 0x3333u;
+    v = (v | (v << 1)) & 0x5555u;
+    return v;
+}
 __device__ __forceinline__ int tile_of(double x, double y, const TileBox &b, int T) {
-    return tile_axis((x - b.x0) * b.sx, T) * T + tile_axis((y - b.y0) * b.sy, T);
+    return (int)((spread_bits((unsigned)tile_axis((x - b.x0) * b.sx, T)) << 1) |
+                 spread_bits((unsigned)tile_axis((y - b.y0) * b.sy, T)));
 }
 
 constexpr int kTileThreads = 1024;

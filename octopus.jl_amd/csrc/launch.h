@@ -77,8 +77,14 @@ struct BoundsArgs {
     int nchunks;
     int srcfld, lum;             // which boxes to fold: source/field (slots 0-7), luminosity (8-11)
     unsigned long long *keys;    // [2*npairs][kBoundSlots], zero on entry, zeroed again by the geometry stage
+    double *chunkbox;            // [2*npairs][nchunks][8]: per chunk, source box (sL, sR) and field box (srcfld only)
     const ErrorInfo *err;
 };
+
+// Particle ranges of a batch: slice members [n*c/nchunks, n*(c+1)/nchunks) for chunk c, the same in
+// the bounds, deposit and kick launches, so that a chunk's box describes the block that takes it.
+constexpr int kChunkSpan = 2048;
+constexpr int kMaxChunks = 64;
 hipError_t launch_bounds(const BoundsArgs &a, hipStream_t s);
 
 struct GeometryArgs {
@@ -104,11 +110,13 @@ struct DepositArgs {
     States st;
     int method;
     int ncells;                  // n1*n2 of these planes
-    int chunks;
+    int chunks;                  // the batch's chunking of each slice
+    int group;                   // chunks per block
     double scale;                // fixed-point scale 2^k
     unsigned long long *charge;  // [plane][charge_stride]
     int64_t charge_stride;
     unsigned long long *dropped; // dropped-particle counter
+    const double *chunkbox;      // chunk boxes of the batch's bounds launch, or null (whole mesh windows)
     const ErrorInfo *err;
 };
 hipError_t launch_deposit(const DepositArgs &a, hipStream_t s);
@@ -122,6 +130,7 @@ struct KickArgs {
     int lkick;
     int chunks;
     unsigned long long *dropped;
+    const double *chunkbox;      // chunk boxes of the batch's bounds launch, or null (no LDS windows)
     const ErrorInfo *err;
 };
 hipError_t launch_kick(const KickArgs &a, hipStream_t s);

@@ -222,6 +222,7 @@ struct octopic_handle {
     DBuf<double *> ptrs;        // [beam_in 2x6][st 2x6]
     DBuf<BatchPair> pairs;
     DBuf<unsigned long long> keys;   // bounds slots [2 * widest batch][kBoundSlots], zero between batches
+    DBuf<double> chunkbox;           // per-chunk boxes of a batch [2 * widest batch][kMaxChunks][8]
     DBuf<Inter> inters;
     DBuf<Plane> planes;
     DBuf<GreenJob> jobs;
@@ -453,7 +454,7 @@ void octopic_destroy(octopic_handle *h) {
         h->perm[b].release(); h->perm_srt[b].release(); h->sid[b].release(); h->hist[b].release();
     }
     h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
-    h->keys.release(); h->box.release(); h->thist.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
+    h->keys.release(); h->chunkbox.release(); h->box.release(); h->thist.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
     h->charge.release(); h->qh.release(); h->phi.release(); h->table.release(); h->records.release();
     h->lum_parts.release(); h->lum_partial.release(); h->lum_arrivals.release(); h->nodes.release(); h->node_jobs.release(); h->node_partial.release(); h->ran_dev.release();
     h->px.tw.release(); h->py.tw.release();
@@ -486,7 +487,8 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         h->q.ensure(2 * kMaxSlices);
         if (const char *e = std::getenv("OCTOPIC_TRANSVERSE_TILES")) {
             const int t = std::atoi(e);
-            if (t < 0 || t > 128) fail(OCTOPIC_ERR_INVALID_ARGUMENT, "OCTOPIC_TRANSVERSE_TILES must be in [0, 128]");
+            if (t < 0 || t > 128 || (t & (t - 1)) != 0)
+                fail(OCTOPIC_ERR_INVALID_ARGUMENT, "OCTOPIC_TRANSVERSE_TILES must be 0 or a power of two up to 128");
             h->tiles = t;
         }
         h->box.ensure(8);
@@ -741,11 +743,11 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     const int64_t field_cells = (int64_t)nx * ny, lum_cells = (int64_t)(h->lnx + 1) * (h->lny + 1);
     const int64_t charge_stride = std::max(field_cells, lum_cells);
     const int nfield_planes = 2 * W * npl, nplanes_max = nfield_planes + 2 * W;
-    const int kMaxChunks = 64;
     h->pairs.ensure(std::max<size_t>(hp.size(), 1));
     if (!hp.empty())
         HIPCHECK(hipMemcpyAsync(h->pairs.p, hp.data(), sizeof(BatchPair) * hp.size(), hipMemcpyHostToDevice, s));
     h->keys.ensure_zero((size_t)2 * W * kBoundSlots, s);
+    h->chunkbox.ensure((size_t)2 * W * kMaxChunks * 8);
     h->inters.ensure_zero(2 * W, s);
     h->planes.ensure_zero(nplanes_max, s);
     h->jobs.ensure_zero(2 * W, s);
@@ -833,10 +835,13 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             const BatchPair &pr = hp[boff[bi] + k];
             bmax = std::max<int64_t>(bmax, std::max<int64_t>(T1.count[pr.i], T2.count[pr.j]));
         }
-        const int nchunks = (int)std::min<int64_t>(kMaxChunks, std::max<int64_t>(1, (bmax + 4095) / 4096));
+        // one particle-range chunking for bounds, deposit and kick (chunk boxes describe the blocks)
+        const int nchunks = (int)std::min<int64_t>(kMaxChunks, std::max<int64_t>(1, (bmax + kChunkSpan - 1) / kChunkSpan));
+        const double *cbox = h->mode == 2 ? nullptr : h->chunkbox.p;   // node mode folds no source/field boxes
         {
             Timed t(h, F_BOUNDS);
-            BoundsArgs a{dp, B, dt1, dt2, st, nchunks, h->mode == 2 ? 0 : 1, compute_lum ? 1 : 0, h->keys.p, h->errinfo.p};
+            BoundsArgs a{dp, B, dt1, dt2, st, nchunks, h->mode == 2 ? 0 : 1, compute_lum ? 1 : 0, h->keys.p,
+                         h->chunkbox.p, h->errinfo.p};
             HIPCHECK(launch_bounds(a, s));
         }
         if (h->mode == 2) {
@@ -863,13 +868,14 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
                                         h->errinfo.p, s));
         }
         const int nfp = 2 * B * npl;
-        const int dchunks = (int)std::min<int64_t>(32, std::max<int64_t>(1, (bmax + 8191) / 8192));
         {
             Timed t(h, F_DEPOSIT);
             DepositArgs a{};
             a.planes = h->planes.p; a.nplanes = nfp; a.st = st; a.method = o.deposit; a.ncells = (int)field_cells;
-            a.chunks = dchunks; a.scale = scale; a.charge = h->charge.p; a.charge_stride = charge_stride;
-            a.dropped = h->counters.p + 4; a.err = h->errinfo.p;
+            // ~8k particles per block: the LDS mesh is zeroed and flushed once per block
+            a.chunks = nchunks; a.group = std::max(1, (int)std::min<int64_t>(nchunks, 8192 * (int64_t)nchunks / std::max<int64_t>(bmax, 1)));
+            a.scale = scale; a.charge = h->charge.p; a.charge_stride = charge_stride;
+            a.dropped = h->counters.p + 4; a.chunkbox = cbox; a.err = h->errinfo.p;
             HIPCHECK(launch_deposit(a, s));
             if (compute_lum) {
                 DepositArgs l = a;
@@ -903,8 +909,9 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             KickArgs a{};
             a.inters = h->inters.p; a.ninters = 2 * B; a.st = st; a.table = h->table.p; a.nx = nx; a.ny = ny; a.npl = npl;
             a.method = o.deposit; a.mode = h->mode; a.lkick = o.longitudinal_kick ? 1 : 0;
-            a.chunks = (int)std::min<int64_t>(64, std::max<int64_t>(1, (bmax + 2047) / 2048));
+            a.chunks = nchunks;
             a.dropped = h->counters.p + 4;
+            a.chunkbox = cbox;
             a.err = h->errinfo.p;
             HIPCHECK(launch_kick(a, s));
         }

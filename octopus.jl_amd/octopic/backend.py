@@ -48,13 +48,15 @@ class UnsupportedOptionError(ArgumentError):
 
 
 def lib():
-    """Load liboctopic.so (raises BackendUnavailable if it is not built)."""
+    """Load liboctopic.so (raises BackendUnavailable if it is not built).  OCTOPIC_LIB, when set, names
+    the library file to load instead of the in-tree build, as on the Julia side."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        path = os.environ.get("OCTOPIC_LIB") or LIB_PATH
+        if not os.path.exists(path):
             raise BackendUnavailable(
-                f"{LIB_PATH} is not built; run __graft_entry__.build() (or make -C octopus.jl_amd/csrc)")
-        L = C.CDLL(LIB_PATH)
+                f"{path} is not built; run __graft_entry__.build() (or make -C octopus.jl_amd/csrc)")
+        L = C.CDLL(path)
         _abi.declare(L)
         _lib = L
     return _lib
