@@ -41,14 +41,15 @@ TRAFFIC_PROFILE = {"quadratic": "hbm_traffic_2p56M_1p024M_15sl_128_quadratic.md"
 COPY_PROBE_BYTES = 1 << 30
 
 
-def make_solver(arm):
+def make_solver(arm, precision="fp64"):
     from octopic import LongitudinalSlicing, PICPoissonSolver
     sl = LongitudinalSlicing(nslices=NSLICES, method="normal_quantile", center_position="centroid")
     if arm == "node":
         return PICPoissonSolver(grid=(GRID, GRID), deposit_method="TSC", field_derivative="fourth",
-                                slice_interpolation="linear", interaction_grid="node", slicing=sl)
+                                slice_interpolation="linear", interaction_grid="node", slicing=sl, precision=precision)
     return PICPoissonSolver(grid=(GRID, GRID), deposit_method="TSC", field_derivative="fourth",
-                            slice_interpolation="quadratic", interaction_grid="slice_pair", slicing=sl)
+                            slice_interpolation="quadratic", interaction_grid="slice_pair", slicing=sl,
+                            precision=precision)
 
 
 def params(cfg, mass):
@@ -66,6 +67,73 @@ def bytes_model(n1, n2, ns1, ns2, npl):
         "slicing": 8.0 * 3 * (n1 + n2) + 2 * 48.0 * (n1 + n2),   # z passes + gather of the states
         "store": 2 * 40.0 * (n1 + n2),       # states -> beams (5 coordinates)
     }
+
+
+def instance_configs(k, seed=7):
+    """Batched form: k independent bunch pairs with e- N ~ U[0.3M, 2.56M], p N ~ U[0.12M, 1.024M],
+    slices in {7, 11, 15}, meshes in {64, 96, 128}, per-instance seeds 1..k and tunes; the C1 beam
+    shapes otherwise (TSC, 4th order, quadratic slice interpolation, FP32 form)."""
+    import numpy as np
+    rs = np.random.RandomState(seed)
+    out = []
+    for i in range(k):
+        out.append(dict(n1=int(rs.uniform(0.3e6, 2.56e6)), n2=int(rs.uniform(0.12e6, 1.024e6)),
+                        ns=int(rs.choice([7, 11, 15])), grid=int(rs.choice([64, 96, 128])), seed=i + 1,
+                        dtune=0.002 * rs.uniform(-1, 1)))
+    return out
+
+
+def run_instances(args):
+    """Aggregate turns/s of the batched form (all instances collided together, then their maps)."""
+    import torch
+    from octopic import (EMASS_EV, PMASS_EV, LongitudinalSlicing, PICCollider, PICPoissonSolver, collide_many,
+                         generate_beam, linear6d_matrix)
+    dev = torch.device("cuda:0")
+    cfgs = instance_configs(args.instances)
+    cols, b1, b2, maps = [], [], [], []
+    for c in cfgs:
+        sl = LongitudinalSlicing(nslices=c["ns"], method="normal_quantile", center_position="centroid")
+        solver = PICPoissonSolver(grid=(c["grid"], c["grid"]), deposit_method="TSC", field_derivative="fourth",
+                                  slice_interpolation="quadratic", slicing=sl, precision=args.precision)
+        col = PICCollider(solver, device=0)
+        cols.append(col)
+        b1.append(generate_beam(col, c["n1"], params(E_CFG, EMASS_EV), E_CFG["beta"], E_CFG["alpha"], E_CFG["sigma"],
+                                seed=c["seed"], rng_id=1, device=dev))
+        b2.append(generate_beam(col, c["n2"], params(P_CFG, PMASS_EV), P_CFG["beta"], P_CFG["alpha"], P_CFG["sigma"],
+                                seed=c["seed"], rng_id=2, device=dev))
+        maps.append((linear6d_matrix(E_CFG["beta"], [2 * math.pi * (t + c["dtune"]) for t in E_CFG["tunes"]]),
+                     linear6d_matrix(P_CFG["beta"], [2 * math.pi * (t + c["dtune"]) for t in P_CFG["tunes"]])))
+
+    def turn(t):
+        if args.one_by_one:
+            lums = [col.collide(x, y, turn=t) for col, x, y in zip(cols, b1, b2)]
+        else:
+            lums = collide_many(cols, b1, b2, [t] * len(cols))
+        for col, x, y, (mx, my) in zip(cols, b1, b2, maps):
+            col.track_linear6d(x, mx)
+            col.track_linear6d(y, my)
+        return lums
+
+    for t in range(args.warmup):
+        turn(t)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(args.warmup, args.warmup + args.steps):
+        turn(t)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    particles = sum(c["n1"] + c["n2"] for c in cfgs)
+    visits = sum(c["n1"] * c["ns"] + c["n2"] * c["ns"] for c in cfgs)
+    print(json.dumps({
+        "metric": "aggregate turns/s of independent bunch-pair instances", "value": round(len(cfgs) / dt, 3),
+        "unit": "turns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32", "data": "synthetic",
+        "config": {"workload": f"{len(cfgs)} instances, N e- in [0.3M, 2.56M], N p in [0.12M, 1.024M], slices "
+                               "{7,11,15}, meshes {64,96,128}, TSC, 4th order, quadratic, one collide + maps each",
+                   "instances": cfgs, "particles": particles,
+                   "schedule": "one by one" if args.one_by_one else "collide_many (concurrent streams)"},
+        "particle_visit_GBps": round(88.0 * visits / dt / 1e9, 1)}))
 
 
 def measured_traffic(arm, family):
@@ -116,8 +184,13 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--arm", choices=["quadratic", "node"], default="quadratic")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="fp32: the FP32 form (deposit weights, FFT, field, kick gather in FP32; FP64 beams)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel HIP event timing")
+    ap.add_argument("--instances", type=int, default=0,
+                    help="batched form: this many independent bunch-pair instances per step (0: the single pair)")
+    ap.add_argument("--one-by-one", action="store_true", help="batched form: collide the instances one at a time")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run plain device copies of 1 GiB (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -135,10 +208,13 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "copies": 3}))
         return
+    if args.instances > 0:
+        run_instances(args)
+        return
     from octopic import EMASS_EV, PMASS_EV, PICCollider, generate_beam, linear6d_matrix
 
     dev = torch.device("cuda:0")
-    solver = make_solver(args.arm)
+    solver = make_solver(args.arm, args.precision)
     col = PICCollider(solver, device=0)
     e = generate_beam(col, E_CFG["n"], params(E_CFG, EMASS_EV), E_CFG["beta"], E_CFG["alpha"], E_CFG["sigma"],
                       seed=0, rng_id=E_CFG["rng_id"], device=dev)
@@ -210,20 +286,21 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic (Philox counter-RNG Gaussian beams generated on the device)",
-        "config": {"workload": f"plain FP64 PIC, {E_CFG['n']} e- / {P_CFG['n']} p, {NSLICES} slices, "
+        "config": {"workload": f"plain PIC, {E_CFG['n']} e- / {P_CFG['n']} p, {NSLICES} slices, "
                                f"{GRID}x{GRID}, TSC, 4th-order, "
                                + ("slice_pair + quadratic" if args.arm == "quadratic" else "node + linear")
+                               + (", FP32 form (FP64 beams, Green, moments, luminosity)" if args.precision == "fp32" else "")
                                + ", one collide + linear 6D maps per turn",
-                   "arm": args.arm, "global_batch": E_CFG["n"] + P_CFG["n"], "seq_len": NSLICES,
+                   "arm": args.arm, "precision": args.precision, "global_batch": E_CFG["n"] + P_CFG["n"], "seq_len": NSLICES,
                    "parallelism": "single GPU"},
         "roofline": roof,
         "luminosity_mean": sum(lums) / len(lums) if lums else None,
         "green_cache": {"hits": st["green_hits"], "misses": st["green_misses"], "rebuilds": st["green_rebuilds"]},
         "kernel_ms_per_turn": {k: round(v[0], 4) for k, v in fam.items()},
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.precision == "fp64":
         try:
             out["cpu_baseline"] = cpu_baseline(args.arm)
         except Exception as exc:  # the CPU leg is reported, never the measurement

@@ -57,6 +57,7 @@ enum {
     OCTOPIC_SLICE_SPECIFIED = 4
 };
 enum { OCTOPIC_CENTER_CENTROID = 0, OCTOPIC_CENTER_MIDPOINT = 1 };
+enum { OCTOPIC_PRECISION_FP64 = 0, OCTOPIC_PRECISION_FP32 = 1 };
 
 /* LongitudinalSlicing (interface.jl:800-835).  `positions` holds nslices-1 internal boundaries in
  * units of the bunch rms (method SPECIFIED only); it may be NULL otherwise. */
@@ -89,7 +90,10 @@ typedef struct octopic_options {
     int32_t batch_mode;            /* OCTOPIC_BATCH_*; recorded only */
     int32_t cuda_flags;            /* bit0 cuda_async, bit1 cuda_batch_fft, bit2 cuda_wavefront_fft,
                                       bit3 cuda_indexed_wavefront; recorded only */
-    int32_t reserved;
+    int32_t precision;             /* OCTOPIC_PRECISION_*: FP64 (the reference's Float64 path) or the FP32
+                                      form: deposit weights, FFT convolution, field and kick gather in FP32
+                                      on FP64 beams; Green evaluation, slice moments, luminosity in FP64.
+                                      The FP32 form has no counterpart in the reference. */
     double grid_extent_sigma;
     double min_extent_x, min_extent_y;
     double grid_quantize;
@@ -138,8 +142,9 @@ int octopic_options_default(octopic_options *opt);
 
 /* PICPoissonSolver{Float64}(...) validation + _validate_pic_solver (interface.jl:1384-1507,
  * pic_cpu.jl:517-576) and the workspace/cache set-up of _pic_cpu_workspace_pool! / _pic_green_cache!
- * (pic_cpu.jl:467-515).  `device` selects the GPU; the handle uses the null stream until
- * octopic_set_stream is called. */
+ * (pic_cpu.jl:467-515).  `device` selects the GPU.  The handle enqueues on a stream of its own, created
+ * with it (blocking: ordered with the null stream, concurrent with other handles), until
+ * octopic_set_stream names another (NULL restores its own). */
 int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **out);
 void octopic_destroy(octopic_handle *h);
 const char *octopic_last_error(const octopic_handle *h);
@@ -150,6 +155,15 @@ int octopic_set_stream(octopic_handle *h, void *hip_stream);
  * luminosity_schedule (interface.jl:2596-2620).  *luminosity is NaN when it is 0. */
 int octopic_collide(octopic_handle *h, octopic_beam *beam1, octopic_beam *beam2, int64_t turn,
                     int32_t compute_luminosity, double *luminosity);
+
+/* `count` independent collides (bunch-pair instances: distinct handles, beams, turns), each on its own
+ * handle's stream from its own host thread, so that small instances fill the GPU together.  Returns the
+ * first non-OK status; per-instance statuses go to `status` (may be NULL), luminosities to
+ * luminosity[k]; `turns` and `compute_luminosity` may be NULL (-1 and 1).  A handle may appear once.
+ * No counterpart in the reference (its instances are separate tasks). */
+int octopic_collide_many(octopic_handle *const *handles, octopic_beam *beams1, octopic_beam *beams2,
+                         const int64_t *turns, const int32_t *compute_luminosity, double *luminosity,
+                         int32_t *status, int32_t count);
 
 int octopic_get_stats(const octopic_handle *h, octopic_stats *out);
 

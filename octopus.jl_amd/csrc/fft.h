@@ -1,10 +1,11 @@
 // fft.h -- in-LDS mixed-radix Stockham FFT for the padded PIC meshes (P = 2g points per axis).
 //
-// One wave (64 lanes) transforms one P-point complex FP64 sequence held in LDS, ping-ponging between
-// two LDS buffers, with the twiddle table staged in LDS once per workgroup.  Stage s of radix r reads r inputs P/r apart,
-// twiddles them by W_P^(k*q*P/(Ns*r)) (k = j mod Ns, Ns = product of the earlier radices), applies
-// the r-point DFT and writes them Ns apart at (j/Ns)*Ns*r + k: the autosort (Stockham) form, so
-// the result is in natural order without a bit-reversal pass.
+// One wave (64 lanes) transforms one P-point complex sequence (double2 in the FP64 mode, float2 in
+// the FP32 mode) held in LDS, ping-ponging between two LDS buffers, with the twiddle table staged in
+// LDS once per workgroup.  Stage s of radix r reads r inputs P/r apart, twiddles them by
+// W_P^(k*q*P/(Ns*r)) (k = j mod Ns, Ns = product of the earlier radices), applies the r-point DFT and
+// writes them Ns apart at (j/Ns)*Ns*r + k: the autosort (Stockham) form, so the result is in natural
+// order without a bit-reversal pass.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,43 +18,59 @@ struct FFTPlanDev {
     int n;
     int nstages;
     int radix[kMaxStages];
-    const double2 *tw;   // tw[t] = exp(-2 pi i t / n), t in [0, n)
+    const double2 *tw;     // tw[t] = exp(-2 pi i t / n), t in [0, n), rounded from long double
+    const float2 *tw32;    // the same twiddles rounded to float
 };
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// Complex arithmetic over double2 / float2 (C::x, C::y of scalar type real_of<C>).
+template <class C> struct real_of_t;
+template <> struct real_of_t<double2> { using type = double; };
+template <> struct real_of_t<float2> { using type = float; };
+template <class C> using real_of = typename real_of_t<C>::type;
+
+template <class C>
+__device__ __forceinline__ C cx(real_of<C> re, real_of<C> im) {
+    C r;
+    r.x = re;
+    r.y = im;
+    return r;
 }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+template <class C>
+__device__ __forceinline__ C cmul(C a, C b) {
+    return cx<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <class C> __device__ __forceinline__ C cadd(C a, C b) { return cx<C>(a.x + b.x, a.y + b.y); }
+template <class C> __device__ __forceinline__ C csub(C a, C b) { return cx<C>(a.x - b.x, a.y - b.y); }
+template <class C> __device__ __forceinline__ C cconj(C a) { return cx<C>(a.x, -a.y); }
 // multiply by -i (SIGN = -1) or +i (SIGN = +1)
-template <int SIGN>
-__device__ __forceinline__ double2 mul_i(double2 a) {
-    return SIGN < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+template <int SIGN, class C>
+__device__ __forceinline__ C mul_i(C a) {
+    return SIGN < 0 ? cx<C>(a.y, -a.x) : cx<C>(-a.y, a.x);
 }
 
-template <int SIGN>
-__device__ __forceinline__ void dft2(double2 *v) {
-    double2 a = v[0], b = v[1];
+template <int SIGN, class C>
+__device__ __forceinline__ void dft2(C *v) {
+    const C a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
 }
-template <int SIGN>
-__device__ __forceinline__ void dft4(double2 *v) {
-    double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-    double2 s13 = cadd(v[1], v[3]), d13 = mul_i<SIGN>(csub(v[1], v[3]));
+template <int SIGN, class C>
+__device__ __forceinline__ void dft4(C *v) {
+    const C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    const C s13 = cadd(v[1], v[3]), d13 = mul_i<SIGN>(csub(v[1], v[3]));
     v[0] = cadd(s02, s13);
     v[2] = csub(s02, s13);
     v[1] = cadd(d02, d13);
     v[3] = csub(d02, d13);
 }
-template <int SIGN>
-__device__ __forceinline__ void dft3(double2 *v) {
-    const double c = 0.86602540378443864676;   // sqrt(3)/2
-    double2 t1 = cadd(v[1], v[2]);
-    double2 t2 = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
-    double2 d = csub(v[1], v[2]);
-    double2 t3 = mul_i<SIGN>(make_double2(c * d.x, c * d.y));
+template <int SIGN, class C>
+__device__ __forceinline__ void dft3(C *v) {
+    using R = real_of<C>;
+    const R c = (R)0.86602540378443864676;   // sqrt(3)/2
+    const C t1 = cadd(v[1], v[2]);
+    const C t2 = cx<C>(v[0].x - (R)0.5 * t1.x, v[0].y - (R)0.5 * t1.y);
+    const C d = csub(v[1], v[2]);
+    const C t3 = mul_i<SIGN>(cx<C>(c * d.x, c * d.y));
     v[0] = cadd(v[0], t1);
     v[1] = cadd(t2, t3);
     v[2] = csub(t2, t3);
@@ -66,23 +83,26 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int SIGN>
-__device__ __forceinline__ double2 twiddle(const double2 *tw, int t) {
-    double2 w = tw[t];
+template <int SIGN, class C>
+__device__ __forceinline__ C twiddle(const C *tw, int t) {
+    const C w = tw[t];
     return SIGN < 0 ? w : cconj(w);
 }
 
-// Copies the plan's twiddles into LDS (`tw_lds`, P entries); the caller barriers afterwards.
+// Copies the plan's twiddles of type C into LDS (`tw_lds`, P entries); the caller barriers afterwards.
 __device__ __forceinline__ void load_twiddles(const FFTPlanDev &P, double2 *tw_lds) {
     for (int t = threadIdx.x; t < P.n; t += blockDim.x) tw_lds[t] = P.tw[t];
+}
+__device__ __forceinline__ void load_twiddles(const FFTPlanDev &P, float2 *tw_lds) {
+    for (int t = threadIdx.x; t < P.n; t += blockDim.x) tw_lds[t] = P.tw32[t];
 }
 
 // Transforms the sequence in `a` (LDS) using `b` as scratch; the result is in `a` when the plan has an
 // even number of stages, else in `b` (see fft_result).  Called by the 64 lanes of one wave, which own
 // `a` and `b`; `tw` is the LDS twiddle table.  The caller orders its writes of `a` before the call
 // (wave_lds_sync or a workgroup barrier).
-template <int SIGN>
-__device__ void fft_lds(double2 *a, double2 *b, const FFTPlanDev &P, const double2 *tw, int lane) {
+template <int SIGN, class C>
+__device__ void fft_lds(C *a, C *b, const FFTPlanDev &P, const C *tw, int lane) {
     int Ns = 1;
     const int n = P.n;
     for (int s = 0; s < P.nstages; ++s) {
@@ -92,7 +112,7 @@ __device__ void fft_lds(double2 *a, double2 *b, const FFTPlanDev &P, const doubl
         if (r == 4) {
             for (int j = lane; j < m; j += 64) {
                 const int q0 = j / Ns, k = j - q0 * Ns;
-                double2 v[4];
+                C v[4];
                 v[0] = a[j];
                 v[1] = cmul(a[j + m], twiddle<SIGN>(tw, k * tstep));
                 v[2] = cmul(a[j + 2 * m], twiddle<SIGN>(tw, 2 * k * tstep));
@@ -104,7 +124,7 @@ __device__ void fft_lds(double2 *a, double2 *b, const FFTPlanDev &P, const doubl
         } else if (r == 2) {
             for (int j = lane; j < m; j += 64) {
                 const int q0 = j / Ns, k = j - q0 * Ns;
-                double2 v[2];
+                C v[2];
                 v[0] = a[j];
                 v[1] = cmul(a[j + m], twiddle<SIGN>(tw, k * tstep));
                 dft2<SIGN>(v);
@@ -114,7 +134,7 @@ __device__ void fft_lds(double2 *a, double2 *b, const FFTPlanDev &P, const doubl
         } else {
             for (int j = lane; j < m; j += 64) {
                 const int q0 = j / Ns, k = j - q0 * Ns;
-                double2 v[3];
+                C v[3];
                 v[0] = a[j];
                 v[1] = cmul(a[j + m], twiddle<SIGN>(tw, k * tstep));
                 v[2] = cmul(a[j + 2 * m], twiddle<SIGN>(tw, 2 * k * tstep));
@@ -124,12 +144,13 @@ __device__ void fft_lds(double2 *a, double2 *b, const FFTPlanDev &P, const doubl
             }
         }
         wave_lds_sync();
-        double2 *t = a; a = b; b = t;
+        C *t = a; a = b; b = t;
         Ns *= r;
     }
 }
 
-__device__ __forceinline__ double2 *fft_result(double2 *a, double2 *b, const FFTPlanDev &P) {
+template <class C>
+__device__ __forceinline__ C *fft_result(C *a, C *b, const FFTPlanDev &P) {
     return (P.nstages & 1) ? b : a;
 }
 

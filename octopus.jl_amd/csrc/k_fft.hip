@@ -55,30 +55,34 @@ __device__ __forceinline__ double green_cell(int i, int j, int nx, int ny, int i
 }
 
 // Unpack the FFT Z of (a + i b), a and b real, into A[k] (odd = 0) or B[k] (odd = 1), k in [0, P/2].
-__device__ __forceinline__ double2 unpack_one(const double2 *Z, int P, int k, int odd) {
-    const double2 z = Z[k];
-    const double2 zc = cconj(Z[k == 0 ? 0 : P - k]);
-    if (!odd) return make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y));
-    const double2 d = csub(z, zc);
-    return make_double2(0.5 * d.y, -0.5 * d.x);   // d / (2i)
+template <class C>
+__device__ __forceinline__ C unpack_one(const C *Z, int P, int k, int odd) {
+    using R = real_of<C>;
+    const C z = Z[k];
+    const C zc = cconj(Z[k == 0 ? 0 : P - k]);
+    if (!odd) return cx<C>((R)0.5 * (z.x + zc.x), (R)0.5 * (z.y + zc.y));
+    const C d = csub(z, zc);
+    return cx<C>((R)0.5 * d.y, (R)-0.5 * d.x);   // d / (2i)
 }
 
 // Dynamic LDS of every transform kernel: the twiddle table (P entries), then per wave group two
 // P-entry buffers a, b.
-__device__ __forceinline__ double2 *group_buffer(double2 *lds, int P, int grp) { return lds + P + (size_t)grp * 2 * P; }
+template <class C>
+__device__ __forceinline__ C *group_buffer(C *lds, int P, int grp) { return lds + P + (size_t)grp * 2 * P; }
 
 // Row-pass epilogue shared by the Green and charge row transforms: the block's 2*kFFTPerBlock rows
 // (ya0 + r, r = 2*grp + {0, 1}) of the x-half-spectrum, stored as out[k * kstride + ya0 + r] so that
 // the 2*kFFTPerBlock consecutive rows of one k form one contiguous run.
-__device__ __forceinline__ void store_row_spectra(double2 *lds, const FFTPlanDev &plan, int nrows, int ya0, double2 *out,
+template <class C>
+__device__ __forceinline__ void store_row_spectra(C *lds, const FFTPlanDev &plan, int nrows, int ya0, C *out,
                                                   int64_t kstride) {
     constexpr int R = 2 * kFFTPerBlock;
     const int P = plan.n, nk = P / 2 + 1;
     for (int t = threadIdx.x; t < nk * R; t += blockDim.x) {
         const int r = t % R, k = t / R;
         if (ya0 + r >= nrows) continue;
-        double2 *ga = group_buffer(lds, P, r >> 1);
-        const double2 *Z = fft_result(ga, ga + P, plan);
+        C *ga = group_buffer(lds, P, r >> 1);
+        const C *Z = fft_result(ga, ga + P, plan);
         out[(size_t)k * kstride + ya0 + r] = unpack_one(Z, P, k, r & 1);
     }
 }
@@ -134,15 +138,18 @@ __global__ __launch_bounds__(kFFTThreads) void k_green_cols(const GreenJob *jobs
 
 // Charge rows: int64 fixed-point rows (ya, ya+1) of plane -> x-half-spectrum Qh[plane][kx][y], y < ny.
 // Each charge cell is read once and reset to zero for the next use of the buffer.
+template <class C>
 __global__ __launch_bounds__(kFFTThreads) void k_rows_fwd(const Plane *planes, unsigned long long *charge, int64_t charge_stride,
-                                                          double2 *qh, int64_t qh_stride, int nx, int ny, double inv_scale,
+                                                          C *qh, int64_t qh_stride, int nx, int ny, double inv_scale,
                                                           FFTPlanDev px, const ErrorInfo *err) {
-    extern __shared__ double2 lds[];
+    using R = real_of<C>;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C *lds = reinterpret_cast<C *>(lds_raw);
     if (failed(err)) return;
     const int plane = blockIdx.y;
     const int P1 = 2 * nx;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
-    double2 *a = group_buffer(lds, P1, grp);
+    C *a = group_buffer(lds, P1, grp);
     const int ya0 = 2 * kFFTPerBlock * blockIdx.x, ya = ya0 + 2 * grp;
     const bool active = ya < ny;
     const bool hasb = ya + 1 < ny;
@@ -160,7 +167,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_fwd(const Plane *planes, u
                 if (ub) q[(size_t)(ya + 1) * nx + x] = 0ull;
             }
         }
-        a[x] = make_double2(va, vb);
+        a[x] = cx<C>((R)va, (R)vb);
     }
     __syncthreads();
     fft_lds<-1>(a, a + P1, px, lds, lane);
@@ -169,72 +176,102 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_fwd(const Plane *planes, u
 }
 
 // Column pass: FFT_y of the zero-padded column, times the Green half spectrum, IFFT_y, keep y < ny.
-__global__ __launch_bounds__(kFFTThreads) void k_cols_conv(const Plane *planes, double2 *qh, int64_t qh_stride,
-                                                           const double2 *gspec, int64_t slot_stride, int nx, int ny,
+// (One plane per block row: sharing a Green column across an interaction's planes in one block
+// measured slower, the lost parallelism costing more than the saved spectrum reads.)
+template <class C>
+__global__ __launch_bounds__(kFFTThreads) void k_cols_conv(const Plane *planes, C *qh, int64_t qh_stride,
+                                                           const C *gspec, int64_t slot_stride, int nx, int ny,
                                                            FFTPlanDev py, const ErrorInfo *err) {
-    extern __shared__ double2 lds[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C *lds = reinterpret_cast<C *>(lds_raw);
     if (failed(err)) return;
     const int plane = blockIdx.y;
     const int P2 = 2 * ny;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
-    double2 *a = group_buffer(lds, P2, grp), *b = a + P2;
+    C *a = group_buffer(lds, P2, grp), *b = a + P2;
     const int kx = blockIdx.x * kFFTPerBlock + grp;
     const bool active = kx <= nx;
-    double2 *col = qh + (size_t)plane * qh_stride + (size_t)(active ? kx : 0) * ny;
+    C *col = qh + (size_t)plane * qh_stride + (size_t)(active ? kx : 0) * ny;
     load_twiddles(py, lds);
-    for (int y = lane; y < P2; y += kFFTLanes) a[y] = (active && y < ny) ? col[y] : make_double2(0.0, 0.0);
+    for (int y = lane; y < P2; y += kFFTLanes) a[y] = (active && y < ny) ? col[y] : cx<C>(0, 0);
     __syncthreads();
     fft_lds<-1>(a, b, py, lds, lane);
-    double2 *Z = fft_result(a, b, py), *W = Z == a ? b : a;
+    C *Z = fft_result(a, b, py), *W = Z == a ? b : a;
     const int slot = planes[plane].slot;
-    const double2 *g = gspec + (size_t)(slot < 0 ? 0 : slot) * slot_stride + (size_t)(active ? kx : 0) * P2;
+    const C *g = gspec + (size_t)(slot < 0 ? 0 : slot) * slot_stride + (size_t)(active ? kx : 0) * P2;
     for (int y = lane; y < P2; y += kFFTLanes) Z[y] = cmul(Z[y], g[y]);
     wave_lds_sync();
     fft_lds<+1>(Z, W, py, lds, lane);
     if (!active) return;
-    const double2 *R = fft_result(Z, W, py);
-    for (int y = lane; y < ny; y += kFFTLanes) col[y] = R[y];
+    const C *Rs = fft_result(Z, W, py);
+    for (int y = lane; y < ny; y += kFFTLanes) col[y] = Rs[y];
 }
 
 // Row inverse: rows (ya, ya+1) from the half spectrum, IFFT_x, keep x < nx, scale 1/(P1 P2).  The
 // block gathers its 2*kFFTPerBlock rows together: thread (kk, grp) reads rows ya, ya+1 of column kk
 // (adjacent threads adjacent rows) and fills both kk and its Hermitian mirror P1 - kk.
-__global__ __launch_bounds__(kFFTThreads) void k_rows_inv(const Plane *planes, const double2 *qh, int64_t qh_stride, double *phi,
-                                                          int nx, int ny, FFTPlanDev px, const ErrorInfo *err) {
-    extern __shared__ double2 lds[];
+template <class C>
+__global__ __launch_bounds__(kFFTThreads) void k_rows_inv(const Plane *planes, const C *qh, int64_t qh_stride,
+                                                          real_of<C> *phi, int nx, int ny, FFTPlanDev px,
+                                                          const ErrorInfo *err) {
+    using R = real_of<C>;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C *lds = reinterpret_cast<C *>(lds_raw);
     if (failed(err)) return;
     const int plane = blockIdx.y;
     const int P1 = 2 * nx, P2 = 2 * ny;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
     const int ya0 = 2 * kFFTPerBlock * blockIdx.x;
-    const double2 *in = qh + (size_t)plane * qh_stride;
+    const C *in = qh + (size_t)plane * qh_stride;
     load_twiddles(px, lds);
     for (int t = threadIdx.x; t < (nx + 1) * kFFTPerBlock; t += blockDim.x) {
         const int g = t % kFFTPerBlock, kk = t / kFFTPerBlock;
         const int y = ya0 + 2 * g;
-        double2 A = make_double2(0.0, 0.0), B = A;
+        C A = cx<C>(0, 0), B = A;
         if (y < ny) A = in[(size_t)kk * ny + y];
         if (y + 1 < ny) B = in[(size_t)kk * ny + y + 1];
-        double2 *ag = group_buffer(lds, P1, g);
-        ag[kk] = make_double2(A.x - B.y, A.y + B.x);   // A + i B
+        C *ag = group_buffer(lds, P1, g);
+        ag[kk] = cx<C>(A.x - B.y, A.y + B.x);   // A + i B
         if (kk > 0 && kk < nx) {
             A = cconj(A);
             B = cconj(B);
-            ag[P1 - kk] = make_double2(A.x - B.y, A.y + B.x);
+            ag[P1 - kk] = cx<C>(A.x - B.y, A.y + B.x);
         }
     }
     __syncthreads();
-    double2 *a = group_buffer(lds, P1, grp);
+    C *a = group_buffer(lds, P1, grp);
     fft_lds<+1>(a, a + P1, px, lds, lane);
     const int ya = ya0 + 2 * grp;
     if (ya >= ny) return;
     const bool hasb = ya + 1 < ny;
-    const double2 *Z = fft_result(a, a + P1, px);
-    const double norm = 1.0 / ((double)P1 * (double)P2);
-    double *out = phi + (size_t)plane * nx * ny;
+    const C *Z = fft_result(a, a + P1, px);
+    const R norm = (R)(1.0 / ((double)P1 * (double)P2));
+    R *out = phi + (size_t)plane * nx * ny;
     for (int x = lane; x < nx; x += kFFTLanes) {
         out[(size_t)ya * nx + x] = Z[x].x * norm;
         if (hasb) out[(size_t)(ya + 1) * nx + x] = Z[x].y * norm;
+    }
+}
+
+// FP32 mode: the cached FP64 Green half spectrum of every freshly built slot, rounded to float, with
+// the table's (0, 0) value G00 taken out of every cell first (the DC term loses G00 * P1 * P2).  A
+// constant in G moves every potential of a plane by that constant times the plane's charge: it leaves
+// E = -grad(phi) unchanged, and phi_L - phi_R too (the L, R and M planes of an interaction deposit the
+// same particles on the same mesh), but it is the bulk of |phi|, and FP32 differences of large
+// potentials would lose the longitudinal kick.
+__global__ __launch_bounds__(256) void k_green_to_f32(const GreenJob *jobs, const double2 *spec, float2 *spec32,
+                                                      int64_t slot_stride, int nslots, int nx, int ny, int integrated,
+                                                      const ErrorInfo *err) {
+    if (failed(err)) return;
+    const GreenJob job = jobs[blockIdx.y];
+    if (!job.valid || job.slot < 0 || job.slot >= nslots) return;
+    const double2 *src = spec + (size_t)job.slot * slot_stride;
+    float2 *dst = spec32 + (size_t)job.slot * slot_stride;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < slot_stride; k += (int64_t)gridDim.x * blockDim.x)
+        dst[k] = make_float2((float)src[k].x, (float)src[k].y);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double g00 = green_cell(0, 0, nx, ny, integrated, job);
+        dst[0] = make_float2((float)(src[0].x - g00 * (double)(2 * nx) * (double)(2 * ny)), (float)src[0].y);
     }
 }
 
@@ -246,53 +283,56 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_inv(const Plane *planes, c
 constexpr int kFieldTile = 16;
 constexpr int kFieldHalo = kFieldTile + 4;
 
-__global__ __launch_bounds__(256) void k_field(const Inter *inters, const Plane *planes, const double *phi, double *table,
+// R = double (FP64 mode) or float (FP32 mode: phi, derivative and table in FP32).
+template <class R>
+__global__ __launch_bounds__(256) void k_field(const Inter *inters, const Plane *planes, const R *phi, R *table,
                                                int nx, int ny, int fourth, int npl, const ErrorInfo *err) {
     if (failed(err)) return;
     const int d = blockIdx.y;
     const Inter it = inters[d];
     if (!it.valid) return;
-    __shared__ double ph[kFieldHalo][kFieldHalo + 1];                 // [y][x], origin (tx0 - 2, ty0 - 2)
-    __shared__ double stage[kFieldTile * kFieldTile * 9];             // [lx][ly][3q + c]
+    __shared__ R ph[kFieldHalo][kFieldHalo + 1];                      // [y][x], origin (tx0 - 2, ty0 - 2)
+    __shared__ R stage[kFieldTile * kFieldTile * 9];                  // [lx][ly][3q + c]
     const int ntx = (nx + kFieldTile - 1) / kFieldTile;
     const int tx0 = (blockIdx.x % ntx) * kFieldTile, ty0 = (blockIdx.x / ntx) * kFieldTile;
     const int C = 3 * npl;
     const int lx = threadIdx.x % kFieldTile, ly = threadIdx.x / kFieldTile;
     const int x = tx0 + lx, y = ty0 + ly;
     const bool inside = x < nx && y < ny;
-    const double c4 = 1.0 / 12.0;
+    const R c4 = (R)(1.0 / 12.0);
     for (int q = 0; q < npl; ++q) {
         const bool live = q < it.nplanes;
         if (live) {
-            const double *P = phi + (size_t)(it.plane0 + q) * nx * ny;   // [y][x]
+            const R *P = phi + (size_t)(it.plane0 + q) * nx * ny;   // [y][x]
             for (int k = threadIdx.x; k < kFieldHalo * kFieldHalo; k += blockDim.x) {
                 const int hy = k / kFieldHalo, hx = k % kFieldHalo;
                 const int gx = tx0 - 2 + hx, gy = ty0 - 2 + hy;
-                ph[hy][hx] = (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? P[(size_t)gy * nx + gx] : 0.0;
+                ph[hy][hx] = (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? P[(size_t)gy * nx + gx] : (R)0;
             }
         }
         __syncthreads();
-        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        R v0 = 0, v1 = 0, v2 = 0;
         if (live && inside) {
             const Plane &pl = planes[it.plane0 + q];
             auto F = [&](int i, int j) { return ph[j - ty0 + 2][i - tx0 + 2]; };
-            const double hxi = 1.0 / pl.hx, hyi = 1.0 / pl.hy;
-            double ex, ey;
-            if (y == 0) ey = hyi * (1.5 * F(x, 0) - 2.0 * F(x, 1) + 0.5 * F(x, 2));
-            else if (y == ny - 1) ey = hyi * (-1.5 * F(x, ny - 1) + 2.0 * F(x, ny - 2) - 0.5 * F(x, ny - 3));
-            else if (fourth && (y == 1 || y == ny - 2)) ey = 0.5 * hyi * (F(x, y - 1) - F(x, y + 1));
-            else if (fourth) ey = c4 * hyi * ((F(x, y + 2) - F(x, y - 2)) + 8.0 * (F(x, y - 1) - F(x, y + 1)));
-            else ey = 0.5 * hyi * (F(x, y - 1) - F(x, y + 1));
-            if (x == 0) ex = hxi * (1.5 * F(0, y) - 2.0 * F(1, y) + 0.5 * F(2, y));
-            else if (x == nx - 1) ex = hxi * (-1.5 * F(nx - 1, y) + 2.0 * F(nx - 2, y) - 0.5 * F(nx - 3, y));
-            else if (fourth && (x == 1 || x == nx - 2)) ex = 0.5 * hxi * (F(x - 1, y) - F(x + 1, y));
-            else if (fourth) ex = c4 * hxi * ((F(x + 2, y) - F(x - 2, y)) + 8.0 * (F(x - 1, y) - F(x + 1, y)));
-            else ex = 0.5 * hxi * (F(x - 1, y) - F(x + 1, y));
+            const R hxi = (R)(1.0 / pl.hx), hyi = (R)(1.0 / pl.hy);
+            const R h5 = (R)0.5, one5 = (R)1.5, two = (R)2.0, eight = (R)8.0;
+            R ex, ey;
+            if (y == 0) ey = hyi * (one5 * F(x, 0) - two * F(x, 1) + h5 * F(x, 2));
+            else if (y == ny - 1) ey = hyi * (-one5 * F(x, ny - 1) + two * F(x, ny - 2) - h5 * F(x, ny - 3));
+            else if (fourth && (y == 1 || y == ny - 2)) ey = h5 * hyi * (F(x, y - 1) - F(x, y + 1));
+            else if (fourth) ey = c4 * hyi * ((F(x, y + 2) - F(x, y - 2)) + eight * (F(x, y - 1) - F(x, y + 1)));
+            else ey = h5 * hyi * (F(x, y - 1) - F(x, y + 1));
+            if (x == 0) ex = hxi * (one5 * F(0, y) - two * F(1, y) + h5 * F(2, y));
+            else if (x == nx - 1) ex = hxi * (-one5 * F(nx - 1, y) + two * F(nx - 2, y) - h5 * F(nx - 3, y));
+            else if (fourth && (x == 1 || x == nx - 2)) ex = h5 * hxi * (F(x - 1, y) - F(x + 1, y));
+            else if (fourth) ex = c4 * hxi * ((F(x + 2, y) - F(x - 2, y)) + eight * (F(x - 1, y) - F(x + 1, y)));
+            else ex = h5 * hxi * (F(x - 1, y) - F(x + 1, y));
             v0 = F(x, y);
             v1 = ex;
             v2 = ey;
         }
-        double *sv = stage + (lx * kFieldTile + ly) * 9 + 3 * q;
+        R *sv = stage + (lx * kFieldTile + ly) * 9 + 3 * q;
         sv[0] = v0; sv[1] = v1; sv[2] = v2;
         __syncthreads();
     }
@@ -331,43 +371,71 @@ hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, in
     return hipGetLastError();
 }
 
-hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long *charge, int64_t charge_stride, double2 *qh,
+template <class C>
+hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long *charge, int64_t charge_stride, C *qh,
                            int64_t qh_stride, int nx, int ny, double inv_scale, const FFTPlanDev &px, const ErrorInfo *err,
                            hipStream_t s) {
     if (nplanes <= 0) return hipSuccess;
-    ensure_lds((const void *)k_rows_fwd, fft_lds_bytes(2 * nx));
+    const size_t lds = fft_lds_bytes(2 * nx) / sizeof(double2) * sizeof(C);
+    ensure_lds((const void *)k_rows_fwd<C>, lds);
     dim3 g((ny + 2 * kFFTPerBlock - 1) / (2 * kFFTPerBlock), nplanes);
-    hipLaunchKernelGGL(k_rows_fwd, g, dim3(kFFTThreads), fft_lds_bytes(2 * nx), s, planes, charge, charge_stride, qh,
-                       qh_stride, nx, ny, inv_scale, px, err);
+    hipLaunchKernelGGL(k_rows_fwd<C>, g, dim3(kFFTThreads), lds, s, planes, charge, charge_stride, qh, qh_stride, nx, ny,
+                       inv_scale, px, err);
     return hipGetLastError();
 }
 
-hipError_t launch_cols_conv(const Plane *planes, int nplanes, double2 *qh, int64_t qh_stride, const double2 *gspec,
-                            int64_t slot_stride, int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, hipStream_t s) {
+template <class C>
+hipError_t launch_cols_conv(const Plane *planes, int nplanes, C *qh, int64_t qh_stride, const C *gspec, int64_t slot_stride,
+                            int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, hipStream_t s) {
     if (nplanes <= 0) return hipSuccess;
-    ensure_lds((const void *)k_cols_conv, fft_lds_bytes(2 * ny));
+    const size_t lds = fft_lds_bytes(2 * ny) / sizeof(double2) * sizeof(C);
+    ensure_lds((const void *)k_cols_conv<C>, lds);
     dim3 g((nx + 1 + kFFTPerBlock - 1) / kFFTPerBlock, nplanes);
-    hipLaunchKernelGGL(k_cols_conv, g, dim3(kFFTThreads), fft_lds_bytes(2 * ny), s, planes, qh, qh_stride, gspec, slot_stride,
-                       nx, ny, py, err);
+    hipLaunchKernelGGL(k_cols_conv<C>, g, dim3(kFFTThreads), lds, s, planes, qh, qh_stride, gspec, slot_stride, nx, ny, py,
+                       err);
     return hipGetLastError();
 }
 
-hipError_t launch_rows_inv(const Plane *planes, int nplanes, const double2 *qh, int64_t qh_stride, double *phi, int nx,
+template <class C>
+hipError_t launch_rows_inv(const Plane *planes, int nplanes, const C *qh, int64_t qh_stride, real_of<C> *phi, int nx,
                            int ny, const FFTPlanDev &px, const ErrorInfo *err, hipStream_t s) {
     if (nplanes <= 0) return hipSuccess;
-    ensure_lds((const void *)k_rows_inv, fft_lds_bytes(2 * nx));
+    const size_t lds = fft_lds_bytes(2 * nx) / sizeof(double2) * sizeof(C);
+    ensure_lds((const void *)k_rows_inv<C>, lds);
     dim3 g((ny + 2 * kFFTPerBlock - 1) / (2 * kFFTPerBlock), nplanes);
-    hipLaunchKernelGGL(k_rows_inv, g, dim3(kFFTThreads), fft_lds_bytes(2 * nx), s, planes, qh, qh_stride, phi, nx, ny, px, err);
+    hipLaunchKernelGGL(k_rows_inv<C>, g, dim3(kFFTThreads), lds, s, planes, qh, qh_stride, phi, nx, ny, px, err);
     return hipGetLastError();
 }
 
-hipError_t launch_field(const Inter *inters, int ninters, const Plane *planes, const double *phi, double *table, int nx,
-                        int ny, int fourth, int npl, const ErrorInfo *err, hipStream_t s) {
+template <class R>
+hipError_t launch_field(const Inter *inters, int ninters, const Plane *planes, const R *phi, R *table, int nx, int ny,
+                        int fourth, int npl, const ErrorInfo *err, hipStream_t s) {
     if (ninters <= 0) return hipSuccess;
     const int ntiles = ((nx + kFieldTile - 1) / kFieldTile) * ((ny + kFieldTile - 1) / kFieldTile);
-    hipLaunchKernelGGL(k_field, dim3(ntiles, ninters), dim3(kFieldTile * kFieldTile), 0, s, inters, planes, phi, table, nx,
-                       ny, fourth, npl, err);
+    hipLaunchKernelGGL(k_field<R>, dim3(ntiles, ninters), dim3(kFieldTile * kFieldTile), 0, s, inters, planes, phi, table,
+                       nx, ny, fourth, npl, err);
     return hipGetLastError();
 }
+
+hipError_t launch_green_to_f32(const GreenJob *jobs, int njobs, const double2 *spec, float2 *spec32, int64_t slot_stride,
+                               int nslots, int nx, int ny, int integrated, const ErrorInfo *err, hipStream_t s) {
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_green_to_f32, dim3(32, njobs), dim3(256), 0, s, jobs, spec, spec32, slot_stride, nslots, nx, ny,
+                       integrated, err);
+    return hipGetLastError();
+}
+
+#define OCTOPIC_CONV_INSTANCES(C)                                                                                     \
+    template hipError_t launch_rows_fwd<C>(const Plane *, int, unsigned long long *, int64_t, C *, int64_t, int, int,  \
+                                           double, const FFTPlanDev &, const ErrorInfo *, hipStream_t);                \
+    template hipError_t launch_cols_conv<C>(const Plane *, int, C *, int64_t, const C *, int64_t, int, int,            \
+                                            const FFTPlanDev &, const ErrorInfo *, hipStream_t);                       \
+    template hipError_t launch_rows_inv<C>(const Plane *, int, const C *, int64_t, real_of<C> *, int, int,             \
+                                           const FFTPlanDev &, const ErrorInfo *, hipStream_t);                        \
+    template hipError_t launch_field<real_of<C>>(const Inter *, int, const Plane *, const real_of<C> *, real_of<C> *, \
+                                                 int, int, int, int, const ErrorInfo *, hipStream_t);
+OCTOPIC_CONV_INSTANCES(double2)
+OCTOPIC_CONV_INSTANCES(float2)
+#undef OCTOPIC_CONV_INSTANCES
 
 }  // namespace octopic

@@ -60,10 +60,8 @@ __global__ __launch_bounds__(kBoundsThreads) void k_bounds(BoundsArgs a) {
                  *PY = a.st.c[side][3] + off, *Z = a.st.c[side][4] + off;
     double v[12];
     for (int q = 0; q < 12; q += 2) { v[q] = INFINITY; v[q + 1] = -INFINITY; }
-    for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
-        const double x = X[t], px = PX[t], y = Y[t], py = PY[t];
+    auto visit = [&](double x, double px, double y, double py, double z) {
         if (SRCFLD) {
-            const double z = Z[t];
             const double xl = x + px * sL, yl = y + py * sL, xr = x + px * sR, yr = y + py * sR;
             v[0] = jmin(jmin(v[0], xl), xr); v[1] = jmax(jmax(v[1], xl), xr);
             v[2] = jmin(jmin(v[2], yl), yr); v[3] = jmax(jmax(v[3], yl), yr);
@@ -75,7 +73,18 @@ __global__ __launch_bounds__(kBoundsThreads) void k_bounds(BoundsArgs a) {
             const double xv = x + px * sM, yv = y + py * sM;
             v[8] = jmin(v[8], xv); v[9] = jmax(v[9], xv); v[10] = jmin(v[10], yv); v[11] = jmax(v[11], yv);
         }
+    };
+    // two particles per step, their loads issued together (exact min/max: any order)
+    const int64_t step = blockDim.x;
+    int64_t t = lo + threadIdx.x;
+    for (; t + step < hi; t += 2 * step) {
+        const double x0 = X[t], px0 = PX[t], y0 = Y[t], py0 = PY[t], z0 = SRCFLD ? Z[t] : 0.0;
+        const double x1 = X[t + step], px1 = PX[t + step], y1 = Y[t + step], py1 = PY[t + step],
+                     z1 = SRCFLD ? Z[t + step] : 0.0;
+        visit(x0, px0, y0, py0, z0);
+        visit(x1, px1, y1, py1, z1);
     }
+    if (t < hi) visit(X[t], PX[t], Y[t], PY[t], SRCFLD ? Z[t] : 0.0);
     constexpr int q0 = SRCFLD ? 0 : 8, q1 = LUM ? 12 : 8;
     constexpr int kWaves = kBoundsThreads / 64;
     __shared__ unsigned long long red[kWaves][12];
@@ -157,7 +166,7 @@ __device__ __forceinline__ double quantize_extent(double w, double q) {
 }
 
 // _pic_interaction_grids (pic_cpu.jl:1674-1718).  Returns false for a degenerate extent.
-__device__ bool interaction_grids(const GeomOpts &o, double sxmin, double sxmax, double symin, double symax, double fxmin,
+__device__ __forceinline__ bool interaction_grids(const GeomOpts &o, double sxmin, double sxmax, double symin, double symax, double fxmin,
                                   double fxmax, double fymin, double fymax, Grid &sg, Grid &fg) {
     double width, height;
     if (!resolve_extent(jmax(sxmax - sxmin, fxmax - fxmin), jmax(symax - symin, fymax - fymin), o.min_x, o.min_y, width,
@@ -220,7 +229,7 @@ __device__ __forceinline__ void fill_record(octopic_interaction_record &r, int p
     r.field_grid[0] = fg.x0; r.field_grid[1] = fg.y0; r.field_grid[2] = fg.width; r.field_grid[3] = fg.height;
 }
 
-__device__ void set_plane(Plane &pl, int inter, int kind, int beam, int n1, int n2, int slot, int64_t off, int64_t n,
+__device__ __forceinline__ void set_plane(Plane &pl, int inter, int kind, int beam, int n1, int n2, int slot, int64_t off, int64_t n,
                           double s, double x0, double y0, double hx, double hy) {
     pl.inter = inter; pl.kind = kind; pl.beam = beam; pl.n1 = n1; pl.n2 = n2; pl.slot = slot; pl.count_box = 0;
     pl.reserved = 0; pl.off = off; pl.n = n; pl.s = s; pl.x0 = x0; pl.y0 = y0;
@@ -230,7 +239,7 @@ __device__ void set_plane(Plane &pl, int inter, int kind, int beam, int n1, int 
 
 // Luminosity mesh of one pair (pic_cpu.jl:2700-2717) and its two deposit planes.  l1, l2 are the
 // luminosity boxes (x min, x max, y min, y max) of slice i of beam 1 and slice j of beam 2 at sM.
-__device__ __noinline__ bool lum_geometry(const GeomOpts &o, const double *l1, const double *l2, int k, int p,
+__device__ __forceinline__ bool lum_geometry(const GeomOpts &o, const double *l1, const double *l2, int k, int p,
                                           const SliceTable *t1, const SliceTable *t2, int i, int j, Plane *planes,
                                           int lum_plane0, double *lumgeo, ErrorInfo *err) {
     double xmin = jmin(l1[0], l2[0]), xmax = jmax(l1[1], l2[1]);
@@ -262,7 +271,7 @@ __device__ __noinline__ bool lum_geometry(const GeomOpts &o, const double *l1, c
 // Every descriptor an interaction owns is made inert before anything can return early, so that no
 // later stage of the batch can read a stale or uninitialised descriptor.  Lane dir 1 of a pair also
 // owns the pair's two luminosity planes.
-__device__ void neutralise_inter(int k, int dir, int npairs, const GeomOpts &o, Inter *inters, Plane *planes,
+__device__ __forceinline__ void neutralise_inter(int k, int dir, int npairs, const GeomOpts &o, Inter *inters, Plane *planes,
                                  GreenJob *jobs) {
     const int d = 2 * k + dir - 1;
     inters[d].valid = 0;
@@ -282,7 +291,7 @@ __device__ void neutralise_inter(int k, int dir, int npairs, const GeomOpts &o, 
     }
 }
 
-__device__ void fill_inter(Inter &it, const GeomOpts &o, int p, int i, int j, int dir, const SliceTable *ts,
+__device__ __forceinline__ void fill_inter(Inter &it, const GeomOpts &o, int p, int i, int j, int dir, const SliceTable *ts,
                            const SliceTable *tf, int si, int fi) {
     const double c_src = ts->center[si];
     it.p = p; it.i = i; it.j = j; it.dir = dir;
@@ -399,7 +408,8 @@ constexpr int kDepositThreads = 1024;
 // chunks' source box: every position it deposits lies in that box (the sM and luminosity drifts lie
 // between sL and sR, and positions are x + px*s, monotonic in s), the first stencil node is monotonic
 // in the position, and positions off the mesh have zero weights.
-template <int METHOD>
+// W = double, or float in the FP32 mode (the weight product in FP32, then the same fixed point).
+template <int METHOD, class W>
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
     extern __shared__ unsigned long long tile[];
     if (failed(a.err)) return;
@@ -447,7 +457,9 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
         for (int m = 0; m < K; ++m)
 #pragma unroll
             for (int q = 0; q < K; ++q) {
-                const unsigned long long v = __double2ull_rn(wx[m] * wy[q] * scale);
+                unsigned long long v;
+                if constexpr (sizeof(W) == sizeof(double)) v = __double2ull_rn(wx[m] * wy[q] * scale);
+                else v = __float2ull_rn((float)wx[m] * (float)wy[q] * (float)scale);
                 if (v) atomicAdd(&tile[(by + q) * n1 + bx + m], v);
             }
     }
@@ -468,13 +480,17 @@ hipError_t launch_deposit(const DepositArgs &a, hipStream_t s) {
     if (a.nplanes <= 0) return hipSuccess;
     const size_t lds = (size_t)a.ncells * sizeof(unsigned long long);
     dim3 g((a.chunks + a.group - 1) / a.group, a.nplanes);
+#define OCTOPIC_DEPOSIT(M, W)                                                                                  \
+    do {                                                                                                       \
+        hipFuncSetAttribute((const void *)k_deposit<M, W>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        hipLaunchKernelGGL((k_deposit<M, W>), g, dim3(kDepositThreads), lds, s, a);                            \
+    } while (0)
     if (a.method == 0) {
-        hipFuncSetAttribute((const void *)k_deposit<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_deposit<0>, g, dim3(kDepositThreads), lds, s, a);
+        if (a.fp32) OCTOPIC_DEPOSIT(0, float); else OCTOPIC_DEPOSIT(0, double);
     } else {
-        hipFuncSetAttribute((const void *)k_deposit<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_deposit<1>, g, dim3(kDepositThreads), lds, s, a);
+        if (a.fp32) OCTOPIC_DEPOSIT(1, float); else OCTOPIC_DEPOSIT(1, double);
     }
+#undef OCTOPIC_DEPOSIT
     return hipGetLastError();
 }
 
@@ -485,33 +501,33 @@ constexpr int kKickLdsBytes = 36 * 1024;   // LDS window of the field table per 
 // Gather of one particle's field from the interleaved table rows starting at e0 (row = one x, rows
 // `rs` doubles apart, C doubles per cell), for the linear (MODE 0) or quadratic (MODE 1) slice
 // interpolation (pic_cpu.jl:1218-1273, 2596-2680).
-template <int METHOD, int MODE>
-__device__ __forceinline__ void gather_field(const double *e0, int rs, int C, const double wx[3], const double wy[3],
-                                             double zL, double &Kx, double &Ky, double &Kz) {
+template <int METHOD, int MODE, class R>
+__device__ __forceinline__ void gather_field(const R *e0, int rs, int C, const R wx[3], const R wy[3], R zL, R &Kx, R &Ky,
+                                             R &Kz) {
     constexpr int K = METHOD == 0 ? 2 : 3;
     if (MODE == 0) {
-        const double zR = 1.0 - zL;
+        const R zR = (R)1 - zL;
 #pragma unroll
         for (int m = 0; m < K; ++m)
 #pragma unroll
             for (int q = 0; q < K; ++q) {
-                const double *e = e0 + m * rs + q * C;
-                const double w = wx[m] * wy[q];
+                const R *e = e0 + m * rs + q * C;
+                const R w = wx[m] * wy[q];
                 Kx += w * (zL * e[1] + zR * e[4]);
                 Ky += w * (zL * e[2] + zR * e[5]);
                 Kz += w * (e[0] - e[3]);
             }
     } else {
-        const double tt = 1.0 - zL;
-        const double t2 = tt * tt;
-        const double aL = 2.0 * t2 - 3.0 * tt + 1.0, aM = 4.0 * tt - 4.0 * t2, aR = 2.0 * t2 - tt;
-        const double bL = 3.0 - 4.0 * tt, bM = 8.0 * tt - 4.0, bR = 1.0 - 4.0 * tt;
+        const R tt = (R)1 - zL;
+        const R t2 = tt * tt;
+        const R aL = (R)2 * t2 - (R)3 * tt + (R)1, aM = (R)4 * tt - (R)4 * t2, aR = (R)2 * t2 - tt;
+        const R bL = (R)3 - (R)4 * tt, bM = (R)8 * tt - (R)4, bR = (R)1 - (R)4 * tt;
 #pragma unroll
         for (int m = 0; m < K; ++m)
 #pragma unroll
             for (int q = 0; q < K; ++q) {
-                const double *e = e0 + m * rs + q * C;   // L = 0..2, R = 3..5, M = 6..8
-                const double w = wx[m] * wy[q];
+                const R *e = e0 + m * rs + q * C;   // L = 0..2, R = 3..5, M = 6..8
+                const R w = wx[m] * wy[q];
                 Kx += w * (aL * e[1] + aM * e[7] + aR * e[4]);
                 Ky += w * (aL * e[2] + aM * e[8] + aR * e[5]);
                 Kz += w * (bL * e[0] + bM * e[6] + bR * e[3]);
@@ -522,9 +538,12 @@ __device__ __forceinline__ void gather_field(const double *e0, int rs, int C, co
 // One block takes one chunk of an interaction's field slice.  In the slice-pair modes the table cells
 // of the chunk's stencil window (from the chunk's field box) are staged in LDS when they fit; a
 // particle whose stencil leaves the window reads the table in HBM.
-template <int METHOD, int MODE>
+// R = double (FP64 mode) or float (FP32 mode: table, weights and gather sums in FP32; the kick is
+// applied to the FP64 coordinates).
+template <int METHOD, int MODE, class R>
 __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
-    extern __shared__ double win[];
+    extern __shared__ __align__(16) unsigned char win_raw[];
+    R *win = reinterpret_cast<R *>(win_raw);
     if (failed(a.err)) return;
     const int lblk = xcd_logical_block(blockIdx.x, gridDim.x);
     if (lblk >= a.chunks * a.ninters) return;
@@ -534,7 +553,7 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     const int64_t n = it.fld_n;
     const int64_t lo = n * chunk / a.chunks, hi = n * (chunk + 1) / a.chunks;
     const int nx = a.nx, ny = a.ny, C = 3 * a.npl;
-    const double *tab = a.table + (size_t)d * nx * ny * C;
+    const R *tab = static_cast<const R *>(a.table) + (size_t)d * nx * ny * C;
     double *X = a.st.c[it.fld_beam][0] + it.fld_off, *PX = a.st.c[it.fld_beam][1] + it.fld_off;
     double *Y = a.st.c[it.fld_beam][2] + it.fld_off, *PY = a.st.c[it.fld_beam][3] + it.fld_off;
     const double *Z = a.st.c[it.fld_beam][4] + it.fld_off;
@@ -558,7 +577,7 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
         int x0, x1, y0, y1;
         stencil_window<METHOD>((b[0] - it.fg.x0) / hx, (b[1] - it.fg.x0) / hx, nx, x0, x1);
         stencil_window<METHOD>((b[2] - it.fg.y0) / hy, (b[3] - it.fg.y0) / hy, ny, y0, y1);
-        if ((x1 - x0 + 1) * (y1 - y0 + 1) * C * (int)sizeof(double) <= kKickLdsBytes) {
+        if ((x1 - x0 + 1) * (y1 - y0 + 1) * C * (int)sizeof(R) <= kKickLdsBytes) {
             wx0 = x0; wy0 = y0; wnx = x1 - x0 + 1; wny = y1 - y0 + 1;
             const int run = wny * C;
             for (int k = threadIdx.x; k < wnx * run; k += blockDim.x) {
@@ -581,50 +600,53 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
         if (it.count_field && !(__builtin_isfinite(x) && it.fg.x0 <= x && x <= fx1 && __builtin_isfinite(y) &&
                                 it.fg.y0 <= y && y <= fy1))
             ++dropped;
-        const double zL = jclamp(-z * hzi + zbias, 0.0, 1.0);
-        double Kx = 0.0, Ky = 0.0, Kz = 0.0;
+        const double zLd = jclamp(-z * hzi + zbias, 0.0, 1.0);
+        const R zL = (R)zLd;
+        R Kx = 0, Ky = 0, Kz = 0;
         int bx, by;
-        double wx[3], wy[3];
-        stencil<METHOD>((x - it.fg.x0) / hx, nx, bx, wx);
-        stencil<METHOD>((y - it.fg.y0) / hy, ny, by, wy);
+        double wxd[3], wyd[3];
+        stencil<METHOD>((x - it.fg.x0) / hx, nx, bx, wxd);
+        stencil<METHOD>((y - it.fg.y0) / hy, ny, by, wyd);
+        const R wx[3] = {(R)wxd[0], (R)wxd[1], (R)wxd[2]}, wy[3] = {(R)wyd[0], (R)wyd[1], (R)wyd[2]};
         if (MODE != 2) {
             if (bx >= wx0 && bx + K <= wx0 + wnx && by >= wy0 && by + K <= wy0 + wny)
-                gather_field<METHOD, MODE>(win + ((bx - wx0) * wny + (by - wy0)) * C, wny * C, C, wx, wy, zL, Kx, Ky, Kz);
+                gather_field<METHOD, MODE, R>(win + ((bx - wx0) * wny + (by - wy0)) * C, wny * C, C, wx, wy, zL, Kx, Ky, Kz);
             else
-                gather_field<METHOD, MODE>(tab + ((size_t)bx * ny + by) * C, ny * C, C, wx, wy, zL, Kx, Ky, Kz);
+                gather_field<METHOD, MODE, R>(tab + ((size_t)bx * ny + by) * C, ny * C, C, wx, wy, zL, Kx, Ky, Kz);
         } else {
             // :node (pic_cpu.jl:1507-1530): L on its own mesh, R on its own mesh, phi_L - phi_Z on L's mesh
-            double KxL = 0.0, KyL = 0.0, KxR = 0.0, KyR = 0.0;
+            R KxL = 0, KyL = 0, KxR = 0, KyR = 0;
 #pragma unroll
             for (int m = 0; m < K; ++m)
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
-                    const double *e = tab + ((size_t)(bx + m) * ny + (by + q)) * C;
-                    const double w = wx[m] * wy[q];
-                    KxL += w * (1.0 * e[1] + 0.0 * e[1]);
-                    KyL += w * (1.0 * e[2] + 0.0 * e[2]);
+                    const R *e = tab + ((size_t)(bx + m) * ny + (by + q)) * C;
+                    const R w = wx[m] * wy[q];
+                    KxL += w * ((R)1 * e[1] + (R)0 * e[1]);
+                    KyL += w * ((R)1 * e[2] + (R)0 * e[2]);
                     Kz += w * (e[0] - e[6]);
                 }
             int bxr, byr;
-            double wxr[3], wyr[3];
-            stencil<METHOD>((x - it.fgR.x0) / hxR, nx, bxr, wxr);
-            stencil<METHOD>((y - it.fgR.y0) / hyR, ny, byr, wyr);
+            double wxrd[3], wyrd[3];
+            stencil<METHOD>((x - it.fgR.x0) / hxR, nx, bxr, wxrd);
+            stencil<METHOD>((y - it.fgR.y0) / hyR, ny, byr, wyrd);
+            const R wxr[3] = {(R)wxrd[0], (R)wxrd[1], (R)wxrd[2]}, wyr[3] = {(R)wyrd[0], (R)wyrd[1], (R)wyrd[2]};
 #pragma unroll
             for (int m = 0; m < K; ++m)
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
-                    const double *e = tab + ((size_t)(bxr + m) * ny + (byr + q)) * C;
-                    const double w = wxr[m] * wyr[q];
-                    KxR += w * (1.0 * e[4] + 0.0 * e[4]);
-                    KyR += w * (1.0 * e[5] + 0.0 * e[5]);
+                    const R *e = tab + ((size_t)(bxr + m) * ny + (byr + q)) * C;
+                    const R w = wxr[m] * wyr[q];
+                    KxR += w * ((R)1 * e[4] + (R)0 * e[4]);
+                    KyR += w * ((R)1 * e[5] + (R)0 * e[5]);
                 }
-            const double zR = 1.0 - zL;
+            const R zR = (R)1 - zL;
             Kx = zL * KxL + zR * KxR;
             Ky = zL * KyL + zR * KyR;
         }
-        px += ks * Kx;
-        py += ks * Ky;
-        if (lk) pz += ks * Kz * hzi;
+        px += ks * (double)Kx;
+        py += ks * (double)Ky;
+        if (lk) pz += ks * (double)Kz * hzi;
         const double s2 = 0.5 * (c - z);
         x += s2 * px;
         y += s2 * py;
@@ -641,10 +663,17 @@ hipError_t launch_kick(const KickArgs &a, hipStream_t s) {
     if (a.ninters <= 0) return hipSuccess;
     dim3 g(xcd_padded(a.chunks * a.ninters)), b(kKickThreads);
     const size_t lds = (a.mode != 2 && a.chunkbox) ? kKickLdsBytes : 0;
-#define OCTOPIC_KICK(M, MODE)                                                                             \
-    do {                                                                                                  \
-        hipFuncSetAttribute((const void *)k_kick<M, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
-        hipLaunchKernelGGL((k_kick<M, MODE>), g, b, lds, s, a);                                           \
+#define OCTOPIC_KICK(M, MODE)                                                                                    \
+    do {                                                                                                         \
+        if (a.fp32) {                                                                                            \
+            hipFuncSetAttribute((const void *)k_kick<M, MODE, float>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                                (int)lds);                                                                       \
+            hipLaunchKernelGGL((k_kick<M, MODE, float>), g, b, lds, s, a);                                       \
+        } else {                                                                                                 \
+            hipFuncSetAttribute((const void *)k_kick<M, MODE, double>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)lds);                                                                       \
+            hipLaunchKernelGGL((k_kick<M, MODE, double>), g, b, lds, s, a);                                      \
+        }                                                                                                        \
     } while (0)
     if (a.method == 0) {
         if (a.mode == 0) OCTOPIC_KICK(0, 0); else if (a.mode == 1) OCTOPIC_KICK(0, 1); else OCTOPIC_KICK(0, 2);
@@ -718,24 +747,24 @@ namespace octopic {
 
 constexpr int kNodeThreads = 256;
 
-__device__ __forceinline__ void block_minmax4(double v[4], double *out) {
-    __shared__ double red[4][kNodeThreads];
-    for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = v[q];
-    __syncthreads();
-    for (int w = kNodeThreads / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            red[0][threadIdx.x] = jmin(red[0][threadIdx.x], red[0][threadIdx.x + w]);
-            red[1][threadIdx.x] = jmax(red[1][threadIdx.x], red[1][threadIdx.x + w]);
-            red[2][threadIdx.x] = jmin(red[2][threadIdx.x], red[2][threadIdx.x + w]);
-            red[3][threadIdx.x] = jmax(red[3][threadIdx.x], red[3][threadIdx.x + w]);
+// Folds this wave's 4 extrema (x min, x max, y min, y max) into 4 bound slots with 64-bit atomicMax.
+__device__ __forceinline__ void wave_fold_box(const double v[4], unsigned long long *slots) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        unsigned long long u = bound_slot(v[q], (q & 1) == 0);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long w = __shfl_xor(u, o);
+            u = w > u ? w : u;
         }
-        __syncthreads();
+        if ((threadIdx.x & 63) == 0 && u) atomicMax(&slots[q], u);
     }
-    if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
-    __syncthreads();
 }
 
-// grid (nchunks, ns_src): source boxes at every node drift; partial[((i*nb + b)*nchunks + c)*4]
+constexpr int kNodeGroup = 4;   // node drifts (source bounds) or source slices (field bounds) per pass
+
+// grid (nchunks, ns_src): source boxes at every node drift, folded into a.skeys[(i*nb + b)*4 + q];
+// kNodeGroup drifts per pass over the chunk
 __global__ __launch_bounds__(kNodeThreads) void k_node_src_bounds(NodeArgs a) {
     if (failed(a.err)) return;
     const int i = blockIdx.y;
@@ -745,47 +774,62 @@ __global__ __launch_bounds__(kNodeThreads) void k_node_src_bounds(NodeArgs a) {
     const int64_t lo = n * blockIdx.x / a.nchunks, hi = n * (blockIdx.x + 1) / a.nchunks;
     const double *X = a.st.c[a.src_beam][0] + off, *PX = a.st.c[a.src_beam][1] + off;
     const double *Y = a.st.c[a.src_beam][2] + off, *PY = a.st.c[a.src_beam][3] + off;
-    for (int b = 0; b < nbF; ++b) {
-        const double d = 0.5 * (c - a.tab_fld->boundary[b]);
-        double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
-        for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
-            const double xv = X[t] + PX[t] * d, yv = Y[t] + PY[t] * d;
-            v[0] = jmin(v[0], xv); v[1] = jmax(v[1], xv); v[2] = jmin(v[2], yv); v[3] = jmax(v[3], yv);
+    for (int b0 = 0; b0 < nbF; b0 += kNodeGroup) {
+        double d[kNodeGroup], v[kNodeGroup][4];
+#pragma unroll
+        for (int g = 0; g < kNodeGroup; ++g) {
+            d[g] = b0 + g < nbF ? 0.5 * (c - a.tab_fld->boundary[b0 + g]) : 0.0;
+            v[g][0] = INFINITY; v[g][1] = -INFINITY; v[g][2] = INFINITY; v[g][3] = -INFINITY;
         }
-        block_minmax4(v, a.partial + (((size_t)i * nbF + b) * a.nchunks + blockIdx.x) * 4);
+        for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+            const double x = X[t], px = PX[t], y = Y[t], py = PY[t];
+#pragma unroll
+            for (int g = 0; g < kNodeGroup; ++g) {
+                const double xv = x + px * d[g], yv = y + py * d[g];
+                v[g][0] = jmin(v[g][0], xv); v[g][1] = jmax(v[g][1], xv);
+                v[g][2] = jmin(v[g][2], yv); v[g][3] = jmax(v[g][3], yv);
+            }
+        }
+        for (int g = 0; g < kNodeGroup && b0 + g < nbF; ++g) wave_fold_box(v[g], a.skeys + ((size_t)i * nbF + b0 + g) * 4);
     }
 }
 
-// grid (nchunks, ns_fld, ns_src): field slice sl drifted to the centre of source slice i
-__global__ __launch_bounds__(kNodeThreads) void k_node_fld_bounds(NodeArgs a, double *fpartial) {
+// grid (nchunks, ns_fld, ceil(ns_src / kNodeGroup)): field slice sl drifted to the centres of
+// kNodeGroup source slices i, folded into a.fkeys[(i*ns_fld + sl)*4 + q]
+__global__ __launch_bounds__(kNodeThreads) void k_node_fld_bounds(NodeArgs a) {
     if (failed(a.err)) return;
-    const int sl = blockIdx.y, i = blockIdx.z;
-    const int nsF = a.tab_fld->ns;
-    const double c = a.tab_src->center[i];
+    const int sl = blockIdx.y, i0 = blockIdx.z * kNodeGroup;
+    const int nsF = a.tab_fld->ns, nsS = a.tab_src->ns;
     const int64_t off = a.tab_fld->offset[sl], n = a.tab_fld->count[sl];
     const int64_t lo = n * blockIdx.x / a.nchunks, hi = n * (blockIdx.x + 1) / a.nchunks;
     const double *X = a.st.c[a.fld_beam][0] + off, *PX = a.st.c[a.fld_beam][1] + off;
     const double *Y = a.st.c[a.fld_beam][2] + off, *PY = a.st.c[a.fld_beam][3] + off;
     const double *Z = a.st.c[a.fld_beam][4] + off;
-    double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
-    for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
-        const double sh = 0.5 * (Z[t] - c);
-        const double xv = X[t] + sh * PX[t], yv = Y[t] + sh * PY[t];
-        v[0] = jmin(v[0], xv); v[1] = jmax(v[1], xv); v[2] = jmin(v[2], yv); v[3] = jmax(v[3], yv);
+    double c[kNodeGroup], v[kNodeGroup][4];
+#pragma unroll
+    for (int g = 0; g < kNodeGroup; ++g) {
+        c[g] = i0 + g < nsS ? a.tab_src->center[i0 + g] : 0.0;
+        v[g][0] = INFINITY; v[g][1] = -INFINITY; v[g][2] = INFINITY; v[g][3] = -INFINITY;
     }
-    block_minmax4(v, fpartial + (((size_t)i * nsF + sl) * a.nchunks + blockIdx.x) * 4);
+    for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+        const double x = X[t], px = PX[t], y = Y[t], py = PY[t], z = Z[t];
+#pragma unroll
+        for (int g = 0; g < kNodeGroup; ++g) {
+            const double sh = 0.5 * (z - c[g]);
+            const double xv = x + sh * px, yv = y + sh * py;
+            v[g][0] = jmin(v[g][0], xv); v[g][1] = jmax(v[g][1], xv);
+            v[g][2] = jmin(v[g][2], yv); v[g][3] = jmax(v[g][3], yv);
+        }
+    }
+    for (int g = 0; g < kNodeGroup && i0 + g < nsS; ++g) wave_fold_box(v[g], a.fkeys + ((size_t)(i0 + g) * nsF + sl) * 4);
 }
 
-__device__ __forceinline__ void reduce4(const double *p, int nchunks, double v[4]) {
-    v[0] = INFINITY; v[1] = -INFINITY; v[2] = INFINITY; v[3] = -INFINITY;
-    for (int c = 0; c < nchunks; ++c) {
-        v[0] = jmin(v[0], p[4 * c]); v[1] = jmax(v[1], p[4 * c + 1]);
-        v[2] = jmin(v[2], p[4 * c + 2]); v[3] = jmax(v[3], p[4 * c + 3]);
-    }
+__device__ __forceinline__ void read_box(const unsigned long long *slots, double v[4]) {
+    for (int q = 0; q < 4; ++q) v[q] = bound_value(slots[q], (q & 1) == 0);
 }
 
 // one lane per (source slice i, node b)
-__global__ void k_node_meshes(NodeArgs a, const double *fpartial) {
+__global__ void k_node_meshes(NodeArgs a) {
     const int nsS = a.tab_src->ns, nsF = a.tab_fld->ns, nbF = nsF + 1;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsS * nbF) return;
@@ -801,8 +845,8 @@ __global__ void k_node_meshes(NodeArgs a, const double *fpartial) {
     if (a.tab_src->count[i] == 0) return;
     const int bn = b + 1 < nbF ? b + 1 : nbF - 1;
     double s0[4], s1[4];
-    reduce4(a.partial + ((size_t)i * nbF + b) * a.nchunks * 4, a.nchunks, s0);
-    reduce4(a.partial + ((size_t)i * nbF + bn) * a.nchunks * 4, a.nchunks, s1);
+    read_box(a.skeys + ((size_t)i * nbF + b) * 4, s0);
+    read_box(a.skeys + ((size_t)i * nbF + bn) * 4, s1);
     for (int q = 0; q < 4; ++q)
         if (s0[q] != s0[q] || s1[q] != s1[q]) { set_error(a.err, OCTOPIC_ERR_NONFINITE, -1, a.dir, 4, 0, 0); return; }
     const double sxmin = jmin(s0[0], s1[0]), sxmax = jmax(s0[1], s1[1]);
@@ -811,7 +855,7 @@ __global__ void k_node_meshes(NodeArgs a, const double *fpartial) {
     for (int adj = b - 1; adj <= b; ++adj) {
         if (adj < 0 || adj >= nsF) continue;
         double f[4];
-        reduce4(fpartial + ((size_t)i * nsF + adj) * a.nchunks * 4, a.nchunks, f);
+        read_box(a.fkeys + ((size_t)i * nsF + adj) * 4, f);
         for (int q = 0; q < 4; ++q)
             if (f[q] != f[q]) { set_error(a.err, OCTOPIC_ERR_NONFINITE, -1, a.dir, 4, 0, 0); return; }
         if (!__builtin_isfinite(f[0])) continue;
@@ -835,11 +879,12 @@ __global__ void k_node_meshes(NodeArgs a, const double *fpartial) {
     atomicAdd(&a.counters[3], 1ull);
 }
 
-hipError_t launch_node_prepare(const NodeArgs &a, double *fpartial, int ns_src, int ns_fld, hipStream_t s) {
+hipError_t launch_node_prepare(const NodeArgs &a, int ns_src, int ns_fld, hipStream_t s) {
     hipLaunchKernelGGL(k_node_src_bounds, dim3(a.nchunks, ns_src), dim3(kNodeThreads), 0, s, a);
-    hipLaunchKernelGGL(k_node_fld_bounds, dim3(a.nchunks, ns_fld, ns_src), dim3(kNodeThreads), 0, s, a, fpartial);
+    hipLaunchKernelGGL(k_node_fld_bounds, dim3(a.nchunks, ns_fld, (ns_src + kNodeGroup - 1) / kNodeGroup),
+                       dim3(kNodeThreads), 0, s, a);
     const int nn = ns_src * (ns_fld + 1);
-    hipLaunchKernelGGL(k_node_meshes, dim3((nn + 63) / 64), dim3(64), 0, s, a, (const double *)fpartial);
+    hipLaunchKernelGGL(k_node_meshes, dim3((nn + 63) / 64), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

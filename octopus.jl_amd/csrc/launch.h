@@ -112,6 +112,7 @@ struct DepositArgs {
     int ncells;                  // n1*n2 of these planes
     int chunks;                  // the batch's chunking of each slice
     int group;                   // chunks per block
+    int fp32;                    // weight product in FP32
     double scale;                // fixed-point scale 2^k
     unsigned long long *charge;  // [plane][charge_stride]
     int64_t charge_stride;
@@ -125,7 +126,8 @@ struct KickArgs {
     const Inter *inters;
     int ninters;
     States st;
-    const double *table;
+    const void *table;           // double (FP64 mode) or float (fp32) kick table
+    int fp32;
     int nx, ny, npl, method, mode;   // mode: 0 linear, 1 quadratic, 2 node
     int lkick;
     int chunks;
@@ -155,15 +157,21 @@ hipError_t launch_lum(const LumArgs &a, hipStream_t s);
 hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride, int nslots, int nx, int ny,
                               int integrated, const FFTPlanDev &px, const FFTPlanDev &py, const ErrorInfo *err,
                               hipStream_t s);
-hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long *charge, int64_t charge_stride, double2 *qh,
+template <class C>   // C = double2 (FP64 mode) or float2 (FP32 mode)
+hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long *charge, int64_t charge_stride, C *qh,
                            int64_t qh_stride, int nx, int ny, double inv_scale, const FFTPlanDev &px, const ErrorInfo *err,
                            hipStream_t s);
-hipError_t launch_cols_conv(const Plane *planes, int nplanes, double2 *qh, int64_t qh_stride, const double2 *gspec,
-                            int64_t slot_stride, int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, hipStream_t s);
-hipError_t launch_rows_inv(const Plane *planes, int nplanes, const double2 *qh, int64_t qh_stride, double *phi, int nx,
+template <class C>
+hipError_t launch_cols_conv(const Plane *planes, int nplanes, C *qh, int64_t qh_stride, const C *gspec, int64_t slot_stride,
+                            int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, hipStream_t s);
+template <class C>
+hipError_t launch_rows_inv(const Plane *planes, int nplanes, const C *qh, int64_t qh_stride, real_of<C> *phi, int nx,
                            int ny, const FFTPlanDev &px, const ErrorInfo *err, hipStream_t s);
-hipError_t launch_field(const Inter *inters, int ninters, const Plane *planes, const double *phi, double *table, int nx,
-                        int ny, int fourth, int npl, const ErrorInfo *err, hipStream_t s);
+template <class R>
+hipError_t launch_field(const Inter *inters, int ninters, const Plane *planes, const R *phi, R *table, int nx, int ny,
+                        int fourth, int npl, const ErrorInfo *err, hipStream_t s);
+hipError_t launch_green_to_f32(const GreenJob *jobs, int njobs, const double2 *spec, float2 *spec32, int64_t slot_stride,
+                               int nslots, int nx, int ny, int integrated, const ErrorInfo *err, hipStream_t s);
 
 // Node-mode mesh preparation (k_node.hip)
 struct NodeArgs {
@@ -172,7 +180,8 @@ struct NodeArgs {
     int src_beam, fld_beam;
     int dir;                     // 1 or 2
     int nchunks;
-    double *partial;             // source boxes [ns_src][nb][nchunks][4]
+    unsigned long long *skeys;   // source boxes [ns_src][nb][4] as bound slots, zero on entry
+    unsigned long long *fkeys;   // field boxes [ns_src][ns_fld][4] as bound slots, zero on entry
     GeomOpts o;
     int slot0;                   // first Green slot of this direction's node meshes
     CacheGeo *nodes;             // [ns_src][nb] node meshes of this direction
@@ -180,7 +189,7 @@ struct NodeArgs {
     unsigned long long *counters;
     ErrorInfo *err;
 };
-hipError_t launch_node_prepare(const NodeArgs &a, double *fpartial, int ns_src, int ns_fld, hipStream_t s);
+hipError_t launch_node_prepare(const NodeArgs &a, int ns_src, int ns_fld, hipStream_t s);
 
 struct NodeGeometryArgs {
     const BatchPair *pairs;

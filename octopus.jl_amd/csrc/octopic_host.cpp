@@ -21,7 +21,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "octopic.h"
@@ -158,6 +160,7 @@ std::vector<std::vector<int>> pair_batches(const std::vector<OrderEntry> &order,
 struct FFTPlanHost {
     FFTPlanDev dev{};
     DBuf<double2> tw;
+    DBuf<float2> tw32;
     void build(int n) {
         dev.n = n;
         int m = n, k = 0;
@@ -177,6 +180,14 @@ struct FFTPlanHost {
         tw.ensure(n);
         HIPCHECK(hipMemcpy(tw.p, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
         dev.tw = tw.p;
+        std::vector<float2> h32(n);
+        for (int t = 0; t < n; ++t) {
+            long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
+            h32[t] = make_float2((float)cosl(a), (float)sinl(a));
+        }
+        tw32.ensure(n);
+        HIPCHECK(hipMemcpy(tw32.p, h32.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+        dev.tw32 = tw32.p;
     }
 };
 
@@ -193,6 +204,7 @@ struct octopic_handle {
     std::vector<double> pos1, pos2;   // owned copies of the slicing positions
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;   // created with the handle; blocking with respect to the null stream
     std::string err;
     int nx = 0, ny = 0, lnx = 0, lny = 0, lum_dep = 0, npl = 2, mode = 0;
     FFTPlanHost px, py;
@@ -230,6 +242,10 @@ struct octopic_handle {
     DBuf<unsigned long long> charge;
     DBuf<double2> qh;
     DBuf<double> phi, table;
+    // FP32 form: half spectra, potentials, kick tables, and the float copy of every cached Green spectrum
+    bool fp32 = false;
+    DBuf<float2> qh32, spec32;
+    DBuf<float> phi32, table32;
     DBuf<octopic_interaction_record> records;
     DBuf<double> lum_parts;
     DBuf<double> lum_partial;
@@ -237,7 +253,7 @@ struct octopic_handle {
     // node-mode buffers
     DBuf<CacheGeo> nodes;
     DBuf<GreenJob> node_jobs;
-    DBuf<double> node_partial;
+    DBuf<unsigned long long> node_keys;   // node-mesh bound slots of both directions
     DBuf<int32_t> ran_dev;
     // pinned host mirrors
     SliceTable *h_tab = nullptr;
@@ -317,6 +333,8 @@ void validate_slicing(const octopic_slicing &s, int beam) {
 // PICPoissonSolver constructor checks (interface.jl:1410-1476), _validate_pic_solver (pic_cpu.jl:517-576)
 // and the device-route checks of this backend (the reference's _require_cuda_pic_options, 730-777).
 void validate(const octopic_options &o) {
+    if (o.precision != OCTOPIC_PRECISION_FP64 && o.precision != OCTOPIC_PRECISION_FP32)
+        fail(OCTOPIC_ERR_INVALID_OPTION, "precision must be OCTOPIC_PRECISION_FP64 or OCTOPIC_PRECISION_FP32");
     if (!(o.green_min_ratio > 0.0 && o.green_min_ratio <= 1.0))
         fail(OCTOPIC_ERR_INVALID_OPTION, "slice_pair_green_min_ratio must be in (0, 1]");
     if (!(o.green_growth >= 0.0)) fail(OCTOPIC_ERR_INVALID_OPTION, "slice_pair_green_growth must be non-negative");
@@ -436,7 +454,7 @@ const char *octopic_last_error(const octopic_handle *h) { return h ? h->err.c_st
 
 int octopic_set_stream(octopic_handle *h, void *s) {
     if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
-    h->stream = (hipStream_t)s;
+    h->stream = s ? (hipStream_t)s : h->own_stream;
     return OCTOPIC_OK;
 }
 
@@ -444,6 +462,7 @@ void octopic_destroy(octopic_handle *h) {
     if (!h) return;
     hipSetDevice(h->device);
     hipDeviceSynchronize();
+    if (h->own_stream) hipStreamDestroy(h->own_stream);
     for (auto e : h->ev_pool) hipEventDestroy(e);
     if (h->h_tab) hipHostFree(h->h_tab);
     if (h->h_err) hipHostFree(h->h_err);
@@ -456,8 +475,9 @@ void octopic_destroy(octopic_handle *h) {
     h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
     h->keys.release(); h->chunkbox.release(); h->box.release(); h->thist.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
     h->charge.release(); h->qh.release(); h->phi.release(); h->table.release(); h->records.release();
-    h->lum_parts.release(); h->lum_partial.release(); h->lum_arrivals.release(); h->nodes.release(); h->node_jobs.release(); h->node_partial.release(); h->ran_dev.release();
-    h->px.tw.release(); h->py.tw.release();
+    h->lum_parts.release(); h->lum_partial.release(); h->lum_arrivals.release(); h->nodes.release(); h->node_jobs.release(); h->node_keys.release(); h->ran_dev.release();
+    h->px.tw.release(); h->py.tw.release(); h->px.tw32.release(); h->py.tw32.release();
+    h->spec32.release(); h->qh32.release(); h->phi32.release(); h->table32.release();
     delete h;
 }
 
@@ -474,7 +494,10 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         h->opt.slicing2.positions = nullptr;
         h->device = device;
         HIPCHECK(hipSetDevice(device));
+        HIPCHECK(hipStreamCreate(&h->own_stream));   // default priority; orders with the null stream
+        h->stream = h->own_stream;
         h->nx = opt->nx; h->ny = opt->ny;
+        h->fp32 = opt->precision == OCTOPIC_PRECISION_FP32;
         h->lnx = opt->lum_nx > 0 ? opt->lum_nx : opt->nx;
         h->lny = opt->lum_ny > 0 ? opt->lum_ny : opt->ny;
         h->lum_dep = opt->lum_deposit >= 0 ? opt->lum_deposit : opt->deposit;
@@ -617,6 +640,36 @@ double kbb_of(double given, const octopic_beam *self, const octopic_beam *other,
     return v / (double)nsrc;
 }
 
+}  // namespace
+
+// Independent bunch-pair instances: each handle runs its collide on its own stream from its own host
+// thread, so the instances' kernels overlap on the device.
+extern "C" int octopic_collide_many(octopic_handle *const *handles, octopic_beam *beams1, octopic_beam *beams2,
+                                    const int64_t *turns, const int32_t *compute_luminosity, double *luminosity,
+                                    int32_t *status, int32_t count) {
+    if (count < 0 || (count > 0 && (!handles || !beams1 || !beams2 || !luminosity))) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    std::set<const octopic_handle *> seen;
+    for (int k = 0; k < count; ++k)
+        if (!handles[k] || !seen.insert(handles[k]).second) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    std::vector<int32_t> st(count, OCTOPIC_OK);
+    std::vector<std::thread> pool;
+    pool.reserve(count);
+    for (int k = 0; k < count; ++k)
+        pool.emplace_back([&, k]() {
+            st[k] = octopic_collide(handles[k], beams1 + k, beams2 + k, turns ? turns[k] : -1,
+                                    compute_luminosity ? compute_luminosity[k] : 1, luminosity + k);
+        });
+    for (auto &t : pool) t.join();
+    int first = OCTOPIC_OK;
+    for (int k = 0; k < count; ++k) {
+        if (status) status[k] = st[k];
+        if (first == OCTOPIC_OK) first = st[k];
+    }
+    return first;
+}
+
+namespace {
+
 void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t turn, int compute_lum, double *lum_out) {
     check_beam(B1, 1);
     check_beam(B2, 2);
@@ -753,9 +806,15 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     h->jobs.ensure_zero(2 * W, s);
     h->lumgeo.ensure(3 * W);
     h->charge.ensure_zero((size_t)nplanes_max * charge_stride, s);
-    h->qh.ensure((size_t)nfield_planes * (nx + 1) * ny);
-    h->phi.ensure((size_t)nfield_planes * field_cells);
-    h->table.ensure((size_t)2 * W * field_cells * 3 * npl);
+    if (h->fp32) {
+        h->qh32.ensure((size_t)nfield_planes * (nx + 1) * ny);
+        h->phi32.ensure((size_t)nfield_planes * field_cells);
+        h->table32.ensure((size_t)2 * W * field_cells * 3 * npl);
+    } else {
+        h->qh.ensure((size_t)nfield_planes * (nx + 1) * ny);
+        h->phi.ensure((size_t)nfield_planes * field_cells);
+        h->table.ensure((size_t)2 * W * field_cells * 3 * npl);
+    }
     h->records.ensure((size_t)2 * std::max(npairs, 1));
     h->lum_parts.ensure(std::max(npairs, 1));
     h->lum_partial.ensure((size_t)W * kLumBlocks);
@@ -765,6 +824,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     const int temp_slot0 = h->nkeys;
     const int nslots = h->nkeys + (o.green_cache == OCTOPIC_CACHE_SLICE_PAIR ? 0 : 2 * W);
     h->spec.ensure((size_t)std::max(nslots, 1) * h->slot_stride);
+    if (h->fp32) h->spec32.ensure((size_t)std::max(nslots, 1) * h->slot_stride);
     unsigned long long counters_before[6];
     HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
@@ -793,11 +853,14 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         slot_node1 = 0;
         slot_node2 = nn1;
         h->spec.ensure((size_t)(nn1 + nn2) * h->slot_stride);
+        if (h->fp32) h->spec32.ensure((size_t)(nn1 + nn2) * h->slot_stride);
         h->nodes.ensure_zero(nn1 + nn2, s);
         h->node_jobs.ensure_zero(nn1 + nn2, s);
         const int nch = (int)std::min<int64_t>(32, std::max<int64_t>(1, (maxcount + 8191) / 8192));
         const int nsmax = std::max(ns1, ns2);
-        h->node_partial.ensure((size_t)nsmax * (nsmax + 1) * nch * 4 * 2);
+        const size_t skn = (size_t)nsmax * (nsmax + 1) * 4, fkn = (size_t)nsmax * nsmax * 4;
+        h->node_keys.ensure(2 * (skn + fkn));
+        HIPCHECK(hipMemsetAsync(h->node_keys.p, 0, 2 * (skn + fkn) * sizeof(unsigned long long), s));
         for (int dir = 1; dir <= 2; ++dir) {
             NodeArgs a{};
             a.tab_src = dir == 1 ? dt1 : dt2;
@@ -807,19 +870,22 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.fld_beam = dir == 1 ? 1 : 0;
             a.dir = dir;
             a.nchunks = nch;
-            a.partial = h->node_partial.p;
+            a.skeys = h->node_keys.p + (dir - 1) * (skn + fkn);
+            a.fkeys = a.skeys + skn;
             a.o = go;
             a.slot0 = dir == 1 ? slot_node1 : slot_node2;
             a.nodes = h->nodes.p + (dir == 1 ? 0 : nn1);
             a.jobs = h->node_jobs.p + (dir == 1 ? 0 : nn1);
             a.counters = h->counters.p;
             a.err = h->errinfo.p;
-            double *fpartial = h->node_partial.p + (size_t)nsmax * (nsmax + 1) * nch * 4;
-            HIPCHECK(launch_node_prepare(a, fpartial, dir == 1 ? ns1 : ns2, dir == 1 ? ns2 : ns1, s));
+            HIPCHECK(launch_node_prepare(a, dir == 1 ? ns1 : ns2, dir == 1 ? ns2 : ns1, s));
         }
         HIPCHECK(launch_green_build(h->node_jobs.p, nn1 + nn2, h->spec.p, h->slot_stride, nn1 + nn2, nx, ny,
                                     o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev, h->errinfo.p,
                                     s));
+        if (h->fp32)
+            HIPCHECK(launch_green_to_f32(h->node_jobs.p, nn1 + nn2, h->spec.p, h->spec32.p, h->slot_stride, nn1 + nn2, nx,
+                                         ny, o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->errinfo.p, s));
         h->ran_dev.ensure(std::max(npairs, 1));
         HIPCHECK(hipMemsetAsync(h->ran_dev.p, 0, sizeof(int32_t) * npairs, s));
     }
@@ -866,6 +932,9 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             HIPCHECK(launch_green_build(h->jobs.p, 2 * B, h->spec.p, h->slot_stride, nslots, nx, ny,
                                         o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev,
                                         h->errinfo.p, s));
+            if (h->fp32)
+                HIPCHECK(launch_green_to_f32(h->jobs.p, 2 * B, h->spec.p, h->spec32.p, h->slot_stride, nslots, nx, ny,
+                                             o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->errinfo.p, s));
         }
         const int nfp = 2 * B * npl;
         {
@@ -876,38 +945,47 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.chunks = nchunks; a.group = std::max(1, (int)std::min<int64_t>(nchunks, 8192 * (int64_t)nchunks / std::max<int64_t>(bmax, 1)));
             a.scale = scale; a.charge = h->charge.p; a.charge_stride = charge_stride;
             a.dropped = h->counters.p + 4; a.chunkbox = cbox; a.err = h->errinfo.p;
+            a.fp32 = h->fp32 ? 1 : 0;
             HIPCHECK(launch_deposit(a, s));
             if (compute_lum) {
                 DepositArgs l = a;
+                l.fp32 = 0;   // luminosity deposits stay FP64
                 l.planes = h->planes.p + nfp; l.nplanes = 2 * B; l.method = h->lum_dep; l.ncells = (int)lum_cells;
                 l.charge = h->charge.p + (size_t)nfp * charge_stride;
                 HIPCHECK(launch_deposit(l, s));
             }
         }
         const int64_t qh_stride = (int64_t)(nx + 1) * ny;
-        {
-            Timed t(h, F_ROWS_FWD);
-            HIPCHECK(launch_rows_fwd(h->planes.p, nfp, h->charge.p, charge_stride, h->qh.p, qh_stride, nx, ny, inv_scale,
-                                     h->px.dev, h->errinfo.p, s));
-        }
-        {
-            Timed t(h, F_COLS);
-            HIPCHECK(launch_cols_conv(h->planes.p, nfp, h->qh.p, qh_stride, h->spec.p, h->slot_stride, nx, ny, h->py.dev,
-                                      h->errinfo.p, s));
-        }
-        {
-            Timed t(h, F_ROWS_INV);
-            HIPCHECK(launch_rows_inv(h->planes.p, nfp, h->qh.p, qh_stride, h->phi.p, nx, ny, h->px.dev, h->errinfo.p, s));
-        }
-        {
-            Timed t(h, F_FIELD);
-            HIPCHECK(launch_field(h->inters.p, 2 * B, h->planes.p, h->phi.p, h->table.p, nx, ny,
-                                  o.field_derivative == 4 ? 1 : 0, npl, h->errinfo.p, s));
-        }
+        auto solve = [&](auto *qh, const auto *spec, auto *phi, auto *table) {
+            {
+                Timed t(h, F_ROWS_FWD);
+                HIPCHECK(launch_rows_fwd(h->planes.p, nfp, h->charge.p, charge_stride, qh, qh_stride, nx, ny, inv_scale,
+                                         h->px.dev, h->errinfo.p, s));
+            }
+            {
+                Timed t(h, F_COLS);
+                HIPCHECK(launch_cols_conv(h->planes.p, nfp, qh, qh_stride, spec, h->slot_stride, nx, ny, h->py.dev,
+                                          h->errinfo.p, s));
+            }
+            {
+                Timed t(h, F_ROWS_INV);
+                HIPCHECK(launch_rows_inv(h->planes.p, nfp, (const decltype(qh))qh, qh_stride, phi, nx, ny, h->px.dev,
+                                         h->errinfo.p, s));
+            }
+            {
+                Timed t(h, F_FIELD);
+                HIPCHECK(launch_field(h->inters.p, 2 * B, h->planes.p, (const decltype(phi))phi, table, nx, ny,
+                                      o.field_derivative == 4 ? 1 : 0, npl, h->errinfo.p, s));
+            }
+        };
+        if (h->fp32) solve(h->qh32.p, h->spec32.p, h->phi32.p, h->table32.p);
+        else solve(h->qh.p, h->spec.p, h->phi.p, h->table.p);
         {
             Timed t(h, F_KICK);
             KickArgs a{};
-            a.inters = h->inters.p; a.ninters = 2 * B; a.st = st; a.table = h->table.p; a.nx = nx; a.ny = ny; a.npl = npl;
+            a.inters = h->inters.p; a.ninters = 2 * B; a.st = st; a.nx = nx; a.ny = ny; a.npl = npl;
+            a.table = h->fp32 ? (const void *)h->table32.p : (const void *)h->table.p;
+            a.fp32 = h->fp32 ? 1 : 0;
             a.method = o.deposit; a.mode = h->mode; a.lkick = o.longitudinal_kick ? 1 : 0;
             a.chunks = nchunks;
             a.dropped = h->counters.p + 4;

@@ -58,6 +58,9 @@ class Slicing(C.Structure):
     ]
 
 
+PRECISION = {"fp64": 0, "fp32": 1}
+
+
 class Options(C.Structure):
     _fields_ = [
         ("nx", C.c_int32),
@@ -75,7 +78,7 @@ class Options(C.Structure):
         ("lum_deposit", C.c_int32),
         ("batch_mode", C.c_int32),
         ("cuda_flags", C.c_int32),
-        ("reserved", C.c_int32),
+        ("precision", C.c_int32),
         ("grid_extent_sigma", C.c_double),
         ("min_extent_x", C.c_double),
         ("min_extent_y", C.c_double),
@@ -165,6 +168,8 @@ def declare(lib):
             [vp, C.c_uint64, C.c_uint64, DPTR, DPTR, DPTR, C.c_double, C.POINTER(BeamStruct)],
         ),
         "octopic_debug_weights": (i32, [vp, i32, DPTR, i64, i32, C.POINTER(i32), DPTR]),
+        "octopic_collide_many": (i32, [C.POINTER(vp), C.POINTER(BeamStruct), C.POINTER(BeamStruct), C.POINTER(i64),
+                                       C.POINTER(i32), DPTR, C.POINTER(i32), i32]),
         "octopic_set_timing": (i32, [vp, i32]),
         "octopic_set_timing_families": (i32, [vp, C.c_char_p]),
         "octopic_kernel_times": (
@@ -193,6 +198,7 @@ EXPORTED = [
     "octopic_track_linear6d",
     "octopic_generate_gaussian_beam",
     "octopic_debug_weights",
+    "octopic_collide_many",
     "octopic_set_timing",
     "octopic_set_timing_families",
     "octopic_kernel_times",

@@ -159,6 +159,7 @@ class PICCollider:
             _raise(rc, lib().octopic_last_error(self.h).decode())
 
     def set_stream(self, stream_ptr):
+        """Enqueue on the given HIP stream (None: the handle's own stream)."""
         self._check(lib().octopic_set_stream(self.h, C.c_void_p(stream_ptr)))
 
     def collide(self, beam1: Beam, beam2: Beam, turn: Optional[int] = None) -> float:
@@ -260,6 +261,37 @@ def generate_beam(collider: PICCollider, n: int, params: BeamParams, beta, alpha
                                               float(cutoff), C.byref(s))
     collider._check(rc)
     return beam
+
+
+def collide_many(colliders: Sequence["PICCollider"], beams1: Sequence[Beam], beams2: Sequence[Beam],
+                 turns: Optional[Sequence[Optional[int]]] = None):
+    """Independent bunch-pair instances collided together: instance k is colliders[k].collide(beams1[k],
+    beams2[k], turns[k]), each on its handle's own stream from its own host thread, so that the
+    instances' kernels overlap on the GPU.  Returns the luminosities; raises the first instance's error."""
+    torch = _torch()
+    n = len(colliders)
+    if not (len(beams1) == len(beams2) == n) or len(set(id(c) for c in colliders)) != n:
+        raise ArgumentError("collide_many needs one distinct collider and two beams per instance")
+    turns = list(turns) if turns is not None else [None] * n
+    s1 = (_abi.BeamStruct * n)(*[b.struct() for b in beams1])
+    s2 = (_abi.BeamStruct * n)(*[b.struct() for b in beams2])
+    hs = (C.c_void_p * n)(*[c.h for c in colliders])
+    tv = (C.c_int64 * n)(*[-1 if t is None else int(t) for t in turns])
+    cl = (C.c_int32 * n)(*[1 if c.solver.compute_luminosity(t) else 0 for c, t in zip(colliders, turns)])
+    lums = (C.c_double * n)()
+    st = (C.c_int32 * n)()
+    torch.cuda.synchronize()      # the handles' streams run beside torch's
+    for c in colliders:
+        c.set_stream(None)
+    try:
+        lib().octopic_collide_many(hs, s1, s2, tv, cl, lums, st, n)
+    finally:
+        for c in colliders:
+            c.set_stream(torch.cuda.current_stream(c.device).cuda_stream)
+    for k in range(n):
+        if st[k] != 0:
+            colliders[k]._check(st[k])
+    return list(lums)
 
 
 def collide_(solver: PICPoissonSolver, beam1: Beam, beam2: Beam, backend=HIPBackend, ctx=None) -> float:

@@ -140,6 +140,10 @@ class PICPoissonSolver:
     slicing: LongitudinalSlicing = LongitudinalSlicing()
     slicing1: Optional[LongitudinalSlicing] = None
     slicing2: Optional[LongitudinalSlicing] = None
+    # "fp64": the reference's PICPoissonSolver{Float64} arithmetic.  "fp32": this backend's FP32 form
+    # (deposit weights, FFT convolution, field and kick gather in FP32 on FP64 beams; Green function,
+    # slice moments and luminosity in FP64); no counterpart in the reference.
+    precision: str = "fp64"
 
     def __post_init__(self):
         # Same order of checks as the reference constructor (interface.jl:1410-1476).
@@ -184,6 +188,8 @@ class PICPoissonSolver:
             if not all(v >= 3 for v in lg):
                 raise ArgumentError(f"luminosity_grid dimensions must both be at least 3; got {self.luminosity_grid}.")
             object.__setattr__(self, "luminosity_grid", lg)
+        if self.precision not in _abi.PRECISION:
+            raise ArgumentError(f"precision must be fp64 or fp32; got {self.precision!r}.")
         if self.luminosity_deposit_method not in (None, "CIC", "TSC"):
             raise ArgumentError(
                 f"luminosity_deposit_method must be nothing, :CIC, or :TSC; got :{self.luminosity_deposit_method}.")
@@ -230,6 +236,7 @@ class PICPoissonSolver:
         o.batch_mode = _abi.BATCH_MODE[self.batch_mode]
         o.cuda_flags = (int(self.cuda_async) | (int(self.cuda_batch_fft) << 1) |
                         (int(self.cuda_wavefront_fft) << 2) | (int(self.cuda_indexed_wavefront) << 3))
+        o.precision = _abi.PRECISION[self.precision]
         o.grid_extent_sigma = float(self.grid_extent_sigma)
         o.min_extent_x, o.min_extent_y = self.min_transverse_extent
         o.grid_quantize = float(self.grid_quantize)

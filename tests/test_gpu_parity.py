@@ -385,3 +385,137 @@ def test_device_beam_generator_matches_oracle(oracle, gpu):
         for k in range(6):
             scale = np.max(np.abs(o[k]))
             assert np.max(np.abs(g[k] - o[k])) <= 1e-13 * scale, (n, k)
+
+
+@pytest.mark.parametrize("kw", [dict(deposit_method="TSC", field_derivative="fourth", slice_interpolation="quadratic"),
+                                dict(interaction_grid="node", deposit_method="TSC", field_derivative="fourth"),
+                                dict()], ids=["quad", "node", "defaults"])
+def test_fp32_form_against_fp64_oracle(oracle, gpu, kw):
+    """precision = fp32 (deposit weights, FFT convolution, field table and kick gather in FP32 on FP64
+    beams; Green function, moments and luminosity in FP64): the per-particle transverse kick (change of
+    px, py over one collide) stays within 1e-4 of the largest FP64 kick and the longitudinal one (pz,
+    from potential differences) within 3e-3, the luminosity within 1e-5, the slicing (FP64) is
+    bit-identical, and the result is bitwise reproducible."""
+    e, p = flat_beams(oracle, 40000, 16000, seed=11)
+    sl = LongitudinalSlicing(nslices=7, method="normal_quantile", center_position="centroid")
+    s64 = PICPoissonSolver(grid=(128, 128), slicing=sl, **kw)
+    s32 = PICPoissonSolver(grid=(128, 128), slicing=sl, precision="fp32", **kw)
+    ce, cp = e.copy(), p.copy()
+    lo = oracle.Collider(s64).collide(ce, PE, cp, PP)
+    outs = []
+    for _ in range(2):
+        g = PICCollider(s32)
+        be, bp = Beam.from_host(e, PE, device=gpu), Beam.from_host(p, PP, device=gpu)
+        lg = g.collide(be, bp)
+        outs.append((lg, be.to_host(), bp.to_host(), g.last_slices(1, e.shape[1])))
+    assert outs[0][0] == outs[1][0]
+    assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][2], outs[1][2])
+    lg, ge, gp, gsl = outs[0]
+    assert math.isclose(lg, lo, rel_tol=1e-5)
+    osl = oracle.Collider(s64)
+    osl.collide(e.copy(), PE, p.copy(), PP)
+    for a, b in zip(osl.last_slices(1, e.shape[1]), gsl):
+        assert np.array_equal(a, b)
+    for init, ref, got in ((e, ce, ge), (p, cp, gp)):
+        for k, tol in ((1, 1e-4), (3, 1e-4), (5, 3e-3)):
+            dref, dgot = ref[k] - init[k], got[k] - init[k]
+            assert np.max(np.abs(dgot - dref)) <= tol * np.max(np.abs(dref)), k
+        for k in (0, 2):
+            assert np.max(np.abs(got[k] - ref[k])) <= 1e-4 * np.max(np.abs(ref[k] - init[k]) + 1e-30) + 1e-12 * np.max(np.abs(ref[k]))
+
+
+def _be_kick(sx, sy, x, y):
+    from test_oracle_solver import be_kick
+    return be_kick(sx, sy, x, y)
+
+
+def _gpu_field_error(oracle, gpu, sx, sy, precision, npts=160, grid=128):
+    """Median |kick - exact Bassetti-Erskine kick| / max |BE kick| on a 41 x 41 probe grid within
+    +-2 sigma, for a deterministic quantile Gaussian source (npts^2 particles, one slice, no drifts),
+    through the whole device collide: deposit, convolution, 4th-order field, TSC gather, kick."""
+    u = (np.arange(1, npts + 1) - 0.5) / npts
+    q = np.array([oracle.erfinv(2 * v - 1) for v in u]) * math.sqrt(2.0)
+    X, Y = np.meshgrid(sx * q, sy * q, indexing="ij")
+    src = np.zeros((6, X.size))
+    src[0], src[2] = X.ravel(), Y.ravel()
+    xg, yg = np.linspace(-2 * sx, 2 * sx, 41), np.linspace(-2 * sy, 2 * sy, 41)
+    PX, PY = np.meshgrid(xg, yg, indexing="ij")
+    probes = np.zeros((6, PX.size))
+    probes[0], probes[2] = PX.ravel(), PY.ravel()
+    sl = LongitudinalSlicing(nslices=1, method="normal_quantile", center_position="centroid")
+    solver = PICPoissonSolver(grid=(grid, grid), deposit_method="TSC", field_derivative="fourth", kbb1=1.0, kbb2=1.0,
+                              longitudinal_kick=False, slicing=sl, precision=precision)
+    col = PICCollider(solver)
+    b1, b2 = Beam.from_host(src, PE, device=gpu), Beam.from_host(probes, PP, device=gpu)
+    col.collide(b1, b2)
+    out = b2.to_host()
+    kx, ky = _be_kick(sx, sy, PX.ravel(), PY.ravel())
+    err = np.hypot(out[1] - kx, out[3] - ky)
+    return float(np.median(err) / np.max(np.hypot(kx, ky))), out
+
+
+@pytest.mark.parametrize("aspect", [1.0, 4.0, 11.0, 25.0])
+def test_field_vs_bassetti_erskine_both_precisions(oracle, gpu, aspect):
+    """The FP64 and FP32 device fields against the exact Bassetti-Erskine field at sigma_x/sigma_y =
+    1, 4, 11, 25 (sigma_y = 8.5 um, 128^2, TSC, 4th order): the FP64 error is the discretisation error
+    (the reference quotes medians of 4.1e-4 round to 1.2e-3 at 25:1 at 128^2), and the FP32 form adds
+    at most 1e-5 of the largest kick to it."""
+    sy = 8.5e-6
+    e64, k64 = _gpu_field_error(oracle, gpu, aspect * sy, sy, "fp64")
+    e32, k32 = _gpu_field_error(oracle, gpu, aspect * sy, sy, "fp32")
+    assert e64 < 3e-3
+    assert abs(e32 - e64) <= 1e-5
+    scale = np.max(np.hypot(k64[1], k64[3]))
+    assert np.max(np.hypot(k32[1] - k64[1], k32[3] - k64[3])) <= 1e-4 * scale
+
+
+def test_reference_cpu_configuration_ten_turns(oracle, gpu):
+    """The reference's CPU-path configuration: round beams (sigma 100 um, sigma_p 1e-4, sigma_z 1 cm /
+    6 cm, sigma_dp 5e-4 / 6.6e-4) of 100k e- and 40k p from seed 0, 1.7e11 / 6.9e10, 10 / 275 GeV;
+    5 slices, 64^2, CIC, integrated Green, 2nd order, linear interpolation, slice-pair cache; 10 turns
+    through the linear 6D maps with tunes (0.08, 0.06, 0.01) and (0.228, 0.21, 0.01).  Every turn the
+    oracle starts from the GPU's turn-start state: slicing bit-identical, coordinates and luminosity
+    within 1e-10, cache history identical."""
+    e = oracle.generate_beam(100_000, 0, 1, (1.0, 1.0, 20.0), (0, 0, 0), (100e-6, 100e-6, 0.01))
+    p = oracle.generate_beam(40_000, 0, 2, (1.0, 1.0, 90.909), (0, 0, 0), (100e-6, 100e-6, 0.06))
+    sl = LongitudinalSlicing(nslices=5, method="normal_quantile", center_position="centroid")
+    solver = PICPoissonSolver(grid=(64, 64), deposit_method="CIC", green_type="integrated",
+                              field_derivative="second", slice_interpolation="linear", slicing=sl)
+    me = linear6d_matrix((1.0, 1.0, 20.0), [2 * math.pi * t for t in (0.08, 0.06, 0.01)])
+    mp = linear6d_matrix((1.0, 1.0, 90.909), [2 * math.pi * t for t in (0.228, 0.21, 0.01)])
+    gcol = compare_collide(oracle, gpu, solver, e, p, collides=10, turn_map=(me, mp))
+    st = gcol.stats()
+    assert st["green_misses"] == 50 and st["green_hits"] + st["green_rebuilds"] == 9 * 50
+
+
+def test_collide_many_equals_one_by_one(oracle, gpu):
+    """Independent instances of uneven shape (slices 7/11/15, meshes 64/96/128, FP64 and FP32) collided
+    together through octopic_collide_many give bit-identical results to the same instances collided one
+    by one, over two turns."""
+    from octopic import collide_many
+    shapes = [(30000, 12000, 7, 64, "fp64"), (50000, 9000, 11, 96, "fp32"), (20000, 16000, 15, 128, "fp64"),
+              (41000, 7000, 7, 96, "fp32")]
+    insts = []
+    for k, (n1, n2, ns, g, prec) in enumerate(shapes):
+        e, p = flat_beams(oracle, n1, n2, seed=20 + k)
+        sl = LongitudinalSlicing(nslices=ns, method="normal_quantile", center_position="centroid")
+        solver = PICPoissonSolver(grid=(g, g), slicing=sl, deposit_method="TSC", field_derivative="fourth",
+                                  slice_interpolation="quadratic", precision=prec)
+        insts.append((solver, e, p))
+    results = []
+    for together in (False, True):
+        cols = [PICCollider(s) for s, _, _ in insts]
+        b1 = [Beam.from_host(e, PE, device=gpu) for _, e, _ in insts]
+        b2 = [Beam.from_host(p, PP, device=gpu) for _, _, p in insts]
+        lums = []
+        for turn in range(2):
+            if together:
+                lums.append(collide_many(cols, b1, b2, [turn] * len(cols)))
+            else:
+                lums.append([c.collide(x, y, turn=turn) for c, x, y in zip(cols, b1, b2)])
+        torch.cuda.synchronize()
+        results.append((lums, [b.to_host() for b in b1], [b.to_host() for b in b2], [c.stats() for c in cols]))
+    a, b = results
+    assert a[0] == b[0] and a[3] == b[3]
+    for x, y in zip(a[1] + a[2], b[1] + b[2]):
+        assert np.array_equal(x, y)
