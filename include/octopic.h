@@ -186,6 +186,17 @@ int octopic_interaction_records(const octopic_handle *h, octopic_interaction_rec
  * row-major 6x6 matrix.  Runs on the handle's stream; beam pointers must be device pointers. */
 int octopic_track_linear6d(octopic_handle *h, const double *matrix36, octopic_beam *beam);
 
+/* track_particle(::NonSymplectic6DMap, ::LorentzBoost | ::RevLorentzBoost, ...) (lorentz_boost.jl:128-166):
+ * Hirata's crossing-angle transformation, `angle` = half the full crossing angle, `reverse` = 1 for
+ * the reverse map; device beam, in place on the handle's stream. */
+int octopic_track_lorentz_boost(octopic_handle *h, double angle, int32_t reverse, octopic_beam *beam);
+
+/* track_particle(::Symplectic6DMap, ::ThinCrabCavity{N}, ...) (crab_cavity.jl:99-164): frequency in Hz
+ * (finite, nonzero), nharmonics <= 16 harmonics of (strength_x, strength_y, phase) host arrays; device
+ * beam, in place on the handle's stream. */
+int octopic_track_crab_cavity(octopic_handle *h, double frequency, int32_t nharmonics, const double *strength_x,
+                              const double *strength_y, const double *phase, octopic_beam *beam);
+
 /* Beam(N, ...) synthetic Gaussian construction on the device (Beam.jl:127-181, 545-553): Philox4x32-10
  * counter normals, clamp to +-cutoff, standardise, scale by sigma/beta with alpha.  beta, alpha,
  * sigma are (x, y, z) triples.  Writes the six device arrays of `beam` (n = beam->n). */

@@ -33,6 +33,83 @@ hipError_t launch_linear6d(const double *m36, double *const c[6], int64_t n, hip
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ crossing-angle boost, crab cavity
+// LorentzBoost / RevLorentzBoost (lorentz_boost.jl:128-166), Hirata's map with angle = half the full
+// crossing angle; cos, sin, tan = sin/cos computed on the host once (lorentz_boost.jl:168-174).
+template <bool REV>
+__global__ __launch_bounds__(256) void k_lorentz_boost(double c, double s, double t, double *X, double *PX, double *Y,
+                                                       double *PY, double *Z, double *PZ, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double x0 = X[i], px0 = PX[i], y0 = Y[i], py0 = PY[i], z0 = Z[i], pz0 = PZ[i];
+        const double ps0 = sqrt((1.0 + pz0) * (1.0 + pz0) - px0 * px0 - py0 * py0);
+        const double h0 = 1.0 + pz0 - ps0;
+        double x1, px1, y1, py1, z1, pz1;
+        if (!REV) {
+            py1 = py0 / c;
+            const double h1 = h0 / (c * c);
+            px1 = px0 / c - h1 * s;
+            pz1 = pz0 - px1 * s;
+            const double ps1 = 1.0 + pz1 - h1;
+            const double ds = x0 * s;
+            x1 = x0 + z0 * t + px1 / ps1 * ds;
+            y1 = y0 + py1 / ps1 * ds;
+            z1 = z0 / c - h1 / ps1 * ds;
+        } else {
+            x1 = x0 - z0 * s;
+            x1 = x1 / (1.0 + (px0 + h0 * s) * s / ps0);
+            z1 = (z0 + h0 / ps0 * x1 * s) * c;
+            y1 = y0 - py0 / ps0 * x1 * s;
+            pz1 = pz0 + px0 * s;
+            px1 = (px0 + h0 * s) * c;
+            py1 = py0 * c;
+        }
+        X[i] = x1; PX[i] = px1; Y[i] = y1; PY[i] = py1; Z[i] = z1; PZ[i] = pz1;
+    }
+}
+
+hipError_t launch_lorentz_boost(double angle, int reverse, double *const c[6], int64_t n, hipStream_t s) {
+    const double co = cos(angle), si = sin(angle), ta = si / co;
+    int g = (int)((n + 255) / 256);
+    if (g > 16384) g = 16384;
+    if (reverse) hipLaunchKernelGGL(k_lorentz_boost<true>, dim3(g), dim3(256), 0, s, co, si, ta, c[0], c[1], c[2], c[3], c[4], c[5], n);
+    else hipLaunchKernelGGL(k_lorentz_boost<false>, dim3(g), dim3(256), 0, s, co, si, ta, c[0], c[1], c[2], c[3], c[4], c[5], n);
+    return hipGetLastError();
+}
+
+// ThinCrabCavity{N} (crab_cavity.jl:99-164): per harmonic i, theta = i kcc z + phase_i,
+// px -= sx_i sin/(i kcc), py -= sy_i sin/(i kcc), pz -= (sx_i x + sy_i y) cos; x, y, z unchanged.
+struct CrabHarmonics { double sx[kMaxCrabHarmonics], sy[kMaxCrabHarmonics], ph[kMaxCrabHarmonics]; };
+
+__global__ __launch_bounds__(256) void k_crab_cavity(double kcc, int nh, CrabHarmonics hm, const double *X, double *PX,
+                                                     const double *Y, double *PY, const double *Z, double *PZ, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double x0 = X[i], y0 = Y[i], z0 = Z[i];
+        double dpx = 0.0, dpy = 0.0, dpz = 0.0;
+        for (int k = 1; k <= nh; ++k) {
+            const double ikcc = (double)k * kcc;
+            const double th = ikcc * z0 + hm.ph[k - 1];
+            const double sn = sin(th), cs = cos(th);
+            dpx -= hm.sx[k - 1] * sn / ikcc;
+            dpy -= hm.sy[k - 1] * sn / ikcc;
+            dpz -= (hm.sx[k - 1] * x0 + hm.sy[k - 1] * y0) * cs;
+        }
+        PX[i] = PX[i] + dpx;
+        PY[i] = PY[i] + dpy;
+        PZ[i] = PZ[i] + dpz;
+    }
+}
+
+hipError_t launch_crab_cavity(double kcc, int nh, const double *sx, const double *sy, const double *phase, double *const c[6],
+                              int64_t n, hipStream_t s) {
+    if (nh < 0 || nh > kMaxCrabHarmonics) return hipErrorInvalidValue;
+    CrabHarmonics hm{};
+    for (int k = 0; k < nh; ++k) { hm.sx[k] = sx[k]; hm.sy[k] = sy[k]; hm.ph[k] = phase[k]; }
+    int g = (int)((n + 255) / 256);
+    if (g > 16384) g = 16384;
+    hipLaunchKernelGGL(k_crab_cavity, dim3(g), dim3(256), 0, s, kcc, nh, hm, c[0], c[1], c[2], c[3], c[4], c[5], n);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ Philox4x32-10 counter RNG
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     uint64_t z = x + 0x9e3779b97f4a7c15ull;

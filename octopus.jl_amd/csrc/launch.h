@@ -210,6 +210,10 @@ hipError_t launch_node_geometry(const NodeGeometryArgs &a, hipStream_t s);
 
 // misc (k_misc.hip)
 hipError_t launch_linear6d(const double *m36, double *const c[6], int64_t n, hipStream_t s);
+hipError_t launch_lorentz_boost(double angle, int reverse, double *const c[6], int64_t n, hipStream_t s);
+constexpr int kMaxCrabHarmonics = 16;
+hipError_t launch_crab_cavity(double kcc, int nh, const double *sx, const double *sy, const double *phase, double *const c[6],
+                              int64_t n, hipStream_t s);
 hipError_t launch_generate(double *const c[6], int64_t n, uint64_t seed, uint64_t rng_id, double cutoff, hipStream_t s);
 hipError_t launch_scale_beam(double *const c[6], int64_t n, const double *beta, const double *alpha, const double *sigma,
                              hipStream_t s);

@@ -1128,6 +1128,34 @@ int octopic_track_linear6d(octopic_handle *h, const double *m, octopic_beam *b) 
     return OCTOPIC_OK;
 }
 
+int octopic_track_lorentz_boost(octopic_handle *h, double angle, int32_t reverse, octopic_beam *b) {
+    if (!h || !b) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    if (!b->on_device) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "octopic_track_lorentz_boost needs device beam pointers");
+    if (!std::isfinite(angle)) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "Lorentz boost angle must be finite");
+    double *c[6] = {b->x, b->px, b->y, b->py, b->z, b->pz};
+    hipSetDevice(h->device);
+    if (launch_lorentz_boost(angle, reverse ? 1 : 0, c, b->n, h->stream) != hipSuccess)
+        return set_err(h, OCTOPIC_ERR_DEVICE, "Lorentz boost launch failed");
+    return OCTOPIC_OK;
+}
+
+int octopic_track_crab_cavity(octopic_handle *h, double frequency, int32_t nharmonics, const double *strength_x,
+                              const double *strength_y, const double *phase, octopic_beam *b) {
+    if (!h || !b || (nharmonics > 0 && (!strength_x || !strength_y || !phase))) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    if (!b->on_device) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "octopic_track_crab_cavity needs device beam pointers");
+    // _validate_crab_frequency (crab_cavity.jl:113-117)
+    if (!(std::isfinite(frequency) && frequency != 0.0))
+        return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "thin crab-cavity frequency must be finite and nonzero");
+    if (nharmonics < 0 || nharmonics > kMaxCrabHarmonics)
+        return set_err(h, OCTOPIC_ERR_UNSUPPORTED, "thin crab cavities with more than 16 harmonics are not supported");
+    const double kcc = 2.0 * 3.141592653589793 * frequency / 299792458.0;   // 2 pi f / CLIGHT (Constants.jl:8)
+    double *c[6] = {b->x, b->px, b->y, b->py, b->z, b->pz};
+    hipSetDevice(h->device);
+    if (launch_crab_cavity(kcc, nharmonics, strength_x, strength_y, phase, c, b->n, h->stream) != hipSuccess)
+        return set_err(h, OCTOPIC_ERR_DEVICE, "crab cavity launch failed");
+    return OCTOPIC_OK;
+}
+
 int octopic_generate_gaussian_beam(octopic_handle *h, uint64_t seed, uint64_t rng_id, const double *beta3,
                                    const double *alpha3, const double *sigma3, double cutoff, octopic_beam *b) {
     if (!h || !b || !beta3 || !alpha3 || !sigma3) return OCTOPIC_ERR_INVALID_ARGUMENT;

@@ -163,6 +163,8 @@ def declare(lib):
         "octopic_last_slices": (i32, [vp, i32, DPTR, DPTR, DPTR, C.POINTER(i64), C.POINTER(i64), i64]),
         "octopic_interaction_records": (i32, [vp, C.POINTER(InteractionRecord), i64, C.POINTER(i64)]),
         "octopic_track_linear6d": (i32, [vp, DPTR, C.POINTER(BeamStruct)]),
+        "octopic_track_lorentz_boost": (i32, [vp, C.c_double, i32, C.POINTER(BeamStruct)]),
+        "octopic_track_crab_cavity": (i32, [vp, C.c_double, i32, DPTR, DPTR, DPTR, C.POINTER(BeamStruct)]),
         "octopic_generate_gaussian_beam": (
             i32,
             [vp, C.c_uint64, C.c_uint64, DPTR, DPTR, DPTR, C.c_double, C.POINTER(BeamStruct)],
@@ -196,6 +198,8 @@ EXPORTED = [
     "octopic_last_slices",
     "octopic_interaction_records",
     "octopic_track_linear6d",
+    "octopic_track_lorentz_boost",
+    "octopic_track_crab_cavity",
     "octopic_generate_gaussian_beam",
     "octopic_debug_weights",
     "octopic_collide_many",

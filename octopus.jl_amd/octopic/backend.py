@@ -219,6 +219,26 @@ class PICCollider:
         s = beam.struct()
         self._check(lib().octopic_track_linear6d(self.h, m, C.byref(s)))
 
+    def track_lorentz_boost(self, beam: Beam, angle: float, reverse: bool = False):
+        """LorentzBoost(angle) / RevLorentzBoost(angle) (lorentz_boost.jl:128-166) in place on the device;
+        angle = half the full crossing angle."""
+        s = beam.struct()
+        self._check(lib().octopic_track_lorentz_boost(self.h, float(angle), 1 if reverse else 0, C.byref(s)))
+
+    def track_crab_cavity(self, beam: Beam, frequency: float, strength_x: Sequence[float] = (),
+                          strength_y: Sequence[float] = (), phase: Sequence[float] = ()):
+        """ThinCrabCavity{N}(frequency; strengthX, strengthY, phase) (crab_cavity.jl:99-164) in place on the
+        device; empty tuples are zeros as _harmonic_tuple (crab_cavity.jl:166-172)."""
+        nh = max(len(strength_x), len(strength_y), len(phase))
+        vals = []
+        for v in (strength_x, strength_y, phase):
+            if len(v) not in (0, nh):
+                raise ArgumentError(f"expected {nh} harmonic values, got {len(v)}")
+            vals.append([float(u) for u in v] if len(v) else [0.0] * nh)
+        arr = [(C.c_double * max(nh, 1))(*v) for v in vals]
+        s = beam.struct()
+        self._check(lib().octopic_track_crab_cavity(self.h, float(frequency), nh, arr[0], arr[1], arr[2], C.byref(s)))
+
     def set_timing(self, enabled: bool, families: Optional[str] = None):
         """Per-family HIP-event timing: every family, or only the comma-separated `families`."""
         if enabled and families:

@@ -8,6 +8,7 @@ Arrays follow the reference's layout: grids are column-major (x fastest), index 
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import sys
 
@@ -84,6 +85,8 @@ def lib():
         L.oracle_collision_order.argtypes = [D, C.c_int32, D, C.c_int32, I32, I32]
         L.oracle_linear6d_matrix.argtypes = [D, D, D, D, D, D]
         L.oracle_track_linear6d.argtypes = [D, C.POINTER(D), C.c_int64]
+        L.oracle_track_lorentz_boost.argtypes = [C.c_double, C.c_int, C.POINTER(D), C.c_int64]
+        L.oracle_track_crab_cavity.argtypes = [C.c_double, C.c_int, D, D, D, C.POINTER(D), C.c_int64]
         L.oracle_philox_block.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         _lib = L
     return _lib
@@ -293,6 +296,25 @@ def linear6d_matrix(beta1, beta2, alpha1, alpha2, dmu):
     out = np.zeros(36)
     lib().oracle_linear6d_matrix(*[_d(_f64(v)) for v in (beta1, beta2, alpha1, alpha2, dmu)], _d(out))
     return out
+
+
+CLIGHT = 299_792_458.0   # src/constants/Constants.jl:8
+
+
+def track_lorentz_boost(angle, coords, reverse=False):
+    """LorentzBoost / RevLorentzBoost (lorentz_boost.jl:128-166) of coords [6, n] in place; angle =
+    half the full crossing angle."""
+    ptrs = (C.POINTER(C.c_double) * 6)(*[_d(coords[k]) for k in range(6)])
+    lib().oracle_track_lorentz_boost(C.c_double(angle), 1 if reverse else 0, ptrs, coords.shape[1])
+
+
+def track_crab_cavity(frequency, strength_x, strength_y, phase, coords):
+    """ThinCrabCavity{N} (crab_cavity.jl:99-164) of coords [6, n] in place."""
+    kcc = 2.0 * math.pi * float(frequency) / CLIGHT
+    nh = len(phase)
+    ptrs = (C.POINTER(C.c_double) * 6)(*[_d(coords[k]) for k in range(6)])
+    lib().oracle_track_crab_cavity(C.c_double(kcc), nh, _d(_f64(strength_x)), _d(_f64(strength_y)), _d(_f64(phase)),
+                                   ptrs, coords.shape[1])
 
 
 def track_linear6d(m, coords):

@@ -11,6 +11,8 @@
 //   src/math/counter_rng.jl              Philox4x32-10 counter RNG + Box-Muller
 //   src/beam/Beam.jl                     synthetic Gaussian beams
 //   src/elements/linear6d.jl             the linear one-turn map
+//   src/elements/lorentz_boost.jl        crossing-angle boost and its reverse
+//   src/elements/crab_cavity.jl          thin crab cavity
 // Julia evaluation order is kept expression by expression (a*b*c == (a*b)*c, a-b+c == (a-b)+c),
 // Julia's NaN-propagating min/max/clamp are restated, and the file is compiled with
 // -ffp-contract=off so that no product is fused into an FMA the reference does not perform.
@@ -1853,6 +1855,63 @@ int oracle_track_linear6d(const double m[36], double *b[6], int64_t n) {
             const double *q = m + 6 * r;
             b[r][i] = q[0] * v[0] + q[1] * v[1] + q[2] * v[2] + q[3] * v[3] + q[4] * v[4] + q[5] * v[5];
         }
+    }
+    return 0;
+}
+
+// track_particle(::NonSymplectic6DMap, ::LorentzBoost / ::RevLorentzBoost, ...) (lorentz_boost.jl:128-166):
+// Hirata's crossing-angle transformation with `angle` = half the full crossing angle; cos, sin and
+// tan = sin/cos as _lorentz_boost (lorentz_boost.jl:168-174).
+int oracle_track_lorentz_boost(double angle, int reverse, double *b[6], int64_t n) {
+    const double c = std::cos(angle), s = std::sin(angle), t = s / c;
+    for (int64_t i = 0; i < n; ++i) {
+        const double x0 = b[0][i], px0 = b[1][i], y0 = b[2][i], py0 = b[3][i], z0 = b[4][i], pz0 = b[5][i];
+        const double ps0 = std::sqrt((1.0 + pz0) * (1.0 + pz0) - px0 * px0 - py0 * py0);
+        const double h0 = 1.0 + pz0 - ps0;
+        double x1, px1, y1, py1, z1, pz1;
+        if (!reverse) {
+            py1 = py0 / c;
+            const double h1 = h0 / (c * c);
+            px1 = px0 / c - h1 * s;
+            pz1 = pz0 - px1 * s;
+            const double ps1 = 1.0 + pz1 - h1;
+            const double ds = x0 * s;
+            x1 = x0 + z0 * t + px1 / ps1 * ds;
+            y1 = y0 + py1 / ps1 * ds;
+            z1 = z0 / c - h1 / ps1 * ds;
+        } else {
+            x1 = x0 - z0 * s;
+            x1 = x1 / (1.0 + (px0 + h0 * s) * s / ps0);
+            z1 = (z0 + h0 / ps0 * x1 * s) * c;
+            y1 = y0 - py0 / ps0 * x1 * s;
+            pz1 = pz0 + px0 * s;
+            px1 = (px0 + h0 * s) * c;
+            py1 = py0 * c;
+        }
+        b[0][i] = x1; b[1][i] = px1; b[2][i] = y1; b[3][i] = py1; b[4][i] = z1; b[5][i] = pz1;
+    }
+    return 0;
+}
+
+// track_particle(::Symplectic6DMap, ::ThinCrabCavity{N}, ...) (crab_cavity.jl:147-164) with
+// kcc = 2 pi f / c (crab_cavity.jl:99-111): per harmonic i, theta = i kcc z + phase_i,
+// px -= sx_i sin/(i kcc), py -= sy_i sin/(i kcc), pz -= (sx_i x + sy_i y) cos.
+int oracle_track_crab_cavity(double kcc, int nh, const double *sx, const double *sy, const double *phase, double *b[6],
+                             int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double x0 = b[0][i], y0 = b[2][i], z0 = b[4][i];
+        double dpx = 0.0, dpy = 0.0, dpz = 0.0;
+        for (int k = 1; k <= nh; ++k) {
+            const double ikcc = (double)k * kcc;
+            const double th = ikcc * z0 + phase[k - 1];
+            const double sn = std::sin(th), cs = std::cos(th);
+            dpx -= sx[k - 1] * sn / ikcc;
+            dpy -= sy[k - 1] * sn / ikcc;
+            dpz -= (sx[k - 1] * x0 + sy[k - 1] * y0) * cs;
+        }
+        b[1][i] = b[1][i] + dpx;
+        b[3][i] = b[3][i] + dpy;
+        b[5][i] = b[5][i] + dpz;
     }
     return 0;
 }
