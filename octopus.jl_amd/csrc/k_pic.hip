@@ -571,18 +571,19 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     const int lk = a.lkick;
     constexpr int K = METHOD == 0 ? 2 : 3;
     // LDS window (uniform per block)
-    int wx0 = 0, wy0 = 0, wnx = 0, wny = 0;
+    int wx0 = 0, wy0 = 0, wnx = 0, wny = 0, wrs = 0;   // window origin, extent, row stride (elements)
     if (MODE != 2 && a.chunkbox && lo < hi) {
         const double *b = a.chunkbox + ((size_t)(2 * (d >> 1) + it.fld_beam) * a.chunks + chunk) * 8 + 4;
         int x0, x1, y0, y1;
         stencil_window<METHOD>((b[0] - it.fg.x0) / hx, (b[1] - it.fg.x0) / hx, nx, x0, x1);
         stencil_window<METHOD>((b[2] - it.fg.y0) / hy, (b[3] - it.fg.y0) / hy, ny, y0, y1);
-        if ((x1 - x0 + 1) * (y1 - y0 + 1) * C * (int)sizeof(R) <= kKickLdsBytes) {
-            wx0 = x0; wy0 = y0; wnx = x1 - x0 + 1; wny = y1 - y0 + 1;
-            const int run = wny * C;
+        // an odd row stride (in elements) puts vertically adjacent nodes in different LDS banks
+        const int run = (y1 - y0 + 1) * C, rs = run | 1;
+        if ((x1 - x0 + 1) * rs * (int)sizeof(R) <= kKickLdsBytes) {
+            wx0 = x0; wy0 = y0; wnx = x1 - x0 + 1; wny = y1 - y0 + 1; wrs = rs;
             for (int k = threadIdx.x; k < wnx * run; k += blockDim.x) {
-                const int r = k / run;
-                win[k] = tab[((size_t)(wx0 + r) * ny + wy0) * C + (k - r * run)];
+                const int r = k / run, e = k - r * run;
+                win[r * rs + e] = tab[((size_t)(wx0 + r) * ny + wy0) * C + e];
             }
             __syncthreads();
         }
@@ -610,7 +611,7 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
         const R wx[3] = {(R)wxd[0], (R)wxd[1], (R)wxd[2]}, wy[3] = {(R)wyd[0], (R)wyd[1], (R)wyd[2]};
         if (MODE != 2) {
             if (bx >= wx0 && bx + K <= wx0 + wnx && by >= wy0 && by + K <= wy0 + wny)
-                gather_field<METHOD, MODE, R>(win + ((bx - wx0) * wny + (by - wy0)) * C, wny * C, C, wx, wy, zL, Kx, Ky, Kz);
+                gather_field<METHOD, MODE, R>(win + (bx - wx0) * wrs + (by - wy0) * C, wrs, C, wx, wy, zL, Kx, Ky, Kz);
             else
                 gather_field<METHOD, MODE, R>(tab + ((size_t)bx * ny + by) * C, ny * C, C, wx, wy, zL, Kx, Ky, Kz);
         } else {

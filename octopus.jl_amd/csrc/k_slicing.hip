@@ -475,7 +475,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_scatter(TileSortArgs a) {
             a.dst[3][pos] = a.src[3][i];
             a.dst[4][pos] = a.src[4][i];
             a.dst[5][pos] = a.src[5][i];
-            a.perm_dst[pos] = a.perm_src[i];
+            const int64_t orig = a.perm_src[i];
+            a.perm_dst[pos] = orig;
+            a.inv[orig] = (int32_t)pos;
         }
 }
 
@@ -487,6 +489,31 @@ hipError_t launch_tile_sort(const TileSortArgs &a, int G, hipStream_t s) {
     hipLaunchKernelGGL(k_tile_hist, dim3(G, a.ns), dim3(kTileThreads), lds, s, a);
     hipLaunchKernelGGL(k_tile_scan, dim3(a.ns), dim3(1024), 0, s, a, G);
     hipLaunchKernelGGL(k_tile_scatter, dim3(G, a.ns), dim3(kTileThreads), lds, s, a);
+    return hipGetLastError();
+}
+
+// Tile-ordered states back into the beam through the inverse permutation: writes run in beam order
+// (coalesced) and the reads gather.
+__global__ __launch_bounds__(256) void k_store_gather(double *const *beam, int64_t n, double *const *st, const int32_t *inv,
+                                                      const ErrorInfo *err) {
+    if (err->code != 0) return;
+    double *X = beam[0], *PX = beam[1], *Y = beam[2], *PY = beam[3], *PZ = beam[5];
+    const double *SX = st[0], *SPX = st[1], *SY = st[2], *SPY = st[3], *SPZ = st[5];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t pos = inv[i];
+        X[i] = SX[pos];
+        PX[i] = SPX[pos];
+        Y[i] = SY[pos];
+        PY[i] = SPY[pos];
+        PZ[i] = SPZ[pos];
+    }
+}
+
+hipError_t launch_store_gather(double *const *beam, int64_t n, double *const *st, const int32_t *inv, const ErrorInfo *err,
+                               hipStream_t s) {
+    int g = (int)((n + 255) / 256);
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_store_gather, dim3(g), dim3(256), 0, s, beam, n, st, inv, err);
     return hipGetLastError();
 }
 

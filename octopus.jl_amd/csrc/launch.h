@@ -45,12 +45,15 @@ struct TileSortArgs {
     const int64_t *perm_src;
     double *dst[6];
     int64_t *perm_dst;
+    int32_t *inv;                    // inv[original index] = tile-ordered position
     int *thist;                      // [ns][T*T][G]
 };
 hipError_t launch_tile_sort(const TileSortArgs &a, int G, hipStream_t s);
 int64_t slicing_hist_entries(int64_t n, int ns);
 hipError_t launch_store(double *const *beam, int64_t n, double *const *st, const int64_t *perm, const ErrorInfo *err,
                         hipStream_t s);
+hipError_t launch_store_gather(double *const *beam, int64_t n, double *const *st, const int32_t *inv, const ErrorInfo *err,
+                               hipStream_t s);
 
 struct BatchPair { int32_t p, i, j, reserved; };
 

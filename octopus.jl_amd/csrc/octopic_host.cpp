@@ -224,6 +224,7 @@ struct octopic_handle {
     int tiles = 16;             // transverse tiles per axis of the in-slice order (0: slice order only)
     DBuf<double> srt[2][6];     // tile-ordered states (tiles > 0)
     DBuf<int64_t> perm_srt[2];
+    DBuf<int32_t> inv_srt[2];   // inverse of perm_srt
     DBuf<unsigned long long> box;   // transverse bound slots of both beams
     std::vector<double> hq[2];      // host quantiles / positions of the slicing
     DBuf<int> thist;
@@ -470,7 +471,7 @@ void octopic_destroy(octopic_handle *h) {
     h->tab.release(); h->q.release(); h->errinfo.release(); h->counters.release(); h->cache.release(); h->spec.release();
     for (int b = 0; b < 2; ++b) {
         for (int c = 0; c < 6; ++c) { h->beam_in[b][c].release(); h->st[b][c].release(); h->srt[b][c].release(); }
-        h->perm[b].release(); h->perm_srt[b].release(); h->sid[b].release(); h->hist[b].release();
+        h->perm[b].release(); h->perm_srt[b].release(); h->inv_srt[b].release(); h->sid[b].release(); h->hist[b].release();
     }
     h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
     h->keys.release(); h->chunkbox.release(); h->box.release(); h->thist.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
@@ -686,6 +687,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         if (h->tiles > 0) {
             for (int c = 0; c < 6; ++c) h->srt[b][c].ensure(n[b]);
             h->perm_srt[b].ensure(n[b]);
+            h->inv_srt[b].ensure(n[b]);
         }
         for (int c = 0; c < 6; ++c) {
             h->st[b][c].ensure(n[b]);
@@ -752,7 +754,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             TileSortArgs a{};
             a.tab = h->tab.p + b; a.ns = T.ns; a.T = h->tiles; a.box = h->box.p + 4 * b;
             for (int c = 0; c < 6; ++c) { a.src[c] = h->st[b][c].p; a.dst[c] = h->srt[b][c].p; }
-            a.perm_src = h->perm[b].p; a.perm_dst = h->perm_srt[b].p;
+            a.perm_src = h->perm[b].p; a.perm_dst = h->perm_srt[b].p; a.inv = h->inv_srt[b].p;
             a.thist = h->thist.p;
             HIPCHECK(launch_tile_sort(a, G, s));
         }
@@ -1008,7 +1010,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         Timed t(h, F_STORE);
         for (int b = 0; b < 2; ++b) {
             if (h->tiles > 0)
-                HIPCHECK(launch_store(h->ptrs.p + b * 6, n[b], h->ptrs.p + 24 + b * 6, h->perm_srt[b].p, h->errinfo.p, s));
+                HIPCHECK(launch_store_gather(h->ptrs.p + b * 6, n[b], h->ptrs.p + 24 + b * 6, h->inv_srt[b].p, h->errinfo.p, s));
             else
                 HIPCHECK(launch_store(h->ptrs.p + b * 6, n[b], h->ptrs.p + 12 + b * 6, h->perm[b].p, h->errinfo.p, s));
         }
