@@ -495,8 +495,43 @@ hipError_t launch_deposit(const DepositArgs &a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------------- kick
-constexpr int kKickThreads = 256;
-constexpr int kKickLdsBytes = 36 * 1024;   // LDS window of the field table per block (4 blocks per CU)
+constexpr int kKickThreads = 512;
+constexpr int kKickLdsBytes = 72 * 1024;   // LDS window of the field table per block (2 blocks per CU)
+
+// LDS window layout: each node's C values padded to a whole number of 16-byte vectors (double2 /
+// float4), rows of wny nodes plus one vector, so a node is read with 16-byte LDS loads and vertically
+// adjacent nodes start in different banks.
+template <class R> __device__ __forceinline__ int node_stride(int C) { return sizeof(R) == 8 ? (C + 1) & ~1 : (C + 3) & ~3; }
+template <class R> __device__ __forceinline__ int row_pad() { return 16 / (int)sizeof(R); }
+
+template <int MODE>
+__device__ __forceinline__ void load_node(const double *e, double *v) {
+    const double2 *p = reinterpret_cast<const double2 *>(e);
+#pragma unroll
+    for (int i = 0; i < (MODE == 1 ? 5 : 3); ++i) { const double2 t = p[i]; v[2 * i] = t.x; v[2 * i + 1] = t.y; }
+}
+template <int MODE>
+__device__ __forceinline__ void load_node(const float *e, float *v) {
+    const float4 *p = reinterpret_cast<const float4 *>(e);
+#pragma unroll
+    for (int i = 0; i < (MODE == 1 ? 3 : 2); ++i) {
+        const float4 t = p[i]; v[4 * i] = t.x; v[4 * i + 1] = t.y; v[4 * i + 2] = t.z; v[4 * i + 3] = t.w;
+    }
+}
+
+// Stages table rows [x0, x1] x columns [y0, y1] in the window when they fit (all threads of the block
+// call it; the caller barriers before the gathers).  Returns the row stride in elements, 0 if not.
+template <class R>
+__device__ __forceinline__ int stage_window(R *win, const R *tab, int ny, int C, int x0, int x1, int y0, int y1) {
+    const int CL = node_stride<R>(C), wny = y1 - y0 + 1, rs = wny * CL + row_pad<R>();
+    if ((x1 - x0 + 1) * rs * (int)sizeof(R) > kKickLdsBytes) return 0;
+    const int run = wny * C;
+    for (int k = threadIdx.x; k < (x1 - x0 + 1) * run; k += blockDim.x) {
+        const int r = k / run, e = k - r * run, node = e / C, c = e - node * C;
+        win[r * rs + node * CL + c] = tab[((size_t)(x0 + r) * ny + y0) * C + e];
+    }
+    return rs;
+}
 
 // Gather of one particle's field from the interleaved table rows starting at e0 (row = one x, rows
 // `rs` doubles apart, C doubles per cell), for the linear (MODE 0) or quadratic (MODE 1) slice
@@ -535,6 +570,42 @@ __device__ __forceinline__ void gather_field(const R *e0, int rs, int C, const R
     }
 }
 
+// The same gather from the LDS window (node stride CL, rows rs elements apart), 16-byte loads per node.
+template <int METHOD, int MODE, class R>
+__device__ __forceinline__ void gather_field_lds(const R *e0, int rs, int CL, const R wx[3], const R wy[3], R zL, R &Kx,
+                                                 R &Ky, R &Kz) {
+    constexpr int K = METHOD == 0 ? 2 : 3;
+    R v[12];
+    if (MODE == 0) {
+        const R zR = (R)1 - zL;
+#pragma unroll
+        for (int m = 0; m < K; ++m)
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                load_node<MODE>(e0 + m * rs + q * CL, v);
+                const R w = wx[m] * wy[q];
+                Kx += w * (zL * v[1] + zR * v[4]);
+                Ky += w * (zL * v[2] + zR * v[5]);
+                Kz += w * (v[0] - v[3]);
+            }
+    } else {
+        const R tt = (R)1 - zL;
+        const R t2 = tt * tt;
+        const R aL = (R)2 * t2 - (R)3 * tt + (R)1, aM = (R)4 * tt - (R)4 * t2, aR = (R)2 * t2 - tt;
+        const R bL = (R)3 - (R)4 * tt, bM = (R)8 * tt - (R)4, bR = (R)1 - (R)4 * tt;
+#pragma unroll
+        for (int m = 0; m < K; ++m)
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                load_node<MODE>(e0 + m * rs + q * CL, v);
+                const R w = wx[m] * wy[q];
+                Kx += w * (aL * v[1] + aM * v[7] + aR * v[4]);
+                Ky += w * (aL * v[2] + aM * v[8] + aR * v[5]);
+                Kz += w * (bL * v[0] + bM * v[6] + bR * v[3]);
+            }
+    }
+}
+
 // One block takes one chunk of an interaction's field slice.  In the slice-pair modes the table cells
 // of the chunk's stencil window (from the chunk's field box) are staged in LDS when they fit; a
 // particle whose stencil leaves the window reads the table in HBM.
@@ -546,12 +617,15 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     R *win = reinterpret_cast<R *>(win_raw);
     if (failed(a.err)) return;
     const int lblk = xcd_logical_block(blockIdx.x, gridDim.x);
-    if (lblk >= a.chunks * a.ninters) return;
-    const int d = lblk / a.chunks, chunk = lblk - d * a.chunks;
+    const int per_inter = a.chunks * a.subdiv;
+    if (lblk >= per_inter * a.ninters) return;
+    const int d = lblk / per_inter, rem = lblk - d * per_inter;
+    const int chunk = rem / a.subdiv, sub = rem - chunk * a.subdiv;
     const Inter it = a.inters[d];
     if (!it.valid) return;
     const int64_t n = it.fld_n;
-    const int64_t lo = n * chunk / a.chunks, hi = n * (chunk + 1) / a.chunks;
+    const int64_t clo = n * chunk / a.chunks, chi = n * (chunk + 1) / a.chunks;
+    const int64_t lo = clo + (chi - clo) * sub / a.subdiv, hi = clo + (chi - clo) * (sub + 1) / a.subdiv;
     const int nx = a.nx, ny = a.ny, C = 3 * a.npl;
     const R *tab = static_cast<const R *>(a.table) + (size_t)d * nx * ny * C;
     double *X = a.st.c[it.fld_beam][0] + it.fld_off, *PX = a.st.c[it.fld_beam][1] + it.fld_off;
@@ -572,46 +646,72 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     constexpr int K = METHOD == 0 ? 2 : 3;
     // LDS window (uniform per block)
     int wx0 = 0, wy0 = 0, wnx = 0, wny = 0, wrs = 0;   // window origin, extent, row stride (elements)
-    if (MODE != 2 && a.chunkbox && lo < hi) {
+    if (MODE != 2 && a.chunkbox && lo < hi && a.pass_windows != 2) {
         const double *b = a.chunkbox + ((size_t)(2 * (d >> 1) + it.fld_beam) * a.chunks + chunk) * 8 + 4;
         int x0, x1, y0, y1;
         stencil_window<METHOD>((b[0] - it.fg.x0) / hx, (b[1] - it.fg.x0) / hx, nx, x0, x1);
         stencil_window<METHOD>((b[2] - it.fg.y0) / hy, (b[3] - it.fg.y0) / hy, ny, y0, y1);
-        // an odd row stride (in elements) puts vertically adjacent nodes in different LDS banks
-        const int run = (y1 - y0 + 1) * C, rs = run | 1;
-        if ((x1 - x0 + 1) * rs * (int)sizeof(R) <= kKickLdsBytes) {
-            wx0 = x0; wy0 = y0; wnx = x1 - x0 + 1; wny = y1 - y0 + 1; wrs = rs;
-            for (int k = threadIdx.x; k < wnx * run; k += blockDim.x) {
-                const int r = k / run, e = k - r * run;
-                win[r * rs + e] = tab[((size_t)(wx0 + r) * ny + wy0) * C + e];
-            }
+        wrs = stage_window<R>(win, tab, ny, C, x0, x1, y0, y1);
+        if (wrs) {
+            wx0 = x0; wy0 = y0; wnx = x1 - x0 + 1; wny = y1 - y0 + 1;
             __syncthreads();
         }
     }
     int dropped = 0;
     const double fx1 = it.fg.x0 + it.fg.width, fy1 = it.fg.y0 + it.fg.height;
-    for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
-        double x = X[t], px = PX[t], y = Y[t], py = PY[t];
-        const double z = Z[t];
-        double pz = PZ[t];
+    // A chunk whose window does not fit stages a window per pass instead: the stencil nodes of the
+    // pass's particles (consecutive in tile order, so compact up to the drift) when they fit.
+    const bool per_pass = a.pass_windows && MODE != 2 && a.chunkbox && lo < hi && wnx == 0;
+    __shared__ int pass_box[kKickThreads / 64][4];
+    for (int64_t base = lo; base < hi; base += blockDim.x) {   // block-uniform trip count
+        const int64_t t = base + threadIdx.x;
+        const bool act = t < hi;
+        double x = 0.0, px = 0.0, y = 0.0, py = 0.0, z = 0.0, pz = 0.0;
+        if (act) { x = X[t]; px = PX[t]; y = Y[t]; py = PY[t]; z = Z[t]; pz = PZ[t]; }
         const double s = 0.5 * (z - c);
         x += s * px;
         y += s * py;
         if (lk) pz -= 0.25 * (px * px + py * py);
-        if (it.count_field && !(__builtin_isfinite(x) && it.fg.x0 <= x && x <= fx1 && __builtin_isfinite(y) &&
-                                it.fg.y0 <= y && y <= fy1))
+        if (act && it.count_field && !(__builtin_isfinite(x) && it.fg.x0 <= x && x <= fx1 && __builtin_isfinite(y) &&
+                                       it.fg.y0 <= y && y <= fy1))
             ++dropped;
         const double zLd = jclamp(-z * hzi + zbias, 0.0, 1.0);
         const R zL = (R)zLd;
         R Kx = 0, Ky = 0, Kz = 0;
         int bx, by;
         double wxd[3], wyd[3];
-        stencil<METHOD>((x - it.fg.x0) / hx, nx, bx, wxd);
-        stencil<METHOD>((y - it.fg.y0) / hy, ny, by, wyd);
+        const double ux = (x - it.fg.x0) / hx, uy = (y - it.fg.y0) / hy;
+        stencil<METHOD>(ux, nx, bx, wxd);
+        stencil<METHOD>(uy, ny, by, wyd);
         const R wx[3] = {(R)wxd[0], (R)wxd[1], (R)wxd[2]}, wy[3] = {(R)wyd[0], (R)wyd[1], (R)wyd[2]};
+        if (per_pass) {
+            // stencil-node box of the pass's in-range particles (out-of-range ones read node 0 with
+            // zero weights, from HBM)
+            const bool inr = act && 0.0 <= ux && ux <= (double)(nx - 1) && 0.0 <= uy && uy <= (double)(ny - 1);
+            int b0 = inr ? bx : 0x7fffffff, b1 = inr ? bx + K - 1 : -1;
+            int b2 = inr ? by : 0x7fffffff, b3 = inr ? by + K - 1 : -1;
+            for (int o = 32; o > 0; o >>= 1) {
+                b0 = min(b0, __shfl_xor(b0, o)); b1 = max(b1, __shfl_xor(b1, o));
+                b2 = min(b2, __shfl_xor(b2, o)); b3 = max(b3, __shfl_xor(b3, o));
+            }
+            __syncthreads();   // the previous pass's gathers are done with the window and pass_box
+            const int wv = threadIdx.x >> 6;
+            if ((threadIdx.x & 63) == 0) { pass_box[wv][0] = b0; pass_box[wv][1] = b1; pass_box[wv][2] = b2; pass_box[wv][3] = b3; }
+            __syncthreads();
+            for (int v = 0; v < (int)(blockDim.x >> 6); ++v) {
+                b0 = min(b0, pass_box[v][0]); b1 = max(b1, pass_box[v][1]);
+                b2 = min(b2, pass_box[v][2]); b3 = max(b3, pass_box[v][3]);
+            }
+            wnx = 0;
+            wrs = b1 >= 0 ? stage_window<R>(win, tab, ny, C, b0, b1, b2, b3) : 0;
+            if (wrs) { wx0 = b0; wy0 = b2; wnx = b1 - b0 + 1; wny = b3 - b2 + 1; }
+            __syncthreads();
+        }
+        if (!act) continue;
         if (MODE != 2) {
             if (bx >= wx0 && bx + K <= wx0 + wnx && by >= wy0 && by + K <= wy0 + wny)
-                gather_field<METHOD, MODE, R>(win + (bx - wx0) * wrs + (by - wy0) * C, wrs, C, wx, wy, zL, Kx, Ky, Kz);
+                gather_field_lds<METHOD, MODE, R>(win + (bx - wx0) * wrs + (by - wy0) * node_stride<R>(C), wrs,
+                                                  node_stride<R>(C), wx, wy, zL, Kx, Ky, Kz);
             else
                 gather_field<METHOD, MODE, R>(tab + ((size_t)bx * ny + by) * C, ny * C, C, wx, wy, zL, Kx, Ky, Kz);
         } else {
@@ -662,7 +762,8 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
 
 hipError_t launch_kick(const KickArgs &a, hipStream_t s) {
     if (a.ninters <= 0) return hipSuccess;
-    dim3 g(xcd_padded(a.chunks * a.ninters)), b(kKickThreads);
+    if (a.subdiv < 1) return hipErrorInvalidValue;
+    dim3 g(xcd_padded(a.chunks * a.subdiv * a.ninters)), b(kKickThreads);
     const size_t lds = (a.mode != 2 && a.chunkbox) ? kKickLdsBytes : 0;
 #define OCTOPIC_KICK(M, MODE)                                                                                    \
     do {                                                                                                         \

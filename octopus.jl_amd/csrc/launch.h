@@ -134,6 +134,8 @@ struct KickArgs {
     int nx, ny, npl, method, mode;   // mode: 0 linear, 1 quadratic, 2 node
     int lkick;
     int chunks;
+    int subdiv;                  // kick blocks per chunk (each takes an equal share of the chunk's members)
+    int pass_windows;            // stage a window per pass when the chunk's window does not fit
     unsigned long long *dropped;
     const double *chunkbox;      // chunk boxes of the batch's bounds launch, or null (no LDS windows)
     const ErrorInfo *err;

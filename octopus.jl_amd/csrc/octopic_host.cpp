@@ -222,6 +222,11 @@ struct octopic_handle {
     DBuf<double> st[2][6];
     DBuf<int64_t> perm[2];
     int tiles = 16;             // transverse tiles per axis of the in-slice order (0: slice order only)
+    // Kick launch shape: each bounds chunk is split over enough blocks for about kick_target_blocks
+    // blocks per launch (0: one block per chunk); per-pass LDS windows for chunks whose window does not
+    // fit (-1: in the FP64 form only, where they pay; 0 never; 1 such chunks; 2 every chunk).
+    int kick_target_blocks = 1024;
+    int kick_pass_windows = -1;
     DBuf<double> srt[2][6];     // tile-ordered states (tiles > 0)
     DBuf<int64_t> perm_srt[2];
     DBuf<int32_t> inv_srt[2];   // inverse of perm_srt
@@ -514,6 +519,16 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
             if (t < 0 || t > 128 || (t & (t - 1)) != 0)
                 fail(OCTOPIC_ERR_INVALID_ARGUMENT, "OCTOPIC_TRANSVERSE_TILES must be 0 or a power of two up to 128");
             h->tiles = t;
+        }
+        if (const char *e = std::getenv("OCTOPIC_KICK_TARGET_BLOCKS")) {
+            const int v = std::atoi(e);
+            if (v < 0 || v > (1 << 20)) fail(OCTOPIC_ERR_INVALID_ARGUMENT, "OCTOPIC_KICK_TARGET_BLOCKS must be in 0..2^20");
+            h->kick_target_blocks = v;
+        }
+        if (const char *e = std::getenv("OCTOPIC_KICK_PASS_WINDOWS")) {
+            const int v = std::atoi(e);
+            if (v < -1 || v > 2) fail(OCTOPIC_ERR_INVALID_ARGUMENT, "OCTOPIC_KICK_PASS_WINDOWS must be -1, 0, 1 or 2");
+            h->kick_pass_windows = v;
         }
         h->box.ensure(8);
         h->errinfo.ensure(1);
@@ -990,6 +1005,13 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.fp32 = h->fp32 ? 1 : 0;
             a.method = o.deposit; a.mode = h->mode; a.lkick = o.longitudinal_kick ? 1 : 0;
             a.chunks = nchunks;
+            a.subdiv = 1;
+            if (h->kick_target_blocks > 0) {
+                const int per = nchunks * 2 * B;
+                const int sd = (h->kick_target_blocks + per - 1) / per;
+                a.subdiv = sd < 1 ? 1 : (sd > 64 ? 64 : sd);
+            }
+            a.pass_windows = h->kick_pass_windows >= 0 ? h->kick_pass_windows : (h->fp32 ? 0 : 1);
             a.dropped = h->counters.p + 4;
             a.chunkbox = cbox;
             a.err = h->errinfo.p;
