@@ -136,10 +136,10 @@ def run_instances(args):
         "particle_visit_GBps": round(88.0 * visits / dt / 1e9, 1)}))
 
 
-def measured_traffic(arm, family):
+def measured_traffic(arm, family, precision):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary (copy-kernel calibrated), when one exists for this workload and kernel."""
-    kernel = {("quadratic", "kick"): "k_kick<1, 1>", ("node", "kick"): "k_kick<1, 2>"}.get((arm, family))
+    summary (copy-kernel calibrated), when one exists for this workload, kernel and precision."""
+    kernel = {("quadratic", "kick", "fp64"): "k_kick<1, 1, double>"}.get((arm, family, precision))
     path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILE[arm]) if arm in TRAFFIC_PROFILE else None
     if kernel is None or path is None or not os.path.exists(path):
         return None
@@ -269,7 +269,7 @@ def main():
             per_launch_bytes = model[best] * args.steps / launches
             avg_s = ms / 1e3 / launches
             achieved = per_launch_bytes / avg_s / 1e9
-            traffic = measured_traffic(args.arm, best)
+            traffic = measured_traffic(args.arm, best, args.precision)
             roof = {"bound": "hbm", "kernel": best, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": round(traffic) if traffic else None,
