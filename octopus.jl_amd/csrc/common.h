@@ -92,6 +92,14 @@ __device__ __forceinline__ void stencil_window(double umin, double umax, int n, 
     if (hi < lo) { lo = 0; hi = n - 1; }
 }
 
+// Round-to-nearest-even of a double in [0, 2^52) to uint64 (what __double2ull_rn gives there): adding
+// 2^52 leaves the rounded integer in the low mantissa bits, one FP64 add instead of the general
+// conversion sequence.
+__device__ __forceinline__ unsigned long long fixed_point(double v) {
+    const double t = v + 4503599627370496.0;   // 2^52
+    return (unsigned long long)__builtin_bit_cast(long long, t) - 0x4330000000000000ull;
+}
+
 // Order-preserving uint64 key of a non-NaN double (-0.0 sorts below +0.0, as Julia's min/max), so that
 // exact min/max reductions can use 64-bit integer atomics in any order.
 __host__ __device__ __forceinline__ unsigned long long order_key(double x) {

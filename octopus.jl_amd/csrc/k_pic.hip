@@ -457,10 +457,12 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
         for (int m = 0; m < K; ++m)
 #pragma unroll
             for (int q = 0; q < K; ++q) {
+                // weights lie in [0, 1] and scale <= 2^40, so the product is in fixed_point's range;
+                // zero weights (off-mesh positions, at node 0) add nothing
                 unsigned long long v;
-                if constexpr (sizeof(W) == sizeof(double)) v = __double2ull_rn(wx[m] * wy[q] * scale);
-                else v = __float2ull_rn((float)wx[m] * (float)wy[q] * (float)scale);
-                if (v) atomicAdd(&tile[(by + q) * n1 + bx + m], v);
+                if constexpr (sizeof(W) == sizeof(double)) v = fixed_point(wx[m] * wy[q] * scale);
+                else v = fixed_point((double)((float)wx[m] * (float)wy[q] * (float)scale));
+                atomicAdd(&tile[(by + q) * n1 + bx + m], v);
             }
     }
     if (pl.count_box) {
