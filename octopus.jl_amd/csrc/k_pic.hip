@@ -404,10 +404,13 @@ hipError_t launch_geometry(const GeometryArgs &a, hipStream_t s) {
 // ---------------------------------------------------------------------------------- deposit
 constexpr int kDepositThreads = 1024;
 
-// The block's LDS copy of the whole mesh is zeroed and flushed only over the stencil window of its
-// chunks' source box: every position it deposits lies in that box (the sM and luminosity drifts lie
-// between sL and sR, and positions are x + px*s, monotonic in s), the first stencil node is monotonic
-// in the position, and positions off the mesh have zero weights.
+// The block keeps an LDS copy of the stencil window of its chunks' source box (window-local rows),
+// zeroed before and flushed after the deposit: every position it deposits lies in that box (the sM and
+// luminosity drifts lie between sL and sR, and positions are x + px*s, monotonic in s), the first
+// stencil node is monotonic in the position, and positions off the mesh have zero weights.  The LDS
+// holds up to lds_cells cells (the whole mesh up to 143 x 143 nodes); a block whose window is larger
+// (larger meshes only) adds straight into the plane in HBM.  Integer sums are associative, so the
+// charge is bit-identical either way.
 // W = double, or float in the FP32 mode (the weight product in FP32, then the same fixed point).
 template <int METHOD, class W>
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
@@ -431,11 +434,11 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
         stencil_window<METHOD>((b[2] - pl.y0) * pl.hyi, (b[3] - pl.y0) * pl.hyi, n2, wy0, wy1);
     }
     const int wnx = wx1 - wx0 + 1, wcells = wnx * (wy1 - wy0 + 1);
-    for (int c = threadIdx.x; c < wcells; c += blockDim.x) {
-        const int r = c / wnx;
-        tile[(wy0 + r) * n1 + wx0 + (c - r * wnx)] = 0ull;
+    const bool in_lds = wcells <= a.lds_cells;   // block-uniform
+    if (in_lds) {
+        for (int c = threadIdx.x; c < wcells; c += blockDim.x) tile[c] = 0ull;
+        __syncthreads();
     }
-    __syncthreads();
     unsigned long long *out = a.charge + (size_t)blockIdx.y * a.charge_stride;
     const double *X = a.st.c[pl.beam][0] + pl.off, *PX = a.st.c[pl.beam][1] + pl.off, *Y = a.st.c[pl.beam][2] + pl.off,
                  *PY = a.st.c[pl.beam][3] + pl.off;
@@ -457,30 +460,37 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
         for (int m = 0; m < K; ++m)
 #pragma unroll
             for (int q = 0; q < K; ++q) {
-                // weights lie in [0, 1] and scale <= 2^40, so the product is in fixed_point's range;
-                // zero weights (off-mesh positions, at node 0) add nothing
+                // weights lie in [0, 1] and scale <= 2^40, so the product is in fixed_point's range
                 unsigned long long v;
                 if constexpr (sizeof(W) == sizeof(double)) v = fixed_point(wx[m] * wy[q] * scale);
                 else v = fixed_point((double)((float)wx[m] * (float)wy[q] * (float)scale));
-                atomicAdd(&tile[(by + q) * n1 + bx + m], v);
+                if (in_lds) {
+                    // a zero weight (off-mesh positions sit at node 0, possibly outside the window) adds
+                    // its nothing to the window's first cell
+                    atomicAdd(&tile[v ? (by + q - wy0) * wnx + (bx + m - wx0) : 0], v);
+                } else if (v) {
+                    atomicAdd(&out[(by + q) * n1 + bx + m], v);
+                }
             }
     }
     if (pl.count_box) {
         for (int o = 32; o > 0; o >>= 1) dropped += __shfl_down(dropped, o);
         if ((threadIdx.x & 63) == 0 && dropped) atomicAdd(&a.dropped[0], (unsigned long long)dropped);
     }
-    __syncthreads();
-    for (int c = threadIdx.x; c < wcells; c += blockDim.x) {
-        const int r = c / wnx;
-        const int cell = (wy0 + r) * n1 + wx0 + (c - r * wnx);
-        const unsigned long long v = tile[cell];
-        if (v) atomicAdd(&out[cell], v);
+    if (in_lds) {
+        __syncthreads();
+        for (int c = threadIdx.x; c < wcells; c += blockDim.x) {
+            const int r = c / wnx;
+            const unsigned long long v = tile[c];
+            if (v) atomicAdd(&out[(wy0 + r) * n1 + wx0 + (c - r * wnx)], v);
+        }
     }
 }
 
-hipError_t launch_deposit(const DepositArgs &a, hipStream_t s) {
+hipError_t launch_deposit(DepositArgs a, hipStream_t s) {
     if (a.nplanes <= 0) return hipSuccess;
-    const size_t lds = (size_t)a.ncells * sizeof(unsigned long long);
+    a.lds_cells = a.ncells < kDepositMaxLdsCells ? a.ncells : kDepositMaxLdsCells;
+    const size_t lds = (size_t)a.lds_cells * sizeof(unsigned long long);
     dim3 g((a.chunks + a.group - 1) / a.group, a.nplanes);
 #define OCTOPIC_DEPOSIT(M, W)                                                                                  \
     do {                                                                                                       \

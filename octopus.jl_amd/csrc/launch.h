@@ -122,8 +122,10 @@ struct DepositArgs {
     unsigned long long *dropped; // dropped-particle counter
     const double *chunkbox;      // chunk boxes of the batch's bounds launch, or null (whole mesh windows)
     const ErrorInfo *err;
+    int lds_cells;               // set by the launcher: LDS window capacity in cells
 };
-hipError_t launch_deposit(const DepositArgs &a, hipStream_t s);
+constexpr int kDepositMaxLdsCells = 20480;   // 160 KB of uint64 cells per block
+hipError_t launch_deposit(DepositArgs a, hipStream_t s);
 
 struct KickArgs {
     const Inter *inters;

@@ -389,9 +389,11 @@ void validate(const octopic_options &o) {
     if (o.green_type == OCTOPIC_GREEN_LATTICE)
         fail(OCTOPIC_ERR_UNSUPPORTED, "green_type = :lattice (experimental in the reference) is not implemented by the HIP "
                                       "PIC backend; use :integrated or :standard.");
-    if ((int64_t)lnx * 0 + (int64_t)(lnx + 1) * (lny + 1) * 8 > 160 * 1024 || (int64_t)o.nx * o.ny * 8 > 160 * 1024)
-        fail(OCTOPIC_ERR_UNSUPPORTED, "meshes above 143x143 nodes (or luminosity meshes above 142x142) exceed the LDS "
-                                      "deposit tile of this backend");
+    if (o.nx > 512 || o.ny > 512)
+        fail(OCTOPIC_ERR_UNSUPPORTED, "meshes above 512 nodes per axis exceed the LDS transforms of this backend "
+                                      "(padded size 2*grid <= 1024)");
+    if ((int64_t)(lnx + 1) * (lny + 1) > (int64_t)1 << 20)
+        fail(OCTOPIC_ERR_UNSUPPORTED, "luminosity meshes above 2^20 nodes are not supported");
 }
 
 void upload_ptrs(octopic_handle *h, double *const in[2][6]) {

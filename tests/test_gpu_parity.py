@@ -215,6 +215,39 @@ def test_slicing_methods_parity(oracle, gpu, method):
     compare_collide(oracle, gpu, PICPoissonSolver(grid=(32, 32), slicing=sl), e, p)
 
 
+def test_empty_slices_skip_their_pairs(oracle, gpu):
+    """`:specified` boundaries at z = -9 m and -8.5 m leave the first two slices of both beams empty:
+    every pair with an empty slice is skipped (pic_cpu.jl:302), the others match the oracle."""
+    e, p = round_beams(oracle, 12000, 6000, seed=4)
+    sl = LongitudinalSlicing(nslices=4, method="specified", center_position="centroid", positions=(-9.0, -8.5, 0.0))
+    gcol = compare_collide(oracle, gpu, PICPoissonSolver(grid=(32, 32), slicing=sl, deposit_method="TSC"), e, p)
+    _, _, ran, _ = gcol.pair_luminosity()
+    assert 0 < sum(ran) < len(ran)
+
+
+def test_tiny_ragged_beams(oracle, gpu):
+    """Beams far smaller than one particle chunk, with sizes that fill no block evenly."""
+    e, p = round_beams(oracle, 61, 23, seed=9)
+    sl = LongitudinalSlicing(nslices=3, method="normal_quantile", center_position="centroid")
+    compare_collide(oracle, gpu, PICPoissonSolver(grid=(16, 16), slicing=sl, deposit_method="TSC",
+                                                  field_derivative="fourth", slice_interpolation="quadratic"), e, p)
+
+
+@pytest.mark.parametrize("interaction_grid", ["slice_pair", "node"])
+def test_large_mesh_parity(oracle, gpu, interaction_grid):
+    """Meshes above 143 x 143 nodes (here 192 x 162, luminosity 160 x 150): deposit windows that fit
+    160 KB of LDS stay there, larger ones (the whole-mesh windows of the node mode) add straight into
+    HBM; the charge is the same integer sum either way."""
+    e, p = round_beams(oracle, 12000, 5000, seed=6)
+    sl = LongitudinalSlicing(nslices=3, method="normal_quantile", center_position="centroid")
+    kw = dict(deposit_method="TSC", field_derivative="fourth")
+    if interaction_grid == "slice_pair":
+        kw["slice_interpolation"] = "quadratic"
+    solver = PICPoissonSolver(grid=(192, 162), slicing=sl, interaction_grid=interaction_grid,
+                              luminosity_grid=(160, 150), **kw)
+    compare_collide(oracle, gpu, solver, e, p)
+
+
 def test_stencil_cell_indices_bitwise(oracle, gpu):
     solver = PICPoissonSolver(grid=(16, 16))
     col = PICCollider(solver)
@@ -310,7 +343,8 @@ def test_degenerate_extent(oracle, gpu):
 def test_unsupported_combinations_are_rejected(gpu):
     for kw in (dict(interaction_grid="node", slice_interpolation="quadratic"), dict(grid_extent="sigma"),
                dict(interaction_grid="source_slice"), dict(green_type="lattice"),
-               dict(slicing=LongitudinalSlicing(nslices=3, method="equal_count"))):
+               dict(slicing=LongitudinalSlicing(nslices=3, method="equal_count")),
+               dict(grid=(80, 80)), dict(grid=(576, 576))):   # 2*80 has a factor 5; 576 > 512 nodes
         with pytest.raises(UnsupportedOptionError):
             PICCollider(PICPoissonSolver(**kw))
 
