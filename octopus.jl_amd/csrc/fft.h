@@ -1,4 +1,4 @@
-// fft.h -- in-LDS mixed-radix Stockham FFT for the padded PIC meshes (P = 2g points per axis).
+// fft.h -- in-LDS mixed-radix (4, 2, 3, 5) Stockham FFT for the padded PIC meshes (P = 2g points per axis).
 //
 // One wave (64 lanes) transforms one P-point complex sequence (double2 in the FP64 mode, float2 in
 // the FP32 mode) held in LDS, ping-ponging between two LDS buffers, with the twiddle table staged in
@@ -76,6 +76,23 @@ __device__ __forceinline__ void dft3(C *v) {
     v[2] = csub(t2, t3);
 }
 
+template <int SIGN, class C>
+__device__ __forceinline__ void dft5(C *v) {
+    using R = real_of<C>;
+    const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;   // cos(2pi/5), cos(4pi/5)
+    const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;    // sin(2pi/5), sin(4pi/5)
+    const C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const C a1 = cx<C>(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    const C a2 = cx<C>(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    const C b1 = mul_i<SIGN>(cx<C>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
+    const C b2 = mul_i<SIGN>(cx<C>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+}
+
 // Orders this wave's earlier LDS writes before its later LDS reads (a transform group is one wave, so
 // no workgroup barrier is needed between its stages).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -131,7 +148,7 @@ __device__ void fft_lds(C *a, C *b, const FFTPlanDev &P, const C *tw, int lane) 
                 const int base = q0 * Ns * 2 + k;
                 b[base] = v[0]; b[base + Ns] = v[1];
             }
-        } else {
+        } else if (r == 3) {
             for (int j = lane; j < m; j += 64) {
                 const int q0 = j / Ns, k = j - q0 * Ns;
                 C v[3];
@@ -141,6 +158,18 @@ __device__ void fft_lds(C *a, C *b, const FFTPlanDev &P, const C *tw, int lane) 
                 dft3<SIGN>(v);
                 const int base = q0 * Ns * 3 + k;
                 b[base] = v[0]; b[base + Ns] = v[1]; b[base + 2 * Ns] = v[2];
+            }
+        } else {
+            for (int j = lane; j < m; j += 64) {
+                const int q0 = j / Ns, k = j - q0 * Ns;
+                C v[5];
+                v[0] = a[j];
+#pragma unroll
+                for (int q = 1; q < 5; ++q) v[q] = cmul(a[j + q * m], twiddle<SIGN>(tw, q * k * tstep));
+                dft5<SIGN>(v);
+                const int base = q0 * Ns * 5 + k;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) b[base + q * Ns] = v[q];
             }
         }
         wave_lds_sync();

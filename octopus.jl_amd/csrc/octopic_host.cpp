@@ -167,10 +167,11 @@ struct FFTPlanHost {
         while (m % 4 == 0) { dev.radix[k++] = 4; m /= 4; }
         while (m % 2 == 0) { dev.radix[k++] = 2; m /= 2; }
         while (m % 3 == 0) { dev.radix[k++] = 3; m /= 3; }
+        while (m % 5 == 0) { dev.radix[k++] = 5; m /= 5; }
         if (m != 1 || k > kMaxStages)
             fail(OCTOPIC_ERR_UNSUPPORTED, "padded mesh size 2*grid = " + std::to_string(n) +
-                                              " has a prime factor other than 2 and 3; this backend's FFT supports "
-                                              "grids whose doubled size factors into 2s and 3s");
+                                              " has a prime factor other than 2, 3 and 5; this backend's FFT supports "
+                                              "grids whose doubled size factors into 2s, 3s and 5s");
         dev.nstages = k;
         std::vector<double2> h(n);
         for (int t = 0; t < n; ++t) {

@@ -248,6 +248,16 @@ def test_large_mesh_parity(oracle, gpu, interaction_grid):
     compare_collide(oracle, gpu, solver, e, p)
 
 
+@pytest.mark.parametrize("grid", [(80, 80), (100, 120)], ids=["80", "100x120"])
+def test_radix5_meshes_parity(oracle, gpu, grid):
+    """Padded sizes 2g with factors of 5 (160 = 2^5 5, 200 = 2^3 5^2, 240 = 2^4 3 5) run the radix-5
+    stage of the LDS transforms."""
+    e, p = round_beams(oracle, 15000, 6000, seed=8)
+    sl = LongitudinalSlicing(nslices=3, method="normal_quantile", center_position="centroid")
+    compare_collide(oracle, gpu, PICPoissonSolver(grid=grid, slicing=sl, deposit_method="TSC", field_derivative="fourth",
+                                                  slice_interpolation="quadratic"), e, p)
+
+
 def test_stencil_cell_indices_bitwise(oracle, gpu):
     solver = PICPoissonSolver(grid=(16, 16))
     col = PICCollider(solver)
@@ -344,7 +354,7 @@ def test_unsupported_combinations_are_rejected(gpu):
     for kw in (dict(interaction_grid="node", slice_interpolation="quadratic"), dict(grid_extent="sigma"),
                dict(interaction_grid="source_slice"), dict(green_type="lattice"),
                dict(slicing=LongitudinalSlicing(nslices=3, method="equal_count")),
-               dict(grid=(80, 80)), dict(grid=(576, 576))):   # 2*80 has a factor 5; 576 > 512 nodes
+               dict(grid=(70, 70)), dict(grid=(576, 576))):   # 2*70 has a factor 7; 576 > 512 nodes
         with pytest.raises(UnsupportedOptionError):
             PICCollider(PICPoissonSolver(**kw))
 
