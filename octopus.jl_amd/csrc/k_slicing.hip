@@ -14,6 +14,8 @@
 //   centroid     : the same 4096-lane fold over each member list (slicing.jl:245-257, 523-559).
 //   tile order   : members regrouped by transverse tile within each slice (no counterpart in the
 //                  reference: a memory-layout choice for the field gathers; results do not depend on it).
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "common.h"
 #include "launch.h"
 
@@ -121,13 +123,15 @@ __global__ __launch_bounds__(256) void k_zfold(const double *lanes, const double
         double step = (zmax - zmin) / (double)ns;
         for (int k = 1; k < ns; ++k) b[k] = zmin + (double)k * step;
     }
-    // equal_area boundaries are produced by k_equal_area after the histogram
+    // equal_area boundaries are produced by k_equal_area after the histogram, equal_count ones by
+    // k_rank_table after the rank sort
     bool finite = true;
-    if (method != 0)
+    if (method != 0 && method != 1)
         for (int k = 0; k <= ns; ++k) finite = finite && __builtin_isfinite(b[k]);
     finite = finite && __builtin_isfinite(mean) && __builtin_isfinite(zmin) && __builtin_isfinite(zmax);
     if (!finite) atomicCAS(&err->code, 0, 3);
-    tab->degenerate = (method == 0 || method == 2) ? (zmin == zmax) : (b[0] == b[ns]);
+    // equal_count membership is by rank: no degenerate special case (slicing.jl:330-365)
+    tab->degenerate = method == 1 ? 0 : ((method == 0 || method == 2) ? (zmin == zmax) : (b[0] == b[ns]));
 }
 
 // :equal_area histogram (_threaded_histogram / _slice_bin, slicing.jl:172-176, 291-328), LDS-privatised.
@@ -203,6 +207,27 @@ __device__ __forceinline__ int slice_of(double zi, const double *b, int ns) {
     return iclamp(lo, 1, ns) - 1;
 }
 
+// Folds this block's (x min, x max, y min, y max) into the 4 bound slots (exact, any order).  Called by
+// every thread of a kSliceThreads block.
+__device__ __forceinline__ void fold_box(const double v[4], unsigned long long *box) {
+    __shared__ unsigned long long red[kSliceThreads / 64][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int q = 0; q < 4; ++q) {
+        unsigned long long u = bound_slot(v[q], (q & 1) == 0);
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long w = __shfl_xor(u, o);
+            u = w > u ? w : u;
+        }
+        if (lane == 0) red[wave][q] = u;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        unsigned long long u = 0ull;
+        for (int w = 0; w < kSliceThreads / 64; ++w) u = red[w][threadIdx.x] > u ? red[w][threadIdx.x] : u;
+        if (u) atomicMax(&box[threadIdx.x], u);
+    }
+}
+
 __global__ __launch_bounds__(kSliceThreads) void k_classify(const double *z, const double *x, const double *y, int64_t n,
                                                             const SliceTable *tab, uint16_t *sid, int *hist, int nblocks,
                                                             int by_bin, unsigned long long *box, const ErrorInfo *err) {
@@ -234,24 +259,7 @@ __global__ __launch_bounds__(kSliceThreads) void k_classify(const double *z, con
     }
     __syncthreads();
     for (int k = threadIdx.x; k < ns; k += blockDim.x) hist[(size_t)k * nblocks + blockIdx.x] = cnt[k];
-    if (box) {   // the beam's transverse box, for the tile order below
-        __shared__ unsigned long long red[kSliceThreads / 64][4];
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        for (int q = 0; q < 4; ++q) {
-            unsigned long long u = bound_slot(v[q], (q & 1) == 0);
-            for (int o = 32; o > 0; o >>= 1) {
-                const unsigned long long w = __shfl_xor(u, o);
-                u = w > u ? w : u;
-            }
-            if (lane == 0) red[wave][q] = u;
-        }
-        __syncthreads();
-        if (threadIdx.x < 4) {
-            unsigned long long u = 0ull;
-            for (int w = 0; w < kSliceThreads / 64; ++w) u = red[w][threadIdx.x] > u ? red[w][threadIdx.x] : u;
-            if (u) atomicMax(&box[threadIdx.x], u);
-        }
-    }
+    if (box) fold_box(v, box);   // the beam's transverse box, for the tile order below
 }
 
 // Exclusive scan of hist (slice-major, ns * nblocks) in place; writes the per-slice count/offset.
@@ -518,6 +526,61 @@ hipError_t launch_store_gather(double *const *beam, int64_t n, double *const *st
 }
 
 // ---------------------------------------------------------------------- launcher
+// ---------------------------------------------------------------- :equal_count (slicing.jl:330-365)
+// Membership by rank: a stable sort of the order-preserving keys of z (ties keep ascending index, as
+// the reference's stable sortperm; -0.0 below +0.0 as isless) gives the rank order; slice s holds ranks
+// [floor(s n / ns), floor((s+1) n / ns)) and its members stay in rank order (the order the centroid
+// lane fold sums them in).  Internal boundaries are midpoints of the neighbouring ranks' z.
+__global__ void k_rank_keys(const double *z, int64_t n, unsigned long long *keys, int64_t *vals) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        keys[i] = order_key(z[i]);
+        vals[i] = i;
+    }
+}
+
+__global__ void k_rank_table(SliceTable *tab, const unsigned long long *sorted, int64_t n, int ns) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double *b = tab->boundary;
+    b[0] = tab->zmin;
+    b[ns] = tab->zmax;
+    for (int s = 0; s < ns; ++s) {
+        const int64_t lo = (int64_t)s * n / ns, hi = (int64_t)(s + 1) * n / ns;
+        tab->offset[s] = lo;
+        tab->count[s] = hi - lo;
+    }
+    for (int s = 1; s < ns; ++s) {
+        const int64_t pos = (int64_t)s * n / ns;   // 1-based rank of the last member of slice s-1
+        if (pos == 0) b[s] = key_value(sorted[0]);
+        else if (pos == n) b[s] = key_value(sorted[n - 1]);
+        else b[s] = (key_value(sorted[pos - 1]) + key_value(sorted[pos])) / 2.0;
+    }
+}
+
+// states in rank order (st[c][r] = beam[c][perm[r]]) and the beam's transverse box
+__global__ __launch_bounds__(kSliceThreads) void k_rank_gather(const double *const *beam, int64_t n, const int64_t *perm,
+                                                               double *const *st, unsigned long long *box) {
+    double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = perm[r];
+        const double x = beam[0][i], y = beam[2][i];
+        st[0][r] = x;
+        st[1][r] = beam[1][i];
+        st[2][r] = y;
+        st[3][r] = beam[3][i];
+        st[4][r] = beam[4][i];
+        st[5][r] = beam[5][i];
+        v[0] = jmin(v[0], x); v[1] = jmax(v[1], x); v[2] = jmin(v[2], y); v[3] = jmax(v[3], y);
+    }
+    if (box) fold_box(v, box);
+}
+
+size_t rank_sort_bytes(int64_t n) {
+    size_t bytes = 0;
+    rocprim::radix_sort_pairs((void *)nullptr, bytes, (const unsigned long long *)nullptr, (unsigned long long *)nullptr,
+                              (const int64_t *)nullptr, (int64_t *)nullptr, (size_t)n);
+    return bytes;
+}
+
 hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s) {
     const int nblocks = (int)((a.n + kSliceBlock - 1) / kSliceBlock);
     hipLaunchKernelGGL(k_zlanes, dim3(kFoldLanes / 256), dim3(256), 0, s, a.z, a.n, a.tab, 1, a.lanes, a.mm_partial);
@@ -532,12 +595,24 @@ hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s) {
         hipLaunchKernelGGL(k_equal_area_hist, dim3(g), dim3(256), bins * 4, s, a.z, a.n, a.tab, bins, a.bin_counts);
         hipLaunchKernelGGL(k_equal_area, dim3(1), dim3(64), 0, s, a.tab, a.n, a.ns, bins, a.bin_counts, a.err);
     }
-    hipLaunchKernelGGL(k_classify, dim3(nblocks), dim3(kSliceThreads), 0, s, a.z, a.x, a.y, a.n, a.tab, a.sid, a.hist, nblocks,
-                       a.method == 2 ? 1 : 0, a.box, a.err);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.hist, a.ns, nblocks, a.tab);
-    hipLaunchKernelGGL(k_counts, dim3(1), dim3(256), 0, s, a.tab, a.n);
-    hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(kSliceThreads), 0, s, a.beam, a.n, a.sid, a.hist, nblocks, a.ns, a.st,
-                       a.perm);
+    if (a.method == 1) {
+        int g = (int)((a.n + kSliceThreads - 1) / kSliceThreads);
+        if (g > 4096) g = 4096;
+        hipLaunchKernelGGL(k_rank_keys, dim3(g), dim3(256), 0, s, a.z, a.n, a.rank_keys, a.rank_vals);
+        size_t bytes = a.rank_temp_bytes;
+        hipError_t e = rocprim::radix_sort_pairs(a.rank_temp, bytes, a.rank_keys, a.rank_sorted, a.rank_vals, a.perm,
+                                                 (size_t)a.n, 0, 64, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rank_table, dim3(1), dim3(64), 0, s, a.tab, a.rank_sorted, a.n, a.ns);
+        hipLaunchKernelGGL(k_rank_gather, dim3(g), dim3(kSliceThreads), 0, s, a.beam, a.n, a.perm, a.st, a.box);
+    } else {
+        hipLaunchKernelGGL(k_classify, dim3(nblocks), dim3(kSliceThreads), 0, s, a.z, a.x, a.y, a.n, a.tab, a.sid, a.hist,
+                           nblocks, a.method == 2 ? 1 : 0, a.box, a.err);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.hist, a.ns, nblocks, a.tab);
+        hipLaunchKernelGGL(k_counts, dim3(1), dim3(256), 0, s, a.tab, a.n);
+        hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(kSliceThreads), 0, s, a.beam, a.n, a.sid, a.hist, nblocks, a.ns,
+                           a.st, a.perm);
+    }
     hipLaunchKernelGGL(k_centroid_lanes, dim3(kFoldLanes / 256, a.ns), dim3(256), 0, s, a.st_z, a.tab, a.centroid_lanes);
     hipLaunchKernelGGL(k_centroid_fold, dim3(a.ns), dim3(256), 0, s, a.centroid_lanes, a.tab, a.n, a.centroid);
     return hipGetLastError();

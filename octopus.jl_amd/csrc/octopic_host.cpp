@@ -231,6 +231,9 @@ struct octopic_handle {
     DBuf<double> srt[2][6];     // tile-ordered states (tiles > 0)
     DBuf<int64_t> perm_srt[2];
     DBuf<int32_t> inv_srt[2];   // inverse of perm_srt
+    DBuf<unsigned long long> rank_keys, rank_sorted;   // :equal_count rank sort (one beam at a time)
+    DBuf<int64_t> rank_vals;
+    DBuf<unsigned char> rank_temp;
     DBuf<unsigned long long> box;   // transverse bound slots of both beams
     std::vector<double> hq[2];      // host quantiles / positions of the slicing
     DBuf<int> thist;
@@ -330,9 +333,6 @@ void validate_slicing(const octopic_slicing &s, int beam) {
         fail(OCTOPIC_ERR_INVALID_OPTION, b + "center_position must be :centroid or :midpoint");
     if (s.method == OCTOPIC_SLICE_SPECIFIED && s.npositions != s.nslices - 1)
         fail(OCTOPIC_ERR_INVALID_OPTION, b + "specified slicing requires nslices-1 internal positions");
-    if (s.method == OCTOPIC_SLICE_EQUAL_COUNT)
-        fail(OCTOPIC_ERR_UNSUPPORTED, b + "method = :equal_count (membership by rank) is not implemented by the HIP "
-                                          "backend; use :normal_quantile, :equal_area, :equal_width or :specified");
     if (s.method == OCTOPIC_SLICE_EQUAL_AREA && (int64_t)s.nslices * s.resolution > 16384)
         fail(OCTOPIC_ERR_UNSUPPORTED, b + ":equal_area with nslices*resolution > 16384 histogram bins is not supported");
 }
@@ -424,6 +424,12 @@ void run_slicing(octopic_handle *h, int b, const octopic_slicing &s, int64_t n) 
     h->perm[b].ensure(n);
     h->sid[b].ensure(n);
     h->hist[b].ensure(slicing_hist_entries(n, ns));
+    if (s.method == OCTOPIC_SLICE_EQUAL_COUNT) {
+        h->rank_keys.ensure(n);
+        h->rank_sorted.ensure(n);
+        h->rank_vals.ensure(n);
+        h->rank_temp.ensure(rank_sort_bytes(n));
+    }
 }
 
 }  // namespace
@@ -481,6 +487,7 @@ void octopic_destroy(octopic_handle *h) {
         for (int c = 0; c < 6; ++c) { h->beam_in[b][c].release(); h->st[b][c].release(); h->srt[b][c].release(); }
         h->perm[b].release(); h->perm_srt[b].release(); h->inv_srt[b].release(); h->sid[b].release(); h->hist[b].release();
     }
+    h->rank_keys.release(); h->rank_sorted.release(); h->rank_vals.release(); h->rank_temp.release();
     h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
     h->keys.release(); h->chunkbox.release(); h->box.release(); h->thist.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
     h->charge.release(); h->qh.release(); h->phi.release(); h->table.release(); h->records.release();
@@ -742,6 +749,10 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.st_z = h->st[b][4].p;
             a.perm = h->perm[b].p;
             a.err = h->errinfo.p;
+            if (a.method == OCTOPIC_SLICE_EQUAL_COUNT) {
+                a.rank_keys = h->rank_keys.p; a.rank_sorted = h->rank_sorted.p; a.rank_vals = h->rank_vals.p;
+                a.rank_temp = h->rank_temp.p; a.rank_temp_bytes = h->rank_temp.n;
+            }
             HIPCHECK(launch_slicing(a, s));
         }
     }

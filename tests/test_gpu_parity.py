@@ -203,7 +203,7 @@ def test_standard_green_cache_divergence_is_a_half_cell_realign(oracle, gpu):
         break
 
 
-@pytest.mark.parametrize("method", ["equal_area", "equal_width", "specified"])
+@pytest.mark.parametrize("method", ["equal_area", "equal_width", "specified", "equal_count"])
 def test_slicing_methods_parity(oracle, gpu, method):
     e, p = round_beams(oracle, 12000, 6000, seed=4)
     kw = dict(nslices=4, method=method, center_position="centroid")
@@ -213,6 +213,24 @@ def test_slicing_methods_parity(oracle, gpu, method):
         kw["center_position"] = "midpoint"
     sl = LongitudinalSlicing(**kw)
     compare_collide(oracle, gpu, PICPoissonSolver(grid=(32, 32), slicing=sl), e, p)
+
+
+def test_equal_count_ties_follow_rank_order(oracle, gpu):
+    """:equal_count (slicing.jl:330-365) splits by rank, not by boundary comparison: with z quantised to
+    a few values (ties everywhere, and a -0.0 / +0.0 pair) the populations stay equal and each slice's
+    members are listed in stable rank order (ascending index within ties), bit-identical to the
+    oracle's stable sortperm; the whole collide then matches."""
+    e, p = round_beams(oracle, 9001, 4003, seed=11)
+    e[4] = np.round(e[4] / 0.004) * 0.004
+    e[4, 5], e[4, 6] = -0.0, 0.0
+    p[4] = np.round(p[4] / 0.02) * 0.02
+    sl = LongitudinalSlicing(nslices=6, method="equal_count", center_position="centroid")
+    gcol = compare_collide(oracle, gpu, PICPoissonSolver(grid=(32, 32), slicing=sl, deposit_method="TSC"), e, p)
+    b, c, w, cnt, idx = gcol.last_slices(1, e.shape[1])
+    assert np.all(np.abs(np.diff(cnt)) <= 1)
+    # Julia's isless order: -0.0 before +0.0, then ascending index within ties
+    order = np.lexsort((np.arange(e.shape[1]), ~np.signbit(e[4]), e[4]))
+    assert np.array_equal(np.asarray(idx), order)
 
 
 def test_empty_slices_skip_their_pairs(oracle, gpu):
@@ -353,7 +371,6 @@ def test_degenerate_extent(oracle, gpu):
 def test_unsupported_combinations_are_rejected(gpu):
     for kw in (dict(interaction_grid="node", slice_interpolation="quadratic"), dict(grid_extent="sigma"),
                dict(interaction_grid="source_slice"), dict(green_type="lattice"),
-               dict(slicing=LongitudinalSlicing(nslices=3, method="equal_count")),
                dict(grid=(70, 70)), dict(grid=(576, 576))):   # 2*70 has a factor 7; 576 > 512 nodes
         with pytest.raises(UnsupportedOptionError):
             PICCollider(PICPoissonSolver(**kw))
