@@ -509,6 +509,7 @@ hipError_t launch_deposit(DepositArgs a, hipStream_t s) {
 // ---------------------------------------------------------------------------------- kick
 constexpr int kKickThreads = 512;
 constexpr int kKickLdsBytes = 72 * 1024;   // LDS window of the field table per block (2 blocks per CU)
+constexpr int kKickBlocksPerCU = 2;
 
 // LDS window layout: each node's C values padded to a whole number of 16-byte vectors (double2 /
 // float4), rows of wny nodes plus one vector, so a node is read with 16-byte LDS loads and vertically
@@ -623,16 +624,9 @@ __device__ __forceinline__ void gather_field_lds(const R *e0, int rs, int CL, co
 // particle whose stencil leaves the window reads the table in HBM.
 // R = double (FP64 mode) or float (FP32 mode: table, weights and gather sums in FP32; the kick is
 // applied to the FP64 coordinates).
+// One work item: part `sub` of chunk `chunk` of interaction d (called by every thread of the block).
 template <int METHOD, int MODE, class R>
-__global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
-    extern __shared__ __align__(16) unsigned char win_raw[];
-    R *win = reinterpret_cast<R *>(win_raw);
-    if (failed(a.err)) return;
-    const int lblk = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int per_inter = a.chunks * a.subdiv;
-    if (lblk >= per_inter * a.ninters) return;
-    const int d = lblk / per_inter, rem = lblk - d * per_inter;
-    const int chunk = rem / a.subdiv, sub = rem - chunk * a.subdiv;
+__device__ __forceinline__ void kick_item(const KickArgs &a, R *win, int d, int chunk, int sub) {
     const Inter it = a.inters[d];
     if (!it.valid) return;
     const int64_t n = it.fld_n;
@@ -772,10 +766,39 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     }
 }
 
+// Items are (interaction, chunk, part).  A grid of kKickBlocksPerCU blocks per CU takes items from a
+// per-launch atomic counter until it passes the item count (every block reaches that exit), in the
+// launch order of the interactions (larger field slices first), so the blocks finish together.
+// In the slice-pair modes the register budget is held to 4 waves per SIMD (two blocks per CU, as the
+// LDS window allows; a few values spill); the node mode keeps its registers (its spills would cost more).
+template <int METHOD, int MODE, class R>
+__global__ __launch_bounds__(kKickThreads, (MODE == 2 ? 1 : 4)) void k_kick(KickArgs a) {
+    extern __shared__ __align__(16) unsigned char win_raw[];
+    R *win = reinterpret_cast<R *>(win_raw);
+    if (failed(a.err)) return;
+    const int per_inter = a.chunks * a.subdiv, total = per_inter * a.ninters;
+    __shared__ int next_item;
+    for (;;) {
+        if (threadIdx.x == 0) next_item = atomicAdd(a.queue, 1);
+        __syncthreads();
+        // uniform across the block: readfirstlane keeps the item's descriptor loads scalar
+        const int item = __builtin_amdgcn_readfirstlane(next_item);
+        __syncthreads();   // every thread has read next_item before thread 0 takes the next one
+        if (item >= total) break;
+        const int dl = item / per_inter, rem = item - dl * per_inter;
+        kick_item<METHOD, MODE, R>(a, win, a.order[dl], rem / a.subdiv, rem % a.subdiv);
+        __syncthreads();   // the item's LDS window and pass boxes are free again
+    }
+}
+
 hipError_t launch_kick(const KickArgs &a, hipStream_t s) {
     if (a.ninters <= 0) return hipSuccess;
-    if (a.subdiv < 1) return hipErrorInvalidValue;
-    dim3 g(xcd_padded(a.chunks * a.subdiv * a.ninters)), b(kKickThreads);
+    if (a.subdiv < 1 || !a.queue || !a.order) return hipErrorInvalidValue;
+    const int total = a.chunks * a.subdiv * a.ninters;
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    const dim3 g(std::min(total, cus * kKickBlocksPerCU)), b(kKickThreads);
     const size_t lds = (a.mode != 2 && a.chunkbox) ? kKickLdsBytes : 0;
 #define OCTOPIC_KICK(M, MODE)                                                                                    \
     do {                                                                                                         \

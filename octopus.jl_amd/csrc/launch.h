@@ -147,6 +147,8 @@ struct KickArgs {
     unsigned long long *dropped;
     const double *chunkbox;      // chunk boxes of the batch's bounds launch, or null (no LDS windows)
     const ErrorInfo *err;
+    int *queue;                  // work-queue counter of this launch (zero on entry)
+    const int32_t *order;        // launch order of the interactions (position -> interaction)
 };
 hipError_t launch_kick(const KickArgs &a, hipStream_t s);
 

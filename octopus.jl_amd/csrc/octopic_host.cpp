@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -228,6 +229,9 @@ struct octopic_handle {
     // fit (-1: in the FP64 form only, where they pay; 0 never; 1 such chunks; 2 every chunk).
     int kick_target_blocks = 1024;
     int kick_pass_windows = -1;
+    // Kick work queue: per-batch counters and the interactions' launch order (larger field slices first).
+    DBuf<int> kqueue;
+    DBuf<int32_t> korder;
     DBuf<double> srt[2][6];     // tile-ordered states (tiles > 0)
     DBuf<int64_t> perm_srt[2];
     DBuf<int32_t> inv_srt[2];   // inverse of perm_srt
@@ -489,6 +493,7 @@ void octopic_destroy(octopic_handle *h) {
     }
     h->rank_keys.release(); h->rank_sorted.release(); h->rank_vals.release(); h->rank_temp.release();
     h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
+    h->kqueue.release(); h->korder.release();
     h->keys.release(); h->chunkbox.release(); h->box.release(); h->thist.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
     h->charge.release(); h->qh.release(); h->phi.release(); h->table.release(); h->records.release();
     h->lum_parts.release(); h->lum_partial.release(); h->lum_arrivals.release(); h->nodes.release(); h->node_jobs.release(); h->node_keys.release(); h->ran_dev.release();
@@ -830,6 +835,25 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     h->pairs.ensure(std::max<size_t>(hp.size(), 1));
     if (!hp.empty())
         HIPCHECK(hipMemcpyAsync(h->pairs.p, hp.data(), sizeof(BatchPair) * hp.size(), hipMemcpyHostToDevice, s));
+    if (!hp.empty()) {
+        // per batch: interactions by field-slice population, largest first (stable)
+        std::vector<int32_t> ord(2 * hp.size());
+        for (size_t bi = 0; bi < batches.size(); ++bi) {
+            const int m = 2 * bcnt[bi], o0 = 2 * boff[bi];
+            auto fld_count = [&](int d) {   // d = 2k + dir - 1: dir 1 kicks beam 2's slice j, dir 2 beam 1's slice i
+                const BatchPair &pr = hp[boff[bi] + d / 2];
+                return (d & 1) ? T1.count[pr.i] : T2.count[pr.j];
+            };
+            std::vector<int32_t> idx(m);
+            std::iota(idx.begin(), idx.end(), 0);
+            std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return fld_count(x) > fld_count(y); });
+            std::copy(idx.begin(), idx.end(), ord.begin() + o0);
+        }
+        h->korder.ensure(ord.size());
+        HIPCHECK(hipMemcpyAsync(h->korder.p, ord.data(), sizeof(int32_t) * ord.size(), hipMemcpyHostToDevice, s));
+        h->kqueue.ensure(batches.size());
+        HIPCHECK(hipMemsetAsync(h->kqueue.p, 0, sizeof(int) * batches.size(), s));
+    }
     h->keys.ensure_zero((size_t)2 * W * kBoundSlots, s);
     h->chunkbox.ensure((size_t)2 * W * kMaxChunks * 8);
     h->inters.ensure_zero(2 * W, s);
@@ -1026,6 +1050,8 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
                 a.subdiv = sd < 1 ? 1 : (sd > 64 ? 64 : sd);
             }
             a.pass_windows = h->kick_pass_windows >= 0 ? h->kick_pass_windows : (h->fp32 ? 0 : 1);
+            a.queue = h->kqueue.p + bi;
+            a.order = h->korder.p + (size_t)2 * boff[bi];
             a.dropped = h->counters.p + 4;
             a.chunkbox = cbox;
             a.err = h->errinfo.p;
