@@ -139,7 +139,8 @@ def run_instances(args):
 def measured_traffic(arm, family, precision):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary (copy-kernel calibrated), when one exists for this workload, kernel and precision."""
-    kernel = {("quadratic", "kick", "fp64"): "k_kick<1, 1, double>"}.get((arm, family, precision))
+    kernel = {("quadratic", "kick", "fp64"): "k_kick<1, 1, double>",
+              ("quadratic", "deposit", "fp64"): "k_deposit<1, double>"}.get((arm, family, precision))
     path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILE[arm]) if arm in TRAFFIC_PROFILE else None
     if kernel is None or path is None or not os.path.exists(path):
         return None
