@@ -412,13 +412,12 @@ constexpr int kDepositThreads = 1024;
 // (larger meshes only) adds straight into the plane in HBM.  Integer sums are associative, so the
 // charge is bit-identical either way.
 // W = double, or float in the FP32 mode (the weight product in FP32, then the same fixed point).
+// One work item: chunk group `grp` of plane `plane` (called by every thread of the block).
 template <int METHOD, class W>
-__global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
-    extern __shared__ unsigned long long tile[];
-    if (failed(a.err)) return;
-    const Plane pl = a.planes[blockIdx.y];
-    // block b takes the chunks [b*group, (b+1)*group) of the batch's chunking
-    const int c0 = blockIdx.x * a.group, c1 = min(c0 + a.group, a.chunks);
+__device__ __forceinline__ void deposit_item(const DepositArgs &a, unsigned long long *tile, int plane, int grp) {
+    const Plane pl = a.planes[plane];
+    // the item takes the chunks [grp*group, (grp+1)*group) of the batch's chunking
+    const int c0 = grp * a.group, c1 = min(c0 + a.group, a.chunks);
     const int64_t lo = pl.n * c0 / a.chunks, hi = pl.n * c1 / a.chunks;
     if (lo >= hi) return;
     const int n1 = pl.n1, n2 = pl.n2;
@@ -439,7 +438,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
         for (int c = threadIdx.x; c < wcells; c += blockDim.x) tile[c] = 0ull;
         __syncthreads();
     }
-    unsigned long long *out = a.charge + (size_t)blockIdx.y * a.charge_stride;
+    unsigned long long *out = a.charge + (size_t)plane * a.charge_stride;
     const double *X = a.st.c[pl.beam][0] + pl.off, *PX = a.st.c[pl.beam][1] + pl.off, *Y = a.st.c[pl.beam][2] + pl.off,
                  *PY = a.st.c[pl.beam][3] + pl.off;
     const double s = pl.s, x0 = pl.x0, y0 = pl.y0, hxi = pl.hxi, hyi = pl.hyi, scale = a.scale;
@@ -487,15 +486,43 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
     }
 }
 
+// Items are (plane, chunk group).  Persistent blocks (as many as fit the CUs) take items from a
+// per-launch atomic counter, planes of the larger source slices first, until it passes the item count.
+template <int METHOD, class W>
+__global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
+    extern __shared__ unsigned long long tile[];
+    if (failed(a.err)) return;
+    const int ngroups = (a.chunks + a.group - 1) / a.group, total = ngroups * a.nplanes;
+    __shared__ int next_item;
+    for (;;) {
+        if (threadIdx.x == 0) next_item = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int item = next_item;
+        __syncthreads();   // every thread has read next_item before thread 0 takes the next one
+        if (item >= total) break;
+        const int pos = item / ngroups;
+        deposit_item<METHOD, W>(a, tile, a.order[pos], item - pos * ngroups);
+        __syncthreads();   // the item's LDS window is flushed before the next item zeroes it
+    }
+}
+
 hipError_t launch_deposit(DepositArgs a, hipStream_t s) {
     if (a.nplanes <= 0) return hipSuccess;
     a.lds_cells = a.ncells < kDepositMaxLdsCells ? a.ncells : kDepositMaxLdsCells;
     const size_t lds = (size_t)a.lds_cells * sizeof(unsigned long long);
-    dim3 g((a.chunks + a.group - 1) / a.group, a.nplanes);
+    if (!a.queue || !a.order) return hipErrorInvalidValue;
+    const int total = (a.chunks + a.group - 1) / a.group * a.nplanes;
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
 #define OCTOPIC_DEPOSIT(M, W)                                                                                  \
     do {                                                                                                       \
         hipFuncSetAttribute((const void *)k_deposit<M, W>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
-        hipLaunchKernelGGL((k_deposit<M, W>), g, dim3(kDepositThreads), lds, s, a);                            \
+        int per_cu = 0;                                                                                        \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_deposit<M, W>, kDepositThreads, lds) !=      \
+                hipSuccess || per_cu < 1)                                                                      \
+            per_cu = 1;                                                                                        \
+        hipLaunchKernelGGL((k_deposit<M, W>), dim3(std::min(total, cus * per_cu)), dim3(kDepositThreads), lds, s, a); \
     } while (0)
     if (a.method == 0) {
         if (a.fp32) OCTOPIC_DEPOSIT(0, float); else OCTOPIC_DEPOSIT(0, double);

@@ -129,8 +129,10 @@ struct DepositArgs {
     const double *chunkbox;      // chunk boxes of the batch's bounds launch, or null (whole mesh windows)
     const ErrorInfo *err;
     int lds_cells;               // set by the launcher: LDS window capacity in cells
+    int *queue;                  // work-queue counter of this launch (zero on entry)
+    const int32_t *order;        // launch order of the planes (position -> plane)
 };
-constexpr int kDepositMaxLdsCells = 20480;   // 160 KB of uint64 cells per block
+constexpr int kDepositMaxLdsCells = 20480 - 2;   // 160 KB per block, less 16 bytes of the block's own LDS
 hipError_t launch_deposit(DepositArgs a, hipStream_t s);
 
 struct KickArgs {

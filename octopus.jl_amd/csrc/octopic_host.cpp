@@ -229,9 +229,11 @@ struct octopic_handle {
     // fit (-1: in the FP64 form only, where they pay; 0 never; 1 such chunks; 2 every chunk).
     int kick_target_blocks = 1024;
     int kick_pass_windows = -1;
-    // Kick work queue: per-batch counters and the interactions' launch order (larger field slices first).
+    // Kick and deposit work queues: per-launch counters and launch orders (larger slices first).
     DBuf<int> kqueue;
     DBuf<int32_t> korder;
+    DBuf<int> dqueue;            // [batch][field planes, luminosity planes]
+    DBuf<int32_t> dorder;        // per batch: field planes, then luminosity planes
     DBuf<double> srt[2][6];     // tile-ordered states (tiles > 0)
     DBuf<int64_t> perm_srt[2];
     DBuf<int32_t> inv_srt[2];   // inverse of perm_srt
@@ -493,7 +495,7 @@ void octopic_destroy(octopic_handle *h) {
     }
     h->rank_keys.release(); h->rank_sorted.release(); h->rank_vals.release(); h->rank_temp.release();
     h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
-    h->kqueue.release(); h->korder.release();
+    h->kqueue.release(); h->korder.release(); h->dqueue.release(); h->dorder.release();
     h->keys.release(); h->chunkbox.release(); h->box.release(); h->thist.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
     h->charge.release(); h->qh.release(); h->phi.release(); h->table.release(); h->records.release();
     h->lum_parts.release(); h->lum_partial.release(); h->lum_arrivals.release(); h->nodes.release(); h->node_jobs.release(); h->node_keys.release(); h->ran_dev.release();
@@ -853,6 +855,34 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         HIPCHECK(hipMemcpyAsync(h->korder.p, ord.data(), sizeof(int32_t) * ord.size(), hipMemcpyHostToDevice, s));
         h->kqueue.ensure(batches.size());
         HIPCHECK(hipMemsetAsync(h->kqueue.p, 0, sizeof(int) * batches.size(), s));
+        // deposit planes by source-slice population, largest first (stable): per batch the field planes
+        // (interaction d = 2k + dir - 1 deposits beam 1's slice i for dir 1, beam 2's slice j for dir 2),
+        // then the luminosity planes (2k + side)
+        std::vector<int32_t> dord((size_t)(h->npl + 1) * 2 * hp.size());
+        for (size_t bi = 0; bi < batches.size(); ++bi) {
+            const int B = bcnt[bi];
+            const size_t o0 = (size_t)(h->npl + 1) * 2 * boff[bi];
+            auto src_count = [&](int beam_side, int k) {
+                const BatchPair &pr = hp[boff[bi] + k];
+                return beam_side == 0 ? T1.count[pr.i] : T2.count[pr.j];
+            };
+            std::vector<int32_t> f(2 * B * h->npl), l(2 * B);
+            std::iota(f.begin(), f.end(), 0);
+            std::iota(l.begin(), l.end(), 0);
+            std::stable_sort(f.begin(), f.end(), [&](int32_t x, int32_t y) {
+                const int dx = x / h->npl, dy = y / h->npl;
+                return src_count(dx & 1, dx / 2) > src_count(dy & 1, dy / 2);
+            });
+            std::stable_sort(l.begin(), l.end(), [&](int32_t x, int32_t y) {
+                return src_count(x & 1, x / 2) > src_count(y & 1, y / 2);
+            });
+            std::copy(f.begin(), f.end(), dord.begin() + o0);
+            std::copy(l.begin(), l.end(), dord.begin() + o0 + f.size());
+        }
+        h->dorder.ensure(dord.size());
+        HIPCHECK(hipMemcpyAsync(h->dorder.p, dord.data(), sizeof(int32_t) * dord.size(), hipMemcpyHostToDevice, s));
+        h->dqueue.ensure(2 * batches.size());
+        HIPCHECK(hipMemsetAsync(h->dqueue.p, 0, sizeof(int) * 2 * batches.size(), s));
     }
     h->keys.ensure_zero((size_t)2 * W * kBoundSlots, s);
     h->chunkbox.ensure((size_t)2 * W * kMaxChunks * 8);
@@ -1001,12 +1031,16 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.scale = scale; a.charge = h->charge.p; a.charge_stride = charge_stride;
             a.dropped = h->counters.p + 4; a.chunkbox = cbox; a.err = h->errinfo.p;
             a.fp32 = h->fp32 ? 1 : 0;
+            a.queue = h->dqueue.p + 2 * bi;
+            a.order = h->dorder.p + (size_t)(npl + 1) * 2 * boff[bi];
             HIPCHECK(launch_deposit(a, s));
             if (compute_lum) {
                 DepositArgs l = a;
                 l.fp32 = 0;   // luminosity deposits stay FP64
                 l.planes = h->planes.p + nfp; l.nplanes = 2 * B; l.method = h->lum_dep; l.ncells = (int)lum_cells;
                 l.charge = h->charge.p + (size_t)nfp * charge_stride;
+                l.queue = h->dqueue.p + 2 * bi + 1;
+                l.order = a.order + nfp;
                 HIPCHECK(launch_deposit(l, s));
             }
         }
