@@ -646,12 +646,11 @@ __device__ __forceinline__ void gather_field_lds(const R *e0, int rs, int CL, co
     }
 }
 
-// One block takes one chunk of an interaction's field slice.  In the slice-pair modes the table cells
-// of the chunk's stencil window (from the chunk's field box) are staged in LDS when they fit; a
-// particle whose stencil leaves the window reads the table in HBM.
-// R = double (FP64 mode) or float (FP32 mode: table, weights and gather sums in FP32; the kick is
-// applied to the FP64 coordinates).
-// One work item: part `sub` of chunk `chunk` of interaction d (called by every thread of the block).
+// One work item: part `sub` of chunk `chunk` of interaction d's field slice (called by every thread
+// of the block).  In the slice-pair modes the table cells of the chunk's stencil window (from the
+// chunk's field box) are staged in LDS when they fit; a particle whose stencil leaves the window reads
+// the table in HBM.  R = double (FP64 mode) or float (FP32 mode: table, weights and gather sums in
+// FP32; the kick is applied to the FP64 coordinates).
 template <int METHOD, int MODE, class R>
 __device__ __forceinline__ void kick_item(const KickArgs &a, R *win, int d, int chunk, int sub) {
     const Inter it = a.inters[d];
