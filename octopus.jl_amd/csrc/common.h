@@ -125,14 +125,6 @@ __device__ __forceinline__ double bound_value(unsigned long long k, bool is_min)
     return key_value(is_min ? ~k : k);
 }
 
-// Workgroups of a launch are dispatched round-robin over the 8 XCDs (each with its own L2).  A 1-D
-// grid padded to a multiple of 8 and read through this map gives every XCD one contiguous range of
-// logical block ids, so the blocks that share a table (one interaction) share an L2.
-constexpr int kXCDs = 8;
-__host__ __device__ __forceinline__ int xcd_padded(int nblocks) { return (nblocks + kXCDs - 1) / kXCDs * kXCDs; }
-__device__ __forceinline__ int xcd_logical_block(int dispatched, int padded) {
-    return (dispatched % kXCDs) * (padded / kXCDs) + dispatched / kXCDs;
-}
 
 // ------------------------------------------------------------------ descriptors in HBM
 struct Grid { double x0, y0, width, height; };
