@@ -136,15 +136,65 @@ __global__ __launch_bounds__(kFFTThreads) void k_green_cols(const GreenJob *jobs
     for (int y = lane; y < P2; y += kFFTLanes) col[y] = Z[y];
 }
 
+// Luminosity overlap (pic_cpu.jl:2731-2735), one of kLumBlocks blocks of pair k: the block sums q1.q2
+// over a fixed cell range of the pair's two luminosity planes (resetting them to zero for the next
+// batch); the block that finishes last (per-pair arrival counter, reset by that block) adds the
+// kLumBlocks partials in block order, so the result does not depend on scheduling.  Called by all
+// kFFTThreads threads of a block.
+__device__ __forceinline__ void lum_block(const LumArgs &a, int k, int b) {
+    unsigned long long *q1 = a.charge + (size_t)(2 * k) * a.charge_stride;
+    unsigned long long *q2 = a.charge + (size_t)(2 * k + 1) * a.charge_stride;
+    const double inv = a.inv_scale;
+    const int c0 = (int)((int64_t)a.ncells * b / kLumBlocks), c1 = (int)((int64_t)a.ncells * (b + 1) / kLumBlocks);
+    double acc = 0.0;
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        const unsigned long long u1 = q1[c], u2 = q2[c];
+        if (u1) q1[c] = 0ull;
+        if (u2) q2[c] = 0ull;
+        acc += ((double)u1 * inv) * ((double)u2 * inv);
+    }
+    __shared__ double red[kFFTThreads];
+    __shared__ int last;
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kFFTThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.lum_partial[k * kLumBlocks + b] = red[0];
+        __threadfence();
+        last = atomicAdd(&a.lum_arrivals[k], 1u) == kLumBlocks - 1;
+    }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    const volatile double *part = a.lum_partial + k * kLumBlocks;
+    double sum = 0.0;
+    for (int q = 0; q < kLumBlocks; ++q) sum += part[q];
+    a.lum_arrivals[k] = 0u;
+    if (!failed(a.err)) {
+        const double *g = a.lumgeo + 3 * k;
+        a.lum_parts[a.pairs[k].p] = sum * g[2] * g[0] * g[1];
+    }
+}
+
 // Charge rows: int64 fixed-point rows (ya, ya+1) of plane -> x-half-spectrum Qh[plane][kx][y], y < ny.
-// Each charge cell is read once and reset to zero for the next use of the buffer.
+// Each charge cell is read once and reset to zero for the next use of the buffer.  Block rows past the
+// planes take the batch's luminosity overlap sums (lum.npairs pairs, kLumBlocks blocks each), which
+// read only the luminosity planes the deposit has finished.
 template <class C>
 __global__ __launch_bounds__(kFFTThreads) void k_rows_fwd(const Plane *planes, unsigned long long *charge, int64_t charge_stride,
                                                           C *qh, int64_t qh_stride, int nx, int ny, double inv_scale,
-                                                          FFTPlanDev px, const ErrorInfo *err) {
+                                                          FFTPlanDev px, const ErrorInfo *err, int nplanes, LumArgs lum) {
     using R = real_of<C>;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *lds = reinterpret_cast<C *>(lds_raw);
+    if ((int)blockIdx.y >= nplanes) {
+        const int t = ((int)blockIdx.y - nplanes) * (int)gridDim.x + (int)blockIdx.x;
+        if (t < lum.npairs * kLumBlocks) lum_block(lum, t / kLumBlocks, t % kLumBlocks);
+        return;
+    }
     if (failed(err)) return;
     const int plane = blockIdx.y;
     const int P1 = 2 * nx;
@@ -374,13 +424,15 @@ hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, in
 template <class C>
 hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long *charge, int64_t charge_stride, C *qh,
                            int64_t qh_stride, int nx, int ny, double inv_scale, const FFTPlanDev &px, const ErrorInfo *err,
-                           hipStream_t s) {
-    if (nplanes <= 0) return hipSuccess;
+                           const LumArgs &lum, hipStream_t s) {
+    if (nplanes <= 0 && lum.npairs <= 0) return hipSuccess;
     const size_t lds = fft_lds_bytes(2 * nx) / sizeof(double2) * sizeof(C);
     ensure_lds((const void *)k_rows_fwd<C>, lds);
-    dim3 g((ny + 2 * kFFTPerBlock - 1) / (2 * kFFTPerBlock), nplanes);
+    const int gx = (ny + 2 * kFFTPerBlock - 1) / (2 * kFFTPerBlock);
+    const int lum_rows = (lum.npairs * kLumBlocks + gx - 1) / gx;
+    dim3 g(gx, nplanes + lum_rows);
     hipLaunchKernelGGL(k_rows_fwd<C>, g, dim3(kFFTThreads), lds, s, planes, charge, charge_stride, qh, qh_stride, nx, ny,
-                       inv_scale, px, err);
+                       inv_scale, px, err, nplanes, lum);
     return hipGetLastError();
 }
 
@@ -427,7 +479,7 @@ hipError_t launch_green_to_f32(const GreenJob *jobs, int njobs, const double2 *s
 
 #define OCTOPIC_CONV_INSTANCES(C)                                                                                     \
     template hipError_t launch_rows_fwd<C>(const Plane *, int, unsigned long long *, int64_t, C *, int64_t, int, int,  \
-                                           double, const FFTPlanDev &, const ErrorInfo *, hipStream_t);                \
+                                           double, const FFTPlanDev &, const ErrorInfo *, const LumArgs &, hipStream_t); \
     template hipError_t launch_cols_conv<C>(const Plane *, int, C *, int64_t, const C *, int64_t, int, int,            \
                                             const FFTPlanDev &, const ErrorInfo *, hipStream_t);                       \
     template hipError_t launch_rows_inv<C>(const Plane *, int, const C *, int64_t, real_of<C> *, int, int,             \

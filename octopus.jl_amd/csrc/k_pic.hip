@@ -847,57 +847,6 @@ hipError_t launch_kick(const KickArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------------- luminosity
-// Grid (kLumBlocks, npairs): block b of pair k sums a fixed cell range; the block that finishes last
-// (per-pair arrival counter, reset by that block) adds the kLumBlocks partials in block order, so the
-// result does not depend on scheduling.
-constexpr int kLumThreads = 256;
-
-__global__ __launch_bounds__(kLumThreads) void k_lum(LumArgs a) {
-    const int k = blockIdx.y, b = blockIdx.x;
-    unsigned long long *q1 = a.charge + (size_t)(2 * k) * a.charge_stride;
-    unsigned long long *q2 = a.charge + (size_t)(2 * k + 1) * a.charge_stride;
-    const double inv = a.inv_scale;
-    const int c0 = (int)((int64_t)a.ncells * b / kLumBlocks), c1 = (int)((int64_t)a.ncells * (b + 1) / kLumBlocks);
-    double acc = 0.0;
-    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-        const unsigned long long u1 = q1[c], u2 = q2[c];
-        if (u1) q1[c] = 0ull;
-        if (u2) q2[c] = 0ull;
-        acc += ((double)u1 * inv) * ((double)u2 * inv);
-    }
-    __shared__ double red[kLumThreads];
-    __shared__ int last;
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = kLumThreads / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        a.lum_partial[k * kLumBlocks + b] = red[0];
-        __threadfence();
-        last = atomicAdd(&a.lum_arrivals[k], 1u) == kLumBlocks - 1;
-    }
-    __syncthreads();
-    if (!last || threadIdx.x != 0) return;
-    __threadfence();
-    const volatile double *part = a.lum_partial + k * kLumBlocks;
-    double sum = 0.0;
-    for (int q = 0; q < kLumBlocks; ++q) sum += part[q];
-    a.lum_arrivals[k] = 0u;
-    if (!failed(a.err)) {
-        const double *g = a.lumgeo + 3 * k;
-        a.lum_parts[a.pairs[k].p] = sum * g[2] * g[0] * g[1];
-    }
-}
-
-hipError_t launch_lum(const LumArgs &a, hipStream_t s) {
-    if (a.npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_lum, dim3(kLumBlocks, a.npairs), dim3(kLumThreads), 0, s, a);
-    return hipGetLastError();
-}
-
 }  // namespace octopic
 
 // ==================================================================================== :node meshes

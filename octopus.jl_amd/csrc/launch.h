@@ -167,8 +167,7 @@ struct LumArgs {
     unsigned int *lum_arrivals;  // [npairs], zero between launches
     const ErrorInfo *err;
 };
-constexpr int kLumBlocks = 16;
-hipError_t launch_lum(const LumArgs &a, hipStream_t s);
+constexpr int kLumBlocks = 16;   // blocks per pair of the luminosity overlap sum (in the charge row launch)
 
 // FFT stages (k_fft.hip)
 hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride, int nslots, int nx, int ny,
@@ -177,7 +176,7 @@ hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, in
 template <class C>   // C = double2 (FP64 mode) or float2 (FP32 mode)
 hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long *charge, int64_t charge_stride, C *qh,
                            int64_t qh_stride, int nx, int ny, double inv_scale, const FFTPlanDev &px, const ErrorInfo *err,
-                           hipStream_t s);
+                           const LumArgs &lum, hipStream_t s);   // lum.npairs = 0: no luminosity sums
 template <class C>
 hipError_t launch_cols_conv(const Plane *planes, int nplanes, C *qh, int64_t qh_stride, const C *gspec, int64_t slot_stride,
                             int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, hipStream_t s);

@@ -1045,11 +1045,19 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             }
         }
         const int64_t qh_stride = (int64_t)(nx + 1) * ny;
+        // the batch's luminosity overlap sums run as extra blocks of the charge row transform
+        LumArgs lum{};
+        if (compute_lum) {
+            lum.pairs = dp; lum.npairs = B; lum.charge = h->charge.p + (size_t)nfp * charge_stride;
+            lum.charge_stride = charge_stride; lum.ncells = (int)lum_cells; lum.inv_scale = inv_scale;
+            lum.lumgeo = h->lumgeo.p; lum.lum_parts = h->lum_parts.p; lum.lum_partial = h->lum_partial.p;
+            lum.lum_arrivals = h->lum_arrivals.p; lum.err = h->errinfo.p;
+        }
         auto solve = [&](auto *qh, const auto *spec, auto *phi, auto *table) {
             {
                 Timed t(h, F_ROWS_FWD);
                 HIPCHECK(launch_rows_fwd(h->planes.p, nfp, h->charge.p, charge_stride, qh, qh_stride, nx, ny, inv_scale,
-                                         h->px.dev, h->errinfo.p, s));
+                                         h->px.dev, h->errinfo.p, lum, s));
             }
             {
                 Timed t(h, F_COLS);
@@ -1090,15 +1098,6 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.chunkbox = cbox;
             a.err = h->errinfo.p;
             HIPCHECK(launch_kick(a, s));
-        }
-        if (compute_lum) {
-            Timed t(h, F_LUM);
-            LumArgs a{};
-            a.pairs = dp; a.npairs = B; a.charge = h->charge.p + (size_t)nfp * charge_stride; a.charge_stride = charge_stride;
-            a.ncells = (int)lum_cells; a.inv_scale = inv_scale; a.lumgeo = h->lumgeo.p; a.lum_parts = h->lum_parts.p;
-            a.lum_partial = h->lum_partial.p; a.lum_arrivals = h->lum_arrivals.p;
-            a.err = h->errinfo.p;
-            HIPCHECK(launch_lum(a, s));
         }
         planes_total += nfp;
     }
