@@ -421,6 +421,7 @@ __device__ __forceinline__ int tile_of(double x, double y, const TileBox &b, int
 }
 
 constexpr int kTileThreads = 1024;
+constexpr int kTileRunsMaxTiles = 4096;   // up to 64 x 64 tiles: the run-sorted scatter (LDS, 16-bit ids)
 
 // grid (G, ns): per-block tile counts, thist[(s * TT + tile) * G + c]
 __global__ __launch_bounds__(kTileThreads) void k_tile_hist(TileSortArgs a) {
@@ -489,14 +490,111 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_scatter(TileSortArgs a) {
         }
 }
 
+// Exclusive scan of v over the block (kTileThreads threads, one value each); returns the block total.
+__device__ __forceinline__ int block_exclusive_scan(int v, int *wsum, int &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int w = 0; w < kTileThreads / 64; ++w) { const int t = wsum[w]; wsum[w] = run; run += t; }
+        wsum[kTileThreads / 64] = run;
+    }
+    __syncthreads();
+    total = wsum[kTileThreads / 64];
+    return wsum[wave] + incl - v;
+}
+
+// The same slots as k_tile_scatter, written in runs: each visit of kTileChunk members is first sorted
+// by tile in LDS (local counting sort), then every tile's run of the visit goes to its consecutive
+// slots, so the coordinate writes coalesce instead of landing one 8-byte word per cache line.
+// Dynamic LDS: cur, count, offset [TT] ints, then the visit's tile and member index [kTileChunk] ushort.
+__global__ __launch_bounds__(kTileThreads) void k_tile_scatter_runs(TileSortArgs a) {
+    extern __shared__ int smem[];
+    const int s = blockIdx.y, c = blockIdx.x, T = a.T, TT = T * T, G = gridDim.x;
+    int *cur = smem, *lcnt = smem + TT, *loff = smem + 2 * TT;
+    unsigned short *ltile = reinterpret_cast<unsigned short *>(smem + 3 * TT), *lidx = ltile + kTileChunk;
+    __shared__ int wsum[kTileThreads / 64 + 1];
+    constexpr int kPer = kTileChunk / kTileThreads;
+    const TileBox tb = tile_box(a.box, T);
+    for (int t = threadIdx.x; t < TT; t += blockDim.x) cur[t] = a.thist[((size_t)s * TT + t) * G + c];
+    const int64_t off = a.tab->offset[s], n = a.tab->count[s];
+    for (int64_t base = (int64_t)c * kTileChunk; base < n; base += (int64_t)G * kTileChunk) {
+        const int cnt = (int)(n - base < kTileChunk ? n - base : kTileChunk);
+        for (int t = threadIdx.x; t < TT; t += blockDim.x) lcnt[t] = 0;
+        __syncthreads();
+        int rt[kPer], rr[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int j = q * kTileThreads + threadIdx.x;
+            rt[q] = -1;
+            if (j < cnt) {
+                const int64_t i = off + base + j;
+                rt[q] = tile_of(a.src[0][i], a.src[2][i], tb, T);
+                rr[q] = atomicAdd(&lcnt[rt[q]], 1);
+            }
+        }
+        __syncthreads();
+        // exclusive scan of the visit's tile counts, TT / kTileThreads consecutive tiles per thread
+        {
+            const int per = (TT + kTileThreads - 1) / kTileThreads;
+            const int t0 = threadIdx.x * per;
+            int mine = 0;
+            for (int t = t0; t < t0 + per && t < TT; ++t) mine += lcnt[t];
+            int total;
+            int run = block_exclusive_scan(mine, wsum, total);
+            for (int t = t0; t < t0 + per && t < TT; ++t) { loff[t] = run; run += lcnt[t]; }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPer; ++q)
+            if (rt[q] >= 0) {
+                const int lp = loff[rt[q]] + rr[q];
+                ltile[lp] = (unsigned short)rt[q];
+                lidx[lp] = (unsigned short)(q * kTileThreads + threadIdx.x);
+            }
+        __syncthreads();
+        for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
+            const int t = ltile[p];
+            const int64_t pos = (int64_t)cur[t] + (p - loff[t]);
+            const int64_t i = off + base + lidx[p];
+            a.dst[0][pos] = a.src[0][i];
+            a.dst[1][pos] = a.src[1][i];
+            a.dst[2][pos] = a.src[2][i];
+            a.dst[3][pos] = a.src[3][i];
+            a.dst[4][pos] = a.src[4][i];
+            a.dst[5][pos] = a.src[5][i];
+            const int64_t orig = a.perm_src[i];
+            a.perm_dst[pos] = orig;
+            a.inv[orig] = (int32_t)pos;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < TT; t += blockDim.x) cur[t] += lcnt[t];
+        __syncthreads();
+    }
+}
+
 hipError_t launch_tile_sort(const TileSortArgs &a, int G, hipStream_t s) {
     if (a.ns <= 0 || a.T <= 0 || G <= 0) return hipSuccess;
-    const size_t lds = (size_t)a.T * a.T * sizeof(int);
+    const int TT = a.T * a.T;
+    const size_t lds = (size_t)TT * sizeof(int);
     hipFuncSetAttribute((const void *)k_tile_hist, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipFuncSetAttribute((const void *)k_tile_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_tile_hist, dim3(G, a.ns), dim3(kTileThreads), lds, s, a);
     hipLaunchKernelGGL(k_tile_scan, dim3(a.ns), dim3(1024), 0, s, a, G);
-    hipLaunchKernelGGL(k_tile_scatter, dim3(G, a.ns), dim3(kTileThreads), lds, s, a);
+    if (TT <= kTileRunsMaxTiles) {   // tiles fit the LDS sort of a visit (16-bit tile and member ids)
+        const size_t lds_runs = (size_t)3 * TT * sizeof(int) + (size_t)2 * kTileChunk * sizeof(unsigned short);
+        hipFuncSetAttribute((const void *)k_tile_scatter_runs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_runs);
+        hipLaunchKernelGGL(k_tile_scatter_runs, dim3(G, a.ns), dim3(kTileThreads), lds_runs, s, a);
+    } else {
+        hipFuncSetAttribute((const void *)k_tile_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_tile_scatter, dim3(G, a.ns), dim3(kTileThreads), lds, s, a);
+    }
     return hipGetLastError();
 }
 
