@@ -512,14 +512,17 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *wsum, int &total
 }
 
 // The same slots as k_tile_scatter, written in runs: each visit of kTileChunk members is first sorted
-// by tile in LDS (local counting sort), then every tile's run of the visit goes to its consecutive
-// slots, so the coordinate writes coalesce instead of landing one 8-byte word per cache line.
-// Dynamic LDS: cur, count, offset [TT] ints, then the visit's tile and member index [kTileChunk] ushort.
+// by tile in LDS (local counting sort); then every coordinate array is read in member order
+// (coalesced), placed at its sorted position in an LDS stage, and written back as the visit's tile
+// runs to their consecutive slots, so both the reads and the writes coalesce.
+// Dynamic LDS: cur, count, offset [TT] ints, the stage [kTileChunk] doubles, the sorted tiles
+// [kTileChunk] ushort.
 __global__ __launch_bounds__(kTileThreads) void k_tile_scatter_runs(TileSortArgs a) {
     extern __shared__ int smem[];
     const int s = blockIdx.y, c = blockIdx.x, T = a.T, TT = T * T, G = gridDim.x;
     int *cur = smem, *lcnt = smem + TT, *loff = smem + 2 * TT;
-    unsigned short *ltile = reinterpret_cast<unsigned short *>(smem + 3 * TT), *lidx = ltile + kTileChunk;
+    double *stage = reinterpret_cast<double *>(smem + ((3 * TT + 1) & ~1));
+    unsigned short *ltile = reinterpret_cast<unsigned short *>(stage + kTileChunk);
     __shared__ int wsum[kTileThreads / 64 + 1];
     constexpr int kPer = kTileChunk / kTileThreads;
     const TileBox tb = tile_box(a.box, T);
@@ -529,7 +532,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_scatter_runs(TileSortArgs
         const int cnt = (int)(n - base < kTileChunk ? n - base : kTileChunk);
         for (int t = threadIdx.x; t < TT; t += blockDim.x) lcnt[t] = 0;
         __syncthreads();
-        int rt[kPer], rr[kPer];
+        int rt[kPer], lp[kPer];
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int j = q * kTileThreads + threadIdx.x;
@@ -537,7 +540,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_scatter_runs(TileSortArgs
             if (j < cnt) {
                 const int64_t i = off + base + j;
                 rt[q] = tile_of(a.src[0][i], a.src[2][i], tb, T);
-                rr[q] = atomicAdd(&lcnt[rt[q]], 1);
+                lp[q] = atomicAdd(&lcnt[rt[q]], 1);
             }
         }
         __syncthreads();
@@ -555,26 +558,29 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_scatter_runs(TileSortArgs
 #pragma unroll
         for (int q = 0; q < kPer; ++q)
             if (rt[q] >= 0) {
-                const int lp = loff[rt[q]] + rr[q];
-                ltile[lp] = (unsigned short)rt[q];
-                lidx[lp] = (unsigned short)(q * kTileThreads + threadIdx.x);
+                lp[q] += loff[rt[q]];
+                ltile[lp[q]] = (unsigned short)rt[q];
             }
-        __syncthreads();
-        for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
-            const int t = ltile[p];
-            const int64_t pos = (int64_t)cur[t] + (p - loff[t]);
-            const int64_t i = off + base + lidx[p];
-            a.dst[0][pos] = a.src[0][i];
-            a.dst[1][pos] = a.src[1][i];
-            a.dst[2][pos] = a.src[2][i];
-            a.dst[3][pos] = a.src[3][i];
-            a.dst[4][pos] = a.src[4][i];
-            a.dst[5][pos] = a.src[5][i];
-            const int64_t orig = a.perm_src[i];
-            a.perm_dst[pos] = orig;
-            a.inv[orig] = (int32_t)pos;
+        // arrays 0-5: coordinates; 6: the original indices (int64 through the same stage)
+        for (int arr = 0; arr < 7; ++arr) {
+            const double *src = arr < 6 ? a.src[arr] : reinterpret_cast<const double *>(a.perm_src);
+#pragma unroll
+            for (int q = 0; q < kPer; ++q)
+                if (rt[q] >= 0) stage[lp[q]] = src[off + base + q * kTileThreads + threadIdx.x];
+            __syncthreads();
+            for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
+                const int t = ltile[p];
+                const int64_t pos = (int64_t)cur[t] + (p - loff[t]);
+                if (arr < 6) {
+                    a.dst[arr][pos] = stage[p];
+                } else {
+                    const int64_t orig = reinterpret_cast<const int64_t *>(stage)[p];
+                    a.perm_dst[pos] = orig;
+                    a.inv[orig] = (int32_t)pos;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
         for (int t = threadIdx.x; t < TT; t += blockDim.x) cur[t] += lcnt[t];
         __syncthreads();
     }
@@ -588,7 +594,8 @@ hipError_t launch_tile_sort(const TileSortArgs &a, int G, hipStream_t s) {
     hipLaunchKernelGGL(k_tile_hist, dim3(G, a.ns), dim3(kTileThreads), lds, s, a);
     hipLaunchKernelGGL(k_tile_scan, dim3(a.ns), dim3(1024), 0, s, a, G);
     if (TT <= kTileRunsMaxTiles) {   // tiles fit the LDS sort of a visit (16-bit tile and member ids)
-        const size_t lds_runs = (size_t)3 * TT * sizeof(int) + (size_t)2 * kTileChunk * sizeof(unsigned short);
+        const size_t lds_runs = (size_t)((3 * TT + 1) & ~1) * sizeof(int) + (size_t)kTileChunk * sizeof(double) +
+                                (size_t)kTileChunk * sizeof(unsigned short);
         hipFuncSetAttribute((const void *)k_tile_scatter_runs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_runs);
         hipLaunchKernelGGL(k_tile_scatter_runs, dim3(G, a.ns), dim3(kTileThreads), lds_runs, s, a);
     } else {
