@@ -221,6 +221,55 @@ int octopic_set_timing_families(octopic_handle *h, const char *families);
 int octopic_kernel_times(const octopic_handle *h, double *ms_total, int64_t *launches, int64_t cap,
                          int64_t *n, char *names, int64_t names_cap);
 
+/* ---------------------------------------------------------------------------------------------
+ * Residual denoiser (no counterpart in the reference: the library's learned extension of the plain
+ * PIC path).  A three-level U-Net (3x3 convolutions with 16/32/64 channels, ReLU, 2x2 average
+ * pooling, nearest upsampling with skip concatenation, 1x1 head; 118129 parameters) that maps a
+ * Gaussian-subtracted residual charge plane to its converged form.  bf16 MFMA products with fp32
+ * accumulation in both the forward and the backward pass; fp32 parameters and Adam.
+ *
+ * Planes are NHWC: the input is [batch][height][width][2] (the residual and the shot-noise standard
+ * deviation, both divided by the subtracted Gaussian's peak node value), the output and the target
+ * [batch][height][width].  height and width: multiples of 16.  All plane pointers are device
+ * memory; `loss` is a device float (the mean squared error over the batch); `grad_out` a device
+ * float[num_params] or null.  Parameters are the flat float vector: per layer (e1a, e1b, e2a, e2b,
+ * ba, bb, d2a, d2b, d1a, d1b) W[cout][3][3][cin] then b[cout]; the head's w[16] and b.  Calls on one
+ * denoiser run in order on its stream (octopic_denoiser_set_stream; default: the null stream).
+ * --------------------------------------------------------------------------------------------- */
+typedef struct octopic_denoiser octopic_denoiser;
+
+int64_t octopic_denoiser_num_params(void);
+/* He-normal weights from `seed` (zero biases); batches of up to max_batch planes. */
+int octopic_denoiser_create(int32_t device, int32_t height, int32_t width, int32_t max_batch, uint64_t seed,
+                            octopic_denoiser **out);
+void octopic_denoiser_destroy(octopic_denoiser *d);
+const char *octopic_denoiser_last_error(const octopic_denoiser *d);
+int octopic_denoiser_set_stream(octopic_denoiser *d, void *hip_stream);
+/* Host float[n] copies of the parameters; set_params also resets the optimiser state. */
+int octopic_denoiser_get_params(const octopic_denoiser *d, float *params, int64_t n);
+int octopic_denoiser_set_params(octopic_denoiser *d, const float *params, int64_t n);
+int octopic_denoiser_forward(octopic_denoiser *d, const float *input, int32_t batch, float *output);
+/* Loss and gradient without an update. */
+int octopic_denoiser_gradient(octopic_denoiser *d, const float *input, const float *target, int32_t batch,
+                              float *loss, float *grad_out);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the mean squared error against target. */
+int octopic_denoiser_train_step(octopic_denoiser *d, const float *input, const float *target, int32_t batch,
+                                float lr, float *loss, float *grad_out);
+/* Training pairs generated by the method itself: for samples first_sample .. first_sample+batch-1, a
+ * two-component Gaussian-mixture slice (core with dipole and quadrupole modes, centred halo; the
+ * distribution of each sample follows from seed and its id) is drawn twice with Philox counter
+ * streams, n_noisy in [16384, 262144] and n_target macroparticles, and deposited (method 0 CIC,
+ * 1 TSC) on the height x width mesh in 2^-32 fixed point.  Each deposit has its own Gaussian (its
+ * particles' moments) subtracted by the neutralised erf profile (gaussian_pic.jl:206-244, 543-576).
+ * input: the noisy sample's network input; target: the converged residual at the noisy sample's
+ * particle count, over the same peak; monitor (optional, device float[batch]): rms of the input
+ * residual.  raw_planes ([batch][2][height*width] uint64) and raw_moments ([batch][2][5]: count,
+ * sum u, sum v, sum u^2, sum v^2, u/v in cells, sums in 2^-24 fixed point), optional device
+ * pointers, receive the deposits.  height * width <= 20480. */
+int octopic_denoiser_make_pairs(octopic_denoiser *d, uint64_t seed, int64_t first_sample, int32_t batch,
+                                int64_t n_target, int32_t method, float *input, float *target, float *monitor,
+                                uint64_t *raw_planes, uint64_t *raw_moments);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,7 @@ class Options(C.Structure):
 
 
 DPTR = C.POINTER(C.c_double)
+FPTR = C.POINTER(C.c_float)
 
 
 class BeamStruct(C.Structure):
@@ -178,6 +179,17 @@ def declare(lib):
             i32,
             [vp, DPTR, C.POINTER(i64), i64, C.POINTER(i64), C.c_char_p, i64],
         ),
+        "octopic_denoiser_num_params": (i64, []),
+        "octopic_denoiser_create": (i32, [i32, i32, i32, i32, C.c_uint64, C.POINTER(vp)]),
+        "octopic_denoiser_destroy": (None, [vp]),
+        "octopic_denoiser_last_error": (C.c_char_p, [vp]),
+        "octopic_denoiser_set_stream": (i32, [vp, vp]),
+        "octopic_denoiser_get_params": (i32, [vp, FPTR, i64]),
+        "octopic_denoiser_set_params": (i32, [vp, FPTR, i64]),
+        "octopic_denoiser_forward": (i32, [vp, vp, i32, vp]),
+        "octopic_denoiser_gradient": (i32, [vp, vp, vp, i32, vp, vp]),
+        "octopic_denoiser_train_step": (i32, [vp, vp, vp, i32, C.c_float, vp, vp]),
+        "octopic_denoiser_make_pairs": (i32, [vp, C.c_uint64, i64, i32, i64, i32, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -206,4 +218,15 @@ EXPORTED = [
     "octopic_set_timing",
     "octopic_set_timing_families",
     "octopic_kernel_times",
+    "octopic_denoiser_num_params",
+    "octopic_denoiser_create",
+    "octopic_denoiser_destroy",
+    "octopic_denoiser_last_error",
+    "octopic_denoiser_set_stream",
+    "octopic_denoiser_get_params",
+    "octopic_denoiser_set_params",
+    "octopic_denoiser_forward",
+    "octopic_denoiser_gradient",
+    "octopic_denoiser_train_step",
+    "octopic_denoiser_make_pairs",
 ]
