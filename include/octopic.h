@@ -122,6 +122,8 @@ typedef struct octopic_stats {
     int64_t green_builds;                               /* Green tables built in the last collide */
     int64_t planes;                                     /* field planes solved in the last collide */
     int64_t collides;
+    int64_t denoised_planes;                            /* last collide: field planes rebuilt by the denoiser */
+    int64_t denoise_fallbacks;                          /* last collide: field planes left to plain PIC */
 } octopic_stats;
 
 /* One directed interaction of the last collide, in execution order (pair position p in collision
@@ -266,6 +268,18 @@ int octopic_denoiser_train_step(octopic_denoiser *d, const float *input, const f
  * residual.  raw_planes ([batch][2][height*width] uint64) and raw_moments ([batch][2][5]: count,
  * sum u, sum v, sum u^2, sum v^2, u/v in cells, sums in 2^-24 fixed point), optional device
  * pointers, receive the deposits.  height * width <= 20480. */
+/* Attach a denoiser to a collision handle (NULL detaches; the handle does not own it).  Every field
+ * plane of every collide is then deposited as usual, its source slice's drifted Gaussian (the
+ * source's own moments, gaussian_pic.jl:374-491) subtracted with the neutralised erf profile
+ * (gaussian_pic.jl:206-244, 543-576), the residual passed through the network, and the plane
+ * rebuilt as Gaussian + denoised residual with its deposited total kept, before the convolution.
+ * A plane stays plain PIC when its source has fewer than two members or no positive width, or when
+ * its monitor (rms input residual) exceeds monitor_limit (the library's fallback: three times the
+ * largest monitor seen in training).  Needs interaction_grid = :slice_pair (else UNSUPPORTED), a
+ * denoiser of height = ny and width = nx, and max_batch >= the field planes of the widest batch
+ * (checked at collide).  The luminosity planes are not touched. */
+int octopic_set_denoiser(octopic_handle *h, octopic_denoiser *d, double monitor_limit);
+
 int octopic_denoiser_make_pairs(octopic_denoiser *d, uint64_t seed, int64_t first_sample, int32_t batch,
                                 int64_t n_target, int32_t method, float *input, float *target, float *monitor,
                                 uint64_t *raw_planes, uint64_t *raw_moments);

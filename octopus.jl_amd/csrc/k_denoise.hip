@@ -40,6 +40,7 @@ __device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even
     return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ bool failed(const ErrorInfo *err) { return *((volatile const int32_t *)&err->code) != 0; }
 
 constexpr int kConvThreads = 256;   // 4 waves, 64 pixels per block
 
@@ -584,6 +585,178 @@ __global__ __launch_bounds__(256) void k_pair_finish(PairArgs a, float *input, f
     }
 }
 
+// ------------------------------------------------------------------------- in the collision
+// Source moments of each interaction (gaussian_pic.jl:374-435, uncoupled): sums of the source
+// slice's x, px, y, py shifted by its first member, per chunk of members, in fixed order.
+__global__ __launch_bounds__(256) void k_dn_moments(DnCollideArgs a) {
+    __shared__ double red[10][256];
+    const int k = blockIdx.y, c = blockIdx.x;
+    const Inter &it = a.inters[k];
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (it.valid && it.src_n > 0 && !failed(a.err)) {
+        const double *X = a.st.c[it.src_beam][0] + it.src_off, *PX = a.st.c[it.src_beam][1] + it.src_off,
+                     *Y = a.st.c[it.src_beam][2] + it.src_off, *PY = a.st.c[it.src_beam][3] + it.src_off;
+        const double x0 = X[0], px0 = PX[0], y0 = Y[0], py0 = PY[0];
+        const int64_t b = it.src_n * c / a.chunks, e = it.src_n * (c + 1) / a.chunks;
+        for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+            const double dx = X[i] - x0, dpx = PX[i] - px0, dy = Y[i] - y0, dpy = PY[i] - py0;
+            acc[0] += dx; acc[1] += dpx; acc[2] += dy; acc[3] += dpy;
+            acc[4] += dx * dx; acc[5] += dpx * dpx; acc[6] += dy * dy; acc[7] += dpy * dpy;
+            acc[8] += dx * dpx; acc[9] += dy * dpy;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 10; ++j) red[j][threadIdx.x] = acc[j];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int j = 0; j < 10; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 10) a.mom_part[((int64_t)k * a.chunks + c) * 10 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// One block per field plane: the Gaussian of the source drifted by the plane's s
+// (gaussian_pic.jl:482-491) on the plane's mesh, its neutralised subtraction (gaussian_pic.jl:543-576)
+// and the network input (residual / peak, sqrt(expected count) / peak); the monitor is the rms of the
+// input residual. A plane is passed to the network only when the source has at least two members and
+// finite positive drifted widths (the reference's plain-PIC fallback, gaussian_pic.jl:525-530) and
+// its monitor is within the limit.
+__global__ __launch_bounds__(256) void k_dn_prepare(DnCollideArgs a) {
+    extern __shared__ double g[];   // gx [nx], gy [ny]
+    __shared__ double red[256];
+    __shared__ double par[8];
+    __shared__ double tot[5];
+    const int p = blockIdx.x, nx = a.nx, ny = a.ny, HW = nx * ny;
+    const Plane &pl = a.planes[p];
+    const Inter &it = a.inters[pl.inter];
+    if (threadIdx.x == 0) {
+        double sm[10];
+        for (int j = 0; j < 10; ++j) {
+            double t = 0.0;
+            for (int c = 0; c < a.chunks; ++c) t += a.mom_part[((int64_t)pl.inter * a.chunks + c) * 10 + j];
+            sm[j] = t;
+        }
+        const int64_t n = it.src_n;
+        int ok = it.valid && n >= 2 && !failed(a.err);
+        double mux = 0, muy = 0, sigx = 0, sigy = 0;
+        if (ok) {
+            const int b = it.src_beam;
+            const double x0 = a.st.c[b][0][it.src_off], px0 = a.st.c[b][1][it.src_off], y0 = a.st.c[b][2][it.src_off],
+                         py0 = a.st.c[b][3][it.src_off];
+            const double invn = 1.0 / (double)n;
+            const double dmx = sm[0] * invn, dmpx = sm[1] * invn, dmy = sm[2] * invn, dmpy = sm[3] * invn;
+            const double mx = x0 + dmx, mpx = px0 + dmpx, my = y0 + dmy, mpy = py0 + dmpy;
+            double varx = sm[4] * invn - dmx * dmx, varpx = sm[5] * invn - dmpx * dmpx;
+            double vary = sm[6] * invn - dmy * dmy, varpy = sm[7] * invn - dmpy * dmpy;
+            varx = varx > 0.0 ? varx : 0.0; varpx = varpx > 0.0 ? varpx : 0.0;
+            vary = vary > 0.0 ? vary : 0.0; varpy = varpy > 0.0 ? varpy : 0.0;
+            const double cxpx = sm[8] * invn - dmx * dmpx, cypy = sm[9] * invn - dmy * dmpy;
+            const double s = pl.s;
+            mux = mx + mpx * s;
+            muy = my + mpy * s;
+            const double vx = varx + 2.0 * s * cxpx + s * s * varpx, vy = vary + 2.0 * s * cypy + s * s * varpy;
+            sigx = sqrt(vx > 0.0 ? vx : 0.0);
+            sigy = sqrt(vy > 0.0 ? vy : 0.0);
+            ok = sigx > 0.0 && sigy > 0.0 && isfinite(sigx) && isfinite(sigy) && isfinite(mux) && isfinite(muy);
+        }
+        par[0] = ok;
+        par[1] = (mux - pl.x0) / pl.hx;
+        par[2] = (muy - pl.y0) / pl.hy;
+        par[3] = sigx / pl.hx;
+        par[4] = sigy / pl.hy;
+    }
+    __syncthreads();
+    const bool ok = par[0] != 0.0;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) g[i] = ok ? gauss_node_weight((double)i, 1.0, par[1], par[3], a.method) : 0.0;
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) g[nx + j] = ok ? gauss_node_weight((double)j, 1.0, par[2], par[4], a.method) : 0.0;
+    __syncthreads();
+    const unsigned long long *q = a.charge + (int64_t)p * a.charge_stride;
+    double acc[5] = {0, 0, 0, 0, 0};   // charge, sum gx, sum gy, max gx, max gy
+    for (int c = threadIdx.x; c < HW; c += blockDim.x) acc[0] += (double)(long long)q[c] * a.inv_scale;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) { acc[1] += g[i]; acc[3] = fmax(acc[3], g[i]); }
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) { acc[2] += g[nx + j]; acc[4] = fmax(acc[4], g[nx + j]); }
+    for (int v = 0; v < 5; ++v) {
+        red[threadIdx.x] = acc[v];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int j = 0; j < (int)blockDim.x; ++j) t = v >= 3 ? fmax(t, red[j]) : t + red[j];
+            tot[v] = t;
+        }
+        __syncthreads();
+    }
+    const double amp = tot[1] * tot[2] > 0.0 ? tot[0] / (tot[1] * tot[2]) : 0.0;
+    const double peak = amp * tot[3] * tot[4];
+    const double inv_peak = peak > 0.0 ? 1.0 / peak : 0.0;
+    double ss = 0.0;
+    float *in = a.input + (int64_t)p * HW * 2;
+    for (int c = threadIdx.x; c < HW; c += blockDim.x) {
+        const int j = c / nx, i = c - j * nx;
+        const double gg = amp * g[i] * g[nx + j];
+        const float r = (float)(((double)(long long)q[c] * a.inv_scale - gg) * inv_peak);
+        in[2 * c] = r;
+        in[2 * c + 1] = (float)(sqrt(gg > 0.0 ? gg : 0.0) * inv_peak);
+        ss += (double)r * (double)r;
+    }
+    red[threadIdx.x] = ss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int j = 0; j < (int)blockDim.x; ++j) t += red[j];
+        const double mon = sqrt(t / HW);
+        double *m = a.meta + (int64_t)p * 4;
+        m[0] = amp;
+        m[1] = peak;
+        m[2] = mon;
+        m[3] = (ok && peak > 0.0 && isfinite(mon) && mon <= a.limit) ? 1.0 : 0.0;
+    }
+    double *pr = a.prof + (int64_t)p * (nx + ny);
+    for (int c = threadIdx.x; c < nx + ny; c += blockDim.x) pr[c] = g[c];
+}
+
+// One block per field plane: where the plane is used, charge = amp gx gy + peak (out - mean(out)),
+// so the plane keeps its deposited total; elsewhere the plain deposit stays.
+__global__ __launch_bounds__(256) void k_dn_apply(DnCollideArgs a, const float *out) {
+    __shared__ double red[256];
+    const int p = blockIdx.x, nx = a.nx, ny = a.ny, HW = nx * ny;
+    const double *m = a.meta + (int64_t)p * 4;
+    const bool use = m[3] != 0.0;
+    if (threadIdx.x == 0) atomicAdd(&a.counters[use ? 0 : 1], 1ull);
+    if (!use) return;
+    const float *o = out + (int64_t)p * HW;
+    double t = 0.0;
+    for (int c = threadIdx.x; c < HW; c += blockDim.x) t += (double)o[c];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double mean = red[0] / HW, amp = m[0], peak = m[1];
+    const double *gx = a.prof + (int64_t)p * (nx + ny), *gy = gx + nx;
+    unsigned long long *q = a.charge + (int64_t)p * a.charge_stride;
+    for (int c = threadIdx.x; c < HW; c += blockDim.x) {
+        const int j = c / nx, i = c - j * nx;
+        const double v = amp * gx[i] * gy[j] + peak * ((double)o[c] - mean);
+        q[c] = (unsigned long long)llrint(v * a.scale);
+    }
+}
+
+hipError_t launch_dn_prepare(const DnCollideArgs &a, hipStream_t s) {
+    if (a.nplanes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dn_moments, dim3(a.chunks, a.ninters), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_dn_prepare, dim3(a.nplanes), dim3(256), (size_t)(a.nx + a.ny) * sizeof(double), s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_dn_apply(const DnCollideArgs &a, const float *out, hipStream_t s) {
+    if (a.nplanes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dn_apply, dim3(a.nplanes), dim3(256), 0, s, a, out);
+    return hipGetLastError();
+}
+
 // --------------------------------------------------------------------------------- the network
 namespace {
 
@@ -756,6 +929,23 @@ using octopic::DenoiserImpl;
 struct octopic_denoiser {
     DenoiserImpl impl;
 };
+
+namespace octopic {
+const float *denoiser_run(octopic_denoiser *d, const float *input, int batch, hipStream_t s, hipError_t *e) {
+    DenoiserImpl &m = d->impl;
+    hipStream_t keep = m.stream;
+    m.stream = s;
+    m.forward(input, batch);
+    m.stream = keep;
+    *e = hipGetLastError();
+    return m.out;
+}
+void denoiser_shape(const octopic_denoiser *d, int *height, int *width, int *max_batch) {
+    *height = d->impl.H;
+    *width = d->impl.W;
+    *max_batch = d->impl.maxb;
+}
+}  // namespace octopic
 
 namespace {
 int fail(octopic_denoiser *d, int code, const std::string &msg) {

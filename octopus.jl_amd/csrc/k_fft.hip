@@ -209,11 +209,11 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_fwd(const Plane *planes, u
         double va = 0.0, vb = 0.0;
         if (active && x < nx) {
             const unsigned long long ua = q[(size_t)ya * nx + x];
-            va = (double)ua * inv_scale;
+            va = (double)(long long)ua * inv_scale;   // signed: denoised planes may hold negative cells
             if (ua) q[(size_t)ya * nx + x] = 0ull;
             if (hasb) {
                 const unsigned long long ub = q[(size_t)(ya + 1) * nx + x];
-                vb = (double)ub * inv_scale;
+                vb = (double)(long long)ub * inv_scale;
                 if (ub) q[(size_t)(ya + 1) * nx + x] = 0ull;
             }
         }

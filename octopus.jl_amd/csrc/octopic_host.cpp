@@ -31,6 +31,7 @@
 #include "common.h"
 #include "fft.h"
 #include "launch.h"
+#include "denoise.h"
 
 using namespace octopic;
 
@@ -194,10 +195,10 @@ struct FFTPlanHost {
 };
 
 const char *kFamilies[] = {"slicing", "bounds", "geometry", "green", "deposit", "fft_rows_fwd", "fft_cols_conv",
-                           "fft_rows_inv", "field", "kick", "lum", "store", "node_prepare", "tile_order"};
+                           "fft_rows_inv", "field", "kick", "lum", "store", "node_prepare", "tile_order", "denoise"};
 constexpr int kNFamilies = sizeof(kFamilies) / sizeof(kFamilies[0]);
 enum Fam { F_SLICING, F_BOUNDS, F_GEOMETRY, F_GREEN, F_DEPOSIT, F_ROWS_FWD, F_COLS, F_ROWS_INV, F_FIELD, F_KICK, F_LUM,
-           F_STORE, F_NODE, F_TILE };
+           F_STORE, F_NODE, F_TILE, F_DENOISE };
 
 }  // namespace
 
@@ -274,7 +275,12 @@ struct octopic_handle {
     // pinned host mirrors
     SliceTable *h_tab = nullptr;
     ErrorInfo *h_err = nullptr;
-    unsigned long long *h_counters = nullptr;
+    unsigned long long *h_counters = nullptr;   // hits, misses, rebuilds, builds, dropped, -, denoised, fallbacks
+    // residual denoiser of the field planes (octopic_set_denoiser): not owned
+    octopic_denoiser *dn = nullptr;
+    double dn_limit = 0.0;
+    DBuf<double> dn_mom, dn_meta, dn_prof;
+    DBuf<float> dn_in;
     // last-collide results
     int ns[2] = {0, 0};
     int64_t nbeam[2] = {0, 0};
@@ -479,6 +485,25 @@ int octopic_set_stream(octopic_handle *h, void *s) {
     return OCTOPIC_OK;
 }
 
+int octopic_set_denoiser(octopic_handle *h, octopic_denoiser *d, double monitor_limit) {
+    if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    if (!d) {
+        h->dn = nullptr;
+        return OCTOPIC_OK;
+    }
+    if (h->mode == 2) return set_err(h, OCTOPIC_ERR_UNSUPPORTED, "the residual denoiser runs with interaction_grid = :slice_pair");
+    int dh = 0, dw = 0, dmb = 0;
+    octopic::denoiser_shape(d, &dh, &dw, &dmb);
+    if (dh != h->ny || dw != h->nx)
+        return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "the denoiser's planes (" + std::to_string(dh) + " x " + std::to_string(dw) +
+                                                            ") must be the grid's (ny x nx = " + std::to_string(h->ny) + " x " +
+                                                            std::to_string(h->nx) + ")");
+    if (!(monitor_limit > 0.0)) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "monitor_limit must be positive");
+    h->dn = d;
+    h->dn_limit = monitor_limit;
+    return OCTOPIC_OK;
+}
+
 void octopic_destroy(octopic_handle *h) {
     if (!h) return;
     hipSetDevice(h->device);
@@ -501,6 +526,7 @@ void octopic_destroy(octopic_handle *h) {
     h->lum_parts.release(); h->lum_partial.release(); h->lum_arrivals.release(); h->nodes.release(); h->node_jobs.release(); h->node_keys.release(); h->ran_dev.release();
     h->px.tw.release(); h->py.tw.release(); h->px.tw32.release(); h->py.tw32.release();
     h->spec32.release(); h->qh32.release(); h->phi32.release(); h->table32.release();
+    h->dn_mom.release(); h->dn_meta.release(); h->dn_prof.release(); h->dn_in.release();
     delete h;
 }
 
@@ -549,9 +575,9 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         }
         h->box.ensure(8);
         h->errinfo.ensure(1);
-        h->counters.ensure(6);
+        h->counters.ensure(8);
         HIPCHECK(hipMemset(h->errinfo.p, 0, sizeof(ErrorInfo)));
-        HIPCHECK(hipMemset(h->counters.p, 0, 6 * sizeof(unsigned long long)));
+        HIPCHECK(hipMemset(h->counters.p, 0, 8 * sizeof(unsigned long long)));
         h->lanes.ensure(kFoldLanes);
         h->mm.ensure(2 * (kFoldLanes / 256));
         h->bins.ensure(16384);
@@ -566,7 +592,7 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         }
         HIPCHECK(hipHostMalloc((void **)&h->h_tab, 2 * sizeof(SliceTable)));
         HIPCHECK(hipHostMalloc((void **)&h->h_err, sizeof(ErrorInfo)));
-        HIPCHECK(hipHostMalloc((void **)&h->h_counters, 6 * sizeof(unsigned long long)));
+        HIPCHECK(hipHostMalloc((void **)&h->h_counters, 8 * sizeof(unsigned long long)));
         HIPCHECK(hipDeviceSynchronize());
     } catch (Fail &f) {
         int code = f.code;
@@ -834,6 +860,17 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     const int64_t field_cells = (int64_t)nx * ny, lum_cells = (int64_t)(h->lnx + 1) * (h->lny + 1);
     const int64_t charge_stride = std::max(field_cells, lum_cells);
     const int nfield_planes = 2 * W * npl, nplanes_max = nfield_planes + 2 * W;
+    if (h->dn) {
+        int dh = 0, dw = 0, dmb = 0;
+        denoiser_shape(h->dn, &dh, &dw, &dmb);
+        if (dmb < nfield_planes)
+            fail(OCTOPIC_ERR_INVALID_ARGUMENT, "the denoiser's max_batch (" + std::to_string(dmb) +
+                                                   ") is below the widest batch's field planes (" + std::to_string(nfield_planes) + ")");
+        h->dn_mom.ensure((size_t)2 * W * kDnMomentChunks * 10);
+        h->dn_meta.ensure((size_t)nfield_planes * 4);
+        h->dn_prof.ensure((size_t)nfield_planes * (nx + ny));
+        h->dn_in.ensure((size_t)nfield_planes * field_cells * 2);
+    }
     h->pairs.ensure(std::max<size_t>(hp.size(), 1));
     if (!hp.empty())
         HIPCHECK(hipMemcpyAsync(h->pairs.p, hp.data(), sizeof(BatchPair) * hp.size(), hipMemcpyHostToDevice, s));
@@ -910,8 +947,8 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     const int nslots = h->nkeys + (o.green_cache == OCTOPIC_CACHE_SLICE_PAIR ? 0 : 2 * W);
     h->spec.ensure((size_t)std::max(nslots, 1) * h->slot_stride);
     if (h->fp32) h->spec32.ensure((size_t)std::max(nslots, 1) * h->slot_stride);
-    unsigned long long counters_before[6];
-    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    unsigned long long counters_before[8];
+    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     std::memcpy(counters_before, h->h_counters, sizeof(counters_before));
 
@@ -1044,6 +1081,21 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
                 HIPCHECK(launch_deposit(l, s));
             }
         }
+        if (h->dn) {
+            // field planes through the residual denoiser before the convolution
+            Timed t(h, F_DENOISE);
+            DnCollideArgs a{};
+            a.inters = h->inters.p; a.ninters = 2 * B; a.planes = h->planes.p; a.nplanes = nfp; a.st = st;
+            a.charge = h->charge.p; a.charge_stride = charge_stride; a.scale = scale; a.inv_scale = inv_scale;
+            a.method = o.deposit; a.nx = nx; a.ny = ny; a.chunks = kDnMomentChunks;
+            a.mom_part = h->dn_mom.p; a.meta = h->dn_meta.p; a.prof = h->dn_prof.p; a.input = h->dn_in.p;
+            a.limit = h->dn_limit; a.counters = h->counters.p + 6; a.err = h->errinfo.p;
+            HIPCHECK(launch_dn_prepare(a, s));
+            hipError_t e = hipSuccess;
+            const float *out = denoiser_run(h->dn, h->dn_in.p, nfp, s, &e);
+            HIPCHECK(e);
+            HIPCHECK(launch_dn_apply(a, out, s));
+        }
         const int64_t qh_stride = (int64_t)(nx + 1) * ny;
         // the batch's luminosity overlap sums run as extra blocks of the charge row transform
         LumArgs lum{};
@@ -1118,7 +1170,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     if (npairs)
         HIPCHECK(hipMemcpyAsync(h->recs.data(), h->records.p, sizeof(octopic_interaction_record) * 2 * npairs,
                                 hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     std::vector<int32_t> ran_node;
     if (h->mode == 2 && npairs) {
         ran_node.resize(npairs);
@@ -1149,6 +1201,8 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     S.green_rebuilds = (int64_t)h->h_counters[2];
     S.green_builds = (int64_t)(h->h_counters[3] - counters_before[3]);
     S.dropped = (int64_t)(h->h_counters[4] - counters_before[4]);
+    S.denoised_planes = (int64_t)(h->h_counters[6] - counters_before[6]);
+    S.denoise_fallbacks = (int64_t)(h->h_counters[7] - counters_before[7]);
     S.pairs = npairs;
     S.batches = (int64_t)batches.size();
     S.widest_batch = widest;

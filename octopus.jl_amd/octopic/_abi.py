@@ -128,6 +128,8 @@ class Stats(C.Structure):
         ("green_builds", C.c_int64),
         ("planes", C.c_int64),
         ("collides", C.c_int64),
+        ("denoised_planes", C.c_int64),
+        ("denoise_fallbacks", C.c_int64),
     ]
 
 
@@ -190,6 +192,7 @@ def declare(lib):
         "octopic_denoiser_gradient": (i32, [vp, vp, vp, i32, vp, vp]),
         "octopic_denoiser_train_step": (i32, [vp, vp, vp, i32, C.c_float, vp, vp]),
         "octopic_denoiser_make_pairs": (i32, [vp, C.c_uint64, i64, i32, i64, i32, vp, vp, vp, vp, vp]),
+        "octopic_set_denoiser": (i32, [vp, vp, C.c_double]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -229,4 +232,5 @@ EXPORTED = [
     "octopic_denoiser_gradient",
     "octopic_denoiser_train_step",
     "octopic_denoiser_make_pairs",
+    "octopic_set_denoiser",
 ]

@@ -173,6 +173,16 @@ class PICCollider:
                                           1 if cl else 0, C.byref(lum)))
         return lum.value
 
+    def set_denoiser(self, denoiser, monitor_limit: float = 0.0):
+        """Attach an octopic.denoise.Denoiser to this collision (None detaches): the field planes are
+        then Gaussian-subtracted, denoised and rebuilt before the convolution (octopic_set_denoiser)."""
+        if denoiser is None:
+            self._denoiser = None
+            self._check(lib().octopic_set_denoiser(self.h, None, 0.0))
+            return
+        self._check(lib().octopic_set_denoiser(self.h, denoiser._h, float(monitor_limit)))
+        self._denoiser = denoiser   # the handle does not own it
+
     def stats(self):
         s = _abi.Stats()
         self._check(lib().octopic_get_stats(self.h, C.byref(s)))

@@ -59,6 +59,7 @@ def lib():
         L.oracle_weights.argtypes = [C.c_int32, C.c_double, C.c_int64, I64, D]
         L.oracle_create.argtypes = [C.POINTER(_abi.Options), C.POINTER(C.c_void_p)]
         L.oracle_destroy.argtypes = [C.c_void_p]
+        L.oracle_set_gaussian_replace.argtypes = [C.c_void_p, C.c_int32]
         L.oracle_collide.argtypes = [C.c_void_p, C.POINTER(D), C.c_int64, D, C.POINTER(D), C.c_int64, D,
                                      C.c_int32, C.c_int32, D]
         L.oracle_stats.argtypes = [C.c_void_p, I64]
@@ -327,7 +328,8 @@ def track_linear6d(m, coords):
 class Collider:
     """A collide! with the persistent task state (Green cache + workspaces)."""
 
-    def __init__(self, solver):
+    def __init__(self, solver, gaussian_replace=False):
+        """gaussian_replace: test mode, every field deposit replaced by its neutralised drifted Gaussian."""
         self.solver = solver
         o, keep = solver.to_options()
         self._keep = keep
@@ -335,6 +337,8 @@ class Collider:
         self.h = None
         _check(lib().oracle_create(C.byref(o), C.byref(h)))
         self.h = h
+        if gaussian_replace:
+            lib().oracle_set_gaussian_replace(self.h, 1)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -919,6 +919,7 @@ struct SliceParam { double weight, lb, center, rb; };
 
 struct Context {
     Opt o;
+    int gaussian_replace = 0;   // test mode: each field deposit replaced by its neutralised drifted Gaussian
     Plans plans;
     std::vector<std::unique_ptr<Workspace>> ws;
     GreenCache cache;
@@ -991,6 +992,69 @@ const std::vector<cd> *slice_pair_green(Context &C, Workspace &w, const std::tup
     return &tmp;
 }
 
+// _gpic_gaussian_profile! (gaussian_pic.jl:206-244): node integrals of the unit Gaussian against the
+// CIC tent / TSC spline, node i at x0 + i h.
+void gaussian_profile(double *g, int n, double x0, double h, double mu, double s, int method) {
+    const double invroot = 1.0 / (s * std::sqrt(2.0)), norm = 1.0 / (s * std::sqrt(2.0 * M_PI));
+    auto pdf = [&](double x) { return norm * std::exp(-((x - mu) * (x - mu)) / (2 * s * s)); };
+    auto m0 = [&](double A, double B) { return 0.5 * (std::erf((B - mu) * invroot) - std::erf((A - mu) * invroot)); };
+    auto m1 = [&](double A, double B, double xi) { return (mu - xi) * m0(A, B) - s * s * (pdf(B) - pdf(A)); };
+    auto m2 = [&](double A, double B, double xi) {
+        return (s * s + (mu - xi) * (mu - xi)) * m0(A, B) - s * s * ((B - mu) * pdf(B) - (A - mu) * pdf(A)) -
+               2 * (mu - xi) * s * s * (pdf(B) - pdf(A));
+    };
+    const double half = h / 2;
+    for (int i = 0; i < n; ++i) {
+        const double xi = x0 + i * h;
+        if (method == OCTOPIC_CIC) {
+            g[i] = m0(xi - h, xi + h) + (m1(xi - h, xi, xi) - m1(xi, xi + h, xi)) / h;
+        } else {
+            const double cL = xi - half, cR = xi + half, lL = xi - 3 * half, lR = xi - half, rL = xi + half, rR = xi + 3 * half;
+            g[i] = 0.75 * m0(cL, cR) - m2(cL, cR, xi) / (h * h) + 1.125 * (m0(lL, lR) + m0(rL, rR)) +
+                   1.5 * (m1(lL, lR, xi) - m1(rL, rR, xi)) / h + 0.5 * (m2(lL, lR, xi) + m2(rL, rR, xi)) / (h * h);
+        }
+    }
+}
+
+// The test mode's replacement of one deposit: the source's moments (_gpic_source_moments,
+// gaussian_pic.jl:374-435, uncoupled), drifted by s (_gpic_drifted_gaussian, :482-491), its profile
+// scaled to the deposited total (the neutralised amplitude, :561-566).  Sources with fewer than two
+// members or no positive width keep their deposit (the plain-PIC fallback, :525-530).
+void replace_by_gaussian(double *charge, int64_t ld, const Coords &src, double s, const Grid &sg, int nx, int ny, int method) {
+    const int64_t n = (int64_t)src.size();
+    if (n < 2) return;
+    const double x0 = src.x[0], px0 = src.px[0], y0 = src.y[0], py0 = src.py[0];
+    double sx = 0, spx = 0, sy = 0, spy = 0, sxx = 0, spxpx = 0, syy = 0, spypy = 0, sxpx = 0, sypy = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double dx = src.x[i] - x0, dpx = src.px[i] - px0, dy = src.y[i] - y0, dpy = src.py[i] - py0;
+        sx += dx; spx += dpx; sy += dy; spy += dpy;
+        sxx += dx * dx; spxpx += dpx * dpx; syy += dy * dy; spypy += dpy * dpy;
+        sxpx += dx * dpx; sypy += dy * dpy;
+    }
+    const double invn = 1.0 / (double)n;
+    const double dmx = sx * invn, dmpx = spx * invn, dmy = sy * invn, dmpy = spy * invn;
+    const double varx = std::max(sxx * invn - dmx * dmx, 0.0), varpx = std::max(spxpx * invn - dmpx * dmpx, 0.0);
+    const double vary = std::max(syy * invn - dmy * dmy, 0.0), varpy = std::max(spypy * invn - dmpy * dmpy, 0.0);
+    const double cxpx = sxpx * invn - dmx * dmpx, cypy = sypy * invn - dmy * dmpy;
+    const double mux = (x0 + dmx) + (px0 + dmpx) * s, muy = (y0 + dmy) + (py0 + dmpy) * s;
+    const double vx = varx + 2 * s * cxpx + s * s * varpx, vy = vary + 2 * s * cypy + s * s * varpy;
+    const double sigx = std::sqrt(std::max(vx, 0.0)), sigy = std::sqrt(std::max(vy, 0.0));
+    if (!(sigx > 0 && sigy > 0 && std::isfinite(sigx) && std::isfinite(sigy))) return;
+    const double hx = sg.width / (double)(nx - 1), hy = sg.height / (double)(ny - 1);
+    std::vector<double> gx(nx), gy(ny);
+    gaussian_profile(gx.data(), nx, sg.x0, hx, mux, sigx, method);
+    gaussian_profile(gy.data(), ny, sg.y0, hy, muy, sigy, method);
+    double q = 0, sgx = 0, sgy = 0;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) q += charge[i + (size_t)j * ld];
+    for (double v : gx) sgx += v;
+    for (double v : gy) sgy += v;
+    if (!(sgx * sgy > 0)) return;
+    const double amp = q / (sgx * sgy);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) charge[i + (size_t)j * ld] = amp * gx[i] * gy[j];
+}
+
 void solve_drifted_field(Context &C, Workspace &w, FieldWS &f, const Coords &src, double s, const Grid &sg,
                          const std::vector<cd> &gfft) {  // pic_cpu.jl:1965-1990
     const Opt &o = C.o;
@@ -1001,6 +1065,7 @@ void solve_drifted_field(Context &C, Workspace &w, FieldWS &f, const Coords &src
     std::fill(w.charge.begin(), w.charge.end(), 0.0);
     deposit(w.charge.data(), P1, P2, o.deposit, src.x.data(), src.px.data(), src.y.data(), src.py.data(), s, true,
             sg.x0, sg.y0, hx, hy, nx, ny, (int64_t)src.size(), &w.local_charge);
+    if (C.gaussian_replace) replace_by_gaussian(w.charge.data(), P1, src, s, sg, nx, ny, o.deposit);
     for (size_t k = 0; k < w.charge.size(); ++k) w.spectral[k] = cd(w.charge[k], 0.0);
     fft2(w.spectral.data(), (int)P1, (int)P2, -1, *C.plans.p1, *C.plans.p2);
     for (size_t k = 0; k < w.spectral.size(); ++k) w.spectral[k] *= gfft[k];
@@ -1785,6 +1850,12 @@ int oracle_create(const octopic_options *opt, void **out) {
     });
 }
 void oracle_destroy(void *s) { delete static_cast<State *>(s); }
+// Test mode: every field deposit replaced by its neutralised drifted Gaussian (what the device's
+// denoised collide gives with a network that outputs zero).
+int oracle_set_gaussian_replace(void *s, int32_t on) {
+    static_cast<State *>(s)->C->gaussian_replace = on ? 1 : 0;
+    return 0;
+}
 
 int oracle_collide(void *s, double *b1[6], int64_t n1, const double p1[5], double *b2[6], int64_t n2, const double p2[5],
                    int32_t compute_lum, int32_t nthreads, double *lum) {

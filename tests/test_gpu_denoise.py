@@ -173,3 +173,105 @@ def test_pairs_are_deterministic(torch):
         assert torch.equal(s, t)
     c = d.make_pairs(seed=4, first=4, batch=2, n_target=500_000, raw=True)
     assert not torch.equal(a[3], c[3])
+
+
+def test_trained_network_beats_raw_residual(torch):
+    """What the learned part gains: after 300 Adam steps on fresh pairs (16 a step, converged deposits
+    of 4M macroparticles), the denoised residual of held-out slices (another seed) is at least 3x
+    closer, in mean squared error, to the converged residual than the raw noisy residual is."""
+    from octopic import train_denoiser as TD
+    d = _make(torch, 64, 64, 16, seed=0)
+    losses, mon = TD.train(d, steps=300, batch=16, seed=1, n_target=4_000_000)
+    assert np.isfinite(losses).all() and mon > 0
+    assert losses[-20:].mean() < 0.5 * losses[:20].mean()
+    raw, den = TD.evaluate(d, seed=1001, samples=64, batch=16, n_target=4_000_000)
+    assert den < raw / 3.0, (raw, den)
+
+
+# ------------------------------------------------------------------------- in the collision
+def _collide_beams(oracle, n1=30000, n2=12000, seed=3):
+    e = oracle.generate_beam(n1, seed, 1, (0.8, 0.072, 12.727), (0, 0, 0), (95e-6, 8.5e-6, 0.007))
+    p = oracle.generate_beam(n2, seed, 2, (0.8, 0.072, 90.909), (0, 0, 0), (95e-6, 8.5e-6, 0.06))
+    return e, p
+
+
+def _params():
+    from octopic import EMASS_EV, PMASS_EV, beam_params
+    return beam_params(-1.0, EMASS_EV, 10e9, 1.7e11), beam_params(1.0, PMASS_EV, 275e9, 6.9e10)
+
+
+def _solver(**kw):
+    from octopic import LongitudinalSlicing, PICPoissonSolver
+    sl = LongitudinalSlicing(5, method="normal_quantile", center_position="centroid")
+    base = dict(grid=(32, 32), deposit_method="TSC", field_derivative="fourth", slicing1=sl, slicing2=sl)
+    base.update(kw)
+    return PICPoissonSolver(**base)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(slice_interpolation="quadratic"), dict(deposit_method="CIC", green_cache="none")],
+                         ids=["linear", "quadratic", "cic"])
+def test_zero_network_collide_matches_gaussian_oracle(oracle, torch, kw):
+    """A network with all parameters zero outputs zero, so the denoised collide deposits each field plane
+    as the source's neutralised drifted Gaussian: the oracle's replacement mode restates exactly that
+    from the reference (gaussian_pic.jl:206-244, 374-491, 543-576). Coordinates within 1e-10 of the
+    largest, luminosity (untouched planes) within 1e-10, the same cache decisions, every plane denoised."""
+    from octopic import Beam, PICCollider
+    PE, PP = _params()
+    solver = _solver(**kw)
+    e, p = _collide_beams(oracle)
+    ce, cp = e.copy(), p.copy()
+    ocol = oracle.Collider(solver, gaussian_replace=True)
+    lo = ocol.collide(ce, PE, cp, PP)
+    d = _make(torch, 32, 32, 32)
+    d.set_params(np.zeros(d.n_params, np.float32))
+    g = PICCollider(solver)
+    g.set_denoiser(d, monitor_limit=1e30)
+    be, bp = Beam.from_host(e, PE), Beam.from_host(p, PP)
+    lg = g.collide(be, bp)
+    torch.cuda.synchronize()
+    st = g.stats()
+    assert st["denoised_planes"] == st["planes"] and st["denoise_fallbacks"] == 0
+    so = ocol.stats()
+    assert (so["hits"], so["misses"], so["rebuilds"]) == (st["green_hits"], st["green_misses"], st["green_rebuilds"])
+    assert math.isclose(lo, lg, rel_tol=1e-10)
+    for a, b in ((ce, be.to_host()), (cp, bp.to_host())):
+        for k in range(6):
+            assert np.max(np.abs(a[k] - b[k])) <= 1e-10 * np.max(np.abs(a[k])), k
+    # and it is not the plain collide: the Gaussian planes move the kicks
+    e2, p2 = e.copy(), p.copy()
+    oracle.Collider(solver).collide(e2, PE, p2, PP)
+    assert np.max(np.abs(e2[1] - ce[1])) > 1e-6 * np.max(np.abs(e2[1] - e[1]))
+
+
+def test_monitor_fallback_is_plain_pic(oracle, torch):
+    """With a monitor limit below every plane's monitor, each plane stays plain PIC: the collide is
+    bit-identical to one without a denoiser, and every plane counts as a fallback."""
+    from octopic import Beam, PICCollider
+    PE, PP = _params()
+    solver = _solver()
+    e, p = _collide_beams(oracle)
+    d = _make(torch, 32, 32, 32)
+    ref = PICCollider(solver)
+    b1, b2 = Beam.from_host(e, PE), Beam.from_host(p, PP)
+    l1 = ref.collide(b1, b2)
+    g = PICCollider(solver)
+    g.set_denoiser(d, monitor_limit=1e-30)
+    c1, c2 = Beam.from_host(e, PE), Beam.from_host(p, PP)
+    l2 = g.collide(c1, c2)
+    torch.cuda.synchronize()
+    st = g.stats()
+    assert st["denoised_planes"] == 0 and st["denoise_fallbacks"] == st["planes"] > 0
+    assert l1 == l2
+    assert torch.equal(b1.coords, c1.coords) and torch.equal(b2.coords, c2.coords)
+
+
+def test_denoiser_attachment_is_validated(torch):
+    from octopic import PICCollider
+    from octopic.backend import ArgumentError, UnsupportedOptionError
+    d = _make(torch, 32, 32, 8)
+    with pytest.raises(ArgumentError):
+        PICCollider(_solver(grid=(64, 64))).set_denoiser(d, 1.0)
+    with pytest.raises(UnsupportedOptionError):
+        PICCollider(_solver(interaction_grid="node")).set_denoiser(d, 1.0)
+    with pytest.raises(ArgumentError):
+        PICCollider(_solver()).set_denoiser(d, 0.0)
