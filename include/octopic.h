@@ -258,8 +258,9 @@ int octopic_denoiser_gradient(octopic_denoiser *d, const float *input, const flo
 int octopic_denoiser_train_step(octopic_denoiser *d, const float *input, const float *target, int32_t batch,
                                 float lr, float *loss, float *grad_out);
 /* Training pairs generated by the method itself: for samples first_sample .. first_sample+batch-1, a
- * two-component Gaussian-mixture slice (core with dipole and quadrupole modes, centred halo; the
- * distribution of each sample follows from seed and its id) is drawn twice with Philox counter
+ * two-component Gaussian-mixture slice (core with dipole and quadrupole modes, centred halo; one
+ * sample in three a plain Gaussian; the distribution of each sample follows from seed and its id)
+ * is drawn twice with Philox counter
  * streams, n_noisy in [16384, 262144] and n_target macroparticles, and deposited (method 0 CIC,
  * 1 TSC) on the height x width mesh in 2^-32 fixed point.  Each deposit has its own Gaussian (its
  * particles' moments) subtracted by the neutralised erf profile (gaussian_pic.jl:206-244, 543-576).

@@ -31,6 +31,7 @@
 namespace octopic {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even
@@ -98,6 +99,95 @@ __global__ __launch_bounds__(kConvThreads) void k_conv(ConvArgs a) {
             if (a.yf) a.yf[q * a.cout + co] = v;
         }
     }
+}
+
+// The same product for inputs of 8k channels, with the block's weights staged in LDS (rows padded
+// by 8 to spread the banks) and v_mfma_f32_16x16x32_bf16: lane l loads 8 consecutive reduction
+// indices 8(l/16)..8(l/16)+7 of its row (A) or column (B) in one 16-byte access; A and B use the same
+// index assignment, so the sum is the full reduction whatever order the instruction takes it in.
+// Each wave takes MT 16-pixel tiles, the block 64 MT pixels.
+template <int NT, int MT>
+__global__ __launch_bounds__(kConvThreads) void k_conv8(ConvArgs a) {
+    extern __shared__ __align__(16) uint16_t wl[];
+    const int ldw = a.kpad + 8;
+    {
+        const int per_row = a.kpad / 8;
+        for (int e = threadIdx.x; e < a.cout * per_row; e += blockDim.x) {
+            const int r = e / per_row, c8 = e - r * per_row;
+            *reinterpret_cast<uint4 *>(wl + r * ldw + c8 * 8) = *reinterpret_cast<const uint4 *>(a.w + (int64_t)r * a.kpad + c8 * 8);
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t P = (int64_t)a.B * a.H * a.W;
+    const int64_t base = (int64_t)blockIdx.x * (64 * MT) + wave * (16 * MT);
+    if (base >= P) return;
+    const int HW = a.H * a.W;
+    int pb[MT], py[MT], px[MT];
+    bool pv[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const int64_t p = base + mt * 16 + (lane & 15);
+        pv[mt] = p < P;
+        const int64_t pp = pv[mt] ? p : 0;
+        pb[mt] = (int)(pp / HW);
+        const int rem = (int)(pp - (int64_t)pb[mt] * HW);
+        py[mt] = rem / a.W;
+        px[mt] = rem - py[mt] * a.W;
+    }
+    const int kq = (lane >> 4) * 8, kmax = 9 * a.cin;
+    int tap = kq / a.cin, c = kq - tap * a.cin;
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t *wrow = wl + (lane & 15) * ldw;
+    // A fragments of reduction step k (this lane's 8 indices at tap, channel c), zero off the mesh
+    auto load_a = [&](bf16x8 (&av)[MT], int k, int tp, int cc) {
+        const bool kin = k < kmax;
+        const int dy = tp / 3 - 1, dx = tp - (tp / 3) * 3 - 1;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const int yy = py[mt] + dy, xx = px[mt] + dx;
+            if (kin && pv[mt] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                av[mt] = *reinterpret_cast<const bf16x8 *>(a.x + (((int64_t)pb[mt] * a.H + yy) * a.W + xx) * a.cin + cc);
+            else
+                av[mt] = bf16x8{};
+        }
+    };
+    bf16x8 cur[MT], nxt[MT];
+    load_a(cur, kq, tap, c);
+    for (int k0 = 0; k0 < a.kpad; k0 += 32) {
+        c += 32;
+        while (c >= a.cin) { c -= a.cin; ++tap; }
+        if (k0 + 32 < a.kpad) load_a(nxt, k0 + 32 + kq, tap, c);   // next step's loads in flight during this step
+        const int k = k0 + kq;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(wrow + t * 16 * ldw + k);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[mt], bv, acc[mt][t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) cur[mt] = nxt[mt];
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int co = t * 16 + (lane & 15);
+            const float bb = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t q = base + mt * 16 + (lane >> 4) * 4 + i;
+                if (q >= P) continue;
+                float v = acc[mt][t][i] + bb;
+                if (a.relu) v = v > 0.f ? v : 0.f;
+                if (a.y) a.y[q * a.cout + co] = f2bf(v);
+                if (a.yf) a.yf[q * a.cout + co] = v;
+            }
+        }
 }
 
 // Weight gradient partials: part[s][co][k] = sum over the pixels of split s of dy[p][co] x_k(p),
@@ -198,15 +288,17 @@ __global__ void k_pool(const uint16_t *x, int B, int H, int W, int C, uint16_t *
 }
 
 // Nearest upsampling of deep [B][h][w][Cd] to [B][2h][2w], concatenated with skip [B][2h][2w][Cs]:
-// out channel c < Cd from deep, else skip.
+// out channel c < Cd from deep, else skip. One thread per pixel and group of 8 channels (Cd, Cs
+// multiples of 8), 16-byte copies.
 __global__ void k_up_concat(const uint16_t *deep, const uint16_t *skip, int B, int H, int W, int Cd, int Cs, uint16_t *y) {
-    const int C = Cd + Cs;
-    const int64_t n = (int64_t)B * H * W * C;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        const int c = (int)(e % C);
-        const int64_t q = e / C;
-        const int xx = (int)(q % W), yy = (int)((q / W) % H), b = (int)(q / ((int64_t)W * H));
-        y[e] = c < Cd ? deep[(((int64_t)b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * Cd + c] : skip[q * Cs + (c - Cd)];
+    const int G = (Cd + Cs) / 8, HW = H * W;
+    const int n = B * HW * G;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const int q = e / G, c = (e - q * G) * 8;
+        const int b = q / HW, r = q - b * HW, yy = r / W, xx = r - yy * W;
+        const uint16_t *src = c < Cd ? deep + (((int64_t)b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * Cd + c
+                                     : skip + (int64_t)q * Cs + (c - Cd);
+        *reinterpret_cast<uint4 *>(y + (int64_t)e * 8) = *reinterpret_cast<const uint4 *>(src);
     }
 }
 
@@ -337,7 +429,8 @@ __global__ void k_adam(float *p, const float *g, float *m, float *v, int64_t n, 
 // fwd[co][k = tap cin + ci] (cin = stored input channels, zero beyond ci_logical and k >= 9 cin);
 // bwd[ci][k' = tap' cout + co] = W[co][8 - tap'][ci] for the input gradient (cin_logical rows).
 __global__ void k_pack_weights(const float *W, int cout, int cin, int cin_logical, int kpad, uint16_t *fwd, uint16_t *bwd) {
-    const int nf = cout * kpad, nb = bwd ? cin_logical * 9 * cout : 0;
+    const int kpadb = (9 * cout + 31) / 32 * 32;   // backward rows padded to a multiple of 32 with zeros
+    const int nf = cout * kpad, nb = bwd ? cin_logical * kpadb : 0;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nf + nb; e += gridDim.x * blockDim.x) {
         if (e < nf) {
             const int co = e / kpad, k = e - co * kpad;
@@ -348,8 +441,13 @@ __global__ void k_pack_weights(const float *W, int cout, int cin, int cin_logica
             }
             fwd[e] = f2bf(v);
         } else {
-            const int f = e - nf, ci = f / (9 * cout), r = f - ci * 9 * cout, tp = r / cout, co = r - tp * cout;
-            bwd[f] = f2bf(W[((int64_t)co * 9 + (8 - tp)) * cin_logical + ci]);
+            const int f = e - nf, ci = f / kpadb, r = f - ci * kpadb;
+            if (r >= 9 * cout) {
+                bwd[f] = 0;
+            } else {
+                const int tp = r / cout, co = r - tp * cout;
+                bwd[f] = f2bf(W[((int64_t)co * 9 + (8 - tp)) * cin_logical + ci]);
+            }
         }
     }
 }
@@ -398,19 +496,21 @@ __host__ __device__ inline double gen_uniform(uint64_t seed, int64_t sample, int
 }
 
 // The distribution of training sample `sample` (a two-component Gaussian mixture in cell units):
-// core widths sx, sy in [2.5, 6] cells, shifted by a dipole mode of amplitude a_d in [0, 0.3] core
-// sigmas and squeezed by a quadrupole mode a_q in [0, 0.3] (sx (1 + a_q), sy (1 - a_q), either
-// sign); a centred halo of fraction f_h in [0.05, 0.2] and width 1.3-2 core sigmas; the mesh centre
-// jittered by up to one cell; the noisy sample draws n_noisy in [16384, 262144] (log-uniform).
+// core widths sx, sy in [1.5, 8] cells, shifted by a dipole mode of amplitude a_d in [0, 0.3] core
+// sigmas and squeezed by a quadrupole mode a_q in [0, 0.3] (sx (1 + a_q), sy (1 - a_q), either sign);
+// a centred halo of fraction f_h in [0, 0.2] and width 1.3-2 core sigmas; one sample in three is a
+// plain Gaussian (no modes, no halo), the form most collision slices take; the mesh centre jittered
+// by up to four cells; the noisy sample draws n_noisy in [16384, 262144] (log-uniform).
 struct PairDist {
     double cx, cy, sx, sy, ox, oy, hx, hy, fh;
     int64_t n_noisy;
 };
 __host__ __device__ inline PairDist pair_dist(uint64_t seed, int64_t sample, int H, int W) {
     PairDist d;
-    const double sx0 = 2.5 + 3.5 * gen_uniform(seed, sample, 0), sy0 = 2.5 + 3.5 * gen_uniform(seed, sample, 1);
-    const double ad = 0.3 * gen_uniform(seed, sample, 2), th = 6.283185307179586 * gen_uniform(seed, sample, 3);
-    const double aq = 0.3 * (2.0 * gen_uniform(seed, sample, 4) - 1.0);
+    const bool plain = gen_uniform(seed, sample, 10) < 1.0 / 3.0;
+    const double sx0 = 1.5 + 6.5 * gen_uniform(seed, sample, 0), sy0 = 1.5 + 6.5 * gen_uniform(seed, sample, 1);
+    const double ad = plain ? 0.0 : 0.3 * gen_uniform(seed, sample, 2), th = 6.283185307179586 * gen_uniform(seed, sample, 3);
+    const double aq = plain ? 0.0 : 0.3 * (2.0 * gen_uniform(seed, sample, 4) - 1.0);
     d.sx = sx0 * (1.0 + aq);
     d.sy = sy0 * (1.0 - aq);
     d.ox = ad * sx0 * cos(th);
@@ -418,9 +518,9 @@ __host__ __device__ inline PairDist pair_dist(uint64_t seed, int64_t sample, int
     const double hk = 1.3 + 0.7 * gen_uniform(seed, sample, 5);
     d.hx = hk * sx0;
     d.hy = hk * sy0;
-    d.fh = 0.05 + 0.15 * gen_uniform(seed, sample, 6);
-    d.cx = 0.5 * (W - 1) + (2.0 * gen_uniform(seed, sample, 7) - 1.0);
-    d.cy = 0.5 * (H - 1) + (2.0 * gen_uniform(seed, sample, 8) - 1.0);
+    d.fh = plain ? 0.0 : 0.2 * gen_uniform(seed, sample, 6);
+    d.cx = 0.5 * (W - 1) + 4.0 * (2.0 * gen_uniform(seed, sample, 7) - 1.0);
+    d.cy = 0.5 * (H - 1) + 4.0 * (2.0 * gen_uniform(seed, sample, 8) - 1.0);
     d.n_noisy = (int64_t)llround(exp(log(16384.0) + (log(262144.0) - log(16384.0)) * gen_uniform(seed, sample, 9)));
     return d;
 }
@@ -631,13 +731,16 @@ __global__ __launch_bounds__(256) void k_dn_prepare(DnCollideArgs a) {
     const int p = blockIdx.x, nx = a.nx, ny = a.ny, HW = nx * ny;
     const Plane &pl = a.planes[p];
     const Inter &it = a.inters[pl.inter];
+    __shared__ double smf[10];
+    if (threadIdx.x < 10) {
+        double t = 0.0;
+        for (int c = 0; c < a.chunks; ++c) t += a.mom_part[((int64_t)pl.inter * a.chunks + c) * 10 + threadIdx.x];
+        smf[threadIdx.x] = t;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         double sm[10];
-        for (int j = 0; j < 10; ++j) {
-            double t = 0.0;
-            for (int c = 0; c < a.chunks; ++c) t += a.mom_part[((int64_t)pl.inter * a.chunks + c) * 10 + j];
-            sm[j] = t;
-        }
+        for (int j = 0; j < 10; ++j) sm[j] = smf[j];
         const int64_t n = it.src_n;
         int ok = it.valid && n >= 2 && !failed(a.err);
         double mux = 0, muy = 0, sigx = 0, sigy = 0;
@@ -677,14 +780,14 @@ __global__ __launch_bounds__(256) void k_dn_prepare(DnCollideArgs a) {
     for (int c = threadIdx.x; c < HW; c += blockDim.x) acc[0] += (double)(long long)q[c] * a.inv_scale;
     for (int i = threadIdx.x; i < nx; i += blockDim.x) { acc[1] += g[i]; acc[3] = fmax(acc[3], g[i]); }
     for (int j = threadIdx.x; j < ny; j += blockDim.x) { acc[2] += g[nx + j]; acc[4] = fmax(acc[4], g[nx + j]); }
-    for (int v = 0; v < 5; ++v) {
+    for (int v = 0; v < 5; ++v) {   // fixed-order tree reductions
         red[threadIdx.x] = acc[v];
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int j = 0; j < (int)blockDim.x; ++j) t = v >= 3 ? fmax(t, red[j]) : t + red[j];
-            tot[v] = t;
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] = v >= 3 ? fmax(red[threadIdx.x], red[threadIdx.x + w]) : red[threadIdx.x] + red[threadIdx.x + w];
+            __syncthreads();
         }
+        if (threadIdx.x == 0) tot[v] = red[0];
         __syncthreads();
     }
     const double amp = tot[1] * tot[2] > 0.0 ? tot[0] / (tot[1] * tot[2]) : 0.0;
@@ -702,10 +805,12 @@ __global__ __launch_bounds__(256) void k_dn_prepare(DnCollideArgs a) {
     }
     red[threadIdx.x] = ss;
     __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int j = 0; j < (int)blockDim.x; ++j) t += red[j];
-        const double mon = sqrt(t / HW);
+        const double mon = sqrt(red[0] / HW);
         double *m = a.meta + (int64_t)p * 4;
         m[0] = amp;
         m[1] = peak;
@@ -766,6 +871,7 @@ constexpr LayerSpec kLayers[10] = {{"e1a", 2, 16}, {"e1b", 16, 16}, {"e2a", 16, 
 enum { E1A, E1B, E2A, E2B, BA, BB, D2A, D2B, D1A, D1B };
 
 inline int ceil16(int v) { return (v + 15) / 16 * 16; }
+inline int ceil32(int v) { return (v + 31) / 32 * 32; }
 inline int grid_for(int64_t n, int threads = 256) {
     int64_t g = (n + threads - 1) / threads;
     if (g < 1) g = 1;
@@ -826,12 +932,32 @@ struct DenoiserImpl {
         if (!backward) {
             c.w = wf[l]; c.bias = params + boff[l]; c.cin = cin_s[l]; c.cout = kLayers[l].cout; c.kpad = kpad[l];
         } else {
-            c.w = wb[l]; c.bias = nullptr; c.cin = kLayers[l].cout; c.cout = kLayers[l].cin; c.kpad = 9 * kLayers[l].cout;
+            c.w = wb[l]; c.bias = nullptr; c.cin = kLayers[l].cout; c.cout = kLayers[l].cin; c.kpad = ceil32(9 * kLayers[l].cout);
         }
         c.relu = relu ? 1 : 0;
         c.y = y;
         c.yf = yf;
         const int64_t Pn = (int64_t)B * c.H * c.W;
+        if (c.cin % 8 == 0) {
+            const size_t lds = (size_t)c.cout * (c.kpad + 8) * sizeof(uint16_t);
+            const int nt = c.cout / 16, mt = nt <= 2 ? 4 : 2;
+            const dim3 g8((unsigned)((Pn + 64 * mt - 1) / (64 * mt)));
+#define OCTOPIC_CONV8(NT_, MT_)                                                                                  \
+    {                                                                                                           \
+        hipFuncSetAttribute((const void *)k_conv8<NT_, MT_>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        hipLaunchKernelGGL((k_conv8<NT_, MT_>), g8, dim3(kConvThreads), lds, stream, c);                        \
+    }
+            switch (nt) {
+                case 1: OCTOPIC_CONV8(1, 4) break;
+                case 2: OCTOPIC_CONV8(2, 4) break;
+                case 3: OCTOPIC_CONV8(3, 2) break;
+                case 4: OCTOPIC_CONV8(4, 2) break;
+                case 6: OCTOPIC_CONV8(6, 2) break;
+                default: break;
+            }
+#undef OCTOPIC_CONV8
+            return;
+        }
         const dim3 g((unsigned)((Pn + 63) / 64));
         switch (c.cout / 16) {
             case 1: hipLaunchKernelGGL(k_conv<1>, g, dim3(kConvThreads), 0, stream, c); break;
@@ -854,11 +980,11 @@ struct DenoiserImpl {
         hipLaunchKernelGGL(k_pool, dim3(grid_for(P(2, B) * 32)), dim3(256), 0, stream, a[E2B], B, H >> 1, W >> 1, 32, q2);
         conv(BA, q2, B, 2, true, a[BA], nullptr, false);
         conv(BB, a[BA], B, 2, true, a[BB], nullptr, false);
-        hipLaunchKernelGGL(k_up_concat, dim3(grid_for(P(1, B) * 96)), dim3(256), 0, stream, a[BB], a[E2B], B, H >> 1, W >> 1, 64,
+        hipLaunchKernelGGL(k_up_concat, dim3(grid_for(P(1, B) * 12)), dim3(256), 0, stream, a[BB], a[E2B], B, H >> 1, W >> 1, 64,
                            32, c2);
         conv(D2A, c2, B, 1, true, a[D2A], nullptr, false);
         conv(D2B, a[D2A], B, 1, true, a[D2B], nullptr, false);
-        hipLaunchKernelGGL(k_up_concat, dim3(grid_for(P0 * 48)), dim3(256), 0, stream, a[D2B], a[E1B], B, H, W, 32, 16, c1);
+        hipLaunchKernelGGL(k_up_concat, dim3(grid_for(P0 * 6)), dim3(256), 0, stream, a[D2B], a[E1B], B, H, W, 32, 16, c1);
         conv(D1A, c1, B, 0, true, a[D1A], nullptr, false);
         conv(D1B, a[D1A], B, 0, true, a[D1B], nullptr, false);
         hipLaunchKernelGGL(k_head, dim3(grid_for(P0)), dim3(256), 0, stream, a[D1B], params + hoff, params + hoff + 16, P0, out);
@@ -1000,7 +1126,7 @@ int octopic_denoiser_create(int32_t device, int32_t height, int32_t width, int32
         m.boff[l] = off;
         off += kLayers[l].cout;
         m.cin_s[l] = l == E1A ? 4 : kLayers[l].cin;
-        m.kpad[l] = ceil16(9 * m.cin_s[l]);
+        m.kpad[l] = ceil32(9 * m.cin_s[l]);
     }
     m.hoff = off;
     m.nparams = off + 17;
@@ -1008,7 +1134,7 @@ int octopic_denoiser_create(int32_t device, int32_t height, int32_t width, int32
     bool ok = m.alloc(m.params, m.nparams) && m.alloc(m.grad, m.nparams) && m.alloc(m.m, m.nparams) && m.alloc(m.v, m.nparams);
     for (int l = 0; ok && l < 10; ++l) {
         ok = m.alloc(m.wf[l], (int64_t)kLayers[l].cout * m.kpad[l]) &&
-             (l == E1A || m.alloc(m.wb[l], (int64_t)kLayers[l].cin * 9 * kLayers[l].cout));
+             (l == E1A || m.alloc(m.wb[l], (int64_t)kLayers[l].cin * ceil32(9 * kLayers[l].cout)));
         const int64_t Pl = m.level_of(l) == 0 ? P0 : m.level_of(l) == 1 ? P1 : P2;
         ok = ok && m.alloc(m.a[l], Pl * kLayers[l].cout);
     }
