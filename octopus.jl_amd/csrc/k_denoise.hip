@@ -190,6 +190,114 @@ __global__ __launch_bounds__(kConvThreads) void k_conv8(ConvArgs a) {
         }
 }
 
+// The same product on 16 x 16-pixel tiles of one plane: the tile and its one-pixel halo (all input
+// channels, pixel stride cin + 8 to spread the banks) and the weights are staged in LDS once, so each
+// input element is read from memory once instead of once per tap. Wave w takes tile rows 4w..4w+3
+// (one 16-pixel MFMA row tile each). Grid: planes x ceil(H/16) x ceil(W/16).
+constexpr int kTile = 16, kHalo = kTile + 2;
+
+__host__ __device__ inline size_t conv_tile_lds(int cin, int cout, int kpad) {
+    return (size_t)kHalo * kHalo * (cin + 8) * 2 + (size_t)cout * (kpad + 8) * 2;
+}
+
+// Staging of n 16-byte chunks into LDS, eight independent loads in flight per thread before their
+// stores (a load-store pair per iteration would serialise on the memory latency).
+template <class Src>
+__device__ __forceinline__ void stage16(uint16_t *lds_base, int n, const Src &src) {
+    for (int e0 = threadIdx.x; e0 < n; e0 += 8 * kConvThreads) {
+        uint4 v[8];
+        int dst[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int e = e0 + j * kConvThreads;
+            dst[j] = -1;
+            if (e < n) v[j] = src(e, dst[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (dst[j] >= 0) *reinterpret_cast<uint4 *>(lds_base + dst[j]) = v[j];
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(kConvThreads) void k_conv_tile(ConvArgs a) {
+    extern __shared__ __align__(16) uint16_t lds[];
+    const int cs = a.cin + 8, ldw = a.kpad + 8;
+    uint16_t *xt = lds, *wl = lds + kHalo * kHalo * cs;
+    const int tiles_x = (a.W + kTile - 1) / kTile, tiles_y = (a.H + kTile - 1) / kTile;
+    const int b = blockIdx.x / (tiles_x * tiles_y), tr = blockIdx.x - b * tiles_x * tiles_y;
+    const int ty0 = (tr / tiles_x) * kTile, tx0 = (tr % tiles_x) * kTile;
+    {
+        const int per_row = a.kpad / 8;
+        stage16(wl, a.cout * per_row, [&](int e, int &dst) {
+            const int r = e / per_row, c8 = e - r * per_row;
+            dst = r * ldw + c8 * 8;
+            return *reinterpret_cast<const uint4 *>(a.w + (int64_t)r * a.kpad + c8 * 8);
+        });
+        const int g8 = a.cin / 8;
+        stage16(xt, kHalo * kHalo * g8, [&](int e, int &dst) {
+            const int pix = e / g8, c8 = e - pix * g8, hy = pix / kHalo, hx = pix - hy * kHalo;
+            const int yy = ty0 + hy - 1, xx = tx0 + hx - 1;
+            dst = pix * cs + c8 * 8;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                v = *reinterpret_cast<const uint4 *>(a.x + (((int64_t)b * a.H + yy) * a.W + xx) * a.cin + c8 * 8);
+            return v;
+        });
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 15;
+    const int kq = (lane >> 4) * 8, kmax = 9 * a.cin;
+    int tap = kq / a.cin, c = kq - tap * a.cin;
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t *wrow = wl + col * ldw;
+    for (int k0 = 0; k0 < a.kpad; k0 += 32) {
+        const int k = k0 + kq;
+        bf16x8 av[4];
+        if (k < kmax) {
+            const int dy = tap / 3, dx = tap - (tap / 3) * 3;   // halo coordinates: +1 offset folded in
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+                av[mt] = *reinterpret_cast<const bf16x8 *>(xt + ((wave * 4 + mt + dy) * kHalo + col + dx) * cs + c);
+        } else {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) av[mt] = bf16x8{};
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(wrow + t * 16 * ldw + k);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[mt], bv, acc[mt][t], 0, 0, 0);
+        }
+        c += 32;
+        while (c >= a.cin) { c -= a.cin; ++tap; }
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const int yy = ty0 + wave * 4 + mt;
+        if (yy >= a.H) continue;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int co = t * 16 + col;
+            const float bb = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int xx = tx0 + (lane >> 4) * 4 + i;
+                if (xx >= a.W) continue;
+                const int64_t q = ((int64_t)b * a.H + yy) * a.W + xx;
+                float v = acc[mt][t][i] + bb;
+                if (a.relu) v = v > 0.f ? v : 0.f;
+                if (a.y) a.y[q * a.cout + co] = f2bf(v);
+                if (a.yf) a.yf[q * a.cout + co] = v;
+            }
+        }
+    }
+}
+
 // Weight gradient partials: part[s][co][k] = sum over the pixels of split s of dy[p][co] x_k(p),
 // with x_k(p) the im2col entry k = tap * cin + ci of pixel p. Grid (kpad/16, cout/16, splits);
 // each of the 4 waves takes a quarter of the split's pixels, the block sums the waves in order.
@@ -938,6 +1046,25 @@ struct DenoiserImpl {
         c.y = y;
         c.yf = yf;
         const int64_t Pn = (int64_t)B * c.H * c.W;
+        if (c.cin % 8 == 0 && conv_tile_lds(c.cin, c.cout, c.kpad) <= 160 * 1024 - 1024) {
+            const size_t lds = conv_tile_lds(c.cin, c.cout, c.kpad);
+            const dim3 gt((unsigned)(B * ((c.H + kTile - 1) / kTile) * ((c.W + kTile - 1) / kTile)));
+#define OCTOPIC_CONVT(NT_)                                                                                       \
+    {                                                                                                           \
+        hipFuncSetAttribute((const void *)k_conv_tile<NT_>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        hipLaunchKernelGGL((k_conv_tile<NT_>), gt, dim3(kConvThreads), lds, stream, c);                         \
+    }
+            switch (c.cout / 16) {
+                case 1: OCTOPIC_CONVT(1) break;
+                case 2: OCTOPIC_CONVT(2) break;
+                case 3: OCTOPIC_CONVT(3) break;
+                case 4: OCTOPIC_CONVT(4) break;
+                case 6: OCTOPIC_CONVT(6) break;
+                default: break;
+            }
+#undef OCTOPIC_CONVT
+            return;
+        }
         if (c.cin % 8 == 0) {
             const size_t lds = (size_t)c.cout * (c.kpad + 8) * sizeof(uint16_t);
             const int nt = c.cout / 16, mt = nt <= 2 ? 4 : 2;
