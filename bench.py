@@ -189,6 +189,8 @@ def main():
                     help="fp32: the FP32 form (deposit weights, FFT, field, kick gather in FP32; FP64 beams)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel HIP event timing")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the launch shapes with the library's autotuner before the warm-up (results are unchanged)")
     ap.add_argument("--instances", type=int, default=0,
                     help="batched form: this many independent bunch-pair instances per step (0: the single pair)")
     ap.add_argument("--one-by-one", action="store_true", help="batched form: collide the instances one at a time")
@@ -223,6 +225,7 @@ def main():
                       seed=0, rng_id=P_CFG["rng_id"], device=dev)
     me = linear6d_matrix(E_CFG["beta"], [2 * math.pi * t for t in E_CFG["tunes"]])
     mp = linear6d_matrix(P_CFG["beta"], [2 * math.pi * t for t in P_CFG["tunes"]])
+    tuned = col.autotune(e, p, trials=3) if args.autotune else None
 
     def turn(t):
         lum = col.collide(e, p, turn=t)
@@ -295,7 +298,7 @@ def main():
                                + (", FP32 form (FP64 beams, Green, moments, luminosity)" if args.precision == "fp32" else "")
                                + ", one collide + linear 6D maps per turn",
                    "arm": args.arm, "precision": args.precision, "global_batch": E_CFG["n"] + P_CFG["n"], "seq_len": NSLICES,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "launch_shape": tuned if tuned is not None else col.launch_shape()},
         "roofline": roof,
         "luminosity_mean": sum(lums) / len(lums) if lums else None,
         "green_cache": {"hits": st["green_hits"], "misses": st["green_misses"], "rebuilds": st["green_rebuilds"]},

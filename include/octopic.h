@@ -223,6 +223,18 @@ int octopic_set_timing_families(octopic_handle *h, const char *families);
 int octopic_kernel_times(const octopic_handle *h, double *ms_total, int64_t *launches, int64_t cap,
                          int64_t *n, char *names, int64_t names_cap);
 
+/* Launch shapes (no counterpart in the reference; results are bit-identical for every value, only
+ * the speed changes): "transverse_tiles" (0 or a power of two <= 128: in-slice tile order, default
+ * 16), "kick_target_blocks" (0..2^20 work items per kick launch, default 1024), "kick_pass_windows"
+ * (-1..2, default -1), "deposit_span" (1024..2^20 source particles per deposit block, default 8192).
+ * octopic_autotune times candidate shapes by coordinate descent (median of `trials` collides per
+ * candidate, after a warming one) on copies of the two beams in a scratch handle with the same
+ * options, so the handle's own Green cache, counters and the beams are untouched, and sets the
+ * fastest shape on the handle; best_ms (optional) receives its collide time. */
+int octopic_set_launch_shape(octopic_handle *h, const char *name, int64_t value);
+int octopic_get_launch_shape(const octopic_handle *h, const char *name, int64_t *value);
+int octopic_autotune(octopic_handle *h, octopic_beam *beam1, octopic_beam *beam2, int32_t trials, double *best_ms);
+
 /* ---------------------------------------------------------------------------------------------
  * Residual denoiser (no counterpart in the reference: the library's learned extension of the plain
  * PIC path).  A three-level U-Net (3x3 convolutions with 16/32/64 channels, ReLU, 2x2 average

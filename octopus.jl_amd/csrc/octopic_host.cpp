@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <numeric>
 #include <cmath>
 #include <cstdio>
@@ -230,6 +231,7 @@ struct octopic_handle {
     // fit (-1: in the FP64 form only, where they pay; 0 never; 1 such chunks; 2 every chunk).
     int kick_target_blocks = 1024;
     int kick_pass_windows = -1;
+    int deposit_span = 8192;    // source particles per deposit block (the LDS mesh is zeroed and flushed once per block)
     // Kick and deposit work queues: per-launch counters and launch orders (larger slices first).
     DBuf<int> kqueue;
     DBuf<int32_t> korder;
@@ -616,6 +618,127 @@ namespace {
 void collide_impl(octopic_handle *h, octopic_beam *b1, octopic_beam *b2, int64_t turn, int compute_lum, double *lum_out);
 
 }  // namespace
+
+// ------------------------------------------------------------------ launch shapes and the autotuner
+namespace {
+struct ShapeKnob {
+    const char *name;
+    int octopic_handle::*field;
+    int64_t lo, hi;
+    bool pow2;
+};
+const ShapeKnob kShapeKnobs[] = {
+    {"transverse_tiles", &octopic_handle::tiles, 0, 128, true},
+    {"kick_target_blocks", &octopic_handle::kick_target_blocks, 0, 1 << 20, false},
+    {"kick_pass_windows", &octopic_handle::kick_pass_windows, -1, 2, false},
+    {"deposit_span", &octopic_handle::deposit_span, 1024, 1 << 20, false},
+};
+const ShapeKnob *find_knob(const char *name) {
+    for (const auto &k : kShapeKnobs)
+        if (name && std::strcmp(k.name, name) == 0) return &k;
+    return nullptr;
+}
+}  // namespace
+
+extern "C" int octopic_set_launch_shape(octopic_handle *h, const char *name, int64_t value) {
+    if (!h || !name) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    const ShapeKnob *k = find_knob(name);
+    if (!k) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, std::string("unknown launch shape '") + name + "'");
+    if (value < k->lo || value > k->hi || (k->pow2 && value > 0 && (value & (value - 1)) != 0))
+        return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, std::string("launch shape '") + name + "' out of range");
+    h->*(k->field) = (int)value;
+    return OCTOPIC_OK;
+}
+
+extern "C" int octopic_get_launch_shape(const octopic_handle *h, const char *name, int64_t *value) {
+    if (!h || !name || !value) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    const ShapeKnob *k = find_knob(name);
+    if (!k) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    *value = h->*(k->field);
+    return OCTOPIC_OK;
+}
+
+extern "C" int octopic_autotune(octopic_handle *h, octopic_beam *beam1, octopic_beam *beam2, int32_t trials, double *best_ms) {
+    if (!h || !beam1 || !beam2 || trials < 1 || trials > 64) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    octopic_handle *t = nullptr;
+    double *buf = nullptr;
+    int rc = OCTOPIC_OK;
+    try {
+        HIPCHECK(hipSetDevice(h->device));
+        if (int r = octopic_create(&h->opt, h->device, &t)) fail(r, "autotune: scratch handle: " + std::string(octopic_last_error(nullptr)));
+        for (const auto &k : kShapeKnobs) t->*(k.field) = h->*(k.field);
+        t->dn = h->dn;
+        t->dn_limit = h->dn_limit;
+        const int64_t n1 = beam1->n, n2 = beam2->n;
+        if (n1 <= 0 || n2 <= 0) fail(OCTOPIC_ERR_INVALID_ARGUMENT, "autotune: empty beam");
+        // originals and working copies of both beams, six coordinates each
+        if (hipMalloc((void **)&buf, sizeof(double) * 12 * (size_t)(n1 + n2)) != hipSuccess) fail(OCTOPIC_ERR_OOM, "autotune: beam copies");
+        double *orig[2][6], *work[2][6];
+        const octopic_beam *src[2] = {beam1, beam2};
+        size_t off = 0;
+        for (int b = 0; b < 2; ++b) {
+            const int64_t n = b ? n2 : n1;
+            const double *const cs[6] = {src[b]->x, src[b]->px, src[b]->y, src[b]->py, src[b]->z, src[b]->pz};
+            for (int c = 0; c < 6; ++c) {
+                orig[b][c] = buf + off; off += (size_t)n;
+                work[b][c] = buf + off; off += (size_t)n;
+                HIPCHECK(hipMemcpy(orig[b][c], cs[c], sizeof(double) * n, hipMemcpyDefault));
+            }
+        }
+        octopic_beam wb[2] = {*beam1, *beam2};
+        for (int b = 0; b < 2; ++b) {
+            wb[b].x = work[b][0]; wb[b].px = work[b][1]; wb[b].y = work[b][2];
+            wb[b].py = work[b][3]; wb[b].z = work[b][4]; wb[b].pz = work[b][5];
+            wb[b].on_device = 1;
+        }
+        // the median collide time of `trials` collides from the same start (after one that warms the
+        // scratch handle's Green cache)
+        auto measure = [&]() {
+            std::vector<double> ms;
+            for (int r = 0; r <= trials; ++r) {
+                for (int b = 0; b < 2; ++b)
+                    for (int c = 0; c < 6; ++c)
+                        HIPCHECK(hipMemcpyAsync(work[b][c], orig[b][c], sizeof(double) * (b ? n2 : n1), hipMemcpyDeviceToDevice,
+                                                t->stream));
+                HIPCHECK(hipStreamSynchronize(t->stream));
+                const auto t0 = std::chrono::steady_clock::now();
+                double lum = 0.0;
+                if (int e = octopic_collide(t, &wb[0], &wb[1], 0, 1, &lum)) fail(e, "autotune: " + t->err);
+                const auto t1 = std::chrono::steady_clock::now();
+                if (r > 0) ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+            }
+            std::sort(ms.begin(), ms.end());
+            return ms[ms.size() / 2];
+        };
+        struct Cand { int octopic_handle::*field; std::vector<int> values; };
+        const std::vector<Cand> cands = {
+            {&octopic_handle::tiles, {8, 16, 32}},
+            {&octopic_handle::kick_target_blocks, {512, 1024, 2048}},
+            {&octopic_handle::deposit_span, {4096, 8192, 16384}},
+            {&octopic_handle::kick_pass_windows, {-1, 0, 1, 2}},
+        };
+        double best = measure();
+        for (const auto &cd : cands) {   // coordinate descent: one knob at a time, keep the fastest value
+            const int keep = t->*(cd.field);
+            int choice = keep;
+            for (int v : cd.values) {
+                if (v == keep) continue;
+                t->*(cd.field) = v;
+                const double m = measure();
+                if (m < best) { best = m; choice = v; }
+            }
+            t->*(cd.field) = choice;
+        }
+        for (const auto &k : kShapeKnobs) h->*(k.field) = t->*(k.field);
+        if (best_ms) *best_ms = best;
+    } catch (Fail &f) {
+        rc = set_err(h, f.code, f.msg);
+    }
+    if (buf) hipFree(buf);
+    if (t) octopic_destroy(t);
+    if (rc == OCTOPIC_OK) h->err.clear();
+    return rc;
+}
 
 extern "C" int octopic_collide(octopic_handle *h, octopic_beam *beam1, octopic_beam *beam2, int64_t turn,
                                int32_t compute_luminosity, double *luminosity) {
@@ -1064,7 +1187,8 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             DepositArgs a{};
             a.planes = h->planes.p; a.nplanes = nfp; a.st = st; a.method = o.deposit; a.ncells = (int)field_cells;
             // ~8k particles per block: the LDS mesh is zeroed and flushed once per block
-            a.chunks = nchunks; a.group = std::max(1, (int)std::min<int64_t>(nchunks, 8192 * (int64_t)nchunks / std::max<int64_t>(bmax, 1)));
+            a.chunks = nchunks;
+            a.group = std::max(1, (int)std::min<int64_t>(nchunks, (int64_t)h->deposit_span * nchunks / std::max<int64_t>(bmax, 1)));
             a.scale = scale; a.charge = h->charge.p; a.charge_stride = charge_stride;
             a.dropped = h->counters.p + 4; a.chunkbox = cbox; a.err = h->errinfo.p;
             a.fp32 = h->fp32 ? 1 : 0;

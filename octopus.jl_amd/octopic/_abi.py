@@ -193,6 +193,9 @@ def declare(lib):
         "octopic_denoiser_train_step": (i32, [vp, vp, vp, i32, C.c_float, vp, vp]),
         "octopic_denoiser_make_pairs": (i32, [vp, C.c_uint64, i64, i32, i64, i32, vp, vp, vp, vp, vp]),
         "octopic_set_denoiser": (i32, [vp, vp, C.c_double]),
+        "octopic_set_launch_shape": (i32, [vp, C.c_char_p, i64]),
+        "octopic_get_launch_shape": (i32, [vp, C.c_char_p, C.POINTER(i64)]),
+        "octopic_autotune": (i32, [vp, C.POINTER(BeamStruct), C.POINTER(BeamStruct), i32, DPTR]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -233,4 +236,7 @@ EXPORTED = [
     "octopic_denoiser_train_step",
     "octopic_denoiser_make_pairs",
     "octopic_set_denoiser",
+    "octopic_set_launch_shape",
+    "octopic_get_launch_shape",
+    "octopic_autotune",
 ]

@@ -173,6 +173,30 @@ class PICCollider:
                                           1 if cl else 0, C.byref(lum)))
         return lum.value
 
+    LAUNCH_SHAPES = ("transverse_tiles", "kick_target_blocks", "kick_pass_windows", "deposit_span")
+
+    def set_launch_shape(self, name: str, value: int):
+        """A launch shape of this handle (results do not depend on it; see octopic_set_launch_shape)."""
+        self._check(lib().octopic_set_launch_shape(self.h, name.encode(), int(value)))
+
+    def launch_shape(self) -> dict:
+        out = {}
+        for n in self.LAUNCH_SHAPES:
+            v = C.c_int64()
+            self._check(lib().octopic_get_launch_shape(self.h, n.encode(), C.byref(v)))
+            out[n] = v.value
+        return out
+
+    def autotune(self, beam1: Beam, beam2: Beam, trials: int = 3) -> dict:
+        """Time candidate launch shapes on copies of the beams (octopic_autotune) and keep the fastest;
+        returns the chosen shape and its collide time in ms. The beams and the Green cache are untouched."""
+        s1, s2 = beam1.struct(), beam2.struct()
+        ms = C.c_double()
+        self._check(lib().octopic_autotune(self.h, C.byref(s1), C.byref(s2), int(trials), C.byref(ms)))
+        out = self.launch_shape()
+        out["collide_ms"] = ms.value
+        return out
+
     def set_denoiser(self, denoiser, monitor_limit: float = 0.0):
         """Attach an octopic.denoise.Denoiser to this collision (None detaches): the field planes are
         then Gaussian-subtracted, denoised and rebuilt before the convolution (octopic_set_denoiser)."""

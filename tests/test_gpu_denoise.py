@@ -275,3 +275,20 @@ def test_denoiser_attachment_is_validated(torch):
         PICCollider(_solver(interaction_grid="node")).set_denoiser(d, 1.0)
     with pytest.raises(ArgumentError):
         PICCollider(_solver()).set_denoiser(d, 0.0)
+
+
+def test_committed_weights_reduce_collide_kick_error(torch):
+    """The committed 64 x 64 TSC weights inside the collision (production flat beams, 15 slices,
+    :slice_pair + :quadratic, FP32 form, 640k e- / 256k p): the e- kicks from the 256k p beam are
+    closer to the kicks from a 4x (1.024M) p beam in FP64 with the denoiser than without it, in x and
+    in y."""
+    import os
+    from octopic import denoise_study as S
+    from octopic import train_denoiser as TD
+    from octopic.denoise import Denoiser
+    w = os.path.join(os.path.dirname(__file__), "..", "octopus.jl_amd", "octopic", "weights", "denoiser_tsc_64.npz")
+    d = Denoiser(64, 64, 96, seed=0)
+    limit = 3.0 * TD.load(d, w)
+    err = S.kick_error(64, d, limit, 640_000, 256_000, 4)
+    assert err["denoised"]["x"] < 0.85 * err["plain"]["x"], err
+    assert err["denoised"]["y"] < 0.9 * err["plain"]["y"], err

@@ -606,3 +606,27 @@ def test_turn_elements_match_oracle(oracle, gpu):
         got = b.to_host()
         for k in range(6):
             assert np.max(np.abs(got[k] - ref[k])) <= 1e-13 * np.max(np.abs(ref[k])), (kind, k)
+
+
+def test_autotune_keeps_results_and_cache(oracle, gpu):
+    """The autotuner picks launch shapes from its candidate sets on a scratch handle: the tuned
+    handle's Green-cache counters and the beams are untouched by tuning, and its next collide is
+    bit-identical to a collide with the default shapes."""
+    solver = PICPoissonSolver(grid=(64, 64), deposit_method="TSC", field_derivative="fourth",
+                              slicing=LongitudinalSlicing(5, method="normal_quantile"))
+    e, p = flat_beams(oracle)
+    tuned = PICCollider(solver)
+    be, bp = Beam.from_host(e, PE, device=gpu), Beam.from_host(p, PP, device=gpu)
+    shape = tuned.autotune(be, bp, trials=1)
+    assert shape["transverse_tiles"] in (8, 16, 32) and shape["kick_target_blocks"] in (512, 1024, 2048)
+    assert shape["deposit_span"] in (4096, 8192, 16384) and shape["kick_pass_windows"] in (-1, 0, 1, 2)
+    assert shape["collide_ms"] > 0
+    assert np.array_equal(be.to_host(), e) and np.array_equal(bp.to_host(), p)
+    st = tuned.stats()
+    assert st["collides"] == 0 and st["green_misses"] == 0
+    plain = PICCollider(solver)
+    ce, cp = Beam.from_host(e, PE, device=gpu), Beam.from_host(p, PP, device=gpu)
+    assert tuned.collide(be, bp) == plain.collide(ce, cp)
+    assert torch.equal(be.coords, ce.coords) and torch.equal(bp.coords, cp.coords)
+    with pytest.raises(Exception):
+        tuned.set_launch_shape("transverse_tiles", 3)
