@@ -290,7 +290,8 @@ int octopic_denoiser_train_step(octopic_denoiser *d, const float *input, const f
  * its monitor (rms input residual) exceeds monitor_limit (the library's fallback: three times the
  * largest monitor seen in training).  Needs interaction_grid = :slice_pair (else UNSUPPORTED), a
  * denoiser of height = ny and width = nx, and max_batch >= the field planes of the widest batch
- * (checked at collide).  The luminosity planes are not touched. */
+ * (checked at collide).  The luminosity planes are not touched.  A denoiser holds the activations of
+ * one collide: handles collided together by octopic_collide_many need denoisers of their own. */
 int octopic_set_denoiser(octopic_handle *h, octopic_denoiser *d, double monitor_limit);
 
 int octopic_denoiser_make_pairs(octopic_denoiser *d, uint64_t seed, int64_t first_sample, int32_t batch,

@@ -831,8 +831,13 @@ extern "C" int octopic_collide_many(octopic_handle *const *handles, octopic_beam
                                     int32_t *status, int32_t count) {
     if (count < 0 || (count > 0 && (!handles || !beams1 || !beams2 || !luminosity))) return OCTOPIC_ERR_INVALID_ARGUMENT;
     std::set<const octopic_handle *> seen;
-    for (int k = 0; k < count; ++k)
+    std::set<const octopic_denoiser *> denoisers;   // a denoiser's buffers serve one collide at a time
+    for (int k = 0; k < count; ++k) {
         if (!handles[k] || !seen.insert(handles[k]).second) return OCTOPIC_ERR_INVALID_ARGUMENT;
+        if (handles[k]->dn && !denoisers.insert(handles[k]->dn).second)
+            return set_err(handles[k], OCTOPIC_ERR_INVALID_ARGUMENT,
+                           "octopic_collide_many: two handles share one denoiser (it holds one collide's activations)");
+    }
     std::vector<int32_t> st(count, OCTOPIC_OK);
     std::vector<std::thread> pool;
     pool.reserve(count);
