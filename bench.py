@@ -62,7 +62,7 @@ def bytes_model(n1, n2, ns1, ns2, npl):
     visits = n1 * ns2 + n2 * ns1          # particle-pair visits of each beam
     return {
         "bounds": 40.0 * visits,             # x, px, y, py, z of both slices of every pair
-        "deposit": 32.0 * (npl + 1) * visits,   # x, px, y, py per field plane + the luminosity plane
+        "deposit": 32.0 * visits,            # x, px, y, py once: every plane of a visit comes from one pre-kick read
         "kick": 88.0 * visits,               # read x, px, y, py, z, pz; write x, px, y, py, pz
         "slicing": 8.0 * 3 * (n1 + n2) + 2 * 48.0 * (n1 + n2),   # z passes + gather of the states
         "store": 2 * 40.0 * (n1 + n2),       # states -> beams (5 coordinates)

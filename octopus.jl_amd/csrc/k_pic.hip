@@ -486,13 +486,101 @@ __device__ __forceinline__ void deposit_item(const DepositArgs &a, unsigned long
     }
 }
 
-// Items are (plane, chunk group).  Persistent blocks (as many as fit the CUs) take items from a
-// per-launch atomic counter, planes of the larger source slices first, until it passes the item count.
+// One work item for the `np` field planes of one interaction (planes plane0 .. plane0 + np - 1: the
+// same source particles on the same mesh at different drifts): each particle's x, px, y, py are read
+// once and deposited into np LDS windows of the same geometry (the box of the sL and sR positions holds
+// every drift between).  When the np windows do not fit, the planes go one at a time through
+// deposit_item.  The charge is the same integer sum either way.
+template <int METHOD, class W>
+__device__ __forceinline__ void deposit_item_multi(const DepositArgs &a, unsigned long long *tile, int plane0, int np, int grp) {
+    const Plane pl = a.planes[plane0];
+    const int c0 = grp * a.group, c1 = min(c0 + a.group, a.chunks);
+    const int64_t lo = pl.n * c0 / a.chunks, hi = pl.n * c1 / a.chunks;
+    if (lo >= hi) return;
+    const int n1 = pl.n1, n2 = pl.n2;
+    int wx0 = 0, wx1 = n1 - 1, wy0 = 0, wy1 = n2 - 1;
+    if (a.chunkbox) {
+        const int seg = 2 * (pl.inter / 2) + pl.beam;
+        double b[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+        for (int c = c0; c < c1; ++c) {
+            const double *bc = a.chunkbox + ((size_t)seg * a.chunks + c) * 8;
+            b[0] = jmin(b[0], bc[0]); b[1] = jmax(b[1], bc[1]); b[2] = jmin(b[2], bc[2]); b[3] = jmax(b[3], bc[3]);
+        }
+        stencil_window<METHOD>((b[0] - pl.x0) * pl.hxi, (b[1] - pl.x0) * pl.hxi, n1, wx0, wx1);
+        stencil_window<METHOD>((b[2] - pl.y0) * pl.hyi, (b[3] - pl.y0) * pl.hyi, n2, wy0, wy1);
+    }
+    const int wnx = wx1 - wx0 + 1, wcells = wnx * (wy1 - wy0 + 1);
+    if (np * wcells > a.lds_cells) {   // block-uniform
+        for (int p = 0; p < np; ++p) {
+            deposit_item<METHOD, W>(a, tile, plane0 + p, grp);
+            __syncthreads();   // the plane's window is flushed before the next zeroes it
+        }
+        return;
+    }
+    for (int c = threadIdx.x; c < np * wcells; c += blockDim.x) tile[c] = 0ull;
+    __syncthreads();
+    const double *X = a.st.c[pl.beam][0] + pl.off, *PX = a.st.c[pl.beam][1] + pl.off, *Y = a.st.c[pl.beam][2] + pl.off,
+                 *PY = a.st.c[pl.beam][3] + pl.off;
+    const double x0 = pl.x0, y0 = pl.y0, hxi = pl.hxi, hyi = pl.hyi, scale = a.scale;
+    double sp[3];
+    int cb[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        sp[p] = p < np ? a.planes[plane0 + p].s : 0.0;
+        cb[p] = p < np ? a.planes[plane0 + p].count_box : 0;
+    }
+    constexpr int K = METHOD == 0 ? 2 : 3;
+    int dropped[3] = {0, 0, 0};
+    for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+        const double xs = X[t], pxs = PX[t], ys = Y[t], pys = PY[t];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            if (p >= np) break;
+            const double xd = xs + pxs * sp[p];
+            const double yd = ys + pys * sp[p];
+            if (cb[p] && !(__builtin_isfinite(xd) && x0 <= xd && xd <= pl.bx1 && __builtin_isfinite(yd) && y0 <= yd &&
+                           yd <= pl.by1))
+                ++dropped[p];
+            int bx, by;
+            double wx[3], wy[3];
+            stencil<METHOD>((xd - x0) * hxi, n1, bx, wx);
+            stencil<METHOD>((yd - y0) * hyi, n2, by, wy);
+            unsigned long long *tp = tile + p * wcells;
+#pragma unroll
+            for (int m = 0; m < K; ++m)
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    unsigned long long v;
+                    if constexpr (sizeof(W) == sizeof(double)) v = fixed_point(wx[m] * wy[q] * scale);
+                    else v = fixed_point((double)((float)wx[m] * (float)wy[q] * (float)scale));
+                    atomicAdd(&tp[v ? (by + q - wy0) * wnx + (bx + m - wx0) : 0], v);
+                }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        if (p >= np || !cb[p]) continue;
+        int d = dropped[p];
+        for (int o = 32; o > 0; o >>= 1) d += __shfl_down(d, o);
+        if ((threadIdx.x & 63) == 0 && d) atomicAdd(&a.dropped[0], (unsigned long long)d);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < np * wcells; c += blockDim.x) {
+        const int p = c / wcells, cc = c - p * wcells, r = cc / wnx;
+        const unsigned long long v = tile[c];
+        if (v) atomicAdd(&a.charge[(size_t)(plane0 + p) * a.charge_stride + (wy0 + r) * n1 + wx0 + (cc - r * wnx)], v);
+    }
+}
+
+// Items are (plane, chunk group), or (interaction, chunk group) when a.multi > 1 (its a.multi field
+// planes together).  Persistent blocks (as many as fit the CUs) take items from a per-launch atomic
+// counter, the larger source slices first, until it passes the item count.
 template <int METHOD, class W>
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
     extern __shared__ unsigned long long tile[];
     if (failed(a.err)) return;
-    const int ngroups = (a.chunks + a.group - 1) / a.group, total = ngroups * a.nplanes;
+    const int units = a.multi > 1 ? a.nplanes / a.multi : a.nplanes;
+    const int ngroups = (a.chunks + a.group - 1) / a.group, total = ngroups * units;
     __shared__ int next_item;
     for (;;) {
         if (threadIdx.x == 0) next_item = atomicAdd(a.queue, 1);
@@ -501,7 +589,10 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) {
         __syncthreads();   // every thread has read next_item before thread 0 takes the next one
         if (item >= total) break;
         const int pos = item / ngroups;
-        deposit_item<METHOD, W>(a, tile, a.order[pos], item - pos * ngroups);
+        if (a.multi > 1)   // the planes of one interaction are consecutive in the launch order
+            deposit_item_multi<METHOD, W>(a, tile, a.order[pos * a.multi], a.multi, item - pos * ngroups);
+        else
+            deposit_item<METHOD, W>(a, tile, a.order[pos], item - pos * ngroups);
         __syncthreads();   // the item's LDS window is flushed before the next item zeroes it
     }
 }
@@ -511,7 +602,8 @@ hipError_t launch_deposit(DepositArgs a, hipStream_t s) {
     a.lds_cells = a.ncells < kDepositMaxLdsCells ? a.ncells : kDepositMaxLdsCells;
     const size_t lds = (size_t)a.lds_cells * sizeof(unsigned long long);
     if (!a.queue || !a.order) return hipErrorInvalidValue;
-    const int total = (a.chunks + a.group - 1) / a.group * a.nplanes;
+    if (a.multi > 3 || (a.multi > 1 && a.nplanes % a.multi != 0)) return hipErrorInvalidValue;
+    const int total = (a.chunks + a.group - 1) / a.group * (a.multi > 1 ? a.nplanes / a.multi : a.nplanes);
     int dev = 0, cus = 0;
     hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;

@@ -131,6 +131,8 @@ struct DepositArgs {
     int lds_cells;               // set by the launcher: LDS window capacity in cells
     int *queue;                  // work-queue counter of this launch (zero on entry)
     const int32_t *order;        // launch order of the planes (position -> plane)
+    int multi;                   // > 1: items take this many consecutive planes of one interaction (same
+                                 // mesh and particles, 2 or 3), which the order lists together
 };
 constexpr int kDepositMaxLdsCells = 20480 - 2;   // 160 KB per block, less 16 bytes of the block's own LDS
 hipError_t launch_deposit(DepositArgs a, hipStream_t s);

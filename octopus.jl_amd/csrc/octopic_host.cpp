@@ -1199,6 +1199,8 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.fp32 = h->fp32 ? 1 : 0;
             a.queue = h->dqueue.p + 2 * bi;
             a.order = h->dorder.p + (size_t)(npl + 1) * 2 * boff[bi];
+            // slice-pair meshes: the planes of an interaction share mesh and particles, one read feeds them all
+            a.multi = h->mode == 2 ? 1 : npl;
             HIPCHECK(launch_deposit(a, s));
             if (compute_lum) {
                 DepositArgs l = a;
@@ -1207,6 +1209,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
                 l.charge = h->charge.p + (size_t)nfp * charge_stride;
                 l.queue = h->dqueue.p + 2 * bi + 1;
                 l.order = a.order + nfp;
+                l.multi = 1;
                 HIPCHECK(launch_deposit(l, s));
             }
         }
