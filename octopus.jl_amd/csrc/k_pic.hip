@@ -494,6 +494,8 @@ __device__ __forceinline__ void deposit_item(const DepositArgs &a, unsigned long
 template <int METHOD, class W>
 __device__ __forceinline__ void deposit_item_multi(const DepositArgs &a, unsigned long long *tile, int plane0, int np, int grp) {
     const Plane pl = a.planes[plane0];
+    // the interaction's luminosity plane (a.lum_plane0 + interaction), on its own mesh, when folded in
+    const int lplane = a.lum_plane0 >= 0 ? a.lum_plane0 + plane0 / np : -1;
     const int c0 = grp * a.group, c1 = min(c0 + a.group, a.chunks);
     const int64_t lo = pl.n * c0 / a.chunks, hi = pl.n * c1 / a.chunks;
     if (lo >= hi) return;
@@ -510,14 +512,38 @@ __device__ __forceinline__ void deposit_item_multi(const DepositArgs &a, unsigne
         stencil_window<METHOD>((b[2] - pl.y0) * pl.hyi, (b[3] - pl.y0) * pl.hyi, n2, wy0, wy1);
     }
     const int wnx = wx1 - wx0 + 1, wcells = wnx * (wy1 - wy0 + 1);
-    if (np * wcells > a.lds_cells) {   // block-uniform
+    // luminosity window: the same chunk box (its sM drift lies between sL and sR) on the luminosity mesh
+    Plane lp{};
+    int lx0 = 0, ly0 = 0, lnx = 0, lcells = 0;
+    if (lplane >= 0) {
+        lp = a.planes[lplane];
+        int lx1 = lp.n1 - 1, ly1 = lp.n2 - 1;
+        lx0 = 0; ly0 = 0;
+        if (a.chunkbox) {
+            const int seg = 2 * (pl.inter / 2) + pl.beam;
+            double b[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+            for (int c = c0; c < c1; ++c) {
+                const double *bc = a.chunkbox + ((size_t)seg * a.chunks + c) * 8;
+                b[0] = jmin(b[0], bc[0]); b[1] = jmax(b[1], bc[1]); b[2] = jmin(b[2], bc[2]); b[3] = jmax(b[3], bc[3]);
+            }
+            stencil_window<METHOD>((b[0] - lp.x0) * lp.hxi, (b[1] - lp.x0) * lp.hxi, lp.n1, lx0, lx1);
+            stencil_window<METHOD>((b[2] - lp.y0) * lp.hyi, (b[3] - lp.y0) * lp.hyi, lp.n2, ly0, ly1);
+        }
+        lnx = lx1 - lx0 + 1;
+        lcells = lnx * (ly1 - ly0 + 1);
+    }
+    if (np * wcells + lcells > a.lds_cells) {   // block-uniform
         for (int p = 0; p < np; ++p) {
             deposit_item<METHOD, W>(a, tile, plane0 + p, grp);
             __syncthreads();   // the plane's window is flushed before the next zeroes it
         }
+        if (lplane >= 0) {
+            deposit_item<METHOD, double>(a, tile, lplane, grp);   // luminosity deposits stay FP64
+            __syncthreads();
+        }
         return;
     }
-    for (int c = threadIdx.x; c < np * wcells; c += blockDim.x) tile[c] = 0ull;
+    for (int c = threadIdx.x; c < np * wcells + lcells; c += blockDim.x) tile[c] = 0ull;
     __syncthreads();
     const double *X = a.st.c[pl.beam][0] + pl.off, *PX = a.st.c[pl.beam][1] + pl.off, *Y = a.st.c[pl.beam][2] + pl.off,
                  *PY = a.st.c[pl.beam][3] + pl.off;
@@ -530,7 +556,7 @@ __device__ __forceinline__ void deposit_item_multi(const DepositArgs &a, unsigne
         cb[p] = p < np ? a.planes[plane0 + p].count_box : 0;
     }
     constexpr int K = METHOD == 0 ? 2 : 3;
-    int dropped[3] = {0, 0, 0};
+    int dropped[3] = {0, 0, 0}, ldropped = 0;
     for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
         const double xs = X[t], pxs = PX[t], ys = Y[t], pys = PY[t];
 #pragma unroll
@@ -556,6 +582,30 @@ __device__ __forceinline__ void deposit_item_multi(const DepositArgs &a, unsigne
                     atomicAdd(&tp[v ? (by + q - wy0) * wnx + (bx + m - wx0) : 0], v);
                 }
         }
+        if (lplane >= 0) {   // the luminosity plane at sM, FP64 weights
+            const double xd = xs + pxs * lp.s;
+            const double yd = ys + pys * lp.s;
+            if (lp.count_box && !(__builtin_isfinite(xd) && lp.x0 <= xd && xd <= lp.bx1 && __builtin_isfinite(yd) &&
+                                  lp.y0 <= yd && yd <= lp.by1))
+                ++ldropped;
+            int bx, by;
+            double wx[3], wy[3];
+            stencil<METHOD>((xd - lp.x0) * lp.hxi, lp.n1, bx, wx);
+            stencil<METHOD>((yd - lp.y0) * lp.hyi, lp.n2, by, wy);
+            unsigned long long *tl = tile + np * wcells;
+#pragma unroll
+            for (int m = 0; m < K; ++m)
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const unsigned long long v = fixed_point(wx[m] * wy[q] * scale);
+                    atomicAdd(&tl[v ? (by + q - ly0) * lnx + (bx + m - lx0) : 0], v);
+                }
+        }
+    }
+    if (lplane >= 0 && lp.count_box) {
+        int d = ldropped;
+        for (int o = 32; o > 0; o >>= 1) d += __shfl_down(d, o);
+        if ((threadIdx.x & 63) == 0 && d) atomicAdd(&a.dropped[0], (unsigned long long)d);
     }
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
@@ -569,6 +619,11 @@ __device__ __forceinline__ void deposit_item_multi(const DepositArgs &a, unsigne
         const int p = c / wcells, cc = c - p * wcells, r = cc / wnx;
         const unsigned long long v = tile[c];
         if (v) atomicAdd(&a.charge[(size_t)(plane0 + p) * a.charge_stride + (wy0 + r) * n1 + wx0 + (cc - r * wnx)], v);
+    }
+    for (int c = threadIdx.x; c < lcells; c += blockDim.x) {
+        const int r = c / lnx;
+        const unsigned long long v = tile[np * wcells + c];
+        if (v) atomicAdd(&a.charge[(size_t)lplane * a.charge_stride + (ly0 + r) * lp.n1 + lx0 + (c - r * lnx)], v);
     }
 }
 

@@ -133,6 +133,8 @@ struct DepositArgs {
     const int32_t *order;        // launch order of the planes (position -> plane)
     int multi;                   // > 1: items take this many consecutive planes of one interaction (same
                                  // mesh and particles, 2 or 3), which the order lists together
+    int lum_plane0;              // >= 0 (with multi > 1): interaction d's luminosity plane lum_plane0 + d is
+                                 // deposited by the same items (same particles at sM, same stencil method)
 };
 constexpr int kDepositMaxLdsCells = 20480 - 2;   // 160 KB per block, less 16 bytes of the block's own LDS
 hipError_t launch_deposit(DepositArgs a, hipStream_t s);

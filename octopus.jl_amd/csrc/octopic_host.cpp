@@ -1201,8 +1201,11 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.order = h->dorder.p + (size_t)(npl + 1) * 2 * boff[bi];
             // slice-pair meshes: the planes of an interaction share mesh and particles, one read feeds them all
             a.multi = h->mode == 2 ? 1 : npl;
+            // the luminosity planes ride along when they use the field deposit's stencil
+            const bool lum_fold = compute_lum && a.multi > 1 && h->lum_dep == o.deposit;
+            a.lum_plane0 = lum_fold ? nfp : -1;
             HIPCHECK(launch_deposit(a, s));
-            if (compute_lum) {
+            if (compute_lum && !lum_fold) {
                 DepositArgs l = a;
                 l.fp32 = 0;   // luminosity deposits stay FP64
                 l.planes = h->planes.p + nfp; l.nplanes = 2 * B; l.method = h->lum_dep; l.ncells = (int)lum_cells;
@@ -1210,6 +1213,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
                 l.queue = h->dqueue.p + 2 * bi + 1;
                 l.order = a.order + nfp;
                 l.multi = 1;
+                l.lum_plane0 = -1;
                 HIPCHECK(launch_deposit(l, s));
             }
         }
