@@ -198,8 +198,9 @@ def main():
                     help="only run plain device copies of 1 GiB (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; the collision path has no multi-GPU "
-                                   "implementation (see DESIGN.md for how it would shard)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the collision path runs on one GPU: there is no multi-GPU implementation "
+                                   "(particle decomposition with one integer all-reduce of the charge planes per "
+                                   "batch is the design, not built)", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch
