@@ -298,6 +298,86 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_tile(ConvArgs a) {
     }
 }
 
+// The same on 8 x 8-pixel tiles (for the coarse levels, where 16 x 16 tiles leave CUs idle): each of
+// the 4 waves takes one 16-pixel MFMA row tile made of two 8-pixel rows (rows 2w, 2w + 1).
+constexpr int kTile8 = 8, kHalo8 = kTile8 + 2;
+// 16 x 16 tiles from this many tiles up, 8 x 8 below (measured on 90 planes of 64 x 64: the coarse
+// levels run 11-13 us with 8 x 8 tiles against 16-22 us with 16 x 16; the full level prefers 16 x 16,
+// 16.9 against 20 us)
+constexpr int64_t kConvTile16From = 1024;
+
+__host__ __device__ inline size_t conv_tile8_lds(int cin, int cout, int kpad) {
+    return (size_t)kHalo8 * kHalo8 * (cin + 8) * 2 + (size_t)cout * (kpad + 8) * 2;
+}
+
+template <int NT>
+__global__ __launch_bounds__(kConvThreads) void k_conv_tile8(ConvArgs a) {
+    extern __shared__ __align__(16) uint16_t lds[];
+    const int cs = a.cin + 8, ldw = a.kpad + 8;
+    uint16_t *xt = lds, *wl = lds + kHalo8 * kHalo8 * cs;
+    const int tiles_x = (a.W + kTile8 - 1) / kTile8, tiles_y = (a.H + kTile8 - 1) / kTile8;
+    const int b = blockIdx.x / (tiles_x * tiles_y), tr = blockIdx.x - b * tiles_x * tiles_y;
+    const int ty0 = (tr / tiles_x) * kTile8, tx0 = (tr % tiles_x) * kTile8;
+    {
+        const int per_row = a.kpad / 8;
+        stage16(wl, a.cout * per_row, [&](int e, int &dst) {
+            const int r = e / per_row, c8 = e - r * per_row;
+            dst = r * ldw + c8 * 8;
+            return *reinterpret_cast<const uint4 *>(a.w + (int64_t)r * a.kpad + c8 * 8);
+        });
+        const int g8 = a.cin / 8;
+        stage16(xt, kHalo8 * kHalo8 * g8, [&](int e, int &dst) {
+            const int pix = e / g8, c8 = e - pix * g8, hy = pix / kHalo8, hx = pix - hy * kHalo8;
+            const int yy = ty0 + hy - 1, xx = tx0 + hx - 1;
+            dst = pix * cs + c8 * 8;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                v = *reinterpret_cast<const uint4 *>(a.x + (((int64_t)b * a.H + yy) * a.W + xx) * a.cin + c8 * 8);
+            return v;
+        });
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ar = wave * 2 + ((lane & 15) >> 3), ac = lane & 7;   // this lane's A row pixel in the tile
+    const int kq = (lane >> 4) * 8, kmax = 9 * a.cin;
+    int tap = kq / a.cin, c = kq - tap * a.cin;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t *wrow = wl + (lane & 15) * ldw;
+    for (int k0 = 0; k0 < a.kpad; k0 += 32) {
+        const int k = k0 + kq;
+        bf16x8 av = bf16x8{};
+        if (k < kmax) {
+            const int dy = tap / 3, dx = tap - (tap / 3) * 3;
+            av = *reinterpret_cast<const bf16x8 *>(xt + ((ar + dy) * kHalo8 + ac + dx) * cs + c);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(wrow + t * 16 * ldw + k);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+        }
+        c += 32;
+        while (c >= a.cin) { c -= a.cin; ++tap; }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + (lane & 15);
+        const float bb = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int pix = (lane >> 4) * 4 + i;   // output row of the 16-pixel tile
+            const int yy = ty0 + wave * 2 + (pix >> 3), xx = tx0 + (pix & 7);
+            if (yy >= a.H || xx >= a.W) continue;
+            const int64_t q = ((int64_t)b * a.H + yy) * a.W + xx;
+            float v = acc[t][i] + bb;
+            if (a.relu) v = v > 0.f ? v : 0.f;
+            if (a.y) a.y[q * a.cout + co] = f2bf(v);
+            if (a.yf) a.yf[q * a.cout + co] = v;
+        }
+    }
+}
+
 // Weight gradient partials: part[s][co][k] = sum over the pixels of split s of dy[p][co] x_k(p),
 // with x_k(p) the im2col entry k = tap * cin + ci of pixel p. Grid (kpad/16, cout/16, splits);
 // each of the 4 waves takes a quarter of the split's pixels, the block sums the waves in order.
@@ -1046,6 +1126,26 @@ struct DenoiserImpl {
         c.y = y;
         c.yf = yf;
         const int64_t Pn = (int64_t)B * c.H * c.W;
+        const int64_t tiles16 = (int64_t)B * ((c.H + kTile - 1) / kTile) * ((c.W + kTile - 1) / kTile);
+        if (c.cin % 8 == 0 && tiles16 < kConvTile16From && conv_tile8_lds(c.cin, c.cout, c.kpad) <= 160 * 1024 - 1024) {
+            const size_t lds = conv_tile8_lds(c.cin, c.cout, c.kpad);
+            const dim3 gt((unsigned)(B * ((c.H + kTile8 - 1) / kTile8) * ((c.W + kTile8 - 1) / kTile8)));
+#define OCTOPIC_CONVT8(NT_)                                                                                       \
+    {                                                                                                            \
+        hipFuncSetAttribute((const void *)k_conv_tile8<NT_>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        hipLaunchKernelGGL((k_conv_tile8<NT_>), gt, dim3(kConvThreads), lds, stream, c);                         \
+    }
+            switch (c.cout / 16) {
+                case 1: OCTOPIC_CONVT8(1) break;
+                case 2: OCTOPIC_CONVT8(2) break;
+                case 3: OCTOPIC_CONVT8(3) break;
+                case 4: OCTOPIC_CONVT8(4) break;
+                case 6: OCTOPIC_CONVT8(6) break;
+                default: break;
+            }
+#undef OCTOPIC_CONVT8
+            return;
+        }
         if (c.cin % 8 == 0 && conv_tile_lds(c.cin, c.cout, c.kpad) <= 160 * 1024 - 1024) {
             const size_t lds = conv_tile_lds(c.cin, c.cout, c.kpad);
             const dim3 gt((unsigned)(B * ((c.H + kTile - 1) / kTile) * ((c.W + kTile - 1) / kTile)));

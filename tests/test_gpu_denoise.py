@@ -48,7 +48,9 @@ def test_param_count_and_roundtrip(torch):
     assert np.array_equal(d.get_params(), q)
 
 
-@pytest.mark.parametrize("hw,b", [((16, 16), 2), ((32, 48), 3)])
+# small batches take the 8 x 8-pixel tile kernel at every level; 64 planes of 64 x 64 take the 16 x 16
+# one at full resolution (1024 tiles and more)
+@pytest.mark.parametrize("hw,b", [((16, 16), 2), ((32, 48), 3), ((64, 64), 64)])
 def test_forward_matches_oracle(torch, hw, b):
     h, w = hw
     d = _make(torch, h, w, b, seed=11)
