@@ -53,7 +53,23 @@ struct ConvArgs {
     int B, H, W, cin, cout, kpad, relu;
     uint16_t *y;            // bf16 activation [P][cout], or null
     float *yf;              // fp32 output [P][cout], or null
+    // two-source input (the decoder's upsampling and skip concatenation, never materialised): with x2
+    // set, input channels [0, cd) are x at half resolution ([B][H/2][W/2][cd], nearest upsampling) and
+    // channels [cd, cin) are x2 ([P][cin - cd])
+    const uint16_t *x2;
+    int cd;
+    uint16_t *yq;           // 2x2 average pool of the bf16 activation [B][H/2][W/2][cout], or null (tile kernels)
 };
+
+// 8 input channels c8*8 .. c8*8+7 of pixel (b, yy, xx) as one 16-byte load (cd and cin - cd are multiples of 8)
+__device__ __forceinline__ uint4 load_in8(const ConvArgs &a, int b, int yy, int xx, int c) {
+    if (a.x2) {
+        if (c < a.cd)
+            return *reinterpret_cast<const uint4 *>(a.x + (((int64_t)b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.cd + c);
+        return *reinterpret_cast<const uint4 *>(a.x2 + (((int64_t)b * a.H + yy) * a.W + xx) * (a.cin - a.cd) + (c - a.cd));
+    }
+    return *reinterpret_cast<const uint4 *>(a.x + (((int64_t)b * a.H + yy) * a.W + xx) * a.cin + c);
+}
 
 template <int NT>
 __global__ __launch_bounds__(kConvThreads) void k_conv(ConvArgs a) {
@@ -150,9 +166,10 @@ __global__ __launch_bounds__(kConvThreads) void k_conv8(ConvArgs a) {
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
             const int yy = py[mt] + dy, xx = px[mt] + dx;
-            if (kin && pv[mt] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-                av[mt] = *reinterpret_cast<const bf16x8 *>(a.x + (((int64_t)pb[mt] * a.H + yy) * a.W + xx) * a.cin + cc);
-            else
+            if (kin && pv[mt] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+                const uint4 u = load_in8(a, pb[mt], yy, xx, cc);
+                av[mt] = *reinterpret_cast<const bf16x8 *>(&u);
+            } else
                 av[mt] = bf16x8{};
         }
     };
@@ -240,8 +257,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_tile(ConvArgs a) {
             const int yy = ty0 + hy - 1, xx = tx0 + hx - 1;
             dst = pix * cs + c8 * 8;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-                v = *reinterpret_cast<const uint4 *>(a.x + (((int64_t)b * a.H + yy) * a.W + xx) * a.cin + c8 * 8);
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = load_in8(a, b, yy, xx, c8 * 8);
             return v;
         });
     }
@@ -276,10 +292,11 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_tile(ConvArgs a) {
         c += 32;
         while (c >= a.cin) { c -= a.cin; ++tap; }
     }
+    // activations: lane holds rows wave*4 + mt, columns 4(lane/16) + i of the tile, channel col (+16 t)
+    uint16_t hv[4][NT][4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
         const int yy = ty0 + wave * 4 + mt;
-        if (yy >= a.H) continue;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int co = t * 16 + col;
@@ -287,13 +304,31 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_tile(ConvArgs a) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int xx = tx0 + (lane >> 4) * 4 + i;
-                if (xx >= a.W) continue;
-                const int64_t q = ((int64_t)b * a.H + yy) * a.W + xx;
                 float v = acc[mt][t][i] + bb;
                 if (a.relu) v = v > 0.f ? v : 0.f;
-                if (a.y) a.y[q * a.cout + co] = f2bf(v);
+                hv[mt][t][i] = f2bf(v);
+                if (yy >= a.H || xx >= a.W) continue;
+                const int64_t q = ((int64_t)b * a.H + yy) * a.W + xx;
+                if (a.y) a.y[q * a.cout + co] = hv[mt][t][i];
                 if (a.yf) a.yf[q * a.cout + co] = v;
             }
+        }
+    }
+    if (a.yq) {   // 2x2 average of the bf16 activations, summed as (top pair) + (bottom pair)
+        const int h2 = a.H >> 1, w2 = a.W >> 1;
+#pragma unroll
+        for (int mp = 0; mp < 4; mp += 2) {
+            const int y2 = (ty0 + wave * 4 + mp) >> 1;
+            if (y2 >= h2) continue;
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; i += 2) {
+                    const int x2 = (tx0 + (lane >> 4) * 4 + i) >> 1;
+                    if (x2 >= w2) continue;
+                    const float sum = (bf2f(hv[mp][t][i]) + bf2f(hv[mp][t][i + 1])) + (bf2f(hv[mp + 1][t][i]) + bf2f(hv[mp + 1][t][i + 1]));
+                    a.yq[(((int64_t)b * h2 + y2) * w2 + x2) * a.cout + t * 16 + col] = f2bf(0.25f * sum);
+                }
         }
     }
 }
@@ -331,8 +366,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_tile8(ConvArgs a) {
             const int yy = ty0 + hy - 1, xx = tx0 + hx - 1;
             dst = pix * cs + c8 * 8;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-                v = *reinterpret_cast<const uint4 *>(a.x + (((int64_t)b * a.H + yy) * a.W + xx) * a.cin + c8 * 8);
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = load_in8(a, b, yy, xx, c8 * 8);
             return v;
         });
     }
@@ -364,16 +398,34 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_tile8(ConvArgs a) {
     for (int t = 0; t < NT; ++t) {
         const int co = t * 16 + (lane & 15);
         const float bb = a.bias ? a.bias[co] : 0.f;
+        uint16_t hv[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int pix = (lane >> 4) * 4 + i;   // output row of the 16-pixel tile
             const int yy = ty0 + wave * 2 + (pix >> 3), xx = tx0 + (pix & 7);
-            if (yy >= a.H || xx >= a.W) continue;
-            const int64_t q = ((int64_t)b * a.H + yy) * a.W + xx;
             float v = acc[t][i] + bb;
             if (a.relu) v = v > 0.f ? v : 0.f;
-            if (a.y) a.y[q * a.cout + co] = f2bf(v);
+            hv[i] = f2bf(v);
+            if (yy >= a.H || xx >= a.W) continue;
+            const int64_t q = ((int64_t)b * a.H + yy) * a.W + xx;
+            if (a.y) a.y[q * a.cout + co] = hv[i];
             if (a.yf) a.yf[q * a.cout + co] = v;
+        }
+        if (a.yq) {   // the row below sits in lane + 32 (pixels 8..15 of the tile); pairs i, i + 1 in this lane
+            int below[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) below[i] = __shfl_down((int)hv[i], 32, 64);
+            if ((lane >> 4) < 2) {
+                const int h2 = a.H >> 1, w2 = a.W >> 1;
+                const int y2 = (ty0 + wave * 2) >> 1;
+#pragma unroll
+                for (int i = 0; i < 4; i += 2) {
+                    const int x2 = (tx0 + (lane >> 4) * 4 + i) >> 1;
+                    if (y2 >= h2 || x2 >= w2) continue;
+                    const float sum = (bf2f(hv[i]) + bf2f(hv[i + 1])) + (bf2f((uint16_t)below[i]) + bf2f((uint16_t)below[i + 1]));
+                    a.yq[(((int64_t)b * h2 + y2) * w2 + x2) * a.cout + co] = f2bf(0.25f * sum);
+                }
+            }
         }
     }
 }
@@ -382,7 +434,9 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_tile8(ConvArgs a) {
 // with x_k(p) the im2col entry k = tap * cin + ci of pixel p. Grid (kpad/16, cout/16, splits);
 // each of the 4 waves takes a quarter of the split's pixels, the block sums the waves in order.
 struct WgradArgs {
-    const uint16_t *x;      // [P][cin] bf16
+    const uint16_t *x;      // [P][cin] bf16 (or the half-resolution first source, see ConvArgs)
+    const uint16_t *x2;     // second source of a two-source input, or null
+    int cd;
     const uint16_t *dy;     // [P][cout] bf16
     int B, H, W, cin, cout, kpad;
     int64_t split_pixels;   // pixels per split (multiple of 64)
@@ -419,8 +473,14 @@ __global__ __launch_bounds__(kConvThreads) void k_wgrad(WgradArgs a) {
                 if (kvalid) {
                     const int b = (int)(p / HW), rem = (int)(p - (int64_t)b * HW), py = rem / a.W, px = rem - py * a.W;
                     const int yy = py + dyo, xx = px + dxo;
-                    if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-                        vb = (short)a.x[(((int64_t)b * a.H + yy) * a.W + xx) * a.cin + ci];
+                    if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+                        if (!a.x2)
+                            vb = (short)a.x[(((int64_t)b * a.H + yy) * a.W + xx) * a.cin + ci];
+                        else if (ci < a.cd)
+                            vb = (short)a.x[(((int64_t)b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.cd + ci];
+                        else
+                            vb = (short)a.x2[(((int64_t)b * a.H + yy) * a.W + xx) * (a.cin - a.cd) + (ci - a.cd)];
+                    }
                 }
             }
             av[i] = va;
@@ -472,21 +532,6 @@ __global__ void k_pool(const uint16_t *x, int B, int H, int W, int C, uint16_t *
         const int64_t base = (((int64_t)b * H + 2 * yy) * W + 2 * xx) * C + c;
         const float s = (bf2f(x[base]) + bf2f(x[base + C])) + (bf2f(x[base + (int64_t)W * C]) + bf2f(x[base + (int64_t)W * C + C]));
         y[e] = f2bf(0.25f * s);
-    }
-}
-
-// Nearest upsampling of deep [B][h][w][Cd] to [B][2h][2w], concatenated with skip [B][2h][2w][Cs]:
-// out channel c < Cd from deep, else skip. One thread per pixel and group of 8 channels (Cd, Cs
-// multiples of 8), 16-byte copies.
-__global__ void k_up_concat(const uint16_t *deep, const uint16_t *skip, int B, int H, int W, int Cd, int Cs, uint16_t *y) {
-    const int G = (Cd + Cs) / 8, HW = H * W;
-    const int n = B * HW * G;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const int q = e / G, c = (e - q * G) * 8;
-        const int b = q / HW, r = q - b * HW, yy = r / W, xx = r - yy * W;
-        const uint16_t *src = c < Cd ? deep + (((int64_t)b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * Cd + c
-                                     : skip + (int64_t)q * Cs + (c - Cd);
-        *reinterpret_cast<uint4 *>(y + (int64_t)e * 8) = *reinterpret_cast<const uint4 *>(src);
     }
 }
 
@@ -1080,7 +1125,7 @@ struct DenoiserImpl {
     uint16_t *wf[10] = {}, *wb[10] = {};
     int64_t step = 0;
     // activations (bf16) at full (P1), half (P2), quarter (P3) resolution pixel counts
-    uint16_t *x0 = nullptr, *a[10] = {}, *q1 = nullptr, *q2 = nullptr, *c2 = nullptr, *c1 = nullptr;
+    uint16_t *x0 = nullptr, *a[10] = {}, *q1 = nullptr, *q2 = nullptr;
     float *out = nullptr;
     // gradients (fp32) and their bf16 copies
     float *gA = nullptr, *gB = nullptr, *gS1 = nullptr, *gS2 = nullptr, *gpart = nullptr, *hpart = nullptr, *bpart = nullptr;
@@ -1111,9 +1156,14 @@ struct DenoiserImpl {
         return hipGetLastError();
     }
 
-    void conv(int l, const uint16_t *in, int B, int level, bool relu, uint16_t *y, float *yf, bool backward) {
+    // Returns whether the pooled activation (yq) was written by the product kernel.
+    bool conv(int l, const uint16_t *in, int B, int level, bool relu, uint16_t *y, float *yf, bool backward,
+              const uint16_t *in2 = nullptr, int cd = 0, uint16_t *yq = nullptr) {
         ConvArgs c{};
         c.x = in;
+        c.x2 = in2;
+        c.cd = cd;
+        c.yq = yq;
         c.B = B;
         c.H = H >> level;
         c.W = W >> level;
@@ -1144,7 +1194,7 @@ struct DenoiserImpl {
                 default: break;
             }
 #undef OCTOPIC_CONVT8
-            return;
+            return c.yq != nullptr;
         }
         if (c.cin % 8 == 0 && conv_tile_lds(c.cin, c.cout, c.kpad) <= 160 * 1024 - 1024) {
             const size_t lds = conv_tile_lds(c.cin, c.cout, c.kpad);
@@ -1163,7 +1213,7 @@ struct DenoiserImpl {
                 default: break;
             }
 #undef OCTOPIC_CONVT
-            return;
+            return c.yq != nullptr;
         }
         if (c.cin % 8 == 0) {
             const size_t lds = (size_t)c.cout * (c.kpad + 8) * sizeof(uint16_t);
@@ -1183,7 +1233,7 @@ struct DenoiserImpl {
                 default: break;
             }
 #undef OCTOPIC_CONV8
-            return;
+            return false;
         }
         const dim3 g((unsigned)((Pn + 63) / 64));
         switch (c.cout / 16) {
@@ -1194,39 +1244,39 @@ struct DenoiserImpl {
             case 6: hipLaunchKernelGGL(k_conv<6>, g, dim3(kConvThreads), 0, stream, c); break;
             default: break;
         }
+        return false;
     }
 
     void forward(const float *input, int B) {
         const int64_t P0 = P(0, B);
         hipLaunchKernelGGL(k_pack_input, dim3(grid_for(P0)), dim3(256), 0, stream, input, P0, x0);
         conv(E1A, x0, B, 0, true, a[E1A], nullptr, false);
-        conv(E1B, a[E1A], B, 0, true, a[E1B], nullptr, false);
-        hipLaunchKernelGGL(k_pool, dim3(grid_for(P(1, B) * 16)), dim3(256), 0, stream, a[E1B], B, H, W, 16, q1);
+        // the pools are written by the products that make their inputs (k_pool only when a product cannot)
+        if (!conv(E1B, a[E1A], B, 0, true, a[E1B], nullptr, false, nullptr, 0, q1))
+            hipLaunchKernelGGL(k_pool, dim3(grid_for(P(1, B) * 16)), dim3(256), 0, stream, a[E1B], B, H, W, 16, q1);
         conv(E2A, q1, B, 1, true, a[E2A], nullptr, false);
-        conv(E2B, a[E2A], B, 1, true, a[E2B], nullptr, false);
-        hipLaunchKernelGGL(k_pool, dim3(grid_for(P(2, B) * 32)), dim3(256), 0, stream, a[E2B], B, H >> 1, W >> 1, 32, q2);
+        if (!conv(E2B, a[E2A], B, 1, true, a[E2B], nullptr, false, nullptr, 0, q2))
+            hipLaunchKernelGGL(k_pool, dim3(grid_for(P(2, B) * 32)), dim3(256), 0, stream, a[E2B], B, H >> 1, W >> 1, 32, q2);
         conv(BA, q2, B, 2, true, a[BA], nullptr, false);
         conv(BB, a[BA], B, 2, true, a[BB], nullptr, false);
-        hipLaunchKernelGGL(k_up_concat, dim3(grid_for(P(1, B) * 12)), dim3(256), 0, stream, a[BB], a[E2B], B, H >> 1, W >> 1, 64,
-                           32, c2);
-        conv(D2A, c2, B, 1, true, a[D2A], nullptr, false);
+        // decoder inputs: upsampled deep features and the skip, read from their own tensors
+        conv(D2A, a[BB], B, 1, true, a[D2A], nullptr, false, a[E2B], 64);
         conv(D2B, a[D2A], B, 1, true, a[D2B], nullptr, false);
-        hipLaunchKernelGGL(k_up_concat, dim3(grid_for(P0 * 6)), dim3(256), 0, stream, a[D2B], a[E1B], B, H, W, 32, 16, c1);
-        conv(D1A, c1, B, 0, true, a[D1A], nullptr, false);
+        conv(D1A, a[D2B], B, 0, true, a[D1A], nullptr, false, a[E1B], 32);
         conv(D1B, a[D1A], B, 0, true, a[D1B], nullptr, false);
         hipLaunchKernelGGL(k_head, dim3(grid_for(P0)), dim3(256), 0, stream, a[D1B], params + hoff, params + hoff + 16, P0, out);
     }
 
     // Gradient of layer l given dL/d(activation) in da (fp32 [P][cout]); input-gradient written to
     // dx (fp32 [P][cin]) unless l is the first layer.
-    void layer_back(int l, const uint16_t *in, const float *da, int B, float *dx) {
+    void layer_back(int l, const uint16_t *in, const float *da, int B, float *dx, const uint16_t *in2 = nullptr, int cd = 0) {
         const int level = level_of(l), C = kLayers[l].cout;
         const int64_t Pn = P(level, B);
         const int nb = (int)((Pn + 63) / 64);
         hipLaunchKernelGGL(k_relu_back, dim3(nb), dim3(256), 0, stream, da, a[l], Pn, C, dq, bpart);
         hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, stream, bpart, nb, C, grad + boff[l]);
         WgradArgs w{};
-        w.x = in; w.dy = dq; w.B = B; w.H = H >> level; w.W = W >> level; w.cin = cin_s[l]; w.cout = C; w.kpad = kpad[l];
+        w.x = in; w.x2 = in2; w.cd = cd; w.dy = dq; w.B = B; w.H = H >> level; w.W = W >> level; w.cin = cin_s[l]; w.cout = C; w.kpad = kpad[l];
         w.split_pixels = 1024;
         const int splits = (int)((Pn + w.split_pixels - 1) / w.split_pixels);
         w.part = gpart;
@@ -1244,11 +1294,11 @@ struct DenoiserImpl {
         hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, stream, hpart, nb0, 18, hpart + (int64_t)nb0 * 18);
         // d1b, d1a (its input gradient is [P0][48])
         layer_back(D1B, a[D1A], gA, B, gB);
-        layer_back(D1A, c1, gB, B, gA);
+        layer_back(D1A, a[D2B], gB, B, gA, a[E1B], 32);
         hipLaunchKernelGGL(k_up_concat_back, dim3(grid_for(P0 * 48)), dim3(256), 0, stream, gA, B, H, W, 32, 16, gB, gS1);
         // d2b (da = gB [P1][32]), d2a (input gradient [P1][96])
         layer_back(D2B, a[D2A], gB, B, gA);
-        layer_back(D2A, c2, gA, B, gB);
+        layer_back(D2A, a[BB], gA, B, gB, a[E2B], 64);
         hipLaunchKernelGGL(k_up_concat_back, dim3(grid_for(P1 * 96)), dim3(256), 0, stream, gB, B, H >> 1, W >> 1, 64, 32, gA, gS2);
         // bottleneck (da = gA [P2][64])
         layer_back(BB, a[BA], gA, B, gB);
@@ -1375,8 +1425,7 @@ int octopic_denoiser_create(int32_t device, int32_t height, int32_t width, int32
         if (v > gp) gp = v;
     }
     m.gpart_floats = gp;
-    ok = ok && m.alloc(m.x0, P0 * 4) && m.alloc(m.q1, P1 * 16) && m.alloc(m.q2, P2 * 32) && m.alloc(m.c2, P1 * 96) &&
-         m.alloc(m.c1, P0 * 48) && m.alloc(m.out, P0) && m.alloc(m.gA, gmax) && m.alloc(m.gB, gmax) && m.alloc(m.gS1, P0 * 16) &&
+    ok = ok && m.alloc(m.x0, P0 * 4) && m.alloc(m.q1, P1 * 16) && m.alloc(m.q2, P2 * 32) && m.alloc(m.out, P0) && m.alloc(m.gA, gmax) && m.alloc(m.gB, gmax) && m.alloc(m.gS1, P0 * 16) &&
          m.alloc(m.gS2, P1 * 32) && m.alloc(m.gpart, gp) && m.alloc(m.hpart, ((P0 + 63) / 64 + 1) * 18) &&
          m.alloc(m.bpart, (P0 + 63) / 64 * 64) && m.alloc(m.dq, gmax) && m.alloc(m.pair_planes, P0 * 2) &&
          m.alloc(m.pair_mom, (int64_t)max_batch * 10);
