@@ -9,7 +9,7 @@
 // Layers (3x3, zero padding 1, ReLU):  e1a 2->16, e1b 16->16 | pool | e2a 16->32, e2b 32->32 | pool |
 // ba 32->64, bb 64->64 | up, concat skip | d2a 96->32, d2b 32->32 | up, concat skip | d1a 48->16,
 // d1b 16->16 | head 1x1 16->1 (linear). 118129 parameters. The network input's two channels are
-// stored padded to four (zero channels 2, 3) so that every product reads four channels at a time.
+// stored padded to eight (zero channels 2..7) so that every product reads 16-byte channel groups.
 //
 // Every 3x3 product is one implicit GEMM kernel (k_conv): rows = pixels, columns = output channels,
 // reduction = 9 x input channels, each wave a 16-pixel x (16 NT)-channel tile of
@@ -511,13 +511,14 @@ __global__ void k_wgrad_reduce(const float *part, int splits, int cout, int kpad
 }
 
 // ------------------------------------------------------------------------------- elementwise
-// Network input [P][2] fp32 -> [P][4] bf16 (channels 2, 3 zero).
+// Network input [P][2] fp32 -> [P][8] bf16 (channels 2..7 zero), so the first product reads whole
+// 16-byte groups like the others.
 __global__ void k_pack_input(const float *in, int64_t P, uint16_t *x) {
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < P; p += (int64_t)gridDim.x * blockDim.x) {
-        x[p * 4 + 0] = f2bf(in[p * 2 + 0]);
-        x[p * 4 + 1] = f2bf(in[p * 2 + 1]);
-        x[p * 4 + 2] = 0;
-        x[p * 4 + 3] = 0;
+        uint4 v;
+        v.x = (uint32_t)f2bf(in[p * 2 + 0]) | ((uint32_t)f2bf(in[p * 2 + 1]) << 16);
+        v.y = v.z = v.w = 0u;
+        *reinterpret_cast<uint4 *>(x + p * 8) = v;
     }
 }
 
@@ -1402,7 +1403,7 @@ int octopic_denoiser_create(int32_t device, int32_t height, int32_t width, int32
         off += (int64_t)kLayers[l].cout * 9 * kLayers[l].cin;
         m.boff[l] = off;
         off += kLayers[l].cout;
-        m.cin_s[l] = l == E1A ? 4 : kLayers[l].cin;
+        m.cin_s[l] = l == E1A ? 8 : kLayers[l].cin;
         m.kpad[l] = ceil32(9 * m.cin_s[l]);
     }
     m.hoff = off;
@@ -1425,7 +1426,7 @@ int octopic_denoiser_create(int32_t device, int32_t height, int32_t width, int32
         if (v > gp) gp = v;
     }
     m.gpart_floats = gp;
-    ok = ok && m.alloc(m.x0, P0 * 4) && m.alloc(m.q1, P1 * 16) && m.alloc(m.q2, P2 * 32) && m.alloc(m.out, P0) && m.alloc(m.gA, gmax) && m.alloc(m.gB, gmax) && m.alloc(m.gS1, P0 * 16) &&
+    ok = ok && m.alloc(m.x0, P0 * 8) && m.alloc(m.q1, P1 * 16) && m.alloc(m.q2, P2 * 32) && m.alloc(m.out, P0) && m.alloc(m.gA, gmax) && m.alloc(m.gB, gmax) && m.alloc(m.gS1, P0 * 16) &&
          m.alloc(m.gS2, P1 * 32) && m.alloc(m.gpart, gp) && m.alloc(m.hpart, ((P0 + 63) / 64 + 1) * 18) &&
          m.alloc(m.bpart, (P0 + 63) / 64 * 64) && m.alloc(m.dq, gmax) && m.alloc(m.pair_planes, P0 * 2) &&
          m.alloc(m.pair_mom, (int64_t)max_batch * 10);
