@@ -705,6 +705,27 @@ __device__ double gauss_node_weight(double xi, double h, double mu, double s, in
            1.5 * (m1(lL, lR, xi) - m1(rL, rR, xi)) / h + 0.5 * (m2(lL, lR, xi) + m2(rL, rR, xi)) / (h * h);
 }
 
+// The same node integrals on unit cells (node i at i) from tables of the erf and the pdf at the cell
+// boundaries: E[k], G[k] at p_k = k - off (off = 1.5 for TSC, whose stencil ends at half cells; 1 for
+// CIC), k = 0 .. n + 2, so that neighbouring nodes share their evaluations: the formulas of
+// gauss_node_weight with h = 1, with one erf and one exp per boundary point instead of per use.
+__device__ __forceinline__ double gauss_node_weight_tab(int i, double mu, double s, int method, const double *E, const double *G) {
+    const double xi = (double)i, h = 1.0;
+    const double off = method == 0 ? 1.0 : 1.5;
+    auto m0 = [&](int ka, int kb) { return 0.5 * (E[kb] - E[ka]); };
+    auto m1 = [&](int ka, int kb) { return (mu - xi) * m0(ka, kb) - s * s * (G[kb] - G[ka]); };
+    auto m2 = [&](int ka, int kb) {
+        const double A = (double)ka - off, B = (double)kb - off;
+        return (s * s + (mu - xi) * (mu - xi)) * m0(ka, kb) - s * s * ((B - mu) * G[kb] - (A - mu) * G[ka]) -
+               2.0 * (mu - xi) * s * s * (G[kb] - G[ka]);
+    };
+    if (method == 0)   // points i - 1, i, i + 1 at k = i, i + 1, i + 2
+        return m0(i, i + 2) + (m1(i, i + 1) - m1(i + 1, i + 2)) / h;
+    // points i - 1.5, i - 0.5, i + 0.5, i + 1.5 at k = i .. i + 3
+    return 0.75 * m0(i + 1, i + 2) - m2(i + 1, i + 2) / (h * h) + 1.125 * (m0(i, i + 1) + m0(i + 2, i + 3)) +
+           1.5 * (m1(i, i + 1) - m1(i + 2, i + 3)) / h + 0.5 * (m2(i, i + 1) + m2(i + 2, i + 3)) / (h * h);
+}
+
 // Counter-based stream of the pair generator: Philox4x32-10 of (particle, sample) under a key from
 // (seed, stream), four 32-bit words per particle.
 __device__ __forceinline__ void philox4(uint32_t c[4], uint32_t k0, uint32_t k1) {
@@ -1006,8 +1027,25 @@ __global__ __launch_bounds__(256) void k_dn_prepare(DnCollideArgs a) {
     }
     __syncthreads();
     const bool ok = par[0] != 0.0;
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) g[i] = ok ? gauss_node_weight((double)i, 1.0, par[1], par[3], a.method) : 0.0;
-    for (int j = threadIdx.x; j < ny; j += blockDim.x) g[nx + j] = ok ? gauss_node_weight((double)j, 1.0, par[2], par[4], a.method) : 0.0;
+    {
+        // erf and pdf at the cell boundaries of both axes (nx + 3 and ny + 3 points), then the nodes
+        double *Ex = g + nx + ny, *Gx = Ex + (nx + 3), *Ey = Gx + (nx + 3), *Gy = Ey + (ny + 3);
+        const double off = a.method == 0 ? 1.0 : 1.5;
+        if (ok) {
+            for (int k = threadIdx.x; k < nx + 3 + ny + 3; k += blockDim.x) {
+                const bool xa = k < nx + 3;
+                const int kk = xa ? k : k - (nx + 3);
+                const double mu = xa ? par[1] : par[2], sg = xa ? par[3] : par[4];
+                const double invroot = 1.0 / (sg * 1.4142135623730951), norm = 1.0 / (sg * 2.5066282746310002);
+                const double p = (double)kk - off;
+                (xa ? Ex : Ey)[kk] = erf((p - mu) * invroot);
+                (xa ? Gx : Gy)[kk] = norm * exp(-((p - mu) * (p - mu)) / (2.0 * sg * sg));
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nx; i += blockDim.x) g[i] = ok ? gauss_node_weight_tab(i, par[1], par[3], a.method, Ex, Gx) : 0.0;
+        for (int j = threadIdx.x; j < ny; j += blockDim.x) g[nx + j] = ok ? gauss_node_weight_tab(j, par[2], par[4], a.method, Ey, Gy) : 0.0;
+    }
     __syncthreads();
     const unsigned long long *q = a.charge + (int64_t)p * a.charge_stride;
     double acc[5] = {0, 0, 0, 0, 0};   // charge, sum gx, sum gy, max gx, max gy
@@ -1086,7 +1124,8 @@ __global__ __launch_bounds__(256) void k_dn_apply(DnCollideArgs a, const float *
 hipError_t launch_dn_prepare(const DnCollideArgs &a, hipStream_t s) {
     if (a.nplanes <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dn_moments, dim3(a.chunks, a.ninters), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_dn_prepare, dim3(a.nplanes), dim3(256), (size_t)(a.nx + a.ny) * sizeof(double), s, a);
+    // LDS: the profiles (nx + ny) and the erf / pdf tables (2 (nx + 3) + 2 (ny + 3)) in doubles
+    hipLaunchKernelGGL(k_dn_prepare, dim3(a.nplanes), dim3(256), (size_t)(3 * (a.nx + a.ny) + 12) * sizeof(double), s, a);
     return hipGetLastError();
 }
 
