@@ -232,6 +232,7 @@ struct octopic_handle {
     int kick_target_blocks = 1024;
     int kick_pass_windows = -1;
     int deposit_span = 8192;    // source particles per deposit block (the LDS mesh is zeroed and flushed once per block)
+    int cus = 0;                // compute units of the device (launch sizing)
     // Kick and deposit work queues: per-launch counters and launch orders (larger slices first).
     DBuf<int> kqueue;
     DBuf<int32_t> korder;
@@ -575,6 +576,8 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
             if (v < -1 || v > 2) fail(OCTOPIC_ERR_INVALID_ARGUMENT, "OCTOPIC_KICK_PASS_WINDOWS must be -1, 0, 1 or 2");
             h->kick_pass_windows = v;
         }
+        if (hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || h->cus <= 0)
+            h->cus = 256;
         h->box.ensure(8);
         h->errinfo.ensure(1);
         h->counters.ensure(8);
@@ -1191,16 +1194,30 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             Timed t(h, F_DEPOSIT);
             DepositArgs a{};
             a.planes = h->planes.p; a.nplanes = nfp; a.st = st; a.method = o.deposit; a.ncells = (int)field_cells;
-            // ~8k particles per block: the LDS mesh is zeroed and flushed once per block
+            a.multi = h->mode == 2 ? 1 : npl;
+            // ~8k particles per block: the LDS mesh is zeroed and flushed once per block; when a block takes
+            // all planes of an interaction and the slices are small, fewer particles per block, down to one
+            // chunk, until the launch has two items per compute unit
             a.chunks = nchunks;
-            a.group = std::max(1, (int)std::min<int64_t>(nchunks, (int64_t)h->deposit_span * nchunks / std::max<int64_t>(bmax, 1)));
+            auto group_of = [&](int64_t span) {
+                return std::max(1, (int)std::min<int64_t>(nchunks, span * nchunks / std::max<int64_t>(bmax, 1)));
+            };
+            int64_t span = h->deposit_span;
+            if (a.multi > 1) {
+                const int units = nfp / a.multi;
+                while (span > kChunkSpan) {
+                    const int g = group_of(span);
+                    if ((int64_t)units * ((nchunks + g - 1) / g) >= 2 * (int64_t)std::max(h->cus, 1)) break;
+                    span /= 2;
+                }
+            }
+            a.group = group_of(span);
             a.scale = scale; a.charge = h->charge.p; a.charge_stride = charge_stride;
             a.dropped = h->counters.p + 4; a.chunkbox = cbox; a.err = h->errinfo.p;
             a.fp32 = h->fp32 ? 1 : 0;
             a.queue = h->dqueue.p + 2 * bi;
             a.order = h->dorder.p + (size_t)(npl + 1) * 2 * boff[bi];
             // slice-pair meshes: the planes of an interaction share mesh and particles, one read feeds them all
-            a.multi = h->mode == 2 ? 1 : npl;
             // the luminosity planes ride along when they use the field deposit's stencil
             const bool lum_fold = compute_lum && a.multi > 1 && h->lum_dep == o.deposit;
             a.lum_plane0 = lum_fold ? nfp : -1;
