@@ -124,6 +124,8 @@ typedef struct octopic_stats {
     int64_t collides;
     int64_t denoised_planes;                            /* last collide: field planes rebuilt by the denoiser */
     int64_t denoise_fallbacks;                          /* last collide: field planes left to plain PIC */
+    int64_t deposit_all_planes;                         /* last collide: 1 if deposit items took all planes of
+                                                           an interaction, 0 if one plane each */
 } octopic_stats;
 
 /* One directed interaction of the last collide, in execution order (pair position p in collision
@@ -226,8 +228,12 @@ int octopic_kernel_times(const octopic_handle *h, double *ms_total, int64_t *lau
 /* Launch shapes (no counterpart in the reference; results are bit-identical for every value, only
  * the speed changes): "transverse_tiles" (0 or a power of two <= 128: in-slice tile order, default
  * 16), "kick_target_blocks" (0..2^20 work items per kick launch, default 1024), "kick_pass_windows"
- * (-1..2, default -1), "deposit_span" (1024..2^20 source particles per deposit block, default 8192).
- * octopic_autotune times candidate shapes by coordinate descent (median of `trials` collides per
+ * (-1..2, default -1), "deposit_span" (1024..2^20 source particles per deposit block, default 8192),
+ * "deposit_multi" (slice-pair meshes: 1 = each deposit item takes all planes of an interaction, one
+ * read of the particles; 0 = one plane per item; -1 = the faster of the two, timed on the device in
+ * two consecutive collides every 256; default -1) and
+ * "deposit_min_span" (1024..2^20: smallest item the all-planes items are split to when a launch would
+ * have fewer than two items per compute unit, default 2048).  octopic_autotune times candidate shapes by coordinate descent (median of `trials` collides per
  * candidate, after a warming one) on copies of the two beams in a scratch handle with the same
  * options, so the handle's own Green cache, counters and the beams are untouched, and sets the
  * fastest shape on the handle; best_ms (optional) receives its collide time. */

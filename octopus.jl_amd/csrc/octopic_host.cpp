@@ -198,6 +198,8 @@ struct FFTPlanHost {
 const char *kFamilies[] = {"slicing", "bounds", "geometry", "green", "deposit", "fft_rows_fwd", "fft_cols_conv",
                            "fft_rows_inv", "field", "kick", "lum", "store", "node_prepare", "tile_order", "denoise"};
 constexpr int kNFamilies = sizeof(kFamilies) / sizeof(kFamilies[0]);
+// collides between timed trials of the two deposit forms (deposit_multi -1)
+constexpr int64_t kDepositTrialEvery = 256;
 enum Fam { F_SLICING, F_BOUNDS, F_GEOMETRY, F_GREEN, F_DEPOSIT, F_ROWS_FWD, F_COLS, F_ROWS_INV, F_FIELD, F_KICK, F_LUM,
            F_STORE, F_NODE, F_TILE, F_DENOISE };
 
@@ -233,6 +235,15 @@ struct octopic_handle {
     int kick_pass_windows = -1;
     int deposit_span = 8192;    // source particles per deposit block (the LDS mesh is zeroed and flushed once per block)
     int cus = 0;                // compute units of the device (launch sizing)
+    // deposit items take all planes of an interaction (slice-pair meshes): 1 always, 0 never, -1 the
+    // faster form, both timed on the device every kDepositTrialEvery collides
+    int deposit_multi = -1;
+    bool deposit_per_plane = false;   // -1: the slower form at the last trial was all planes
+    bool deposit_all_planes_used = false;
+    int64_t deposit_trial = 0;        // collides since the first trial
+    double deposit_trial_ms[2] = {0.0, 0.0};
+    std::vector<hipEvent_t> deposit_ev;   // per batch: before and after the deposit launches (trials)
+    int deposit_min_span = kChunkSpan;   // smallest items the two-per-CU rule may make
     // Kick and deposit work queues: per-launch counters and launch orders (larger slices first).
     DBuf<int> kqueue;
     DBuf<int32_t> korder;
@@ -513,6 +524,7 @@ void octopic_destroy(octopic_handle *h) {
     hipDeviceSynchronize();
     if (h->own_stream) hipStreamDestroy(h->own_stream);
     for (auto e : h->ev_pool) hipEventDestroy(e);
+    for (auto e : h->deposit_ev) hipEventDestroy(e);
     if (h->h_tab) hipHostFree(h->h_tab);
     if (h->h_err) hipHostFree(h->h_err);
     if (h->h_counters) hipHostFree(h->h_counters);
@@ -635,6 +647,8 @@ const ShapeKnob kShapeKnobs[] = {
     {"kick_target_blocks", &octopic_handle::kick_target_blocks, 0, 1 << 20, false},
     {"kick_pass_windows", &octopic_handle::kick_pass_windows, -1, 2, false},
     {"deposit_span", &octopic_handle::deposit_span, 1024, 1 << 20, false},
+    {"deposit_multi", &octopic_handle::deposit_multi, -1, 1, false},
+    {"deposit_min_span", &octopic_handle::deposit_min_span, 1024, 1 << 20, false},
 };
 const ShapeKnob *find_knob(const char *name) {
     for (const auto &k : kShapeKnobs)
@@ -715,6 +729,7 @@ extern "C" int octopic_autotune(octopic_handle *h, octopic_beam *beam1, octopic_
         };
         struct Cand { int octopic_handle::*field; std::vector<int> values; };
         const std::vector<Cand> cands = {
+            {&octopic_handle::deposit_multi, {-1, 1, 0}},
             {&octopic_handle::tiles, {8, 16, 32}},
             {&octopic_handle::kick_target_blocks, {512, 1024, 2048}},
             {&octopic_handle::deposit_span, {4096, 8192, 16384}},
@@ -1145,6 +1160,16 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
 
     int64_t planes_total = 0, builds_guess = 0;
     (void)builds_guess;
+    // the automatic deposit form: all-planes items timed in one collide, per-plane items in the next
+    const int trial = (h->deposit_multi == -1 && h->mode != 2 && npl > 1 && h->deposit_trial % kDepositTrialEvery < 2)
+                          ? (int)(h->deposit_trial % kDepositTrialEvery) : -1;
+    if (h->deposit_multi == -1 && h->mode != 2 && npl > 1) ++h->deposit_trial;
+    if (trial >= 0)
+        while (h->deposit_ev.size() < 2 * batches.size()) {
+            hipEvent_t ev;
+            HIPCHECK(hipEventCreate(&ev));
+            h->deposit_ev.push_back(ev);
+        }
     for (size_t bi = 0; bi < batches.size(); ++bi) {
         const int B = bcnt[bi];
         if (B == 0) continue;
@@ -1194,7 +1219,9 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             Timed t(h, F_DEPOSIT);
             DepositArgs a{};
             a.planes = h->planes.p; a.nplanes = nfp; a.st = st; a.method = o.deposit; a.ncells = (int)field_cells;
-            a.multi = h->mode == 2 ? 1 : npl;
+            const bool all_planes = h->deposit_multi == 1 ||
+                                    (h->deposit_multi == -1 && (trial >= 0 ? trial == 0 : !h->deposit_per_plane));
+            a.multi = (h->mode == 2 || !all_planes) ? 1 : npl;
             // ~8k particles per block: the LDS mesh is zeroed and flushed once per block; when a block takes
             // all planes of an interaction and the slices are small, fewer particles per block, down to one
             // chunk, until the launch has two items per compute unit
@@ -1205,7 +1232,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             int64_t span = h->deposit_span;
             if (a.multi > 1) {
                 const int units = nfp / a.multi;
-                while (span > kChunkSpan) {
+                while (span > std::max<int64_t>(kChunkSpan, h->deposit_min_span)) {
                     const int g = group_of(span);
                     if ((int64_t)units * ((nchunks + g - 1) / g) >= 2 * (int64_t)std::max(h->cus, 1)) break;
                     span /= 2;
@@ -1221,6 +1248,8 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             // the luminosity planes ride along when they use the field deposit's stencil
             const bool lum_fold = compute_lum && a.multi > 1 && h->lum_dep == o.deposit;
             a.lum_plane0 = lum_fold ? nfp : -1;
+            h->deposit_all_planes_used = a.multi > 1;
+            if (trial >= 0) HIPCHECK(hipEventRecord(h->deposit_ev[2 * bi], s));
             HIPCHECK(launch_deposit(a, s));
             if (compute_lum && !lum_fold) {
                 DepositArgs l = a;
@@ -1233,6 +1262,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
                 l.lum_plane0 = -1;
                 HIPCHECK(launch_deposit(l, s));
             }
+            if (trial >= 0) HIPCHECK(hipEventRecord(h->deposit_ev[2 * bi + 1], s));
         }
         if (h->dn) {
             // field planes through the residual denoiser before the convolution
@@ -1333,6 +1363,17 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     if (h->h_err->code) raise_device_error(h, *h->h_err, turn);
     if (h->mode == 2)
         for (int p = 0; p < npairs; ++p) h->pran[p] = h->pran[p] && ran_node[p];
+    if (trial >= 0) {
+        double ms = 0.0;
+        for (size_t bi = 0; bi < batches.size(); ++bi) {
+            if (bcnt[bi] == 0) continue;
+            float e = 0.f;
+            HIPCHECK(hipEventElapsedTime(&e, h->deposit_ev[2 * bi], h->deposit_ev[2 * bi + 1]));
+            ms += e;
+        }
+        h->deposit_trial_ms[trial] = ms;
+        if (trial == 1) h->deposit_per_plane = h->deposit_trial_ms[1] < h->deposit_trial_ms[0];
+    }
     for (int b = 0; b < 2; ++b) {
         if (B[b]->on_device) continue;
         double *hp2[6] = {B[b]->x, B[b]->px, B[b]->y, B[b]->py, B[b]->z, B[b]->pz};
@@ -1356,6 +1397,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     S.dropped = (int64_t)(h->h_counters[4] - counters_before[4]);
     S.denoised_planes = (int64_t)(h->h_counters[6] - counters_before[6]);
     S.denoise_fallbacks = (int64_t)(h->h_counters[7] - counters_before[7]);
+    S.deposit_all_planes = h->deposit_all_planes_used ? 1 : 0;
     S.pairs = npairs;
     S.batches = (int64_t)batches.size();
     S.widest_batch = widest;

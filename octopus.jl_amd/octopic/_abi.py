@@ -130,6 +130,7 @@ class Stats(C.Structure):
         ("collides", C.c_int64),
         ("denoised_planes", C.c_int64),
         ("denoise_fallbacks", C.c_int64),
+        ("deposit_all_planes", C.c_int64),
     ]
 
 

@@ -173,7 +173,8 @@ class PICCollider:
                                           1 if cl else 0, C.byref(lum)))
         return lum.value
 
-    LAUNCH_SHAPES = ("transverse_tiles", "kick_target_blocks", "kick_pass_windows", "deposit_span")
+    LAUNCH_SHAPES = ("transverse_tiles", "kick_target_blocks", "kick_pass_windows", "deposit_span", "deposit_multi",
+                     "deposit_min_span")
 
     def set_launch_shape(self, name: str, value: int):
         """A launch shape of this handle (results do not depend on it; see octopic_set_launch_shape)."""

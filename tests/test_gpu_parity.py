@@ -325,7 +325,9 @@ def test_in_slice_order_does_not_change_results(oracle, gpu, monkeypatch, tiles)
         col = PICCollider(solver)
         be, bp = Beam.from_host(e, PE, device=gpu), Beam.from_host(p, PP, device=gpu)
         lums = [col.collide(be, bp) for _ in range(2)]
-        outs.append((lums, be.to_host(), bp.to_host(), col.pair_luminosity(), col.stats(),
+        st = col.stats()
+        st.pop("deposit_all_planes")   # a launch-shape figure, not a result
+        outs.append((lums, be.to_host(), bp.to_host(), col.pair_luminosity(), st,
                      [(r.p, r.dir, r.cache_event, tuple(r.source_grid), tuple(r.field_grid))
                       for r in col.interaction_records()]))
     monkeypatch.delenv("OCTOPIC_TRANSVERSE_TILES", raising=False)
@@ -620,6 +622,7 @@ def test_autotune_keeps_results_and_cache(oracle, gpu):
     shape = tuned.autotune(be, bp, trials=1)
     assert shape["transverse_tiles"] in (8, 16, 32) and shape["kick_target_blocks"] in (512, 1024, 2048)
     assert shape["deposit_span"] in (4096, 8192, 16384) and shape["kick_pass_windows"] in (-1, 0, 1, 2)
+    assert shape["deposit_multi"] in (-1, 0, 1)
     assert shape["collide_ms"] > 0
     assert np.array_equal(be.to_host(), e) and np.array_equal(bp.to_host(), p)
     st = tuned.stats()
@@ -630,3 +633,50 @@ def test_autotune_keeps_results_and_cache(oracle, gpu):
     assert torch.equal(be.coords, ce.coords) and torch.equal(bp.coords, cp.coords)
     with pytest.raises(Exception):
         tuned.set_launch_shape("transverse_tiles", 3)
+
+
+@pytest.mark.parametrize("beams", ["round", "flat"])
+def test_deposit_shapes_bit_identical(oracle, gpu, monkeypatch, beams):
+    """Per-plane deposit items, items that take all planes of an interaction (with the luminosity plane
+    folded in), their split into smaller items, their plane-by-plane fallback for windows too large
+    for one block (slice order instead of transverse tiles makes every chunk box span the beam), and
+    the timed choice between the two forms all give the same integer charge sums, so coordinates,
+    luminosities and cache history are bit-identical."""
+    e, p = (round_beams if beams == "round" else flat_beams)(oracle, 30000, 12000)
+    sl = LongitudinalSlicing(nslices=7, method="normal_quantile")
+    solver = PICPoissonSolver(grid=(128, 128), slicing=sl, deposit_method="TSC", field_derivative="fourth",
+                              slice_interpolation="quadratic")
+    shapes = [dict(deposit_multi=0), dict(deposit_multi=1), dict(),
+              dict(deposit_multi=1, deposit_span=16384, deposit_min_span=1024),
+              dict(deposit_multi=1, deposit_span=4096, deposit_min_span=4096),
+              dict(deposit_multi=1, tiles="0"), dict(tiles="0"), dict(deposit_multi=1, tiles="0", deposit_span=16384)]
+    outs, figures = [], []
+    for shape in shapes:
+        shape = dict(shape)
+        tiles = shape.pop("tiles", None)
+        if tiles is None:
+            monkeypatch.delenv("OCTOPIC_TRANSVERSE_TILES", raising=False)
+        else:
+            monkeypatch.setenv("OCTOPIC_TRANSVERSE_TILES", tiles)
+        col = PICCollider(solver)
+        for k, v in shape.items():
+            col.set_launch_shape(k, v)
+        be, bp = Beam.from_host(e, PE, device=gpu), Beam.from_host(p, PP, device=gpu)
+        lums, used = [], []
+        for _ in range(3):
+            lums.append(col.collide(be, bp))
+            st = col.stats()
+            used.append(st.pop("deposit_all_planes"))
+        figures.append(used)
+        outs.append((lums, be.to_host(), bp.to_host(), np.asarray(col.pair_luminosity()), st))
+    monkeypatch.delenv("OCTOPIC_TRANSVERSE_TILES", raising=False)
+    for shape, o in zip(shapes[1:], outs[1:]):
+        assert o[0] == outs[0][0], shape
+        assert np.array_equal(o[1], outs[0][1]) and np.array_equal(o[2], outs[0][2]), shape
+        assert np.array_equal(o[3], outs[0][3]), shape
+        assert o[4] == outs[0][4], shape
+    assert figures[0] == [0, 0, 0] and figures[1] == [1, 1, 1]
+    # the automatic form times all-planes items, then per-plane items, then keeps the faster
+    assert figures[2][:2] == [1, 0] and figures[6][:2] == [1, 0]
+    # the default shape against the oracle (one collide)
+    compare_collide(oracle, gpu, solver, e, p)
