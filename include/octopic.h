@@ -201,6 +201,19 @@ int octopic_track_lorentz_boost(octopic_handle *h, double angle, int32_t reverse
 int octopic_track_crab_cavity(octopic_handle *h, double frequency, int32_t nharmonics, const double *strength_x,
                               const double *strength_y, const double *phase, octopic_beam *beam);
 
+/* gaussian_beambeam_kick(sigx, sigy, x, y) (src/elements/strong_beam.jl:1139-1146, principal-axis
+ * branches :1040-1059): the normalised Bassetti-Erskine kick (kx[i], ky[i]) of a transverse Gaussian
+ * of rms sizes (sigx, sigy) at (x[i], y[i]), i < n; callers multiply by the beam-beam strength.  A zero
+ * size gives zero kicks.  FP64; device pointers; on the handle's stream. */
+int octopic_gaussian_beambeam_kick(octopic_handle *h, const double *x, const double *y, int64_t n, double sigx,
+                                   double sigy, double *kx, double *ky);
+
+/* faddeeva_w_approx_reim(zr, zi) (src/math/SpecialMath.jl:60-151): w(z) = exp(-z^2) erfc(-iz) by
+ * Weideman's 32-term rational approximation, the Laplace continued fraction for large |z| and the
+ * reflection w(z) = 2 exp(-z^2) - w(-z) below the real axis.  FP64; device pointers; on the handle's
+ * stream. */
+int octopic_faddeeva_w(octopic_handle *h, const double *zr, const double *zi, int64_t n, double *wr, double *wi);
+
 /* Beam(N, ...) synthetic Gaussian construction on the device (Beam.jl:127-181, 545-553): Philox4x32-10
  * counter normals, clamp to +-cutoff, standardise, scale by sigma/beta with alpha.  beta, alpha,
  * sigma are (x, y, z) triples.  Writes the six device arrays of `beam` (n = beam->n). */

@@ -139,6 +139,16 @@ struct DepositArgs {
 constexpr int kDepositMaxLdsCells = 20480 - 2;   // 160 KB per block, less 16 bytes of the block's own LDS
 hipError_t launch_deposit(DepositArgs a, hipStream_t s);
 
+// Bassetti-Erskine kick and the Faddeeva function (k_be.hip)
+struct WeidemanCoeffs {
+    double c[32];   // polynomial coefficients, highest power first
+    double L;       // sqrt(32 / sqrt 2)
+};
+const WeidemanCoeffs &weideman_coeffs();
+hipError_t launch_be_kick(const double *x, const double *y, int64_t n, double sigx, double sigy, double *kx, double *ky,
+                          hipStream_t s);
+hipError_t launch_faddeeva(const double *zr, const double *zi, int64_t n, double *wr, double *wi, hipStream_t s);
+
 struct KickArgs {
     const Inter *inters;
     int ninters;

@@ -1474,6 +1474,29 @@ int octopic_track_linear6d(octopic_handle *h, const double *m, octopic_beam *b) 
     return OCTOPIC_OK;
 }
 
+int octopic_gaussian_beambeam_kick(octopic_handle *h, const double *x, const double *y, int64_t n, double sigx,
+                                   double sigy, double *kx, double *ky) {
+    if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    if (n < 0 || (n > 0 && (!x || !y || !kx || !ky)))
+        return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "gaussian_beambeam_kick: null array or negative count");
+    if (!(std::isfinite(sigx) && std::isfinite(sigy) && sigx >= 0.0 && sigy >= 0.0))
+        return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "gaussian_beambeam_kick: sizes must be finite and non-negative");
+    hipSetDevice(h->device);
+    if (launch_be_kick(x, y, n, sigx, sigy, kx, ky, h->stream) != hipSuccess)
+        return set_err(h, OCTOPIC_ERR_DEVICE, "gaussian_beambeam_kick launch failed");
+    return OCTOPIC_OK;
+}
+
+int octopic_faddeeva_w(octopic_handle *h, const double *zr, const double *zi, int64_t n, double *wr, double *wi) {
+    if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    if (n < 0 || (n > 0 && (!zr || !zi || !wr || !wi)))
+        return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "faddeeva_w: null array or negative count");
+    hipSetDevice(h->device);
+    if (launch_faddeeva(zr, zi, n, wr, wi, h->stream) != hipSuccess)
+        return set_err(h, OCTOPIC_ERR_DEVICE, "faddeeva_w launch failed");
+    return OCTOPIC_OK;
+}
+
 int octopic_track_lorentz_boost(octopic_handle *h, double angle, int32_t reverse, octopic_beam *b) {
     if (!h || !b) return OCTOPIC_ERR_INVALID_ARGUMENT;
     if (!b->on_device) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "octopic_track_lorentz_boost needs device beam pointers");

@@ -169,6 +169,8 @@ def declare(lib):
         "octopic_track_linear6d": (i32, [vp, DPTR, C.POINTER(BeamStruct)]),
         "octopic_track_lorentz_boost": (i32, [vp, C.c_double, i32, C.POINTER(BeamStruct)]),
         "octopic_track_crab_cavity": (i32, [vp, C.c_double, i32, DPTR, DPTR, DPTR, C.POINTER(BeamStruct)]),
+        "octopic_gaussian_beambeam_kick": (i32, [vp, vp, vp, i64, C.c_double, C.c_double, vp, vp]),
+        "octopic_faddeeva_w": (i32, [vp, vp, vp, i64, vp, vp]),
         "octopic_generate_gaussian_beam": (
             i32,
             [vp, C.c_uint64, C.c_uint64, DPTR, DPTR, DPTR, C.c_double, C.POINTER(BeamStruct)],
@@ -219,6 +221,8 @@ EXPORTED = [
     "octopic_track_linear6d",
     "octopic_track_lorentz_boost",
     "octopic_track_crab_cavity",
+    "octopic_gaussian_beambeam_kick",
+    "octopic_faddeeva_w",
     "octopic_generate_gaussian_beam",
     "octopic_debug_weights",
     "octopic_collide_many",

@@ -274,6 +274,29 @@ class PICCollider:
         s = beam.struct()
         self._check(lib().octopic_track_crab_cavity(self.h, float(frequency), nh, arr[0], arr[1], arr[2], C.byref(s)))
 
+    def gaussian_beambeam_kick(self, x, y, sigx: float, sigy: float):
+        """gaussian_beambeam_kick(sigx, sigy, x, y) (strong_beam.jl:1139-1146) elementwise over device
+        float64 tensors x, y of one shape; returns (kx, ky), the normalised Bassetti-Erskine kick."""
+        import torch
+        if x.shape != y.shape or x.dtype != torch.float64 or y.dtype != torch.float64 or not (x.is_cuda and y.is_cuda):
+            raise ArgumentError("x and y must be float64 device tensors of one shape")
+        x, y = x.contiguous(), y.contiguous()
+        kx, ky = torch.empty_like(x), torch.empty_like(y)
+        self._check(lib().octopic_gaussian_beambeam_kick(self.h, x.data_ptr(), y.data_ptr(), x.numel(), float(sigx),
+                                                         float(sigy), kx.data_ptr(), ky.data_ptr()))
+        return kx, ky
+
+    def faddeeva_w(self, z):
+        """w(z) = exp(-z^2) erfc(-iz) (SpecialMath.jl:60-151) elementwise over a complex128 device tensor."""
+        import torch
+        if z.dtype != torch.complex128 or not z.is_cuda:
+            raise ArgumentError("z must be a complex128 device tensor")
+        zr, zi = z.real.contiguous(), z.imag.contiguous()
+        wr, wi = torch.empty_like(zr), torch.empty_like(zi)
+        self._check(lib().octopic_faddeeva_w(self.h, zr.data_ptr(), zi.data_ptr(), zr.numel(), wr.data_ptr(),
+                                             wi.data_ptr()))
+        return torch.complex(wr, wi)
+
     def set_timing(self, enabled: bool, families: Optional[str] = None):
         """Per-family HIP-event timing: every family, or only the comma-separated `families`."""
         if enabled and families:

@@ -52,6 +52,12 @@ def lib():
         L.oracle_octopus_normal.argtypes = [C.c_uint64] * 4 + [C.c_int64]
         L.oracle_erfinv.restype = C.c_double
         L.oracle_erfinv.argtypes = [C.c_double]
+        L.oracle_faddeeva_w.restype = None
+        L.oracle_faddeeva_w.argtypes = [C.c_double, C.c_double, D, D]
+        L.oracle_gaussian_beambeam_kick.restype = None
+        L.oracle_gaussian_beambeam_kick.argtypes = [C.c_double] * 4 + [D, D]
+        L.oracle_weideman_coefficients.restype = None
+        L.oracle_weideman_coefficients.argtypes = [D, D]
         L.oracle_kernel_integral.restype = C.c_double
         L.oracle_kernel_integral.argtypes = [C.c_double, C.c_double]
         L.oracle_atan_ratio.restype = C.c_double
@@ -141,6 +147,27 @@ def weights(method, u, n):
     lib().oracle_weights(_abi.DEPOSIT[method] if isinstance(method, str) else method, float(u), n, C.byref(b), w)
     k = 2 if (method == "CIC" or method == 0) else 3
     return int(b.value), tuple(w[:k])
+
+
+def faddeeva_w(z):
+    """w(z) as SpecialMath.jl:60-151 evaluates it (Weideman N = 32 / continued fraction)."""
+    wr, wi = C.c_double(), C.c_double()
+    lib().oracle_faddeeva_w(float(z.real), float(z.imag), C.byref(wr), C.byref(wi))
+    return complex(wr.value, wi.value)
+
+
+def gaussian_beambeam_kick(sigx, sigy, x, y):
+    """(Kx, Ky) of strong_beam.jl:1139-1146."""
+    kx, ky = C.c_double(), C.c_double()
+    lib().oracle_gaussian_beambeam_kick(float(sigx), float(sigy), float(x), float(y), C.byref(kx), C.byref(ky))
+    return kx.value, ky.value
+
+
+def weideman_coefficients():
+    c = (C.c_double * 32)()
+    L = C.c_double()
+    lib().oracle_weideman_coefficients(c, C.byref(L))
+    return list(c), L.value
 
 
 def kernel_integral(x, y):

@@ -13,6 +13,10 @@
 //   src/elements/linear6d.jl             the linear one-turn map
 //   src/elements/lorentz_boost.jl        crossing-angle boost and its reverse
 //   src/elements/crab_cavity.jl          thin crab cavity
+//   src/elements/strong_beam.jl          gaussian_beambeam_kick (Bassetti-Erskine), pinned by
+//                                        quadrature of the field integral and the reference's KATs
+//   src/math/SpecialMath.jl              the Faddeeva function of the device path, pinned by
+//                                        scipy.special.wofz vectors (tests/golden/bassetti_erskine.npz)
 // Julia evaluation order is kept expression by expression (a*b*c == (a*b)*c, a-b+c == (a-b)+c),
 // Julia's NaN-propagating min/max/clamp are restated, and the file is compiled with
 // -ffp-contract=off so that no product is fused into an FMA the reference does not perform.
@@ -1655,12 +1659,165 @@ int guarded(F &&f) {
     }
 }
 
+// ==========================================================================================
+// Bassetti-Erskine kick of a transverse Gaussian and the Faddeeva function (test infrastructure)
+// ==========================================================================================
+// w(z) = exp(-z^2) erfc(-iz) as the reference's device path evaluates it (SpecialMath.jl:60-151):
+// Weideman's N = 32 rational approximation (SIAM J. Numer. Anal. 31 (1994) 1497-1518) inside the
+// band, the Laplace continued fraction with Gautschi's term count outside, the asymptotic forms
+// for |x| + y > 4000 and > 1e7, and w(z) = 2 exp(-z^2) - w(-z) below the axis.
+struct OracleWeideman {
+    double c[32];
+    double L;
+    OracleWeideman() {
+        // a_m = (1/2M) sum_k f(k) cos(pi m k / M) over k = -M+1..M-1, f(k) = exp(-t^2)(L^2 + t^2),
+        // t = L tan(k pi / 2M), M = 2N; the polynomial in Z takes a_N first
+        const int N = 32, M = 64;
+        const long double pi = 3.141592653589793238462643383279502884L;
+        const long double LL = std::sqrt((long double)N / std::sqrt(2.0L));
+        for (int m = 1; m <= N; ++m) {
+            long double acc = 0.0L;
+            for (int k = -M + 1; k <= M - 1; ++k) {
+                const long double t = LL * std::tan(pi * k / (2.0L * M));
+                acc += std::exp(-t * t) * (LL * LL + t * t) * std::cos(pi * (long double)m * k / M);
+            }
+            c[N - m] = (double)(acc / (2.0L * M));
+        }
+        L = (double)LL;
+    }
+};
+const OracleWeideman &oracle_weideman() {
+    static const OracleWeideman w;
+    return w;
+}
+
+std::complex<double> faddeeva_far(double zr, double y) {
+    const double isp = 0.56418958354775628695, x = std::fabs(zr);
+    const std::complex<double> I(0.0, 1.0), z(zr, y);
+    if (x + y > 4000.0) {
+        if (x + y > 1e7) return I * isp / z;
+        return I * isp * z / (z * z - 0.5);
+    }
+    double nu = std::floor(3.9 + 11.398 / (0.08254 * x + 0.1421 * y + 0.2023));
+    std::complex<double> t = z;
+    for (nu = 0.5 * (nu - 1.0); nu > 0.4; nu -= 0.5) t = z - nu / t;
+    return I * isp / t;
+}
+
+std::complex<double> faddeeva_upper(double zr, double zi) {
+    const double x = std::fabs(zr), y = zi;
+    if (y > 7.0 || (x > 6.0 && (y > 0.1 || (x > 8.0 && y > 1e-10) || x > 28.0))) return faddeeva_far(zr, y);
+    const OracleWeideman &wc = oracle_weideman();
+    const std::complex<double> iz(-zi, zr), den = wc.L - iz, Z = (wc.L + iz) / den;
+    std::complex<double> p(0.0, 0.0);
+    for (int k = 0; k < 32; ++k) p = p * Z + wc.c[k];
+    return 2.0 * p / (den * den) + 0.56418958354775628695 / den;
+}
+
+std::complex<double> faddeeva(double zr, double zi) {
+    if (zi < 0.0) {
+        const std::complex<double> z(zr, zi);
+        return 2.0 * std::exp(-z * z) - faddeeva_upper(-zr, -zi);
+    }
+    return faddeeva_upper(zr, zi);
+}
+
+// gaussian_beambeam_kick (strong_beam.jl:1139-1146) and its principal-axis branches (:705-709 round,
+// :779-797 near-round bounds and blend, :799-895 near-round series, :962-1038 elliptic form)
+void near_round_series(double v, double eta, double x, double y, double &kx, double &ky) {
+    const double q = (x * x + y * y) / (2 * v), d = (x * x - y * y) / (2 * v);
+    double m[7];
+    if (q <= 2.0) {
+        std::fill(m, m + 7, 0.0);
+        double term = 1.0;
+        for (int k = 0; k <= 24; ++k) {
+            for (int j = 0; j < 7; ++j) m[j] += term / (k + 1 + j);
+            term *= -q / (k + 1);
+        }
+    } else {
+        const double e = std::exp(-q);
+        m[0] = -std::expm1(-q) / q;
+        for (int j = 1; j < 7; ++j) m[j] = (j * m[j - 1] - e) / q;
+    }
+    const double c1x = d * m[2] - m[1], c1y = d * m[2] + m[1];
+    const double c2x = 1.5 * m[2] + (-d - q) * m[3] + d * d / 2 * m[4];
+    const double c2y = 1.5 * m[2] + (d - q) * m[3] + d * d / 2 * m[4];
+    const double c3x = -1.5 * m[3] + (2.5 * d + q) * m[4] + (-d * q - d * d / 2) * m[5] + d * d * d / 6 * m[6];
+    const double c3y = 1.5 * m[3] + (2.5 * d - q) * m[4] + (-d * q + d * d / 2) * m[5] + d * d * d / 6 * m[6];
+    kx = x / v * (m[0] + eta * (c1x + eta * (c2x + eta * c3x)));
+    ky = y / v * (m[0] + eta * (c1y + eta * (c2y + eta * c3y)));
+}
+
+double axis_series(double a, double b, double u, double w) {
+    const double s = a + b;
+    const double j00 = 2 / (a * s), j10 = 2 * (2 * a + b) / (3 * a * a * a * s * s), j01 = 2 / (a * b * s * s);
+    const double j20 = 2 * (8 * a * a + 9 * a * b + 3 * b * b) / (15 * std::pow(a, 5) * s * s * s);
+    const double j11 = 2 * (3 * a + b) / (3 * a * a * a * b * s * s * s);
+    const double j02 = 2 * (a + 3 * b) / (3 * a * b * b * b * s * s * s);
+    const double u2 = u * u, w2 = w * w;
+    return u * (j00 - (u2 * j10 + w2 * j01) / 2 + u2 * u2 * j20 / 8 + u2 * w2 * j11 / 4 + w2 * w2 * j02 / 8);
+}
+
+void elliptic(double s1, double s2, double x, double y, double eta, double &kx, double &ky) {
+    const double rho2 = x * x / (s1 * s1) + y * y / (s2 * s2);
+    if (std::pow(rho2, 3.5) <= std::ldexp(1.0, -52) / std::sqrt(eta)) {
+        kx = axis_series(s1, s2, x, y);
+        ky = axis_series(s2, s1, y, x);
+        return;
+    }
+    const double ax = std::fabs(x), ay = std::fabs(y), den = std::sqrt(2.0 * (s1 * s1 - s2 * s2));
+    const std::complex<double> r =
+        (2.0 * std::sqrt(M_PI) / den) *
+        (faddeeva_upper(ax / den, ay / den) -
+         std::exp(-ax * ax / (2 * s1 * s1) - ay * ay / (2 * s2 * s2)) * faddeeva_upper(s2 / s1 * ax / den, s1 / s2 * ay / den));
+    kx = x < 0 ? -r.imag() : r.imag();
+    ky = y < 0 ? -r.real() : r.real();
+}
+
+void be_principal(double s1, double s2, double x, double y, double &kx, double &ky) {
+    const double v = (s1 * s1 + s2 * s2) / 2, eta = (s1 * s1 - s2 * s2) / (2 * v);
+    if (eta == 0.0) {
+        const double u = (x * x + y * y) / (2 * v), phi = u == 0.0 ? 1.0 : -std::expm1(-u) / u;
+        kx = phi / v * x;
+        ky = phi / v * y;
+        return;
+    }
+    const double outer = std::sqrt(std::sqrt(8.0 / 3.0 * 64.0 * std::ldexp(1.0, -52))), inner = outer / 2;
+    if (eta <= inner) return near_round_series(v, eta, x, y, kx, ky);
+    double ex, ey;
+    elliptic(s1, s2, x, y, eta, ex, ey);
+    if (eta >= outer) { kx = ex; ky = ey; return; }
+    const double t = (eta - inner) / (outer - inner), w = t * t * t * (10 + t * (-15 + 6 * t));
+    double sx, sy;
+    near_round_series(v, eta, x, y, sx, sy);
+    kx = sx + w * (ex - sx);
+    ky = sy + w * (ey - sy);
+}
+
 }  // namespace
 
 // ==========================================================================================
 // C API of the oracle (ctypes; test infrastructure)
 // ==========================================================================================
 extern "C" {
+
+void oracle_faddeeva_w(double zr, double zi, double *wr, double *wi) {
+    const std::complex<double> w = faddeeva(zr, zi);
+    *wr = w.real();
+    *wi = w.imag();
+}
+
+void oracle_gaussian_beambeam_kick(double sigx, double sigy, double x, double y, double *kx, double *ky) {
+    *kx = *ky = 0.0;
+    if (sigx == 0.0 || sigy == 0.0) return;
+    if (sigx >= sigy) be_principal(sigx, sigy, x, y, *kx, *ky);
+    else be_principal(sigy, sigx, y, x, *ky, *kx);
+}
+
+void oracle_weideman_coefficients(double *c, double *L) {
+    std::memcpy(c, oracle_weideman().c, sizeof(double) * 32);
+    *L = oracle_weideman().L;
+}
 
 const char *oracle_last_error(void) { return g_last_error.c_str(); }
 
