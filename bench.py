@@ -191,6 +191,8 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel HIP event timing")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the launch shapes with the library's autotuner before the warm-up (results are unchanged)")
+    ap.add_argument("--launch-shape", action="append", default=[], metavar="NAME=VALUE",
+                    help="set a launch shape of the collider (repeatable; results are unchanged)")
     ap.add_argument("--instances", type=int, default=0,
                     help="batched form: this many independent bunch-pair instances per step (0: the single pair)")
     ap.add_argument("--one-by-one", action="store_true", help="batched form: collide the instances one at a time")
@@ -226,6 +228,9 @@ def main():
                       seed=0, rng_id=P_CFG["rng_id"], device=dev)
     me = linear6d_matrix(E_CFG["beta"], [2 * math.pi * t for t in E_CFG["tunes"]])
     mp = linear6d_matrix(P_CFG["beta"], [2 * math.pi * t for t in P_CFG["tunes"]])
+    for item in args.launch_shape:
+        name, _, value = item.partition("=")
+        col.set_launch_shape(name.strip(), int(value))
     tuned = col.autotune(e, p, trials=3) if args.autotune else None
 
     def turn(t):
