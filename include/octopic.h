@@ -103,15 +103,24 @@ typedef struct octopic_options {
     octopic_slicing slicing1, slicing2;
 } octopic_options;
 
+/* Element type of a beam's coordinate arrays: Phase6DRep{Float64} or Phase6DRep{Float32}
+ * (src/beam/Beam.jl:45-56). */
+enum { OCTOPIC_DTYPE_F64 = 0, OCTOPIC_DTYPE_F32 = 1 };
+
 /* One beam: Phase6DRep + BeamParams (src/beam/Beam.jl:29-56).  With on_device = 1 the six pointers
  * are device (HBM) pointers and the collision runs on them in place; with on_device = 0 they are
- * host pointers that are copied in and out around the collision. */
+ * host pointers that are copied in and out around the collision.
+ * dtype = OCTOPIC_DTYPE_F32: the six pointers address float arrays (cast to double * here).  Such a
+ * beam is widened exactly to FP64 for the collision and its x, px, y, py, pz are rounded back to
+ * float (round to nearest) at the end; the arithmetic in between is the handle's precision option.
+ * The reference's Phase6DRep{Float32} runs its whole collision in Float32 (pic_cpu.jl:1646-1660);
+ * here a Float32 beam gives the Float32 rounding of this library's FP64 (or FP32-form) result. */
 typedef struct octopic_beam {
     double *x, *px, *y, *py, *z, *pz;
     int64_t n;
     double charge, mc2, E0, r0, npart;
     int32_t on_device;
-    int32_t reserved;
+    int32_t dtype;                 /* OCTOPIC_DTYPE_F64 | OCTOPIC_DTYPE_F32 */
 } octopic_beam;
 
 /* Counters of the last collide and of the handle's lifetime. */

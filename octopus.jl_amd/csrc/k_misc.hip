@@ -3,6 +3,8 @@
 //   linear6d : track_particle(::Symplectic6DMap, ::Linear6D, ...) (src/elements/linear6d.jl:110-118).
 //   generate : Beam(N, ...) counter-RNG construction (src/beam/Beam.jl:127-181, 545-553) with the
 //              Philox4x32-10 stream of src/math/counter_rng.jl:184-277 and Box-Muller (153-173).
+#include <algorithm>
+
 #include "common.h"
 #include "launch.h"
 
@@ -191,26 +193,40 @@ __global__ __launch_bounds__(256) void k_alpha(double *q, double *p, int64_t n, 
         p[i] = fac * (p[i] - alpha * q[i]);
 }
 
-hipError_t launch_generate(double *const c[6], int64_t n, uint64_t seed, uint64_t rng_id, double cutoff, hipStream_t s) {
-    // Standard-normal coordinates, standardised to mean 0 / rms 1; the Twiss scaling is applied by
-    // the host wrapper (octopic_generate_gaussian_beam) with launch_scale_beam.
-    int g = (int)((n + 255) / 256);
-    if (g > 2048) g = 2048;
-    double *scratch = nullptr;
-    hipError_t e = hipMallocAsync((void **)&scratch, sizeof(double) * (g + 2), s);
-    if (e != hipSuccess) return e;
+// Blocks of the generator's partial sums: the partition (and so every sum's rounding) depends only on n.
+constexpr int kGenMaxBlocks = 2048;
+int64_t generate_workspace_doubles(int64_t n) {
+    const int64_t g = std::min<int64_t>((n + 255) / 256, kGenMaxBlocks);
+    return g + 2;
+}
+
+// Standard-normal coordinates, standardised to mean 0 / rms 1; the Twiss scaling is applied by the host
+// wrapper with launch_scale_beam.  ws holds generate_workspace_doubles(n) doubles: the block partial sums,
+// then the mean and the rms of the coordinate in hand.  Every entry is written before it is read.
+hipError_t launch_generate_ws(double *const c[6], int64_t n, uint64_t seed, uint64_t rng_id, double cutoff, double *ws,
+                              hipStream_t s) {
+    const int g = (int)(generate_workspace_doubles(n) - 2);
     for (int d = 0; d < 6; ++d) {
         hipLaunchKernelGGL(k_normals, dim3(g), dim3(256), 0, s, c[d], n, seed, rng_id, d, cutoff);
-        hipLaunchKernelGGL(k_partial_sum, dim3(g), dim3(256), 0, s, c[d], n, (const double *)nullptr, scratch);
-        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(64), 0, s, scratch, g, n, scratch + g, 0);
-        hipLaunchKernelGGL(k_partial_sum, dim3(g), dim3(256), 0, s, c[d], n, (const double *)(scratch + g), scratch);
-        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(64), 0, s, scratch, g, n, scratch + g + 1, 1);
-        hipLaunchKernelGGL(k_standardize_scale, dim3(g), dim3(256), 0, s, c[d], n, (const double *)(scratch + g),
-                           (const double *)(scratch + g + 1), 1.0);
+        hipLaunchKernelGGL(k_partial_sum, dim3(g), dim3(256), 0, s, c[d], n, (const double *)nullptr, ws);
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(64), 0, s, ws, g, n, ws + g, 0);
+        hipLaunchKernelGGL(k_partial_sum, dim3(g), dim3(256), 0, s, c[d], n, (const double *)(ws + g), ws);
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(64), 0, s, ws, g, n, ws + g + 1, 1);
+        hipLaunchKernelGGL(k_standardize_scale, dim3(g), dim3(256), 0, s, c[d], n, (const double *)(ws + g),
+                           (const double *)(ws + g + 1), 1.0);
     }
-    hipFreeAsync(scratch, s);
     return hipGetLastError();
 }
+
+hipError_t launch_generate(double *const c[6], int64_t n, uint64_t seed, uint64_t rng_id, double cutoff, hipStream_t s) {
+    double *ws = nullptr;
+    hipError_t e = hipMallocAsync((void **)&ws, sizeof(double) * generate_workspace_doubles(n), s);
+    if (e != hipSuccess) return e;
+    e = launch_generate_ws(c, n, seed, rng_id, cutoff, ws, s);
+    hipFreeAsync(ws, s);
+    return e != hipSuccess ? e : hipGetLastError();
+}
+
 
 __global__ __launch_bounds__(256) void k_scale(double *v, int64_t n, double mul) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) v[i] *= mul;
@@ -224,6 +240,30 @@ hipError_t launch_scale_beam(double *const c[6], int64_t n, const double *beta, 
         hipLaunchKernelGGL(k_alpha, dim3(g), dim3(256), 0, s, c[2 * k], c[2 * k + 1], n, alpha[k], sigma[k] / beta[k]);
         hipLaunchKernelGGL(k_scale, dim3(g), dim3(256), 0, s, c[2 * k], n, sigma[k]);
     }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ Float32 beams
+// Exact widening of a float coordinate array into the FP64 collision copy, and the rounding back.
+__global__ __launch_bounds__(256) void k_widen(const float *src, double *dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (double)src[i];
+}
+__global__ __launch_bounds__(256) void k_narrow(const double *src, float *dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
+hipError_t launch_widen(const float *src, double *dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int g = (int)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_widen, dim3(g), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+hipError_t launch_narrow(const double *src, float *dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int g = (int)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_narrow, dim3(g), dim3(256), 0, s, src, dst, n);
     return hipGetLastError();
 }
 

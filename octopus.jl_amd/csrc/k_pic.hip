@@ -359,6 +359,7 @@ __global__ void k_geometry(GeometryArgs a) {
             event = e.valid ? 3 : 2;
             atomicAdd(&a.counters[e.valid ? 2 : 1], 1ull);
             e.valid = 1;
+            e.epoch = o.epoch;
             e.sg = es;
             e.fg = ef;
             sg = es;
@@ -398,6 +399,19 @@ __global__ void k_geometry(GeometryArgs a) {
 hipError_t launch_geometry(const GeometryArgs &a, hipStream_t s) {
     if (a.npairs <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_geometry, dim3((2 * a.npairs + 63) / 64), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+// A collide that fails leaves no entry it built: the Green kernels stop once an error is set, so an
+// entry committed by its geometry stage may have no spectrum (or an older mesh's).
+__global__ void k_cache_invalidate(CacheGeo *cache, int nkeys, int epoch) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nkeys && cache[k].epoch == epoch) cache[k].valid = 0;
+}
+
+hipError_t launch_cache_invalidate(CacheGeo *cache, int nkeys, int epoch, hipStream_t s) {
+    if (!cache || nkeys <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cache_invalidate, dim3((nkeys + 255) / 256), dim3(256), 0, s, cache, nkeys, epoch);
     return hipGetLastError();
 }
 
@@ -463,13 +477,10 @@ __device__ __forceinline__ void deposit_item(const DepositArgs &a, unsigned long
                 unsigned long long v;
                 if constexpr (sizeof(W) == sizeof(double)) v = fixed_point(wx[m] * wy[q] * scale);
                 else v = fixed_point((double)((float)wx[m] * (float)wy[q] * (float)scale));
-                if (in_lds) {
-                    // a zero weight (off-mesh positions sit at node 0, possibly outside the window) adds
-                    // its nothing to the window's first cell
-                    atomicAdd(&tile[v ? (by + q - wy0) * wnx + (bx + m - wx0) : 0], v);
-                } else if (v) {
-                    atomicAdd(&out[(by + q) * n1 + bx + m], v);
-                }
+                // a zero weight adds nothing (off-mesh positions sit at node 0, possibly outside the window)
+                if (!v) continue;
+                if (in_lds) atomicAdd(&tile[(by + q - wy0) * wnx + (bx + m - wx0)], v);
+                else atomicAdd(&out[(by + q) * n1 + bx + m], v);
             }
     }
     if (pl.count_box) {
@@ -514,7 +525,7 @@ __device__ __forceinline__ void deposit_item_multi(const DepositArgs &a, unsigne
     const int wnx = wx1 - wx0 + 1, wcells = wnx * (wy1 - wy0 + 1);
     // luminosity window: the same chunk box (its sM drift lies between sL and sR) on the luminosity mesh
     Plane lp{};
-    int lx0 = 0, ly0 = 0, lnx = 0, lcells = 0;
+    int lx0 = 0, ly0 = 0, lnx = 0, lny = 0, lcells = 0;
     if (lplane >= 0) {
         lp = a.planes[lplane];
         int lx1 = lp.n1 - 1, ly1 = lp.n2 - 1;
@@ -530,7 +541,8 @@ __device__ __forceinline__ void deposit_item_multi(const DepositArgs &a, unsigne
             stencil_window<METHOD>((b[2] - lp.y0) * lp.hyi, (b[3] - lp.y0) * lp.hyi, lp.n2, ly0, ly1);
         }
         lnx = lx1 - lx0 + 1;
-        lcells = lnx * (ly1 - ly0 + 1);
+        lny = ly1 - ly0 + 1;
+        lcells = lnx * lny;
     }
     if (np * wcells + lcells > a.lds_cells) {   // block-uniform
         for (int p = 0; p < np; ++p) {
@@ -579,7 +591,7 @@ __device__ __forceinline__ void deposit_item_multi(const DepositArgs &a, unsigne
                     unsigned long long v;
                     if constexpr (sizeof(W) == sizeof(double)) v = fixed_point(wx[m] * wy[q] * scale);
                     else v = fixed_point((double)((float)wx[m] * (float)wy[q] * (float)scale));
-                    atomicAdd(&tp[v ? (by + q - wy0) * wnx + (bx + m - wx0) : 0], v);
+                    if (v) atomicAdd(&tp[(by + q - wy0) * wnx + (bx + m - wx0)], v);
                 }
         }
         if (lplane >= 0) {   // the luminosity plane at sM, FP64 weights
@@ -593,12 +605,18 @@ __device__ __forceinline__ void deposit_item_multi(const DepositArgs &a, unsigne
             stencil<METHOD>((xd - lp.x0) * lp.hxi, lp.n1, bx, wx);
             stencil<METHOD>((yd - lp.y0) * lp.hyi, lp.n2, by, wy);
             unsigned long long *tl = tile + np * wcells;
+            unsigned long long *lout = a.charge + (size_t)lplane * a.charge_stride;
 #pragma unroll
             for (int m = 0; m < K; ++m)
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
                     const unsigned long long v = fixed_point(wx[m] * wy[q] * scale);
-                    atomicAdd(&tl[v ? (by + q - ly0) * lnx + (bx + m - lx0) : 0], v);
+                    if (!v) continue;
+                    // the window is the chunk box's, which holds the sM positions when the field slice's
+                    // centre lies between its boundaries; a node outside it adds straight into HBM
+                    const int lxi = bx + m - lx0, lyi = by + q - ly0;
+                    if ((unsigned)lxi < (unsigned)lnx && (unsigned)lyi < (unsigned)lny) atomicAdd(&tl[lyi * lnx + lxi], v);
+                    else atomicAdd(&lout[(by + q) * lp.n1 + bx + m], v);
                 }
         }
     }

@@ -63,7 +63,9 @@ hipError_t launch_store_gather(double *const *beam, int64_t n, double *const *st
 
 struct BatchPair { int32_t p, i, j, reserved; };
 
-struct CacheGeo { int32_t valid, reserved; Grid sg, fg; };
+// A cached slice-pair mesh pair.  `epoch` is the collide (numbered from 1) that built the entry, so that
+// a collide that fails can invalidate every entry it built (their Green spectra may be unbuilt).
+struct CacheGeo { int32_t valid, epoch; Grid sg, fg; };
 
 // Static options the geometry kernel needs.
 struct GeomOpts {
@@ -72,6 +74,7 @@ struct GeomOpts {
     int ns1, ns2;
     int npl;                     // planes per interaction (2 or 3)
     int temp_slot0;              // first per-batch temporary Green slot (green_cache = none)
+    int epoch;                   // this collide's number (CacheGeo::epoch of the entries it builds)
     double min_x, min_y, quantize, min_ratio, growth;
     double kbb1, kbb2, klum;
 };
@@ -112,6 +115,8 @@ struct GeometryArgs {
     ErrorInfo *err;
 };
 hipError_t launch_geometry(const GeometryArgs &a, hipStream_t s);
+// Invalidate the cache entries built by collide `epoch` (after that collide failed).
+hipError_t launch_cache_invalidate(CacheGeo *cache, int nkeys, int epoch, hipStream_t s);
 
 struct DepositArgs {
     const Plane *planes;
@@ -245,8 +250,14 @@ constexpr int kMaxCrabHarmonics = 16;
 hipError_t launch_crab_cavity(double kcc, int nh, const double *sx, const double *sy, const double *phase, double *const c[6],
                               int64_t n, hipStream_t s);
 hipError_t launch_generate(double *const c[6], int64_t n, uint64_t seed, uint64_t rng_id, double cutoff, hipStream_t s);
+// The same with a caller-provided workspace of generate_workspace_doubles(n) doubles (no allocation).
+int64_t generate_workspace_doubles(int64_t n);
+hipError_t launch_generate_ws(double *const c[6], int64_t n, uint64_t seed, uint64_t rng_id, double cutoff, double *ws,
+                              hipStream_t s);
 hipError_t launch_scale_beam(double *const c[6], int64_t n, const double *beta, const double *alpha, const double *sigma,
                              hipStream_t s);
+hipError_t launch_widen(const float *src, double *dst, int64_t n, hipStream_t s);
+hipError_t launch_narrow(const double *src, float *dst, int64_t n, hipStream_t s);
 hipError_t launch_debug_weights(int method, const double *u, int64_t n, int npts, int *base, double *w, hipStream_t s);
 
 }  // namespace octopic

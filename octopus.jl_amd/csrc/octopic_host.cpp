@@ -222,9 +222,11 @@ struct octopic_handle {
     DBuf<CacheGeo> cache;
     DBuf<double2> spec;
     int nkeys = 0;
+    int epoch = 0;              // the current (or last) collide's number, 1-based: CacheGeo::epoch
     int64_t slot_stride = 0;
     // per-collide buffers
     DBuf<double> beam_in[2][6];
+    DBuf<float> beam_f32[2][6];  // host Float32 beams: device staging of the float arrays
     DBuf<double> st[2][6];
     DBuf<int64_t> perm[2];
     int tiles = 16;             // transverse tiles per axis of the in-slice order (0: slice order only)
@@ -530,7 +532,9 @@ void octopic_destroy(octopic_handle *h) {
     if (h->h_counters) hipHostFree(h->h_counters);
     h->tab.release(); h->q.release(); h->errinfo.release(); h->counters.release(); h->cache.release(); h->spec.release();
     for (int b = 0; b < 2; ++b) {
-        for (int c = 0; c < 6; ++c) { h->beam_in[b][c].release(); h->st[b][c].release(); h->srt[b][c].release(); }
+        for (int c = 0; c < 6; ++c) {
+            h->beam_in[b][c].release(); h->beam_f32[b][c].release(); h->st[b][c].release(); h->srt[b][c].release();
+        }
         h->perm[b].release(); h->perm_srt[b].release(); h->inv_srt[b].release(); h->sid[b].release(); h->hist[b].release();
     }
     h->rank_keys.release(); h->rank_sorted.release(); h->rank_vals.release(); h->rank_temp.release();
@@ -689,6 +693,11 @@ extern "C" int octopic_autotune(octopic_handle *h, octopic_beam *beam1, octopic_
         const int64_t n1 = beam1->n, n2 = beam2->n;
         if (n1 <= 0 || n2 <= 0) fail(OCTOPIC_ERR_INVALID_ARGUMENT, "autotune: empty beam");
         // originals and working copies of both beams, six coordinates each
+        for (const octopic_beam *bm : {beam1, beam2})
+            if (bm->dtype != OCTOPIC_DTYPE_F64 && bm->dtype != OCTOPIC_DTYPE_F32) fail(OCTOPIC_ERR_INVALID_ARGUMENT, "autotune: beam dtype");
+        // element sizes of the two beams (Float32 beams are copied as floats)
+        const size_t es[2] = {beam1->dtype == OCTOPIC_DTYPE_F32 ? sizeof(float) : sizeof(double),
+                              beam2->dtype == OCTOPIC_DTYPE_F32 ? sizeof(float) : sizeof(double)};
         if (hipMalloc((void **)&buf, sizeof(double) * 12 * (size_t)(n1 + n2)) != hipSuccess) fail(OCTOPIC_ERR_OOM, "autotune: beam copies");
         double *orig[2][6], *work[2][6];
         const octopic_beam *src[2] = {beam1, beam2};
@@ -699,7 +708,7 @@ extern "C" int octopic_autotune(octopic_handle *h, octopic_beam *beam1, octopic_
             for (int c = 0; c < 6; ++c) {
                 orig[b][c] = buf + off; off += (size_t)n;
                 work[b][c] = buf + off; off += (size_t)n;
-                HIPCHECK(hipMemcpy(orig[b][c], cs[c], sizeof(double) * n, hipMemcpyDefault));
+                HIPCHECK(hipMemcpy(orig[b][c], cs[c], es[b] * n, hipMemcpyDefault));
             }
         }
         octopic_beam wb[2] = {*beam1, *beam2};
@@ -715,7 +724,7 @@ extern "C" int octopic_autotune(octopic_handle *h, octopic_beam *beam1, octopic_
             for (int r = 0; r <= trials; ++r) {
                 for (int b = 0; b < 2; ++b)
                     for (int c = 0; c < 6; ++c)
-                        HIPCHECK(hipMemcpyAsync(work[b][c], orig[b][c], sizeof(double) * (b ? n2 : n1), hipMemcpyDeviceToDevice,
+                        HIPCHECK(hipMemcpyAsync(work[b][c], orig[b][c], es[b] * (b ? n2 : n1), hipMemcpyDeviceToDevice,
                                                 t->stream));
                 HIPCHECK(hipStreamSynchronize(t->stream));
                 const auto t0 = std::chrono::steady_clock::now();
@@ -767,11 +776,13 @@ extern "C" int octopic_collide(octopic_handle *h, octopic_beam *beam1, octopic_b
             // kernels of the abandoned collide may have left charge or bounds slots filled
             if (h->charge.p) HIPCHECK(hipMemsetAsync(h->charge.p, 0, h->charge.n * sizeof(unsigned long long), h->stream));
             if (h->keys.p) HIPCHECK(hipMemsetAsync(h->keys.p, 0, h->keys.n * sizeof(unsigned long long), h->stream));
+            HIPCHECK(launch_cache_invalidate(h->cache.p, h->nkeys, h->epoch, h->stream));
             HIPCHECK(hipMemsetAsync(h->errinfo.p, 0, sizeof(ErrorInfo), h->stream));
             HIPCHECK(hipStreamSynchronize(h->stream));
             h->unfinished = false;
         }
         h->unfinished = true;
+        h->epoch = (int)(h->stats.collides % 2147483646) + 1;
         collide_impl(h, beam1, beam2, turn, compute_luminosity, luminosity);
         h->unfinished = false;
     } catch (Fail &f) {
@@ -812,8 +823,10 @@ void raise_device_error(octopic_handle *h, const ErrorInfo &e, int64_t turn) {
     } else {
         snprintf(buf, sizeof buf, "device-side failure code %d at stage %s", e.code, stage_name(e.stage));
     }
-    // Kernels after the failure returned early; clear any charge they left so the next collide starts clean.
+    // Kernels after the failure returned early; clear any charge they left so the next collide starts clean,
+    // and drop the cache entries this collide built (their spectra may not have been built).
     if (h->charge.p) hipMemsetAsync(h->charge.p, 0, h->charge.n * sizeof(unsigned long long), h->stream);
+    launch_cache_invalidate(h->cache.p, h->nkeys, h->epoch, h->stream);
     hipMemsetAsync(h->errinfo.p, 0, sizeof(ErrorInfo), h->stream);
     hipStreamSynchronize(h->stream);
     fail(e.code, buf);
@@ -825,6 +838,8 @@ void check_beam(const octopic_beam *b, int k) {
     if (!b->x || !b->px || !b->y || !b->py || !b->z || !b->pz)
         fail(OCTOPIC_ERR_INVALID_ARGUMENT, "beam" + std::to_string(k) + " has a null coordinate pointer");
     if (b->n > (int64_t)2147483647) fail(OCTOPIC_ERR_UNSUPPORTED, "more than 2^31-1 macroparticles per beam");
+    if (b->dtype != OCTOPIC_DTYPE_F64 && b->dtype != OCTOPIC_DTYPE_F32)
+        fail(OCTOPIC_ERR_INVALID_ARGUMENT, "beam" + std::to_string(k) + " dtype must be OCTOPIC_DTYPE_F64 or OCTOPIC_DTYPE_F32");
 }
 
 // _strong_strong_kbb1/2 (slicing.jl:77-91) divided by the source macroparticle count (pic_cpu.jl:821-824)
@@ -856,6 +871,24 @@ extern "C" int octopic_collide_many(octopic_handle *const *handles, octopic_beam
             return set_err(handles[k], OCTOPIC_ERR_INVALID_ARGUMENT,
                            "octopic_collide_many: two handles share one denoiser (it holds one collide's activations)");
     }
+    // instances update their beams in place from concurrent threads and streams: no coordinate array may
+    // belong to two beams
+    std::map<const double *, int> owner;
+    for (int k = 0; k < count; ++k)
+        for (int b = 0; b < 2; ++b) {
+            const octopic_beam &bm = b ? beams2[k] : beams1[k];
+            const double *cs[6] = {bm.x, bm.px, bm.y, bm.py, bm.z, bm.pz};
+            for (const double *c : cs) {
+                if (!c) continue;   // reported by the collide itself
+                auto ins = owner.emplace(c, 2 * k + b);
+                if (!ins.second && ins.first->second != 2 * k + b)
+                    return set_err(handles[k], OCTOPIC_ERR_INVALID_ARGUMENT,
+                                   "octopic_collide_many: instance " + std::to_string(k) + " beam" + std::to_string(b + 1) +
+                                       " shares a coordinate array with instance " + std::to_string(ins.first->second / 2) +
+                                       " beam" + std::to_string(ins.first->second % 2 + 1) +
+                                       " (instances collide concurrently and update their beams in place)");
+            }
+        }
     std::vector<int32_t> st(count, OCTOPIC_OK);
     std::vector<std::thread> pool;
     pool.reserve(count);
@@ -895,7 +928,18 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         }
         for (int c = 0; c < 6; ++c) {
             h->st[b][c].ensure(n[b]);
-            if (B[b]->on_device) {
+            if (B[b]->dtype == OCTOPIC_DTYPE_F32) {
+                // Float32 beam: an exact FP64 copy for the collision, rounded back at the end
+                h->beam_in[b][c].ensure(n[b]);
+                const float *src = reinterpret_cast<const float *>(hp[c]);
+                if (!B[b]->on_device) {
+                    h->beam_f32[b][c].ensure(n[b]);
+                    HIPCHECK(hipMemcpyAsync(h->beam_f32[b][c].p, hp[c], sizeof(float) * n[b], hipMemcpyHostToDevice, s));
+                    src = h->beam_f32[b][c].p;
+                }
+                HIPCHECK(launch_widen(src, h->beam_in[b][c].p, n[b], s));
+                bp[b][c] = h->beam_in[b][c].p;
+            } else if (B[b]->on_device) {
                 bp[b][c] = hp[c];
             } else {
                 h->beam_in[b][c].ensure(n[b]);
@@ -1103,7 +1147,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     go.green_type = o.green_type; go.green_cache = o.green_cache; go.quadratic = h->mode == 1;
     go.interaction_grid = o.interaction_grid; go.compute_lum = compute_lum ? 1 : 0;
     go.lkick = o.longitudinal_kick ? 1 : 0;
-    go.ns1 = ns1; go.ns2 = ns2; go.npl = npl; go.temp_slot0 = temp_slot0;
+    go.ns1 = ns1; go.ns2 = ns2; go.npl = npl; go.temp_slot0 = temp_slot0; go.epoch = h->epoch;
     go.min_x = o.min_extent_x; go.min_y = o.min_extent_y; go.quantize = o.grid_quantize;
     go.min_ratio = o.green_min_ratio; go.growth = o.green_growth;
     go.kbb1 = kbb1; go.kbb2 = kbb2; go.klum = klum;
@@ -1375,8 +1419,20 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         if (trial == 1) h->deposit_per_plane = h->deposit_trial_ms[1] < h->deposit_trial_ms[0];
     }
     for (int b = 0; b < 2; ++b) {
-        if (B[b]->on_device) continue;
         double *hp2[6] = {B[b]->x, B[b]->px, B[b]->y, B[b]->py, B[b]->z, B[b]->pz};
+        if (B[b]->dtype == OCTOPIC_DTYPE_F32) {
+            for (int c = 0; c < 6; ++c) {
+                if (c == 4) continue;   // z is never modified
+                if (B[b]->on_device) {
+                    HIPCHECK(launch_narrow(bp[b][c], reinterpret_cast<float *>(hp2[c]), n[b], s));
+                } else {
+                    HIPCHECK(launch_narrow(bp[b][c], h->beam_f32[b][c].p, n[b], s));
+                    HIPCHECK(hipMemcpyAsync(hp2[c], h->beam_f32[b][c].p, sizeof(float) * n[b], hipMemcpyDeviceToHost, s));
+                }
+            }
+            continue;
+        }
+        if (B[b]->on_device) continue;
         for (int c = 0; c < 6; ++c)
             if (c != 4) HIPCHECK(hipMemcpyAsync(hp2[c], bp[b][c], sizeof(double) * n[b], hipMemcpyDeviceToHost, s));
     }
@@ -1530,6 +1586,7 @@ int octopic_generate_gaussian_beam(octopic_handle *h, uint64_t seed, uint64_t rn
     if (!h || !b || !beta3 || !alpha3 || !sigma3) return OCTOPIC_ERR_INVALID_ARGUMENT;
     if (!b->on_device) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "octopic_generate_gaussian_beam needs device pointers");
     if (b->n <= 0) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "random Beam construction requires a positive particle count");
+    if (b->dtype != OCTOPIC_DTYPE_F64) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "the beam generator writes FP64 beams");
     double *c[6] = {b->x, b->px, b->y, b->py, b->z, b->pz};
     hipSetDevice(h->device);
     if (launch_generate(c, b->n, seed, rng_id, cutoff, h->stream) != hipSuccess)

@@ -43,6 +43,8 @@ SLICE_METHOD = {
     "specified": 4,
 }
 CENTER_POSITION = {"centroid": 0, "midpoint": 1}
+OCTOPIC_DTYPE_F64 = 0
+OCTOPIC_DTYPE_F32 = 1
 FIELD_DERIVATIVE = {"second": 2, "fourth": 4}
 
 
@@ -112,7 +114,7 @@ class BeamStruct(C.Structure):
         ("r0", C.c_double),
         ("npart", C.c_double),
         ("on_device", C.c_int32),
-        ("reserved", C.c_int32),
+        ("dtype", C.c_int32),
     ]
 
 

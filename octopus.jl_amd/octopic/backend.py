@@ -104,21 +104,25 @@ class Beam:
     def struct(self):
         b = _abi.BeamStruct()
         c = self.coords
-        assert c.dtype.is_floating_point and c.element_size() == 8 and c.is_contiguous()
+        torch = _torch()
+        if c.dtype not in (torch.float64, torch.float32) or not c.is_contiguous():
+            raise ValueError("beam coordinates must be one contiguous float64 or float32 [6, n] tensor")
         base = c.data_ptr()
-        stride = c.stride(0) * 8
+        stride = c.stride(0) * c.element_size()
         b.x, b.px, b.y, b.py, b.z, b.pz = (base + k * stride for k in range(6))
         b.n = self.n
         b.charge, b.mc2, b.E0, b.r0, b.npart = (float(self.params.charge), float(self.params.mc2),
                                                float(self.params.E0), float(self.params.r0),
                                                float(self.params.npart))
         b.on_device = 1
+        b.dtype = _abi.OCTOPIC_DTYPE_F32 if c.dtype == torch.float32 else _abi.OCTOPIC_DTYPE_F64
         return b
 
     @staticmethod
-    def from_host(coords, params: BeamParams, device="cuda"):
+    def from_host(coords, params: BeamParams, device="cuda", dtype=None):
+        """dtype: torch.float64 (default; Phase6DRep{Float64}) or torch.float32 (Phase6DRep{Float32})."""
         torch = _torch()
-        t = torch.as_tensor(coords, dtype=torch.float64).contiguous().to(device)
+        t = torch.as_tensor(coords, dtype=dtype or torch.float64).contiguous().to(device)
         assert t.ndim == 2 and t.shape[0] == 6
         return Beam(t, params)
 
@@ -362,13 +366,16 @@ def collide_many(colliders: Sequence["PICCollider"], beams1: Sequence[Beam], bea
     for c in colliders:
         c.set_stream(None)
     try:
-        lib().octopic_collide_many(hs, s1, s2, tv, cl, lums, st, n)
+        rc = lib().octopic_collide_many(hs, s1, s2, tv, cl, lums, st, n)
     finally:
         for c in colliders:
             c.set_stream(torch.cuda.current_stream(c.device).cuda_stream)
     for k in range(n):
         if st[k] != 0:
             colliders[k]._check(st[k])
+    if rc != 0:   # rejected before any instance ran: the message is on the instance that was checked
+        msgs = [lib().octopic_last_error(c.h).decode() for c in colliders]
+        _raise(rc, "; ".join(m for m in msgs if m) or "octopic_collide_many rejected its arguments")
     return list(lums)
 
 
