@@ -73,6 +73,8 @@ def lib():
         L.oracle_npairs.argtypes = [C.c_void_p]
         L.oracle_pair_luminosity.argtypes = [C.c_void_p, I32, I32, I32, D]
         L.oracle_records.argtypes = [C.c_void_p, C.POINTER(_abi.InteractionRecord), C.c_int64, I64]
+        L.oracle_node_mesh.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, D]
+        L.oracle_node_mesh.restype = C.c_int32
         L.oracle_last_slices.argtypes = [C.c_void_p, C.c_int32, D, D, D, I64, I64]
         L.oracle_slices.argtypes = [D, C.c_int64, C.POINTER(_abi.Slicing), D, D, D, I64, I64]
         L.oracle_generate_beam.argtypes = [C.c_int64, C.c_uint64, C.c_uint64, D, D, D, C.c_double, C.POINTER(D)]
@@ -407,6 +409,16 @@ class Collider:
         arr = (_abi.InteractionRecord * max(n.value, 1))()
         lib().oracle_records(self.h, arr, n.value, C.byref(n))
         return list(arr)[: n.value]
+
+    def node_mesh(self, direction, slice_index, node):
+        """The last collide's :node mesh of `node` (0-based boundary of the field beam's slicing) for source
+        slice `slice_index` of direction `direction` (1: beam 1 is the source): (source grid, field grid) as
+        (x0, y0, width, height) each, or None when the node has no mesh (_pic_build_node_grids!,
+        pic_cpu.jl:1317-1382)."""
+        out = np.zeros(8)
+        if not lib().oracle_node_mesh(self.h, int(direction), int(slice_index), int(node), _d(out)):
+            return None
+        return tuple(out[:4]), tuple(out[4:])
 
     def last_slices(self, which, n):
         sl = self.solver.resolved_slicing1 if which == 1 else self.solver.resolved_slicing2

@@ -1496,6 +1496,8 @@ struct CollideOut {
 struct State {
     std::unique_ptr<Context> C;
     std::vector<octopic_interaction_record> records;
+    // :node meshes of the last collide: (dir, source slice, node) -> source and field mesh
+    std::map<std::tuple<int, int, int>, std::pair<Grid, Grid>> node_meshes;
     Slices last1, last2;
     CollideOut last;
 };
@@ -1545,6 +1547,10 @@ void collide(State &S, double *b1[6], int64_t n1, const double p1[5], double *b2
             build_node_grids(C, *C.ws[0], node_cache[{j, 2}], src, s2.center[j], rep1, s1);
         }
     }
+    S.node_meshes.clear();
+    for (const auto &kv : node_cache)
+        for (const auto &nb : kv.second)
+            S.node_meshes[std::make_tuple(kv.first.second, kv.first.first, nb.first)] = {nb.second.sg, nb.second.fg};
     auto order = collision_order(s1.center, s2.center);
     int npairs = (int)order.size();
     std::map<std::pair<int, int>, int> pair_pos;
@@ -2028,6 +2034,19 @@ int oracle_stats(void *s, int64_t out[6]) {
     out[0] = S.C->cache.hits; out[1] = S.C->cache.misses; out[2] = S.C->cache.rebuilds;
     out[3] = S.last.dropped; out[4] = S.last.batches; out[5] = S.last.widest;
     return 0;
+}
+
+// The last collide's :node mesh of node `node` (0-based boundary index) for source slice `slice` of
+// direction `dir` (1: beam 1 is the source): source grid then field grid, (x0, y0, width, height) each.
+// Returns 1, or 0 when that node has no mesh (no field particle in its adjacent slices).
+int oracle_node_mesh(void *s, int dir, int slice, int node, double *out8) {
+    State &S = *static_cast<State *>(s);
+    auto it = S.node_meshes.find(std::make_tuple(dir, slice, node));
+    if (it == S.node_meshes.end()) return 0;
+    const Grid &a = it->second.first, &b = it->second.second;
+    const double v[8] = {a.x0, a.y0, a.width, a.height, b.x0, b.y0, b.width, b.height};
+    for (int k = 0; k < 8; ++k) out8[k] = v[k];
+    return 1;
 }
 
 int64_t oracle_npairs(void *s) { return (int64_t)static_cast<State *>(s)->last.pi.size(); }
