@@ -58,6 +58,9 @@ enum {
 };
 enum { OCTOPIC_CENTER_CENTROID = 0, OCTOPIC_CENTER_MIDPOINT = 1 };
 enum { OCTOPIC_PRECISION_FP64 = 0, OCTOPIC_PRECISION_FP32 = 1 };
+/* The solver a handle runs: PICPoissonSolver, or GaussianPICPoissonSolver (gaussian_pic.jl:87-131), the
+ * Gaussian-subtracted hybrid that composes it. */
+enum { OCTOPIC_SOLVER_PIC = 0, OCTOPIC_SOLVER_GAUSSIAN_PIC = 1 };
 
 /* LongitudinalSlicing (interface.jl:800-835).  `positions` holds nslices-1 internal boundaries in
  * units of the bunch rms (method SPECIFIED only); it may be NULL otherwise. */
@@ -101,6 +104,17 @@ typedef struct octopic_options {
     double green_growth;
     double kbb1, kbb2, luminosity_scale;
     octopic_slicing slicing1, slicing2;
+    /* GaussianPICPoissonSolver (gaussian_pic.jl:87-131), with every field above its embedded
+     * PICPoissonSolver: solver = OCTOPIC_SOLVER_GAUSSIAN_PIC subtracts each source slice's drifted
+     * Gaussian (its own moments) from the deposits before the solve and adds the exact Bassetti-Erskine
+     * field of that Gaussian back per field particle.  margin_sigma >= 0 widens the source mesh to
+     * +-margin_sigma of the Gaussian; neutralize scales the subtracted Gaussian to the deposited total;
+     * coupling_tol is the correlation threshold of the rotated subtraction (only +Inf, the reference's
+     * default, the separable subtraction, is built here).  Ignored for OCTOPIC_SOLVER_PIC. */
+    int32_t solver;                /* OCTOPIC_SOLVER_* */
+    int32_t neutralize;            /* 0 | 1 */
+    double margin_sigma;
+    double coupling_tol;
 } octopic_options;
 
 /* Element type of a beam's coordinate arrays: Phase6DRep{Float64} or Phase6DRep{Float32}
@@ -216,6 +230,14 @@ int octopic_track_crab_cavity(octopic_handle *h, double frequency, int32_t nharm
  * size gives zero kicks.  FP64; device pointers; on the handle's stream. */
 int octopic_gaussian_beambeam_kick(octopic_handle *h, const double *x, const double *y, int64_t n, double sigx,
                                    double sigy, double *kx, double *ky);
+
+/* _gaussian_beambeam_kick_response(kbb, sigx, sigy, x, y) (src/elements/strong_beam.jl:1061-1128): per point
+ * out4[4i..4i+3] = (Kx, Ky, Hxx, Hyy), the normalised Bassetti-Erskine kick and kbb times the diagonal of
+ * the Gaussian potential's Hessian (with the near-round blend's chain term), as the Gaussian-subtracted
+ * solver's longitudinal add-back uses them.  Sizes must be positive.  FP64; device pointers; on the
+ * handle's stream. */
+int octopic_gaussian_beambeam_kick_response(octopic_handle *h, const double *x, const double *y, int64_t n, double kbb,
+                                            double sigx, double sigy, double *out4);
 
 /* faddeeva_w_approx_reim(zr, zi) (src/math/SpecialMath.jl:60-151): w(z) = exp(-z^2) erfc(-iz) by
  * Weideman's 32-term rational approximation, the Laplace continued fraction for large |z| and the

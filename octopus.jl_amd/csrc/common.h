@@ -142,6 +142,10 @@ struct SliceTable {
 };
 constexpr int kMaxSlices = 256;
 
+// The source slice's Gaussian drifted to one field-slice boundary (Gaussian-subtracted solver): centroid,
+// rms sizes and the drift derivatives of the variances, d(sigma^2)/ds (gaussian_pic.jl:482-523).
+struct GaussBnd { double mux, muy, sigx, sigy, rx, ry; };
+
 // One directed interaction of a wavefront batch (a source slice solved on a mesh, kicking a field slice).
 struct Inter {
     int32_t p, i, j, dir;          // pair position in collision order, slices, direction (1, 2)
@@ -157,7 +161,9 @@ struct Inter {
     int32_t valid;
     Grid fgR;                      // node mode: the right node's field mesh
     int32_t count_field;           // 1: count field particles outside fg after the forward drift
-    int32_t reserved2;
+    int32_t gauss;                 // 1: Gaussian-subtracted planes with the exact field added back per particle
+    GaussBnd g[2];                 // the source's drifted Gaussian at sL and sR (gauss = 1)
+    double gmpx, gmpy;             // the source's mean px, py (gauss = 1)
 };
 
 // One deposited plane: a source slice drifted by s onto a mesh.

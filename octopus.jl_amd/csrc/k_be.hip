@@ -225,6 +225,211 @@ __device__ void principal_kick(const WeidemanCoeffs &wc, double s1, double s2, d
     ky = sy + w * (ey - sy);
 }
 
+// ---- the kick with the diagonal of the potential's Hessian (_gaussian_beambeam_kick_response,
+// strong_beam.jl:1061-1128).  K is normalised; h1, h2 carry the factor kbb.  Used by the Gaussian-subtracted
+// solver's add-back, whose pz term needs d(potential)/d(sigma^2) = Hessian / 2 (heat equation), with the
+// blend's chain term where the near-round series and the elliptic form are blended (:1095-1115).
+
+// round beam Hessian (strong_beam.jl:711-772): f = 2(1 - e)/r^2, fp = df/d(r^2), series below u = 0.01
+__device__ void round_hessian(double kbb, double sigma, double x, double y, double expterm, double &hxx, double &hyy) {
+    const double r2 = x * x + y * y, sigma2 = sigma * sigma;
+    if (r2 == 0.0) { hxx = hyy = -kbb / sigma2; return; }
+    const double u = r2 / (2.0 * sigma2);
+    double f, fp;
+    if (u < 1.0e-2) {
+        const double phi = 1.0 + u * (-1.0 / 2 + u * (1.0 / 6 + u * (-1.0 / 24 + u * (1.0 / 120 + u * (-1.0 / 720 + u * (1.0 / 5040))))));
+        const double dphi =
+            -1.0 / 2 + u * (1.0 / 3 + u * (-1.0 / 8 + u * (1.0 / 30 + u * (-1.0 / 144 + u * (1.0 / 840 - u * (1.0 / 5760))))));
+        const double inv = 1.0 / sigma2;
+        f = phi * inv;
+        fp = dphi * inv * inv / 2.0;
+    } else {
+        const double ome = 1.0 - expterm;
+        f = 2.0 * ome / r2;
+        fp = 2.0 * (expterm * u - ome) / (r2 * r2);
+    }
+    hxx = -kbb * (f + 2.0 * x * x * fp);
+    hyy = -kbb * (f + 2.0 * y * y * fp);
+}
+
+// near-round series with its Hessian diagonal (strong_beam.jl:897-937)
+__device__ void near_round_response(double kbb, double v, double eta, double x, double y, double &kx, double &ky,
+                                    double &h1, double &h2) {
+    const double q = (x * x + y * y) / (2.0 * v), dr = (x * x - y * y) / (2.0 * v);
+    double m[7];
+    moments_0_6(q, m);
+    const double e2 = eta * eta, e3 = e2 * eta;
+    const double u1 = dr * m[1];
+    const double u2 = m[1] / 2.0 - q * m[2] + (dr * dr / 2.0) * m[3];
+    const double u3 = 1.5 * dr * m[3] - dr * q * m[4] + dr * dr * dr / 6.0 * m[5];
+    const double v1 = dr * (q * m[2] - m[1]);
+    const double v2 = 1.5 * q * m[2] - (q * q + dr * dr) * m[3] + (q * dr * dr / 2.0) * m[4];
+    const double v3 = -1.5 * dr * m[3] + 3.5 * dr * q * m[4] - (dr * q * q + dr * dr * dr / 2.0) * m[5] + dr * dr * dr * q / 6.0 * m[6];
+    const double c1x = dr * m[2] - m[1], c1y = dr * m[2] + m[1];
+    const double c2x = 1.5 * m[2] + (-dr - q) * m[3] + (dr * dr / 2.0) * m[4];
+    const double c2y = 1.5 * m[2] + (dr - q) * m[3] + (dr * dr / 2.0) * m[4];
+    const double c3x = -1.5 * m[3] + (2.5 * dr + q) * m[4] + (-dr * q - dr * dr / 2.0) * m[5] + dr * dr * dr / 6.0 * m[6];
+    const double c3y = 1.5 * m[3] + (2.5 * dr - q) * m[4] + (-dr * q + dr * dr / 2.0) * m[5] + dr * dr * dr / 6.0 * m[6];
+    kx = x / v * (m[0] + eta * c1x + e2 * c2x + e3 * c3x);
+    ky = y / v * (m[0] + eta * c1y + e2 * c2y + e3 * c3y);
+    const double uv = -exp(-q) + eta * v1 + e2 * v2 + e3 * v3;
+    const double ueta = u1 + 2.0 * eta * u2 + 3.0 * e2 * u3;
+    h1 = kbb / v * (uv + (1.0 - eta) * ueta);
+    h2 = kbb / v * (uv - (1.0 + eta) * ueta);
+}
+
+// U_exact - U_series through eta^6 (strong_beam.jl:939-960), from the radial moments m3..m11
+__device__ double near_round_potential_residual(double v, double eta, double x, double y) {
+    const double q = (x * x + y * y) / (2.0 * v), dr = (x * x - y * y) / (2.0 * v);
+    double m[12];
+    if (q <= 2.0) {
+        for (int j = 3; j < 12; ++j) m[j] = 0.0;
+        double term = 1.0;
+        for (int k = 0; k <= 24; ++k) {
+            for (int j = 3; j < 12; ++j) m[j] += term / (double)(k + 1 + j);
+            term *= -q / (double)(k + 1);
+        }
+    } else {
+        const double eq = exp(-q);
+        double prev = -expm1(-q) / q;
+        for (int j = 1; j < 12; ++j) {
+            prev = ((double)j * prev - eq) / q;
+            m[j] = prev;
+        }
+    }
+    const double dr2 = dr * dr, q2 = q * q;
+    const double u4 = 3.0 / 8.0 * m[3] - 1.5 * q * m[4] + (5.0 * dr2 + 2.0 * q2) / 4.0 * m[5] - dr2 * q / 2.0 * m[6] +
+                      dr2 * dr2 / 24.0 * m[7];
+    const double u5 = 15.0 / 8.0 * dr * m[5] - 2.5 * dr * q * m[6] + dr * (7.0 * dr2 + 6.0 * q2) / 12.0 * m[7] -
+                      dr * dr2 * q / 6.0 * m[8] + dr * dr2 * dr2 / 120.0 * m[9];
+    const double u6 = 5.0 / 16.0 * m[5] - 15.0 / 8.0 * q * m[6] + 5.0 * (7.0 * dr2 + 4.0 * q2) / 16.0 * m[7] -
+                      q * (21.0 * dr2 + 2.0 * q2) / 12.0 * m[8] + dr2 * (3.0 * dr2 + 4.0 * q2) / 16.0 * m[9] -
+                      dr2 * dr2 * q / 24.0 * m[10] + dr2 * dr2 * dr2 / 720.0 * m[11];
+    return eta * eta * eta * eta * (u4 + eta * (u5 + eta * u6));
+}
+
+// d/dx of the axis series component (strong_beam.jl:974-1007), times -kbb
+__device__ double axis_hessian(double kbb, double sx, double sy, double x, double y) {
+    const double s = sx + sy, sx2 = sx * sx, sx3 = sx2 * sx, sx5 = sx3 * sx2, sy2 = sy * sy, sy3 = sy2 * sy;
+    const double s2 = s * s, s3 = s2 * s;
+    const double j00 = 2.0 / (sx * s);
+    const double j10 = 2.0 * (2.0 * sx + sy) / (3.0 * sx3 * s2);
+    const double j01 = 2.0 / (sx * sy * s2);
+    const double j20 = 2.0 * (8.0 * sx2 + 9.0 * sx * sy + 3.0 * sy2) / (15.0 * sx5 * s3);
+    const double j11 = 2.0 * (3.0 * sx + sy) / (3.0 * sx3 * sy * s3);
+    const double j02 = 2.0 * (sx + 3.0 * sy) / (3.0 * sx * sy3 * s3);
+    const double x2 = x * x, y2 = y * y;
+    const double derivative = j00 - 1.5 * x2 * j10 - y2 * j01 / 2.0 + 0.625 * x2 * x2 * j20 + 0.75 * x2 * y2 * j11 + y2 * y2 * j02 / 8.0;
+    return -kbb * derivative;
+}
+
+// sig1 >= sig2 (strong_beam.jl:1061-1117)
+__device__ void principal_response(const WeidemanCoeffs &wc, double kbb, double s1, double s2, double x, double y,
+                                   double &kx, double &ky, double &h1, double &h2) {
+    const double v = (s1 * s1 + s2 * s2) / 2.0;
+    const double eta = (s1 * s1 - s2 * s2) / (2.0 * v);
+    if (eta == 0.0) {
+        const double r2 = x * x + y * y, u = r2 / (2.0 * v);
+        const double scale = (u == 0.0 ? 1.0 : -expm1(-u) / u) / v;
+        kx = scale * x;
+        ky = scale * y;
+        round_hessian(kbb, sqrt(v), x, y, exp(-r2 / (2.0 * v)), h1, h2);
+        return;
+    }
+    const double outer = near_round_outer(), inner = outer / 2.0;
+    if (eta <= inner) {
+        near_round_response(kbb, v, eta, x, y, kx, ky, h1, h2);
+        return;
+    }
+    double ex, ey;
+    elliptic_kick(wc, s1, s2, x, y, eta, ex, ey);
+    const double expterm = exp(-0.5 * (x * x / (s1 * s1) + y * y / (s2 * s2)));
+    double he1, he2;
+    const double rho2 = x * x / (s1 * s1) + y * y / (s2 * s2);
+    if (rho2 * rho2 * rho2 * sqrt(rho2) <= 0x1p-52 / sqrt(eta)) {   // the axis series' Hessian (:1009-1014)
+        he1 = axis_hessian(kbb, s1, s2, x, y);
+        he2 = axis_hessian(kbb, s2, s1, y, x);
+    } else {   // _elliptic_gaussian_hessian_diagonal (:695-703)
+        const double t1 = kbb * (x * ex + y * ey), t2 = s1 * s1 - s2 * s2, t3 = s2 / s1;
+        he1 = (t1 - 2.0 * kbb * (1.0 - expterm * t3)) / t2;
+        he2 = (-t1 + 2.0 * kbb * (1.0 - expterm / t3)) / t2;
+    }
+    if (eta >= outer) {
+        kx = ex; ky = ey; h1 = he1; h2 = he2;
+        return;
+    }
+    const double invw = 1.0 / (outer - inner), t = (eta - inner) * invw;
+    const double w = t * t * t * (10.0 + t * (-15.0 + 6.0 * t));
+    const double dw = 30.0 * t * t * (t - 1.0) * (t - 1.0) * invw;
+    double sx, sy, hs1, hs2;
+    near_round_response(kbb, v, eta, x, y, sx, sy, hs1, hs2);
+    const double chain = kbb * dw * near_round_potential_residual(v, eta, x, y) / v;
+    kx = sx + w * (ex - sx);
+    ky = sy + w * (ey - sy);
+    h1 = hs1 + w * (he1 - hs1) + (1.0 - eta) * chain;
+    h2 = hs2 + w * (he2 - hs2) - (1.0 + eta) * chain;
+}
+
+__device__ void beambeam_response(const WeidemanCoeffs &wc, double kbb, double sigx, double sigy, double x, double y,
+                                  double &kx, double &ky, double &hx, double &hy) {
+    if (sigx >= sigy) principal_response(wc, kbb, sigx, sigy, x, y, kx, ky, hx, hy);
+    else principal_response(wc, kbb, sigy, sigx, y, x, ky, kx, hy, hx);
+}
+
+__global__ __launch_bounds__(256) void k_be_response(WeidemanCoeffs wc, const double *X, const double *Y, int64_t n,
+                                                     double kbb, double sigx, double sigy, double *OUT) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double kx, ky, hx, hy;
+    beambeam_response(wc, kbb, sigx, sigy, X[i], Y[i], kx, ky, hx, hy);
+    OUT[4 * i] = kx; OUT[4 * i + 1] = ky; OUT[4 * i + 2] = hx; OUT[4 * i + 3] = hy;
+}
+
+// The Gaussian-subtracted solver's add-back per field particle (_gpic_apply_kick_range!, gaussian_pic.jl:
+// 818-892, uncoupled): at the particle's forward-drifted position (the kick stage recomputes the same
+// drift), the exact field of the source's drifted Gaussians at the two boundaries, interpolated as the
+// mesh kick.  Writes (dpx, dpy, pz covariance term, pz centroid term) for the kick stage to add.
+__global__ __launch_bounds__(256) void k_gpic_addback(WeidemanCoeffs wc, GpicAddbackArgs a) {
+    const int d = blockIdx.y;
+    const Inter &it = a.inters[d];
+    if (!it.valid || !it.gauss || *((volatile const int32_t *)&a.err->code) != 0) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= it.fld_n) return;
+    const int fb = it.fld_beam;
+    const int64_t k = it.fld_off + t;
+    const double x0 = a.st.c[fb][0][k], px = a.st.c[fb][1][k], y0 = a.st.c[fb][2][k], py = a.st.c[fb][3][k];
+    const double z = a.st.c[fb][4][k];
+    const double s = 0.5 * (z - it.c_src);
+    const double x = x0 + s * px, y = y0 + s * py;
+    double hzi, zbias;
+    {
+        const double denom = it.rb - it.lb;
+        if (!isfinite(denom) || denom == 0.0) { hzi = 0.0; zbias = 0.5; }
+        else { hzi = 1.0 / denom; zbias = it.rb * hzi; }
+    }
+    const double zl = -z * hzi + zbias;
+    const double zL = zl > 1.0 ? 1.0 : (zl < 0.0 ? 0.0 : zl), zR = 1.0 - zL;
+    const double ks = it.kick_scale, half_ns = 0.5 * (double)it.src_n, kbb_eff = ks * half_ns;
+    double kxL = 0, kyL = 0, hxL = 0, hyL = 0, kxR = 0, kyR = 0, hxR = 0, hyR = 0;
+    if (a.lkick) {
+        beambeam_response(wc, kbb_eff, it.g[0].sigx, it.g[0].sigy, x - it.g[0].mux, y - it.g[0].muy, kxL, kyL, hxL, hyL);
+        beambeam_response(wc, kbb_eff, it.g[1].sigx, it.g[1].sigy, x - it.g[1].mux, y - it.g[1].muy, kxR, kyR, hxR, hyR);
+    } else {
+        if (it.g[0].sigx >= it.g[0].sigy) principal_kick(wc, it.g[0].sigx, it.g[0].sigy, x - it.g[0].mux, y - it.g[0].muy, kxL, kyL);
+        else principal_kick(wc, it.g[0].sigy, it.g[0].sigx, y - it.g[0].muy, x - it.g[0].mux, kyL, kxL);
+        if (it.g[1].sigx >= it.g[1].sigy) principal_kick(wc, it.g[1].sigx, it.g[1].sigy, x - it.g[1].mux, y - it.g[1].muy, kxR, kyR);
+        else principal_kick(wc, it.g[1].sigy, it.g[1].sigx, y - it.g[1].muy, x - it.g[1].mux, kyR, kxR);
+    }
+    const double Kxa = half_ns * (zL * kxL + zR * kxR), Kya = half_ns * (zL * kyL + zR * kyR);
+    const double dpxa = ks * Kxa, dpya = ks * Kya;
+    const double covL = (hxL * it.g[0].rx + hyL * it.g[0].ry) / 4.0, covR = (hxR * it.g[1].rx + hyR * it.g[1].ry) / 4.0;
+    double *o = a.ab + 4 * (a.base[fb] + k);
+    o[0] = dpxa;
+    o[1] = dpya;
+    o[2] = zL * covL + zR * covR;
+    o[3] = 0.5 * (dpxa * it.gmpx + dpya * it.gmpy);
+}
+
 __global__ __launch_bounds__(256) void k_be_kick(WeidemanCoeffs wc, const double *X, const double *Y, int64_t n,
                                                  double sigx, double sigy, double *KX, double *KY) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -283,6 +488,21 @@ hipError_t launch_be_kick(const double *x, const double *y, int64_t n, double si
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_be_kick, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, weideman_coeffs(), x, y, n, sigx,
                        sigy, kx, ky);
+    return hipGetLastError();
+}
+
+hipError_t launch_be_response(const double *x, const double *y, int64_t n, double kbb, double sigx, double sigy,
+                              double *out4, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_be_response, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, weideman_coeffs(), x, y, n, kbb,
+                       sigx, sigy, out4);
+    return hipGetLastError();
+}
+
+hipError_t launch_gpic_addback(const GpicAddbackArgs &a, int64_t max_fld, hipStream_t s) {
+    if (a.ninters <= 0 || max_fld <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gpic_addback, dim3((unsigned)((max_fld + 255) / 256), a.ninters), dim3(256), 0, s,
+                       weideman_coeffs(), a);
     return hipGetLastError();
 }
 

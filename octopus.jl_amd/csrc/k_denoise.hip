@@ -1121,6 +1121,121 @@ __global__ __launch_bounds__(256) void k_dn_apply(DnCollideArgs a, const float *
     }
 }
 
+// ------------------------------------------------------------------ Gaussian-subtracted solver
+// Source moments of interaction d = 2k + dir - 1 of a batch (gaussian_pic.jl:374-435, uncoupled), before
+// the geometry stage: the source slice of pair k is beam 1's slice i (dir 1) or beam 2's slice j (dir 2).
+// Sums of x, px, y, py shifted by the slice's first member, per chunk of members, in fixed order.
+__global__ __launch_bounds__(256) void k_gpic_moments(GpicMomentArgs a) {
+    __shared__ double red[10][256];
+    const int d = blockIdx.y, c = blockIdx.x, k = d >> 1, side = d & 1;
+    const BatchPair pr = a.pairs[k];
+    const SliceTable *tab = side == 0 ? a.tab1 : a.tab2;
+    const int sl = side == 0 ? pr.i : pr.j;
+    const int64_t off = tab->offset[sl], n = tab->count[sl];
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (n > 0 && !failed(a.err)) {
+        const double *X = a.st.c[side][0] + off, *PX = a.st.c[side][1] + off, *Y = a.st.c[side][2] + off,
+                     *PY = a.st.c[side][3] + off;
+        const double x0 = X[0], px0 = PX[0], y0 = Y[0], py0 = PY[0];
+        const int64_t b = n * c / a.chunks, e = n * (c + 1) / a.chunks;
+        for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+            const double dx = X[i] - x0, dpx = PX[i] - px0, dy = Y[i] - y0, dpy = PY[i] - py0;
+            acc[0] += dx; acc[1] += dpx; acc[2] += dy; acc[3] += dpy;
+            acc[4] += dx * dx; acc[5] += dpx * dpx; acc[6] += dy * dy; acc[7] += dpy * dpy;
+            acc[8] += dx * dpx; acc[9] += dy * dpy;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 10; ++j) red[j][threadIdx.x] = acc[j];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int j = 0; j < 10; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 10) a.mom_part[((int64_t)d * a.chunks + c) * 10 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// One block per field plane of an interaction that subtracts (gaussian_pic.jl:543-595): the erf profile of
+// the source's Gaussian drifted to the plane's boundary, on the plane's mesh (node integrals against the
+// CIC tent / TSC spline, in cell units), scaled to the deposited total (neutralize) or to the slice
+// population, subtracted from the fixed-point deposit.  The deposit's total is an exact integer sum.
+__global__ __launch_bounds__(256) void k_gpic_residual(GpicResidualArgs a) {
+    extern __shared__ double g[];   // gx [nx], gy [ny], then the erf / pdf tables
+    __shared__ double red[256];
+    __shared__ long long ired[256];
+    __shared__ double tot[2];
+    __shared__ long long qtot;
+    const int p = blockIdx.x, nx = a.nx, ny = a.ny, HW = nx * ny;
+    const Plane &pl = a.planes[p];
+    const Inter &it = a.inters[pl.inter];
+    if (!it.valid || !it.gauss || failed(a.err)) return;
+    const GaussBnd &gb = it.g[p - it.plane0];
+    const double mu_x = (gb.mux - pl.x0) / pl.hx, mu_y = (gb.muy - pl.y0) / pl.hy;
+    const double sg_x = gb.sigx / pl.hx, sg_y = gb.sigy / pl.hy;
+    {
+        double *Ex = g + nx + ny, *Gx = Ex + (nx + 3), *Ey = Gx + (nx + 3), *Gy = Ey + (ny + 3);
+        const double off = a.method == 0 ? 1.0 : 1.5;
+        for (int k = threadIdx.x; k < nx + 3 + ny + 3; k += blockDim.x) {
+            const bool xa = k < nx + 3;
+            const int kk = xa ? k : k - (nx + 3);
+            const double mu = xa ? mu_x : mu_y, sg = xa ? sg_x : sg_y;
+            const double invroot = 1.0 / (sg * 1.4142135623730951), norm = 1.0 / (sg * 2.5066282746310002);
+            const double q = (double)kk - off;
+            (xa ? Ex : Ey)[kk] = erf((q - mu) * invroot);
+            (xa ? Gx : Gy)[kk] = norm * exp(-((q - mu) * (q - mu)) / (2.0 * sg * sg));
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nx; i += blockDim.x) g[i] = gauss_node_weight_tab(i, mu_x, sg_x, a.method, Ex, Gx);
+        for (int j = threadIdx.x; j < ny; j += blockDim.x) g[nx + j] = gauss_node_weight_tab(j, mu_y, sg_y, a.method, Ey, Gy);
+    }
+    __syncthreads();
+    unsigned long long *q = a.charge + (int64_t)p * a.charge_stride;
+    long long qs = 0;
+    for (int c = threadIdx.x; c < HW; c += blockDim.x) qs += (long long)q[c];
+    double sx = 0.0, sy = 0.0;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) sx += g[i];
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) sy += g[nx + j];
+    ired[threadIdx.x] = qs;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) ired[threadIdx.x] += ired[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) qtot = ired[0];
+    for (int v = 0; v < 2; ++v) {   // fixed-order tree reductions of the profile sums
+        red[threadIdx.x] = v == 0 ? sx : sy;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) tot[v] = red[0];
+        __syncthreads();
+    }
+    const double sgx = tot[0], sgy = tot[1];
+    double amp = (double)it.src_n;
+    if (a.neutralize && sgx * sgy > 0.0) amp = ((double)qtot * a.inv_scale) / (sgx * sgy);
+    for (int c = threadIdx.x; c < HW; c += blockDim.x) {
+        const int j = c / nx, i = c - j * nx;
+        const double v = g[i] * (amp * g[nx + j]);
+        q[c] = (unsigned long long)((long long)q[c] - llrint(v * a.scale));
+    }
+}
+
+hipError_t launch_gpic_moments(const GpicMomentArgs &a, hipStream_t s) {
+    if (a.npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gpic_moments, dim3(a.chunks, 2 * a.npairs), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_gpic_residual(const GpicResidualArgs &a, hipStream_t s) {
+    if (a.nplanes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gpic_residual, dim3(a.nplanes), dim3(256), (size_t)(3 * (a.nx + a.ny) + 12) * sizeof(double), s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_dn_prepare(const DnCollideArgs &a, hipStream_t s) {
     if (a.nplanes <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dn_moments, dim3(a.chunks, a.ninters), dim3(256), 0, s, a);

@@ -277,6 +277,7 @@ __device__ __forceinline__ void neutralise_inter(int k, int dir, int npairs, con
     inters[d].valid = 0;
     inters[d].fld_n = 0;
     inters[d].nplanes = 0;
+    inters[d].gauss = 0;
     if (jobs) { jobs[d].valid = 0; jobs[d].slot = -1; }
     for (int q = 0; q < o.npl; ++q) {
         Plane &pl = planes[d * o.npl + q];
@@ -307,6 +308,40 @@ __device__ __forceinline__ void fill_inter(Inter &it, const GeomOpts &o, int p, 
     it.kick_scale = 2.0 * (dir == 1 ? o.kbb2 : o.kbb1);
 }
 
+// The source slice's Gaussian at the two field-slice boundary drifts (_gpic_source_moments,
+// _gpic_drifted_gaussian, _gpic_boundary, gaussian_pic.jl:374-523, uncoupled) from the moment partials of
+// interaction d, folded in chunk order.  Returns whether the control variate is defined
+// (_gpic_control_variate_mode, :525-541): at least two members and positive finite widths at both.
+__device__ bool gpic_gaussians(const GeometryArgs &a, int d, int beam, int64_t off, int64_t n, double sL, double sR,
+                               GaussBnd g[2], double &mpx, double &mpy) {
+    if (n < 1) return false;
+    double sm[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < a.o.mom_chunks; ++c)
+        for (int j = 0; j < 10; ++j) sm[j] += a.mom_part[((size_t)d * a.o.mom_chunks + c) * 10 + j];
+    const double x0 = a.st.c[beam][0][off], px0 = a.st.c[beam][1][off], y0 = a.st.c[beam][2][off], py0 = a.st.c[beam][3][off];
+    const double invn = 1.0 / (double)n;
+    const double dmx = sm[0] * invn, dmpx = sm[1] * invn, dmy = sm[2] * invn, dmpy = sm[3] * invn;
+    const double mx = x0 + dmx, my = y0 + dmy;
+    mpx = px0 + dmpx;
+    mpy = py0 + dmpy;
+    // _shifted_second_moment / _shifted_cross_moment (slicing.jl:664-668): muladd(-a, b, sum * invn)
+    const double varx = jmax(fma(-dmx, dmx, sm[4] * invn), 0.0), varpx = jmax(fma(-dmpx, dmpx, sm[5] * invn), 0.0);
+    const double vary = jmax(fma(-dmy, dmy, sm[6] * invn), 0.0), varpy = jmax(fma(-dmpy, dmpy, sm[7] * invn), 0.0);
+    const double cxpx = fma(-dmx, dmpx, sm[8] * invn), cypy = fma(-dmy, dmpy, sm[9] * invn);
+    bool ok = n >= 2;
+    for (int b = 0; b < 2; ++b) {
+        const double s = b == 0 ? sL : sR;
+        g[b].mux = mx + mpx * s;
+        g[b].muy = my + mpy * s;
+        g[b].sigx = sqrt(jmax(varx + 2.0 * s * cxpx + s * s * varpx, 0.0));
+        g[b].sigy = sqrt(jmax(vary + 2.0 * s * cypy + s * s * varpy, 0.0));
+        g[b].rx = 2.0 * (cxpx + s * varpx);
+        g[b].ry = 2.0 * (cypy + s * varpy);
+        ok = ok && __builtin_isfinite(g[b].sigx) && __builtin_isfinite(g[b].sigy) && g[b].sigx > 0.0 && g[b].sigy > 0.0;
+    }
+    return ok;
+}
+
 // one lane per (pair k, direction): 2*npairs lanes
 __global__ void k_geometry(GeometryArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
@@ -324,6 +359,24 @@ __global__ void k_geometry(GeometryArgs a) {
     Inter &it = a.inters[d];
     if (!finite4(vs[0], vs[1], vs[2], vs[3])) { set_error(a.err, OCTOPIC_ERR_NONFINITE, p, dir, 0, 0, 0); return; }
     if (!finite4(vf[0], vf[1], vf[2], vf[3])) { set_error(a.err, OCTOPIC_ERR_NONFINITE, p, dir, 1, 0, 0); return; }
+    // Gaussian-subtracted solver: the source mesh also holds +-margin_sigma of the subtracted Gaussian
+    // (gaussian_pic.jl:687-699); without a defined Gaussian the interaction is plain PIC
+    bool gauss = false;
+    GaussBnd gb[2];
+    double gmpx = 0.0, gmpy = 0.0;
+    if (o.gauss) {
+        const double c_src = ts->center[si];
+        const double sL = 0.5 * (c_src - tf->boundary[fi]), sR = 0.5 * (c_src - tf->boundary[fi + 1]);
+        gauss = gpic_gaussians(a, d, dir == 1 ? 0 : 1, ts->offset[si], ts->count[si], sL, sR, gb, gmpx, gmpy);
+        if (gauss && o.margin > 0.0) {
+            const double m = o.margin;
+            vs[0] = jmin(vs[0], jmin(gb[0].mux - m * gb[0].sigx, gb[1].mux - m * gb[1].sigx));
+            vs[1] = jmax(vs[1], jmax(gb[0].mux + m * gb[0].sigx, gb[1].mux + m * gb[1].sigx));
+            vs[2] = jmin(vs[2], jmin(gb[0].muy - m * gb[0].sigy, gb[1].muy - m * gb[1].sigy));
+            vs[3] = jmax(vs[3], jmax(gb[0].muy + m * gb[0].sigy, gb[1].muy + m * gb[1].sigy));
+            if (!finite4(vs[0], vs[1], vs[2], vs[3])) { set_error(a.err, OCTOPIC_ERR_NONFINITE, p, dir, 0, 0, 0); return; }
+        }
+    }
     Grid sg, fg;
     if (!interaction_grids(o, vs[0], vs[1], vs[2], vs[3], vf[0], vf[1], vf[2], vf[3], sg, fg)) {
         set_error(a.err, OCTOPIC_ERR_DEGENERATE_EXTENT, p, dir, 3, jmax(vs[1] - vs[0], vf[1] - vf[0]),
@@ -379,6 +432,12 @@ __global__ void k_geometry(GeometryArgs a) {
     if (build) atomicAdd(&a.counters[3], 1ull);
     fill_inter(it, o, p, i, j, dir, ts, tf, si, fi);
     it.sg = sg; it.fg = fg; it.fgR = fg;
+    if (gauss) {
+        it.g[0] = gb[0];
+        it.g[1] = gb[1];
+        it.gmpx = gmpx;
+        it.gmpy = gmpy;
+    }
     it.plane0 = d * o.npl;
     it.nplanes = o.npl;
     it.event = event;
@@ -390,6 +449,7 @@ __global__ void k_geometry(GeometryArgs a) {
         set_plane(a.planes[d * o.npl + 2], d, 0, src_beam, o.nx, o.ny, slot, it.src_off, it.src_n, 0.5 * (it.sL + it.sR),
                   sg.x0, sg.y0, hx, hy);
     fill_record(a.records[2 * p + dir - 1], p, i, j, dir, event, sg, fg);
+    it.gauss = gauss ? 1 : 0;
     it.valid = 1;
     // after this lane's own checks, which see every x, px, y, py of both slices
     if (dir == 1 && o.compute_lum)
@@ -906,11 +966,12 @@ __device__ __forceinline__ void kick_item(const KickArgs &a, R *win, int d, int 
         }
         if (!act) continue;
         if (MODE != 2) {
+            constexpr int GM = MODE == 3 ? 0 : MODE;   // the add-back mode gathers as the linear mode
             if (bx >= wx0 && bx + K <= wx0 + wnx && by >= wy0 && by + K <= wy0 + wny)
-                gather_field_lds<METHOD, MODE, R>(win + (bx - wx0) * wrs + (by - wy0) * node_stride<R>(C), wrs,
-                                                  node_stride<R>(C), wx, wy, zL, Kx, Ky, Kz);
+                gather_field_lds<METHOD, GM, R>(win + (bx - wx0) * wrs + (by - wy0) * node_stride<R>(C), wrs,
+                                                node_stride<R>(C), wx, wy, zL, Kx, Ky, Kz);
             else
-                gather_field<METHOD, MODE, R>(tab + ((size_t)bx * ny + by) * C, ny * C, C, wx, wy, zL, Kx, Ky, Kz);
+                gather_field<METHOD, GM, R>(tab + ((size_t)bx * ny + by) * C, ny * C, C, wx, wy, zL, Kx, Ky, Kz);
         } else {
             // :node (pic_cpu.jl:1507-1530): L on its own mesh, R on its own mesh, phi_L - phi_Z on L's mesh
             R KxL = 0, KyL = 0, KxR = 0, KyR = 0;
@@ -942,9 +1003,22 @@ __device__ __forceinline__ void kick_item(const KickArgs &a, R *win, int d, int 
             Kx = zL * KxL + zR * KxR;
             Ky = zL * KyL + zR * KyR;
         }
-        px += ks * (double)Kx;
-        py += ks * (double)Ky;
-        if (lk) pz += ks * (double)Kz * hzi;
+        if (MODE == 3 && it.gauss) {
+            // Gaussian-subtracted solver (gaussian_pic.jl:864-880): the residual's mesh kick plus the exact
+            // field of the subtracted Gaussian, precomputed per particle at this same drifted position
+            const double *ab = a.ab + 4 * (a.ab_base[it.fld_beam] + it.fld_off + t);
+            px += ks * (double)Kx + ab[0];
+            py += ks * (double)Ky + ab[1];
+            if (lk) {
+                pz += ks * (double)Kz * hzi;
+                pz += ab[2];
+                pz += ab[3];
+            }
+        } else {
+            px += ks * (double)Kx;
+            py += ks * (double)Ky;
+            if (lk) pz += ks * (double)Kz * hzi;
+        }
         const double s2 = 0.5 * (c - z);
         x += s2 * px;
         y += s2 * py;
@@ -1003,10 +1077,17 @@ hipError_t launch_kick(const KickArgs &a, hipStream_t s) {
             hipLaunchKernelGGL((k_kick<M, MODE, double>), g, b, lds, s, a);                                      \
         }                                                                                                        \
     } while (0)
+    if (a.mode == 3 && !a.ab) return hipErrorInvalidValue;
     if (a.method == 0) {
-        if (a.mode == 0) OCTOPIC_KICK(0, 0); else if (a.mode == 1) OCTOPIC_KICK(0, 1); else OCTOPIC_KICK(0, 2);
+        if (a.mode == 0) OCTOPIC_KICK(0, 0);
+        else if (a.mode == 1) OCTOPIC_KICK(0, 1);
+        else if (a.mode == 2) OCTOPIC_KICK(0, 2);
+        else OCTOPIC_KICK(0, 3);
     } else {
-        if (a.mode == 0) OCTOPIC_KICK(1, 0); else if (a.mode == 1) OCTOPIC_KICK(1, 1); else OCTOPIC_KICK(1, 2);
+        if (a.mode == 0) OCTOPIC_KICK(1, 0);
+        else if (a.mode == 1) OCTOPIC_KICK(1, 1);
+        else if (a.mode == 2) OCTOPIC_KICK(1, 2);
+        else OCTOPIC_KICK(1, 3);
     }
 #undef OCTOPIC_KICK
     return hipGetLastError();

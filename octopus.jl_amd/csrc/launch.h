@@ -75,6 +75,9 @@ struct GeomOpts {
     int npl;                     // planes per interaction (2 or 3)
     int temp_slot0;              // first per-batch temporary Green slot (green_cache = none)
     int epoch;                   // this collide's number (CacheGeo::epoch of the entries it builds)
+    int gauss;                   // Gaussian-subtracted solver: fold the source moments, widen the source box
+    int mom_chunks;              // moment chunks per interaction (gauss)
+    double margin;               // margin_sigma (gauss)
     double min_x, min_y, quantize, min_ratio, growth;
     double kbb1, kbb2, klum;
 };
@@ -112,6 +115,8 @@ struct GeometryArgs {
     double *lumgeo;              // [npairs][3] = hxi, hyi, klum
     octopic_interaction_record *records;
     unsigned long long *counters;   // hits, misses, rebuilds, builds
+    const double *mom_part;         // gauss: [2*npairs][mom_chunks][10] source moment partials
+    States st;                      // gauss: the states (first member of each source slice)
     ErrorInfo *err;
 };
 hipError_t launch_geometry(const GeometryArgs &a, hipStream_t s);
@@ -152,6 +157,42 @@ struct WeidemanCoeffs {
 const WeidemanCoeffs &weideman_coeffs();
 hipError_t launch_be_kick(const double *x, const double *y, int64_t n, double sigx, double sigy, double *kx, double *ky,
                           hipStream_t s);
+// the kick and kbb times the Hessian diagonal, (kx, ky, hxx, hyy) per point (strong_beam.jl:1119-1128)
+hipError_t launch_be_response(const double *x, const double *y, int64_t n, double kbb, double sigx, double sigy,
+                              double *out4, hipStream_t s);
+
+// Gaussian-subtracted solver (gaussian_pic.jl): source moments, residual planes, per-particle add-back.
+struct GpicMomentArgs {
+    const BatchPair *pairs;
+    int npairs;
+    const SliceTable *tab1, *tab2;
+    States st;
+    int chunks;                  // chunks per source slice
+    double *mom_part;            // [2*npairs][chunks][10], interaction d = 2k + dir - 1
+    const ErrorInfo *err;
+};
+hipError_t launch_gpic_moments(const GpicMomentArgs &a, hipStream_t s);
+struct GpicResidualArgs {
+    const Inter *inters;
+    const Plane *planes;
+    int nplanes;                 // field planes of the batch (L, R per interaction)
+    unsigned long long *charge;  // fixed-point planes, the residual replaces the deposit
+    int64_t charge_stride;
+    double scale, inv_scale;
+    int method, nx, ny, neutralize;
+    const ErrorInfo *err;
+};
+hipError_t launch_gpic_residual(const GpicResidualArgs &a, hipStream_t s);
+struct GpicAddbackArgs {
+    const Inter *inters;
+    int ninters;
+    States st;
+    double *ab;                  // [N1 + N2][4]: dpx, dpy, pz covariance term, pz centroid term
+    int64_t base[2];             // first row of each beam's particles in ab
+    int lkick;
+    const ErrorInfo *err;
+};
+hipError_t launch_gpic_addback(const GpicAddbackArgs &a, int64_t max_fld, hipStream_t s);
 hipError_t launch_faddeeva(const double *zr, const double *zi, int64_t n, double *wr, double *wi, hipStream_t s);
 
 struct KickArgs {
@@ -170,6 +211,8 @@ struct KickArgs {
     const ErrorInfo *err;
     int *queue;                  // work-queue counter of this launch (zero on entry)
     const int32_t *order;        // launch order of the interactions (position -> interaction)
+    const double *ab;            // Gaussian-subtracted add-back [N1 + N2][4] (mode 0 only), or null
+    int64_t ab_base[2];
 };
 hipError_t launch_kick(const KickArgs &a, hipStream_t s);
 

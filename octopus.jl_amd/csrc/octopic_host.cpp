@@ -213,6 +213,9 @@ struct octopic_handle {
     hipStream_t own_stream = nullptr;   // created with the handle; blocking with respect to the null stream
     std::string err;
     int nx = 0, ny = 0, lnx = 0, lny = 0, lum_dep = 0, npl = 2, mode = 0;
+    bool gauss = false;         // GaussianPICPoissonSolver: Gaussian-subtracted planes with the exact field added back
+    DBuf<double> gmom;          // source moment partials of a batch's interactions [2 * widest][chunks][10]
+    DBuf<double> gab;           // per-particle add-back of both beams [N1 + N2][4]
     FFTPlanHost px, py;
     // persistent device state
     DBuf<SliceTable> tab;       // 2
@@ -423,6 +426,24 @@ void validate(const octopic_options &o) {
                                       "(padded size 2*grid <= 1024)");
     if ((int64_t)(lnx + 1) * (lny + 1) > (int64_t)1 << 20)
         fail(OCTOPIC_ERR_UNSUPPORTED, "luminosity meshes above 2^20 nodes are not supported");
+    if (o.solver != OCTOPIC_SOLVER_PIC && o.solver != OCTOPIC_SOLVER_GAUSSIAN_PIC)
+        fail(OCTOPIC_ERR_INVALID_OPTION, "solver must be OCTOPIC_SOLVER_PIC or OCTOPIC_SOLVER_GAUSSIAN_PIC");
+    if (o.solver == OCTOPIC_SOLVER_GAUSSIAN_PIC) {
+        // GaussianPICPoissonSolver constructor (gaussian_pic.jl:94-110) and the options it rejects at collide
+        // time (_require_linear_slice_interpolation, pic_cpu.jl:788-809)
+        if (!(o.margin_sigma >= 0.0)) fail(OCTOPIC_ERR_INVALID_OPTION, "margin_sigma must be non-negative");
+        if (!(o.coupling_tol >= 0.0)) fail(OCTOPIC_ERR_INVALID_OPTION, "coupling_tol must be non-negative");
+        if (o.slice_interpolation != OCTOPIC_INTERP_LINEAR)
+            fail(OCTOPIC_ERR_UNSUPPORTED, "slice_interpolation = :quadratic is not implemented by GaussianPICPoissonSolver; "
+                                          "set slice_interpolation = :linear.");
+        if (o.interaction_grid != OCTOPIC_GRID_SLICE_PAIR)
+            fail(OCTOPIC_ERR_UNSUPPORTED, "interaction_grid other than :slice_pair is not implemented by "
+                                          "GaussianPICPoissonSolver; set interaction_grid = :slice_pair.");
+        if (std::isfinite(o.coupling_tol))
+            fail(OCTOPIC_ERR_UNSUPPORTED, "a finite coupling_tol (the coupled, rotated subtraction) is not implemented by "
+                                          "the HIP GaussianPICPoissonSolver; use coupling_tol = Inf (the separable "
+                                          "subtraction).");
+    }
 }
 
 void upload_ptrs(octopic_handle *h, double *const in[2][6]) {
@@ -490,6 +511,10 @@ int octopic_options_default(octopic_options *o) {
         s->nslices = 1; s->method = OCTOPIC_SLICE_EQUAL_AREA; s->resolution = 100;
         s->center_position = OCTOPIC_CENTER_CENTROID; s->npositions = 0; s->positions = nullptr;
     }
+    o->solver = OCTOPIC_SOLVER_PIC;
+    o->neutralize = 1;
+    o->margin_sigma = 5.0;
+    o->coupling_tol = INFINITY;
     return OCTOPIC_OK;
 }
 
@@ -508,6 +533,7 @@ int octopic_set_denoiser(octopic_handle *h, octopic_denoiser *d, double monitor_
         return OCTOPIC_OK;
     }
     if (h->mode == 2) return set_err(h, OCTOPIC_ERR_UNSUPPORTED, "the residual denoiser runs with interaction_grid = :slice_pair");
+    if (h->gauss) return set_err(h, OCTOPIC_ERR_UNSUPPORTED, "the residual denoiser runs with PICPoissonSolver, not the Gaussian-subtracted solver");
     int dh = 0, dw = 0, dmb = 0;
     octopic::denoiser_shape(d, &dh, &dw, &dmb);
     if (dh != h->ny || dw != h->nx)
@@ -546,6 +572,7 @@ void octopic_destroy(octopic_handle *h) {
     h->px.tw.release(); h->py.tw.release(); h->px.tw32.release(); h->py.tw32.release();
     h->spec32.release(); h->qh32.release(); h->phi32.release(); h->table32.release();
     h->dn_mom.release(); h->dn_meta.release(); h->dn_prof.release(); h->dn_in.release();
+    h->gmom.release(); h->gab.release();
     delete h;
 }
 
@@ -570,6 +597,7 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         h->lny = opt->lum_ny > 0 ? opt->lum_ny : opt->ny;
         h->lum_dep = opt->lum_deposit >= 0 ? opt->lum_deposit : opt->deposit;
         h->mode = opt->interaction_grid == OCTOPIC_GRID_NODE ? 2 : (opt->slice_interpolation == OCTOPIC_INTERP_QUADRATIC ? 1 : 0);
+        h->gauss = opt->solver == OCTOPIC_SOLVER_GAUSSIAN_PIC;
         h->npl = h->mode == 0 ? 2 : 3;
         if (h->mode == 2 && !opt->longitudinal_kick) h->npl = 3;   // Z plane slot kept; unused without the longitudinal kick
         h->px.build(2 * h->nx);
@@ -1127,6 +1155,10 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         h->phi.ensure((size_t)nfield_planes * field_cells);
         h->table.ensure((size_t)2 * W * field_cells * 3 * npl);
     }
+    if (h->gauss) {
+        h->gmom.ensure((size_t)2 * W * kDnMomentChunks * 10);
+        h->gab.ensure((size_t)4 * (n[0] + n[1]));
+    }
     h->records.ensure((size_t)2 * std::max(npairs, 1));
     h->lum_parts.ensure(std::max(npairs, 1));
     h->lum_partial.ensure((size_t)W * kLumBlocks);
@@ -1148,6 +1180,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     go.interaction_grid = o.interaction_grid; go.compute_lum = compute_lum ? 1 : 0;
     go.lkick = o.longitudinal_kick ? 1 : 0;
     go.ns1 = ns1; go.ns2 = ns2; go.npl = npl; go.temp_slot0 = temp_slot0; go.epoch = h->epoch;
+    go.gauss = h->gauss ? 1 : 0; go.mom_chunks = kDnMomentChunks; go.margin = o.margin_sigma;
     go.min_x = o.min_extent_x; go.min_y = o.min_extent_y; go.quantize = o.grid_quantize;
     go.min_ratio = o.green_min_ratio; go.growth = o.green_growth;
     go.kbb1 = kbb1; go.kbb2 = kbb2; go.klum = klum;
@@ -1232,6 +1265,11 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
                          h->chunkbox.p, h->errinfo.p};
             HIPCHECK(launch_bounds(a, s));
         }
+        if (h->gauss) {   // the source moments the geometry stage builds the subtracted Gaussians from
+            Timed t(h, F_GEOMETRY);
+            GpicMomentArgs a{dp, B, dt1, dt2, st, kDnMomentChunks, h->gmom.p, h->errinfo.p};
+            HIPCHECK(launch_gpic_moments(a, s));
+        }
         if (h->mode == 2) {
             Timed t(h, F_GEOMETRY);
             NodeGeometryArgs a{};
@@ -1247,6 +1285,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.pairs = dp; a.npairs = B; a.tab1 = dt1; a.tab2 = dt2; a.keys = h->keys.p;
             a.o = go; a.cache = h->cache.p; a.inters = h->inters.p; a.planes = h->planes.p; a.jobs = h->jobs.p;
             a.lumgeo = h->lumgeo.p; a.records = h->records.p; a.counters = h->counters.p; a.err = h->errinfo.p;
+            a.mom_part = h->gmom.p; a.st = st;
             HIPCHECK(launch_geometry(a, s));
         }
         if (h->mode != 2) {
@@ -1308,6 +1347,12 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             }
             if (trial >= 0) HIPCHECK(hipEventRecord(h->deposit_ev[2 * bi + 1], s));
         }
+        if (h->gauss) {   // the Gaussian subtracted from each field plane before the convolution
+            Timed t(h, F_DEPOSIT);
+            GpicResidualArgs a{h->inters.p, h->planes.p, nfp, h->charge.p, charge_stride, scale, inv_scale, o.deposit,
+                               nx, ny, o.neutralize ? 1 : 0, h->errinfo.p};
+            HIPCHECK(launch_gpic_residual(a, s));
+        }
         if (h->dn) {
             // field planes through the residual denoiser before the convolution
             Timed t(h, F_DENOISE);
@@ -1356,13 +1401,19 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         };
         if (h->fp32) solve(h->qh32.p, h->spec32.p, h->phi32.p, h->table32.p);
         else solve(h->qh.p, h->spec.p, h->phi.p, h->table.p);
+        if (h->gauss) {   // the exact field of the subtracted Gaussians at every field particle
+            Timed t(h, F_KICK);
+            GpicAddbackArgs a{h->inters.p, 2 * B, st, h->gab.p, {0, n[0]}, o.longitudinal_kick ? 1 : 0, h->errinfo.p};
+            HIPCHECK(launch_gpic_addback(a, bmax, s));
+        }
         {
             Timed t(h, F_KICK);
             KickArgs a{};
             a.inters = h->inters.p; a.ninters = 2 * B; a.st = st; a.nx = nx; a.ny = ny; a.npl = npl;
             a.table = h->fp32 ? (const void *)h->table32.p : (const void *)h->table.p;
             a.fp32 = h->fp32 ? 1 : 0;
-            a.method = o.deposit; a.mode = h->mode; a.lkick = o.longitudinal_kick ? 1 : 0;
+            a.method = o.deposit; a.mode = h->gauss ? 3 : h->mode; a.lkick = o.longitudinal_kick ? 1 : 0;
+            if (h->gauss) { a.ab = h->gab.p; a.ab_base[0] = 0; a.ab_base[1] = n[0]; }
             a.chunks = nchunks;
             a.subdiv = 1;
             if (h->kick_target_blocks > 0) {
@@ -1540,6 +1591,19 @@ int octopic_gaussian_beambeam_kick(octopic_handle *h, const double *x, const dou
     hipSetDevice(h->device);
     if (launch_be_kick(x, y, n, sigx, sigy, kx, ky, h->stream) != hipSuccess)
         return set_err(h, OCTOPIC_ERR_DEVICE, "gaussian_beambeam_kick launch failed");
+    return OCTOPIC_OK;
+}
+
+int octopic_gaussian_beambeam_kick_response(octopic_handle *h, const double *x, const double *y, int64_t n, double kbb,
+                                            double sigx, double sigy, double *out4) {
+    if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    if (n < 0 || (n > 0 && (!x || !y || !out4)))
+        return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "gaussian_beambeam_kick_response: null array or negative count");
+    if (!(std::isfinite(sigx) && std::isfinite(sigy) && sigx > 0.0 && sigy > 0.0 && std::isfinite(kbb)))
+        return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "gaussian_beambeam_kick_response: sizes must be finite and positive");
+    hipSetDevice(h->device);
+    if (launch_be_response(x, y, n, kbb, sigx, sigy, out4, h->stream) != hipSuccess)
+        return set_err(h, OCTOPIC_ERR_DEVICE, "gaussian_beambeam_kick_response launch failed");
     return OCTOPIC_OK;
 }
 

@@ -61,6 +61,8 @@ class Slicing(C.Structure):
 
 
 PRECISION = {"fp64": 0, "fp32": 1}
+SOLVER_PIC = 0
+SOLVER_GAUSSIAN_PIC = 1
 
 
 class Options(C.Structure):
@@ -92,6 +94,10 @@ class Options(C.Structure):
         ("luminosity_scale", C.c_double),
         ("slicing1", Slicing),
         ("slicing2", Slicing),
+        ("solver", C.c_int32),
+        ("neutralize", C.c_int32),
+        ("margin_sigma", C.c_double),
+        ("coupling_tol", C.c_double),
     ]
 
 
@@ -173,6 +179,7 @@ def declare(lib):
         "octopic_track_crab_cavity": (i32, [vp, C.c_double, i32, DPTR, DPTR, DPTR, C.POINTER(BeamStruct)]),
         "octopic_gaussian_beambeam_kick": (i32, [vp, vp, vp, i64, C.c_double, C.c_double, vp, vp]),
         "octopic_faddeeva_w": (i32, [vp, vp, vp, i64, vp, vp]),
+        "octopic_gaussian_beambeam_kick_response": (i32, [vp, vp, vp, i64, C.c_double, C.c_double, C.c_double, vp]),
         "octopic_generate_gaussian_beam": (
             i32,
             [vp, C.c_uint64, C.c_uint64, DPTR, DPTR, DPTR, C.c_double, C.POINTER(BeamStruct)],
@@ -225,6 +232,7 @@ EXPORTED = [
     "octopic_track_crab_cavity",
     "octopic_gaussian_beambeam_kick",
     "octopic_faddeeva_w",
+    "octopic_gaussian_beambeam_kick_response",
     "octopic_generate_gaussian_beam",
     "octopic_debug_weights",
     "octopic_collide_many",

@@ -290,6 +290,20 @@ class PICCollider:
                                                          float(sigy), kx.data_ptr(), ky.data_ptr()))
         return kx, ky
 
+    def gaussian_beambeam_kick_response(self, x, y, kbb: float, sigx: float, sigy: float):
+        """_gaussian_beambeam_kick_response(kbb, sigx, sigy, x, y) (strong_beam.jl:1119-1128) elementwise over
+        device float64 tensors x, y: a [n, 4] tensor of (Kx, Ky, Hxx, Hyy), the normalised kick and kbb times
+        the potential's Hessian diagonal."""
+        import torch
+        if x.shape != y.shape or x.dtype != torch.float64 or y.dtype != torch.float64 or not (x.is_cuda and y.is_cuda):
+            raise ArgumentError("x and y must be float64 device tensors of one shape")
+        x, y = x.contiguous().reshape(-1), y.contiguous().reshape(-1)
+        out = torch.empty((x.numel(), 4), dtype=torch.float64, device=x.device)
+        self._check(lib().octopic_gaussian_beambeam_kick_response(self.h, x.data_ptr(), y.data_ptr(), x.numel(),
+                                                                  float(kbb), float(sigx), float(sigy),
+                                                                  out.data_ptr()))
+        return out
+
     def faddeeva_w(self, z):
         """w(z) = exp(-z^2) erfc(-iz) (SpecialMath.jl:60-151) elementwise over a complex128 device tensor."""
         import torch

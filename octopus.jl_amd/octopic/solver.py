@@ -248,6 +248,48 @@ class PICPoissonSolver:
         o.luminosity_scale = nan if self.luminosity_scale is None else float(self.luminosity_scale)
         o.slicing1 = self.resolved_slicing1._lower(keep)
         o.slicing2 = self.resolved_slicing2._lower(keep)
+        o.solver = _abi.SOLVER_PIC
+        o.neutralize = 1
+        o.margin_sigma = 5.0
+        o.coupling_tol = math.inf
+        return o, keep
+
+
+class GaussianPICPoissonSolver:
+    """GaussianPICPoissonSolver{Float64}(; margin_sigma=5.0, neutralize=true, coupling_tol=Inf, kwargs...)
+    (src/tasks/strongstrong/gaussian_pic.jl:87-131): the Gaussian-subtracted PIC hybrid.  Every other keyword
+    builds the embedded PICPoissonSolver (`self.pic`), with its own validation.  The hybrid deposits each
+    source slice, subtracts the erf-integrated drifted Gaussian of the slice's own moments, solves the
+    residual with the PIC machinery and adds the exact Bassetti-Erskine field of that Gaussian back per field
+    particle.  It runs with interaction_grid = :slice_pair, slice_interpolation = :linear and grid_extent =
+    :extrema only (the reference rejects the others at collide time, gaussian_pic.jl:33-41)."""
+
+    def __init__(self, margin_sigma: float = 5.0, neutralize: bool = True, coupling_tol: float = math.inf, **kwargs):
+        self.pic = PICPoissonSolver(**kwargs)
+        ms, ct = float(margin_sigma), float(coupling_tol)
+        if not ms >= 0.0:
+            raise ArgumentError(f"margin_sigma must be non-negative; got {margin_sigma}.")
+        if not ct >= 0.0:
+            raise ArgumentError(f"coupling_tol must be non-negative; got {coupling_tol}.")
+        self.margin_sigma = ms
+        self.neutralize = bool(neutralize)
+        self.coupling_tol = ct
+
+    def __getattr__(self, name):
+        # the embedded PIC solver's options and resolved views (slicing, luminosity grid, schedule, ...)
+        if name == "pic":
+            raise AttributeError(name)
+        return getattr(self.pic, name)
+
+    def compute_luminosity(self, turn):
+        return self.pic.compute_luminosity(turn)
+
+    def to_options(self):
+        o, keep = self.pic.to_options()
+        o.solver = _abi.SOLVER_GAUSSIAN_PIC
+        o.neutralize = 1 if self.neutralize else 0
+        o.margin_sigma = self.margin_sigma
+        o.coupling_tol = self.coupling_tol
         return o, keep
 
 

@@ -73,6 +73,8 @@ def lib():
         L.oracle_npairs.argtypes = [C.c_void_p]
         L.oracle_pair_luminosity.argtypes = [C.c_void_p, I32, I32, I32, D]
         L.oracle_records.argtypes = [C.c_void_p, C.POINTER(_abi.InteractionRecord), C.c_int64, I64]
+        L.oracle_gaussian_beambeam_kick_response.argtypes = [C.c_double] * 5 + [D]
+        L.oracle_gaussian_beambeam_kick_response.restype = None
         L.oracle_node_mesh.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, D]
         L.oracle_node_mesh.restype = C.c_int32
         L.oracle_last_slices.argtypes = [C.c_void_p, C.c_int32, D, D, D, I64, I64]
@@ -163,6 +165,14 @@ def gaussian_beambeam_kick(sigx, sigy, x, y):
     kx, ky = C.c_double(), C.c_double()
     lib().oracle_gaussian_beambeam_kick(float(sigx), float(sigy), float(x), float(y), C.byref(kx), C.byref(ky))
     return kx.value, ky.value
+
+
+def gaussian_beambeam_kick_response(kbb, sigx, sigy, x, y):
+    """_gaussian_beambeam_kick_response(kbb, sigx, sigy, x, y) (strong_beam.jl:1119-1128): (Kx, Ky), the
+    normalised kick, and (Hxx, Hyy), the diagonal of the potential's Hessian times kbb."""
+    out = np.zeros(4)
+    lib().oracle_gaussian_beambeam_kick_response(float(kbb), float(sigx), float(sigy), float(x), float(y), _d(out))
+    return tuple(out)
 
 
 def weideman_coefficients():

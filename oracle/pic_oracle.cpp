@@ -315,6 +315,9 @@ struct Opt {
     double extent_sigma = 6.0, min_x = 0.0, min_y = 0.0, quantize = 0.0, min_ratio = 0.5, growth = 0.25;
     double kbb1 = NAN, kbb2 = NAN, lum_scale = NAN;
     Slicing s1, s2;
+    // GaussianPICPoissonSolver (gaussian_pic.jl:87-131)
+    int solver = OCTOPIC_SOLVER_PIC, neutralize = 1;
+    double margin = 5.0, coupling_tol = INFINITY;
 };
 
 Slicing to_slicing(const octopic_slicing &s) {
@@ -346,6 +349,7 @@ Opt to_opt(const octopic_options &o) {
     r.quantize = o.grid_quantize; r.min_ratio = o.green_min_ratio; r.growth = o.green_growth;
     r.kbb1 = o.kbb1; r.kbb2 = o.kbb2; r.lum_scale = o.luminosity_scale;
     r.s1 = to_slicing(o.slicing1); r.s2 = to_slicing(o.slicing2);
+    r.solver = o.solver; r.neutralize = o.neutralize ? 1 : 0; r.margin = o.margin_sigma; r.coupling_tol = o.coupling_tol;
     // Constructor checks (interface.jl:1410-1476) and _validate_pic_solver (pic_cpu.jl:517-576).
     if (!(r.min_ratio > 0.0 && r.min_ratio <= 1.0))
         throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "slice_pair_green_min_ratio must be in (0, 1]");
@@ -378,6 +382,22 @@ Opt to_opt(const octopic_options &o) {
         throw OracleError(OCTOPIC_ERR_UNSUPPORTED, "green_type = :lattice is not restated by the oracle");
     validate_slicing(r.s1);
     validate_slicing(r.s2);
+    if (r.solver != OCTOPIC_SOLVER_PIC && r.solver != OCTOPIC_SOLVER_GAUSSIAN_PIC)
+        throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "solver must be OCTOPIC_SOLVER_PIC or OCTOPIC_SOLVER_GAUSSIAN_PIC");
+    if (r.solver == OCTOPIC_SOLVER_GAUSSIAN_PIC) {
+        // GaussianPICPoissonSolver constructor (gaussian_pic.jl:94-110) and _require_linear_slice_interpolation
+        // (pic_cpu.jl:788-809)
+        if (!(r.margin >= 0.0)) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "margin_sigma must be non-negative");
+        if (!(r.coupling_tol >= 0.0)) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "coupling_tol must be non-negative");
+        if (r.interp != OCTOPIC_INTERP_LINEAR)
+            throw OracleError(OCTOPIC_ERR_UNSUPPORTED, "slice_interpolation = :quadratic is not implemented by GaussianPICPoissonSolver");
+        if (r.igrid != OCTOPIC_GRID_SLICE_PAIR)
+            throw OracleError(OCTOPIC_ERR_UNSUPPORTED, "interaction_grid other than :slice_pair is not implemented by GaussianPICPoissonSolver");
+        if (r.extent != OCTOPIC_EXTENT_EXTREMA)
+            throw OracleError(OCTOPIC_ERR_UNSUPPORTED, "grid_extent = :sigma is not implemented by GaussianPICPoissonSolver");
+        if (std::isfinite(r.coupling_tol))
+            throw OracleError(OCTOPIC_ERR_UNSUPPORTED, "the coupled (rotated) subtraction (finite coupling_tol) is not restated");
+    }
     return r;
 }
 
@@ -1486,6 +1506,9 @@ bool union_bounds(const Coords &src, double center, const std::vector<Coords> &s
     return true;
 }
 
+void gpic_interaction(Context &C, Workspace &w, const Coords &src, const SliceParam &ps, Coords &fld, const SliceParam &pf,
+                      double kbb, const std::tuple<int, int, int> *key, Vec &vx, Vec &vy, octopic_interaction_record *rec);
+
 struct CollideOut {
     double lum = NAN;
     std::vector<int32_t> pi, pj, ran;
@@ -1596,8 +1619,13 @@ void collide(State &S, double *b1[6], int64_t n1, const double p1[5], double *b2
             }
             auto key1 = share ? std::make_tuple(i, -1, 1) : std::make_tuple(i, j, 1);
             auto key2 = share ? std::make_tuple(j, -1, 2) : std::make_tuple(i, j, 2);
-            interaction(C, w, coord1, q1, field2, q2, kbb2, &key1, ov1, vx1, vy1, r1);
-            interaction(C, w, st2[j], q2, coord1, q1, kbb1, &key2, ov2, vx2, vy2, r2);
+            if (o.solver == OCTOPIC_SOLVER_GAUSSIAN_PIC) {   // _gpic_collide_pair! (gaussian_pic.jl:1024-1060)
+                gpic_interaction(C, w, coord1, q1, field2, q2, kbb2, &key1, vx1, vy1, r1);
+                gpic_interaction(C, w, st2[j], q2, coord1, q1, kbb1, &key2, vx2, vy2, r2);
+            } else {
+                interaction(C, w, coord1, q1, field2, q2, kbb2, &key1, ov1, vx1, vy1, r1);
+                interaction(C, w, st2[j], q2, coord1, q1, kbb1, &key2, ov2, vx2, vy2, r2);
+            }
         }
         st2[j] = std::move(field2);
         if (compute_lum) lum_parts[p] = luminosity(o, vx1, vy1, vx2, vy2, klum, w.q1, w.q2);
@@ -1800,6 +1828,342 @@ void be_principal(double s1, double s2, double x, double y, double &kx, double &
     ky = sy + w * (ey - sy);
 }
 
+// gaussian_beambeam_kick (strong_beam.jl:1139-1146): zero sizes give zero kicks, the larger size is principal
+void be_kick(double sigx, double sigy, double x, double y, double &kx, double &ky) {
+    kx = ky = 0.0;
+    if (sigx == 0.0 || sigy == 0.0) return;
+    if (sigx >= sigy) be_principal(sigx, sigy, x, y, kx, ky);
+    else be_principal(sigy, sigx, y, x, ky, kx);
+}
+
+// ---- the kick with its Hessian diagonal: _gaussian_beambeam_kick_response (strong_beam.jl:1061-1128) and the
+// helpers it combines (:695-703 elliptic Hessian, :705-772 round force scale and Hessian, :779-797 bounds and
+// blend, :799-876 radial moments, :897-960 near-round series response and potential residual, :962-1014 the
+// near-axis series).  K is the normalised kick; H1, H2 carry the factor kbb.
+struct Resp { double kx, ky, h1, h2; };
+
+void radial_moments(double q, int first, int count, double *m) {   // m[k] = m_{first+k}
+    if (q <= 2.0) {
+        std::fill(m, m + count, 0.0);
+        double term = 1.0;
+        for (int k = 0; k <= 24; ++k) {
+            for (int j = 0; j < count; ++j) m[j] += term / (double)(k + 1 + first + j);
+            term *= -q / (double)(k + 1);
+        }
+        return;
+    }
+    const double eq = std::exp(-q);
+    double prev = -std::expm1(-q) / q;   // m0
+    if (first == 0) m[0] = prev;
+    for (int j = 1; j < first + count; ++j) {
+        prev = ((double)j * prev - eq) / q;
+        if (j >= first) m[j - first] = prev;
+    }
+}
+
+void round_hessian(double kbb, double sigma, double x, double y, double expterm, double &hxx, double &hyy) {
+    const double r2 = x * x + y * y, sigma2 = sigma * sigma;
+    if (r2 == 0.0) { hxx = hyy = -kbb / sigma2; return; }
+    const double u = r2 / (2 * sigma2);
+    double f, fp;
+    if (u < 1.0e-2) {
+        const double phi = 1.0 + u * (-1.0 / 2 + u * (1.0 / 6 + u * (-1.0 / 24 + u * (1.0 / 120 + u * (-1.0 / 720 + u * (1.0 / 5040))))));
+        const double dphi = -1.0 / 2 + u * (1.0 / 3 + u * (-1.0 / 8 + u * (1.0 / 30 + u * (-1.0 / 144 + u * (1.0 / 840 - u * (1.0 / 5760))))));
+        const double inv = 1.0 / sigma2;
+        f = phi * inv;
+        fp = dphi * inv * inv / 2;
+    } else {
+        const double ome = 1 - expterm;
+        f = 2 * ome / r2;
+        fp = 2 * (expterm * u - ome) / (r2 * r2);
+    }
+    hxx = -kbb * (f + 2 * x * x * fp);
+    hyy = -kbb * (f + 2 * y * y * fp);
+}
+
+Resp series_response(double kbb, double v, double eta, double x, double y) {
+    const double q = (x * x + y * y) / (2 * v), dr = (x * x - y * y) / (2 * v);
+    double m[7];
+    radial_moments(q, 0, 7, m);
+    const double eta2 = eta * eta, eta3 = eta2 * eta;
+    const double u1 = dr * m[1];
+    const double u2 = m[1] / 2 - q * m[2] + (dr * dr / 2) * m[3];
+    const double u3 = 1.5 * dr * m[3] - dr * q * m[4] + dr * dr * dr / 6 * m[5];
+    const double v1 = dr * (q * m[2] - m[1]);
+    const double v2 = 1.5 * q * m[2] - (q * q + dr * dr) * m[3] + (q * dr * dr / 2) * m[4];
+    const double v3 = -1.5 * dr * m[3] + 3.5 * dr * q * m[4] - (dr * q * q + dr * dr * dr / 2) * m[5] + dr * dr * dr * q / 6 * m[6];
+    const double c1x = dr * m[2] - m[1], c1y = dr * m[2] + m[1];
+    const double c2x = 1.5 * m[2] + (-dr - q) * m[3] + (dr * dr / 2) * m[4];
+    const double c2y = 1.5 * m[2] + (dr - q) * m[3] + (dr * dr / 2) * m[4];
+    const double c3x = -1.5 * m[3] + (2.5 * dr + q) * m[4] + (-dr * q - dr * dr / 2) * m[5] + dr * dr * dr / 6 * m[6];
+    const double c3y = 1.5 * m[3] + (2.5 * dr - q) * m[4] + (-dr * q + dr * dr / 2) * m[5] + dr * dr * dr / 6 * m[6];
+    Resp r;
+    r.kx = x / v * (m[0] + eta * c1x + eta2 * c2x + eta3 * c3x);
+    r.ky = y / v * (m[0] + eta * c1y + eta2 * c2y + eta3 * c3y);
+    const double uv = -std::exp(-q) + eta * v1 + eta2 * v2 + eta3 * v3;
+    const double ueta = u1 + 2 * eta * u2 + 3 * eta2 * u3;
+    r.h1 = kbb / v * (uv + (1 - eta) * ueta);
+    r.h2 = kbb / v * (uv - (1 + eta) * ueta);
+    return r;
+}
+
+double potential_residual(double v, double eta, double x, double y) {   // U_exact - U_series through eta^6
+    const double q = (x * x + y * y) / (2 * v), dr = (x * x - y * y) / (2 * v);
+    double mm[9];
+    radial_moments(q, 3, 9, mm);
+    const double m3 = mm[0], m4 = mm[1], m5 = mm[2], m6 = mm[3], m7 = mm[4], m8 = mm[5], m9 = mm[6], m10 = mm[7], m11 = mm[8];
+    const double dr2 = dr * dr, q2 = q * q;
+    const double u4 = 3.0 / 8 * m3 - 1.5 * q * m4 + (5 * dr2 + 2 * q2) / 4 * m5 - dr2 * q / 2 * m6 + dr2 * dr2 / 24 * m7;
+    const double u5 = 15.0 / 8 * dr * m5 - 2.5 * dr * q * m6 + dr * (7 * dr2 + 6 * q2) / 12 * m7 - dr * dr2 * q / 6 * m8 +
+                      dr * dr2 * dr2 / 120 * m9;
+    const double u6 = 5.0 / 16 * m5 - 15.0 / 8 * q * m6 + 5 * (7 * dr2 + 4 * q2) / 16 * m7 - q * (21 * dr2 + 2 * q2) / 12 * m8 +
+                      dr2 * (3 * dr2 + 4 * q2) / 16 * m9 - dr2 * dr2 * q / 24 * m10 + dr2 * dr2 * dr2 / 720 * m11;
+    return eta * eta * eta * eta * (u4 + eta * (u5 + eta * u6));
+}
+
+void axis_component(double kbb, double sigx, double sigy, double x, double y, double &k, double &h) {
+    const double s = sigx + sigy, sx2 = sigx * sigx, sx3 = sx2 * sigx, sx5 = sx3 * sx2, sy2 = sigy * sigy, sy3 = sy2 * sigy;
+    const double s2 = s * s, s3 = s2 * s;
+    const double j00 = 2 / (sigx * s), j10 = 2 * (2 * sigx + sigy) / (3 * sx3 * s2), j01 = 2 / (sigx * sigy * s2);
+    const double j20 = 2 * (8 * sx2 + 9 * sigx * sigy + 3 * sy2) / (15 * sx5 * s3);
+    const double j11 = 2 * (3 * sigx + sigy) / (3 * sx3 * sigy * s3), j02 = 2 * (sigx + 3 * sigy) / (3 * sigx * sy3 * s3);
+    const double x2 = x * x, y2 = y * y, x4 = x2 * x2, y4 = y2 * y2, x2y2 = x2 * y2;
+    const double scale = j00 - (x2 * j10 + y2 * j01) / 2 + x4 * j20 / 8 + x2y2 * j11 / 4 + y4 * j02 / 8;
+    const double derivative = j00 - 1.5 * x2 * j10 - y2 * j01 / 2 + 0.625 * x4 * j20 + 0.75 * x2y2 * j11 + y4 * j02 / 8;
+    k = x * scale;
+    h = -kbb * derivative;
+}
+
+bool near_axis(double s1, double s2, double x, double y, double eta) {
+    const double rho2 = x * x / (s1 * s1) + y * y / (s2 * s2);
+    return rho2 * rho2 * rho2 * std::sqrt(rho2) <= std::ldexp(1.0, -52) / std::sqrt(eta);
+}
+
+Resp response_principal(double kbb, double s1, double s2, double x, double y) {
+    const double v = (s1 * s1 + s2 * s2) / 2, eta = (s1 * s1 - s2 * s2) / (2 * v);
+    Resp r;
+    if (eta == 0.0) {
+        const double r2 = x * x + y * y, u = r2 / (2 * v);
+        const double scale = (u == 0.0 ? 1.0 : -std::expm1(-u) / u) / v;
+        const double expterm = std::exp(-r2 / (2 * v));
+        r.kx = scale * x;
+        r.ky = scale * y;
+        round_hessian(kbb, std::sqrt(v), x, y, expterm, r.h1, r.h2);
+        return r;
+    }
+    const double outer = std::sqrt(std::sqrt(8.0 / 3.0 * 64.0 * std::ldexp(1.0, -52))), inner = outer / 2;
+    if (eta <= inner) return series_response(kbb, v, eta, x, y);
+    double kex, key;
+    elliptic(s1, s2, x, y, eta, kex, key);
+    const double expterm = std::exp(-0.5 * (x * x / (s1 * s1) + y * y / (s2 * s2)));
+    double he1, he2;
+    if (near_axis(s1, s2, x, y, eta)) {
+        double k;
+        axis_component(kbb, s1, s2, x, y, k, he1);
+        axis_component(kbb, s2, s1, y, x, k, he2);
+    } else {   // _elliptic_gaussian_hessian_diagonal (strong_beam.jl:695-703)
+        const double t1 = kbb * (x * kex + y * key), t2 = s1 * s1 - s2 * s2, t3 = s2 / s1;
+        he1 = (t1 - 2 * kbb * (1 - expterm * t3)) / t2;
+        he2 = (-t1 + 2 * kbb * (1 - expterm / t3)) / t2;
+    }
+    if (eta >= outer) return Resp{kex, key, he1, he2};
+    const double invw = 1.0 / (outer - inner), t = (eta - inner) * invw;
+    const double w = t * t * t * (10 + t * (-15 + 6 * t)), dw = 30 * t * t * (t - 1) * (t - 1) * invw;
+    const Resp sr = series_response(kbb, v, eta, x, y);
+    const double chain = kbb * dw * potential_residual(v, eta, x, y) / v;
+    r.kx = sr.kx + w * (kex - sr.kx);
+    r.ky = sr.ky + w * (key - sr.ky);
+    r.h1 = sr.h1 + w * (he1 - sr.h1) + (1 - eta) * chain;
+    r.h2 = sr.h2 + w * (he2 - sr.h2) - (1 + eta) * chain;
+    return r;
+}
+
+Resp beambeam_response(double kbb, double sigx, double sigy, double x, double y) {
+    if (sigx >= sigy) return response_principal(kbb, sigx, sigy, x, y);
+    const Resp r = response_principal(kbb, sigy, sigx, y, x);
+    return Resp{r.ky, r.kx, r.h2, r.h1};
+}
+
+// ---- GaussianPICPoissonSolver, the separable (uncoupled) subtraction (gaussian_pic.jl:374-491, 543-595,
+// 646-892)
+struct GMom { int64_t n; double mx, mpx, varx, cxpx, varpx, my, mpy, vary, cypy, varpy; };
+struct GBnd { double mux, muy, sigx, sigy, rx, ry; };
+
+GMom gpic_moments(const Coords &src) {   // _gpic_source_moments, uncoupled
+    const int64_t n = (int64_t)src.size();
+    const double x0 = src.x[0], px0 = src.px[0], y0 = src.y[0], py0 = src.py[0];
+    double sx = 0, spx = 0, sy = 0, spy = 0, sxx = 0, spxpx = 0, syy = 0, spypy = 0, sxpx = 0, sypy = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double dx = src.x[i] - x0, dpx = src.px[i] - px0, dy = src.y[i] - y0, dpy = src.py[i] - py0;
+        sx += dx; spx += dpx; sy += dy; spy += dpy;
+        sxx += dx * dx; spxpx += dpx * dpx; syy += dy * dy; spypy += dpy * dpy;
+        sxpx += dx * dpx; sypy += dy * dpy;
+    }
+    const double invn = 1.0 / (double)n;
+    const double dmx = sx * invn, dmpx = spx * invn, dmy = sy * invn, dmpy = spy * invn;
+    // _shifted_second_moment / _shifted_cross_moment (slicing.jl:664-668): muladd(-a, b, sum * invn)
+    auto shifted = [&](double sum, double a, double b) { return std::fma(-a, b, sum * invn); };
+    GMom m;
+    m.n = n;
+    m.mx = x0 + dmx; m.mpx = px0 + dmpx; m.my = y0 + dmy; m.mpy = py0 + dmpy;
+    m.varx = std::max(shifted(sxx, dmx, dmx), 0.0);
+    m.vary = std::max(shifted(syy, dmy, dmy), 0.0);
+    m.cxpx = shifted(sxpx, dmx, dmpx);
+    m.varpx = std::max(shifted(spxpx, dmpx, dmpx), 0.0);
+    m.cypy = shifted(sypy, dmy, dmpy);
+    m.varpy = std::max(shifted(spypy, dmpy, dmpy), 0.0);
+    return m;
+}
+
+GBnd gpic_boundary(const GMom &m, double s) {   // _gpic_drifted_gaussian + _gpic_boundary, uncoupled
+    GBnd b;
+    b.mux = m.mx + m.mpx * s;
+    b.muy = m.my + m.mpy * s;
+    const double vx = m.varx + 2 * s * m.cxpx + s * s * m.varpx;
+    const double vy = m.vary + 2 * s * m.cypy + s * s * m.varpy;
+    b.sigx = std::sqrt(std::max(vx, 0.0));
+    b.sigy = std::sqrt(std::max(vy, 0.0));
+    b.rx = 2 * (m.cxpx + s * m.varpx);
+    b.ry = 2 * (m.cypy + s * m.varpy);
+    return b;
+}
+
+// _gpic_solve_drifted_field! (gaussian_pic.jl:543-595): deposit, subtract the neutralised erf profile of the
+// drifted Gaussian, solve the residual.
+void gpic_solve(Context &C, Workspace &w, FieldWS &f, const Coords &src, double s, const Grid &sg,
+                const std::vector<cd> &gfft, const GBnd &b) {
+    const Opt &o = C.o;
+    const int nx = o.nx, ny = o.ny;
+    const int64_t P1 = 2 * nx, P2 = 2 * ny;
+    const double hx = sg.width / (double)(nx - 1), hy = sg.height / (double)(ny - 1);
+    std::fill(w.charge.begin(), w.charge.end(), 0.0);
+    deposit(w.charge.data(), P1, P2, o.deposit, src.x.data(), src.px.data(), src.y.data(), src.py.data(), s, true,
+            sg.x0, sg.y0, hx, hy, nx, ny, (int64_t)src.size(), &w.local_charge);
+    std::vector<double> gx(nx), gy(ny);
+    gaussian_profile(gx.data(), nx, sg.x0, hx, b.mux, b.sigx, o.deposit);
+    gaussian_profile(gy.data(), ny, sg.y0, hy, b.muy, b.sigy, o.deposit);
+    double sgx = 0, sgy = 0, qsum = 0;
+    for (double v : gx) sgx += v;
+    for (double v : gy) sgy += v;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) qsum += w.charge[i + (size_t)j * P1];
+    double amp = (double)src.size();
+    if (o.neutralize && sgx * sgy > 0.0) amp = qsum / (sgx * sgy);
+    for (int j = 0; j < ny; ++j) {
+        const double gj = amp * gy[j];
+        for (int i = 0; i < nx; ++i) w.charge[i + (size_t)j * P1] -= gx[i] * gj;
+    }
+    for (size_t k = 0; k < w.charge.size(); ++k) w.spectral[k] = cd(w.charge[k], 0.0);
+    fft2(w.spectral.data(), (int)P1, (int)P2, -1, *C.plans.p1, *C.plans.p2);
+    for (size_t k = 0; k < w.spectral.size(); ++k) w.spectral[k] *= gfft[k];
+    fft2(w.spectral.data(), (int)P1, (int)P2, +1, *C.plans.p1, *C.plans.p2);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) f.phi[i + (size_t)j * nx] = w.spectral[i + (size_t)j * P1].real();
+    field_derivative(f.Ex.data(), f.Ey.data(), f.phi.data(), nx, ny, hx, hy, o.field_derivative == 4);
+}
+
+// _gpic_interaction! (gaussian_pic.jl:646-816) with _gpic_apply_kick_range! (:818-892), uncoupled.
+void gpic_interaction(Context &C, Workspace &w, const Coords &src, const SliceParam &ps, Coords &fld, const SliceParam &pf,
+                      double kbb, const std::tuple<int, int, int> *key, Vec &vx, Vec &vy, octopic_interaction_record *rec) {
+    const Opt &o = C.o;
+    const int64_t nsource = (int64_t)src.size(), nfield = (int64_t)fld.size();
+    const double sL = 0.5 * (ps.center - pf.lb), sR = 0.5 * (ps.center - pf.rb);
+    const GMom mom = gpic_moments(src);
+    const GBnd bL = gpic_boundary(mom, sL), bR = gpic_boundary(mom, sR);
+    const bool resolved = nsource >= 2 && std::isfinite(bL.sigx) && std::isfinite(bL.sigy) && std::isfinite(bR.sigx) &&
+                          std::isfinite(bR.sigy) && bL.sigx > 0 && bL.sigy > 0 && bR.sigx > 0 && bR.sigy > 0;
+    if (!resolved) {   // _gpic_control_variate_mode -> :pic: the whole interaction is plain PIC
+        interaction(C, w, src, ps, fld, pf, kbb, key, nullptr, vx, vy, rec);
+        return;
+    }
+    const double margin = o.margin;
+    double sxmin = jmin(src.x[0] + src.px[0] * sL, src.x[0] + src.px[0] * sR);
+    double sxmax = jmax(src.x[0] + src.px[0] * sL, src.x[0] + src.px[0] * sR);
+    double symin = jmin(src.y[0] + src.py[0] * sL, src.y[0] + src.py[0] * sR);
+    double symax = jmax(src.y[0] + src.py[0] * sL, src.y[0] + src.py[0] * sR);
+    for (int64_t i = 1; i < nsource; ++i) {
+        const double xl = src.x[i] + src.px[i] * sL, yl = src.y[i] + src.py[i] * sL;
+        const double xr = src.x[i] + src.px[i] * sR, yr = src.y[i] + src.py[i] * sR;
+        sxmin = jmin(jmin(sxmin, xl), xr); sxmax = jmax(jmax(sxmax, xl), xr);
+        symin = jmin(jmin(symin, yl), yr); symax = jmax(jmax(symax, yl), yr);
+    }
+    if (margin > 0) {
+        const double gxlo = jmin(bL.mux - margin * bL.sigx, bR.mux - margin * bR.sigx);
+        const double gxhi = jmax(bL.mux + margin * bL.sigx, bR.mux + margin * bR.sigx);
+        const double gylo = jmin(bL.muy - margin * bL.sigy, bR.muy - margin * bR.sigy);
+        const double gyhi = jmax(bL.muy + margin * bL.sigy, bR.muy + margin * bR.sigy);
+        sxmin = jmin(sxmin, gxlo); sxmax = jmax(sxmax, gxhi);
+        symin = jmin(symin, gylo); symax = jmax(symax, gyhi);
+    }
+    if (!(std::isfinite(sxmin) && std::isfinite(sxmax) && std::isfinite(symin) && std::isfinite(symax)))
+        nonfinite("source", "slice pair");
+    const double c = ps.center;
+    double fxmin = fld.x[0] + 0.5 * (fld.z[0] - c) * fld.px[0], fxmax = fxmin;
+    double fymin = fld.y[0] + 0.5 * (fld.z[0] - c) * fld.py[0], fymax = fymin;
+    for (int64_t i = 0; i < nfield; ++i) {
+        const double s = 0.5 * (fld.z[i] - c);
+        fld.x[i] += s * fld.px[i];
+        fld.y[i] += s * fld.py[i];
+        if (o.lkick) fld.pz[i] -= 0.25 * (fld.px[i] * fld.px[i] + fld.py[i] * fld.py[i]);
+        fxmin = jmin(fxmin, fld.x[i]); fxmax = jmax(fxmax, fld.x[i]);
+        fymin = jmin(fymin, fld.y[i]); fymax = jmax(fymax, fld.y[i]);
+    }
+    if (!(std::isfinite(fxmin) && std::isfinite(fxmax) && std::isfinite(fymin) && std::isfinite(fymax)))
+        nonfinite("field", "slice pair");
+    Grid sg, fg;
+    interaction_grids(o, sxmin, sxmax, symin, symax, fxmin, fxmax, fymin, fymax, sg, fg);
+    Box sb{sxmin, sxmax, symin, symax}, fb{fxmin, fxmax, fymin, fymax};
+    std::vector<cd> tmp;
+    int event = 0;
+    const std::vector<cd> *gfft = slice_pair_green(C, w, key, sg, fg, sb, fb, tmp, event);
+    if (rec) {
+        rec->cache_event = event;
+        rec->source_grid[0] = sg.x0; rec->source_grid[1] = sg.y0; rec->source_grid[2] = sg.width; rec->source_grid[3] = sg.height;
+        rec->field_grid[0] = fg.x0; rec->field_grid[1] = fg.y0; rec->field_grid[2] = fg.width; rec->field_grid[3] = fg.height;
+    }
+    gpic_solve(C, w, w.left, src, sL, sg, *gfft, bL);
+    gpic_solve(C, w, w.right, src, sR, sg, *gfft, bR);
+    const double kick_scale = 2.0 * kbb, half_ns = 0.5 * (double)nsource, kbb_eff = kick_scale * half_ns;
+    double hzi, zbias;
+    slice_interp_params(pf.lb, pf.rb, hzi, zbias);
+    for (int64_t i = 0; i < nfield; ++i) {
+        const double x = fld.x[i], y = fld.y[i];
+        const double zL = jclamp(-fld.z[i] * hzi + zbias, 0.0, 1.0), zR = 1.0 - zL;
+        double Kx, Ky, Kz;
+        interpolate_kick(o, fg, x, y, w.left, w.right, zL, zR, Kx, Ky, Kz);
+        Resp rl, rr;
+        if (o.lkick) {
+            rl = beambeam_response(kbb_eff, bL.sigx, bL.sigy, x - bL.mux, y - bL.muy);
+            rr = beambeam_response(kbb_eff, bR.sigx, bR.sigy, x - bR.mux, y - bR.muy);
+        } else {
+            rl = Resp{0, 0, 0, 0};
+            rr = Resp{0, 0, 0, 0};
+            be_kick(bL.sigx, bL.sigy, x - bL.mux, y - bL.muy, rl.kx, rl.ky);
+            be_kick(bR.sigx, bR.sigy, x - bR.mux, y - bR.muy, rr.kx, rr.ky);
+        }
+        const double Kxa = half_ns * (zL * rl.kx + zR * rr.kx), Kya = half_ns * (zL * rl.ky + zR * rr.ky);
+        const double dpxa = kick_scale * Kxa, dpya = kick_scale * Kya;
+        fld.px[i] += kick_scale * Kx + dpxa;
+        fld.py[i] += kick_scale * Ky + dpya;
+        if (o.lkick) {
+            const double covL = (rl.h1 * bL.rx + rl.h2 * bL.ry) / 4, covR = (rr.h1 * bR.rx + rr.h2 * bR.ry) / 4;
+            fld.pz[i] += kick_scale * Kz * hzi;
+            fld.pz[i] += zL * covL + zR * covR;
+            fld.pz[i] += 0.5 * (dpxa * mom.mpx + dpya * mom.mpy);
+        }
+        const double s = 0.5 * (c - fld.z[i]);
+        fld.x[i] += s * fld.px[i];
+        fld.y[i] += s * fld.py[i];
+        if (o.lkick) fld.pz[i] += 0.25 * (fld.px[i] * fld.px[i] + fld.py[i] * fld.py[i]);
+    }
+    const double sM = 0.5 * (ps.center - pf.center);
+    vx.resize(nsource); vy.resize(nsource);
+    for (int64_t i = 0; i < nsource; ++i) {
+        vx[i] = src.x[i] + src.px[i] * sM;
+        vy[i] = src.y[i] + src.py[i] * sM;
+    }
+}
+
 }  // namespace
 
 // ==========================================================================================
@@ -1814,10 +2178,14 @@ void oracle_faddeeva_w(double zr, double zi, double *wr, double *wi) {
 }
 
 void oracle_gaussian_beambeam_kick(double sigx, double sigy, double x, double y, double *kx, double *ky) {
-    *kx = *ky = 0.0;
-    if (sigx == 0.0 || sigy == 0.0) return;
-    if (sigx >= sigy) be_principal(sigx, sigy, x, y, *kx, *ky);
-    else be_principal(sigy, sigx, y, x, *ky, *kx);
+    be_kick(sigx, sigy, x, y, *kx, *ky);
+}
+
+// _gaussian_beambeam_kick_response(kbb, sigx, sigy, x, y) (strong_beam.jl:1119-1128): the normalised kick
+// and the diagonal of the potential's Hessian times kbb.
+void oracle_gaussian_beambeam_kick_response(double kbb, double sigx, double sigy, double x, double y, double out[4]) {
+    const Resp r = beambeam_response(kbb, sigx, sigy, x, y);
+    out[0] = r.kx; out[1] = r.ky; out[2] = r.h1; out[3] = r.h2;
 }
 
 void oracle_weideman_coefficients(double *c, double *L) {
