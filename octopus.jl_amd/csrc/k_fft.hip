@@ -303,12 +303,17 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_inv(const Plane *planes, c
     }
 }
 
-// FP32 mode: the cached FP64 Green half spectrum of every freshly built slot, rounded to float, with
-// the table's (0, 0) value G00 taken out of every cell first (the DC term loses G00 * P1 * P2).  A
-// constant in G moves every potential of a plane by that constant times the plane's charge: it leaves
+// FP32 mode: the cached FP64 Green half spectrum of every freshly built slot, rounded to float, with a
+// reference value Gref of the table taken out of every cell first (the DC term loses Gref * P1 * P2).
+// A constant in G moves every potential of a plane by that constant times the plane's charge: it leaves
 // E = -grad(phi) unchanged, and phi_L - phi_R too (the L, R and M planes of an interaction deposit the
 // same particles on the same mesh), but it is the bulk of |phi|, and FP32 differences of large
-// potentials would lose the longitudinal kick.
+// potentials lose digits of both kicks: the FP32 error of E is about eps * max|phi - Gref Q| / (h |E|).
+// Gref is the table at cell (nx/4, ny/4), i.e. G a quarter of the mesh away from the source-field
+// offset.  The potentials over the mesh lie between Q G(~sigma) near the source and Q G(~diagonal) at
+// the far corners; G at a quarter mesh sits near the middle of that range in log distance, so
+// max|phi - Gref Q| stays within about 1.4 Q for any offset (the cell (0, 0), which is the near field
+// when the two meshes nearly coincide, leaves up to 4.6 Q).
 __global__ __launch_bounds__(256) void k_green_to_f32(const GreenJob *jobs, const double2 *spec, float2 *spec32,
                                                       int64_t slot_stride, int nslots, int nx, int ny, int integrated,
                                                       const ErrorInfo *err) {
@@ -320,8 +325,8 @@ __global__ __launch_bounds__(256) void k_green_to_f32(const GreenJob *jobs, cons
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < slot_stride; k += (int64_t)gridDim.x * blockDim.x)
         dst[k] = make_float2((float)src[k].x, (float)src[k].y);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const double g00 = green_cell(0, 0, nx, ny, integrated, job);
-        dst[0] = make_float2((float)(src[0].x - g00 * (double)(2 * nx) * (double)(2 * ny)), (float)src[0].y);
+        const double gref = green_cell(nx / 4, ny / 4, nx, ny, integrated, job);
+        dst[0] = make_float2((float)(src[0].x - gref * (double)(2 * nx) * (double)(2 * ny)), (float)src[0].y);
     }
 }
 

@@ -68,7 +68,7 @@ def first_use_pairs(order):
     return first
 
 
-def compare_collide(oracle, gpu, solver, e, p, collides=1, turn_map=None, floor=None):
+def compare_collide(oracle, gpu, solver, e, p, collides=1, turn_map=None, floor=None, first_use_exact=True):
     ce, cp = e.copy(), p.copy()
     ocol = oracle.Collider(solver)
     gcol = PICCollider(solver)
@@ -98,10 +98,12 @@ def compare_collide(oracle, gpu, solver, e, p, collides=1, turn_map=None, floor=
             assert (a.p, a.i, a.j, a.dir, a.cache_event) == (b.p, b.i, b.j, b.dir, b.cache_event)
             ga = np.array(list(a.source_grid) + list(a.field_grid))
             gb = np.array(list(b.source_grid) + list(b.field_grid))
-            if a.p in first:
+            if a.p in first and first_use_exact:
                 assert np.array_equal(ga, gb), (a.p, a.dir)
             else:
-                assert np.allclose(ga, gb, rtol=1e-12, atol=0)
+                # origins are differences of nearly equal coordinates and can sit near zero: the bar is
+                # relative to the meshes' extent
+                assert np.allclose(ga, gb, rtol=1e-12, atol=1e-12 * np.max(np.abs(ga))), (a.p, a.dir)
         ol, gl = np.array(ol), np.array(gl)
         assert np.max(np.abs(ol - gl)) <= 1e-10 * np.max(np.abs(ol))
         so, sg = ocol.stats(), gcol.stats()
