@@ -252,6 +252,16 @@ int octopic_generate_gaussian_beam(octopic_handle *h, uint64_t seed, uint64_t rn
                                    const double *beta3, const double *alpha3, const double *sigma3,
                                    double cutoff, octopic_beam *beam);
 
+/* The same beam as octopic_generate_gaussian_beam, bit for bit, with no device allocation: the partial
+ * sums of the standardisation live in a static per-device workspace of 2050 doubles (16400 bytes, the
+ * partition's cap of 2048 blocks plus mean and rms, so every n fits), set up by octopic_create.  No
+ * counterpart in the reference: for callers that cannot allocate (HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Calls that overlap on one device share the workspace and
+ * race; calls in order on one stream, or on streams the caller orders, are safe. */
+int octopic_generate_gaussian_beam_noalloc(octopic_handle *h, uint64_t seed, uint64_t rng_id,
+                                           const double *beta3, const double *alpha3, const double *sigma3,
+                                           double cutoff, octopic_beam *beam);
+
 /* Device-side CIC/TSC cell index and weights (pic_cpu.jl:2182-2252) for n normalised coordinates u
  * on an axis of `npts` nodes: base (0-based first stencil node) and w (2 or 3 per point).
  * u, base, w are host pointers.  Used by the parity tests of the cell indices. */

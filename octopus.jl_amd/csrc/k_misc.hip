@@ -4,6 +4,7 @@
 //   generate : Beam(N, ...) counter-RNG construction (src/beam/Beam.jl:127-181, 545-553) with the
 //              Philox4x32-10 stream of src/math/counter_rng.jl:184-277 and Box-Muller (153-173).
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 #include "launch.h"
@@ -225,6 +226,48 @@ hipError_t launch_generate(double *const c[6], int64_t n, uint64_t seed, uint64_
     e = launch_generate_ws(c, n, seed, rng_id, cutoff, ws, s);
     hipFreeAsync(ws, s);
     return e != hipSuccess ? e : hipGetLastError();
+}
+
+// The generator's workspace for callers that cannot allocate: one static array per device, as large as
+// the partition's cap needs (2048 block partials + mean + rms = 16400 bytes), so every n fits.  Every
+// entry is written before it is read within a call, so nothing is cleared; two calls that overlap on
+// one device share the array and race.  In host code `scratch` names a host-side stand-in: its device
+// address comes from hipGetSymbolAddress, once per device, and only that address is used.
+constexpr int kGenScratchDoubles = kGenMaxBlocks + 2;
+static __device__ __align__(256) double scratch[kGenScratchDoubles];
+constexpr int kScratchMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kScratchMaxDevices];
+
+int64_t generate_scratch_doubles() { return kGenScratchDoubles; }
+
+hipError_t generate_scratch_init(int device) {
+    if (device < 0 || device >= kScratchMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;   // the caller makes `device` current
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return hipSuccess;
+}
+
+hipError_t launch_generate_noalloc(double *const c[6], int64_t n, uint64_t seed, uint64_t rng_id, double cutoff, int device,
+                                   hipStream_t s) {
+    if (device < 0 || device >= kScratchMaxDevices) return hipErrorInvalidDevice;
+    if (n <= 0 || generate_workspace_doubles(n) > kGenScratchDoubles) return hipErrorInvalidValue;
+    double *ws = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        ws = g_scratch_ptr[device];
+    }
+    if (!ws) return hipErrorNotInitialized;
+    return launch_generate_ws(c, n, seed, rng_id, cutoff, ws, s);
 }
 
 

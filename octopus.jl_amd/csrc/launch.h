@@ -297,6 +297,12 @@ hipError_t launch_generate(double *const c[6], int64_t n, uint64_t seed, uint64_
 int64_t generate_workspace_doubles(int64_t n);
 hipError_t launch_generate_ws(double *const c[6], int64_t n, uint64_t seed, uint64_t rng_id, double cutoff, double *ws,
                               hipStream_t s);
+// The same on the library's static per-device workspace (no allocation): generate_scratch_init(device)
+// once per device with that device current (octopic_create does it), then launch_generate_noalloc.
+int64_t generate_scratch_doubles();
+hipError_t generate_scratch_init(int device);
+hipError_t launch_generate_noalloc(double *const c[6], int64_t n, uint64_t seed, uint64_t rng_id, double cutoff, int device,
+                                   hipStream_t s);
 hipError_t launch_scale_beam(double *const c[6], int64_t n, const double *beta, const double *alpha, const double *sigma,
                              hipStream_t s);
 hipError_t launch_widen(const float *src, double *dst, int64_t n, hipStream_t s);

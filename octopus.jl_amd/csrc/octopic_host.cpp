@@ -590,6 +590,7 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         h->device = device;
         HIPCHECK(hipSetDevice(device));
         HIPCHECK(hipStreamCreate(&h->own_stream));   // default priority; orders with the null stream
+        HIPCHECK(generate_scratch_init(device));     // the allocation-free generator's per-device workspace
         h->stream = h->own_stream;
         h->nx = opt->nx; h->ny = opt->ny;
         h->fp32 = opt->precision == OCTOPIC_PRECISION_FP32;
@@ -1655,6 +1656,29 @@ int octopic_generate_gaussian_beam(octopic_handle *h, uint64_t seed, uint64_t rn
     hipSetDevice(h->device);
     if (launch_generate(c, b->n, seed, rng_id, cutoff, h->stream) != hipSuccess)
         return set_err(h, OCTOPIC_ERR_DEVICE, "beam generation failed");
+    if (launch_scale_beam(c, b->n, beta3, alpha3, sigma3, h->stream) != hipSuccess)
+        return set_err(h, OCTOPIC_ERR_DEVICE, "beam scaling failed");
+    return OCTOPIC_OK;
+}
+
+int octopic_generate_gaussian_beam_noalloc(octopic_handle *h, uint64_t seed, uint64_t rng_id, const double *beta3,
+                                           const double *alpha3, const double *sigma3, double cutoff, octopic_beam *b) {
+    if (!h || !b || !beta3 || !alpha3 || !sigma3) return OCTOPIC_ERR_INVALID_ARGUMENT;
+    if (!b->on_device)
+        return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "octopic_generate_gaussian_beam_noalloc needs device pointers");
+    if (b->n <= 0) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "random Beam construction requires a positive particle count");
+    if (b->dtype != OCTOPIC_DTYPE_F64) return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "the beam generator writes FP64 beams");
+    if (generate_workspace_doubles(b->n) > generate_scratch_doubles())
+        return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT,
+                       "octopic_generate_gaussian_beam_noalloc: the workspace for n = " + std::to_string(b->n) + " needs " +
+                           std::to_string(generate_workspace_doubles(b->n)) + " doubles; the static workspace holds " +
+                           std::to_string(generate_scratch_doubles()));
+    double *c[6] = {b->x, b->px, b->y, b->py, b->z, b->pz};
+    if (hipSetDevice(h->device) != hipSuccess) return set_err(h, OCTOPIC_ERR_DEVICE, "hipSetDevice failed");
+    const hipError_t e = launch_generate_noalloc(c, b->n, seed, rng_id, cutoff, h->device, h->stream);
+    if (e == hipErrorNotInitialized)
+        return set_err(h, OCTOPIC_ERR_DEVICE, "the static generator workspace of this device was not set up");
+    if (e != hipSuccess) return set_err(h, OCTOPIC_ERR_DEVICE, "beam generation failed");
     if (launch_scale_beam(c, b->n, beta3, alpha3, sigma3, h->stream) != hipSuccess)
         return set_err(h, OCTOPIC_ERR_DEVICE, "beam scaling failed");
     return OCTOPIC_OK;

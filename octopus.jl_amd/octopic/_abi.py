@@ -184,6 +184,10 @@ def declare(lib):
             i32,
             [vp, C.c_uint64, C.c_uint64, DPTR, DPTR, DPTR, C.c_double, C.POINTER(BeamStruct)],
         ),
+        "octopic_generate_gaussian_beam_noalloc": (
+            i32,
+            [vp, C.c_uint64, C.c_uint64, DPTR, DPTR, DPTR, C.c_double, C.POINTER(BeamStruct)],
+        ),
         "octopic_debug_weights": (i32, [vp, i32, DPTR, i64, i32, C.POINTER(i32), DPTR]),
         "octopic_collide_many": (i32, [C.POINTER(vp), C.POINTER(BeamStruct), C.POINTER(BeamStruct), C.POINTER(i64),
                                        C.POINTER(i32), DPTR, C.POINTER(i32), i32]),
@@ -234,6 +238,7 @@ EXPORTED = [
     "octopic_faddeeva_w",
     "octopic_gaussian_beambeam_kick_response",
     "octopic_generate_gaussian_beam",
+    "octopic_generate_gaussian_beam_noalloc",
     "octopic_debug_weights",
     "octopic_collide_many",
     "octopic_set_timing",
