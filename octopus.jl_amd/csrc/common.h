@@ -100,6 +100,15 @@ __device__ __forceinline__ unsigned long long fixed_point(double v) {
     return (unsigned long long)__builtin_bit_cast(long long, t) - 0x4330000000000000ull;
 }
 
+// round(a * b) to uint64 for a * b in [0, 2^52), in one rounding of the exact product: the fused
+// a * b + 2^52 lands the rounded integer in the low 52 mantissa bits, which a mask keeps.  With a the
+// weight times the fixed-point scale (a power of two, so that product is exact) this is the deposit's
+// quantisation round(wx * wy * 2^k).
+__device__ __forceinline__ unsigned long long fixed_point_product(double a, double b) {
+    const double t = __builtin_fma(a, b, 4503599627370496.0);   // 2^52
+    return (unsigned long long)__builtin_bit_cast(long long, t) & 0x000FFFFFFFFFFFFFull;
+}
+
 // Order-preserving uint64 key of a non-NaN double (-0.0 sorts below +0.0, as Julia's min/max), so that
 // exact min/max reductions can use 64-bit integer atomics in any order.
 __host__ __device__ __forceinline__ unsigned long long order_key(double x) {

@@ -11,6 +11,8 @@
 //   rows_inv   : per row pair, Hermitian rebuild + IFFT_x, keep x < nx         -> phi[y][x]
 //   field      : E = -grad(phi), 2nd/4th order (pic_cpu.jl:2531-2591)           -> table[x][y][3 per plane]
 // Green spectra are built the same way from the full 2nx x 2ny real table (green_rows, green_cols).
+#include <algorithm>
+
 #include "common.h"
 #include "fft.h"
 #include "launch.h"
@@ -87,53 +89,101 @@ __device__ __forceinline__ void store_row_spectra(C *lds, const FFTPlanDev &plan
     }
 }
 
+// The jobs of a Green launch that build (valid, slot in range), in job order, compacted into `list` by
+// every block of the launch the same way (ballot per wave, then the waves in order); returns their
+// number.  Called by all threads of the block; the list is visible to all of them on return.
+constexpr int kMaxGreenJobs = 2048;
+constexpr int kGreenSmallGrid = 64;   // blocks of a Green launch after a collide that built little
+__device__ __forceinline__ int compact_green_jobs(const GreenJob *jobs, int njobs, int nslots, int *list, int *wcount) {
+    int total = 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    for (int base = 0; base < njobs; base += blockDim.x) {
+        const int k = base + threadIdx.x;
+        bool v = false;
+        if (k < njobs) {
+            const GreenJob &j = jobs[k];
+            v = j.valid && j.slot >= 0 && j.slot < nslots;
+        }
+        const unsigned long long m = __ballot(v);
+        if (lane == 0) wcount[wave] = __popcll(m);
+        __syncthreads();
+        int off = total, add = 0;
+        for (int w = 0; w < nwaves; ++w) {
+            if (w < wave) off += wcount[w];
+            add += wcount[w];
+        }
+        if (v) list[off + __popcll(m & ((1ull << lane) - 1ull))] = k;
+        __syncthreads();
+        total += add;
+    }
+    return total;
+}
+
 // Green rows: rows (ya, ya+1) of the real table -> x-half-spectrum into slot[kx][y], y in [0, 2ny).
+// Items are (building job, block of 2 * kFFTPerBlock rows); the blocks of the launch take them in
+// turn, so a launch whose batch builds nothing is one pass of its blocks over the job flags.
 __global__ __launch_bounds__(kFFTThreads) void k_green_rows(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride,
                                                             int nslots, int nx, int ny, int integrated, FFTPlanDev px,
                                                             const ErrorInfo *err) {
     extern __shared__ double2 lds[];
+    __shared__ int list[kMaxGreenJobs];
+    __shared__ int wcount[kFFTThreads / 64];
     if (failed(err)) return;
-    const GreenJob job = jobs[blockIdx.y];
-    if (!job.valid || job.slot < 0 || job.slot >= nslots) return;
+    const int nbuild = compact_green_jobs(jobs, njobs, nslots, list, wcount);
+    if (nbuild == 0) return;
     const int P1 = 2 * nx, P2 = 2 * ny;
+    const int rblocks = (P2 + 2 * kFFTPerBlock - 1) / (2 * kFFTPerBlock);
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
     double2 *a = group_buffer(lds, P1, grp);
-    const int ya0 = 2 * kFFTPerBlock * blockIdx.x, ya = ya0 + 2 * grp;
     load_twiddles(px, lds);
-    for (int x = lane; x < P1; x += kFFTLanes) {
-        double va = 0.0, vb = 0.0;
-        if (ya < P2) {
-            va = green_cell(x, ya, nx, ny, integrated, job);
-            vb = green_cell(x, ya + 1, nx, ny, integrated, job);
+    for (int item = blockIdx.x; item < nbuild * rblocks; item += gridDim.x) {
+        const GreenJob job = jobs[list[item / rblocks]];
+        const int ya0 = 2 * kFFTPerBlock * (item % rblocks), ya = ya0 + 2 * grp;
+        for (int x = lane; x < P1; x += kFFTLanes) {
+            double va = 0.0, vb = 0.0;
+            if (ya < P2) {
+                va = green_cell(x, ya, nx, ny, integrated, job);
+                vb = green_cell(x, ya + 1, nx, ny, integrated, job);
+            }
+            a[x] = make_double2(va, vb);
         }
-        a[x] = make_double2(va, vb);
+        __syncthreads();
+        fft_lds<-1>(a, a + P1, px, lds, lane);
+        __syncthreads();
+        store_row_spectra(lds, px, P2, ya0, spec + (size_t)job.slot * slot_stride, P2);
+        __syncthreads();   // the item's buffers are read before the next item writes them
     }
-    __syncthreads();
-    fft_lds<-1>(a, a + P1, px, lds, lane);
-    __syncthreads();
-    store_row_spectra(lds, px, P2, ya0, spec + (size_t)job.slot * slot_stride, P2);
 }
 
-// Green columns: in-place FFT along y of slot[kx][0..2ny).
+// Green columns: in-place FFT along y of slot[kx][0..2ny); items are (building job, block of
+// kFFTPerBlock columns).
 __global__ __launch_bounds__(kFFTThreads) void k_green_cols(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride,
                                                             int nslots, int nx, int ny, FFTPlanDev py, const ErrorInfo *err) {
     extern __shared__ double2 lds[];
+    __shared__ int list[kMaxGreenJobs];
+    __shared__ int wcount[kFFTThreads / 64];
     if (failed(err)) return;
-    const GreenJob job = jobs[blockIdx.y];
-    if (!job.valid || job.slot < 0 || job.slot >= nslots) return;
+    const int nbuild = compact_green_jobs(jobs, njobs, nslots, list, wcount);
+    if (nbuild == 0) return;
     const int P2 = 2 * ny;
+    const int cblocks = (nx + 1 + kFFTPerBlock - 1) / kFFTPerBlock;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
     double2 *a = group_buffer(lds, P2, grp);
-    const int kx = blockIdx.x * kFFTPerBlock + grp;
-    const bool active = kx <= nx;
-    double2 *col = spec + (size_t)job.slot * slot_stride + (size_t)(active ? kx : 0) * P2;
     load_twiddles(py, lds);
-    for (int y = lane; y < P2; y += kFFTLanes) a[y] = active ? col[y] : make_double2(0.0, 0.0);
-    __syncthreads();
-    fft_lds<-1>(a, a + P2, py, lds, lane);
-    if (!active) return;
-    const double2 *Z = fft_result(a, a + P2, py);
-    for (int y = lane; y < P2; y += kFFTLanes) col[y] = Z[y];
+    for (int item = blockIdx.x; item < nbuild * cblocks; item += gridDim.x) {
+        const GreenJob job = jobs[list[item / cblocks]];
+        const int kx = (item % cblocks) * kFFTPerBlock + grp;
+        const bool active = kx <= nx;
+        double2 *col = spec + (size_t)job.slot * slot_stride + (size_t)(active ? kx : 0) * P2;
+        for (int y = lane; y < P2; y += kFFTLanes) a[y] = active ? col[y] : make_double2(0.0, 0.0);
+        __syncthreads();
+        fft_lds<-1>(a, a + P2, py, lds, lane);
+        if (active) {
+            const double2 *Z = fft_result(a, a + P2, py);
+            for (int y = lane; y < P2; y += kFFTLanes) col[y] = Z[y];
+        }
+        __syncthreads();   // the item's buffers are read before the next item writes them
+    }
 }
 
 // Luminosity overlap (pic_cpu.jl:2731-2735), one of kLumBlocks blocks of pair k: the block sums q1.q2
@@ -314,19 +364,25 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_inv(const Plane *planes, c
 // the far corners; G at a quarter mesh sits near the middle of that range in log distance, so
 // max|phi - Gref Q| stays within about 1.4 Q for any offset (the cell (0, 0), which is the near field
 // when the two meshes nearly coincide, leaves up to 4.6 Q).
-__global__ __launch_bounds__(256) void k_green_to_f32(const GreenJob *jobs, const double2 *spec, float2 *spec32,
+__global__ __launch_bounds__(256) void k_green_to_f32(const GreenJob *jobs, int njobs, const double2 *spec, float2 *spec32,
                                                       int64_t slot_stride, int nslots, int nx, int ny, int integrated,
                                                       const ErrorInfo *err) {
+    __shared__ int list[kMaxGreenJobs];
+    __shared__ int wcount[256 / 64];
     if (failed(err)) return;
-    const GreenJob job = jobs[blockIdx.y];
-    if (!job.valid || job.slot < 0 || job.slot >= nslots) return;
-    const double2 *src = spec + (size_t)job.slot * slot_stride;
-    float2 *dst = spec32 + (size_t)job.slot * slot_stride;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < slot_stride; k += (int64_t)gridDim.x * blockDim.x)
-        dst[k] = make_float2((float)src[k].x, (float)src[k].y);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const double gref = green_cell(nx / 4, ny / 4, nx, ny, integrated, job);
-        dst[0] = make_float2((float)(src[0].x - gref * (double)(2 * nx) * (double)(2 * ny)), (float)src[0].y);
+    const int nbuild = compact_green_jobs(jobs, njobs, nslots, list, wcount);
+    constexpr int kParts = 32;   // blocks' worth of one slot
+    for (int item = blockIdx.x; item < nbuild * kParts; item += gridDim.x) {
+        const GreenJob job = jobs[list[item / kParts]];
+        const int part = item % kParts;
+        const double2 *src = spec + (size_t)job.slot * slot_stride;
+        float2 *dst = spec32 + (size_t)job.slot * slot_stride;
+        for (int64_t k = (int64_t)part * blockDim.x + threadIdx.x; k < slot_stride; k += (int64_t)kParts * blockDim.x)
+            dst[k] = make_float2((float)src[k].x, (float)src[k].y);
+        if (part == 0 && threadIdx.x == 0) {
+            const double gref = green_cell(nx / 4, ny / 4, nx, ny, integrated, job);
+            dst[0] = make_float2((float)(src[0].x - gref * (double)(2 * nx) * (double)(2 * ny)), (float)src[0].y);
+        }
     }
 }
 
@@ -410,19 +466,28 @@ static hipError_t ensure_lds(const void *fn, size_t bytes) {
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+// `full` launches a block per item of every job (the cold case: every job builds); otherwise a small
+// persistent grid takes the items of the few jobs that build, so batches of cache hits cost one short
+// pass.  Results do not depend on the grid.
 hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride, int nslots, int nx, int ny,
                               int integrated, const FFTPlanDev &px, const FFTPlanDev &py, const ErrorInfo *err,
-                              hipStream_t s) {
+                              hipStream_t s, int full) {
     if (njobs <= 0) return hipSuccess;
+    for (; njobs > kMaxGreenJobs; jobs += kMaxGreenJobs, njobs -= kMaxGreenJobs) {   // each launch compacts its own jobs
+        const hipError_t e = launch_green_build(jobs, kMaxGreenJobs, spec, slot_stride, nslots, nx, ny, integrated, px, py, err,
+                                                s, full);
+        if (e != hipSuccess) return e;
+    }
     const int P1 = 2 * nx, P2 = 2 * ny;
     ensure_lds((const void *)k_green_rows, fft_lds_bytes(P1));
     ensure_lds((const void *)k_green_cols, fft_lds_bytes(P2));
-    dim3 g1((P2 + 2 * kFFTPerBlock - 1) / (2 * kFFTPerBlock), njobs);
-    hipLaunchKernelGGL(k_green_rows, g1, dim3(kFFTThreads), fft_lds_bytes(P1), s, jobs, njobs, spec, slot_stride, nslots, nx,
-                       ny, integrated, px, err);
-    dim3 g2((nx + 1 + kFFTPerBlock - 1) / kFFTPerBlock, njobs);
-    hipLaunchKernelGGL(k_green_cols, g2, dim3(kFFTThreads), fft_lds_bytes(P2), s, jobs, njobs, spec, slot_stride, nslots, nx,
-                       ny, py, err);
+    const int rblocks = (P2 + 2 * kFFTPerBlock - 1) / (2 * kFFTPerBlock), cblocks = (nx + 1 + kFFTPerBlock - 1) / kFFTPerBlock;
+    const int g1 = full ? njobs * rblocks : std::min(njobs * rblocks, kGreenSmallGrid);
+    const int g2 = full ? njobs * cblocks : std::min(njobs * cblocks, kGreenSmallGrid);
+    hipLaunchKernelGGL(k_green_rows, dim3(g1), dim3(kFFTThreads), fft_lds_bytes(P1), s, jobs, njobs, spec, slot_stride, nslots,
+                       nx, ny, integrated, px, err);
+    hipLaunchKernelGGL(k_green_cols, dim3(g2), dim3(kFFTThreads), fft_lds_bytes(P2), s, jobs, njobs, spec, slot_stride, nslots,
+                       nx, ny, py, err);
     return hipGetLastError();
 }
 
@@ -475,9 +540,15 @@ hipError_t launch_field(const Inter *inters, int ninters, const Plane *planes, c
 }
 
 hipError_t launch_green_to_f32(const GreenJob *jobs, int njobs, const double2 *spec, float2 *spec32, int64_t slot_stride,
-                               int nslots, int nx, int ny, int integrated, const ErrorInfo *err, hipStream_t s) {
+                               int nslots, int nx, int ny, int integrated, const ErrorInfo *err, hipStream_t s, int full) {
     if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_green_to_f32, dim3(32, njobs), dim3(256), 0, s, jobs, spec, spec32, slot_stride, nslots, nx, ny,
+    for (; njobs > kMaxGreenJobs; jobs += kMaxGreenJobs, njobs -= kMaxGreenJobs) {
+        const hipError_t e = launch_green_to_f32(jobs, kMaxGreenJobs, spec, spec32, slot_stride, nslots, nx, ny, integrated, err,
+                                                 s, full);
+        if (e != hipSuccess) return e;
+    }
+    const int g = full ? 32 * njobs : std::min(32 * njobs, kGreenSmallGrid);
+    hipLaunchKernelGGL(k_green_to_f32, dim3(g), dim3(256), 0, s, jobs, njobs, spec, spec32, slot_stride, nslots, nx, ny,
                        integrated, err);
     return hipGetLastError();
 }

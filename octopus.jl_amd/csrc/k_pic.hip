@@ -535,7 +535,7 @@ __device__ __forceinline__ void deposit_item(const DepositArgs &a, unsigned long
             for (int q = 0; q < K; ++q) {
                 // weights lie in [0, 1] and scale <= 2^40, so the product is in fixed_point's range
                 unsigned long long v;
-                if constexpr (sizeof(W) == sizeof(double)) v = fixed_point(wx[m] * wy[q] * scale);
+                if constexpr (sizeof(W) == sizeof(double)) v = fixed_point_product(wx[m] * scale, wy[q]);
                 else v = fixed_point((double)((float)wx[m] * (float)wy[q] * (float)scale));
                 // a zero weight adds nothing (off-mesh positions sit at node 0, possibly outside the window)
                 if (!v) continue;
@@ -649,7 +649,7 @@ __device__ __forceinline__ void deposit_item_multi(const DepositArgs &a, unsigne
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
                     unsigned long long v;
-                    if constexpr (sizeof(W) == sizeof(double)) v = fixed_point(wx[m] * wy[q] * scale);
+                    if constexpr (sizeof(W) == sizeof(double)) v = fixed_point_product(wx[m] * scale, wy[q]);
                     else v = fixed_point((double)((float)wx[m] * (float)wy[q] * (float)scale));
                     if (v) atomicAdd(&tp[(by + q - wy0) * wnx + (bx + m - wx0)], v);
                 }
@@ -670,7 +670,7 @@ __device__ __forceinline__ void deposit_item_multi(const DepositArgs &a, unsigne
             for (int m = 0; m < K; ++m)
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
-                    const unsigned long long v = fixed_point(wx[m] * wy[q] * scale);
+                    const unsigned long long v = fixed_point_product(wx[m] * scale, wy[q]);
                     if (!v) continue;
                     // the window is the chunk box's, which holds the sM positions when the field slice's
                     // centre lies between its boundaries; a node outside it adds straight into HBM

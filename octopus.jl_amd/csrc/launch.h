@@ -232,9 +232,11 @@ struct LumArgs {
 constexpr int kLumBlocks = 16;   // blocks per pair of the luminosity overlap sum (in the charge row launch)
 
 // FFT stages (k_fft.hip)
+// full = 1: a block per (job, row block) of every job, for launches where most jobs build; 0: a small
+// persistent grid over the jobs that build.  Results do not depend on it.
 hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride, int nslots, int nx, int ny,
                               int integrated, const FFTPlanDev &px, const FFTPlanDev &py, const ErrorInfo *err,
-                              hipStream_t s);
+                              hipStream_t s, int full);
 template <class C>   // C = double2 (FP64 mode) or float2 (FP32 mode)
 hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long *charge, int64_t charge_stride, C *qh,
                            int64_t qh_stride, int nx, int ny, double inv_scale, const FFTPlanDev &px, const ErrorInfo *err,
@@ -249,7 +251,7 @@ template <class R>
 hipError_t launch_field(const Inter *inters, int ninters, const Plane *planes, const R *phi, R *table, int nx, int ny,
                         int fourth, int npl, const ErrorInfo *err, hipStream_t s);
 hipError_t launch_green_to_f32(const GreenJob *jobs, int njobs, const double2 *spec, float2 *spec32, int64_t slot_stride,
-                               int nslots, int nx, int ny, int integrated, const ErrorInfo *err, hipStream_t s);
+                               int nslots, int nx, int ny, int integrated, const ErrorInfo *err, hipStream_t s, int full);
 
 // Node-mode mesh preparation (k_node.hip)
 struct NodeArgs {

@@ -246,6 +246,9 @@ struct octopic_handle {
     bool deposit_per_plane = false;   // -1: the slower form at the last trial was all planes
     bool deposit_all_planes_used = false;
     int64_t deposit_trial = 0;        // collides since the first trial
+    // Green tables built by the previous collide (-1: none yet).  A collide after one that built at least a
+    // table per batch launches its Green builds over every job; otherwise over the few that build.
+    int64_t last_builds = -1;
     double deposit_trial_ms[2] = {0.0, 0.0};
     std::vector<hipEvent_t> deposit_ev;   // per batch: before and after the deposit launches (trials)
     int deposit_min_span = kChunkSpan;   // smallest items the two-per-CU rule may make
@@ -625,9 +628,9 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
             h->cus = 256;
         h->box.ensure(8);
         h->errinfo.ensure(1);
-        h->counters.ensure(8);
+        h->counters.ensure(16);   // live counters, then the snapshot a collide takes before its batches
         HIPCHECK(hipMemset(h->errinfo.p, 0, sizeof(ErrorInfo)));
-        HIPCHECK(hipMemset(h->counters.p, 0, 8 * sizeof(unsigned long long)));
+        HIPCHECK(hipMemset(h->counters.p, 0, 16 * sizeof(unsigned long long)));
         h->lanes.ensure(kFoldLanes);
         h->mm.ensure(2 * (kFoldLanes / 256));
         h->bins.ensure(16384);
@@ -642,7 +645,7 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         }
         HIPCHECK(hipHostMalloc((void **)&h->h_tab, 2 * sizeof(SliceTable)));
         HIPCHECK(hipHostMalloc((void **)&h->h_err, sizeof(ErrorInfo)));
-        HIPCHECK(hipHostMalloc((void **)&h->h_counters, 8 * sizeof(unsigned long long)));
+        HIPCHECK(hipHostMalloc((void **)&h->h_counters, 16 * sizeof(unsigned long long)));
         HIPCHECK(hipDeviceSynchronize());
     } catch (Fail &f) {
         int code = f.code;
@@ -1170,10 +1173,8 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     const int nslots = h->nkeys + (o.green_cache == OCTOPIC_CACHE_SLICE_PAIR ? 0 : 2 * W);
     h->spec.ensure((size_t)std::max(nslots, 1) * h->slot_stride);
     if (h->fp32) h->spec32.ensure((size_t)std::max(nslots, 1) * h->slot_stride);
-    unsigned long long counters_before[8];
-    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipStreamSynchronize(s));
-    std::memcpy(counters_before, h->h_counters, sizeof(counters_before));
+    // the counters as this collide's batches start, snapshotted on the device (no host wait here)
+    HIPCHECK(hipMemcpyAsync(h->counters.p + 8, h->counters.p, 8 * sizeof(unsigned long long), hipMemcpyDeviceToDevice, s));
 
     GeomOpts go{};
     go.nx = nx; go.ny = ny; go.lum_nx = h->lnx; go.lum_ny = h->lny;
@@ -1228,10 +1229,10 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         }
         HIPCHECK(launch_green_build(h->node_jobs.p, nn1 + nn2, h->spec.p, h->slot_stride, nn1 + nn2, nx, ny,
                                     o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev, h->errinfo.p,
-                                    s));
+                                    s, 1));
         if (h->fp32)
             HIPCHECK(launch_green_to_f32(h->node_jobs.p, nn1 + nn2, h->spec.p, h->spec32.p, h->slot_stride, nn1 + nn2, nx,
-                                         ny, o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->errinfo.p, s));
+                                         ny, o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->errinfo.p, s, 1));
         h->ran_dev.ensure(std::max(npairs, 1));
         HIPCHECK(hipMemsetAsync(h->ran_dev.p, 0, sizeof(int32_t) * npairs, s));
     }
@@ -1248,6 +1249,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             HIPCHECK(hipEventCreate(&ev));
             h->deposit_ev.push_back(ev);
         }
+    const int green_full = (h->last_builds < 0 || h->last_builds >= (int64_t)batches.size()) ? 1 : 0;
     for (size_t bi = 0; bi < batches.size(); ++bi) {
         const int B = bcnt[bi];
         if (B == 0) continue;
@@ -1293,10 +1295,10 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             Timed t(h, F_GREEN);
             HIPCHECK(launch_green_build(h->jobs.p, 2 * B, h->spec.p, h->slot_stride, nslots, nx, ny,
                                         o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev,
-                                        h->errinfo.p, s));
+                                        h->errinfo.p, s, green_full));
             if (h->fp32)
                 HIPCHECK(launch_green_to_f32(h->jobs.p, 2 * B, h->spec.p, h->spec32.p, h->slot_stride, nslots, nx, ny,
-                                             o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->errinfo.p, s));
+                                             o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->errinfo.p, s, green_full));
         }
         const int nfp = 2 * B * npl;
         {
@@ -1449,7 +1451,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     if (npairs)
         HIPCHECK(hipMemcpyAsync(h->recs.data(), h->records.p, sizeof(octopic_interaction_record) * 2 * npairs,
                                 hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(h->h_counters, h->counters.p, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     std::vector<int32_t> ran_node;
     if (h->mode == 2 && npairs) {
         ran_node.resize(npairs);
@@ -1498,10 +1500,12 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     }
     if (lum_out) *lum_out = lum;
     octopic_stats &S = h->stats;
+    const unsigned long long *counters_before = h->h_counters + 8;
     S.green_hits = (int64_t)h->h_counters[0];
     S.green_misses = (int64_t)h->h_counters[1];
     S.green_rebuilds = (int64_t)h->h_counters[2];
     S.green_builds = (int64_t)(h->h_counters[3] - counters_before[3]);
+    h->last_builds = S.green_builds;
     S.dropped = (int64_t)(h->h_counters[4] - counters_before[4]);
     S.denoised_planes = (int64_t)(h->h_counters[6] - counters_before[6]);
     S.denoise_fallbacks = (int64_t)(h->h_counters[7] - counters_before[7]);
