@@ -58,9 +58,21 @@ enum {
 };
 enum { OCTOPIC_CENTER_CENTROID = 0, OCTOPIC_CENTER_MIDPOINT = 1 };
 enum { OCTOPIC_PRECISION_FP64 = 0, OCTOPIC_PRECISION_FP32 = 1 };
-/* The solver a handle runs: PICPoissonSolver, or GaussianPICPoissonSolver (gaussian_pic.jl:87-131), the
- * Gaussian-subtracted hybrid that composes it. */
-enum { OCTOPIC_SOLVER_PIC = 0, OCTOPIC_SOLVER_GAUSSIAN_PIC = 1 };
+/* The solver a handle runs: PICPoissonSolver, GaussianPICPoissonSolver (gaussian_pic.jl:87-131), the
+ * Gaussian-subtracted hybrid that composes it, or GaussianPoissonSolver (interface.jl:876-1010,
+ * gaussian.jl), the sliced soft-Gaussian solver (no mesh: every slice pair kicks with the exact
+ * Bassetti-Erskine field of the other slice's Gaussian moments). */
+enum { OCTOPIC_SOLVER_PIC = 0, OCTOPIC_SOLVER_GAUSSIAN_PIC = 1, OCTOPIC_SOLVER_SOFT_GAUSSIAN = 2 };
+/* The soft-Gaussian solver's virtual drift to and from the collision point (strong_beam.jl:11-37,
+ * 497-617): HirataParaxialDrift (the default), ChromaticDrift, ExactHamiltonianDrift, and the two
+ * non-symplectic UnsafeVirtualDrift models. */
+enum {
+    OCTOPIC_DRIFT_HIRATA = 0,
+    OCTOPIC_DRIFT_CHROMATIC = 1,
+    OCTOPIC_DRIFT_EXACT = 2,
+    OCTOPIC_DRIFT_UNSAFE_PARAXIAL_FROZEN_LONGITUDINAL = 3,
+    OCTOPIC_DRIFT_UNSAFE_CHROMATIC_FROZEN_ENERGY = 4
+};
 
 /* LongitudinalSlicing (interface.jl:800-835).  `positions` holds nslices-1 internal boundaries in
  * units of the bunch rms (method SPECIFIED only); it may be NULL otherwise. */
@@ -115,6 +127,18 @@ typedef struct octopic_options {
     int32_t neutralize;            /* 0 | 1 */
     double margin_sigma;
     double coupling_tol;
+    /* GaussianPoissonSolver (interface.jl:876-1010), with kbb1/kbb2/luminosity_scale, slicing1/2,
+     * longitudinal_kick and batch_mode above (the mesh fields are ignored): min_sigma >= 0 floors each
+     * slice's transverse rms; gaussian_when_luminosity 1 | 2 names the beam whose particles sample the
+     * other beam's Gaussian for the luminosity; ignore_centroid1/2 zero a beam's slice centroids;
+     * include_sigma_xy = 1 (the coupled, rotated kick) is not implemented here (UNSUPPORTED).  Ignored
+     * for the other solvers. */
+    double min_sigma;
+    int32_t virtual_drift;         /* OCTOPIC_DRIFT_* */
+    int32_t gaussian_when_luminosity;
+    int32_t ignore_centroid1, ignore_centroid2;
+    int32_t include_sigma_xy;
+    int32_t reserved;
 } octopic_options;
 
 /* Element type of a beam's coordinate arrays: Phase6DRep{Float64} or Phase6DRep{Float32}

@@ -430,6 +430,192 @@ __global__ __launch_bounds__(256) void k_gpic_addback(WeidemanCoeffs wc, GpicAdd
     o[3] = 0.5 * (dpxa * it.gmpx + dpya * it.gmpy);
 }
 
+// ---------------------------------------------------------------------- the soft-Gaussian solver
+// GaussianPoissonSolver (gaussian.jl), uncoupled: the virtual drifts (strong_beam.jl:497-617; forward returns
+// S = (z - zo)/2 of the drift's rule), and one particle's map between them (_apply_slice_kick_one!,
+// gaussian.jl:152-188, with the uncoupled _cp_covariance_kick and _transport_transverse_moments,
+// strong_beam.jl:619-632, 679-687).  Returns the luminosity density times 2 pi.
+struct SoftMoments { double mx, mpx, my, mpy, a0, d0, bxx, bypy, qxx, qyy; };
+constexpr double kTwoPi = 6.283185307179586;
+
+__device__ __forceinline__ double soft_forward_drift(int d, double &x, double px, double &y, double py, double &z, double &pz,
+                                                     double zo) {
+    if (d == OCTOPIC_DRIFT_EXACT) {
+        const double ps = sqrt((1 + pz) * (1 + pz) - px * px - py * py);
+        const double H = 1 + pz - ps;
+        const double rr = 0.5 * H / ps;
+        const double z2 = (z + rr * zo) / (1 + rr);
+        const double S = 0.5 * (z2 - zo);
+        z -= H / ps * S;
+        pz -= 0.5 * H;
+        x += px / ps * S;
+        y += py / ps * S;
+        return S;
+    }
+    const double S = 0.5 * (z - zo);
+    if (d == OCTOPIC_DRIFT_HIRATA) {
+        x += S * px; y += S * py;
+        pz -= 0.25 * (px * px + py * py);
+    } else if (d == OCTOPIC_DRIFT_UNSAFE_PARAXIAL_FROZEN_LONGITUDINAL) {
+        x += S * px; y += S * py;
+    } else {   // chromatic, and its frozen-energy form
+        const double phi = sqrt(1 - 0.5 * (px * px + py * py) / ((1 + pz) * (1 + pz))) - 1;
+        x += S * px / (1 + pz);
+        y += S * py / (1 + pz);
+        z += 2 * S * phi;
+        if (d == OCTOPIC_DRIFT_CHROMATIC) pz += (1 + pz) * phi;
+    }
+    return S;
+}
+
+__device__ __forceinline__ void soft_reverse_drift(int d, double &x, double px, double &y, double py, double &z, double &pz,
+                                                   double zo) {
+    const double S = 0.5 * (z - zo);
+    if (d == OCTOPIC_DRIFT_EXACT) {
+        const double H0 = 0.5 * (px * px + py * py) / (1 + pz);
+        const double ps0 = 1 + pz - 0.5 * H0;
+        pz += 0.5 * H0;
+        x -= px / ps0 * S;
+        y -= py / ps0 * S;
+        z += H0 / ps0 * S;
+    } else if (d == OCTOPIC_DRIFT_HIRATA) {
+        x -= S * px; y -= S * py;
+        pz += 0.25 * (px * px + py * py);
+    } else if (d == OCTOPIC_DRIFT_UNSAFE_PARAXIAL_FROZEN_LONGITUDINAL) {
+        x -= S * px; y -= S * py;
+    } else {
+        const double psi = sqrt(1 + 0.5 * (px * px + py * py) / ((1 + pz) * (1 + pz))) - 1;
+        x -= S * px / (1 + pz);
+        y -= S * py / (1 + pz);
+        z += 2 * S * psi;
+        if (d == OCTOPIC_DRIFT_CHROMATIC) pz += (1 + pz) * psi;
+    }
+}
+
+__device__ __forceinline__ double soft_kick_one(const WeidemanCoeffs &wc, double &x, double &px, double &y, double &py,
+                                                double &z, double &pz, const SoftMoments &m, double center, double kbb,
+                                                int drift, int lkick) {
+    const int d = lkick ? drift : OCTOPIC_DRIFT_UNSAFE_PARAXIAL_FROZEN_LONGITUDINAL;   // _soft_gaussian_drift
+    const double S = soft_forward_drift(d, x, px, y, py, z, pz, center);
+    const double xx = x - m.mx + m.mpx * S;
+    const double yy = y - m.my + m.mpy * S;
+    const double px0 = px, py0 = py, pz0 = pz;
+    const double a = m.a0 - 2 * S * m.bxx + S * S * m.qxx;
+    const double dd = m.d0 - 2 * S * m.bypy + S * S * m.qyy;
+    const double au = 2 * (m.bxx - S * m.qxx);
+    const double du = 2 * (m.bypy - S * m.qyy);
+    double density = 0.0;
+    if (!(a <= 0 || dd <= 0)) {
+        const double sigx = sqrt(a), sigy = sqrt(dd);
+        double kx, ky, hx, hy;
+        beambeam_response(wc, kbb, sigx, sigy, xx, yy, kx, ky, hx, hy);
+        const double expterm = exp(-0.5 * (xx * xx / (sigx * sigx) + yy * yy / (sigy * sigy)));
+        px += kx * kbb;
+        py += ky * kbb;
+        pz += 0.25 * (hx * au + hy * du);
+        density = expterm / kTwoPi / sigx / sigy;
+    }
+    if (lkick) pz += 0.5 * ((px - px0) * m.mpx + (py - py0) * m.mpy);
+    else pz = pz0;
+    soft_reverse_drift(d, x, px, y, py, z, pz, center);
+    return density * kTwoPi;
+}
+
+__device__ __forceinline__ bool finite_all(const SoftMoments &m) {   // _gaussian_moments_finite (gaussian.jl:126-130)
+    return isfinite(m.mx) && isfinite(m.mpx) && isfinite(m.my) && isfinite(m.mpy) && isfinite(sqrt(m.a0)) &&
+           isfinite(sqrt(m.d0)) && isfinite(sqrt(m.qxx)) && isfinite(sqrt(m.qyy)) && isfinite(m.bxx) && isfinite(m.bypy);
+}
+
+// Block (b, d): interaction d = 2k + dir - 1 of the batch, pair k.  Its source slice (beam 1's slice i for
+// dir 1, beam 2's slice j for dir 2) becomes its Gaussian from the moment partials (_slice_transverse_moments,
+// slicing.jl:670-782: shifted sums folded over the chunks in order, the shifted second moments as fused
+// multiply-adds, the min_sigma floor, ignore_centroid), and the block's share of the field slice's
+// particles goes through the map in place.  The sampled direction's block sums of density x 2 pi (fixed
+// tree) go to lum_block[p][b] for an ordered fold at the end of the collide.
+__global__ __launch_bounds__(256) void k_soft_kick(WeidemanCoeffs wc, SoftKickArgs a) {
+    __shared__ double sm[10];
+    __shared__ SoftMoments mom;
+    __shared__ int ok;
+    __shared__ double red[256];
+    const int d = blockIdx.y, k = d >> 1, side = d & 1;
+    const BatchPair pr = a.pairs[k];
+    const SliceTable *ts = side == 0 ? a.tab1 : a.tab2, *tf = side == 0 ? a.tab2 : a.tab1;
+    const int si = side == 0 ? pr.i : pr.j, fi = side == 0 ? pr.j : pr.i;
+    const int64_t ns = ts->count[si], nf = tf->count[fi], foff = tf->offset[fi];
+    if (threadIdx.x < 10) {
+        double acc = 0.0;
+        for (int c = 0; c < a.mom_chunks; ++c) acc += a.mom_part[((int64_t)d * a.mom_chunks + c) * 10 + threadIdx.x];
+        sm[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        SoftMoments m;
+        const double floor2 = a.min_sigma * a.min_sigma;
+        if (ns == 0) {
+            m = SoftMoments{0, 0, 0, 0, floor2, floor2, 0, 0, 0, 0};
+        } else {
+            const double *sh = a.shift + (int64_t)d * 4;
+            const double x0 = sh[0], px0 = sh[1], y0 = sh[2], py0 = sh[3];
+            const double invn = 1.0 / (double)ns;
+            const double dmx = sm[0] * invn, dmpx = sm[1] * invn, dmy = sm[2] * invn, dmpy = sm[3] * invn;
+            m.mx = x0 + dmx; m.mpx = px0 + dmpx; m.my = y0 + dmy; m.mpy = py0 + dmpy;
+            m.a0 = fmax(fma(-dmx, dmx, sm[4] * invn), floor2);
+            m.d0 = fmax(fma(-dmy, dmy, sm[6] * invn), floor2);
+            m.qxx = fmax(fma(-dmpx, dmpx, sm[5] * invn), 0.0);
+            m.qyy = fmax(fma(-dmpy, dmpy, sm[7] * invn), 0.0);
+            m.bxx = fma(-dmx, dmpx, sm[8] * invn);
+            m.bypy = fma(-dmy, dmpy, sm[9] * invn);
+            if (side == 0 ? a.ignore1 : a.ignore2) { m.mx = 0.0; m.mpx = 0.0; m.my = 0.0; m.mpy = 0.0; }
+        }
+        mom = m;
+        ok = finite_all(m) ? 1 : 0;
+        if (!ok && atomicCAS(&a.err->code, 0, OCTOPIC_ERR_NONFINITE) == 0) {
+            a.err->p = pr.p; a.err->dir = side + 1; a.err->stage = 6; a.err->a = 0.0; a.err->b = 0.0;
+        }
+    }
+    __syncthreads();
+    // dir 1 (side 0) kicks beam 2 with beam 1's slice: luminosity lum1, sampled when gaussian_when_luminosity = 1
+    const bool want = a.compute_lum && (side == 0 ? a.sample1 : !a.sample1);
+    const double w = ts->weight[si];
+    const double kbb = w * (side == 0 ? a.kbb2 : a.kbb1);
+    double acc = 0.0;
+    if (ok && *((volatile const int32_t *)&a.err->code) == 0) {
+        const SoftMoments m = mom;
+        const double center = ts->center[si];
+        const int fb = side == 0 ? 1 : 0;
+        double *X = a.st.c[fb][0] + foff, *PX = a.st.c[fb][1] + foff, *Y = a.st.c[fb][2] + foff,
+               *PY = a.st.c[fb][3] + foff, *Z = a.st.c[fb][4] + foff, *PZ = a.st.c[fb][5] + foff;
+        for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += (int64_t)gridDim.x * blockDim.x) {
+            double x = X[t], px = PX[t], y = Y[t], py = PY[t], z = Z[t], pz = PZ[t];
+            const double dens = soft_kick_one(wc, x, px, y, py, z, pz, m, center, kbb, a.drift, a.lkick);
+            if (want) acc += dens;
+            X[t] = x; PX[t] = px; Y[t] = y; PY[t] = py; Z[t] = z; PZ[t] = pz;
+        }
+    }
+    if (!want) return;
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.lum_block[(int64_t)pr.p * kSoftBlocks + blockIdx.x] = red[0];
+        if (blockIdx.x == 0) a.lum_scale[pr.p] = w * (side == 0 ? a.klum2 : a.klum1);
+    }
+}
+
+// lum_parts[p] = (sum of the pair's block partials, in block order) / 2 pi * klum_slice
+// (_slice_slice_gaussian_kick!, gaussian.jl:97-123); pairs whose kicks did not run read zeros.
+__global__ __launch_bounds__(256) void k_soft_lum_fold(const double *lum_block, const double *lum_scale, int npairs,
+                                                       double *lum_parts) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    double s = 0.0;
+    for (int b = 0; b < kSoftBlocks; ++b) s += lum_block[(int64_t)p * kSoftBlocks + b];
+    lum_parts[p] = s / kTwoPi * lum_scale[p];
+}
+
 __global__ __launch_bounds__(256) void k_be_kick(WeidemanCoeffs wc, const double *X, const double *Y, int64_t n,
                                                  double sigx, double sigy, double *KX, double *KY) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -503,6 +689,19 @@ hipError_t launch_gpic_addback(const GpicAddbackArgs &a, int64_t max_fld, hipStr
     if (a.ninters <= 0 || max_fld <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gpic_addback, dim3((unsigned)((max_fld + 255) / 256), a.ninters), dim3(256), 0, s,
                        weideman_coeffs(), a);
+    return hipGetLastError();
+}
+
+hipError_t launch_soft_kick(const SoftKickArgs &a, hipStream_t s) {
+    if (a.npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_soft_kick, dim3(kSoftBlocks, 2 * a.npairs), dim3(256), 0, s, weideman_coeffs(), a);
+    return hipGetLastError();
+}
+
+hipError_t launch_soft_lum_fold(const double *lum_block, const double *lum_scale, int npairs, double *lum_parts,
+                                hipStream_t s) {
+    if (npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_soft_lum_fold, dim3((npairs + 255) / 256), dim3(256), 0, s, lum_block, lum_scale, npairs, lum_parts);
     return hipGetLastError();
 }
 

@@ -1137,6 +1137,10 @@ __global__ __launch_bounds__(256) void k_gpic_moments(GpicMomentArgs a) {
         const double *X = a.st.c[side][0] + off, *PX = a.st.c[side][1] + off, *Y = a.st.c[side][2] + off,
                      *PY = a.st.c[side][3] + off;
         const double x0 = X[0], px0 = PX[0], y0 = Y[0], py0 = PY[0];
+        if (a.shift && c == 0 && threadIdx.x == 0) {
+            double *sh = a.shift + (int64_t)d * 4;
+            sh[0] = x0; sh[1] = px0; sh[2] = y0; sh[3] = py0;
+        }
         const int64_t b = n * c / a.chunks, e = n * (c + 1) / a.chunks;
         for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
             const double dx = X[i] - x0, dpx = PX[i] - px0, dy = Y[i] - y0, dpy = PY[i] - py0;

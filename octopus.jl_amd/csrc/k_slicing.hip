@@ -372,7 +372,7 @@ __global__ __launch_bounds__(256) void k_centroid_fold(const double *lanes, Slic
 
 // Scatter the slice-contiguous states back into the beam (x, px, y, py, pz; z is never modified).
 __global__ __launch_bounds__(256) void k_store(double *const *beam, int64_t n, double *const *st, const int64_t *perm,
-                                               const ErrorInfo *err) {
+                                               const ErrorInfo *err, int with_z) {
     if (err->code != 0) return;
     for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < n; pos += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = perm[pos];
@@ -380,6 +380,7 @@ __global__ __launch_bounds__(256) void k_store(double *const *beam, int64_t n, d
         beam[1][i] = st[1][pos];
         beam[2][i] = st[2][pos];
         beam[3][i] = st[3][pos];
+        if (with_z) beam[4][i] = st[4][pos];
         beam[5][i] = st[5][pos];
     }
 }
@@ -726,10 +727,10 @@ hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s) {
 int64_t slicing_hist_entries(int64_t n, int ns) { return ((n + kSliceBlock - 1) / kSliceBlock) * (int64_t)ns; }
 
 hipError_t launch_store(double *const *beam, int64_t n, double *const *st, const int64_t *perm, const ErrorInfo *err,
-                        hipStream_t s) {
+                        hipStream_t s, int with_z) {
     int g = (int)((n + 255) / 256);
     if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_store, dim3(g), dim3(256), 0, s, beam, n, st, perm, err);
+    hipLaunchKernelGGL(k_store, dim3(g), dim3(256), 0, s, beam, n, st, perm, err, with_z);
     return hipGetLastError();
 }
 

@@ -56,8 +56,9 @@ struct TileSortArgs {
 };
 hipError_t launch_tile_sort(const TileSortArgs &a, int G, hipStream_t s);
 int64_t slicing_hist_entries(int64_t n, int ns);
+// with_z: z written back too (solvers whose drifts move it)
 hipError_t launch_store(double *const *beam, int64_t n, double *const *st, const int64_t *perm, const ErrorInfo *err,
-                        hipStream_t s);
+                        hipStream_t s, int with_z = 0);
 hipError_t launch_store_gather(double *const *beam, int64_t n, double *const *st, const int32_t *inv, const ErrorInfo *err,
                                hipStream_t s);
 
@@ -170,6 +171,7 @@ struct GpicMomentArgs {
     int chunks;                  // chunks per source slice
     double *mom_part;            // [2*npairs][chunks][10], interaction d = 2k + dir - 1
     const ErrorInfo *err;
+    double *shift = nullptr;     // when set: [2*npairs][4], the sums' shift (the slice's first x, px, y, py)
 };
 hipError_t launch_gpic_moments(const GpicMomentArgs &a, hipStream_t s);
 struct GpicResidualArgs {
@@ -194,6 +196,31 @@ struct GpicAddbackArgs {
 };
 hipError_t launch_gpic_addback(const GpicAddbackArgs &a, int64_t max_fld, hipStream_t s);
 hipError_t launch_faddeeva(const double *zr, const double *zi, int64_t n, double *wr, double *wi, hipStream_t s);
+
+// The soft-Gaussian solver (GaussianPoissonSolver, gaussian.jl): per batch, after launch_gpic_moments over
+// the batch's source slices, every interaction's field slice through the map; kSoftBlocks blocks per
+// interaction, whose luminosity partials a collide folds per pair at its end.
+constexpr int kSoftBlocks = 128;
+struct SoftKickArgs {
+    const BatchPair *pairs;
+    int npairs;
+    const SliceTable *tab1, *tab2;
+    States st;
+    int mom_chunks;              // chunks of the moment partials
+    const double *mom_part;      // [2*npairs][mom_chunks][10], interaction d = 2k + dir - 1
+    const double *shift;         // [2*npairs][4], the shift of those sums (read here: a launch kicks in place
+                                 // the slices other interactions of it take as sources)
+    double kbb1, kbb2;           // kicks on beam 1 and beam 2 (times the source slice's weight)
+    double klum1, klum2;         // luminosity scales sampling beam 1 and beam 2 (times the weight)
+    double min_sigma;
+    int ignore1, ignore2;        // ignore_centroid of each beam's slices
+    int drift, lkick, sample1, compute_lum;
+    double *lum_block;           // [npairs of the collide][kSoftBlocks]
+    double *lum_scale;           // [npairs of the collide]
+    ErrorInfo *err;
+};
+hipError_t launch_soft_kick(const SoftKickArgs &a, hipStream_t s);
+hipError_t launch_soft_lum_fold(const double *lum_block, const double *lum_scale, int npairs, double *lum_parts, hipStream_t s);
 
 struct KickArgs {
     const Inter *inters;

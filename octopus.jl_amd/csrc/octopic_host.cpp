@@ -214,6 +214,10 @@ struct octopic_handle {
     std::string err;
     int nx = 0, ny = 0, lnx = 0, lny = 0, lum_dep = 0, npl = 2, mode = 0;
     bool gauss = false;         // GaussianPICPoissonSolver: Gaussian-subtracted planes with the exact field added back
+    bool soft = false;          // GaussianPoissonSolver: the sliced soft-Gaussian kick, no mesh
+    DBuf<double> soft_lum_block;   // [pairs][kSoftBlocks] luminosity partials
+    DBuf<double> soft_lum_scale;   // [pairs] the sampled direction's klum_slice
+    DBuf<double> soft_shift;       // [2 * widest][4] the moment sums' shifts
     DBuf<double> gmom;          // source moment partials of a batch's interactions [2 * widest][chunks][10]
     DBuf<double> gab;           // per-particle add-back of both beams [N1 + N2][4]
     FFTPlanHost px, py;
@@ -429,8 +433,24 @@ void validate(const octopic_options &o) {
                                       "(padded size 2*grid <= 1024)");
     if ((int64_t)(lnx + 1) * (lny + 1) > (int64_t)1 << 20)
         fail(OCTOPIC_ERR_UNSUPPORTED, "luminosity meshes above 2^20 nodes are not supported");
-    if (o.solver != OCTOPIC_SOLVER_PIC && o.solver != OCTOPIC_SOLVER_GAUSSIAN_PIC)
-        fail(OCTOPIC_ERR_INVALID_OPTION, "solver must be OCTOPIC_SOLVER_PIC or OCTOPIC_SOLVER_GAUSSIAN_PIC");
+    if (o.solver != OCTOPIC_SOLVER_PIC && o.solver != OCTOPIC_SOLVER_GAUSSIAN_PIC && o.solver != OCTOPIC_SOLVER_SOFT_GAUSSIAN)
+        fail(OCTOPIC_ERR_INVALID_OPTION,
+             "solver must be OCTOPIC_SOLVER_PIC, OCTOPIC_SOLVER_GAUSSIAN_PIC or OCTOPIC_SOLVER_SOFT_GAUSSIAN");
+    if (o.solver == OCTOPIC_SOLVER_SOFT_GAUSSIAN) {
+        // GaussianPoissonSolver constructor (interface.jl:976-1000); the mesh fields above keep their defaults
+        if (!(std::isfinite(o.min_sigma) && o.min_sigma >= 0.0))
+            fail(OCTOPIC_ERR_INVALID_OPTION, "min_sigma must be finite and nonnegative");
+        if (o.gaussian_when_luminosity != 1 && o.gaussian_when_luminosity != 2)
+            fail(OCTOPIC_ERR_INVALID_OPTION, "gaussian_when_luminosity must be 1 or 2");
+        if (o.virtual_drift < OCTOPIC_DRIFT_HIRATA || o.virtual_drift > OCTOPIC_DRIFT_UNSAFE_CHROMATIC_FROZEN_ENERGY)
+            fail(OCTOPIC_ERR_INVALID_OPTION, "unknown virtual_drift; expected :hirata, :chromatic, :exact or an unsafe drift");
+        if (o.include_sigma_xy)
+            fail(OCTOPIC_ERR_UNSUPPORTED, "include_sigma_xy = true (the coupled, rotated soft-Gaussian kick) is not "
+                                          "implemented by the HIP GaussianPoissonSolver; use include_sigma_xy = false.");
+        if (o.precision != OCTOPIC_PRECISION_FP64)
+            fail(OCTOPIC_ERR_UNSUPPORTED, "the HIP GaussianPoissonSolver computes in FP64 (it has no mesh to hold in FP32); "
+                                          "use precision = FP64 (Float32 beams are accepted).");
+    }
     if (o.solver == OCTOPIC_SOLVER_GAUSSIAN_PIC) {
         // GaussianPICPoissonSolver constructor (gaussian_pic.jl:94-110) and the options it rejects at collide
         // time (_require_linear_slice_interpolation, pic_cpu.jl:788-809)
@@ -518,6 +538,12 @@ int octopic_options_default(octopic_options *o) {
     o->neutralize = 1;
     o->margin_sigma = 5.0;
     o->coupling_tol = INFINITY;
+    o->min_sigma = 0.0;
+    o->virtual_drift = OCTOPIC_DRIFT_HIRATA;
+    o->gaussian_when_luminosity = 2;
+    o->ignore_centroid1 = 0; o->ignore_centroid2 = 0;
+    o->include_sigma_xy = 0;
+    o->reserved = 0;
     return OCTOPIC_OK;
 }
 
@@ -536,7 +562,8 @@ int octopic_set_denoiser(octopic_handle *h, octopic_denoiser *d, double monitor_
         return OCTOPIC_OK;
     }
     if (h->mode == 2) return set_err(h, OCTOPIC_ERR_UNSUPPORTED, "the residual denoiser runs with interaction_grid = :slice_pair");
-    if (h->gauss) return set_err(h, OCTOPIC_ERR_UNSUPPORTED, "the residual denoiser runs with PICPoissonSolver, not the Gaussian-subtracted solver");
+    if (h->gauss || h->soft)
+        return set_err(h, OCTOPIC_ERR_UNSUPPORTED, "the residual denoiser runs with PICPoissonSolver, not the Gaussian solvers");
     int dh = 0, dw = 0, dmb = 0;
     octopic::denoiser_shape(d, &dh, &dw, &dmb);
     if (dh != h->ny || dw != h->nx)
@@ -576,6 +603,7 @@ void octopic_destroy(octopic_handle *h) {
     h->spec32.release(); h->qh32.release(); h->phi32.release(); h->table32.release();
     h->dn_mom.release(); h->dn_meta.release(); h->dn_prof.release(); h->dn_in.release();
     h->gmom.release(); h->gab.release();
+    h->soft_lum_block.release(); h->soft_lum_scale.release(); h->soft_shift.release();
     delete h;
 }
 
@@ -602,6 +630,7 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         h->lum_dep = opt->lum_deposit >= 0 ? opt->lum_deposit : opt->deposit;
         h->mode = opt->interaction_grid == OCTOPIC_GRID_NODE ? 2 : (opt->slice_interpolation == OCTOPIC_INTERP_QUADRATIC ? 1 : 0);
         h->gauss = opt->solver == OCTOPIC_SOLVER_GAUSSIAN_PIC;
+        h->soft = opt->solver == OCTOPIC_SOLVER_SOFT_GAUSSIAN;
         h->npl = h->mode == 0 ? 2 : 3;
         if (h->mode == 2 && !opt->longitudinal_kick) h->npl = 3;   // Z plane slot kept; unused without the longitudinal kick
         h->px.build(2 * h->nx);
@@ -638,7 +667,7 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         const int ns1 = opt->slicing1.nslices, ns2 = opt->slicing2.nslices;
         h->clanes.ensure((size_t)std::max(ns1, ns2) * kFoldLanes);
         h->slot_stride = (int64_t)(h->nx + 1) * 2 * h->ny;
-        if (opt->green_cache == OCTOPIC_CACHE_SLICE_PAIR && h->mode != 2) {
+        if (opt->green_cache == OCTOPIC_CACHE_SLICE_PAIR && h->mode != 2 && !h->soft) {
             h->nkeys = 2 * ns1 * ns2;
             h->cache.ensure(h->nkeys);
             HIPCHECK(hipMemset(h->cache.p, 0, sizeof(CacheGeo) * h->nkeys));
@@ -833,6 +862,7 @@ const char *stage_name(int stage) {
         case 2: return "luminosity grid";
         case 3: return "interaction grid";
         case 4: return "node-mesh build";
+        case 6: return "soft-Gaussian slice moments";
         default: return "slicing";
     }
 }
@@ -940,6 +970,140 @@ extern "C" int octopic_collide_many(octopic_handle *const *handles, octopic_beam
 
 namespace {
 
+// The collided coordinates back to the caller's arrays: Float32 beams rounded, host beams copied (z only
+// where the solver moves it: the soft-Gaussian solver's exact and chromatic drifts).
+void write_back(octopic_handle *h, octopic_beam *const B[2], double *const bp[2][6], const int64_t n[2], bool with_z) {
+    hipStream_t s = h->stream;
+    for (int b = 0; b < 2; ++b) {
+        double *hp2[6] = {B[b]->x, B[b]->px, B[b]->y, B[b]->py, B[b]->z, B[b]->pz};
+        if (B[b]->dtype == OCTOPIC_DTYPE_F32) {
+            for (int c = 0; c < 6; ++c) {
+                if (c == 4 && !with_z) continue;
+                if (B[b]->on_device) {
+                    HIPCHECK(launch_narrow(bp[b][c], reinterpret_cast<float *>(hp2[c]), n[b], s));
+                } else {
+                    HIPCHECK(launch_narrow(bp[b][c], h->beam_f32[b][c].p, n[b], s));
+                    HIPCHECK(hipMemcpyAsync(hp2[c], h->beam_f32[b][c].p, sizeof(float) * n[b], hipMemcpyDeviceToHost, s));
+                }
+            }
+            continue;
+        }
+        if (B[b]->on_device) continue;
+        for (int c = 0; c < 6; ++c)
+            if (c != 4 || with_z) HIPCHECK(hipMemcpyAsync(hp2[c], bp[b][c], sizeof(double) * n[b], hipMemcpyDeviceToHost, s));
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
+// collide!(::GaussianPoissonSolver, ...) (gaussian.jl:7-57) after the slicing: per conflict-free batch of
+// the collision order, the source moments of both slices of every pair, then every field slice through
+// the soft-Gaussian map in place (the slice order, no mesh).  Every pair runs, empty slices included; the
+// sampled direction's per-pair luminosity is folded in collision order.
+void soft_collide(octopic_handle *h, octopic_beam *const B[2], double *const bp[2][6], const int64_t n[2], int64_t turn,
+                  int compute_lum, double *lum_out) {
+    hipStream_t s = h->stream;
+    const octopic_options &o = h->opt;
+    const SliceTable &T1 = h->h_tab[0], &T2 = h->h_tab[1];
+    const int ns1 = o.slicing1.nslices, ns2 = o.slicing2.nslices;
+    const double kbb1 = kbb_of(o.kbb1, B[0], B[1], true, 1);   // gaussian.jl:12-13: not divided by N
+    const double kbb2 = kbb_of(o.kbb2, B[1], B[0], false, 1);
+    double klum1 = o.luminosity_scale, klum2 = o.luminosity_scale;   // slicing.jl:93-102
+    if (std::isnan(o.luminosity_scale)) {
+        klum1 = B[0]->npart * B[1]->npart / (double)n[0];
+        klum2 = B[0]->npart * B[1]->npart / (double)n[1];
+    }
+    auto order = collision_order(T1.center, ns1, T2.center, ns2);
+    auto batches = pair_batches(order, ns1, ns2);
+    const int npairs = (int)order.size();
+    std::vector<BatchPair> hp;
+    std::vector<int> boff, bcnt;
+    int widest = 0;
+    for (auto &bt : batches) {
+        boff.push_back((int)hp.size());
+        for (int pos : bt) hp.push_back(BatchPair{pos, order[pos].i, order[pos].j, 0});
+        bcnt.push_back((int)bt.size());
+        widest = std::max(widest, (int)bt.size());
+    }
+    const int W = std::max(widest, 1);
+    h->pairs.ensure(std::max<size_t>(hp.size(), 1));
+    if (!hp.empty())
+        HIPCHECK(hipMemcpyAsync(h->pairs.p, hp.data(), sizeof(BatchPair) * hp.size(), hipMemcpyHostToDevice, s));
+    h->gmom.ensure((size_t)2 * W * kDnMomentChunks * 10);
+    h->soft_shift.ensure((size_t)2 * W * 4);
+    h->soft_lum_block.ensure((size_t)std::max(npairs, 1) * kSoftBlocks);
+    h->soft_lum_scale.ensure(std::max(npairs, 1));
+    h->lum_parts.ensure(std::max(npairs, 1));
+    States st{};
+    for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 6; ++c) st.c[b][c] = h->st[b][c].p;
+    const SliceTable *dt1 = h->tab.p, *dt2 = h->tab.p + 1;
+    for (size_t bi = 0; bi < batches.size(); ++bi) {
+        const int Bn = bcnt[bi];
+        if (Bn == 0) continue;
+        const BatchPair *dp = h->pairs.p + boff[bi];
+        {
+            Timed t(h, F_GEOMETRY);
+            GpicMomentArgs a{dp, Bn, dt1, dt2, st, kDnMomentChunks, h->gmom.p, h->errinfo.p, h->soft_shift.p};
+            HIPCHECK(launch_gpic_moments(a, s));
+        }
+        {
+            Timed t(h, F_KICK);
+            SoftKickArgs a{};
+            a.pairs = dp; a.npairs = Bn; a.tab1 = dt1; a.tab2 = dt2; a.st = st;
+            a.mom_chunks = kDnMomentChunks; a.mom_part = h->gmom.p; a.shift = h->soft_shift.p;
+            a.kbb1 = kbb1; a.kbb2 = kbb2; a.klum1 = klum1; a.klum2 = klum2; a.min_sigma = o.min_sigma;
+            a.ignore1 = o.ignore_centroid1 ? 1 : 0; a.ignore2 = o.ignore_centroid2 ? 1 : 0;
+            a.drift = o.virtual_drift; a.lkick = o.longitudinal_kick ? 1 : 0;
+            a.sample1 = o.gaussian_when_luminosity == 1 ? 1 : 0; a.compute_lum = compute_lum ? 1 : 0;
+            a.lum_block = h->soft_lum_block.p; a.lum_scale = h->soft_lum_scale.p; a.err = h->errinfo.p;
+            HIPCHECK(launch_soft_kick(a, s));
+        }
+    }
+    if (compute_lum) HIPCHECK(launch_soft_lum_fold(h->soft_lum_block.p, h->soft_lum_scale.p, npairs, h->lum_parts.p, s));
+    {
+        Timed t(h, F_STORE);
+        for (int b = 0; b < 2; ++b)
+            HIPCHECK(launch_store(h->ptrs.p + b * 6, n[b], h->ptrs.p + 12 + b * 6, h->perm[b].p, h->errinfo.p, s, 1));
+    }
+    HIPCHECK(hipMemcpyAsync(h->h_err, h->errinfo.p, sizeof(ErrorInfo), hipMemcpyDeviceToHost, s));
+    h->plum.assign(npairs, 0.0);
+    if (npairs && compute_lum)
+        HIPCHECK(hipMemcpyAsync(h->plum.data(), h->lum_parts.p, sizeof(double) * npairs, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (h->h_err->code) raise_device_error(h, *h->h_err, turn);
+    write_back(h, B, bp, n, true);
+    h->pran.assign(npairs, 1);
+    h->recs.clear();
+    h->pi.resize(npairs); h->pj.resize(npairs);
+    for (int p = 0; p < npairs; ++p) { h->pi[p] = order[p].i; h->pj[p] = order[p].j; }
+    double lum = NAN;
+    if (compute_lum) {   // the fold in collision order (gaussian.jl:50-56)
+        lum = 0.0;
+        for (int p = 0; p < npairs; ++p) lum += h->plum[p];
+    }
+    if (lum_out) *lum_out = lum;
+    octopic_stats &S = h->stats;
+    S.green_builds = 0;
+    h->last_builds = 0;
+    S.dropped = 0;
+    S.denoised_planes = 0;
+    S.denoise_fallbacks = 0;
+    S.deposit_all_planes = 0;
+    S.pairs = npairs;
+    S.batches = (int64_t)batches.size();
+    S.widest_batch = widest;
+    S.planes = 0;
+    S.collides += 1;
+    if (h->timing) {
+        for (auto &e : h->ev) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e.second.first, e.second.second);
+            h->fam_ms[e.first] += ms;
+            h->fam_n[e.first] += 1;
+        }
+    }
+}
+
 void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t turn, int compute_lum, double *lum_out) {
     check_beam(B1, 1);
     check_beam(B2, 2);
@@ -947,13 +1111,14 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     const octopic_options &o = h->opt;
     const int64_t n[2] = {B1->n, B2->n};
     octopic_beam *B[2] = {B1, B2};
+    const int tiles = h->soft ? 0 : h->tiles;   // the soft-Gaussian solver keeps the slice order
     h->ev.clear();
     h->ev_next = 0;
     // ---- beams on the device
     double *bp[2][6];
     for (int b = 0; b < 2; ++b) {
         double *hp[6] = {B[b]->x, B[b]->px, B[b]->y, B[b]->py, B[b]->z, B[b]->pz};
-        if (h->tiles > 0) {
+        if (tiles > 0) {
             for (int c = 0; c < 6; ++c) h->srt[b][c].ensure(n[b]);
             h->perm_srt[b].ensure(n[b]);
             h->inv_srt[b].ensure(n[b]);
@@ -981,7 +1146,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         }
     }
     upload_ptrs(h, bp);
-    if (h->tiles > 0) HIPCHECK(hipMemsetAsync(h->box.p, 0, 8 * sizeof(unsigned long long), s));
+    if (tiles > 0) HIPCHECK(hipMemsetAsync(h->box.p, 0, 8 * sizeof(unsigned long long), s));
     // ---- slicing
     {
         Timed t(h, F_SLICING);
@@ -990,7 +1155,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             run_slicing(h, b, *sl[b], n[b]);
             SlicingArgs a{};
             a.z = bp[b][4]; a.x = bp[b][0]; a.y = bp[b][2];
-            a.box = h->tiles > 0 ? h->box.p + 4 * b : nullptr;
+            a.box = tiles > 0 ? h->box.p + 4 * b : nullptr;
             a.n = n[b];
             a.method = sl[b]->method; a.ns = sl[b]->nslices; a.resolution = sl[b]->resolution;
             a.centroid = sl[b]->center_position == OCTOPIC_CENTER_CENTROID ? 1 : 0;
@@ -1022,11 +1187,15 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         if (!std::isfinite(T1.boundary[k])) raise_device_error(h, ErrorInfo{OCTOPIC_ERR_NONFINITE, -1, 0, 5, 0, 0}, turn);
     for (int k = 0; k <= ns2; ++k)
         if (!std::isfinite(T2.boundary[k])) raise_device_error(h, ErrorInfo{OCTOPIC_ERR_NONFINITE, -1, 0, 5, 0, 0}, turn);
+    if (h->soft) {
+        soft_collide(h, B, bp, n, turn, compute_lum, lum_out);
+        return;
+    }
 
     // ---- transverse tile order inside the slices (queued before the host builds the schedule)
-    if (h->tiles > 0) {
+    if (tiles > 0) {
         Timed t(h, F_TILE);
-        const int TT = h->tiles * h->tiles;
+        const int TT = tiles * tiles;
         for (int b = 0; b < 2; ++b) {
             const SliceTable &T = h->h_tab[b];
             int64_t mc = 0;
@@ -1036,7 +1205,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             G = std::max(1, std::min<int>(G, (int)((int64_t)(1 << 22) / ((int64_t)T.ns * TT))));
             h->thist.ensure((size_t)T.ns * TT * G);
             TileSortArgs a{};
-            a.tab = h->tab.p + b; a.ns = T.ns; a.T = h->tiles; a.box = h->box.p + 4 * b;
+            a.tab = h->tab.p + b; a.ns = T.ns; a.T = tiles; a.box = h->box.p + 4 * b;
             for (int c = 0; c < 6; ++c) { a.src[c] = h->st[b][c].p; a.dst[c] = h->srt[b][c].p; }
             a.perm_src = h->perm[b].p; a.perm_dst = h->perm_srt[b].p; a.inv = h->inv_srt[b].p;
             a.thist = h->thist.p;
@@ -1189,7 +1358,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
 
     States st{};
     for (int b = 0; b < 2; ++b)
-        for (int c = 0; c < 6; ++c) st.c[b][c] = h->tiles > 0 ? h->srt[b][c].p : h->st[b][c].p;
+        for (int c = 0; c < 6; ++c) st.c[b][c] = tiles > 0 ? h->srt[b][c].p : h->st[b][c].p;
     const SliceTable *dt1 = h->tab.p, *dt2 = h->tab.p + 1;
 
     int slot_node1 = 0, slot_node2 = 0;
@@ -1437,7 +1606,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     {
         Timed t(h, F_STORE);
         for (int b = 0; b < 2; ++b) {
-            if (h->tiles > 0)
+            if (tiles > 0)
                 HIPCHECK(launch_store_gather(h->ptrs.p + b * 6, n[b], h->ptrs.p + 24 + b * 6, h->inv_srt[b].p, h->errinfo.p, s));
             else
                 HIPCHECK(launch_store(h->ptrs.p + b * 6, n[b], h->ptrs.p + 12 + b * 6, h->perm[b].p, h->errinfo.p, s));
@@ -1472,25 +1641,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         h->deposit_trial_ms[trial] = ms;
         if (trial == 1) h->deposit_per_plane = h->deposit_trial_ms[1] < h->deposit_trial_ms[0];
     }
-    for (int b = 0; b < 2; ++b) {
-        double *hp2[6] = {B[b]->x, B[b]->px, B[b]->y, B[b]->py, B[b]->z, B[b]->pz};
-        if (B[b]->dtype == OCTOPIC_DTYPE_F32) {
-            for (int c = 0; c < 6; ++c) {
-                if (c == 4) continue;   // z is never modified
-                if (B[b]->on_device) {
-                    HIPCHECK(launch_narrow(bp[b][c], reinterpret_cast<float *>(hp2[c]), n[b], s));
-                } else {
-                    HIPCHECK(launch_narrow(bp[b][c], h->beam_f32[b][c].p, n[b], s));
-                    HIPCHECK(hipMemcpyAsync(hp2[c], h->beam_f32[b][c].p, sizeof(float) * n[b], hipMemcpyDeviceToHost, s));
-                }
-            }
-            continue;
-        }
-        if (B[b]->on_device) continue;
-        for (int c = 0; c < 6; ++c)
-            if (c != 4) HIPCHECK(hipMemcpyAsync(hp2[c], bp[b][c], sizeof(double) * n[b], hipMemcpyDeviceToHost, s));
-    }
-    HIPCHECK(hipStreamSynchronize(s));
+    write_back(h, B, bp, n, false);
     h->pi.resize(npairs); h->pj.resize(npairs);
     for (int p = 0; p < npairs; ++p) { h->pi[p] = order[p].i; h->pj[p] = order[p].j; }
     double lum = NAN;

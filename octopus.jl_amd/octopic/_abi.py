@@ -63,6 +63,8 @@ class Slicing(C.Structure):
 PRECISION = {"fp64": 0, "fp32": 1}
 SOLVER_PIC = 0
 SOLVER_GAUSSIAN_PIC = 1
+SOLVER_SOFT_GAUSSIAN = 2
+VIRTUAL_DRIFT = {"hirata": 0, "chromatic": 1, "exact": 2, "paraxial_frozen_longitudinal": 3, "chromatic_frozen_energy": 4}
 
 
 class Options(C.Structure):
@@ -98,6 +100,13 @@ class Options(C.Structure):
         ("neutralize", C.c_int32),
         ("margin_sigma", C.c_double),
         ("coupling_tol", C.c_double),
+        ("min_sigma", C.c_double),
+        ("virtual_drift", C.c_int32),
+        ("gaussian_when_luminosity", C.c_int32),
+        ("ignore_centroid1", C.c_int32),
+        ("ignore_centroid2", C.c_int32),
+        ("include_sigma_xy", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

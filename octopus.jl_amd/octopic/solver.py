@@ -252,6 +252,9 @@ class PICPoissonSolver:
         o.neutralize = 1
         o.margin_sigma = 5.0
         o.coupling_tol = math.inf
+        o.min_sigma = 0.0
+        o.virtual_drift = _abi.VIRTUAL_DRIFT["hirata"]
+        o.gaussian_when_luminosity = 2
         return o, keep
 
 
@@ -290,6 +293,96 @@ class GaussianPICPoissonSolver:
         o.neutralize = 1 if self.neutralize else 0
         o.margin_sigma = self.margin_sigma
         o.coupling_tol = self.coupling_tol
+        return o, keep
+
+
+@dataclass(frozen=True)
+class UnsafeVirtualDrift:
+    """UnsafeVirtualDrift(name) (strong_beam.jl:41-59): the non-symplectic diagnostic drifts
+    :paraxial_frozen_longitudinal and :chromatic_frozen_energy."""
+
+    name: str
+
+    def __post_init__(self):
+        if self.name not in ("paraxial_frozen_longitudinal", "chromatic_frozen_energy"):
+            raise ArgumentError(f"unknown unsafe virtual drift :{self.name}; expected one of "
+                                "(:chromatic_frozen_energy, :paraxial_frozen_longitudinal)")
+
+
+@dataclass(frozen=True)
+class GaussianPoissonSolver:
+    """GaussianPoissonSolver{Float64}(; kbb1=nothing, kbb2=nothing, luminosity_scale=nothing, slicing, slicing1,
+    slicing2, min_sigma=0, gaussian_when_luminosity=2, ignore_centroid1=false, ignore_centroid2=false,
+    longitudinal_kick=true, virtual_drift=:hirata, include_sigma_xy=false, batch_mode=:wavefront,
+    luminosity_schedule=nothing) (interface.jl:876-1010): the sliced soft-Gaussian solver (gaussian.jl).  Every
+    slice pair, in collision order, kicks each beam's slice with the exact Bassetti-Erskine field of the other
+    slice's Gaussian moments, transported to each particle's collision point, between the virtual drifts.
+    include_sigma_xy = true (the coupled, rotated kick) is accepted here as by the reference's constructor and
+    rejected by the backends with UnsupportedOptionError."""
+
+    kbb1: Optional[float] = None
+    kbb2: Optional[float] = None
+    luminosity_scale: Optional[float] = None
+    slicing: LongitudinalSlicing = LongitudinalSlicing()
+    slicing1: Optional[LongitudinalSlicing] = None
+    slicing2: Optional[LongitudinalSlicing] = None
+    min_sigma: float = 0.0
+    gaussian_when_luminosity: int = 2
+    ignore_centroid1: bool = False
+    ignore_centroid2: bool = False
+    longitudinal_kick: bool = True
+    virtual_drift: object = "hirata"
+    include_sigma_xy: bool = False
+    batch_mode: str = "wavefront"
+    luminosity_schedule: object = None
+
+    def __post_init__(self):
+        # The reference constructor's order of checks (interface.jl:976-1000).
+        ms = float(self.min_sigma)
+        if not (math.isfinite(ms) and ms >= 0.0):
+            raise ArgumentError(f"min_sigma must be finite and nonnegative; got {self.min_sigma}.")
+        object.__setattr__(self, "min_sigma", ms)
+        if self.gaussian_when_luminosity not in (1, 2):
+            raise ArgumentError("gaussian_when_luminosity must be 1 or 2")
+        if self.batch_mode not in _abi.BATCH_MODE:
+            raise ArgumentError("batch_mode must be :sequential or :wavefront")
+        vd = self.virtual_drift
+        if isinstance(vd, str) and vd not in ("hirata", "chromatic", "exact"):
+            raise ArgumentError(f"unknown virtual_drift :{vd}; expected one of (:hirata, :chromatic, :exact)")
+        if not isinstance(vd, (str, UnsafeVirtualDrift)):
+            raise ArgumentError(f"virtual_drift must be a supported Symbol or AbstractVirtualDrift, got {type(vd).__name__}")
+        for k in ("kbb1", "kbb2", "luminosity_scale"):
+            v = getattr(self, k)
+            if v is not None:
+                object.__setattr__(self, k, float(v))
+
+    @property
+    def resolved_slicing1(self) -> LongitudinalSlicing:
+        return self.slicing1 if self.slicing1 is not None else self.slicing
+
+    @property
+    def resolved_slicing2(self) -> LongitudinalSlicing:
+        return self.slicing2 if self.slicing2 is not None else self.slicing
+
+    def compute_luminosity(self, turn: Optional[int]) -> bool:
+        """The schedule gates only the reported value (the kicks are the same on every turn): unscheduled
+        turns return NaN (interface.jl:906-912)."""
+        if self.luminosity_schedule is None or turn is None:
+            return True
+        return bool(self.luminosity_schedule.should_run(int(turn)))
+
+    def to_options(self):
+        o, keep = PICPoissonSolver(kbb1=self.kbb1, kbb2=self.kbb2, luminosity_scale=self.luminosity_scale,
+                                   longitudinal_kick=self.longitudinal_kick, batch_mode=self.batch_mode,
+                                   slicing=self.slicing, slicing1=self.slicing1, slicing2=self.slicing2).to_options()
+        o.solver = _abi.SOLVER_SOFT_GAUSSIAN
+        o.min_sigma = self.min_sigma
+        vd = self.virtual_drift
+        o.virtual_drift = _abi.VIRTUAL_DRIFT[vd.name if isinstance(vd, UnsafeVirtualDrift) else vd]
+        o.gaussian_when_luminosity = int(self.gaussian_when_luminosity)
+        o.ignore_centroid1 = 1 if self.ignore_centroid1 else 0
+        o.ignore_centroid2 = 1 if self.ignore_centroid2 else 0
+        o.include_sigma_xy = 1 if self.include_sigma_xy else 0
         return o, keep
 
 

@@ -318,6 +318,9 @@ struct Opt {
     // GaussianPICPoissonSolver (gaussian_pic.jl:87-131)
     int solver = OCTOPIC_SOLVER_PIC, neutralize = 1;
     double margin = 5.0, coupling_tol = INFINITY;
+    // GaussianPoissonSolver (interface.jl:876-1010)
+    double min_sigma = 0.0;
+    int vdrift = OCTOPIC_DRIFT_HIRATA, gwl = 2, ignore1 = 0, ignore2 = 0, sigma_xy = 0;
 };
 
 Slicing to_slicing(const octopic_slicing &s) {
@@ -350,6 +353,8 @@ Opt to_opt(const octopic_options &o) {
     r.kbb1 = o.kbb1; r.kbb2 = o.kbb2; r.lum_scale = o.luminosity_scale;
     r.s1 = to_slicing(o.slicing1); r.s2 = to_slicing(o.slicing2);
     r.solver = o.solver; r.neutralize = o.neutralize ? 1 : 0; r.margin = o.margin_sigma; r.coupling_tol = o.coupling_tol;
+    r.min_sigma = o.min_sigma; r.vdrift = o.virtual_drift; r.gwl = o.gaussian_when_luminosity;
+    r.ignore1 = o.ignore_centroid1 ? 1 : 0; r.ignore2 = o.ignore_centroid2 ? 1 : 0; r.sigma_xy = o.include_sigma_xy ? 1 : 0;
     // Constructor checks (interface.jl:1410-1476) and _validate_pic_solver (pic_cpu.jl:517-576).
     if (!(r.min_ratio > 0.0 && r.min_ratio <= 1.0))
         throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "slice_pair_green_min_ratio must be in (0, 1]");
@@ -382,8 +387,17 @@ Opt to_opt(const octopic_options &o) {
         throw OracleError(OCTOPIC_ERR_UNSUPPORTED, "green_type = :lattice is not restated by the oracle");
     validate_slicing(r.s1);
     validate_slicing(r.s2);
-    if (r.solver != OCTOPIC_SOLVER_PIC && r.solver != OCTOPIC_SOLVER_GAUSSIAN_PIC)
-        throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "solver must be OCTOPIC_SOLVER_PIC or OCTOPIC_SOLVER_GAUSSIAN_PIC");
+    if (r.solver != OCTOPIC_SOLVER_PIC && r.solver != OCTOPIC_SOLVER_GAUSSIAN_PIC && r.solver != OCTOPIC_SOLVER_SOFT_GAUSSIAN)
+        throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "solver must be one of the OCTOPIC_SOLVER_* values");
+    if (r.solver == OCTOPIC_SOLVER_SOFT_GAUSSIAN) {   // GaussianPoissonSolver constructor (interface.jl:976-1000)
+        if (!(std::isfinite(r.min_sigma) && r.min_sigma >= 0.0))
+            throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "min_sigma must be finite and nonnegative");
+        if (r.gwl != 1 && r.gwl != 2) throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "gaussian_when_luminosity must be 1 or 2");
+        if (r.vdrift < OCTOPIC_DRIFT_HIRATA || r.vdrift > OCTOPIC_DRIFT_UNSAFE_CHROMATIC_FROZEN_ENERGY)
+            throw OracleError(OCTOPIC_ERR_INVALID_OPTION, "unknown virtual_drift");
+        if (r.sigma_xy)
+            throw OracleError(OCTOPIC_ERR_UNSUPPORTED, "include_sigma_xy = true (the coupled soft-Gaussian kick) is not restated");
+    }
     if (r.solver == OCTOPIC_SOLVER_GAUSSIAN_PIC) {
         // GaussianPICPoissonSolver constructor (gaussian_pic.jl:94-110) and _require_linear_slice_interpolation
         // (pic_cpu.jl:788-809)
@@ -1537,10 +1551,245 @@ double pic_kbb_phys(const Opt &, double given, const double p_self[5], const dou
     return p1[0] * p2[0] * p_self[3] * p_other[4] * p_self[1] / p_self[2];
 }
 
+// ---- GaussianPoissonSolver: the sliced soft-Gaussian solver (gaussian.jl), uncoupled
+constexpr int64_t REDUCTION_CHUNKS = 64;       // _REDUCTION_CHUNKS (interface.jl:664)
+constexpr int64_t PARALLEL_MOMENT_MIN = 4096;  // _STRONG_STRONG_PARALLEL_MOMENT_MIN (interface.jl:609)
+constexpr int64_t PARALLEL_KICK_MIN = 4096;    // _STRONG_STRONG_PARALLEL_KICK_MIN (interface.jl:610)
+constexpr double TWOPI_ = 6.283185307179586;
+
+// The Bassetti-Erskine kick with its Hessian diagonal (defined with the Faddeeva evaluation below):
+// out = (Kx, Ky, kbb Hxx, kbb Hyy).
+void be_response4(double kbb, double sigx, double sigy, double x, double y, double out[4]);
+
+// A slice's moments (_slice_transverse_moments, slicing.jl:670-782, uncoupled): the centroid (zeroed by
+// ignore_centroid) and the StrongTransverseMoments a0 = var x (floored), d0 = var y (floored), bxx =
+// cov(x, px), bypy = cov(y, py), qxx = var px, qyy = var py.
+struct SoftMom { double mx, mpx, my, mpy, a0, d0, bxx, bypy, qxx, qyy; };
+
+SoftMom soft_moments(double *const b[6], const std::vector<int64_t> &idx, bool ignore_centroid, double min_sigma) {
+    const int64_t n = (int64_t)idx.size();
+    const double floor2 = min_sigma * min_sigma;
+    if (n == 0) return SoftMom{0, 0, 0, 0, floor2, floor2, 0, 0, 0, 0};
+    const int64_t f = idx[0];
+    const double x0 = b[0][f], px0 = b[1][f], y0 = b[2][f], py0 = b[3][f];
+    double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // dx, dpx, dy, dpy, dx2, dpx2, dy2, dpy2, dxdpx, dydpy
+    auto add = [&](double *acc, int64_t i) {
+        const double dx = b[0][i] - x0, dpx = b[1][i] - px0, dy = b[2][i] - y0, dpy = b[3][i] - py0;
+        acc[0] += dx; acc[1] += dpx; acc[2] += dy; acc[3] += dpy;
+        acc[4] += dx * dx; acc[5] += dpx * dpx; acc[6] += dy * dy; acc[7] += dpy * dpy;
+        acc[8] += dx * dpx; acc[9] += dy * dpy;
+    };
+    if (n < PARALLEL_MOMENT_MIN) {
+        for (int64_t k = 0; k < n; ++k) add(s, idx[k]);
+    } else {   // the fixed chunk grid (_chunk_bounds, slicing.jl:517-521), chunks folded in order
+        for (int64_t c = 0; c < REDUCTION_CHUNKS; ++c) {
+            double part[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int64_t k = fld(c * n, REDUCTION_CHUNKS); k < fld((c + 1) * n, REDUCTION_CHUNKS); ++k) add(part, idx[k]);
+            for (int q = 0; q < 10; ++q) s[q] += part[q];
+        }
+    }
+    const double invn = 1.0 / (double)n;
+    const double dmx = s[0] * invn, dmpx = s[1] * invn, dmy = s[2] * invn, dmpy = s[3] * invn;
+    auto shifted = [&](double sum, double a, double bb) { return std::fma(-a, bb, sum * invn); };
+    SoftMom m;
+    m.mx = x0 + dmx; m.mpx = px0 + dmpx; m.my = y0 + dmy; m.mpy = py0 + dmpy;
+    m.a0 = std::max(shifted(s[4], dmx, dmx), floor2);
+    m.d0 = std::max(shifted(s[6], dmy, dmy), floor2);
+    m.qxx = std::max(shifted(s[5], dmpx, dmpx), 0.0);
+    m.qyy = std::max(shifted(s[7], dmpy, dmpy), 0.0);
+    m.bxx = shifted(s[8], dmx, dmpx);
+    m.bypy = shifted(s[9], dmy, dmpy);
+    if (ignore_centroid) { m.mx = 0.0; m.mpx = 0.0; m.my = 0.0; m.mpy = 0.0; }
+    return m;
+}
+
+bool soft_moments_finite(const SoftMom &m) {   // _gaussian_moments_finite (gaussian.jl:126-130)
+    return std::isfinite(m.mx) && std::isfinite(m.mpx) && std::isfinite(m.my) && std::isfinite(m.mpy) &&
+           std::isfinite(std::sqrt(m.a0)) && std::isfinite(std::sqrt(m.d0)) && std::isfinite(std::sqrt(m.qxx)) &&
+           std::isfinite(std::sqrt(m.qyy)) && std::isfinite(m.bxx) && std::isfinite(m.bypy);
+}
+
+// The virtual drifts (strong_beam.jl:497-617): forward returns S = (z - zo)/2 of the drift's own rule.
+double forward_drift(int d, double &x, double &px, double &y, double &py, double &z, double &pz, double zo) {
+    (void)px; (void)py;
+    if (d == OCTOPIC_DRIFT_EXACT) {
+        const double ps = std::sqrt((1 + pz) * (1 + pz) - px * px - py * py);
+        const double H = 1 + pz - ps;
+        const double rr = 0.5 * H / ps;
+        const double z2 = (z + rr * zo) / (1 + rr);
+        const double S = 0.5 * (z2 - zo);
+        z -= H / ps * S;
+        pz -= 0.5 * H;
+        x += px / ps * S;
+        y += py / ps * S;
+        return S;
+    }
+    const double S = 0.5 * (z - zo);
+    if (d == OCTOPIC_DRIFT_HIRATA) {
+        x += S * px; y += S * py;
+        pz -= 0.25 * (px * px + py * py);
+    } else if (d == OCTOPIC_DRIFT_UNSAFE_PARAXIAL_FROZEN_LONGITUDINAL) {
+        x += S * px; y += S * py;
+    } else {   // chromatic (and its frozen-energy form)
+        const double phi = std::sqrt(1 - 0.5 * (px * px + py * py) / ((1 + pz) * (1 + pz))) - 1;
+        x += S * px / (1 + pz);
+        y += S * py / (1 + pz);
+        z += 2 * S * phi;
+        if (d == OCTOPIC_DRIFT_CHROMATIC) pz += (1 + pz) * phi;
+    }
+    return S;
+}
+
+void reverse_drift(int d, double &x, double &px, double &y, double &py, double &z, double &pz, double zo) {
+    const double S = 0.5 * (z - zo);
+    if (d == OCTOPIC_DRIFT_EXACT) {
+        const double H0 = 0.5 * (px * px + py * py) / (1 + pz);
+        const double ps0 = 1 + pz - 0.5 * H0;
+        pz += 0.5 * H0;
+        x -= px / ps0 * S;
+        y -= py / ps0 * S;
+        z += H0 / ps0 * S;
+    } else if (d == OCTOPIC_DRIFT_HIRATA) {
+        x -= S * px; y -= S * py;
+        pz += 0.25 * (px * px + py * py);
+    } else if (d == OCTOPIC_DRIFT_UNSAFE_PARAXIAL_FROZEN_LONGITUDINAL) {
+        x -= S * px; y -= S * py;
+    } else {
+        const double psi = std::sqrt(1 + 0.5 * (px * px + py * py) / ((1 + pz) * (1 + pz))) - 1;
+        x -= S * px / (1 + pz);
+        y -= S * py / (1 + pz);
+        z += 2 * S * psi;
+        if (d == OCTOPIC_DRIFT_CHROMATIC) pz += (1 + pz) * psi;
+    }
+}
+
+// _apply_slice_kick_one! (gaussian.jl:152-188) with the uncoupled _cp_covariance_kick (strong_beam.jl:619-632)
+// and _transport_transverse_moments (:679-687); returns the luminosity density times 2 pi (0 if not wanted).
+double soft_kick_one(double &x, double &px, double &y, double &py, double &z, double &pz, const SoftMom &m, double center,
+                     double kbb, int drift, bool lkick, bool want_lum) {
+    const int d = lkick ? drift : OCTOPIC_DRIFT_UNSAFE_PARAXIAL_FROZEN_LONGITUDINAL;   // _soft_gaussian_drift
+    const double S = forward_drift(d, x, px, y, py, z, pz, center);
+    const double xx = x - m.mx + m.mpx * S;
+    const double yy = y - m.my + m.mpy * S;
+    const double px0 = px, py0 = py, pz0 = pz;
+    const double a = m.a0 - 2 * S * m.bxx + S * S * m.qxx;
+    const double dd = m.d0 - 2 * S * m.bypy + S * S * m.qyy;
+    const double au = 2 * (m.bxx - S * m.qxx);
+    const double du = 2 * (m.bypy - S * m.qyy);
+    double density = 0.0;
+    if (!(a <= 0 || dd <= 0)) {
+        const double sigx = std::sqrt(a), sigy = std::sqrt(dd);
+        double r[4];
+        be_response4(kbb, sigx, sigy, xx, yy, r);
+        const double expterm = std::exp(-0.5 * (xx * xx / (sigx * sigx) + yy * yy / (sigy * sigy)));
+        px += r[0] * kbb;
+        py += r[1] * kbb;
+        pz += 0.25 * (r[2] * au + r[3] * du);
+        density = expterm / TWOPI_ / sigx / sigy;
+    }
+    if (lkick) pz += 0.5 * ((px - px0) * m.mpx + (py - py0) * m.mpy);
+    else pz = pz0;
+    reverse_drift(d, x, px, y, py, z, pz, center);
+    return want_lum ? density * TWOPI_ : 0.0;
+}
+
+// _slice_slice_gaussian_kick! (gaussian.jl:59-124): kicks the slice `idx` of beam b; returns its luminosity.
+double soft_slice_kick(double *const b[6], const std::vector<int64_t> &idx, const SoftMom &m, double center, double kbb_slice,
+                       double klum_slice, int drift, bool lkick, bool want_lum) {
+    const int64_t n = (int64_t)idx.size();
+    if (n == 0) return 0.0;
+    auto one = [&](int64_t i) {
+        return soft_kick_one(b[0][i], b[1][i], b[2][i], b[3][i], b[4][i], b[5][i], m, center, kbb_slice, drift, lkick, want_lum);
+    };
+    double lum = 0.0;
+    if (n < PARALLEL_KICK_MIN) {
+        for (int64_t k = 0; k < n; ++k) lum += one(idx[k]);
+    } else {
+        for (int64_t c = 0; c < REDUCTION_CHUNKS; ++c) {
+            double part = 0.0;
+            for (int64_t k = fld(c * n, REDUCTION_CHUNKS); k < fld((c + 1) * n, REDUCTION_CHUNKS); ++k) part += one(idx[k]);
+            lum += part;
+        }
+    }
+    return lum / TWOPI_ * klum_slice;
+}
+
+// collide!(::GaussianPoissonSolver, ...) (gaussian.jl:7-57): pairs in collision order (the conflict-free
+// batches of the order run concurrently, each pair touching only its two slices); the per-pair luminosity
+// is folded in collision order.
+void soft_collide(State &S, double *b1[6], int64_t n1, const double p1[5], double *b2[6], int64_t n2, const double p2[5],
+                  int compute_lum, int nthreads) {
+    const Opt &o = S.C->o;
+    if (n1 <= 0 || n2 <= 0) throw OracleError(OCTOPIC_ERR_INVALID_ARGUMENT, "longitudinal slicing requires at least one particle");
+    Slices s1 = longitudinal_slices(b1[4], n1, o.s1);
+    Slices s2 = longitudinal_slices(b2[4], n2, o.s2);
+    const double kbb1 = pic_kbb_phys(o, o.kbb1, p1, p2, true), kbb2 = pic_kbb_phys(o, o.kbb2, p2, p1, false);
+    double klum1 = o.lum_scale, klum2 = o.lum_scale;   // _strong_strong_luminosity_scales (slicing.jl:93-102)
+    if (std::isnan(o.lum_scale)) { klum1 = p1[4] * p2[4] / (double)n1; klum2 = p1[4] * p2[4] / (double)n2; }
+    const bool sample1 = o.gwl == 1;
+    auto order = collision_order(s1.center, s2.center);
+    const int npairs = (int)order.size();
+    Vec lum_parts(npairs, 0.0);
+    std::vector<int32_t> ran(npairs, 1);
+    std::map<std::pair<int, int>, int> pair_pos;
+    for (int p = 0; p < npairs; ++p) pair_pos[{order[p].i, order[p].j}] = p;
+    auto pair_fn = [&](int p, int i, int j) {   // _cpu_gaussian_slice_pair! (gaussian.jl:24-57)
+        const SoftMom m1 = soft_moments(b1, s1.indices[i], o.ignore1, o.min_sigma);
+        const SoftMom m2 = soft_moments(b2, s2.indices[j], o.ignore2, o.min_sigma);
+        if (!soft_moments_finite(m1))
+            throw OracleError(OCTOPIC_ERR_NONFINITE, "non-finite source coordinate (beam 1, slice " + std::to_string(i + 1) + ")");
+        if (!soft_moments_finite(m2))
+            throw OracleError(OCTOPIC_ERR_NONFINITE, "non-finite source coordinate (beam 2, slice " + std::to_string(j + 1) + ")");
+        const double l2 = soft_slice_kick(b1, s1.indices[i], m2, s2.center[j], s2.weight[j] * kbb1, s2.weight[j] * klum1,
+                                          o.vdrift, o.lkick != 0, !sample1);
+        const double l1 = soft_slice_kick(b2, s2.indices[j], m1, s1.center[i], s1.weight[i] * kbb2, s1.weight[i] * klum2,
+                                          o.vdrift, o.lkick != 0, sample1);
+        lum_parts[p] = sample1 ? l1 : l2;
+    };
+    auto batches = collision_pair_batches(s1.center, s2.center);
+    S.last.batches = (int64_t)batches.size();
+    S.last.widest = 0;
+    for (auto &b : batches) S.last.widest = std::max<int64_t>(S.last.widest, (int64_t)b.size());
+    for (auto &batch : batches) {
+        const int nb = (int)batch.size();
+        std::vector<std::string> errs(nb);
+        std::vector<int> codes(nb, 0);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(std::max(1, std::min(nthreads, nb))) schedule(static, 1)
+#endif
+        for (int k = 0; k < nb; ++k) {
+            try {
+                pair_fn(pair_pos.at({batch[k].i, batch[k].j}), batch[k].i, batch[k].j);
+            } catch (OracleError &e) {
+                codes[k] = e.code; errs[k] = e.what();
+            }
+        }
+        for (int k = 0; k < nb; ++k)
+            if (codes[k]) throw OracleError(codes[k], errs[k]);
+    }
+    double lum = NAN;
+    if (compute_lum) {   // the reported value; the kicks are the same either way
+        lum = 0.0;
+        for (int p = 0; p < npairs; ++p) lum += lum_parts[p];
+    }
+    S.records.clear();
+    S.node_meshes.clear();
+    S.last.lum = lum;
+    S.last.pi.resize(npairs); S.last.pj.resize(npairs); S.last.ran = ran; S.last.plum = lum_parts;
+    for (int p = 0; p < npairs; ++p) { S.last.pi[p] = order[p].i; S.last.pj[p] = order[p].j; }
+    S.last.dropped = 0;
+    S.last1 = std::move(s1);
+    S.last2 = std::move(s2);
+}
+
 void collide(State &S, double *b1[6], int64_t n1, const double p1[5], double *b2[6], int64_t n2, const double p2[5],
              int compute_lum, int nthreads) {
     Context &C = *S.C;
     const Opt &o = C.o;
+    if (o.solver == OCTOPIC_SOLVER_SOFT_GAUSSIAN) {
+        soft_collide(S, b1, n1, p1, b2, n2, p2, compute_lum, nthreads);
+        return;
+    }
     if (n1 <= 0 || n2 <= 0) throw OracleError(OCTOPIC_ERR_INVALID_ARGUMENT, "longitudinal slicing requires at least one particle");
     Slices s1 = longitudinal_slices(b1[4], n1, o.s1);
     Slices s2 = longitudinal_slices(b2[4], n2, o.s2);
@@ -1982,6 +2231,11 @@ Resp beambeam_response(double kbb, double sigx, double sigy, double x, double y)
     if (sigx >= sigy) return response_principal(kbb, sigx, sigy, x, y);
     const Resp r = response_principal(kbb, sigy, sigx, y, x);
     return Resp{r.ky, r.kx, r.h2, r.h1};
+}
+
+void be_response4(double kbb, double sigx, double sigy, double x, double y, double out[4]) {
+    const Resp r = beambeam_response(kbb, sigx, sigy, x, y);
+    out[0] = r.kx; out[1] = r.ky; out[2] = r.h1; out[3] = r.h2;
 }
 
 // ---- GaussianPICPoissonSolver, the separable (uncoupled) subtraction (gaussian_pic.jl:374-491, 543-595,

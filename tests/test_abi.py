@@ -48,8 +48,9 @@ def test_struct_layouts_match_header():
 int main(void) {
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(octopic_slicing), sizeof(octopic_options), sizeof(octopic_beam),
          sizeof(octopic_stats), sizeof(octopic_interaction_record), offsetof(octopic_options, slicing2));
-  printf("%zu %zu %zu\n", offsetof(octopic_options, grid_extent_sigma), offsetof(octopic_beam, n),
-         offsetof(octopic_interaction_record, source_grid));
+  printf("%zu %zu %zu %zu %zu\n", offsetof(octopic_options, grid_extent_sigma), offsetof(octopic_beam, n),
+         offsetof(octopic_interaction_record, source_grid), offsetof(octopic_options, min_sigma),
+         offsetof(octopic_options, include_sigma_xy));
   return 0;
 }
 """
@@ -64,7 +65,8 @@ int main(void) {
     assert a == [C.sizeof(_abi.Slicing), C.sizeof(_abi.Options), C.sizeof(_abi.BeamStruct), C.sizeof(_abi.Stats),
                  C.sizeof(_abi.InteractionRecord), _abi.Options.slicing2.offset]
     assert b == [_abi.Options.grid_extent_sigma.offset, _abi.BeamStruct.n.offset,
-                 _abi.InteractionRecord.source_grid.offset]
+                 _abi.InteractionRecord.source_grid.offset, _abi.Options.min_sigma.offset,
+                 _abi.Options.include_sigma_xy.offset]
 
 
 @pytest.mark.parametrize("kw", [
