@@ -7,6 +7,8 @@ derivative, and either
   --arm quadratic : interaction_grid = :slice_pair, slice_interpolation = :quadratic  (default)
   --arm node      : interaction_grid = :node,       slice_interpolation = :linear
 (the reference rejects :node with :quadratic, src/tasks/strongstrong/pic_cpu.jl:560-574).
+`--solver soft` runs the soft-Gaussian solver (GaussianPoissonSolver, gaussian.jl) on the same beams and
+slicing instead (no mesh; the Hirata drift and the longitudinal kick, the reference defaults).
 A turn is one collide! plus the linear 6D one-turn map of each beam (tunes (0.08, 0.06, 0.01) and
 (0.228, 0.21, 0.01)).  W warm-up turns, then K timed turns; beams are synthetic (Philox counter RNG
 on the device) and resident in HBM before the timed region.
@@ -41,9 +43,11 @@ TRAFFIC_PROFILE = {"quadratic": "hbm_traffic_2p56M_1p024M_15sl_128_quadratic.md"
 COPY_PROBE_BYTES = 1 << 30
 
 
-def make_solver(arm, precision="fp64"):
-    from octopic import LongitudinalSlicing, PICPoissonSolver
+def make_solver(arm, precision="fp64", solver="pic"):
+    from octopic import GaussianPoissonSolver, LongitudinalSlicing, PICPoissonSolver
     sl = LongitudinalSlicing(nslices=NSLICES, method="normal_quantile", center_position="centroid")
+    if solver == "soft":
+        return GaussianPoissonSolver(slicing=sl)
     if arm == "node":
         return PICPoissonSolver(grid=(GRID, GRID), deposit_method="TSC", field_derivative="fourth",
                                 slice_interpolation="linear", interaction_grid="node", slicing=sl, precision=precision)
@@ -57,9 +61,16 @@ def params(cfg, mass):
     return beam_params(cfg["charge"], mass, cfg["energy"], cfg["npart"])
 
 
-def bytes_model(n1, n2, ns1, ns2, npl):
+def bytes_model(n1, n2, ns1, ns2, npl, solver="pic"):
     """Algorithmic HBM bytes per collide of each kernel family (FP64 coordinates)."""
     visits = n1 * ns2 + n2 * ns1          # particle-pair visits of each beam
+    if solver == "soft":
+        return {
+            "geometry": 32.0 * visits,        # the source moments: x, px, y, py of both slices of every pair
+            "kick": 96.0 * visits,            # read and write the six coordinates (the drifts move z)
+            "slicing": 8.0 * 3 * (n1 + n2) + 2 * 48.0 * (n1 + n2),
+            "store": 2 * 48.0 * (n1 + n2),    # states -> beams (6 coordinates)
+        }
     return {
         "bounds": 40.0 * visits,             # x, px, y, py, z of both slices of every pair
         "deposit": 32.0 * visits,            # x, px, y, py once: every plane of a visit comes from one pre-kick read
@@ -203,6 +214,8 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--arm", choices=["quadratic", "node"], default="quadratic")
+    ap.add_argument("--solver", choices=["pic", "soft"], default="pic",
+                    help="soft: the soft-Gaussian solver on the same beams and slicing (no mesh)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="fp32: the FP32 form (deposit weights, FFT, field, kick gather in FP32; FP64 beams)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -238,7 +251,7 @@ def main():
     from octopic import EMASS_EV, PMASS_EV, PICCollider, generate_beam, linear6d_matrix
 
     dev = torch.device("cuda:0")
-    solver = make_solver(args.arm, args.precision)
+    solver = make_solver(args.arm, args.precision, args.solver)
     col = PICCollider(solver, device=0)
     e = generate_beam(col, E_CFG["n"], params(E_CFG, EMASS_EV), E_CFG["beta"], E_CFG["alpha"], E_CFG["sigma"],
                       seed=0, rng_id=E_CFG["rng_id"], device=dev)
@@ -262,7 +275,7 @@ def main():
     torch.cuda.synchronize()
 
     npl = 3
-    model = bytes_model(E_CFG["n"], P_CFG["n"], NSLICES, NSLICES, npl)
+    model = bytes_model(E_CFG["n"], P_CFG["n"], NSLICES, NSLICES, npl, args.solver)
     # Per-family breakdown on BREAKDOWN_TURNS untimed turns (events around every launch group); the
     # dominant modelled family is then the only one timed inside the timed region.
     fam = {}
@@ -297,7 +310,7 @@ def main():
             per_launch_bytes = model[best] * args.steps / launches
             avg_s = ms / 1e3 / launches
             achieved = per_launch_bytes / avg_s / 1e9
-            traffic = measured_traffic(args.arm, best, args.precision)
+            traffic = measured_traffic(args.arm, best, args.precision) if args.solver == "pic" else None
             roof = {"bound": "hbm", "kernel": best, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": round(traffic) if traffic else None,
@@ -316,11 +329,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic (Philox counter-RNG Gaussian beams generated on the device)",
-        "config": {"workload": f"plain PIC, {E_CFG['n']} e- / {P_CFG['n']} p, {NSLICES} slices, "
-                               f"{GRID}x{GRID}, TSC, 4th-order, "
-                               + ("slice_pair + quadratic" if args.arm == "quadratic" else "node + linear")
-                               + (", FP32 form (FP64 beams, Green, moments, luminosity)" if args.precision == "fp32" else "")
+        "config": {"workload": (f"plain PIC, {E_CFG['n']} e- / {P_CFG['n']} p, {NSLICES} slices, "
+                                f"{GRID}x{GRID}, TSC, 4th-order, "
+                                + ("slice_pair + quadratic" if args.arm == "quadratic" else "node + linear")
+                                + (", FP32 form (FP64 beams, Green, moments, luminosity)" if args.precision == "fp32" else "")
+                                if args.solver == "pic" else
+                                f"soft-Gaussian solver, {E_CFG['n']} e- / {P_CFG['n']} p, {NSLICES} slices, Hirata "
+                                f"drift, longitudinal kick")
                                + ", one collide + linear 6D maps per turn",
+                   "solver": args.solver,
                    "arm": args.arm, "precision": args.precision, "global_batch": E_CFG["n"] + P_CFG["n"], "seq_len": NSLICES,
                    "parallelism": "single GPU", "launch_shape": tuned if tuned is not None else col.launch_shape()},
         "roofline": roof,
@@ -328,7 +345,10 @@ def main():
         "green_cache": {"hits": st["green_hits"], "misses": st["green_misses"], "rebuilds": st["green_rebuilds"]},
         "kernel_ms_per_turn": {k: round(v[0], 4) for k, v in fam.items()},
     }
-    if not args.no_cpu_baseline and args.precision == "fp64":
+    if args.solver == "soft":
+        out["metric"] = "s/turn soft-Gaussian solver; HBM %peak"
+        out.pop("green_cache")
+    if not args.no_cpu_baseline and args.precision == "fp64" and args.solver == "pic":
         try:
             out["cpu_baseline"] = cpu_baseline(args.arm)
         except Exception as exc:  # the CPU leg is reported, never the measurement

@@ -542,12 +542,24 @@ __global__ __launch_bounds__(256) void k_soft_kick(WeidemanCoeffs wc, SoftKickAr
     const SliceTable *ts = side == 0 ? a.tab1 : a.tab2, *tf = side == 0 ? a.tab2 : a.tab1;
     const int si = side == 0 ? pr.i : pr.j, fi = side == 0 ? pr.j : pr.i;
     const int64_t ns = ts->count[si], nf = tf->count[fi], foff = tf->offset[fi];
-    if (threadIdx.x < 10) {
-        double acc = 0.0;
-        for (int c = 0; c < a.mom_chunks; ++c) acc += a.mom_part[((int64_t)d * a.mom_chunks + c) * 10 + threadIdx.x];
-        sm[threadIdx.x] = acc;
+    {   // the chunk partials folded in a fixed order: 25 strided runs per moment, then the runs in order
+        constexpr int kRuns = 25;
+        if (threadIdx.x < 10 * kRuns) {
+            const int j = threadIdx.x % 10, r = threadIdx.x / 10;
+            const double *part = a.mom_part + (int64_t)d * a.mom_chunks * 10;
+            double acc = 0.0;
+#pragma unroll 4
+            for (int c = r; c < a.mom_chunks; c += kRuns) acc += part[c * 10 + j];
+            red[threadIdx.x] = acc;
+        }
+        __syncthreads();
+        if (threadIdx.x < 10) {
+            double acc = 0.0;
+            for (int r = 0; r < kRuns; ++r) acc += red[r * 10 + threadIdx.x];
+            sm[threadIdx.x] = acc;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     if (threadIdx.x == 0) {
         SoftMoments m;
         const double floor2 = a.min_sigma * a.min_sigma;

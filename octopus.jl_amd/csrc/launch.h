@@ -174,6 +174,14 @@ struct GpicMomentArgs {
     double *shift = nullptr;     // when set: [2*npairs][4], the sums' shift (the slice's first x, px, y, py)
 };
 hipError_t launch_gpic_moments(const GpicMomentArgs &a, hipStream_t s);
+// Chunks of the source moment sums for a batch whose largest slice has `bmax` members: about 1024
+// members per chunk (a few loads per thread, so the launch is not a chain of dependent reads), 16 to
+// kMaxMomentChunks.
+constexpr int kMaxMomentChunks = 256;
+inline int moment_chunks(int64_t bmax) {
+    const int64_t c = (bmax + 1023) / 1024;
+    return (int)(c < 16 ? 16 : (c > kMaxMomentChunks ? kMaxMomentChunks : c));
+}
 struct GpicResidualArgs {
     const Inter *inters;
     const Plane *planes;

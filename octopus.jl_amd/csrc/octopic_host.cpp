@@ -1028,7 +1028,7 @@ void soft_collide(octopic_handle *h, octopic_beam *const B[2], double *const bp[
     h->pairs.ensure(std::max<size_t>(hp.size(), 1));
     if (!hp.empty())
         HIPCHECK(hipMemcpyAsync(h->pairs.p, hp.data(), sizeof(BatchPair) * hp.size(), hipMemcpyHostToDevice, s));
-    h->gmom.ensure((size_t)2 * W * kDnMomentChunks * 10);
+    h->gmom.ensure((size_t)2 * W * kMaxMomentChunks * 10);
     h->soft_shift.ensure((size_t)2 * W * 4);
     h->soft_lum_block.ensure((size_t)std::max(npairs, 1) * kSoftBlocks);
     h->soft_lum_scale.ensure(std::max(npairs, 1));
@@ -1041,16 +1041,22 @@ void soft_collide(octopic_handle *h, octopic_beam *const B[2], double *const bp[
         const int Bn = bcnt[bi];
         if (Bn == 0) continue;
         const BatchPair *dp = h->pairs.p + boff[bi];
+        int64_t bmax = 0;
+        for (int k = 0; k < Bn; ++k) {
+            const BatchPair &pr = hp[boff[bi] + k];
+            bmax = std::max<int64_t>(bmax, std::max<int64_t>(T1.count[pr.i], T2.count[pr.j]));
+        }
+        const int mch = moment_chunks(bmax);
         {
             Timed t(h, F_GEOMETRY);
-            GpicMomentArgs a{dp, Bn, dt1, dt2, st, kDnMomentChunks, h->gmom.p, h->errinfo.p, h->soft_shift.p};
+            GpicMomentArgs a{dp, Bn, dt1, dt2, st, mch, h->gmom.p, h->errinfo.p, h->soft_shift.p};
             HIPCHECK(launch_gpic_moments(a, s));
         }
         {
             Timed t(h, F_KICK);
             SoftKickArgs a{};
             a.pairs = dp; a.npairs = Bn; a.tab1 = dt1; a.tab2 = dt2; a.st = st;
-            a.mom_chunks = kDnMomentChunks; a.mom_part = h->gmom.p; a.shift = h->soft_shift.p;
+            a.mom_chunks = mch; a.mom_part = h->gmom.p; a.shift = h->soft_shift.p;
             a.kbb1 = kbb1; a.kbb2 = kbb2; a.klum1 = klum1; a.klum2 = klum2; a.min_sigma = o.min_sigma;
             a.ignore1 = o.ignore_centroid1 ? 1 : 0; a.ignore2 = o.ignore_centroid2 ? 1 : 0;
             a.drift = o.virtual_drift; a.lkick = o.longitudinal_kick ? 1 : 0;
