@@ -100,7 +100,7 @@ struct BoundsArgs {
 // Particle ranges of a batch: slice members [n*c/nchunks, n*(c+1)/nchunks) for chunk c, the same in
 // the bounds, deposit and kick launches, so that a chunk's box describes the block that takes it.
 constexpr int kChunkSpan = 2048;
-constexpr int kMaxChunks = 64;
+constexpr int kMaxChunks = 128;
 hipError_t launch_bounds(const BoundsArgs &a, hipStream_t s);
 
 struct GeometryArgs {

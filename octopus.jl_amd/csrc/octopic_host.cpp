@@ -1596,7 +1596,13 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.subdiv = 1;
             if (h->kick_target_blocks > 0) {
                 const int per = nchunks * 2 * B;
-                const int sd = (h->kick_target_blocks + per - 1) / per;
+                // parts of chunks for about kick_target_blocks items, with items of at least a block's
+                // threads in particles on average (each item stages its chunk's LDS window)
+                int sd = (h->kick_target_blocks + per - 1) / per;
+                int64_t tot = 0;
+                for (int k = 0; k < B; ++k) { const BatchPair &pr = hp[boff[bi] + k]; tot += T1.count[pr.i] + T2.count[pr.j]; }
+                const int64_t cap = std::max<int64_t>(1, tot / ((int64_t)per * 512));   // 512: the kick's block size
+                if (sd > cap) sd = (int)cap;
                 a.subdiv = sd < 1 ? 1 : (sd > 64 ? 64 : sd);
             }
             a.pass_windows = h->kick_pass_windows >= 0 ? h->kick_pass_windows : (h->fp32 ? 0 : 1);
