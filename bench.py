@@ -344,6 +344,9 @@ def main():
         "luminosity_mean": sum(lums) / len(lums) if lums else None,
         "green_cache": {"hits": st["green_hits"], "misses": st["green_misses"], "rebuilds": st["green_rebuilds"]},
         "kernel_ms_per_turn": {k: round(v[0], 4) for k, v in fam.items()},
+        # every modelled family's algorithmic HBM bytes per turn over its time per turn (breakdown turns)
+        "family_hbm_frac": {k: round(model[k] / (fam[k][0] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+                            for k in fam if k in model and fam[k][0] > 0},
     }
     if args.solver == "soft":
         out["metric"] = "s/turn soft-Gaussian solver; HBM %peak"

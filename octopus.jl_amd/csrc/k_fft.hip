@@ -188,9 +188,9 @@ __global__ __launch_bounds__(kFFTThreads) void k_green_cols(const GreenJob *jobs
 
 // Luminosity overlap (pic_cpu.jl:2731-2735), one of kLumBlocks blocks of pair k: the block sums q1.q2
 // over a fixed cell range of the pair's two luminosity planes (resetting them to zero for the next
-// batch); the block that finishes last (per-pair arrival counter, reset by that block) adds the
-// kLumBlocks partials in block order, so the result does not depend on scheduling.  Called by all
-// kFFTThreads threads of a block.
+// batch) into its partial; the column launch that follows adds the kLumBlocks partials in block order
+// (lum_fold), so the result does not depend on scheduling and no block waits on another (a device-wide
+// release per block costs an L2 write-back on this chip).  Called by all kFFTThreads threads of a block.
 __device__ __forceinline__ void lum_block(const LumArgs &a, int k, int b) {
     unsigned long long *q1 = a.charge + (size_t)(2 * k) * a.charge_stride;
     unsigned long long *q2 = a.charge + (size_t)(2 * k + 1) * a.charge_stride;
@@ -204,26 +204,21 @@ __device__ __forceinline__ void lum_block(const LumArgs &a, int k, int b) {
         acc += ((double)u1 * inv) * ((double)u2 * inv);
     }
     __shared__ double red[kFFTThreads];
-    __shared__ int last;
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int w = kFFTThreads / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        a.lum_partial[k * kLumBlocks + b] = red[0];
-        __threadfence();
-        last = atomicAdd(&a.lum_arrivals[k], 1u) == kLumBlocks - 1;
-    }
-    __syncthreads();
-    if (!last || threadIdx.x != 0) return;
-    __threadfence();
-    const volatile double *part = a.lum_partial + k * kLumBlocks;
-    double sum = 0.0;
-    for (int q = 0; q < kLumBlocks; ++q) sum += part[q];
-    a.lum_arrivals[k] = 0u;
-    if (!failed(a.err)) {
+    if (threadIdx.x == 0) a.lum_partial[k * kLumBlocks + b] = red[0];
+}
+
+// The batch's per-pair luminosity from the partials of the row launch before (thread k: pair k).
+__device__ __forceinline__ void lum_fold(const LumArgs &a) {
+    for (int k = threadIdx.x; k < a.npairs; k += blockDim.x) {
+        const double *part = a.lum_partial + k * kLumBlocks;
+        double sum = 0.0;
+        for (int q = 0; q < kLumBlocks; ++q) sum += part[q];
         const double *g = a.lumgeo + 3 * k;
         a.lum_parts[a.pairs[k].p] = sum * g[2] * g[0] * g[1];
     }
@@ -281,10 +276,11 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_fwd(const Plane *planes, u
 template <class C>
 __global__ __launch_bounds__(kFFTThreads) void k_cols_conv(const Plane *planes, C *qh, int64_t qh_stride,
                                                            const C *gspec, int64_t slot_stride, int nx, int ny,
-                                                           FFTPlanDev py, const ErrorInfo *err) {
+                                                           FFTPlanDev py, const ErrorInfo *err, LumArgs lum) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *lds = reinterpret_cast<C *>(lds_raw);
     if (failed(err)) return;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && lum.npairs > 0) lum_fold(lum);
     const int plane = blockIdx.y;
     const int P2 = 2 * ny;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
@@ -508,13 +504,13 @@ hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long 
 
 template <class C>
 hipError_t launch_cols_conv(const Plane *planes, int nplanes, C *qh, int64_t qh_stride, const C *gspec, int64_t slot_stride,
-                            int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, hipStream_t s) {
+                            int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, const LumArgs &lum, hipStream_t s) {
     if (nplanes <= 0) return hipSuccess;
     const size_t lds = fft_lds_bytes(2 * ny) / sizeof(double2) * sizeof(C);
     ensure_lds((const void *)k_cols_conv<C>, lds);
     dim3 g((nx + 1 + kFFTPerBlock - 1) / kFFTPerBlock, nplanes);
     hipLaunchKernelGGL(k_cols_conv<C>, g, dim3(kFFTThreads), lds, s, planes, qh, qh_stride, gspec, slot_stride, nx, ny, py,
-                       err);
+                       err, lum);
     return hipGetLastError();
 }
 
@@ -557,7 +553,7 @@ hipError_t launch_green_to_f32(const GreenJob *jobs, int njobs, const double2 *s
     template hipError_t launch_rows_fwd<C>(const Plane *, int, unsigned long long *, int64_t, C *, int64_t, int, int,  \
                                            double, const FFTPlanDev &, const ErrorInfo *, const LumArgs &, hipStream_t); \
     template hipError_t launch_cols_conv<C>(const Plane *, int, C *, int64_t, const C *, int64_t, int, int,            \
-                                            const FFTPlanDev &, const ErrorInfo *, hipStream_t);                       \
+                                            const FFTPlanDev &, const ErrorInfo *, const LumArgs &, hipStream_t);      \
     template hipError_t launch_rows_inv<C>(const Plane *, int, const C *, int64_t, real_of<C> *, int, int,             \
                                            const FFTPlanDev &, const ErrorInfo *, hipStream_t);                        \
     template hipError_t launch_field<real_of<C>>(const Inter *, int, const Plane *, const real_of<C> *, real_of<C> *, \

@@ -261,7 +261,6 @@ struct LumArgs {
     const double *lumgeo;
     double *lum_parts;           // indexed by pair position p
     double *lum_partial;         // [npairs][kLumBlocks]
-    unsigned int *lum_arrivals;  // [npairs], zero between launches
     const ErrorInfo *err;
 };
 constexpr int kLumBlocks = 16;   // blocks per pair of the luminosity overlap sum (in the charge row launch)
@@ -277,8 +276,9 @@ hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long 
                            int64_t qh_stride, int nx, int ny, double inv_scale, const FFTPlanDev &px, const ErrorInfo *err,
                            const LumArgs &lum, hipStream_t s);   // lum.npairs = 0: no luminosity sums
 template <class C>
+// lum.npairs > 0: the launch also folds the luminosity partials of the row launch before it
 hipError_t launch_cols_conv(const Plane *planes, int nplanes, C *qh, int64_t qh_stride, const C *gspec, int64_t slot_stride,
-                            int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, hipStream_t s);
+                            int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, const LumArgs &lum, hipStream_t s);
 template <class C>
 hipError_t launch_rows_inv(const Plane *planes, int nplanes, const C *qh, int64_t qh_stride, real_of<C> *phi, int nx,
                            int ny, const FFTPlanDev &px, const ErrorInfo *err, hipStream_t s);

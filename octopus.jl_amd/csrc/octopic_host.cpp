@@ -292,7 +292,6 @@ struct octopic_handle {
     DBuf<octopic_interaction_record> records;
     DBuf<double> lum_parts;
     DBuf<double> lum_partial;
-    DBuf<unsigned int> lum_arrivals;
     // node-mode buffers
     DBuf<CacheGeo> nodes;
     DBuf<GreenJob> node_jobs;
@@ -598,7 +597,7 @@ void octopic_destroy(octopic_handle *h) {
     h->kqueue.release(); h->korder.release(); h->dqueue.release(); h->dorder.release();
     h->keys.release(); h->chunkbox.release(); h->box.release(); h->thist.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
     h->charge.release(); h->qh.release(); h->phi.release(); h->table.release(); h->records.release();
-    h->lum_parts.release(); h->lum_partial.release(); h->lum_arrivals.release(); h->nodes.release(); h->node_jobs.release(); h->node_keys.release(); h->ran_dev.release();
+    h->lum_parts.release(); h->lum_partial.release(); h->nodes.release(); h->node_jobs.release(); h->node_keys.release(); h->ran_dev.release();
     h->px.tw.release(); h->py.tw.release(); h->px.tw32.release(); h->py.tw32.release();
     h->spec32.release(); h->qh32.release(); h->phi32.release(); h->table32.release();
     h->dn_mom.release(); h->dn_meta.release(); h->dn_prof.release(); h->dn_in.release();
@@ -1341,7 +1340,6 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     h->records.ensure((size_t)2 * std::max(npairs, 1));
     h->lum_parts.ensure(std::max(npairs, 1));
     h->lum_partial.ensure((size_t)W * kLumBlocks);
-    h->lum_arrivals.ensure_zero(W, s);
     HIPCHECK(hipMemsetAsync(h->records.p, 0, sizeof(octopic_interaction_record) * 2 * npairs, s));
     HIPCHECK(hipMemsetAsync(h->lum_parts.p, 0, sizeof(double) * npairs, s));
     const int temp_slot0 = h->nkeys;
@@ -1553,7 +1551,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             lum.pairs = dp; lum.npairs = B; lum.charge = h->charge.p + (size_t)nfp * charge_stride;
             lum.charge_stride = charge_stride; lum.ncells = (int)lum_cells; lum.inv_scale = inv_scale;
             lum.lumgeo = h->lumgeo.p; lum.lum_parts = h->lum_parts.p; lum.lum_partial = h->lum_partial.p;
-            lum.lum_arrivals = h->lum_arrivals.p; lum.err = h->errinfo.p;
+            lum.err = h->errinfo.p;
         }
         auto solve = [&](auto *qh, const auto *spec, auto *phi, auto *table) {
             {
@@ -1564,7 +1562,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             {
                 Timed t(h, F_COLS);
                 HIPCHECK(launch_cols_conv(h->planes.p, nfp, qh, qh_stride, spec, h->slot_stride, nx, ny, h->py.dev,
-                                          h->errinfo.p, s));
+                                          h->errinfo.p, lum, s));
             }
             {
                 Timed t(h, F_ROWS_INV);
