@@ -31,18 +31,20 @@ __global__ __launch_bounds__(256) void k_zlanes(const double *z, int64_t n, cons
     double acc = 0.0;
     double lo = INFINITY, hi = -INFINITY;
     int64_t i = t;
-    // Loads are issued ahead of the in-order accumulation; the summation order is the lane fold's.
-    for (; i + 7 * kFoldLanes < n; i += 8 * kFoldLanes) {
-        double v[8];
+    // Loads are issued ahead of the in-order accumulation (32 in flight per lane: the 4096 lanes are
+    // few); the summation order is the lane fold's.
+    constexpr int U = 32;
+    for (; i + (U - 1) * kFoldLanes < n; i += U * kFoldLanes) {
+        double v[U];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = z[i + u * kFoldLanes];
+        for (int u = 0; u < U; ++u) v[u] = z[i + u * kFoldLanes];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < U; ++u) {
             if (pass == 2) v[u] = (v[u] - mu) * (v[u] - mu);
             else { lo = jmin(lo, v[u]); hi = jmax(hi, v[u]); }
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc += v[u];
+        for (int u = 0; u < U; ++u) acc += v[u];
     }
     for (; i < n; i += kFoldLanes) {
         double v = z[i];
