@@ -244,7 +244,6 @@ struct KickArgs {
     unsigned long long *dropped;
     const double *chunkbox;      // chunk boxes of the batch's bounds launch, or null (no LDS windows)
     const ErrorInfo *err;
-    int *queue;                  // work-queue counter of this launch (zero on entry)
     const int32_t *order;        // launch order of the interactions (position -> interaction)
     const double *ab;            // Gaussian-subtracted add-back [N1 + N2][4] (mode 0 only), or null
     int64_t ab_base[2];

@@ -256,8 +256,7 @@ struct octopic_handle {
     double deposit_trial_ms[2] = {0.0, 0.0};
     std::vector<hipEvent_t> deposit_ev;   // per batch: before and after the deposit launches (trials)
     int deposit_min_span = kChunkSpan;   // smallest items the two-per-CU rule may make
-    // Kick and deposit work queues: per-launch counters and launch orders (larger slices first).
-    DBuf<int> kqueue;
+    // Kick launch orders and deposit work queues: per-launch counters and launch orders (larger slices first).
     DBuf<int32_t> korder;
     DBuf<int> dqueue;            // [batch][field planes, luminosity planes]
     DBuf<int32_t> dorder;        // per batch: field planes, then luminosity planes
@@ -594,7 +593,7 @@ void octopic_destroy(octopic_handle *h) {
     }
     h->rank_keys.release(); h->rank_sorted.release(); h->rank_vals.release(); h->rank_temp.release();
     h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
-    h->kqueue.release(); h->korder.release(); h->dqueue.release(); h->dorder.release();
+    h->korder.release(); h->dqueue.release(); h->dorder.release();
     h->keys.release(); h->chunkbox.release(); h->box.release(); h->thist.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
     h->charge.release(); h->qh.release(); h->phi.release(); h->table.release(); h->records.release();
     h->lum_parts.release(); h->lum_partial.release(); h->nodes.release(); h->node_jobs.release(); h->node_keys.release(); h->ran_dev.release();
@@ -1286,8 +1285,6 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         }
         h->korder.ensure(ord.size());
         HIPCHECK(hipMemcpyAsync(h->korder.p, ord.data(), sizeof(int32_t) * ord.size(), hipMemcpyHostToDevice, s));
-        h->kqueue.ensure(batches.size());
-        HIPCHECK(hipMemsetAsync(h->kqueue.p, 0, sizeof(int) * batches.size(), s));
         // deposit planes by source-slice population, largest first (stable): per batch the field planes
         // (interaction d = 2k + dir - 1 deposits beam 1's slice i for dir 1, beam 2's slice j for dir 2),
         // then the luminosity planes (2k + side)
@@ -1604,7 +1601,6 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
                 a.subdiv = sd < 1 ? 1 : (sd > 64 ? 64 : sd);
             }
             a.pass_windows = h->kick_pass_windows >= 0 ? h->kick_pass_windows : (h->fp32 ? 0 : 1);
-            a.queue = h->kqueue.p + bi;
             a.order = h->korder.p + (size_t)2 * boff[bi];
             a.dropped = h->counters.p + 4;
             a.chunkbox = cbox;
