@@ -9,6 +9,7 @@ from .solver import (AtTurns, ArgumentError, BeamParams, EveryNSteps, GaussianPI
                      beam_params, linear6d_matrix)
 from .backend import (Beam, BackendUnavailable, DegenerateExtentError, HIPBackend, NonFiniteCoordinateError,
                       OctopicError, PICCollider, UnsupportedOptionError, collide_, collide_many, generate_beam, lib)
+from .artifact import RunArtifact, read_luminosity
 
 __all__ = [
     "AtTurns", "ArgumentError", "BeamParams", "EveryNSteps", "GaussianPICPoissonSolver", "GaussianPoissonSolver",
@@ -16,4 +17,5 @@ __all__ = [
     "EMASS_EV", "ME0", "PMASS_EV", "RE", "TWOPI", "beam_params", "linear6d_matrix",
     "Beam", "BackendUnavailable", "DegenerateExtentError", "HIPBackend", "NonFiniteCoordinateError",
     "OctopicError", "PICCollider", "UnsupportedOptionError", "collide_", "collide_many", "generate_beam", "lib",
+    "RunArtifact", "read_luminosity",
 ]
