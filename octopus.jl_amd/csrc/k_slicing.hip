@@ -265,21 +265,33 @@ __global__ __launch_bounds__(kSliceThreads) void k_classify(const double *z, con
 }
 
 // Exclusive scan of hist (slice-major, ns * nblocks) in place; writes the per-slice count/offset.
+// Exclusive prefix sum of one int64 per thread over a block of up to 1024 threads (wave shuffles, then
+// the wave totals), plus `base`.
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t base) {
+    __shared__ int64_t wave_tot[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (int)(blockDim.x + 63) >> 6;
+    int64_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t u = __shfl_up(inc, o);
+        if (lane >= o) inc += u;
+    }
+    if (lane == 63) wave_tot[wave] = inc;
+    __syncthreads();
+    int64_t before = base;
+    for (int w = 0; w < wave; ++w) before += wave_tot[w];
+    (void)nw;
+    __syncthreads();   // wave_tot may be reused by the caller's next scan
+    return before + inc - v;
+}
+
 __global__ __launch_bounds__(1024) void k_scan(int *hist, int ns, int nblocks, SliceTable *tab) {
-    __shared__ int64_t part[1024];
     const int64_t total = (int64_t)ns * nblocks;
     const int64_t per = (total + blockDim.x - 1) / blockDim.x;
-    const int64_t lo = threadIdx.x * per, hi = lo + per < total ? lo + per : total;
+    const int64_t lo = threadIdx.x * per < total ? threadIdx.x * per : total, hi = lo + per < total ? lo + per : total;
     int64_t sum = 0;
     for (int64_t k = lo; k < hi; ++k) sum += hist[k];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int64_t run = 0;
-        for (int t = 0; t < (int)blockDim.x; ++t) { int64_t v = part[t]; part[t] = run; run += v; }
-    }
-    __syncthreads();
-    int64_t run = part[threadIdx.x];
+    int64_t run = block_exclusive_scan(sum, 0);
     for (int64_t k = lo; k < hi; ++k) {
         int v = hist[k];
         hist[k] = (int)run;
@@ -444,7 +456,6 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_hist(TileSortArgs a) {
 
 // one block per slice: exclusive scan of the slice's TT * G counts, plus the slice offset
 __global__ __launch_bounds__(1024) void k_tile_scan(TileSortArgs a, int G) {
-    __shared__ int64_t part[1024];
     const int s = blockIdx.x;
     const int64_t total = (int64_t)a.T * a.T * G;
     int *h = a.thist + (size_t)s * total;
@@ -453,14 +464,7 @@ __global__ __launch_bounds__(1024) void k_tile_scan(TileSortArgs a, int G) {
     const int64_t hi = lo + per < total ? lo + per : total;
     int64_t sum = 0;
     for (int64_t k = lo; k < hi; ++k) sum += h[k];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int64_t run = a.tab->offset[s];
-        for (int t = 0; t < (int)blockDim.x; ++t) { const int64_t v = part[t]; part[t] = run; run += v; }
-    }
-    __syncthreads();
-    int64_t run = part[threadIdx.x];
+    int64_t run = block_exclusive_scan(sum, a.tab->offset[s]);
     for (int64_t k = lo; k < hi; ++k) {
         const int v = h[k];
         h[k] = (int)run;
