@@ -124,19 +124,30 @@ hipError_t launch_bounds(const BoundsArgs &a, hipStream_t s) {
 // Lane (k, dir) takes the slots it consumes: the source box (0-3) of the source side and the field box
 // (4-7) of the field side; lane dir 1 also takes both luminosity boxes (8-11).  Every slot of the batch
 // is thereby zeroed exactly once, before any early return.
+// All slots are read before any is zeroed, so the loads are in flight together.
 __device__ __forceinline__ void take_boxes(unsigned long long *keys, int k, int dir, double vs[4], double vf[4],
                                            double l1[4], double l2[4]) {
     unsigned long long *s0 = keys + (size_t)(2 * k) * kBoundSlots, *s1 = s0 + kBoundSlots;
     unsigned long long *src = dir == 1 ? s0 : s1, *fld = dir == 1 ? s1 : s0;
+    unsigned long long u[16];
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
-        vs[q] = take_bound(src + q, (q & 1) == 0);
-        vf[q] = take_bound(fld + 4 + q, (q & 1) == 0);
+        u[q] = src[q];
+        u[4 + q] = fld[4 + q];
+        u[8 + q] = dir == 1 ? s0[8 + q] : 0ull;
+        u[12 + q] = dir == 1 ? s1[8 + q] : 0ull;
     }
-    if (dir == 1)
-        for (int q = 0; q < 4; ++q) {
-            l1[q] = take_bound(s0 + 8 + q, (q & 1) == 0);
-            l2[q] = take_bound(s1 + 8 + q, (q & 1) == 0);
-        }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const bool mn = (q & 1) == 0;
+        vs[q] = bound_value(u[q], mn);
+        vf[q] = bound_value(u[4 + q], mn);
+        l1[q] = bound_value(u[8 + q], mn);
+        l2[q] = bound_value(u[12 + q], mn);
+        src[q] = 0ull;
+        fld[4 + q] = 0ull;
+        if (dir == 1) { s0[8 + q] = 0ull; s1[8 + q] = 0ull; }
+    }
 }
 
 __device__ __forceinline__ bool resolve_extent(double xspan, double yspan, double fx, double fy, double &w, double &h) {
@@ -347,12 +358,13 @@ __global__ void k_geometry(GeometryArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= 2 * a.npairs) return;
     const int k = lane >> 1, dir = (lane & 1) + 1, d = lane;
-    neutralise_inter(k, dir, a.npairs, a.o, a.inters, a.planes, a.jobs);
+    const BatchPair pr = a.pairs[k];     // loads first: the stores below may alias them for the compiler
     double vs[4], vf[4], l1[4], l2[4];   // source box, field box, luminosity boxes
     take_boxes(a.keys, k, dir, vs, vf, l1, l2);
-    if (failed(a.err)) return;
+    const bool fail = failed(a.err);
+    neutralise_inter(k, dir, a.npairs, a.o, a.inters, a.planes, a.jobs);
+    if (fail) return;
     const GeomOpts &o = a.o;
-    const BatchPair pr = a.pairs[k];
     const int i = pr.i, j = pr.j, p = pr.p;
     const SliceTable *ts = dir == 1 ? a.tab1 : a.tab2, *tf = dir == 1 ? a.tab2 : a.tab1;
     const int si = dir == 1 ? i : j, fi = dir == 1 ? j : i;
