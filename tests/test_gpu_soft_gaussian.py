@@ -11,6 +11,7 @@ C ABI.
 - Float32 beams: the FP64 collide of the widened beams rounded back, z included, on device and host arrays.
 - A non-finite source coordinate fails the collide and leaves the beams as they were; the unscheduled
   turns of a luminosity schedule return NaN with the same kicks; rejected combinations.
+- Empty slices (every pair still runs, with and without a min_sigma floor) and beams smaller than a block.
 """
 import ctypes as C
 import math
@@ -153,3 +154,23 @@ def test_rejected_combinations(gpu):
     o.precision = _abi.PRECISION["fp32"]
     h = C.c_void_p()
     assert lib().octopic_create(C.byref(o), 0, C.byref(h)) == _abi.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("min_sigma", [0.0, 2e-5])
+def test_empty_slices_run_every_pair(oracle, gpu, min_sigma):
+    """`:specified` boundaries at z = -9 m and -8.5 m leave the first two slices of both beams empty. The
+    soft-Gaussian solver runs every pair (gaussian.jl:7-22): an empty source is a zero-width Gaussian at
+    the origin (floored by min_sigma), whose kick on the other slice is zero without the floor."""
+    e, p = round_beams(oracle, 12000, 6000, seed=4)
+    sl = LongitudinalSlicing(nslices=4, method="specified", center_position="centroid", positions=(-9.0, -8.5, 0.0))
+    gcol = compare_collide(oracle, gpu, GaussianPoissonSolver(slicing=sl, min_sigma=min_sigma), e, p)
+    _, _, ran, _ = gcol.pair_luminosity()
+    assert all(ran)
+    assert gcol.stats()["pairs"] == 16
+
+
+def test_tiny_ragged_beams(oracle, gpu):
+    """Beams far smaller than one block, with sizes that fill no block evenly."""
+    e, p = round_beams(oracle, 61, 23, seed=9)
+    compare_collide(oracle, gpu, GaussianPoissonSolver(slicing=LongitudinalSlicing(nslices=3, method="equal_count"),
+                                                       virtual_drift="exact"), e, p, collides=2)
