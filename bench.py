@@ -47,7 +47,7 @@ def make_solver(arm, precision="fp64", solver="pic"):
     from octopic import GaussianPoissonSolver, LongitudinalSlicing, PICPoissonSolver
     sl = LongitudinalSlicing(nslices=NSLICES, method="normal_quantile", center_position="centroid")
     if solver == "soft":
-        return GaussianPoissonSolver(slicing=sl)
+        return GaussianPoissonSolver(slicing=sl, precision=precision)
     if arm == "node":
         return PICPoissonSolver(grid=(GRID, GRID), deposit_method="TSC", field_derivative="fourth",
                                 slice_interpolation="linear", interaction_grid="node", slicing=sl, precision=precision)
@@ -335,7 +335,9 @@ def main():
                                 + (", FP32 form (FP64 beams, Green, moments, luminosity)" if args.precision == "fp32" else "")
                                 if args.solver == "pic" else
                                 f"soft-Gaussian solver, {E_CFG['n']} e- / {P_CFG['n']} p, {NSLICES} slices, Hirata "
-                                f"drift, longitudinal kick")
+                                f"drift, longitudinal kick"
+                                + (", FP32 Bassetti-Erskine evaluation (FP64 beams, drifts, moments)"
+                                   if args.precision == "fp32" else ""))
                                + ", one collide + linear 6D maps per turn",
                    "solver": args.solver,
                    "arm": args.arm, "precision": args.precision, "global_batch": E_CFG["n"] + P_CFG["n"], "seq_len": NSLICES,

@@ -108,7 +108,11 @@ typedef struct octopic_options {
     int32_t precision;             /* OCTOPIC_PRECISION_*: FP64 (the reference's Float64 path) or the FP32
                                       form: deposit weights, FFT convolution, field and kick gather in FP32
                                       on FP64 beams; Green evaluation, slice moments, luminosity in FP64.
-                                      The FP32 form has no counterpart in the reference. */
+                                      The FP32 form of the PIC solvers has no counterpart in the reference.
+                                      For OCTOPIC_SOLVER_SOFT_GAUSSIAN, FP32 is GaussianPoissonSolver{Float32}'s
+                                      Bassetti-Erskine evaluation (field, Hessian, density) in FP32 with the
+                                      Float32 constants; drifts, slice moments, coordinates and luminosity
+                                      sums stay FP64. */
     double grid_extent_sigma;
     double min_extent_x, min_extent_y;
     double grid_quantize;
@@ -255,6 +259,14 @@ int octopic_track_crab_cavity(octopic_handle *h, double frequency, int32_t nharm
 int octopic_gaussian_beambeam_kick(octopic_handle *h, const double *x, const double *y, int64_t n, double sigx,
                                    double sigy, double *kx, double *ky);
 
+/* gaussian_beambeam_kick(sigx::Float32, sigy, x, y) (the same, strong_beam.jl:770-783, 809-811 for the
+ * Float32 constants: near-round conditioning factor 8, eps = 2^-23, radial series through order 16, and
+ * the Weideman coefficients rounded to Float32, SpecialMath.jl:98-100): FP32 arithmetic throughout,
+ * evaluated on the scale of max(sigx, sigy) (the kick is homogeneous of degree -1), so that sizes of
+ * any magnitude stay within the FP32 range.  Device float pointers. */
+int octopic_gaussian_beambeam_kick_f32(octopic_handle *h, const float *x, const float *y, int64_t n, float sigx,
+                                       float sigy, float *kx, float *ky);
+
 /* _gaussian_beambeam_kick_response(kbb, sigx, sigy, x, y) (src/elements/strong_beam.jl:1061-1128): per point
  * out4[4i..4i+3] = (Kx, Ky, Hxx, Hyy), the normalised Bassetti-Erskine kick and kbb times the diagonal of
  * the Gaussian potential's Hessian (with the near-round blend's chain term), as the Gaussian-subtracted
@@ -263,11 +275,20 @@ int octopic_gaussian_beambeam_kick(octopic_handle *h, const double *x, const dou
 int octopic_gaussian_beambeam_kick_response(octopic_handle *h, const double *x, const double *y, int64_t n, double kbb,
                                             double sigx, double sigy, double *out4);
 
+/* _gaussian_beambeam_kick_response for Float32 arguments: as above, in FP32 with the Float32 constants,
+ * on the scale of max(sigx, sigy) (Hessian homogeneous of degree -2).  Device float pointers. */
+int octopic_gaussian_beambeam_kick_response_f32(octopic_handle *h, const float *x, const float *y, int64_t n, float kbb,
+                                                float sigx, float sigy, float *out4);
+
 /* faddeeva_w_approx_reim(zr, zi) (src/math/SpecialMath.jl:60-151): w(z) = exp(-z^2) erfc(-iz) by
  * Weideman's 32-term rational approximation, the Laplace continued fraction for large |z| and the
  * reflection w(z) = 2 exp(-z^2) - w(-z) below the real axis.  FP64; device pointers; on the handle's
  * stream. */
 int octopic_faddeeva_w(octopic_handle *h, const double *zr, const double *zi, int64_t n, double *wr, double *wi);
+
+/* faddeeva_w_approx_reim(zr::Float32, zi::Float32) (SpecialMath.jl:60-151 in T = Float32): the same
+ * algorithm in FP32 arithmetic with T(c) coefficients.  Device float pointers. */
+int octopic_faddeeva_w_f32(octopic_handle *h, const float *zr, const float *zi, int64_t n, float *wr, float *wi);
 
 /* Beam(N, ...) synthetic Gaussian construction on the device (Beam.jl:127-181, 545-553): Philox4x32-10
  * counter normals, clamp to +-cutoff, standardise, scale by sigma/beta with alpha.  beta, alpha,

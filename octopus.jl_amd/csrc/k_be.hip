@@ -30,358 +30,431 @@ namespace octopic {
 
 namespace {
 
+// The evaluation is one source for both working types, FP64 and FP32, with the per-type constants of the
+// reference (strong_beam.jl:770-783, 809-811): the near-round blend's conditioning factor C (64, 8) and
+// rounding unit eps (2^-52, 2^-23), which set eta* = (8/3 C eps)^(1/4) and the axis-series switch
+// rho^7 <= eps / sqrt(eta), and the last order of the radial-moment series (24, 16).
+template <typename T> struct BeType;
+template <> struct BeType<double> {
+    static constexpr double eps = 0x1p-52, cond = 64.0;
+    static constexpr int last_order = 24;
+    using Coeffs = WeidemanCoeffs;
+};
+template <> struct BeType<float> {
+    static constexpr float eps = 0x1p-23f, cond = 8.0f;
+    static constexpr int last_order = 16;
+    using Coeffs = WeidemanCoeffsF;
+};
+template <typename T> using Coeffs = typename BeType<T>::Coeffs;
+
 constexpr double kInvSqrtPi = 0.56418958354775628695;
 constexpr double kSqrtPi = 1.7724538509055160273;
 constexpr double kSqrt2 = 1.4142135623730950488;
 
-__device__ __forceinline__ void cmul(double ar, double ai, double br, double bi, double &r, double &i) {
+template <typename T>
+__device__ __forceinline__ void cmul(T ar, T ai, T br, T bi, T &r, T &i) {
     r = ar * br - ai * bi;
     i = ar * bi + ai * br;
 }
 
-__device__ __forceinline__ void cdiv(double ar, double ai, double br, double bi, double &r, double &i) {
-    const double inv = 1.0 / (br * br + bi * bi);
+template <typename T>
+__device__ __forceinline__ void cdiv(T ar, T ai, T br, T bi, T &r, T &i) {
+    const T inv = T(1) / (br * br + bi * bi);
     r = (ar * br + ai * bi) * inv;
     i = (ai * br - ar * bi) * inv;
 }
 
 // Laplace continued fraction of w for large |z| (Im z >= 0), with Gautschi's term-count estimate
 // nu = floor(3.9 + 11.398 / (0.08254 x + 0.1421 y + 0.2023)) (SpecialMath.jl:113-151)
-__device__ void w_far(double zr, double zi, double &wr, double &wi) {
-    const double x = fabs(zr), y = zi;
-    if (x + y > 4000.0) {
-        if (x + y > 1e7) {   // w ~ i / (sqrt(pi) z)
+template <typename T>
+__device__ void w_far(T zr, T zi, T &wr, T &wi) {
+    const T isp = T(kInvSqrtPi);
+    const T x = fabs(zr), y = zi;
+    if (x + y > T(4000)) {
+        if (x + y > T(1e7)) {   // w ~ i / (sqrt(pi) z)
             if (x > y) {
-                const double q = y / zr, d = kInvSqrtPi / (zr + q * y);
+                const T q = y / zr, d = isp / (zr + q * y);
                 wr = d * q; wi = d;
             } else {
-                const double q = zr / y, d = kInvSqrtPi / (q * zr + y);
+                const T q = zr / y, d = isp / (q * zr + y);
                 wr = d; wi = d * q;
             }
             return;
         }
         // w ~ i z / (sqrt(pi) (z^2 - 1/2))
-        const double dr = zr * zr - y * y - 0.5, di = 2.0 * zr * y;
-        const double d = kInvSqrtPi / (dr * dr + di * di);
+        const T dr = zr * zr - y * y - T(0.5), di = T(2) * zr * y;
+        const T d = isp / (dr * dr + di * di);
         wr = d * (zr * di - y * dr);
         wi = d * (zr * dr + y * di);
         return;
     }
-    double nu = floor(3.9 + 11.398 / (0.08254 * x + 0.1421 * y + 0.2023));
-    double tr = zr, ti = y;
-    nu = 0.5 * (nu - 1.0);
-    while (nu > 0.4) {   // t <- z - nu / t, nu = (n-1)/2, ..., 1/2
-        const double s = nu / (tr * tr + ti * ti);
-        const double nr = zr - tr * s, ni = y + ti * s;
+    T nu = floor(T(3.9) + T(11.398) / (T(0.08254) * x + T(0.1421) * y + T(0.2023)));
+    T tr = zr, ti = y;
+    nu = T(0.5) * (nu - T(1));
+    while (nu > T(0.4)) {   // t <- z - nu / t, nu = (n-1)/2, ..., 1/2
+        const T s = nu / (tr * tr + ti * ti);
+        const T nr = zr - tr * s, ni = y + ti * s;
         tr = nr; ti = ni;
-        nu -= 0.5;
+        nu -= T(0.5);
     }
-    const double s = kInvSqrtPi / (tr * tr + ti * ti);   // i / (sqrt(pi) t)
+    const T s = isp / (tr * tr + ti * ti);   // i / (sqrt(pi) t)
     wr = s * ti;
     wi = s * tr;
 }
 
-__device__ void w_upper(const WeidemanCoeffs &wc, double zr, double zi, double &wr, double &wi) {
-    const double x = fabs(zr), y = zi;
-    if (y > 7.0 || (x > 6.0 && (y > 0.1 || (x > 8.0 && y > 1e-10) || x > 28.0))) {
+template <typename T>
+__device__ void w_upper(const Coeffs<T> &wc, T zr, T zi, T &wr, T &wi) {
+    const T x = fabs(zr), y = zi;
+    if (y > T(7) || (x > T(6) && (y > T(0.1) || (x > T(8) && y > T(1e-10)) || x > T(28)))) {
         w_far(zr, y, wr, wi);
         return;
     }
     // Z = (L + iz) / (L - iz); w = 2 p(Z) / (L - iz)^2 + 1 / (sqrt(pi) (L - iz))
-    const double L = wc.L;
-    const double dr = L + zi, di = -zr;
-    double Zr, Zi;
+    const T L = wc.L;
+    const T dr = L + zi, di = -zr;
+    T Zr, Zi;
     cdiv(L - zi, zr, dr, di, Zr, Zi);
-    double pr = 0.0, pi_ = 0.0;
+    T pr = T(0), pi_ = T(0);
 #pragma unroll
     for (int k = 0; k < 32; ++k) {
-        double r, i;
+        T r, i;
         cmul(pr, pi_, Zr, Zi, r, i);
         pr = r + wc.c[k];
         pi_ = i;
     }
-    double d2r, d2i, tr, ti, ir, ii;
+    T d2r, d2i, tr, ti, ir, ii;
     cmul(dr, di, dr, di, d2r, d2i);
-    cdiv(2.0 * pr, 2.0 * pi_, d2r, d2i, tr, ti);
-    cdiv(kInvSqrtPi, 0.0, dr, di, ir, ii);
+    cdiv(T(2) * pr, T(2) * pi_, d2r, d2i, tr, ti);
+    cdiv(T(kInvSqrtPi), T(0), dr, di, ir, ii);
     wr = tr + ir;
     wi = ti + ii;
 }
 
-__device__ void w_any(const WeidemanCoeffs &wc, double zr, double zi, double &wr, double &wi) {
-    if (zi < 0.0) {   // w(z) = 2 exp(-z^2) - w(-z)
-        double ur, ui;
+template <typename T>
+__device__ void w_any(const Coeffs<T> &wc, T zr, T zi, T &wr, T &wi) {
+    if (zi < T(0)) {   // w(z) = 2 exp(-z^2) - w(-z)
+        T ur, ui;
         w_upper(wc, -zr, -zi, ur, ui);
-        const double e = exp(-zr * zr + zi * zi), a = -2.0 * zr * zi;
-        wr = 2.0 * (e * cos(a)) - ur;
-        wi = 2.0 * (e * sin(a)) - ui;
+        const T e = exp(-zr * zr + zi * zi), a = T(-2) * zr * zi;
+        wr = T(2) * (e * cos(a)) - ur;
+        wi = T(2) * (e * sin(a)) - ui;
         return;
     }
     w_upper(wc, zr, zi, wr, wi);
 }
 
-// eta* of the FP64 near-round blend: (8/3 * 64 * 2^-52)^(1/4) (strong_beam.jl:770-783)
-__device__ __forceinline__ double near_round_outer() { return sqrt(sqrt((8.0 / 3.0) * 64.0 * 0x1p-52)); }
+// eta* of the near-round blend: (8/3 C eps)^(1/4) in the working type (strong_beam.jl:779-783)
+template <typename T>
+__device__ __forceinline__ T near_round_outer() {
+    return sqrt(sqrt((T(8) / T(3)) * BeType<T>::cond * BeType<T>::eps));
+}
 
-// radial moments m_k(q) = int_0^1 s^k exp(-q s) ds, k = 0..6: 25-term series for q <= 2, the upward
-// recurrence m_k = (k m_{k-1} - e^-q) / q above (strong_beam.jl:799-833)
-__device__ void moments_0_6(double q, double m[7]) {
-    if (q <= 2.0) {
-        for (int j = 0; j < 7; ++j) m[j] = 0.0;
-        double term = 1.0;
-        for (int k = 0; k <= 24; ++k) {
-            for (int j = 0; j < 7; ++j) m[j] += term / (double)(k + 1 + j);
-            term *= -q / (double)(k + 1);
+// radial moments m_k(q) = int_0^1 s^k exp(-q s) ds, k = 0..6: the series through the type's last order for
+// q <= 2, the upward recurrence m_k = (k m_{k-1} - e^-q) / q above (strong_beam.jl:799-833)
+template <typename T>
+__device__ void moments_0_6(T q, T m[7]) {
+    if (q <= T(2)) {
+        for (int j = 0; j < 7; ++j) m[j] = T(0);
+        T term = T(1);
+        for (int k = 0; k <= BeType<T>::last_order; ++k) {
+            for (int j = 0; j < 7; ++j) m[j] += term / (T)(k + 1 + j);
+            term *= -q / (T)(k + 1);
         }
         return;
     }
-    const double eq = exp(-q);
+    const T eq = exp(-q);
     m[0] = -expm1(-q) / q;
-    for (int j = 1; j < 7; ++j) m[j] = ((double)j * m[j - 1] - eq) / q;
+    for (int j = 1; j < 7; ++j) m[j] = ((T)j * m[j - 1] - eq) / q;
 }
 
 // cubic series in eta about the round beam (strong_beam.jl:878-895)
-__device__ void near_round_kick(double v, double eta, double x, double y, double &kx, double &ky) {
-    const double q = (x * x + y * y) / (2.0 * v), dr = (x * x - y * y) / (2.0 * v);
-    double m[7];
+template <typename T>
+__device__ void near_round_kick(T v, T eta, T x, T y, T &kx, T &ky) {
+    const T q = (x * x + y * y) / (T(2) * v), dr = (x * x - y * y) / (T(2) * v);
+    T m[7];
     moments_0_6(q, m);
-    const double e2 = eta * eta, e3 = e2 * eta;
-    const double c1x = dr * m[2] - m[1], c1y = dr * m[2] + m[1];
-    const double c2x = 1.5 * m[2] + (-dr - q) * m[3] + (dr * dr / 2.0) * m[4];
-    const double c2y = 1.5 * m[2] + (dr - q) * m[3] + (dr * dr / 2.0) * m[4];
-    const double c3x = -1.5 * m[3] + (2.5 * dr + q) * m[4] + (-dr * q - dr * dr / 2.0) * m[5] + dr * dr * dr / 6.0 * m[6];
-    const double c3y = 1.5 * m[3] + (2.5 * dr - q) * m[4] + (-dr * q + dr * dr / 2.0) * m[5] + dr * dr * dr / 6.0 * m[6];
+    const T e2 = eta * eta, e3 = e2 * eta;
+    const T c1x = dr * m[2] - m[1], c1y = dr * m[2] + m[1];
+    const T c2x = T(1.5) * m[2] + (-dr - q) * m[3] + (dr * dr / T(2)) * m[4];
+    const T c2y = T(1.5) * m[2] + (dr - q) * m[3] + (dr * dr / T(2)) * m[4];
+    const T c3x = T(-1.5) * m[3] + (T(2.5) * dr + q) * m[4] + (-dr * q - dr * dr / T(2)) * m[5] + dr * dr * dr / T(6) * m[6];
+    const T c3y = T(1.5) * m[3] + (T(2.5) * dr - q) * m[4] + (-dr * q + dr * dr / T(2)) * m[5] + dr * dr * dr / T(6) * m[6];
     kx = x / v * (m[0] + eta * c1x + e2 * c2x + e3 * c3x);
     ky = y / v * (m[0] + eta * c1y + e2 * c2y + e3 * c3y);
 }
 
-// fifth-degree axis series of one component (strong_beam.jl:974-1007): x times the expansion of
-// Kx / x in x^2 and y^2
-__device__ double axis_component(double sx, double sy, double x, double y) {
-    const double s = sx + sy, sx2 = sx * sx, sx3 = sx2 * sx, sx5 = sx3 * sx2, sy2 = sy * sy, sy3 = sy2 * sy;
-    const double s2 = s * s, s3 = s2 * s;
-    const double j00 = 2.0 / (sx * s);
-    const double j10 = 2.0 * (2.0 * sx + sy) / (3.0 * sx3 * s2);
-    const double j01 = 2.0 / (sx * sy * s2);
-    const double j20 = 2.0 * (8.0 * sx2 + 9.0 * sx * sy + 3.0 * sy2) / (15.0 * sx5 * s3);
-    const double j11 = 2.0 * (3.0 * sx + sy) / (3.0 * sx3 * sy * s3);
-    const double j02 = 2.0 * (sx + 3.0 * sy) / (3.0 * sx * sy3 * s3);
-    const double x2 = x * x, y2 = y * y;
-    const double scale = j00 - (x2 * j10 + y2 * j01) / 2.0 + x2 * x2 * j20 / 8.0 + x2 * y2 * j11 / 4.0 + y2 * y2 * j02 / 8.0;
+// the fifth-degree axis series' coefficients (strong_beam.jl:974-1007)
+template <typename T>
+struct AxisSeries {
+    T j00, j10, j01, j20, j11, j02;
+    __device__ AxisSeries(T sx, T sy) {
+        const T s = sx + sy, sx2 = sx * sx, sx3 = sx2 * sx, sx5 = sx3 * sx2, sy2 = sy * sy, sy3 = sy2 * sy;
+        const T s2 = s * s, s3 = s2 * s;
+        j00 = T(2) / (sx * s);
+        j10 = T(2) * (T(2) * sx + sy) / (T(3) * sx3 * s2);
+        j01 = T(2) / (sx * sy * s2);
+        j20 = T(2) * (T(8) * sx2 + T(9) * sx * sy + T(3) * sy2) / (T(15) * sx5 * s3);
+        j11 = T(2) * (T(3) * sx + sy) / (T(3) * sx3 * sy * s3);
+        j02 = T(2) * (sx + T(3) * sy) / (T(3) * sx * sy3 * s3);
+    }
+};
+
+// one component of the axis series: x times the expansion of Kx / x in x^2 and y^2
+template <typename T>
+__device__ T axis_component(T sx, T sy, T x, T y) {
+    const AxisSeries<T> j(sx, sy);
+    const T x2 = x * x, y2 = y * y;
+    const T scale = j.j00 - (x2 * j.j10 + y2 * j.j01) / T(2) + x2 * x2 * j.j20 / T(8) + x2 * y2 * j.j11 / T(4) +
+                    y2 * y2 * j.j02 / T(8);
     return x * scale;
 }
 
+// the axis series is used where rho^7 <= eps / sqrt(eta) (strong_beam.jl:962-972)
+template <typename T>
+__device__ __forceinline__ bool near_axis(T s1, T s2, T x, T y, T eta) {
+    const T rho2 = x * x / (s1 * s1) + y * y / (s2 * s2);
+    return rho2 * rho2 * rho2 * sqrt(rho2) <= BeType<T>::eps / sqrt(eta);
+}
+
 // the elliptic Bassetti-Erskine form on the principal axes (strong_beam.jl:1016-1038)
-__device__ void elliptic_kick(const WeidemanCoeffs &wc, double s1, double s2, double x, double y, double eta,
-                              double &kx, double &ky) {
-    const double rho2 = x * x / (s1 * s1) + y * y / (s2 * s2);
-    if (rho2 * rho2 * rho2 * sqrt(rho2) <= 0x1p-52 / sqrt(eta)) {
+template <typename T>
+__device__ void elliptic_kick(const Coeffs<T> &wc, T s1, T s2, T x, T y, T eta, T &kx, T &ky) {
+    if (near_axis(s1, s2, x, y, eta)) {
         kx = axis_component(s1, s2, x, y);
         ky = axis_component(s2, s1, y, x);
         return;
     }
-    const bool negx = x < 0.0, negy = y < 0.0;
+    const bool negx = x < T(0), negy = y < T(0);
     x = fabs(x);
     y = fabs(y);
-    const double den = kSqrt2 * sqrt(s1 * s1 - s2 * s2);
-    double w1r, w1i, w2r, w2i;
+    const T den = T(kSqrt2) * sqrt(s1 * s1 - s2 * s2);
+    T w1r, w1i, w2r, w2i;
     w_upper(wc, x / den, y / den, w1r, w1i);
     w_upper(wc, s2 / s1 * x / den, s1 / s2 * y / den, w2r, w2i);
-    const double A = 2.0 * kSqrtPi / den;
-    const double B = exp(-x * x / (2.0 * s1 * s1) - y * y / (2.0 * s2 * s2));
-    const double rr = A * (w1r - B * w2r), ri = A * (w1i - B * w2i);
+    const T A = T(2) * T(kSqrtPi) / den;
+    const T B = exp(-x * x / (T(2) * s1 * s1) - y * y / (T(2) * s2 * s2));
+    const T rr = A * (w1r - B * w2r), ri = A * (w1i - B * w2i);
     kx = negx ? -ri : ri;
     ky = negy ? -rr : rr;
 }
 
 // sig1 >= sig2 (strong_beam.jl:1040-1059)
-__device__ void principal_kick(const WeidemanCoeffs &wc, double s1, double s2, double x, double y, double &kx,
-                               double &ky) {
-    const double v = (s1 * s1 + s2 * s2) / 2.0;
-    const double eta = (s1 * s1 - s2 * s2) / (2.0 * v);
-    if (eta == 0.0) {
-        const double r2 = x * x + y * y, u = r2 / (2.0 * v);
-        const double phi = u == 0.0 ? 1.0 : -expm1(-u) / u;
+template <typename T>
+__device__ void principal_kick(const Coeffs<T> &wc, T s1, T s2, T x, T y, T &kx, T &ky) {
+    const T v = (s1 * s1 + s2 * s2) / T(2);
+    const T eta = (s1 * s1 - s2 * s2) / (T(2) * v);
+    if (eta == T(0)) {
+        const T r2 = x * x + y * y, u = r2 / (T(2) * v);
+        const T phi = u == T(0) ? T(1) : -expm1(-u) / u;
         kx = phi / v * x;
         ky = phi / v * y;
         return;
     }
-    const double outer = near_round_outer(), inner = outer / 2.0;
+    const T outer = near_round_outer<T>(), inner = outer / T(2);
     if (eta <= inner) {
         near_round_kick(v, eta, x, y, kx, ky);
         return;
     }
-    double ex, ey;
+    T ex, ey;
     elliptic_kick(wc, s1, s2, x, y, eta, ex, ey);
     if (eta >= outer) {
         kx = ex; ky = ey;
         return;
     }
-    const double t = (eta - inner) * (1.0 / (outer - inner));
-    const double w = t * t * t * (10.0 + t * (-15.0 + 6.0 * t));
-    double sx, sy;
+    const T t = (eta - inner) * (T(1) / (outer - inner));
+    const T w = t * t * t * (T(10) + t * (T(-15) + T(6) * t));
+    T sx, sy;
     near_round_kick(v, eta, x, y, sx, sy);
     kx = sx + w * (ex - sx);
     ky = sy + w * (ey - sy);
 }
 
+template <typename T>
+__device__ void beambeam_kick(const Coeffs<T> &wc, T sigx, T sigy, T x, T y, T &kx, T &ky) {
+    if (sigx >= sigy) principal_kick(wc, sigx, sigy, x, y, kx, ky);
+    else principal_kick(wc, sigy, sigx, y, x, ky, kx);
+}
+
 // ---- the kick with the diagonal of the potential's Hessian (_gaussian_beambeam_kick_response,
-// strong_beam.jl:1061-1128).  K is normalised; h1, h2 carry the factor kbb.  Used by the Gaussian-subtracted
-// solver's add-back, whose pz term needs d(potential)/d(sigma^2) = Hessian / 2 (heat equation), with the
-// blend's chain term where the near-round series and the elliptic form are blended (:1095-1115).
+// strong_beam.jl:1061-1128).  K is normalised; h1, h2 carry the factor kbb.  The Gaussian-subtracted solver's
+// add-back and the soft-Gaussian map take their pz terms from it: d(potential)/d(sigma^2) = Hessian / 2 (heat
+// equation), with the blend's chain term where the near-round series and the elliptic form are blended
+// (:1095-1115).
 
 // round beam Hessian (strong_beam.jl:711-772): f = 2(1 - e)/r^2, fp = df/d(r^2), series below u = 0.01
-__device__ void round_hessian(double kbb, double sigma, double x, double y, double expterm, double &hxx, double &hyy) {
-    const double r2 = x * x + y * y, sigma2 = sigma * sigma;
-    if (r2 == 0.0) { hxx = hyy = -kbb / sigma2; return; }
-    const double u = r2 / (2.0 * sigma2);
-    double f, fp;
-    if (u < 1.0e-2) {
-        const double phi = 1.0 + u * (-1.0 / 2 + u * (1.0 / 6 + u * (-1.0 / 24 + u * (1.0 / 120 + u * (-1.0 / 720 + u * (1.0 / 5040))))));
-        const double dphi =
-            -1.0 / 2 + u * (1.0 / 3 + u * (-1.0 / 8 + u * (1.0 / 30 + u * (-1.0 / 144 + u * (1.0 / 840 - u * (1.0 / 5760))))));
-        const double inv = 1.0 / sigma2;
+template <typename T>
+__device__ void round_hessian(T kbb, T sigma, T x, T y, T expterm, T &hxx, T &hyy) {
+    const T r2 = x * x + y * y, sigma2 = sigma * sigma;
+    if (r2 == T(0)) { hxx = hyy = -kbb / sigma2; return; }
+    const T u = r2 / (T(2) * sigma2);
+    T f, fp;
+    if (u < T(1.0e-2)) {
+        const T phi = T(1) + u * (T(-1) / T(2) + u * (T(1) / T(6) + u * (T(-1) / T(24) + u * (T(1) / T(120) +
+                      u * (T(-1) / T(720) + u * (T(1) / T(5040)))))));
+        const T dphi = T(-1) / T(2) + u * (T(1) / T(3) + u * (T(-1) / T(8) + u * (T(1) / T(30) + u * (T(-1) / T(144) +
+                       u * (T(1) / T(840) - u * (T(1) / T(5760)))))));
+        const T inv = T(1) / sigma2;
         f = phi * inv;
-        fp = dphi * inv * inv / 2.0;
+        fp = dphi * inv * inv / T(2);
     } else {
-        const double ome = 1.0 - expterm;
-        f = 2.0 * ome / r2;
-        fp = 2.0 * (expterm * u - ome) / (r2 * r2);
+        const T ome = T(1) - expterm;
+        f = T(2) * ome / r2;
+        fp = T(2) * (expterm * u - ome) / (r2 * r2);
     }
-    hxx = -kbb * (f + 2.0 * x * x * fp);
-    hyy = -kbb * (f + 2.0 * y * y * fp);
+    hxx = -kbb * (f + T(2) * x * x * fp);
+    hyy = -kbb * (f + T(2) * y * y * fp);
 }
 
 // near-round series with its Hessian diagonal (strong_beam.jl:897-937)
-__device__ void near_round_response(double kbb, double v, double eta, double x, double y, double &kx, double &ky,
-                                    double &h1, double &h2) {
-    const double q = (x * x + y * y) / (2.0 * v), dr = (x * x - y * y) / (2.0 * v);
-    double m[7];
+template <typename T>
+__device__ void near_round_response(T kbb, T v, T eta, T x, T y, T &kx, T &ky, T &h1, T &h2) {
+    const T q = (x * x + y * y) / (T(2) * v), dr = (x * x - y * y) / (T(2) * v);
+    T m[7];
     moments_0_6(q, m);
-    const double e2 = eta * eta, e3 = e2 * eta;
-    const double u1 = dr * m[1];
-    const double u2 = m[1] / 2.0 - q * m[2] + (dr * dr / 2.0) * m[3];
-    const double u3 = 1.5 * dr * m[3] - dr * q * m[4] + dr * dr * dr / 6.0 * m[5];
-    const double v1 = dr * (q * m[2] - m[1]);
-    const double v2 = 1.5 * q * m[2] - (q * q + dr * dr) * m[3] + (q * dr * dr / 2.0) * m[4];
-    const double v3 = -1.5 * dr * m[3] + 3.5 * dr * q * m[4] - (dr * q * q + dr * dr * dr / 2.0) * m[5] + dr * dr * dr * q / 6.0 * m[6];
-    const double c1x = dr * m[2] - m[1], c1y = dr * m[2] + m[1];
-    const double c2x = 1.5 * m[2] + (-dr - q) * m[3] + (dr * dr / 2.0) * m[4];
-    const double c2y = 1.5 * m[2] + (dr - q) * m[3] + (dr * dr / 2.0) * m[4];
-    const double c3x = -1.5 * m[3] + (2.5 * dr + q) * m[4] + (-dr * q - dr * dr / 2.0) * m[5] + dr * dr * dr / 6.0 * m[6];
-    const double c3y = 1.5 * m[3] + (2.5 * dr - q) * m[4] + (-dr * q + dr * dr / 2.0) * m[5] + dr * dr * dr / 6.0 * m[6];
+    const T e2 = eta * eta, e3 = e2 * eta;
+    const T u1 = dr * m[1];
+    const T u2 = m[1] / T(2) - q * m[2] + (dr * dr / T(2)) * m[3];
+    const T u3 = T(1.5) * dr * m[3] - dr * q * m[4] + dr * dr * dr / T(6) * m[5];
+    const T v1 = dr * (q * m[2] - m[1]);
+    const T v2 = T(1.5) * q * m[2] - (q * q + dr * dr) * m[3] + (q * dr * dr / T(2)) * m[4];
+    const T v3 = T(-1.5) * dr * m[3] + T(3.5) * dr * q * m[4] - (dr * q * q + dr * dr * dr / T(2)) * m[5] +
+                 dr * dr * dr * q / T(6) * m[6];
+    const T c1x = dr * m[2] - m[1], c1y = dr * m[2] + m[1];
+    const T c2x = T(1.5) * m[2] + (-dr - q) * m[3] + (dr * dr / T(2)) * m[4];
+    const T c2y = T(1.5) * m[2] + (dr - q) * m[3] + (dr * dr / T(2)) * m[4];
+    const T c3x = T(-1.5) * m[3] + (T(2.5) * dr + q) * m[4] + (-dr * q - dr * dr / T(2)) * m[5] + dr * dr * dr / T(6) * m[6];
+    const T c3y = T(1.5) * m[3] + (T(2.5) * dr - q) * m[4] + (-dr * q + dr * dr / T(2)) * m[5] + dr * dr * dr / T(6) * m[6];
     kx = x / v * (m[0] + eta * c1x + e2 * c2x + e3 * c3x);
     ky = y / v * (m[0] + eta * c1y + e2 * c2y + e3 * c3y);
-    const double uv = -exp(-q) + eta * v1 + e2 * v2 + e3 * v3;
-    const double ueta = u1 + 2.0 * eta * u2 + 3.0 * e2 * u3;
-    h1 = kbb / v * (uv + (1.0 - eta) * ueta);
-    h2 = kbb / v * (uv - (1.0 + eta) * ueta);
+    const T uv = -exp(-q) + eta * v1 + e2 * v2 + e3 * v3;
+    const T ueta = u1 + T(2) * eta * u2 + T(3) * e2 * u3;
+    h1 = kbb / v * (uv + (T(1) - eta) * ueta);
+    h2 = kbb / v * (uv - (T(1) + eta) * ueta);
 }
 
 // U_exact - U_series through eta^6 (strong_beam.jl:939-960), from the radial moments m3..m11
-__device__ double near_round_potential_residual(double v, double eta, double x, double y) {
-    const double q = (x * x + y * y) / (2.0 * v), dr = (x * x - y * y) / (2.0 * v);
-    double m[12];
-    if (q <= 2.0) {
-        for (int j = 3; j < 12; ++j) m[j] = 0.0;
-        double term = 1.0;
-        for (int k = 0; k <= 24; ++k) {
-            for (int j = 3; j < 12; ++j) m[j] += term / (double)(k + 1 + j);
-            term *= -q / (double)(k + 1);
+template <typename T>
+__device__ T near_round_potential_residual(T v, T eta, T x, T y) {
+    const T q = (x * x + y * y) / (T(2) * v), dr = (x * x - y * y) / (T(2) * v);
+    T m[12];
+    if (q <= T(2)) {
+        for (int j = 3; j < 12; ++j) m[j] = T(0);
+        T term = T(1);
+        for (int k = 0; k <= BeType<T>::last_order; ++k) {
+            for (int j = 3; j < 12; ++j) m[j] += term / (T)(k + 1 + j);
+            term *= -q / (T)(k + 1);
         }
     } else {
-        const double eq = exp(-q);
-        double prev = -expm1(-q) / q;
+        const T eq = exp(-q);
+        T prev = -expm1(-q) / q;
         for (int j = 1; j < 12; ++j) {
-            prev = ((double)j * prev - eq) / q;
+            prev = ((T)j * prev - eq) / q;
             m[j] = prev;
         }
     }
-    const double dr2 = dr * dr, q2 = q * q;
-    const double u4 = 3.0 / 8.0 * m[3] - 1.5 * q * m[4] + (5.0 * dr2 + 2.0 * q2) / 4.0 * m[5] - dr2 * q / 2.0 * m[6] +
-                      dr2 * dr2 / 24.0 * m[7];
-    const double u5 = 15.0 / 8.0 * dr * m[5] - 2.5 * dr * q * m[6] + dr * (7.0 * dr2 + 6.0 * q2) / 12.0 * m[7] -
-                      dr * dr2 * q / 6.0 * m[8] + dr * dr2 * dr2 / 120.0 * m[9];
-    const double u6 = 5.0 / 16.0 * m[5] - 15.0 / 8.0 * q * m[6] + 5.0 * (7.0 * dr2 + 4.0 * q2) / 16.0 * m[7] -
-                      q * (21.0 * dr2 + 2.0 * q2) / 12.0 * m[8] + dr2 * (3.0 * dr2 + 4.0 * q2) / 16.0 * m[9] -
-                      dr2 * dr2 * q / 24.0 * m[10] + dr2 * dr2 * dr2 / 720.0 * m[11];
+    const T dr2 = dr * dr, q2 = q * q;
+    const T u4 = T(3) / T(8) * m[3] - T(1.5) * q * m[4] + (T(5) * dr2 + T(2) * q2) / T(4) * m[5] - dr2 * q / T(2) * m[6] +
+                 dr2 * dr2 / T(24) * m[7];
+    const T u5 = T(15) / T(8) * dr * m[5] - T(2.5) * dr * q * m[6] + dr * (T(7) * dr2 + T(6) * q2) / T(12) * m[7] -
+                 dr * dr2 * q / T(6) * m[8] + dr * dr2 * dr2 / T(120) * m[9];
+    const T u6 = T(5) / T(16) * m[5] - T(15) / T(8) * q * m[6] + T(5) * (T(7) * dr2 + T(4) * q2) / T(16) * m[7] -
+                 q * (T(21) * dr2 + T(2) * q2) / T(12) * m[8] + dr2 * (T(3) * dr2 + T(4) * q2) / T(16) * m[9] -
+                 dr2 * dr2 * q / T(24) * m[10] + dr2 * dr2 * dr2 / T(720) * m[11];
     return eta * eta * eta * eta * (u4 + eta * (u5 + eta * u6));
 }
 
 // d/dx of the axis series component (strong_beam.jl:974-1007), times -kbb
-__device__ double axis_hessian(double kbb, double sx, double sy, double x, double y) {
-    const double s = sx + sy, sx2 = sx * sx, sx3 = sx2 * sx, sx5 = sx3 * sx2, sy2 = sy * sy, sy3 = sy2 * sy;
-    const double s2 = s * s, s3 = s2 * s;
-    const double j00 = 2.0 / (sx * s);
-    const double j10 = 2.0 * (2.0 * sx + sy) / (3.0 * sx3 * s2);
-    const double j01 = 2.0 / (sx * sy * s2);
-    const double j20 = 2.0 * (8.0 * sx2 + 9.0 * sx * sy + 3.0 * sy2) / (15.0 * sx5 * s3);
-    const double j11 = 2.0 * (3.0 * sx + sy) / (3.0 * sx3 * sy * s3);
-    const double j02 = 2.0 * (sx + 3.0 * sy) / (3.0 * sx * sy3 * s3);
-    const double x2 = x * x, y2 = y * y;
-    const double derivative = j00 - 1.5 * x2 * j10 - y2 * j01 / 2.0 + 0.625 * x2 * x2 * j20 + 0.75 * x2 * y2 * j11 + y2 * y2 * j02 / 8.0;
+template <typename T>
+__device__ T axis_hessian(T kbb, T sx, T sy, T x, T y) {
+    const AxisSeries<T> j(sx, sy);
+    const T x2 = x * x, y2 = y * y;
+    const T derivative = j.j00 - T(1.5) * x2 * j.j10 - y2 * j.j01 / T(2) + T(0.625) * x2 * x2 * j.j20 +
+                         T(0.75) * x2 * y2 * j.j11 + y2 * y2 * j.j02 / T(8);
     return -kbb * derivative;
 }
 
 // sig1 >= sig2 (strong_beam.jl:1061-1117)
-__device__ void principal_response(const WeidemanCoeffs &wc, double kbb, double s1, double s2, double x, double y,
-                                   double &kx, double &ky, double &h1, double &h2) {
-    const double v = (s1 * s1 + s2 * s2) / 2.0;
-    const double eta = (s1 * s1 - s2 * s2) / (2.0 * v);
-    if (eta == 0.0) {
-        const double r2 = x * x + y * y, u = r2 / (2.0 * v);
-        const double scale = (u == 0.0 ? 1.0 : -expm1(-u) / u) / v;
+template <typename T>
+__device__ void principal_response(const Coeffs<T> &wc, T kbb, T s1, T s2, T x, T y, T &kx, T &ky, T &h1, T &h2) {
+    const T v = (s1 * s1 + s2 * s2) / T(2);
+    const T eta = (s1 * s1 - s2 * s2) / (T(2) * v);
+    if (eta == T(0)) {
+        const T r2 = x * x + y * y, u = r2 / (T(2) * v);
+        const T scale = (u == T(0) ? T(1) : -expm1(-u) / u) / v;
         kx = scale * x;
         ky = scale * y;
-        round_hessian(kbb, sqrt(v), x, y, exp(-r2 / (2.0 * v)), h1, h2);
+        round_hessian(kbb, sqrt(v), x, y, exp(-r2 / (T(2) * v)), h1, h2);
         return;
     }
-    const double outer = near_round_outer(), inner = outer / 2.0;
+    const T outer = near_round_outer<T>(), inner = outer / T(2);
     if (eta <= inner) {
         near_round_response(kbb, v, eta, x, y, kx, ky, h1, h2);
         return;
     }
-    double ex, ey;
+    T ex, ey;
     elliptic_kick(wc, s1, s2, x, y, eta, ex, ey);
-    const double expterm = exp(-0.5 * (x * x / (s1 * s1) + y * y / (s2 * s2)));
-    double he1, he2;
-    const double rho2 = x * x / (s1 * s1) + y * y / (s2 * s2);
-    if (rho2 * rho2 * rho2 * sqrt(rho2) <= 0x1p-52 / sqrt(eta)) {   // the axis series' Hessian (:1009-1014)
+    const T expterm = exp(T(-0.5) * (x * x / (s1 * s1) + y * y / (s2 * s2)));
+    T he1, he2;
+    if (near_axis(s1, s2, x, y, eta)) {   // the axis series' Hessian (:1009-1014)
         he1 = axis_hessian(kbb, s1, s2, x, y);
         he2 = axis_hessian(kbb, s2, s1, y, x);
     } else {   // _elliptic_gaussian_hessian_diagonal (:695-703)
-        const double t1 = kbb * (x * ex + y * ey), t2 = s1 * s1 - s2 * s2, t3 = s2 / s1;
-        he1 = (t1 - 2.0 * kbb * (1.0 - expterm * t3)) / t2;
-        he2 = (-t1 + 2.0 * kbb * (1.0 - expterm / t3)) / t2;
+        const T t1 = kbb * (x * ex + y * ey), t2 = s1 * s1 - s2 * s2, t3 = s2 / s1;
+        he1 = (t1 - T(2) * kbb * (T(1) - expterm * t3)) / t2;
+        he2 = (-t1 + T(2) * kbb * (T(1) - expterm / t3)) / t2;
     }
     if (eta >= outer) {
         kx = ex; ky = ey; h1 = he1; h2 = he2;
         return;
     }
-    const double invw = 1.0 / (outer - inner), t = (eta - inner) * invw;
-    const double w = t * t * t * (10.0 + t * (-15.0 + 6.0 * t));
-    const double dw = 30.0 * t * t * (t - 1.0) * (t - 1.0) * invw;
-    double sx, sy, hs1, hs2;
+    const T invw = T(1) / (outer - inner), t = (eta - inner) * invw;
+    const T w = t * t * t * (T(10) + t * (T(-15) + T(6) * t));
+    const T dw = T(30) * t * t * (t - T(1)) * (t - T(1)) * invw;
+    T sx, sy, hs1, hs2;
     near_round_response(kbb, v, eta, x, y, sx, sy, hs1, hs2);
-    const double chain = kbb * dw * near_round_potential_residual(v, eta, x, y) / v;
+    const T chain = kbb * dw * near_round_potential_residual(v, eta, x, y) / v;
     kx = sx + w * (ex - sx);
     ky = sy + w * (ey - sy);
-    h1 = hs1 + w * (he1 - hs1) + (1.0 - eta) * chain;
-    h2 = hs2 + w * (he2 - hs2) - (1.0 + eta) * chain;
+    h1 = hs1 + w * (he1 - hs1) + (T(1) - eta) * chain;
+    h2 = hs2 + w * (he2 - hs2) - (T(1) + eta) * chain;
 }
 
-__device__ void beambeam_response(const WeidemanCoeffs &wc, double kbb, double sigx, double sigy, double x, double y,
-                                  double &kx, double &ky, double &hx, double &hy) {
+template <typename T>
+__device__ void beambeam_response(const Coeffs<T> &wc, T kbb, T sigx, T sigy, T x, T y, T &kx, T &ky, T &hx, T &hy) {
     if (sigx >= sigy) principal_response(wc, kbb, sigx, sigy, x, y, kx, ky, hx, hy);
     else principal_response(wc, kbb, sigy, sigx, y, x, ky, kx, hy, hx);
 }
 
-__global__ __launch_bounds__(256) void k_be_response(WeidemanCoeffs wc, const double *X, const double *Y, int64_t n,
-                                                     double kbb, double sigx, double sigy, double *OUT) {
+// FP32 evaluation on the scale of the larger rms size l: the kick is homogeneous of degree -1 and the
+// Hessian of degree -2 in (sigx, sigy, x, y), so K = K(sigx/l, sigy/l, x/l, y/l) / l and H likewise / l^2.
+// The series coefficients of the unscaled form reach sig^8 (about 1e-40 for sizes of 1e-5 m), beyond the
+// FP32 normal range; scaled, every intermediate is of order one.  (The FP64 path evaluates unscaled.)
+__device__ __forceinline__ void scaled_kick_f32(const WeidemanCoeffsF &wc, float sigx, float sigy, float x, float y,
+                                                float &kx, float &ky) {
+    const float l = fmaxf(sigx, sigy), il = 1.0f / l;
+    beambeam_kick(wc, sigx * il, sigy * il, x * il, y * il, kx, ky);
+    kx *= il;
+    ky *= il;
+}
+
+__device__ __forceinline__ void scaled_response_f32(const WeidemanCoeffsF &wc, float kbb, float sigx, float sigy,
+                                                    float x, float y, float &kx, float &ky, float &hx, float &hy) {
+    const float l = fmaxf(sigx, sigy), il = 1.0f / l;
+    beambeam_response(wc, kbb, sigx * il, sigy * il, x * il, y * il, kx, ky, hx, hy);
+    kx *= il;
+    ky *= il;
+    hx *= il * il;
+    hy *= il * il;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_be_response(Coeffs<T> wc, const T *X, const T *Y, int64_t n, T kbb, T sigx,
+                                                     T sigy, T *OUT) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double kx, ky, hx, hy;
-    beambeam_response(wc, kbb, sigx, sigy, X[i], Y[i], kx, ky, hx, hy);
+    T kx, ky, hx, hy;
+    if constexpr (sizeof(T) == 8) beambeam_response(wc, kbb, sigx, sigy, X[i], Y[i], kx, ky, hx, hy);
+    else scaled_response_f32(wc, kbb, sigx, sigy, X[i], Y[i], kx, ky, hx, hy);
     OUT[4 * i] = kx; OUT[4 * i + 1] = ky; OUT[4 * i + 2] = hx; OUT[4 * i + 3] = hy;
 }
 
@@ -415,10 +488,8 @@ __global__ __launch_bounds__(256) void k_gpic_addback(WeidemanCoeffs wc, GpicAdd
         beambeam_response(wc, kbb_eff, it.g[0].sigx, it.g[0].sigy, x - it.g[0].mux, y - it.g[0].muy, kxL, kyL, hxL, hyL);
         beambeam_response(wc, kbb_eff, it.g[1].sigx, it.g[1].sigy, x - it.g[1].mux, y - it.g[1].muy, kxR, kyR, hxR, hyR);
     } else {
-        if (it.g[0].sigx >= it.g[0].sigy) principal_kick(wc, it.g[0].sigx, it.g[0].sigy, x - it.g[0].mux, y - it.g[0].muy, kxL, kyL);
-        else principal_kick(wc, it.g[0].sigy, it.g[0].sigx, y - it.g[0].muy, x - it.g[0].mux, kyL, kxL);
-        if (it.g[1].sigx >= it.g[1].sigy) principal_kick(wc, it.g[1].sigx, it.g[1].sigy, x - it.g[1].mux, y - it.g[1].muy, kxR, kyR);
-        else principal_kick(wc, it.g[1].sigy, it.g[1].sigx, y - it.g[1].muy, x - it.g[1].mux, kyR, kxR);
+        beambeam_kick(wc, it.g[0].sigx, it.g[0].sigy, x - it.g[0].mux, y - it.g[0].muy, kxL, kyL);
+        beambeam_kick(wc, it.g[1].sigx, it.g[1].sigy, x - it.g[1].mux, y - it.g[1].muy, kxR, kyR);
     }
     const double Kxa = half_ns * (zL * kxL + zR * kxR), Kya = half_ns * (zL * kyL + zR * kyR);
     const double dpxa = ks * Kxa, dpya = ks * Kya;
@@ -492,7 +563,10 @@ __device__ __forceinline__ void soft_reverse_drift(int d, double &x, double px, 
     }
 }
 
-__device__ __forceinline__ double soft_kick_one(const WeidemanCoeffs &wc, double &x, double &px, double &y, double &py,
+// TB: the working type of the Bassetti-Erskine evaluation (the field, its Hessian and the density); the drifts,
+// the moment transport and the coordinate updates are FP64 either way.
+template <typename TB>
+__device__ __forceinline__ double soft_kick_one(const Coeffs<TB> &wc, double &x, double &px, double &y, double &py,
                                                 double &z, double &pz, const SoftMoments &m, double center, double kbb,
                                                 int drift, int lkick) {
     const int d = lkick ? drift : OCTOPIC_DRIFT_UNSAFE_PARAXIAL_FROZEN_LONGITUDINAL;   // _soft_gaussian_drift
@@ -507,9 +581,21 @@ __device__ __forceinline__ double soft_kick_one(const WeidemanCoeffs &wc, double
     double density = 0.0;
     if (!(a <= 0 || dd <= 0)) {
         const double sigx = sqrt(a), sigy = sqrt(dd);
-        double kx, ky, hx, hy;
-        beambeam_response(wc, kbb, sigx, sigy, xx, yy, kx, ky, hx, hy);
-        const double expterm = exp(-0.5 * (xx * xx / (sigx * sigx) + yy * yy / (sigy * sigy)));
+        double kx, ky, hx, hy, expterm;
+        if constexpr (sizeof(TB) == 8) {
+            beambeam_response(wc, kbb, sigx, sigy, xx, yy, kx, ky, hx, hy);
+            expterm = exp(-0.5 * (xx * xx / (sigx * sigx) + yy * yy / (sigy * sigy)));
+        } else {   // scaled by the larger size in FP64 before the rounding to FP32 (see scaled_response_f32)
+            const double l = fmax(sigx, sigy), il = 1.0 / l;
+            const float fx = (float)(xx * il), fy = (float)(yy * il), fsx = (float)(sigx * il), fsy = (float)(sigy * il);
+            float k1, k2, h1, h2;
+            beambeam_response(wc, (float)kbb, fsx, fsy, fx, fy, k1, k2, h1, h2);
+            kx = (double)k1 * il;
+            ky = (double)k2 * il;
+            hx = (double)h1 * (il * il);
+            hy = (double)h2 * (il * il);
+            expterm = (double)exp(-0.5f * (fx * fx / (fsx * fsx) + fy * fy / (fsy * fsy)));
+        }
         px += kx * kbb;
         py += ky * kbb;
         pz += 0.25 * (hx * au + hy * du);
@@ -532,7 +618,8 @@ __device__ __forceinline__ bool finite_all(const SoftMoments &m) {   // _gaussia
 // multiply-adds, the min_sigma floor, ignore_centroid), and the block's share of the field slice's
 // particles goes through the map in place.  The sampled direction's block sums of density x 2 pi (fixed
 // tree) go to lum_block[p][b] for an ordered fold at the end of the collide.
-__global__ __launch_bounds__(256) void k_soft_kick(WeidemanCoeffs wc, SoftKickArgs a) {
+template <typename TB>
+__global__ __launch_bounds__(256) void k_soft_kick(Coeffs<TB> wc, SoftKickArgs a) {
     __shared__ double sm[10];
     __shared__ SoftMoments mom;
     __shared__ int ok;
@@ -599,7 +686,7 @@ __global__ __launch_bounds__(256) void k_soft_kick(WeidemanCoeffs wc, SoftKickAr
                *PY = a.st.c[fb][3] + foff, *Z = a.st.c[fb][4] + foff, *PZ = a.st.c[fb][5] + foff;
         for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += (int64_t)gridDim.x * blockDim.x) {
             double x = X[t], px = PX[t], y = Y[t], py = PY[t], z = Z[t], pz = PZ[t];
-            const double dens = soft_kick_one(wc, x, px, y, py, z, pz, m, center, kbb, a.drift, a.lkick);
+            const double dens = soft_kick_one<TB>(wc, x, px, y, py, z, pz, m, center, kbb, a.drift, a.lkick);
             if (want) acc += dens;
             X[t] = x; PX[t] = px; Y[t] = y; PY[t] = py; Z[t] = z; PZ[t] = pz;
         }
@@ -628,25 +715,26 @@ __global__ __launch_bounds__(256) void k_soft_lum_fold(const double *lum_block, 
     lum_parts[p] = s / kTwoPi * lum_scale[p];
 }
 
-__global__ __launch_bounds__(256) void k_be_kick(WeidemanCoeffs wc, const double *X, const double *Y, int64_t n,
-                                                 double sigx, double sigy, double *KX, double *KY) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_be_kick(Coeffs<T> wc, const T *X, const T *Y, int64_t n, T sigx, T sigy, T *KX,
+                                                 T *KY) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double x = X[i], y = Y[i];
-    double kx = 0.0, ky = 0.0;
-    if (sigx != 0.0 && sigy != 0.0) {
-        if (sigx >= sigy) principal_kick(wc, sigx, sigy, x, y, kx, ky);
-        else principal_kick(wc, sigy, sigx, y, x, ky, kx);
+    const T x = X[i], y = Y[i];
+    T kx = T(0), ky = T(0);
+    if (sigx != T(0) && sigy != T(0)) {
+        if constexpr (sizeof(T) == 8) beambeam_kick(wc, sigx, sigy, x, y, kx, ky);
+        else scaled_kick_f32(wc, sigx, sigy, x, y, kx, ky);
     }
     KX[i] = kx;
     KY[i] = ky;
 }
 
-__global__ __launch_bounds__(256) void k_faddeeva(WeidemanCoeffs wc, const double *ZR, const double *ZI, int64_t n,
-                                                  double *WR, double *WI) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_faddeeva(Coeffs<T> wc, const T *ZR, const T *ZI, int64_t n, T *WR, T *WI) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double wr, wi;
+    T wr, wi;
     w_any(wc, ZR[i], ZI[i], wr, wi);
     WR[i] = wr;
     WI[i] = wi;
@@ -681,21 +769,53 @@ const WeidemanCoeffs &weideman_coeffs() {
     return wc;
 }
 
-hipError_t launch_be_kick(const double *x, const double *y, int64_t n, double sigx, double sigy, double *kx, double *ky,
-                          hipStream_t s) {
+const WeidemanCoeffsF &weideman_coeffs_f32() {
+    static WeidemanCoeffsF wf;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const WeidemanCoeffs &wc = weideman_coeffs();
+        for (int k = 0; k < 32; ++k) wf.c[k] = (float)wc.c[k];   // T(c) of the FP64 constants
+        wf.L = (float)wc.L;
+    });
+    return wf;
+}
+
+namespace {
+template <typename T> const Coeffs<T> &coeffs();
+template <> const WeidemanCoeffs &coeffs<double>() { return weideman_coeffs(); }
+template <> const WeidemanCoeffsF &coeffs<float>() { return weideman_coeffs_f32(); }
+}  // namespace
+
+template <typename T>
+hipError_t launch_be_kick(const T *x, const T *y, int64_t n, T sigx, T sigy, T *kx, T *ky, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_be_kick, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, weideman_coeffs(), x, y, n, sigx,
-                       sigy, kx, ky);
+    hipLaunchKernelGGL(k_be_kick<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, coeffs<T>(), x, y, n, sigx, sigy,
+                       kx, ky);
     return hipGetLastError();
 }
 
-hipError_t launch_be_response(const double *x, const double *y, int64_t n, double kbb, double sigx, double sigy,
-                              double *out4, hipStream_t s) {
+template <typename T>
+hipError_t launch_be_response(const T *x, const T *y, int64_t n, T kbb, T sigx, T sigy, T *out4, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_be_response, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, weideman_coeffs(), x, y, n, kbb,
+    hipLaunchKernelGGL(k_be_response<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, coeffs<T>(), x, y, n, kbb,
                        sigx, sigy, out4);
     return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_faddeeva(const T *zr, const T *zi, int64_t n, T *wr, T *wi, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_faddeeva<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, coeffs<T>(), zr, zi, n, wr, wi);
+    return hipGetLastError();
+}
+
+#define OCTOPIC_BE_INSTANTIATE(T)                                                                                     \
+    template hipError_t launch_be_kick<T>(const T *, const T *, int64_t, T, T, T *, T *, hipStream_t);                \
+    template hipError_t launch_be_response<T>(const T *, const T *, int64_t, T, T, T, T *, hipStream_t);              \
+    template hipError_t launch_faddeeva<T>(const T *, const T *, int64_t, T *, T *, hipStream_t);
+OCTOPIC_BE_INSTANTIATE(double)
+OCTOPIC_BE_INSTANTIATE(float)
+#undef OCTOPIC_BE_INSTANTIATE
 
 hipError_t launch_gpic_addback(const GpicAddbackArgs &a, int64_t max_fld, hipStream_t s) {
     if (a.ninters <= 0 || max_fld <= 0) return hipSuccess;
@@ -704,9 +824,10 @@ hipError_t launch_gpic_addback(const GpicAddbackArgs &a, int64_t max_fld, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_soft_kick(const SoftKickArgs &a, hipStream_t s) {
+hipError_t launch_soft_kick(const SoftKickArgs &a, bool fp32, hipStream_t s) {
     if (a.npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_soft_kick, dim3(kSoftBlocks, 2 * a.npairs), dim3(256), 0, s, weideman_coeffs(), a);
+    if (fp32) hipLaunchKernelGGL(k_soft_kick<float>, dim3(kSoftBlocks, 2 * a.npairs), dim3(256), 0, s, weideman_coeffs_f32(), a);
+    else hipLaunchKernelGGL(k_soft_kick<double>, dim3(kSoftBlocks, 2 * a.npairs), dim3(256), 0, s, weideman_coeffs(), a);
     return hipGetLastError();
 }
 
@@ -714,13 +835,6 @@ hipError_t launch_soft_lum_fold(const double *lum_block, const double *lum_scale
                                 hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_soft_lum_fold, dim3((npairs + 255) / 256), dim3(256), 0, s, lum_block, lum_scale, npairs, lum_parts);
-    return hipGetLastError();
-}
-
-hipError_t launch_faddeeva(const double *zr, const double *zi, int64_t n, double *wr, double *wi, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_faddeeva, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, weideman_coeffs(), zr, zi, n, wr,
-                       wi);
     return hipGetLastError();
 }
 

@@ -155,12 +155,20 @@ struct WeidemanCoeffs {
     double c[32];   // polynomial coefficients, highest power first
     double L;       // sqrt(32 / sqrt 2)
 };
+struct WeidemanCoeffsF {   // the same, rounded to FP32 (T(c), SpecialMath.jl:98-100)
+    float c[32];
+    float L;
+};
 const WeidemanCoeffs &weideman_coeffs();
-hipError_t launch_be_kick(const double *x, const double *y, int64_t n, double sigx, double sigy, double *kx, double *ky,
-                          hipStream_t s);
+const WeidemanCoeffsF &weideman_coeffs_f32();
+// T = double or float: the FP64 evaluation or the FP32 one with the reference's Float32 constants
+template <typename T>
+hipError_t launch_be_kick(const T *x, const T *y, int64_t n, T sigx, T sigy, T *kx, T *ky, hipStream_t s);
 // the kick and kbb times the Hessian diagonal, (kx, ky, hxx, hyy) per point (strong_beam.jl:1119-1128)
-hipError_t launch_be_response(const double *x, const double *y, int64_t n, double kbb, double sigx, double sigy,
-                              double *out4, hipStream_t s);
+template <typename T>
+hipError_t launch_be_response(const T *x, const T *y, int64_t n, T kbb, T sigx, T sigy, T *out4, hipStream_t s);
+template <typename T>
+hipError_t launch_faddeeva(const T *zr, const T *zi, int64_t n, T *wr, T *wi, hipStream_t s);
 
 // Gaussian-subtracted solver (gaussian_pic.jl): source moments, residual planes, per-particle add-back.
 struct GpicMomentArgs {
@@ -203,7 +211,6 @@ struct GpicAddbackArgs {
     const ErrorInfo *err;
 };
 hipError_t launch_gpic_addback(const GpicAddbackArgs &a, int64_t max_fld, hipStream_t s);
-hipError_t launch_faddeeva(const double *zr, const double *zi, int64_t n, double *wr, double *wi, hipStream_t s);
 
 // The soft-Gaussian solver (GaussianPoissonSolver, gaussian.jl): per batch, after launch_gpic_moments over
 // the batch's source slices, every interaction's field slice through the map; kSoftBlocks blocks per
@@ -227,7 +234,8 @@ struct SoftKickArgs {
     double *lum_scale;           // [npairs of the collide]
     ErrorInfo *err;
 };
-hipError_t launch_soft_kick(const SoftKickArgs &a, hipStream_t s);
+// fp32: the Bassetti-Erskine evaluation in FP32 (GaussianPoissonSolver{Float32}); FP64 otherwise
+hipError_t launch_soft_kick(const SoftKickArgs &a, bool fp32, hipStream_t s);
 hipError_t launch_soft_lum_fold(const double *lum_block, const double *lum_scale, int npairs, double *lum_parts, hipStream_t s);
 
 struct KickArgs {

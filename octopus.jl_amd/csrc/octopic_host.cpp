@@ -445,9 +445,6 @@ void validate(const octopic_options &o) {
         if (o.include_sigma_xy)
             fail(OCTOPIC_ERR_UNSUPPORTED, "include_sigma_xy = true (the coupled, rotated soft-Gaussian kick) is not "
                                           "implemented by the HIP GaussianPoissonSolver; use include_sigma_xy = false.");
-        if (o.precision != OCTOPIC_PRECISION_FP64)
-            fail(OCTOPIC_ERR_UNSUPPORTED, "the HIP GaussianPoissonSolver computes in FP64 (it has no mesh to hold in FP32); "
-                                          "use precision = FP64 (Float32 beams are accepted).");
     }
     if (o.solver == OCTOPIC_SOLVER_GAUSSIAN_PIC) {
         // GaussianPICPoissonSolver constructor (gaussian_pic.jl:94-110) and the options it rejects at collide
@@ -1060,7 +1057,7 @@ void soft_collide(octopic_handle *h, octopic_beam *const B[2], double *const bp[
             a.drift = o.virtual_drift; a.lkick = o.longitudinal_kick ? 1 : 0;
             a.sample1 = o.gaussian_when_luminosity == 1 ? 1 : 0; a.compute_lum = compute_lum ? 1 : 0;
             a.lum_block = h->soft_lum_block.p; a.lum_scale = h->soft_lum_scale.p; a.err = h->errinfo.p;
-            HIPCHECK(launch_soft_kick(a, s));
+            HIPCHECK(launch_soft_kick(a, h->fp32, s));
         }
     }
     if (compute_lum) HIPCHECK(launch_soft_lum_fold(h->soft_lum_block.p, h->soft_lum_scale.p, npairs, h->lum_parts.p, s));
@@ -1743,40 +1740,73 @@ int octopic_track_linear6d(octopic_handle *h, const double *m, octopic_beam *b) 
     return OCTOPIC_OK;
 }
 
-int octopic_gaussian_beambeam_kick(octopic_handle *h, const double *x, const double *y, int64_t n, double sigx,
-                                   double sigy, double *kx, double *ky) {
+extern "C++" {
+namespace {
+template <typename T>
+int be_kick(octopic_handle *h, const T *x, const T *y, int64_t n, T sigx, T sigy, T *kx, T *ky) {
     if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
     if (n < 0 || (n > 0 && (!x || !y || !kx || !ky)))
         return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "gaussian_beambeam_kick: null array or negative count");
-    if (!(std::isfinite(sigx) && std::isfinite(sigy) && sigx >= 0.0 && sigy >= 0.0))
+    if (!(std::isfinite(sigx) && std::isfinite(sigy) && sigx >= T(0) && sigy >= T(0)))
         return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "gaussian_beambeam_kick: sizes must be finite and non-negative");
     hipSetDevice(h->device);
-    if (launch_be_kick(x, y, n, sigx, sigy, kx, ky, h->stream) != hipSuccess)
+    if (launch_be_kick<T>(x, y, n, sigx, sigy, kx, ky, h->stream) != hipSuccess)
         return set_err(h, OCTOPIC_ERR_DEVICE, "gaussian_beambeam_kick launch failed");
     return OCTOPIC_OK;
 }
 
-int octopic_gaussian_beambeam_kick_response(octopic_handle *h, const double *x, const double *y, int64_t n, double kbb,
-                                            double sigx, double sigy, double *out4) {
+template <typename T>
+int be_response(octopic_handle *h, const T *x, const T *y, int64_t n, T kbb, T sigx, T sigy, T *out4) {
     if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
     if (n < 0 || (n > 0 && (!x || !y || !out4)))
         return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "gaussian_beambeam_kick_response: null array or negative count");
-    if (!(std::isfinite(sigx) && std::isfinite(sigy) && sigx > 0.0 && sigy > 0.0 && std::isfinite(kbb)))
+    if (!(std::isfinite(sigx) && std::isfinite(sigy) && sigx > T(0) && sigy > T(0) && std::isfinite(kbb)))
         return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "gaussian_beambeam_kick_response: sizes must be finite and positive");
     hipSetDevice(h->device);
-    if (launch_be_response(x, y, n, kbb, sigx, sigy, out4, h->stream) != hipSuccess)
+    if (launch_be_response<T>(x, y, n, kbb, sigx, sigy, out4, h->stream) != hipSuccess)
         return set_err(h, OCTOPIC_ERR_DEVICE, "gaussian_beambeam_kick_response launch failed");
     return OCTOPIC_OK;
 }
 
-int octopic_faddeeva_w(octopic_handle *h, const double *zr, const double *zi, int64_t n, double *wr, double *wi) {
+template <typename T>
+int faddeeva(octopic_handle *h, const T *zr, const T *zi, int64_t n, T *wr, T *wi) {
     if (!h) return OCTOPIC_ERR_INVALID_ARGUMENT;
     if (n < 0 || (n > 0 && (!zr || !zi || !wr || !wi)))
         return set_err(h, OCTOPIC_ERR_INVALID_ARGUMENT, "faddeeva_w: null array or negative count");
     hipSetDevice(h->device);
-    if (launch_faddeeva(zr, zi, n, wr, wi, h->stream) != hipSuccess)
+    if (launch_faddeeva<T>(zr, zi, n, wr, wi, h->stream) != hipSuccess)
         return set_err(h, OCTOPIC_ERR_DEVICE, "faddeeva_w launch failed");
     return OCTOPIC_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+int octopic_gaussian_beambeam_kick(octopic_handle *h, const double *x, const double *y, int64_t n, double sigx,
+                                   double sigy, double *kx, double *ky) {
+    return be_kick<double>(h, x, y, n, sigx, sigy, kx, ky);
+}
+
+int octopic_gaussian_beambeam_kick_f32(octopic_handle *h, const float *x, const float *y, int64_t n, float sigx,
+                                       float sigy, float *kx, float *ky) {
+    return be_kick<float>(h, x, y, n, sigx, sigy, kx, ky);
+}
+
+int octopic_gaussian_beambeam_kick_response(octopic_handle *h, const double *x, const double *y, int64_t n, double kbb,
+                                            double sigx, double sigy, double *out4) {
+    return be_response<double>(h, x, y, n, kbb, sigx, sigy, out4);
+}
+
+int octopic_gaussian_beambeam_kick_response_f32(octopic_handle *h, const float *x, const float *y, int64_t n, float kbb,
+                                                float sigx, float sigy, float *out4) {
+    return be_response<float>(h, x, y, n, kbb, sigx, sigy, out4);
+}
+
+int octopic_faddeeva_w(octopic_handle *h, const double *zr, const double *zi, int64_t n, double *wr, double *wi) {
+    return faddeeva<double>(h, zr, zi, n, wr, wi);
+}
+
+int octopic_faddeeva_w_f32(octopic_handle *h, const float *zr, const float *zi, int64_t n, float *wr, float *wi) {
+    return faddeeva<float>(h, zr, zi, n, wr, wi);
 }
 
 int octopic_track_lorentz_boost(octopic_handle *h, double angle, int32_t reverse, octopic_beam *b) {

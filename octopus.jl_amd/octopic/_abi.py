@@ -189,6 +189,9 @@ def declare(lib):
         "octopic_gaussian_beambeam_kick": (i32, [vp, vp, vp, i64, C.c_double, C.c_double, vp, vp]),
         "octopic_faddeeva_w": (i32, [vp, vp, vp, i64, vp, vp]),
         "octopic_gaussian_beambeam_kick_response": (i32, [vp, vp, vp, i64, C.c_double, C.c_double, C.c_double, vp]),
+        "octopic_gaussian_beambeam_kick_f32": (i32, [vp, vp, vp, i64, C.c_float, C.c_float, vp, vp]),
+        "octopic_faddeeva_w_f32": (i32, [vp, vp, vp, i64, vp, vp]),
+        "octopic_gaussian_beambeam_kick_response_f32": (i32, [vp, vp, vp, i64, C.c_float, C.c_float, C.c_float, vp]),
         "octopic_generate_gaussian_beam": (
             i32,
             [vp, C.c_uint64, C.c_uint64, DPTR, DPTR, DPTR, C.c_double, C.POINTER(BeamStruct)],
@@ -246,6 +249,9 @@ EXPORTED = [
     "octopic_gaussian_beambeam_kick",
     "octopic_faddeeva_w",
     "octopic_gaussian_beambeam_kick_response",
+    "octopic_gaussian_beambeam_kick_f32",
+    "octopic_faddeeva_w_f32",
+    "octopic_gaussian_beambeam_kick_response_f32",
     "octopic_generate_gaussian_beam",
     "octopic_generate_gaussian_beam_noalloc",
     "octopic_debug_weights",

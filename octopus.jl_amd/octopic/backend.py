@@ -278,41 +278,49 @@ class PICCollider:
         s = beam.struct()
         self._check(lib().octopic_track_crab_cavity(self.h, float(frequency), nh, arr[0], arr[1], arr[2], C.byref(s)))
 
-    def gaussian_beambeam_kick(self, x, y, sigx: float, sigy: float):
-        """gaussian_beambeam_kick(sigx, sigy, x, y) (strong_beam.jl:1139-1146) elementwise over device
-        float64 tensors x, y of one shape; returns (kx, ky), the normalised Bassetti-Erskine kick."""
+    @staticmethod
+    def _be_args(x, y):
         import torch
-        if x.shape != y.shape or x.dtype != torch.float64 or y.dtype != torch.float64 or not (x.is_cuda and y.is_cuda):
-            raise ArgumentError("x and y must be float64 device tensors of one shape")
-        x, y = x.contiguous(), y.contiguous()
+        if (x.shape != y.shape or x.dtype != y.dtype or x.dtype not in (torch.float64, torch.float32)
+                or not (x.is_cuda and y.is_cuda)):
+            raise ArgumentError("x and y must be float64 or float32 device tensors of one shape and dtype")
+        return x.contiguous(), y.contiguous(), x.dtype == torch.float32
+
+    def gaussian_beambeam_kick(self, x, y, sigx: float, sigy: float):
+        """gaussian_beambeam_kick(sigx, sigy, x, y) (strong_beam.jl:1139-1146) elementwise over device tensors
+        x, y of one shape; returns (kx, ky), the normalised Bassetti-Erskine kick.  float32 tensors take the
+        Float32 evaluation (the reference's Float32 constants, FP32 arithmetic)."""
+        import torch
+        x, y, f32 = self._be_args(x, y)
         kx, ky = torch.empty_like(x), torch.empty_like(y)
-        self._check(lib().octopic_gaussian_beambeam_kick(self.h, x.data_ptr(), y.data_ptr(), x.numel(), float(sigx),
-                                                         float(sigy), kx.data_ptr(), ky.data_ptr()))
+        fn = lib().octopic_gaussian_beambeam_kick_f32 if f32 else lib().octopic_gaussian_beambeam_kick
+        self._check(fn(self.h, x.data_ptr(), y.data_ptr(), x.numel(), float(sigx), float(sigy), kx.data_ptr(),
+                       ky.data_ptr()))
         return kx, ky
 
     def gaussian_beambeam_kick_response(self, x, y, kbb: float, sigx: float, sigy: float):
         """_gaussian_beambeam_kick_response(kbb, sigx, sigy, x, y) (strong_beam.jl:1119-1128) elementwise over
-        device float64 tensors x, y: a [n, 4] tensor of (Kx, Ky, Hxx, Hyy), the normalised kick and kbb times
-        the potential's Hessian diagonal."""
+        device tensors x, y: a [n, 4] tensor of (Kx, Ky, Hxx, Hyy), the normalised kick and kbb times the
+        potential's Hessian diagonal, of the inputs' dtype (float32: the Float32 evaluation)."""
         import torch
-        if x.shape != y.shape or x.dtype != torch.float64 or y.dtype != torch.float64 or not (x.is_cuda and y.is_cuda):
-            raise ArgumentError("x and y must be float64 device tensors of one shape")
-        x, y = x.contiguous().reshape(-1), y.contiguous().reshape(-1)
-        out = torch.empty((x.numel(), 4), dtype=torch.float64, device=x.device)
-        self._check(lib().octopic_gaussian_beambeam_kick_response(self.h, x.data_ptr(), y.data_ptr(), x.numel(),
-                                                                  float(kbb), float(sigx), float(sigy),
-                                                                  out.data_ptr()))
+        x, y, f32 = self._be_args(x, y)
+        x, y = x.reshape(-1), y.reshape(-1)
+        out = torch.empty((x.numel(), 4), dtype=x.dtype, device=x.device)
+        fn = lib().octopic_gaussian_beambeam_kick_response_f32 if f32 else lib().octopic_gaussian_beambeam_kick_response
+        self._check(fn(self.h, x.data_ptr(), y.data_ptr(), x.numel(), float(kbb), float(sigx), float(sigy),
+                       out.data_ptr()))
         return out
 
     def faddeeva_w(self, z):
-        """w(z) = exp(-z^2) erfc(-iz) (SpecialMath.jl:60-151) elementwise over a complex128 device tensor."""
+        """w(z) = exp(-z^2) erfc(-iz) (SpecialMath.jl:60-151) elementwise over a complex128 device tensor, or a
+        complex64 one in FP32 arithmetic."""
         import torch
-        if z.dtype != torch.complex128 or not z.is_cuda:
-            raise ArgumentError("z must be a complex128 device tensor")
+        if z.dtype not in (torch.complex128, torch.complex64) or not z.is_cuda:
+            raise ArgumentError("z must be a complex128 or complex64 device tensor")
         zr, zi = z.real.contiguous(), z.imag.contiguous()
         wr, wi = torch.empty_like(zr), torch.empty_like(zi)
-        self._check(lib().octopic_faddeeva_w(self.h, zr.data_ptr(), zi.data_ptr(), zr.numel(), wr.data_ptr(),
-                                             wi.data_ptr()))
+        fn = lib().octopic_faddeeva_w_f32 if z.dtype == torch.complex64 else lib().octopic_faddeeva_w
+        self._check(fn(self.h, zr.data_ptr(), zi.data_ptr(), zr.numel(), wr.data_ptr(), wi.data_ptr()))
         return torch.complex(wr, wi)
 
     def set_timing(self, enabled: bool, families: Optional[str] = None):

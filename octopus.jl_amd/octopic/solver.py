@@ -318,7 +318,9 @@ class GaussianPoissonSolver:
     slice pair, in collision order, kicks each beam's slice with the exact Bassetti-Erskine field of the other
     slice's Gaussian moments, transported to each particle's collision point, between the virtual drifts.
     include_sigma_xy = true (the coupled, rotated kick) is accepted here as by the reference's constructor and
-    rejected by the backends with UnsupportedOptionError."""
+    rejected by the backends with UnsupportedOptionError.  precision = "fp32" is GaussianPoissonSolver{Float32}'s
+    Bassetti-Erskine evaluation (field, Hessian and density in FP32 with the reference's Float32 constants,
+    strong_beam.jl:770-783, 809-811); the drifts, slice moments, coordinates and luminosity sums stay FP64."""
 
     kbb1: Optional[float] = None
     kbb2: Optional[float] = None
@@ -335,6 +337,7 @@ class GaussianPoissonSolver:
     include_sigma_xy: bool = False
     batch_mode: str = "wavefront"
     luminosity_schedule: object = None
+    precision: str = "fp64"
 
     def __post_init__(self):
         # The reference constructor's order of checks (interface.jl:976-1000).
@@ -355,6 +358,8 @@ class GaussianPoissonSolver:
             v = getattr(self, k)
             if v is not None:
                 object.__setattr__(self, k, float(v))
+        if self.precision not in _abi.PRECISION:
+            raise ArgumentError(f"precision must be fp64 or fp32; got {self.precision!r}.")
 
     @property
     def resolved_slicing1(self) -> LongitudinalSlicing:
@@ -383,6 +388,7 @@ class GaussianPoissonSolver:
         o.ignore_centroid1 = 1 if self.ignore_centroid1 else 0
         o.ignore_centroid2 = 1 if self.ignore_centroid2 else 0
         o.include_sigma_xy = 1 if self.include_sigma_xy else 0
+        o.precision = _abi.PRECISION[self.precision]
         return o, keep
 
 

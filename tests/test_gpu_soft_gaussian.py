@@ -12,6 +12,8 @@ C ABI.
 - A non-finite source coordinate fails the collide and leaves the beams as they were; the unscheduled
   turns of a luminosity schedule return NaN with the same kicks; rejected combinations.
 - Empty slices (every pair still runs, with and without a min_sigma floor) and beams smaller than a block.
+- precision = "fp32" (GaussianPoissonSolver{Float32}'s Bassetti-Erskine evaluation in FP32) against the FP64
+  oracle within the FP32 evaluator's error bounds, and on Float32 beams.
 """
 import ctypes as C
 import math
@@ -151,9 +153,9 @@ def test_rejected_combinations(gpu):
     with pytest.raises(UnsupportedOptionError):
         PICCollider(GaussianPoissonSolver(include_sigma_xy=True))
     o, keep = GaussianPoissonSolver().to_options()
-    o.precision = _abi.PRECISION["fp32"]
+    o.precision = 7
     h = C.c_void_p()
-    assert lib().octopic_create(C.byref(o), 0, C.byref(h)) == _abi.ERR_UNSUPPORTED
+    assert lib().octopic_create(C.byref(o), 0, C.byref(h)) == _abi.ERR_INVALID_OPTION
 
 
 @pytest.mark.parametrize("min_sigma", [0.0, 2e-5])
@@ -174,3 +176,54 @@ def test_tiny_ragged_beams(oracle, gpu):
     e, p = round_beams(oracle, 61, 23, seed=9)
     compare_collide(oracle, gpu, GaussianPoissonSolver(slicing=LongitudinalSlicing(nslices=3, method="equal_count"),
                                                        virtual_drift="exact"), e, p, collides=2)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(virtual_drift="exact", gaussian_when_luminosity=1),
+                                dict(virtual_drift="chromatic", min_sigma=3e-5, ignore_centroid2=True)])
+def test_fp32_evaluation_against_fp64_oracle(oracle, gpu, kw):
+    """GaussianPoissonSolver{Float32}'s Bassetti-Erskine evaluation (precision="fp32": field, Hessian and density
+    in FP32 with the Float32 constants) against the FP64 oracle on the same inputs.  The FP32 force's natural-scale
+    error is below C eps / sqrt(eta*/2) = 6.8e-6 of 1/sigma (the near-round transition sweep), its response's below
+    6e-5 of 1/sigma^2, and every coordinate change is linear in them: each coordinate's difference from the
+    oracle stays within 1e-4 of that coordinate's largest change in the collide (px, py within 2e-5), and the
+    luminosity within 1e-5 relative.  The FP32 path must differ from the FP64 one (it ran)."""
+    e, p = flat_beams(oracle, 20000, 9000, seed=5)
+    e, p = _offset(e, p)
+    sl = LongitudinalSlicing(nslices=5, method="normal_quantile")
+    s64 = GaussianPoissonSolver(slicing=sl, **kw)
+    s32 = GaussianPoissonSolver(slicing=sl, precision="fp32", **kw)
+    oe, op = e.copy(), p.copy()
+    lum_ref = oracle.Collider(s64).collide(oe, PE, op, PP)
+    col = PICCollider(s32)
+    b1, b2 = Beam.from_host(e, PE, device=gpu), Beam.from_host(p, PP, device=gpu)
+    lum = col.collide(b1, b2, turn=0)
+    g1, g2 = b1.to_host(), b2.to_host()
+    assert abs(lum - lum_ref) <= 1e-5 * abs(lum_ref), (lum, lum_ref)
+    differs = False
+    for g, o, start in ((g1, oe, e), (g2, op, p)):
+        for c in range(6):
+            change = np.abs(o[c] - start[c]).max()
+            d = np.abs(g[c] - o[c]).max()
+            tol = (2e-5 if c in (1, 3) else 1e-4) * change + 1e-300
+            assert d <= tol, (c, d, change)
+            differs |= d > 0
+    assert differs
+    # the FP64 handle on the same inputs stays within the path's 1e-10 of the oracle
+    c64 = PICCollider(s64)
+    h1, h2 = Beam.from_host(e, PE, device=gpu), Beam.from_host(p, PP, device=gpu)
+    assert abs(c64.collide(h1, h2, turn=0) - lum_ref) <= 1e-10 * abs(lum_ref)
+
+
+def test_fp32_evaluation_float32_beams(oracle, gpu):
+    """Float32 beams under the FP32 evaluation: the FP32-evaluation collide of the widened beams, rounded back."""
+    e, p = flat_beams(oracle, 12000, 6000, seed=8)
+    e32, p32 = e.astype(np.float32), p.astype(np.float32)
+    solver = GaussianPoissonSolver(slicing=LongitudinalSlicing(nslices=4, method="normal_quantile"), precision="fp32")
+    w1 = Beam.from_host(e32.astype(np.float64), PE, device=gpu)
+    w2 = Beam.from_host(p32.astype(np.float64), PP, device=gpu)
+    lum_w = PICCollider(solver).collide(w1, w2, turn=0)
+    f1 = Beam.from_host(e32, PE, device=gpu, dtype=torch.float32)
+    f2 = Beam.from_host(p32, PP, device=gpu, dtype=torch.float32)
+    assert PICCollider(solver).collide(f1, f2, turn=0) == lum_w
+    assert np.array_equal(f1.to_host(), w1.to_host().astype(np.float32))
+    assert np.array_equal(f2.to_host(), w2.to_host().astype(np.float32))

@@ -225,6 +225,11 @@ def test_constructor_checks():
         GaussianPoissonSolver(gaussian_when_luminosity=3)
     o, _ = GaussianPoissonSolver(virtual_drift=UnsafeVirtualDrift("chromatic_frozen_energy")).to_options()
     assert o.virtual_drift == 4
+    # GaussianPoissonSolver{Float32}: the working precision is an option of the solver
+    assert GaussianPoissonSolver().to_options()[0].precision == 0
+    assert GaussianPoissonSolver(precision="fp32").to_options()[0].precision == 1
+    with pytest.raises(ArgumentError):
+        GaussianPoissonSolver(precision="fp16")
 
 
 def test_collide_errors(oracle):
