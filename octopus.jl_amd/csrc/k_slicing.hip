@@ -14,7 +14,6 @@
 //   centroid     : the same 4096-lane fold over each member list (slicing.jl:245-257, 523-559).
 //   tile order   : members regrouped by transverse tile within each slice (no counterpart in the
 //                  reference: a memory-layout choice for the field gathers; results do not depend on it).
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "common.h"
 #include "launch.h"
@@ -686,11 +685,82 @@ __global__ __launch_bounds__(kSliceThreads) void k_rank_gather(const double *con
     if (box) fold_box(v, box);
 }
 
-size_t rank_sort_bytes(int64_t n) {
-    size_t bytes = 0;
-    rocprim::radix_sort_pairs((void *)nullptr, bytes, (const unsigned long long *)nullptr, (unsigned long long *)nullptr,
-                              (const int64_t *)nullptr, (int64_t *)nullptr, (size_t)n);
-    return bytes;
+// The rank order itself: a stable LSD radix sort of the 64-bit keys with their indices, eight passes of
+// 8-bit digits.  Each pass is a stable counting sort over kSliceBlock-element blocks: the blocks' digit
+// histograms, their exclusive scan in digit-major order (a digit's elements of block b follow those of
+// blocks < b), and a scatter that ranks a wave's elements within their digit by eight digit-bit ballots and
+// the waves of a block in order, so every digit keeps the input order.
+constexpr int kRadixBins = 256;
+
+__global__ __launch_bounds__(kSliceThreads) void k_radix_hist(const unsigned long long *keys, int64_t n, int shift,
+                                                              int *hist, int nblocks) {
+    __shared__ int cnt[kRadixBins];
+    for (int k = threadIdx.x; k < kRadixBins; k += blockDim.x) cnt[k] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSliceBlock;
+    for (int r = 0; r < kSliceBlock / kSliceThreads; ++r) {
+        const int64_t i = base + r * kSliceThreads + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(int)((keys[i] >> shift) & (kRadixBins - 1))], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kRadixBins; k += blockDim.x) hist[(size_t)k * nblocks + blockIdx.x] = cnt[k];
+}
+
+// exclusive scan of hist (digit-major, kRadixBins * nblocks) in place
+__global__ __launch_bounds__(1024) void k_radix_scan(int *hist, int nblocks) {
+    const int64_t total = (int64_t)kRadixBins * nblocks;
+    const int64_t per = (total + blockDim.x - 1) / blockDim.x;
+    const int64_t lo = threadIdx.x * per < total ? threadIdx.x * per : total, hi = lo + per < total ? lo + per : total;
+    int64_t sum = 0;
+    for (int64_t k = lo; k < hi; ++k) sum += hist[k];
+    int64_t run = block_exclusive_scan(sum, 0);
+    for (int64_t k = lo; k < hi; ++k) {
+        const int v = hist[k];
+        hist[k] = (int)run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kSliceThreads) void k_radix_scatter(const unsigned long long *kin, const int64_t *vin,
+                                                                 int64_t n, int shift, const int *hist, int nblocks,
+                                                                 unsigned long long *kout, int64_t *vout) {
+    __shared__ int run[kRadixBins];
+    __shared__ int wcnt[kSliceThreads / 64][kRadixBins];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int k = threadIdx.x; k < kRadixBins; k += blockDim.x) run[k] = 0;
+    const int64_t base = (int64_t)blockIdx.x * kSliceBlock;
+    for (int r = 0; r < kSliceBlock / kSliceThreads; ++r) {
+        for (int k = threadIdx.x; k < (kSliceThreads / 64) * kRadixBins; k += blockDim.x) (&wcnt[0][0])[k] = 0;
+        __syncthreads();
+        const int64_t i = base + r * kSliceThreads + threadIdx.x;
+        const bool valid = i < n;
+        const unsigned long long key = valid ? kin[i] : 0ull;
+        const int d = (int)((key >> shift) & (kRadixBins - 1));
+        // the wave's lanes holding the same digit: agree with this lane on every digit bit
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const unsigned long long m = __ballot(valid && ((d >> bit) & 1));
+            peers &= ((d >> bit) & 1) ? m : ~m;
+        }
+        const int rank = __popcll(peers & lt);
+        if (valid && rank == 0) wcnt[wave][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int pos = hist[(size_t)d * nblocks + blockIdx.x] + run[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+            kout[pos] = key;
+            vout[pos] = vin[i];
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < kRadixBins; k += blockDim.x) {
+            int add = 0;
+            for (int w = 0; w < kSliceThreads / 64; ++w) add += wcnt[w][k];
+            run[k] += add;
+        }
+        __syncthreads();
+    }
 }
 
 hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s) {
@@ -710,11 +780,20 @@ hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s) {
     if (a.method == 1) {
         int g = (int)((a.n + kSliceThreads - 1) / kSliceThreads);
         if (g > 4096) g = 4096;
-        hipLaunchKernelGGL(k_rank_keys, dim3(g), dim3(256), 0, s, a.z, a.n, a.rank_keys, a.rank_vals);
-        size_t bytes = a.rank_temp_bytes;
-        hipError_t e = rocprim::radix_sort_pairs(a.rank_temp, bytes, a.rank_keys, a.rank_sorted, a.rank_vals, a.perm,
-                                                 (size_t)a.n, 0, 64, s);
-        if (e != hipSuccess) return e;
+        // keys and indices start in (rank_sorted, perm); the eight passes alternate with (rank_keys, rank_vals)
+        // and end back in (rank_sorted, perm)
+        hipLaunchKernelGGL(k_rank_keys, dim3(g), dim3(256), 0, s, a.z, a.n, a.rank_sorted, a.perm);
+        for (int pass = 0; pass < 8; ++pass) {
+            const bool even = (pass & 1) == 0;
+            const unsigned long long *kin = even ? a.rank_sorted : a.rank_keys;
+            const int64_t *vin = even ? a.perm : a.rank_vals;
+            unsigned long long *kout = even ? a.rank_keys : a.rank_sorted;
+            int64_t *vout = even ? a.rank_vals : a.perm;
+            hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSliceThreads), 0, s, kin, a.n, 8 * pass, a.hist, nblocks);
+            hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, a.hist, nblocks);
+            hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSliceThreads), 0, s, kin, vin, a.n, 8 * pass, a.hist,
+                               nblocks, kout, vout);
+        }
         hipLaunchKernelGGL(k_rank_table, dim3(1), dim3(64), 0, s, a.tab, a.rank_sorted, a.n, a.ns);
         hipLaunchKernelGGL(k_rank_gather, dim3(g), dim3(kSliceThreads), 0, s, a.beam, a.n, a.perm, a.st, a.box);
     } else {

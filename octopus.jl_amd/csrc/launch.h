@@ -30,14 +30,12 @@ struct SlicingArgs {
     int64_t *perm;
     unsigned long long *box;         // 4 bound slots (x min/max, y min/max), zero on entry; null: not folded
     ErrorInfo *err;
-    // :equal_count rank sort: keys and indices in, sorted keys out (n each), sort workspace
+    // :equal_count rank sort: n keys and n indices each way (the sorted keys end in rank_sorted, the indices
+    // in perm); hist then holds 256 digit counts per block
     unsigned long long *rank_keys, *rank_sorted;
     int64_t *rank_vals;
-    void *rank_temp;
-    size_t rank_temp_bytes;
 };
 hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s);
-size_t rank_sort_bytes(int64_t n);
 
 constexpr int kTileChunk = 4096;     // particles per block visit; block c of a slice takes visits c, c+G, ...
 constexpr int kTileBlockSpan = 16384;   // particles per block and slice the tile order aims for (long tile runs)

@@ -265,7 +265,6 @@ struct octopic_handle {
     DBuf<int32_t> inv_srt[2];   // inverse of perm_srt
     DBuf<unsigned long long> rank_keys, rank_sorted;   // :equal_count rank sort (one beam at a time)
     DBuf<int64_t> rank_vals;
-    DBuf<unsigned char> rank_temp;
     DBuf<unsigned long long> box;   // transverse bound slots of both beams
     std::vector<double> hq[2];      // host quantiles / positions of the slicing
     DBuf<int> thist;
@@ -490,12 +489,12 @@ void run_slicing(octopic_handle *h, int b, const octopic_slicing &s, int64_t n) 
                             h->stream));
     h->perm[b].ensure(n);
     h->sid[b].ensure(n);
-    h->hist[b].ensure(slicing_hist_entries(n, ns));
+    // the :equal_count rank sort's passes count 256 digits per block
+    h->hist[b].ensure(slicing_hist_entries(n, s.method == OCTOPIC_SLICE_EQUAL_COUNT ? 256 : ns));
     if (s.method == OCTOPIC_SLICE_EQUAL_COUNT) {
         h->rank_keys.ensure(n);
         h->rank_sorted.ensure(n);
         h->rank_vals.ensure(n);
-        h->rank_temp.ensure(rank_sort_bytes(n));
     }
 }
 
@@ -588,7 +587,7 @@ void octopic_destroy(octopic_handle *h) {
         }
         h->perm[b].release(); h->perm_srt[b].release(); h->inv_srt[b].release(); h->sid[b].release(); h->hist[b].release();
     }
-    h->rank_keys.release(); h->rank_sorted.release(); h->rank_vals.release(); h->rank_temp.release();
+    h->rank_keys.release(); h->rank_sorted.release(); h->rank_vals.release();
     h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
     h->korder.release(); h->dqueue.release(); h->dorder.release();
     h->keys.release(); h->chunkbox.release(); h->box.release(); h->thist.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
@@ -1172,7 +1171,6 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.err = h->errinfo.p;
             if (a.method == OCTOPIC_SLICE_EQUAL_COUNT) {
                 a.rank_keys = h->rank_keys.p; a.rank_sorted = h->rank_sorted.p; a.rank_vals = h->rank_vals.p;
-                a.rank_temp = h->rank_temp.p; a.rank_temp_bytes = h->rank_temp.n;
             }
             HIPCHECK(launch_slicing(a, s));
         }
