@@ -330,9 +330,11 @@ int octopic_kernel_times(const octopic_handle *h, double *ms_total, int64_t *lau
  * (-1..2, default -1), "deposit_span" (1024..2^20 source particles per deposit block, default 8192),
  * "deposit_multi" (slice-pair meshes: 1 = each deposit item takes all planes of an interaction, one
  * read of the particles; 0 = one plane per item; -1 = the faster of the two, timed on the device in
- * two consecutive collides every 256; default -1) and
+ * two consecutive collides every 256; default -1),
  * "deposit_min_span" (1024..2^20: smallest item the all-planes items are split to when a launch would
- * have fewer than two items per compute unit, default 2048).  octopic_autotune times candidate shapes by coordinate descent (median of `trials` collides per
+ * have fewer than two items per compute unit, default 2048) and "fft_transforms_per_block" (2, 4 or 8
+ * transforms per workgroup of the charge rows, the convolution columns and the inverse rows, halved
+ * while a workgroup's LDS would not fit; default 4).  octopic_autotune times candidate shapes by coordinate descent (median of `trials` collides per
  * candidate, after a warming one) on copies of the two beams in a scratch handle with the same
  * options, so the handle's own Green cache, counters and the beams are untouched, and sets the
  * fastest shape on the handle; best_ms (optional) receives its collide time. */

@@ -72,13 +72,13 @@ __device__ __forceinline__ C unpack_one(const C *Z, int P, int k, int odd) {
 template <class C>
 __device__ __forceinline__ C *group_buffer(C *lds, int P, int grp) { return lds + P + (size_t)grp * 2 * P; }
 
-// Row-pass epilogue shared by the Green and charge row transforms: the block's 2*kFFTPerBlock rows
-// (ya0 + r, r = 2*grp + {0, 1}) of the x-half-spectrum, stored as out[k * kstride + ya0 + r] so that
-// the 2*kFFTPerBlock consecutive rows of one k form one contiguous run.
-template <class C>
+// Row-pass epilogue shared by the Green and charge row transforms: the block's 2*G rows (G transforms
+// of two real rows each; ya0 + r, r = 2*grp + {0, 1}) of the x-half-spectrum, stored as
+// out[k * kstride + ya0 + r] so that the 2*G consecutive rows of one k form one contiguous run.
+template <int G, class C>
 __device__ __forceinline__ void store_row_spectra(C *lds, const FFTPlanDev &plan, int nrows, int ya0, C *out,
                                                   int64_t kstride) {
-    constexpr int R = 2 * kFFTPerBlock;
+    constexpr int R = 2 * G;
     const int P = plan.n, nk = P / 2 + 1;
     for (int t = threadIdx.x; t < nk * R; t += blockDim.x) {
         const int r = t % R, k = t / R;
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_green_rows(const GreenJob *jobs
         __syncthreads();
         fft_lds<-1>(a, a + P1, px, lds, lane);
         __syncthreads();
-        store_row_spectra(lds, px, P2, ya0, spec + (size_t)job.slot * slot_stride, P2);
+        store_row_spectra<kFFTPerBlock>(lds, px, P2, ya0, spec + (size_t)job.slot * slot_stride, P2);
         __syncthreads();   // the item's buffers are read before the next item writes them
     }
 }
@@ -190,24 +190,30 @@ __global__ __launch_bounds__(kFFTThreads) void k_green_cols(const GreenJob *jobs
 // over a fixed cell range of the pair's two luminosity planes (resetting them to zero for the next
 // batch) into its partial; the column launch that follows adds the kLumBlocks partials in block order
 // (lum_fold), so the result does not depend on scheduling and no block waits on another (a device-wide
-// release per block costs an L2 write-back on this chip).  Called by all kFFTThreads threads of a block.
+// release per block costs an L2 write-back on this chip).  Called by all threads of a block.
+// The block's sum has a fixed shape whatever its thread count: kLumLanes lane sums (lane v takes cells
+// c0 + v, c0 + v + kLumLanes, ...), then a fixed tree, so the partial does not depend on the transforms
+// per workgroup of the launch it rides in.
+constexpr int kLumLanes = 256;
 __device__ __forceinline__ void lum_block(const LumArgs &a, int k, int b) {
     unsigned long long *q1 = a.charge + (size_t)(2 * k) * a.charge_stride;
     unsigned long long *q2 = a.charge + (size_t)(2 * k + 1) * a.charge_stride;
     const double inv = a.inv_scale;
     const int c0 = (int)((int64_t)a.ncells * b / kLumBlocks), c1 = (int)((int64_t)a.ncells * (b + 1) / kLumBlocks);
-    double acc = 0.0;
-    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-        const unsigned long long u1 = q1[c], u2 = q2[c];
-        if (u1) q1[c] = 0ull;
-        if (u2) q2[c] = 0ull;
-        acc += ((double)u1 * inv) * ((double)u2 * inv);
+    __shared__ double red[kLumLanes];
+    for (int v = threadIdx.x; v < kLumLanes; v += blockDim.x) {
+        double acc = 0.0;
+        for (int c = c0 + v; c < c1; c += kLumLanes) {
+            const unsigned long long u1 = q1[c], u2 = q2[c];
+            if (u1) q1[c] = 0ull;
+            if (u2) q2[c] = 0ull;
+            acc += ((double)u1 * inv) * ((double)u2 * inv);
+        }
+        red[v] = acc;
     }
-    __shared__ double red[kFFTThreads];
-    red[threadIdx.x] = acc;
     __syncthreads();
-    for (int w = kFFTThreads / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    for (int w = kLumLanes / 2; w > 0; w >>= 1) {
+        for (int v = threadIdx.x; v < w; v += blockDim.x) red[v] += red[v + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) a.lum_partial[k * kLumBlocks + b] = red[0];
@@ -227,9 +233,9 @@ __device__ __forceinline__ void lum_fold(const LumArgs &a) {
 // Charge rows: int64 fixed-point rows (ya, ya+1) of plane -> x-half-spectrum Qh[plane][kx][y], y < ny.
 // Each charge cell is read once and reset to zero for the next use of the buffer.  Block rows past the
 // planes take the batch's luminosity overlap sums (lum.npairs pairs, kLumBlocks blocks each), which
-// read only the luminosity planes the deposit has finished.
-template <class C>
-__global__ __launch_bounds__(kFFTThreads) void k_rows_fwd(const Plane *planes, unsigned long long *charge, int64_t charge_stride,
+// read only the luminosity planes the deposit has finished.  G transforms (2*G rows) per block.
+template <class C, int G>
+__global__ __launch_bounds__(kFFTLanes * G) void k_rows_fwd(const Plane *planes, unsigned long long *charge, int64_t charge_stride,
                                                           C *qh, int64_t qh_stride, int nx, int ny, double inv_scale,
                                                           FFTPlanDev px, const ErrorInfo *err, int nplanes, LumArgs lum) {
     using R = real_of<C>;
@@ -245,7 +251,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_fwd(const Plane *planes, u
     const int P1 = 2 * nx;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
     C *a = group_buffer(lds, P1, grp);
-    const int ya0 = 2 * kFFTPerBlock * blockIdx.x, ya = ya0 + 2 * grp;
+    const int ya0 = 2 * G * blockIdx.x, ya = ya0 + 2 * grp;
     const bool active = ya < ny;
     const bool hasb = ya + 1 < ny;
     unsigned long long *q = charge + (size_t)plane * charge_stride;
@@ -267,14 +273,14 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_fwd(const Plane *planes, u
     __syncthreads();
     fft_lds<-1>(a, a + P1, px, lds, lane);
     __syncthreads();
-    store_row_spectra(lds, px, ny, ya0, qh + (size_t)plane * qh_stride, ny);
+    store_row_spectra<G>(lds, px, ny, ya0, qh + (size_t)plane * qh_stride, ny);
 }
 
 // Column pass: FFT_y of the zero-padded column, times the Green half spectrum, IFFT_y, keep y < ny.
 // (One plane per block row: sharing a Green column across an interaction's planes in one block
-// measured slower, the lost parallelism costing more than the saved spectrum reads.)
-template <class C>
-__global__ __launch_bounds__(kFFTThreads) void k_cols_conv(const Plane *planes, C *qh, int64_t qh_stride,
+// measured slower, the lost parallelism costing more than the saved spectrum reads.)  G columns per block.
+template <class C, int G>
+__global__ __launch_bounds__(kFFTLanes * G) void k_cols_conv(const Plane *planes, C *qh, int64_t qh_stride,
                                                            const C *gspec, int64_t slot_stride, int nx, int ny,
                                                            FFTPlanDev py, const ErrorInfo *err, LumArgs lum) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -285,7 +291,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_cols_conv(const Plane *planes, 
     const int P2 = 2 * ny;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
     C *a = group_buffer(lds, P2, grp), *b = a + P2;
-    const int kx = blockIdx.x * kFFTPerBlock + grp;
+    const int kx = blockIdx.x * G + grp;
     const bool active = kx <= nx;
     C *col = qh + (size_t)plane * qh_stride + (size_t)(active ? kx : 0) * ny;
     load_twiddles(py, lds);
@@ -304,10 +310,10 @@ __global__ __launch_bounds__(kFFTThreads) void k_cols_conv(const Plane *planes, 
 }
 
 // Row inverse: rows (ya, ya+1) from the half spectrum, IFFT_x, keep x < nx, scale 1/(P1 P2).  The
-// block gathers its 2*kFFTPerBlock rows together: thread (kk, grp) reads rows ya, ya+1 of column kk
-// (adjacent threads adjacent rows) and fills both kk and its Hermitian mirror P1 - kk.
-template <class C>
-__global__ __launch_bounds__(kFFTThreads) void k_rows_inv(const Plane *planes, const C *qh, int64_t qh_stride,
+// block gathers its 2*G rows together: thread (kk, grp) reads rows ya, ya+1 of column kk (adjacent
+// threads adjacent rows) and fills both kk and its Hermitian mirror P1 - kk.
+template <class C, int G>
+__global__ __launch_bounds__(kFFTLanes * G) void k_rows_inv(const Plane *planes, const C *qh, int64_t qh_stride,
                                                           real_of<C> *phi, int nx, int ny, FFTPlanDev px,
                                                           const ErrorInfo *err) {
     using R = real_of<C>;
@@ -317,11 +323,11 @@ __global__ __launch_bounds__(kFFTThreads) void k_rows_inv(const Plane *planes, c
     const int plane = blockIdx.y;
     const int P1 = 2 * nx, P2 = 2 * ny;
     const int grp = threadIdx.x / kFFTLanes, lane = threadIdx.x % kFFTLanes;
-    const int ya0 = 2 * kFFTPerBlock * blockIdx.x;
+    const int ya0 = 2 * G * blockIdx.x;
     const C *in = qh + (size_t)plane * qh_stride;
     load_twiddles(px, lds);
-    for (int t = threadIdx.x; t < (nx + 1) * kFFTPerBlock; t += blockDim.x) {
-        const int g = t % kFFTPerBlock, kk = t / kFFTPerBlock;
+    for (int t = threadIdx.x; t < (nx + 1) * G; t += blockDim.x) {
+        const int g = t % G, kk = t / G;
         const int y = ya0 + 2 * g;
         C A = cx<C>(0, 0), B = A;
         if (y < ny) A = in[(size_t)kk * ny + y];
@@ -456,10 +462,19 @@ __global__ __launch_bounds__(256) void k_field(const Inter *inters, const Plane 
 }
 
 // ---------------------------------------------------------------------- launchers
-static size_t fft_lds_bytes(int P) { return (size_t)(1 + kFFTPerBlock * 2) * P * sizeof(double2); }
+static size_t fft_lds_bytes(int P, int G = kFFTPerBlock) { return (size_t)(1 + G * 2) * P * sizeof(double2); }
 
 static hipError_t ensure_lds(const void *fn, size_t bytes) {
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+// Transforms per block of the charge-path passes: the requested 2, 4 or 8, halved while its dynamic LDS (the
+// twiddles and two buffers per transform, of transform length P and element size `es`) would not leave
+// 4 KB of the 160 KB for the kernels' static LDS.
+static int fft_group(int requested, int P, size_t es) {
+    int g = requested == 2 || requested == 8 ? requested : kFFTPerBlock;
+    while (g > 2 && (size_t)(1 + 2 * g) * P * es > (size_t)156 * 1024) g /= 2;
+    return g;
 }
 
 // `full` launches a block per item of every job (the cold case: every job builds); otherwise a small
@@ -487,41 +502,73 @@ hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, in
     return hipGetLastError();
 }
 
+template <class C, int G>
+static void rows_fwd_g(const Plane *planes, int nplanes, unsigned long long *charge, int64_t charge_stride, C *qh,
+                       int64_t qh_stride, int nx, int ny, double inv_scale, const FFTPlanDev &px, const ErrorInfo *err,
+                       const LumArgs &lum, hipStream_t s) {
+    const size_t lds = (size_t)(1 + 2 * G) * (2 * nx) * sizeof(C);
+    ensure_lds((const void *)k_rows_fwd<C, G>, lds);
+    const int gx = (ny + 2 * G - 1) / (2 * G);
+    const int lum_rows = (lum.npairs * kLumBlocks + gx - 1) / gx;
+    dim3 g(gx, nplanes + lum_rows);
+    hipLaunchKernelGGL((k_rows_fwd<C, G>), g, dim3(kFFTLanes * G), lds, s, planes, charge, charge_stride, qh, qh_stride, nx,
+                       ny, inv_scale, px, err, nplanes, lum);
+}
+
 template <class C>
 hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long *charge, int64_t charge_stride, C *qh,
                            int64_t qh_stride, int nx, int ny, double inv_scale, const FFTPlanDev &px, const ErrorInfo *err,
-                           const LumArgs &lum, hipStream_t s) {
+                           const LumArgs &lum, hipStream_t s, int group) {
     if (nplanes <= 0 && lum.npairs <= 0) return hipSuccess;
-    const size_t lds = fft_lds_bytes(2 * nx) / sizeof(double2) * sizeof(C);
-    ensure_lds((const void *)k_rows_fwd<C>, lds);
-    const int gx = (ny + 2 * kFFTPerBlock - 1) / (2 * kFFTPerBlock);
-    const int lum_rows = (lum.npairs * kLumBlocks + gx - 1) / gx;
-    dim3 g(gx, nplanes + lum_rows);
-    hipLaunchKernelGGL(k_rows_fwd<C>, g, dim3(kFFTThreads), lds, s, planes, charge, charge_stride, qh, qh_stride, nx, ny,
-                       inv_scale, px, err, nplanes, lum);
+    switch (fft_group(group, 2 * nx, sizeof(C))) {
+        case 2: rows_fwd_g<C, 2>(planes, nplanes, charge, charge_stride, qh, qh_stride, nx, ny, inv_scale, px, err, lum, s); break;
+        case 8: rows_fwd_g<C, 8>(planes, nplanes, charge, charge_stride, qh, qh_stride, nx, ny, inv_scale, px, err, lum, s); break;
+        default: rows_fwd_g<C, 4>(planes, nplanes, charge, charge_stride, qh, qh_stride, nx, ny, inv_scale, px, err, lum, s);
+    }
     return hipGetLastError();
+}
+
+template <class C, int G>
+static void cols_conv_g(const Plane *planes, int nplanes, C *qh, int64_t qh_stride, const C *gspec, int64_t slot_stride,
+                        int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, const LumArgs &lum, hipStream_t s) {
+    const size_t lds = (size_t)(1 + 2 * G) * (2 * ny) * sizeof(C);
+    ensure_lds((const void *)k_cols_conv<C, G>, lds);
+    dim3 g((nx + 1 + G - 1) / G, nplanes);
+    hipLaunchKernelGGL((k_cols_conv<C, G>), g, dim3(kFFTLanes * G), lds, s, planes, qh, qh_stride, gspec, slot_stride, nx, ny,
+                       py, err, lum);
 }
 
 template <class C>
 hipError_t launch_cols_conv(const Plane *planes, int nplanes, C *qh, int64_t qh_stride, const C *gspec, int64_t slot_stride,
-                            int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, const LumArgs &lum, hipStream_t s) {
+                            int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, const LumArgs &lum, hipStream_t s,
+                            int group) {
     if (nplanes <= 0) return hipSuccess;
-    const size_t lds = fft_lds_bytes(2 * ny) / sizeof(double2) * sizeof(C);
-    ensure_lds((const void *)k_cols_conv<C>, lds);
-    dim3 g((nx + 1 + kFFTPerBlock - 1) / kFFTPerBlock, nplanes);
-    hipLaunchKernelGGL(k_cols_conv<C>, g, dim3(kFFTThreads), lds, s, planes, qh, qh_stride, gspec, slot_stride, nx, ny, py,
-                       err, lum);
+    switch (fft_group(group, 2 * ny, sizeof(C))) {
+        case 2: cols_conv_g<C, 2>(planes, nplanes, qh, qh_stride, gspec, slot_stride, nx, ny, py, err, lum, s); break;
+        case 8: cols_conv_g<C, 8>(planes, nplanes, qh, qh_stride, gspec, slot_stride, nx, ny, py, err, lum, s); break;
+        default: cols_conv_g<C, 4>(planes, nplanes, qh, qh_stride, gspec, slot_stride, nx, ny, py, err, lum, s);
+    }
     return hipGetLastError();
+}
+
+template <class C, int G>
+static void rows_inv_g(const Plane *planes, int nplanes, const C *qh, int64_t qh_stride, real_of<C> *phi, int nx, int ny,
+                       const FFTPlanDev &px, const ErrorInfo *err, hipStream_t s) {
+    const size_t lds = (size_t)(1 + 2 * G) * (2 * nx) * sizeof(C);
+    ensure_lds((const void *)k_rows_inv<C, G>, lds);
+    dim3 g((ny + 2 * G - 1) / (2 * G), nplanes);
+    hipLaunchKernelGGL((k_rows_inv<C, G>), g, dim3(kFFTLanes * G), lds, s, planes, qh, qh_stride, phi, nx, ny, px, err);
 }
 
 template <class C>
 hipError_t launch_rows_inv(const Plane *planes, int nplanes, const C *qh, int64_t qh_stride, real_of<C> *phi, int nx,
-                           int ny, const FFTPlanDev &px, const ErrorInfo *err, hipStream_t s) {
+                           int ny, const FFTPlanDev &px, const ErrorInfo *err, hipStream_t s, int group) {
     if (nplanes <= 0) return hipSuccess;
-    const size_t lds = fft_lds_bytes(2 * nx) / sizeof(double2) * sizeof(C);
-    ensure_lds((const void *)k_rows_inv<C>, lds);
-    dim3 g((ny + 2 * kFFTPerBlock - 1) / (2 * kFFTPerBlock), nplanes);
-    hipLaunchKernelGGL(k_rows_inv<C>, g, dim3(kFFTThreads), lds, s, planes, qh, qh_stride, phi, nx, ny, px, err);
+    switch (fft_group(group, 2 * nx, sizeof(C))) {
+        case 2: rows_inv_g<C, 2>(planes, nplanes, qh, qh_stride, phi, nx, ny, px, err, s); break;
+        case 8: rows_inv_g<C, 8>(planes, nplanes, qh, qh_stride, phi, nx, ny, px, err, s); break;
+        default: rows_inv_g<C, 4>(planes, nplanes, qh, qh_stride, phi, nx, ny, px, err, s);
+    }
     return hipGetLastError();
 }
 
@@ -551,11 +598,12 @@ hipError_t launch_green_to_f32(const GreenJob *jobs, int njobs, const double2 *s
 
 #define OCTOPIC_CONV_INSTANCES(C)                                                                                     \
     template hipError_t launch_rows_fwd<C>(const Plane *, int, unsigned long long *, int64_t, C *, int64_t, int, int,  \
-                                           double, const FFTPlanDev &, const ErrorInfo *, const LumArgs &, hipStream_t); \
+                                           double, const FFTPlanDev &, const ErrorInfo *, const LumArgs &, hipStream_t, \
+                                           int);                                                                       \
     template hipError_t launch_cols_conv<C>(const Plane *, int, C *, int64_t, const C *, int64_t, int, int,            \
-                                            const FFTPlanDev &, const ErrorInfo *, const LumArgs &, hipStream_t);      \
+                                            const FFTPlanDev &, const ErrorInfo *, const LumArgs &, hipStream_t, int); \
     template hipError_t launch_rows_inv<C>(const Plane *, int, const C *, int64_t, real_of<C> *, int, int,             \
-                                           const FFTPlanDev &, const ErrorInfo *, hipStream_t);                        \
+                                           const FFTPlanDev &, const ErrorInfo *, hipStream_t, int);                   \
     template hipError_t launch_field<real_of<C>>(const Inter *, int, const Plane *, const real_of<C> *, real_of<C> *, \
                                                  int, int, int, int, const ErrorInfo *, hipStream_t);
 OCTOPIC_CONV_INSTANCES(double2)

@@ -276,17 +276,20 @@ constexpr int kLumBlocks = 16;   // blocks per pair of the luminosity overlap su
 hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride, int nslots, int nx, int ny,
                               int integrated, const FFTPlanDev &px, const FFTPlanDev &py, const ErrorInfo *err,
                               hipStream_t s, int full);
+// The charge-path transform passes.  group: transforms per workgroup, 2, 4 or 8 (halved while its LDS does
+// not fit); results do not depend on it.
 template <class C>   // C = double2 (FP64 mode) or float2 (FP32 mode)
 hipError_t launch_rows_fwd(const Plane *planes, int nplanes, unsigned long long *charge, int64_t charge_stride, C *qh,
                            int64_t qh_stride, int nx, int ny, double inv_scale, const FFTPlanDev &px, const ErrorInfo *err,
-                           const LumArgs &lum, hipStream_t s);   // lum.npairs = 0: no luminosity sums
+                           const LumArgs &lum, hipStream_t s, int group);   // lum.npairs = 0: no luminosity sums
 template <class C>
 // lum.npairs > 0: the launch also folds the luminosity partials of the row launch before it
 hipError_t launch_cols_conv(const Plane *planes, int nplanes, C *qh, int64_t qh_stride, const C *gspec, int64_t slot_stride,
-                            int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, const LumArgs &lum, hipStream_t s);
+                            int nx, int ny, const FFTPlanDev &py, const ErrorInfo *err, const LumArgs &lum, hipStream_t s,
+                            int group);
 template <class C>
 hipError_t launch_rows_inv(const Plane *planes, int nplanes, const C *qh, int64_t qh_stride, real_of<C> *phi, int nx,
-                           int ny, const FFTPlanDev &px, const ErrorInfo *err, hipStream_t s);
+                           int ny, const FFTPlanDev &px, const ErrorInfo *err, hipStream_t s, int group);
 template <class R>
 hipError_t launch_field(const Inter *inters, int ninters, const Plane *planes, const R *phi, R *table, int nx, int ny,
                         int fourth, int npl, const ErrorInfo *err, hipStream_t s);

@@ -256,6 +256,7 @@ struct octopic_handle {
     double deposit_trial_ms[2] = {0.0, 0.0};
     std::vector<hipEvent_t> deposit_ev;   // per batch: before and after the deposit launches (trials)
     int deposit_min_span = kChunkSpan;   // smallest items the two-per-CU rule may make
+    int fft_group = 4;                   // transforms per workgroup of the charge-path transform passes (2, 4, 8)
     // Kick launch orders and deposit work queues: per-launch counters and launch orders (larger slices first).
     DBuf<int32_t> korder;
     DBuf<int> dqueue;            // [batch][field planes, luminosity planes]
@@ -708,6 +709,7 @@ const ShapeKnob kShapeKnobs[] = {
     {"deposit_span", &octopic_handle::deposit_span, 1024, 1 << 20, false},
     {"deposit_multi", &octopic_handle::deposit_multi, -1, 1, false},
     {"deposit_min_span", &octopic_handle::deposit_min_span, 1024, 1 << 20, false},
+    {"fft_transforms_per_block", &octopic_handle::fft_group, 2, 8, true},
 };
 const ShapeKnob *find_knob(const char *name) {
     for (const auto &k : kShapeKnobs)
@@ -798,6 +800,7 @@ extern "C" int octopic_autotune(octopic_handle *h, octopic_beam *beam1, octopic_
             {&octopic_handle::kick_target_blocks, {512, 1024, 2048}},
             {&octopic_handle::deposit_span, {4096, 8192, 16384}},
             {&octopic_handle::kick_pass_windows, {-1, 0, 1, 2}},
+            {&octopic_handle::fft_group, {4, 2, 8}},
         };
         double best = measure();
         for (const auto &cd : cands) {   // coordinate descent: one knob at a time, keep the fastest value
@@ -1549,17 +1552,17 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             {
                 Timed t(h, F_ROWS_FWD);
                 HIPCHECK(launch_rows_fwd(h->planes.p, nfp, h->charge.p, charge_stride, qh, qh_stride, nx, ny, inv_scale,
-                                         h->px.dev, h->errinfo.p, lum, s));
+                                         h->px.dev, h->errinfo.p, lum, s, h->fft_group));
             }
             {
                 Timed t(h, F_COLS);
                 HIPCHECK(launch_cols_conv(h->planes.p, nfp, qh, qh_stride, spec, h->slot_stride, nx, ny, h->py.dev,
-                                          h->errinfo.p, lum, s));
+                                          h->errinfo.p, lum, s, h->fft_group));
             }
             {
                 Timed t(h, F_ROWS_INV);
                 HIPCHECK(launch_rows_inv(h->planes.p, nfp, (const decltype(qh))qh, qh_stride, phi, nx, ny, h->px.dev,
-                                         h->errinfo.p, s));
+                                         h->errinfo.p, s, h->fft_group));
             }
             {
                 Timed t(h, F_FIELD);

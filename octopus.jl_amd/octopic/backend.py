@@ -178,7 +178,7 @@ class PICCollider:
         return lum.value
 
     LAUNCH_SHAPES = ("transverse_tiles", "kick_target_blocks", "kick_pass_windows", "deposit_span", "deposit_multi",
-                     "deposit_min_span")
+                     "deposit_min_span", "fft_transforms_per_block")
 
     def set_launch_shape(self, name: str, value: int):
         """A launch shape of this handle (results do not depend on it; see octopic_set_launch_shape)."""

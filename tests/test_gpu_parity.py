@@ -624,7 +624,7 @@ def test_autotune_keeps_results_and_cache(oracle, gpu):
     shape = tuned.autotune(be, bp, trials=1)
     assert shape["transverse_tiles"] in (8, 16, 32) and shape["kick_target_blocks"] in (512, 1024, 2048)
     assert shape["deposit_span"] in (4096, 8192, 16384) and shape["kick_pass_windows"] in (-1, 0, 1, 2)
-    assert shape["deposit_multi"] in (-1, 0, 1)
+    assert shape["deposit_multi"] in (-1, 0, 1) and shape["fft_transforms_per_block"] in (2, 4, 8)
     assert shape["collide_ms"] > 0
     assert np.array_equal(be.to_host(), e) and np.array_equal(bp.to_host(), p)
     st = tuned.stats()
@@ -635,6 +635,34 @@ def test_autotune_keeps_results_and_cache(oracle, gpu):
     assert torch.equal(be.coords, ce.coords) and torch.equal(bp.coords, cp.coords)
     with pytest.raises(Exception):
         tuned.set_launch_shape("transverse_tiles", 3)
+
+
+@pytest.mark.parametrize("kw", [dict(grid=(128, 128)), dict(grid=(128, 128), precision="fp32"),
+                                dict(grid=(64, 48), interaction_grid="node", slice_interpolation="linear"),
+                                dict(grid=(320, 40))])
+def test_fft_group_shapes_bit_identical(oracle, gpu, kw):
+    """Two, four or eight transforms per workgroup in the charge-row, convolution-column and inverse-row passes
+    give the same transforms, so coordinates, luminosities and cache history are bit-identical; a 320-node
+    axis (640-point transforms) is past the LDS of eight FP64 transforms per workgroup and runs with four."""
+    e, p = flat_beams(oracle, 20000, 8000)
+    solver = PICPoissonSolver(slicing=LongitudinalSlicing(nslices=5, method="normal_quantile"), deposit_method="TSC",
+                              field_derivative="fourth", **kw)
+    outs = []
+    for g in (4, 2, 8):
+        col = PICCollider(solver)
+        col.set_launch_shape("fft_transforms_per_block", g)
+        assert col.launch_shape()["fft_transforms_per_block"] == g
+        be, bp = Beam.from_host(e, PE, device=gpu), Beam.from_host(p, PP, device=gpu)
+        lums = [col.collide(be, bp) for _ in range(2)]
+        outs.append((lums, be.to_host(), bp.to_host(), col.stats()))
+    for g, o in zip((2, 8), outs[1:]):
+        assert o[0] == outs[0][0], g
+        assert np.array_equal(o[1], outs[0][1]) and np.array_equal(o[2], outs[0][2]), g
+        assert o[3] == outs[0][3], g
+    col = PICCollider(solver)
+    for bad in (1, 3, 16):
+        with pytest.raises(Exception):
+            col.set_launch_shape("fft_transforms_per_block", bad)
 
 
 @pytest.mark.parametrize("beams", ["round", "flat"])
