@@ -264,15 +264,19 @@ struct octopic_handle {
     DBuf<double> srt[2][6];     // tile-ordered states (tiles > 0)
     DBuf<int64_t> perm_srt[2];
     DBuf<int32_t> inv_srt[2];   // inverse of perm_srt
-    DBuf<unsigned long long> rank_keys, rank_sorted;   // :equal_count rank sort (one beam at a time)
-    DBuf<int64_t> rank_vals;
+    DBuf<unsigned long long> rank_keys[2], rank_sorted[2];   // :equal_count rank sort, per beam
+    DBuf<int64_t> rank_vals[2];
     DBuf<unsigned long long> box;   // transverse bound slots of both beams
     std::vector<double> hq[2];      // host quantiles / positions of the slicing
     DBuf<int> thist;
     DBuf<uint16_t> sid[2];
     DBuf<int> hist[2];
-    DBuf<double> lanes, mm, clanes;
+    DBuf<double> lanes, mm, clanes;     // slicing scratch: one share per beam (the two beams slice concurrently)
     DBuf<unsigned long long> bins;
+    int clanes_share = 0;               // slices per beam share of clanes
+    // beam 2's slicing runs on side_stream between two events of the handle's stream
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DBuf<double *> ptrs;        // [beam_in 2x6][st 2x6]
     DBuf<BatchPair> pairs;
     DBuf<unsigned long long> keys;   // bounds slots [2 * widest batch][kBoundSlots], zero between batches
@@ -493,9 +497,9 @@ void run_slicing(octopic_handle *h, int b, const octopic_slicing &s, int64_t n) 
     // the :equal_count rank sort's passes count 256 digits per block
     h->hist[b].ensure(slicing_hist_entries(n, s.method == OCTOPIC_SLICE_EQUAL_COUNT ? 256 : ns));
     if (s.method == OCTOPIC_SLICE_EQUAL_COUNT) {
-        h->rank_keys.ensure(n);
-        h->rank_sorted.ensure(n);
-        h->rank_vals.ensure(n);
+        h->rank_keys[b].ensure(n);
+        h->rank_sorted[b].ensure(n);
+        h->rank_vals[b].ensure(n);
     }
 }
 
@@ -576,6 +580,9 @@ void octopic_destroy(octopic_handle *h) {
     hipSetDevice(h->device);
     hipDeviceSynchronize();
     if (h->own_stream) hipStreamDestroy(h->own_stream);
+    if (h->side_stream) hipStreamDestroy(h->side_stream);
+    if (h->ev_fork) hipEventDestroy(h->ev_fork);
+    if (h->ev_join) hipEventDestroy(h->ev_join);
     for (auto e : h->ev_pool) hipEventDestroy(e);
     for (auto e : h->deposit_ev) hipEventDestroy(e);
     if (h->h_tab) hipHostFree(h->h_tab);
@@ -588,7 +595,7 @@ void octopic_destroy(octopic_handle *h) {
         }
         h->perm[b].release(); h->perm_srt[b].release(); h->inv_srt[b].release(); h->sid[b].release(); h->hist[b].release();
     }
-    h->rank_keys.release(); h->rank_sorted.release(); h->rank_vals.release();
+    for (int b = 0; b < 2; ++b) { h->rank_keys[b].release(); h->rank_sorted[b].release(); h->rank_vals[b].release(); }
     h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
     h->korder.release(); h->dqueue.release(); h->dorder.release();
     h->keys.release(); h->chunkbox.release(); h->box.release(); h->thist.release(); h->inters.release(); h->planes.release(); h->jobs.release(); h->lumgeo.release();
@@ -616,6 +623,9 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         h->device = device;
         HIPCHECK(hipSetDevice(device));
         HIPCHECK(hipStreamCreate(&h->own_stream));   // default priority; orders with the null stream
+        HIPCHECK(hipStreamCreate(&h->side_stream));  // default priority too
+        HIPCHECK(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
         HIPCHECK(generate_scratch_init(device));     // the allocation-free generator's per-device workspace
         h->stream = h->own_stream;
         h->nx = opt->nx; h->ny = opt->ny;
@@ -655,12 +665,13 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         h->counters.ensure(16);   // live counters, then the snapshot a collide takes before its batches
         HIPCHECK(hipMemset(h->errinfo.p, 0, sizeof(ErrorInfo)));
         HIPCHECK(hipMemset(h->counters.p, 0, 16 * sizeof(unsigned long long)));
-        h->lanes.ensure(kFoldLanes);
-        h->mm.ensure(2 * (kFoldLanes / 256));
-        h->bins.ensure(16384);
-        HIPCHECK(hipMemset(h->bins.p, 0, 16384 * sizeof(unsigned long long)));
+        h->lanes.ensure(2 * kFoldLanes);
+        h->mm.ensure(2 * 2 * (kFoldLanes / 256));
+        h->bins.ensure(2 * 16384);
+        HIPCHECK(hipMemset(h->bins.p, 0, 2 * 16384 * sizeof(unsigned long long)));
         const int ns1 = opt->slicing1.nslices, ns2 = opt->slicing2.nslices;
-        h->clanes.ensure((size_t)std::max(ns1, ns2) * kFoldLanes);
+        h->clanes_share = std::max(ns1, ns2);
+        h->clanes.ensure((size_t)2 * h->clanes_share * kFoldLanes);
         h->slot_stride = (int64_t)(h->nx + 1) * 2 * h->ny;
         if (opt->green_cache == OCTOPIC_CACHE_SLICE_PAIR && h->mode != 2 && !h->soft) {
             h->nkeys = 2 * ns1 * ns2;
@@ -1154,8 +1165,12 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
     {
         Timed t(h, F_SLICING);
         const octopic_slicing *sl[2] = {&o.slicing1, &o.slicing2};
+        for (int b = 0; b < 2; ++b) run_slicing(h, b, *sl[b], n[b]);
+        // the two beams' slicing chains are independent (own tables, states and scratch shares): beam 2's
+        // runs on the side stream, forked from and joined back into the handle's stream
+        HIPCHECK(hipEventRecord(h->ev_fork, s));
+        HIPCHECK(hipStreamWaitEvent(h->side_stream, h->ev_fork, 0));
         for (int b = 0; b < 2; ++b) {
-            run_slicing(h, b, *sl[b], n[b]);
             SlicingArgs a{};
             a.z = bp[b][4]; a.x = bp[b][0]; a.y = bp[b][2];
             a.box = tiles > 0 ? h->box.p + 4 * b : nullptr;
@@ -1164,8 +1179,10 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.centroid = sl[b]->center_position == OCTOPIC_CENTER_CENTROID ? 1 : 0;
             a.q = h->q.p + (size_t)b * kMaxSlices;
             a.tab = h->tab.p + b;
-            a.lanes = h->lanes.p; a.mm_partial = h->mm.p; a.centroid_lanes = h->clanes.p;
-            a.bin_counts = h->bins.p;
+            a.lanes = h->lanes.p + (size_t)b * kFoldLanes;
+            a.mm_partial = h->mm.p + (size_t)b * 2 * (kFoldLanes / 256);
+            a.centroid_lanes = h->clanes.p + (size_t)b * h->clanes_share * kFoldLanes;
+            a.bin_counts = h->bins.p + (size_t)b * 16384;
             a.sid = h->sid[b].p; a.hist = h->hist[b].p;
             a.beam = h->ptrs.p + b * 6;
             a.st = h->ptrs.p + 12 + b * 6;
@@ -1173,10 +1190,12 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.perm = h->perm[b].p;
             a.err = h->errinfo.p;
             if (a.method == OCTOPIC_SLICE_EQUAL_COUNT) {
-                a.rank_keys = h->rank_keys.p; a.rank_sorted = h->rank_sorted.p; a.rank_vals = h->rank_vals.p;
+                a.rank_keys = h->rank_keys[b].p; a.rank_sorted = h->rank_sorted[b].p; a.rank_vals = h->rank_vals[b].p;
             }
-            HIPCHECK(launch_slicing(a, s));
+            HIPCHECK(launch_slicing(a, b == 0 ? s : h->side_stream));
         }
+        HIPCHECK(hipEventRecord(h->ev_join, h->side_stream));
+        HIPCHECK(hipStreamWaitEvent(s, h->ev_join, 0));
     }
     HIPCHECK(hipMemcpyAsync(h->h_tab, h->tab.p, 2 * sizeof(SliceTable), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(h->h_err, h->errinfo.p, sizeof(ErrorInfo), hipMemcpyDeviceToHost, s));
