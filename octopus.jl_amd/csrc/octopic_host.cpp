@@ -842,7 +842,9 @@ extern "C" int octopic_collide(octopic_handle *h, octopic_beam *beam1, octopic_b
     try {
         HIPCHECK(hipSetDevice(h->device));
         if (h->unfinished) {
-            // kernels of the abandoned collide may have left charge or bounds slots filled
+            // kernels of the abandoned collide may have left charge or bounds slots filled (and a failure
+            // between the slicing's fork and join may have left work on the side stream)
+            HIPCHECK(hipStreamSynchronize(h->side_stream));
             if (h->charge.p) HIPCHECK(hipMemsetAsync(h->charge.p, 0, h->charge.n * sizeof(unsigned long long), h->stream));
             if (h->keys.p) HIPCHECK(hipMemsetAsync(h->keys.p, 0, h->keys.n * sizeof(unsigned long long), h->stream));
             HIPCHECK(launch_cache_invalidate(h->cache.p, h->nkeys, h->epoch, h->stream));
