@@ -23,47 +23,84 @@ namespace octopic {
 constexpr int kSliceBlock = 1024;     // particles per classify/scatter block
 constexpr int kSliceThreads = 256;
 
-__global__ __launch_bounds__(256) void k_zlanes(const double *z, int64_t n, const SliceTable *tab, int pass, double *lanes,
-                                                double *mm_partial) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;   // lane, 0..4095
+// A block takes 64 lanes: wave 0 sums them in the fold's order from LDS while waves 1..15 stream the next
+// kZChunk rows of them into the other half of a double buffer (one row of 64 lanes is 64 consecutive
+// values), so the in-order sums do not wait on HBM.  Pass 1: the sums and z's min/max (block partials
+// to mm_partial); pass 2: the sums of (z - mean)^2.
+constexpr int kZWaves = 16;                                  // per block: one summing wave, 15 loading
+constexpr int kZRowsPerLoader = 8;
+constexpr int kZChunk = (kZWaves - 1) * kZRowsPerLoader;     // rows per buffer half
+static_assert(kZFoldBlocks * 64 == kFoldLanes, "64 lanes per block");
+
+__global__ __launch_bounds__(64 * kZWaves) void k_zlanes(const double *z, int64_t n, const SliceTable *tab, int pass,
+                                                         double *lanes, double *mm_partial) {
+    extern __shared__ double zbuf[];                          // [2][kZChunk][64]
+    __shared__ double slo[kZWaves], shi[kZWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = blockIdx.x * 64 + lane;                     // this thread's lane of the fold
+    const int64_t rows = (n + kFoldLanes - 1) / kFoldLanes;  // rows of the fold, the last one partial
+    const int64_t nchunks = (rows + kZChunk - 1) / kZChunk;
     const double mu = pass == 2 ? tab->mean : 0.0;
-    double acc = 0.0;
     double lo = INFINITY, hi = -INFINITY;
-    int64_t i = t;
-    // Loads are issued ahead of the in-order accumulation (32 in flight per lane: the 4096 lanes are
-    // few); the summation order is the lane fold's.
-    constexpr int U = 32;
-    for (; i + (U - 1) * kFoldLanes < n; i += U * kFoldLanes) {
-        double v[U];
+    auto load = [&](int64_t c) {   // loader wave w: rows c*kZChunk + (w - 1) + 15 k of the chunk
+        double v[kZRowsPerLoader];
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = z[i + u * kFoldLanes];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (pass == 2) v[u] = (v[u] - mu) * (v[u] - mu);
-            else { lo = jmin(lo, v[u]); hi = jmax(hi, v[u]); }
+        for (int k = 0; k < kZRowsPerLoader; ++k) {
+            const int64_t row = c * kZChunk + (wave - 1) + (kZWaves - 1) * k;
+            const int64_t i = row * kFoldLanes + t;
+            v[k] = (row < rows && i < n) ? z[i] : 0.0;
+            if (pass == 1 && row < rows && i < n) { lo = jmin(lo, v[k]); hi = jmax(hi, v[k]); }
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) acc += v[u];
-    }
-    for (; i < n; i += kFoldLanes) {
-        double v = z[i];
-        if (pass == 2) v = (v - mu) * (v - mu);
-        else { lo = jmin(lo, v); hi = jmax(hi, v); }
-        acc += v;
-    }
-    lanes[t] = acc;
-    if (pass == 1) {
-        __shared__ double slo[256], shi[256];
-        slo[threadIdx.x] = lo; shi[threadIdx.x] = hi;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (threadIdx.x < w) {
-                slo[threadIdx.x] = jmin(slo[threadIdx.x], slo[threadIdx.x + w]);
-                shi[threadIdx.x] = jmax(shi[threadIdx.x], shi[threadIdx.x + w]);
+        for (int k = 0; k < kZRowsPerLoader; ++k)
+            zbuf[((c & 1) * kZChunk + (wave - 1) + (kZWaves - 1) * k) * 64 + lane] = v[k];
+    };
+    double acc = 0.0;
+    if (wave > 0) load(0);
+    __syncthreads();
+    for (int64_t c = 0; c < nchunks; ++c) {
+        if (wave > 0) {
+            if (c + 1 < nchunks) load(c + 1);
+        } else {
+            // this lane's rows of the chunk (the last row of the fold may miss it), read 16 at a time ahead
+            // of the in-order adds
+            const int64_t r0 = c * kZChunk;
+            const int64_t mine = n > t ? (n - t + kFoldLanes - 1) / kFoldLanes : 0;
+            const int rc = (int)((mine - r0) < kZChunk ? ((mine - r0) > 0 ? (mine - r0) : 0) : kZChunk);
+            const double *b = zbuf + (c & 1) * kZChunk * 64 + lane;
+            int r = 0;
+            for (; r + 16 <= rc; r += 16) {
+                double v[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) v[u] = b[(r + u) * 64];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    if (pass == 2) v[u] = (v[u] - mu) * (v[u] - mu);
+                    acc += v[u];
+                }
             }
-            __syncthreads();
+            for (; r < rc; ++r) {
+                double v = b[r * 64];
+                if (pass == 2) v = (v - mu) * (v - mu);
+                acc += v;
+            }
         }
-        if (threadIdx.x == 0) { mm_partial[2 * blockIdx.x] = slo[0]; mm_partial[2 * blockIdx.x + 1] = shi[0]; }
+        __syncthreads();   // the chunk is summed before its half is loaded again
+    }
+    if (wave == 0) lanes[t] = acc;
+    if (pass == 1) {
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = jmin(lo, __shfl_xor(lo, o));
+            hi = jmax(hi, __shfl_xor(hi, o));
+        }
+        if (lane == 0) { slo[wave] = lo; shi[wave] = hi; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double l = slo[0], h = shi[0];
+            for (int w = 1; w < kZWaves; ++w) { l = jmin(l, slo[w]); h = jmax(h, shi[w]); }
+            mm_partial[2 * blockIdx.x] = l;
+            mm_partial[2 * blockIdx.x + 1] = h;
+        }
     }
 }
 
@@ -71,11 +108,14 @@ __global__ __launch_bounds__(256) void k_zlanes(const double *z, int64_t n, cons
 // sqrt(2)*erfinv(2s/ns - 1) or the sorted `specified` offsets, from the host.
 // The 4096 lane partials are staged in LDS by the whole block, then folded serially in lane order.
 __device__ __forceinline__ double fold_lanes_serial(const double *l) {
+    // 32 values read ahead of their in-order adds, so the chain of adds does not wait on each LDS read
     double s = 0.0;
-    for (int t = 0; t < kFoldLanes; t += 8) {
-        const double l0 = l[t], l1 = l[t + 1], l2 = l[t + 2], l3 = l[t + 3];
-        const double l4 = l[t + 4], l5 = l[t + 5], l6 = l[t + 6], l7 = l[t + 7];
-        s += l0; s += l1; s += l2; s += l3; s += l4; s += l5; s += l6; s += l7;
+    for (int t = 0; t < kFoldLanes; t += 32) {
+        double v[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) v[u] = l[t + u];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) s += v[u];
     }
     return s;
 }
@@ -83,13 +123,16 @@ __device__ __forceinline__ double fold_lanes_serial(const double *l) {
 __global__ __launch_bounds__(256) void k_zfold(const double *lanes, const double *mm_partial, int64_t n, SliceTable *tab,
                                                int pass, int method, int ns, const double *q, ErrorInfo *err) {
     __shared__ double sl[kFoldLanes];
+    __shared__ double smm[2 * kZFoldBlocks];
     for (int t = threadIdx.x; t < kFoldLanes; t += blockDim.x) sl[t] = lanes[t];
+    if (pass == 1)
+        for (int t = threadIdx.x; t < 2 * kZFoldBlocks; t += blockDim.x) smm[t] = mm_partial[t];
     __syncthreads();
     if (threadIdx.x != 0) return;
     const double s = fold_lanes_serial(sl);
     if (pass == 1) {
         double lo = INFINITY, hi = -INFINITY;
-        for (int b = 0; b < kFoldLanes / 256; ++b) { lo = jmin(lo, mm_partial[2 * b]); hi = jmax(hi, mm_partial[2 * b + 1]); }
+        for (int b = 0; b < kZFoldBlocks; ++b) { lo = jmin(lo, smm[2 * b]); hi = jmax(hi, smm[2 * b + 1]); }
         tab->zmin = lo;
         tab->zmax = hi;
         tab->mean = s / (double)n;
@@ -765,9 +808,11 @@ __global__ __launch_bounds__(kSliceThreads) void k_radix_scatter(const unsigned 
 
 hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s) {
     const int nblocks = (int)((a.n + kSliceBlock - 1) / kSliceBlock);
-    hipLaunchKernelGGL(k_zlanes, dim3(kFoldLanes / 256), dim3(256), 0, s, a.z, a.n, a.tab, 1, a.lanes, a.mm_partial);
+    const size_t zlds = sizeof(double) * 2 * kZChunk * 64;
+    hipFuncSetAttribute((const void *)k_zlanes, hipFuncAttributeMaxDynamicSharedMemorySize, (int)zlds);
+    hipLaunchKernelGGL(k_zlanes, dim3(kZFoldBlocks), dim3(64 * kZWaves), zlds, s, a.z, a.n, a.tab, 1, a.lanes, a.mm_partial);
     hipLaunchKernelGGL(k_zfold, dim3(1), dim3(256), 0, s, a.lanes, a.mm_partial, a.n, a.tab, 1, a.method, a.ns, a.q, a.err);
-    hipLaunchKernelGGL(k_zlanes, dim3(kFoldLanes / 256), dim3(256), 0, s, a.z, a.n, a.tab, 2, a.lanes, a.mm_partial);
+    hipLaunchKernelGGL(k_zlanes, dim3(kZFoldBlocks), dim3(64 * kZWaves), zlds, s, a.z, a.n, a.tab, 2, a.lanes, a.mm_partial);
     hipLaunchKernelGGL(k_zfold, dim3(1), dim3(256), 0, s, a.lanes, a.mm_partial, a.n, a.tab, 2, a.method, a.ns, a.q, a.err);
     if (a.method == 0) {
         const int bins = a.ns * a.resolution;

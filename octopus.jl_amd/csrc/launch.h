@@ -36,6 +36,8 @@ struct SlicingArgs {
     int64_t *rank_vals;
 };
 hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s);
+// blocks of the z lane fold (64 lanes each); mm_partial holds 2 values per block
+constexpr int kZFoldBlocks = kFoldLanes / 64;
 
 constexpr int kTileChunk = 4096;     // particles per block visit; block c of a slice takes visits c, c+G, ...
 constexpr int kTileBlockSpan = 16384;   // particles per block and slice the tile order aims for (long tile runs)

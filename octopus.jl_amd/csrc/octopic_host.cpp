@@ -666,7 +666,7 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         HIPCHECK(hipMemset(h->errinfo.p, 0, sizeof(ErrorInfo)));
         HIPCHECK(hipMemset(h->counters.p, 0, 16 * sizeof(unsigned long long)));
         h->lanes.ensure(2 * kFoldLanes);
-        h->mm.ensure(2 * 2 * (kFoldLanes / 256));
+        h->mm.ensure(2 * 2 * kZFoldBlocks);
         h->bins.ensure(2 * 16384);
         HIPCHECK(hipMemset(h->bins.p, 0, 2 * 16384 * sizeof(unsigned long long)));
         const int ns1 = opt->slicing1.nslices, ns2 = opt->slicing2.nslices;
@@ -1182,7 +1182,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.q = h->q.p + (size_t)b * kMaxSlices;
             a.tab = h->tab.p + b;
             a.lanes = h->lanes.p + (size_t)b * kFoldLanes;
-            a.mm_partial = h->mm.p + (size_t)b * 2 * (kFoldLanes / 256);
+            a.mm_partial = h->mm.p + (size_t)b * 2 * kZFoldBlocks;
             a.centroid_lanes = h->clanes.p + (size_t)b * h->clanes_share * kFoldLanes;
             a.bin_counts = h->bins.p + (size_t)b * 16384;
             a.sid = h->sid[b].p; a.hist = h->hist[b].p;
