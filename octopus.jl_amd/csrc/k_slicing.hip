@@ -327,19 +327,63 @@ __device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t base)
     return before + inc - v;
 }
 
-__global__ __launch_bounds__(1024) void k_scan(int *hist, int ns, int nblocks, SliceTable *tab) {
-    const int64_t total = (int64_t)ns * nblocks;
-    const int64_t per = (total + blockDim.x - 1) / blockDim.x;
-    const int64_t lo = threadIdx.x * per < total ? threadIdx.x * per : total, hi = lo + per < total ? lo + per : total;
+// Exclusive scan of `total` ints in place by one block: each wave takes a contiguous segment and scans it
+// 64 values at a time (coalesced loads, shuffles, a running carry), after the segments' bases from a first
+// pass of wave sums.  on(k, value) is called with each entry's exclusive prefix.
+template <class On>
+__device__ __forceinline__ void scan_ints_inplace(int *h, int64_t total, On on) {
+    __shared__ int64_t seg_sum[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    const int64_t seg = (total + nw - 1) / nw;
+    const int64_t lo = wave * seg < total ? wave * seg : total, hi = lo + seg < total ? lo + seg : total;
     int64_t sum = 0;
-    for (int64_t k = lo; k < hi; ++k) sum += hist[k];
-    int64_t run = block_exclusive_scan(sum, 0);
-    for (int64_t k = lo; k < hi; ++k) {
-        int v = hist[k];
-        hist[k] = (int)run;
-        if (k % nblocks == 0) tab->offset[k / nblocks] = run;
-        run += v;
+    for (int64_t k0 = lo; k0 < hi; k0 += 4 * 64) {
+        int v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t k = k0 + u * 64 + lane;
+            v[u] = k < hi ? h[k] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sum += v[u];
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    if (lane == 0) seg_sum[wave] = sum;
+    __syncthreads();
+    int64_t base = 0;
+    for (int w = 0; w < wave; ++w) base += seg_sum[w];
+    for (int64_t k0 = lo; k0 < hi; k0 += 4 * 64) {
+        int v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t k = k0 + u * 64 + lane;
+            v[u] = k < hi ? h[k] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t k = k0 + u * 64 + lane;
+            int64_t inc = v[u];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int64_t t = __shfl_up(inc, o);
+                if (lane >= o) inc += t;
+            }
+            const int64_t ex = base + inc - v[u];
+            if (k < hi) {
+                h[k] = (int)ex;
+                on(k, ex);
+            }
+            base += __shfl(inc, 63);
+        }
+    }
+}
+
+// Exclusive scan of hist (slice-major, ns * nblocks) in place; writes the per-slice offsets.
+__global__ __launch_bounds__(1024) void k_scan(int *hist, int ns, int nblocks, SliceTable *tab) {
+    scan_ints_inplace(hist, (int64_t)ns * nblocks, [&](int64_t k, int64_t ex) {
+        if (k % nblocks == 0) tab->offset[k / nblocks] = ex;
+    });
 }
 
 __global__ void k_counts(SliceTable *tab, int64_t n) {
@@ -751,17 +795,7 @@ __global__ __launch_bounds__(kSliceThreads) void k_radix_hist(const unsigned lon
 
 // exclusive scan of hist (digit-major, kRadixBins * nblocks) in place
 __global__ __launch_bounds__(1024) void k_radix_scan(int *hist, int nblocks) {
-    const int64_t total = (int64_t)kRadixBins * nblocks;
-    const int64_t per = (total + blockDim.x - 1) / blockDim.x;
-    const int64_t lo = threadIdx.x * per < total ? threadIdx.x * per : total, hi = lo + per < total ? lo + per : total;
-    int64_t sum = 0;
-    for (int64_t k = lo; k < hi; ++k) sum += hist[k];
-    int64_t run = block_exclusive_scan(sum, 0);
-    for (int64_t k = lo; k < hi; ++k) {
-        const int v = hist[k];
-        hist[k] = (int)run;
-        run += v;
-    }
+    scan_ints_inplace(hist, (int64_t)kRadixBins * nblocks, [](int64_t, int64_t) {});
 }
 
 __global__ __launch_bounds__(kSliceThreads) void k_radix_scatter(const unsigned long long *kin, const int64_t *vin,
