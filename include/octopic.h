@@ -324,9 +324,10 @@ int octopic_set_timing_families(octopic_handle *h, const char *families);
 int octopic_kernel_times(const octopic_handle *h, double *ms_total, int64_t *launches, int64_t cap,
                          int64_t *n, char *names, int64_t names_cap);
 
-/* Launch shapes (no counterpart in the reference; results are bit-identical for every value, only
- * the speed changes): "transverse_tiles" (0 or a power of two <= 128: in-slice tile order, default
- * 16), "kick_target_blocks" (0..2^20 work items per kick launch, default 1024), "kick_pass_windows"
+/* Launch shapes (no counterpart in the reference; only the speed changes, and the results of the plain
+ * PIC solver are bit-identical for every value; "transverse_tiles" sets the order in which the Gaussian-
+ * subtracted solver and the denoiser sum their slice moments, so their results follow it in the last
+ * bits): "transverse_tiles" (0 or a power of two <= 128: in-slice tile order, default 16), "kick_target_blocks" (0..2^20 work items per kick launch, default 1024), "kick_pass_windows"
  * (-1..2, default -1), "deposit_span" (1024..2^20 source particles per deposit block, default 8192),
  * "deposit_multi" (slice-pair meshes: 1 = each deposit item takes all planes of an interaction, one
  * read of the particles; 0 = one plane per item; -1 = the faster of the two, timed on the device in
