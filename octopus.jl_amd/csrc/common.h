@@ -204,4 +204,29 @@ struct ErrorInfo {
     double a, b;
 };
 
+// phi and E = -grad(phi) at node (x, y) of an nx x ny plane of spacings 1/hxi, 1/hyi (pic_cpu.jl:2531-2591):
+// one-sided second-order differences on the edges, the centred second-order difference one node in when
+// `fourth`, the fourth-order one inside (else second order throughout).  F(i, j) returns phi at node
+// (i, j); only nodes within two of (x, y) on the mesh are read.
+template <class R, class F>
+__device__ __forceinline__ void field_node(const F &phi, int x, int y, int nx, int ny, int fourth, R hxi, R hyi, R &v0,
+                                           R &v1, R &v2) {
+    const R c4 = (R)(1.0 / 12.0);
+    const R h5 = (R)0.5, one5 = (R)1.5, two = (R)2.0, eight = (R)8.0;
+    R ex, ey;
+    if (y == 0) ey = hyi * (one5 * phi(x, 0) - two * phi(x, 1) + h5 * phi(x, 2));
+    else if (y == ny - 1) ey = hyi * (-one5 * phi(x, ny - 1) + two * phi(x, ny - 2) - h5 * phi(x, ny - 3));
+    else if (fourth && (y == 1 || y == ny - 2)) ey = h5 * hyi * (phi(x, y - 1) - phi(x, y + 1));
+    else if (fourth) ey = c4 * hyi * ((phi(x, y + 2) - phi(x, y - 2)) + eight * (phi(x, y - 1) - phi(x, y + 1)));
+    else ey = h5 * hyi * (phi(x, y - 1) - phi(x, y + 1));
+    if (x == 0) ex = hxi * (one5 * phi(0, y) - two * phi(1, y) + h5 * phi(2, y));
+    else if (x == nx - 1) ex = hxi * (-one5 * phi(nx - 1, y) + two * phi(nx - 2, y) - h5 * phi(nx - 3, y));
+    else if (fourth && (x == 1 || x == nx - 2)) ex = h5 * hxi * (phi(x - 1, y) - phi(x + 1, y));
+    else if (fourth) ex = c4 * hxi * ((phi(x + 2, y) - phi(x - 2, y)) + eight * (phi(x - 1, y) - phi(x + 1, y)));
+    else ex = h5 * hxi * (phi(x - 1, y) - phi(x + 1, y));
+    v0 = phi(x, y);
+    v1 = ex;
+    v2 = ey;
+}
+
 }  // namespace octopic

@@ -412,7 +412,6 @@ __global__ __launch_bounds__(256) void k_field(const Inter *inters, const Plane 
     const int lx = threadIdx.x % kFieldTile, ly = threadIdx.x / kFieldTile;
     const int x = tx0 + lx, y = ty0 + ly;
     const bool inside = x < nx && y < ny;
-    const R c4 = (R)(1.0 / 12.0);
     for (int q = 0; q < npl; ++q) {
         const bool live = q < it.nplanes;
         if (live) {
@@ -428,22 +427,7 @@ __global__ __launch_bounds__(256) void k_field(const Inter *inters, const Plane 
         if (live && inside) {
             const Plane &pl = planes[it.plane0 + q];
             auto F = [&](int i, int j) { return ph[j - ty0 + 2][i - tx0 + 2]; };
-            const R hxi = (R)(1.0 / pl.hx), hyi = (R)(1.0 / pl.hy);
-            const R h5 = (R)0.5, one5 = (R)1.5, two = (R)2.0, eight = (R)8.0;
-            R ex, ey;
-            if (y == 0) ey = hyi * (one5 * F(x, 0) - two * F(x, 1) + h5 * F(x, 2));
-            else if (y == ny - 1) ey = hyi * (-one5 * F(x, ny - 1) + two * F(x, ny - 2) - h5 * F(x, ny - 3));
-            else if (fourth && (y == 1 || y == ny - 2)) ey = h5 * hyi * (F(x, y - 1) - F(x, y + 1));
-            else if (fourth) ey = c4 * hyi * ((F(x, y + 2) - F(x, y - 2)) + eight * (F(x, y - 1) - F(x, y + 1)));
-            else ey = h5 * hyi * (F(x, y - 1) - F(x, y + 1));
-            if (x == 0) ex = hxi * (one5 * F(0, y) - two * F(1, y) + h5 * F(2, y));
-            else if (x == nx - 1) ex = hxi * (-one5 * F(nx - 1, y) + two * F(nx - 2, y) - h5 * F(nx - 3, y));
-            else if (fourth && (x == 1 || x == nx - 2)) ex = h5 * hxi * (F(x - 1, y) - F(x + 1, y));
-            else if (fourth) ex = c4 * hxi * ((F(x + 2, y) - F(x - 2, y)) + eight * (F(x - 1, y) - F(x + 1, y)));
-            else ex = h5 * hxi * (F(x - 1, y) - F(x + 1, y));
-            v0 = F(x, y);
-            v1 = ex;
-            v2 = ey;
+            field_node<R>(F, x, y, nx, ny, fourth, (R)(1.0 / pl.hx), (R)(1.0 / pl.hy), v0, v1, v2);
         }
         R *sv = stage + (lx * kFieldTile + ly) * 9 + 3 * q;
         sv[0] = v0; sv[1] = v1; sv[2] = v2;
