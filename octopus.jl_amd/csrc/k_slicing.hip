@@ -394,6 +394,7 @@ __global__ void k_counts(SliceTable *tab, int64_t n) {
     }
 }
 
+static_assert(kMaxSlices <= 256, "slice ids are ranked by their 8 low bits");
 __global__ __launch_bounds__(kSliceThreads) void k_scatter(const double *const *beam, int64_t n, const uint16_t *sid,
                                                            const int *hist, int nblocks, int ns, double *const *st,
                                                            int64_t *perm) {
@@ -407,19 +408,19 @@ __global__ __launch_bounds__(kSliceThreads) void k_scatter(const double *const *
         for (int k = threadIdx.x; k < (kSliceThreads / 64) * kMaxSlices; k += blockDim.x) (&wcnt[0][0])[k] = 0;
         __syncthreads();
         const int64_t i = base + r * kSliceThreads + threadIdx.x;
-        const int s = i < n ? (int)sid[i] : -1;
-        int rank = 0;
-        unsigned long long remaining = __ballot(s >= 0);
-        while (remaining) {
-            int leader = __ffsll((long long)remaining) - 1;
-            int ls = __shfl(s, leader);
-            unsigned long long m = __ballot(s == ls);
-            if (s == ls) rank = __popcll(m & lt);
-            if (lane == leader) wcnt[wave][ls] = __popcll(m);
-            remaining &= ~m;
+        const bool valid = i < n;
+        const int s = valid ? (int)sid[i] : 0;
+        // the wave's lanes in the same slice: agree on every id bit (ids < kMaxSlices <= 256: 8 bits)
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const unsigned long long m = __ballot(valid && ((s >> bit) & 1));
+            peers &= ((s >> bit) & 1) ? m : ~m;
         }
+        const int rank = __popcll(peers & lt);
+        if (valid && rank == 0) wcnt[wave][s] = __popcll(peers);
         __syncthreads();
-        if (s >= 0) {
+        if (valid) {
             int pos = hist[(size_t)s * nblocks + blockIdx.x] + run[s] + rank;
             for (int w = 0; w < wave; ++w) pos += wcnt[w][s];
             for (int c = 0; c < 6; ++c) st[c][pos] = beam[c][i];
