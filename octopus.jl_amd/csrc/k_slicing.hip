@@ -397,7 +397,7 @@ __global__ void k_counts(SliceTable *tab, int64_t n) {
 static_assert(kMaxSlices <= 256, "slice ids are ranked by their 8 low bits");
 __global__ __launch_bounds__(kSliceThreads) void k_scatter(const double *const *beam, int64_t n, const uint16_t *sid,
                                                            const int *hist, int nblocks, int ns, double *const *st,
-                                                           int64_t *perm) {
+                                                           int64_t *perm, int32_t *inv) {
     __shared__ int run[kMaxSlices];
     __shared__ int wcnt[kSliceThreads / 64][kMaxSlices];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -425,6 +425,7 @@ __global__ __launch_bounds__(kSliceThreads) void k_scatter(const double *const *
             for (int w = 0; w < wave; ++w) pos += wcnt[w][s];
             for (int c = 0; c < 6; ++c) st[c][pos] = beam[c][i];
             perm[pos] = i;
+            if (inv) inv[i] = pos;   // coalesced: i is this thread's own index
         }
         __syncthreads();
         for (int k = threadIdx.x; k < ns; k += blockDim.x) {
@@ -468,21 +469,6 @@ __global__ __launch_bounds__(256) void k_centroid_fold(const double *lanes, Slic
         tab->center[s] = fold_lanes_serial(sl) / (double)cnt;
     } else {
         tab->center[s] = (tab->boundary[s] + tab->boundary[s + 1]) / 2.0;
-    }
-}
-
-// Scatter the slice-contiguous states back into the beam (x, px, y, py, pz; z is never modified).
-__global__ __launch_bounds__(256) void k_store(double *const *beam, int64_t n, double *const *st, const int64_t *perm,
-                                               const ErrorInfo *err, int with_z) {
-    if (err->code != 0) return;
-    for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < n; pos += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = perm[pos];
-        beam[0][i] = st[0][pos];
-        beam[1][i] = st[1][pos];
-        beam[2][i] = st[2][pos];
-        beam[3][i] = st[3][pos];
-        if (with_z) beam[4][i] = st[4][pos];
-        beam[5][i] = st[5][pos];
     }
 }
 
@@ -702,25 +688,26 @@ hipError_t launch_tile_sort(const TileSortArgs &a, int G, hipStream_t s) {
 // Tile-ordered states back into the beam through the inverse permutation: writes run in beam order
 // (coalesced) and the reads gather.
 __global__ __launch_bounds__(256) void k_store_gather(double *const *beam, int64_t n, double *const *st, const int32_t *inv,
-                                                      const ErrorInfo *err) {
+                                                      const ErrorInfo *err, int with_z) {
     if (err->code != 0) return;
-    double *X = beam[0], *PX = beam[1], *Y = beam[2], *PY = beam[3], *PZ = beam[5];
-    const double *SX = st[0], *SPX = st[1], *SY = st[2], *SPY = st[3], *SPZ = st[5];
+    double *X = beam[0], *PX = beam[1], *Y = beam[2], *PY = beam[3], *Z = beam[4], *PZ = beam[5];
+    const double *SX = st[0], *SPX = st[1], *SY = st[2], *SPY = st[3], *SZ = st[4], *SPZ = st[5];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t pos = inv[i];
         X[i] = SX[pos];
         PX[i] = SPX[pos];
         Y[i] = SY[pos];
         PY[i] = SPY[pos];
+        if (with_z) Z[i] = SZ[pos];
         PZ[i] = SPZ[pos];
     }
 }
 
 hipError_t launch_store_gather(double *const *beam, int64_t n, double *const *st, const int32_t *inv, const ErrorInfo *err,
-                               hipStream_t s) {
+                               hipStream_t s, int with_z) {
     int g = (int)((n + 255) / 256);
     if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_store_gather, dim3(g), dim3(256), 0, s, beam, n, st, inv, err);
+    hipLaunchKernelGGL(k_store_gather, dim3(g), dim3(256), 0, s, beam, n, st, inv, err, with_z);
     return hipGetLastError();
 }
 
@@ -757,7 +744,7 @@ __global__ void k_rank_table(SliceTable *tab, const unsigned long long *sorted, 
 
 // states in rank order (st[c][r] = beam[c][perm[r]]) and the beam's transverse box
 __global__ __launch_bounds__(kSliceThreads) void k_rank_gather(const double *const *beam, int64_t n, const int64_t *perm,
-                                                               double *const *st, unsigned long long *box) {
+                                                               int32_t *inv, double *const *st, unsigned long long *box) {
     double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = perm[r];
@@ -768,6 +755,7 @@ __global__ __launch_bounds__(kSliceThreads) void k_rank_gather(const double *con
         st[3][r] = beam[3][i];
         st[4][r] = beam[4][i];
         st[5][r] = beam[5][i];
+        if (inv) inv[i] = (int32_t)r;
         v[0] = jmin(v[0], x); v[1] = jmax(v[1], x); v[2] = jmin(v[2], y); v[3] = jmax(v[3], y);
     }
     if (box) fold_box(v, box);
@@ -875,14 +863,14 @@ hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s) {
                                nblocks, kout, vout);
         }
         hipLaunchKernelGGL(k_rank_table, dim3(1), dim3(64), 0, s, a.tab, a.rank_sorted, a.n, a.ns);
-        hipLaunchKernelGGL(k_rank_gather, dim3(g), dim3(kSliceThreads), 0, s, a.beam, a.n, a.perm, a.st, a.box);
+        hipLaunchKernelGGL(k_rank_gather, dim3(g), dim3(kSliceThreads), 0, s, a.beam, a.n, a.perm, a.inv, a.st, a.box);
     } else {
         hipLaunchKernelGGL(k_classify, dim3(nblocks), dim3(kSliceThreads), 0, s, a.z, a.x, a.y, a.n, a.tab, a.sid, a.hist,
                            nblocks, a.method == 2 ? 1 : 0, a.box, a.err);
         hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.hist, a.ns, nblocks, a.tab);
         hipLaunchKernelGGL(k_counts, dim3(1), dim3(256), 0, s, a.tab, a.n);
         hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(kSliceThreads), 0, s, a.beam, a.n, a.sid, a.hist, nblocks, a.ns,
-                           a.st, a.perm);
+                           a.st, a.perm, a.inv);
     }
     hipLaunchKernelGGL(k_centroid_lanes, dim3(kFoldLanes / 256, a.ns), dim3(256), 0, s, a.st_z, a.tab, a.centroid_lanes);
     hipLaunchKernelGGL(k_centroid_fold, dim3(a.ns), dim3(256), 0, s, a.centroid_lanes, a.tab, a.n, a.centroid);
@@ -891,12 +879,5 @@ hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s) {
 
 int64_t slicing_hist_entries(int64_t n, int ns) { return ((n + kSliceBlock - 1) / kSliceBlock) * (int64_t)ns; }
 
-hipError_t launch_store(double *const *beam, int64_t n, double *const *st, const int64_t *perm, const ErrorInfo *err,
-                        hipStream_t s, int with_z) {
-    int g = (int)((n + 255) / 256);
-    if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_store, dim3(g), dim3(256), 0, s, beam, n, st, perm, err, with_z);
-    return hipGetLastError();
-}
 
 }  // namespace octopic

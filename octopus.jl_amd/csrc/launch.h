@@ -34,6 +34,7 @@ struct SlicingArgs {
     // in perm); hist then holds 256 digit counts per block
     unsigned long long *rank_keys, *rank_sorted;
     int64_t *rank_vals;
+    int32_t *inv = nullptr;          // when set: inv[i] = the slice-order position of particle i
 };
 hipError_t launch_slicing(const SlicingArgs &a, hipStream_t s);
 // blocks of the z lane fold (64 lanes each); mm_partial holds 2 values per block
@@ -58,11 +59,10 @@ struct TileSortArgs {
 };
 hipError_t launch_tile_sort(const TileSortArgs &a, int G, hipStream_t s);
 int64_t slicing_hist_entries(int64_t n, int ns);
-// with_z: z written back too (solvers whose drifts move it)
-hipError_t launch_store(double *const *beam, int64_t n, double *const *st, const int64_t *perm, const ErrorInfo *err,
-                        hipStream_t s, int with_z = 0);
+// The slice-ordered (or tile-ordered) states back into the beam through the inverse permutation; with_z:
+// z written back too (solvers whose drifts move it)
 hipError_t launch_store_gather(double *const *beam, int64_t n, double *const *st, const int32_t *inv, const ErrorInfo *err,
-                               hipStream_t s);
+                               hipStream_t s, int with_z = 0);
 
 struct BatchPair { int32_t p, i, j, reserved; };
 

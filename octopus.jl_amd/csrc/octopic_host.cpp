@@ -264,6 +264,7 @@ struct octopic_handle {
     DBuf<double> srt[2][6];     // tile-ordered states (tiles > 0)
     DBuf<int64_t> perm_srt[2];
     DBuf<int32_t> inv_srt[2];   // inverse of perm_srt
+    DBuf<int32_t> inv_sl[2];    // inverse of perm (slice order), for the gathering store without tiles
     DBuf<unsigned long long> rank_keys[2], rank_sorted[2];   // :equal_count rank sort, per beam
     DBuf<int64_t> rank_vals[2];
     DBuf<unsigned long long> box;   // transverse bound slots of both beams
@@ -493,6 +494,7 @@ void run_slicing(octopic_handle *h, int b, const octopic_slicing &s, int64_t n) 
     HIPCHECK(hipMemcpyAsync(h->q.p + (size_t)b * kMaxSlices, q.data(), sizeof(double) * kMaxSlices, hipMemcpyHostToDevice,
                             h->stream));
     h->perm[b].ensure(n);
+    h->inv_sl[b].ensure(n);
     h->sid[b].ensure(n);
     // the :equal_count rank sort's passes count 256 digits per block
     h->hist[b].ensure(slicing_hist_entries(n, s.method == OCTOPIC_SLICE_EQUAL_COUNT ? 256 : ns));
@@ -593,7 +595,7 @@ void octopic_destroy(octopic_handle *h) {
         for (int c = 0; c < 6; ++c) {
             h->beam_in[b][c].release(); h->beam_f32[b][c].release(); h->st[b][c].release(); h->srt[b][c].release();
         }
-        h->perm[b].release(); h->perm_srt[b].release(); h->inv_srt[b].release(); h->sid[b].release(); h->hist[b].release();
+        h->perm[b].release(); h->perm_srt[b].release(); h->inv_srt[b].release(); h->inv_sl[b].release(); h->sid[b].release(); h->hist[b].release();
     }
     for (int b = 0; b < 2; ++b) { h->rank_keys[b].release(); h->rank_sorted[b].release(); h->rank_vals[b].release(); }
     h->lanes.release(); h->mm.release(); h->clanes.release(); h->bins.release(); h->ptrs.release(); h->pairs.release();
@@ -1078,8 +1080,8 @@ void soft_collide(octopic_handle *h, octopic_beam *const B[2], double *const bp[
     if (compute_lum) HIPCHECK(launch_soft_lum_fold(h->soft_lum_block.p, h->soft_lum_scale.p, npairs, h->lum_parts.p, s));
     {
         Timed t(h, F_STORE);
-        for (int b = 0; b < 2; ++b)
-            HIPCHECK(launch_store(h->ptrs.p + b * 6, n[b], h->ptrs.p + 12 + b * 6, h->perm[b].p, h->errinfo.p, s, 1));
+        for (int b = 0; b < 2; ++b)   // gathered through the inverse permutation: the writes run in beam order
+            HIPCHECK(launch_store_gather(h->ptrs.p + b * 6, n[b], h->ptrs.p + 12 + b * 6, h->inv_sl[b].p, h->errinfo.p, s, 1));
     }
     HIPCHECK(hipMemcpyAsync(h->h_err, h->errinfo.p, sizeof(ErrorInfo), hipMemcpyDeviceToHost, s));
     h->plum.assign(npairs, 0.0);
@@ -1190,6 +1192,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.st = h->ptrs.p + 12 + b * 6;
             a.st_z = h->st[b][4].p;
             a.perm = h->perm[b].p;
+            a.inv = tiles > 0 ? nullptr : h->inv_sl[b].p;   // the tile order stores through its own inverse
             a.err = h->errinfo.p;
             if (a.method == OCTOPIC_SLICE_EQUAL_COUNT) {
                 a.rank_keys = h->rank_keys[b].p; a.rank_sorted = h->rank_sorted[b].p; a.rank_vals = h->rank_vals[b].p;
@@ -1634,7 +1637,7 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             if (tiles > 0)
                 HIPCHECK(launch_store_gather(h->ptrs.p + b * 6, n[b], h->ptrs.p + 24 + b * 6, h->inv_srt[b].p, h->errinfo.p, s));
             else
-                HIPCHECK(launch_store(h->ptrs.p + b * 6, n[b], h->ptrs.p + 12 + b * 6, h->perm[b].p, h->errinfo.p, s));
+                HIPCHECK(launch_store_gather(h->ptrs.p + b * 6, n[b], h->ptrs.p + 12 + b * 6, h->inv_sl[b].p, h->errinfo.p, s));
         }
     }
     // ---- results
