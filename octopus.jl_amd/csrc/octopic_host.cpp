@@ -278,6 +278,7 @@ struct octopic_handle {
     // beam 2's slicing runs on side_stream between two events of the handle's stream
     hipStream_t side_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool in_many = false;               // set while octopic_collide_many runs the handle
     DBuf<double *> ptrs;        // [beam_in 2x6][st 2x6]
     DBuf<BatchPair> pairs;
     DBuf<unsigned long long> keys;   // bounds slots [2 * widest batch][kBoundSlots], zero between batches
@@ -625,9 +626,6 @@ int octopic_create(const octopic_options *opt, int32_t device, octopic_handle **
         h->device = device;
         HIPCHECK(hipSetDevice(device));
         HIPCHECK(hipStreamCreate(&h->own_stream));   // default priority; orders with the null stream
-        HIPCHECK(hipStreamCreate(&h->side_stream));  // default priority too
-        HIPCHECK(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
-        HIPCHECK(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
         HIPCHECK(generate_scratch_init(device));     // the allocation-free generator's per-device workspace
         h->stream = h->own_stream;
         h->nx = opt->nx; h->ny = opt->ny;
@@ -846,7 +844,7 @@ extern "C" int octopic_collide(octopic_handle *h, octopic_beam *beam1, octopic_b
         if (h->unfinished) {
             // kernels of the abandoned collide may have left charge or bounds slots filled (and a failure
             // between the slicing's fork and join may have left work on the side stream)
-            HIPCHECK(hipStreamSynchronize(h->side_stream));
+            if (h->side_stream) HIPCHECK(hipStreamSynchronize(h->side_stream));
             if (h->charge.p) HIPCHECK(hipMemsetAsync(h->charge.p, 0, h->charge.n * sizeof(unsigned long long), h->stream));
             if (h->keys.p) HIPCHECK(hipMemsetAsync(h->keys.p, 0, h->keys.n * sizeof(unsigned long long), h->stream));
             HIPCHECK(launch_cache_invalidate(h->cache.p, h->nkeys, h->epoch, h->stream));
@@ -968,8 +966,10 @@ extern "C" int octopic_collide_many(octopic_handle *const *handles, octopic_beam
     pool.reserve(count);
     for (int k = 0; k < count; ++k)
         pool.emplace_back([&, k]() {
+            handles[k]->in_many = true;
             st[k] = octopic_collide(handles[k], beams1 + k, beams2 + k, turns ? turns[k] : -1,
                                     compute_luminosity ? compute_luminosity[k] : 1, luminosity + k);
+            handles[k]->in_many = false;
         });
     for (auto &t : pool) t.join();
     int first = OCTOPIC_OK;
@@ -1171,9 +1171,21 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         const octopic_slicing *sl[2] = {&o.slicing1, &o.slicing2};
         for (int b = 0; b < 2; ++b) run_slicing(h, b, *sl[b], n[b]);
         // the two beams' slicing chains are independent (own tables, states and scratch shares): beam 2's
-        // runs on the side stream, forked from and joined back into the handle's stream
-        HIPCHECK(hipEventRecord(h->ev_fork, s));
-        HIPCHECK(hipStreamWaitEvent(h->side_stream, h->ev_fork, 0));
+        // runs on the side stream, forked from and joined back into the handle's stream -- except under
+        // octopic_collide_many, whose instances already fill the GPU side by side (and whose streams share
+        // the hardware queues, where an instance's join would wait behind other instances' work)
+        const bool split = !h->in_many;
+        if (split && !h->side_stream) {
+            // created on first use: a stream takes one of the process's hardware queues in creation order, and
+            // handles that only run under octopic_collide_many never need it
+            HIPCHECK(hipStreamCreate(&h->side_stream));   // default priority
+            HIPCHECK(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+            HIPCHECK(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+        }
+        if (split) {
+            HIPCHECK(hipEventRecord(h->ev_fork, s));
+            HIPCHECK(hipStreamWaitEvent(h->side_stream, h->ev_fork, 0));
+        }
         for (int b = 0; b < 2; ++b) {
             SlicingArgs a{};
             a.z = bp[b][4]; a.x = bp[b][0]; a.y = bp[b][2];
@@ -1197,10 +1209,12 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             if (a.method == OCTOPIC_SLICE_EQUAL_COUNT) {
                 a.rank_keys = h->rank_keys[b].p; a.rank_sorted = h->rank_sorted[b].p; a.rank_vals = h->rank_vals[b].p;
             }
-            HIPCHECK(launch_slicing(a, b == 0 ? s : h->side_stream));
+            HIPCHECK(launch_slicing(a, b == 0 || !split ? s : h->side_stream));
         }
-        HIPCHECK(hipEventRecord(h->ev_join, h->side_stream));
-        HIPCHECK(hipStreamWaitEvent(s, h->ev_join, 0));
+        if (split) {
+            HIPCHECK(hipEventRecord(h->ev_join, h->side_stream));
+            HIPCHECK(hipStreamWaitEvent(s, h->ev_join, 0));
+        }
     }
     HIPCHECK(hipMemcpyAsync(h->h_tab, h->tab.p, 2 * sizeof(SliceTable), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(h->h_err, h->errinfo.p, sizeof(ErrorInfo), hipMemcpyDeviceToHost, s));
