@@ -237,3 +237,26 @@ def test_kick_fp32_physical_sizes(col, sizes):
     inner = float(_eta_bounds(np.float32, 2.0 ** -23, 8.0)[0])
     assert np.all(np.isfinite(kx)) and np.all(np.isfinite(ky))
     assert err.max() <= 8.0 * 2.0 ** -23 / math.sqrt(inner), err.max()
+
+
+def test_fp32_entry_points_check_their_arguments(col):
+    """The Float32 entry points validate as the FP64 ones: zero sizes give zero kicks, negative or non-finite
+    sizes and non-positive response sizes are argument errors, empty inputs are no-ops."""
+    x = torch.tensor([0.0, 1e-6, -2e-6], dtype=torch.float32, device="cuda:0")
+    y = torch.tensor([0.0, -3e-6, 0.0], dtype=torch.float32, device="cuda:0")
+    kx, ky = col.gaussian_beambeam_kick(x, y, 0.0, 1e-5)
+    assert kx.dtype == torch.float32 and torch.count_nonzero(kx) == 0 and torch.count_nonzero(ky) == 0
+    kx, ky = col.gaussian_beambeam_kick(x, y, 1e-5, 1e-5)
+    assert kx[0].item() == 0.0 and ky[0].item() == 0.0 and kx[1].item() > 0.0
+    e = torch.empty(0, dtype=torch.float32, device="cuda:0")
+    assert col.gaussian_beambeam_kick(e, e, 1e-5, 1e-5)[0].numel() == 0
+    for bad in (-1.0, float("nan"), float("inf")):
+        with pytest.raises(ArgumentError):
+            col.gaussian_beambeam_kick(x, y, bad, 1e-5)
+    with pytest.raises(ArgumentError):
+        col.gaussian_beambeam_kick_response(x, y, 1.0, 0.0, 1e-5)
+    with pytest.raises(ArgumentError):
+        col.gaussian_beambeam_kick(x, y.double(), 1e-5, 1e-5)   # mixed dtypes
+    w = col.faddeeva_w(torch.tensor([0.5 + 0.5j, -1.0 - 0.3j], dtype=torch.complex64, device="cuda:0")).cpu().numpy()
+    ref = np.array([O.faddeeva_w(complex(np.complex64(v))) for v in (0.5 + 0.5j, -1.0 - 0.3j)])
+    assert np.all(np.abs(w - ref) <= 2.0 ** -13 * np.abs(ref))
