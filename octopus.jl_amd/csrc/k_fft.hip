@@ -157,8 +157,20 @@ __global__ __launch_bounds__(kFFTThreads) void k_green_rows(const GreenJob *jobs
 
 // Green columns: in-place FFT along y of slot[kx][0..2ny); items are (building job, block of
 // kFFTPerBlock columns).
-__global__ __launch_bounds__(kFFTThreads) void k_green_cols(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride,
-                                                            int nslots, int nx, int ny, FFTPlanDev py, const ErrorInfo *err) {
+// FP32 mode (spec32 set): each freshly built slot's FP64 half spectrum is also stored rounded to float, with a
+// reference value Gref of the table taken out of every cell first (the DC term loses Gref * P1 * P2).
+// A constant in G moves every potential of a plane by that constant times the plane's charge: it leaves
+// E = -grad(phi) unchanged, and phi_L - phi_R too (the L, R and M planes of an interaction deposit the
+// same particles on the same mesh), but it is the bulk of |phi|, and FP32 differences of large
+// potentials lose digits of both kicks: the FP32 error of E is about eps * max|phi - Gref Q| / (h |E|).
+// Gref is the table at cell (nx/4, ny/4), i.e. G a quarter of the mesh away from the source-field
+// offset.  The potentials over the mesh lie between Q G(~sigma) near the source and Q G(~diagonal) at
+// the far corners; G at a quarter mesh sits near the middle of that range in log distance, so
+// max|phi - Gref Q| stays within about 1.4 Q for any offset (the cell (0, 0), which is the near field
+// when the two meshes nearly coincide, leaves up to 4.6 Q).
+__global__ __launch_bounds__(kFFTThreads) void k_green_cols(const GreenJob *jobs, int njobs, double2 *spec, float2 *spec32,
+                                                            int64_t slot_stride, int nslots, int nx, int ny, int integrated,
+                                                            FFTPlanDev py, const ErrorInfo *err) {
     extern __shared__ double2 lds[];
     __shared__ int list[kMaxGreenJobs];
     __shared__ int wcount[kFFTThreads / 64];
@@ -180,7 +192,17 @@ __global__ __launch_bounds__(kFFTThreads) void k_green_cols(const GreenJob *jobs
         fft_lds<-1>(a, a + P2, py, lds, lane);
         if (active) {
             const double2 *Z = fft_result(a, a + P2, py);
-            for (int y = lane; y < P2; y += kFFTLanes) col[y] = Z[y];
+            float2 *col32 = spec32 ? spec32 + (size_t)job.slot * slot_stride + (size_t)kx * P2 : nullptr;
+            for (int y = lane; y < P2; y += kFFTLanes) {
+                const double2 z = Z[y];
+                col[y] = z;
+                if (col32) {
+                    double re = z.x;
+                    if (kx == 0 && y == 0)
+                        re -= green_cell(nx / 4, ny / 4, nx, ny, integrated, job) * (double)(2 * nx) * (double)(2 * ny);
+                    col32[y] = make_float2((float)re, (float)z.y);
+                }
+            }
         }
         __syncthreads();   // the item's buffers are read before the next item writes them
     }
@@ -355,39 +377,6 @@ __global__ __launch_bounds__(kFFTLanes * G) void k_rows_inv(const Plane *planes,
     }
 }
 
-// FP32 mode: the cached FP64 Green half spectrum of every freshly built slot, rounded to float, with a
-// reference value Gref of the table taken out of every cell first (the DC term loses Gref * P1 * P2).
-// A constant in G moves every potential of a plane by that constant times the plane's charge: it leaves
-// E = -grad(phi) unchanged, and phi_L - phi_R too (the L, R and M planes of an interaction deposit the
-// same particles on the same mesh), but it is the bulk of |phi|, and FP32 differences of large
-// potentials lose digits of both kicks: the FP32 error of E is about eps * max|phi - Gref Q| / (h |E|).
-// Gref is the table at cell (nx/4, ny/4), i.e. G a quarter of the mesh away from the source-field
-// offset.  The potentials over the mesh lie between Q G(~sigma) near the source and Q G(~diagonal) at
-// the far corners; G at a quarter mesh sits near the middle of that range in log distance, so
-// max|phi - Gref Q| stays within about 1.4 Q for any offset (the cell (0, 0), which is the near field
-// when the two meshes nearly coincide, leaves up to 4.6 Q).
-__global__ __launch_bounds__(256) void k_green_to_f32(const GreenJob *jobs, int njobs, const double2 *spec, float2 *spec32,
-                                                      int64_t slot_stride, int nslots, int nx, int ny, int integrated,
-                                                      const ErrorInfo *err) {
-    __shared__ int list[kMaxGreenJobs];
-    __shared__ int wcount[256 / 64];
-    if (failed(err)) return;
-    const int nbuild = compact_green_jobs(jobs, njobs, nslots, list, wcount);
-    constexpr int kParts = 32;   // blocks' worth of one slot
-    for (int item = blockIdx.x; item < nbuild * kParts; item += gridDim.x) {
-        const GreenJob job = jobs[list[item / kParts]];
-        const int part = item % kParts;
-        const double2 *src = spec + (size_t)job.slot * slot_stride;
-        float2 *dst = spec32 + (size_t)job.slot * slot_stride;
-        for (int64_t k = (int64_t)part * blockDim.x + threadIdx.x; k < slot_stride; k += (int64_t)kParts * blockDim.x)
-            dst[k] = make_float2((float)src[k].x, (float)src[k].y);
-        if (part == 0 && threadIdx.x == 0) {
-            const double gref = green_cell(nx / 4, ny / 4, nx, ny, integrated, job);
-            dst[0] = make_float2((float)(src[0].x - gref * (double)(2 * nx) * (double)(2 * ny)), (float)src[0].y);
-        }
-    }
-}
-
 // E = -grad(phi) of every plane of an interaction into the interleaved kick table
 // table[inter][x][y][3*q + {phi, Ex, Ey}] (pic_cpu.jl:2531-2591; h of the plane's deposit mesh).
 // A block takes a kFieldTile x kFieldTile cell tile of one interaction: phi with a 2-cell halo is
@@ -464,13 +453,13 @@ static int fft_group(int requested, int P, size_t es) {
 // `full` launches a block per item of every job (the cold case: every job builds); otherwise a small
 // persistent grid takes the items of the few jobs that build, so batches of cache hits cost one short
 // pass.  Results do not depend on the grid.
-hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride, int nslots, int nx, int ny,
-                              int integrated, const FFTPlanDev &px, const FFTPlanDev &py, const ErrorInfo *err,
+hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, float2 *spec32, int64_t slot_stride, int nslots,
+                              int nx, int ny, int integrated, const FFTPlanDev &px, const FFTPlanDev &py, const ErrorInfo *err,
                               hipStream_t s, int full) {
     if (njobs <= 0) return hipSuccess;
     for (; njobs > kMaxGreenJobs; jobs += kMaxGreenJobs, njobs -= kMaxGreenJobs) {   // each launch compacts its own jobs
-        const hipError_t e = launch_green_build(jobs, kMaxGreenJobs, spec, slot_stride, nslots, nx, ny, integrated, px, py, err,
-                                                s, full);
+        const hipError_t e = launch_green_build(jobs, kMaxGreenJobs, spec, spec32, slot_stride, nslots, nx, ny, integrated, px,
+                                                py, err, s, full);
         if (e != hipSuccess) return e;
     }
     const int P1 = 2 * nx, P2 = 2 * ny;
@@ -481,8 +470,8 @@ hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, in
     const int g2 = full ? njobs * cblocks : std::min(njobs * cblocks, kGreenSmallGrid);
     hipLaunchKernelGGL(k_green_rows, dim3(g1), dim3(kFFTThreads), fft_lds_bytes(P1), s, jobs, njobs, spec, slot_stride, nslots,
                        nx, ny, integrated, px, err);
-    hipLaunchKernelGGL(k_green_cols, dim3(g2), dim3(kFFTThreads), fft_lds_bytes(P2), s, jobs, njobs, spec, slot_stride, nslots,
-                       nx, ny, py, err);
+    hipLaunchKernelGGL(k_green_cols, dim3(g2), dim3(kFFTThreads), fft_lds_bytes(P2), s, jobs, njobs, spec, spec32, slot_stride,
+                       nslots, nx, ny, integrated, py, err);
     return hipGetLastError();
 }
 
@@ -563,20 +552,6 @@ hipError_t launch_field(const Inter *inters, int ninters, const Plane *planes, c
     const int ntiles = ((nx + kFieldTile - 1) / kFieldTile) * ((ny + kFieldTile - 1) / kFieldTile);
     hipLaunchKernelGGL(k_field<R>, dim3(ntiles, ninters), dim3(kFieldTile * kFieldTile), 0, s, inters, planes, phi, table,
                        nx, ny, fourth, npl, err);
-    return hipGetLastError();
-}
-
-hipError_t launch_green_to_f32(const GreenJob *jobs, int njobs, const double2 *spec, float2 *spec32, int64_t slot_stride,
-                               int nslots, int nx, int ny, int integrated, const ErrorInfo *err, hipStream_t s, int full) {
-    if (njobs <= 0) return hipSuccess;
-    for (; njobs > kMaxGreenJobs; jobs += kMaxGreenJobs, njobs -= kMaxGreenJobs) {
-        const hipError_t e = launch_green_to_f32(jobs, kMaxGreenJobs, spec, spec32, slot_stride, nslots, nx, ny, integrated, err,
-                                                 s, full);
-        if (e != hipSuccess) return e;
-    }
-    const int g = full ? 32 * njobs : std::min(32 * njobs, kGreenSmallGrid);
-    hipLaunchKernelGGL(k_green_to_f32, dim3(g), dim3(256), 0, s, jobs, njobs, spec, spec32, slot_stride, nslots, nx, ny,
-                       integrated, err);
     return hipGetLastError();
 }
 

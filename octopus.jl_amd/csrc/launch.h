@@ -277,8 +277,9 @@ constexpr int kLumBlocks = 16;   // blocks per pair of the luminosity overlap su
 // FFT stages (k_fft.hip)
 // full = 1: a block per (job, row block) of every job, for launches where most jobs build; 0: a small
 // persistent grid over the jobs that build.  Results do not depend on it.
-hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, int64_t slot_stride, int nslots, int nx, int ny,
-                              int integrated, const FFTPlanDev &px, const FFTPlanDev &py, const ErrorInfo *err,
+// spec32: the FP32 mode's float copy of the slots (written by the column pass), or nullptr.
+hipError_t launch_green_build(const GreenJob *jobs, int njobs, double2 *spec, float2 *spec32, int64_t slot_stride, int nslots,
+                              int nx, int ny, int integrated, const FFTPlanDev &px, const FFTPlanDev &py, const ErrorInfo *err,
                               hipStream_t s, int full);
 // The charge-path transform passes.  group: transforms per workgroup, 2, 4 or 8 (halved while its LDS does
 // not fit); results do not depend on it.
@@ -297,8 +298,6 @@ hipError_t launch_rows_inv(const Plane *planes, int nplanes, const C *qh, int64_
 template <class R>
 hipError_t launch_field(const Inter *inters, int ninters, const Plane *planes, const R *phi, R *table, int nx, int ny,
                         int fourth, int npl, const ErrorInfo *err, hipStream_t s);
-hipError_t launch_green_to_f32(const GreenJob *jobs, int njobs, const double2 *spec, float2 *spec32, int64_t slot_stride,
-                               int nslots, int nx, int ny, int integrated, const ErrorInfo *err, hipStream_t s, int full);
 
 // Node-mode mesh preparation (k_node.hip)
 struct NodeArgs {

@@ -1433,12 +1433,9 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
             a.err = h->errinfo.p;
             HIPCHECK(launch_node_prepare(a, dir == 1 ? ns1 : ns2, dir == 1 ? ns2 : ns1, s));
         }
-        HIPCHECK(launch_green_build(h->node_jobs.p, nn1 + nn2, h->spec.p, h->slot_stride, nn1 + nn2, nx, ny,
-                                    o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev, h->errinfo.p,
-                                    s, 1));
-        if (h->fp32)
-            HIPCHECK(launch_green_to_f32(h->node_jobs.p, nn1 + nn2, h->spec.p, h->spec32.p, h->slot_stride, nn1 + nn2, nx,
-                                         ny, o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->errinfo.p, s, 1));
+        HIPCHECK(launch_green_build(h->node_jobs.p, nn1 + nn2, h->spec.p, h->fp32 ? h->spec32.p : nullptr, h->slot_stride,
+                                    nn1 + nn2, nx, ny, o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev,
+                                    h->py.dev, h->errinfo.p, s, 1));
         h->ran_dev.ensure(std::max(npairs, 1));
         HIPCHECK(hipMemsetAsync(h->ran_dev.p, 0, sizeof(int32_t) * npairs, s));
     }
@@ -1499,12 +1496,9 @@ void collide_impl(octopic_handle *h, octopic_beam *B1, octopic_beam *B2, int64_t
         }
         if (h->mode != 2) {
             Timed t(h, F_GREEN);
-            HIPCHECK(launch_green_build(h->jobs.p, 2 * B, h->spec.p, h->slot_stride, nslots, nx, ny,
-                                        o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev,
+            HIPCHECK(launch_green_build(h->jobs.p, 2 * B, h->spec.p, h->fp32 ? h->spec32.p : nullptr, h->slot_stride, nslots, nx,
+                                        ny, o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->px.dev, h->py.dev,
                                         h->errinfo.p, s, green_full));
-            if (h->fp32)
-                HIPCHECK(launch_green_to_f32(h->jobs.p, 2 * B, h->spec.p, h->spec32.p, h->slot_stride, nslots, nx, ny,
-                                             o.green_type == OCTOPIC_GREEN_INTEGRATED ? 1 : 0, h->errinfo.p, s, green_full));
         }
         const int nfp = 2 * B * npl;
         {
